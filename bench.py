@@ -1,0 +1,168 @@
+"""Benchmark of the pSi3D time step on one MI355X.
+
+Workload: the synthetic stratified elliptic lake of 1024 x 512 cells, dx = 100 m,
+60 stretched layers (0.5 -> 5 m), GRF bathymetry, tanh thermocline, sinusoidal
+wind from 30 deg, TVD transport + two-equation turbulence, dt = 60 s.  A step is
+one full si3d.f90 time step (leapfrog fd + settrap + 2 trapezoidal fd +
+settrap2 + save) with all three free-surface solves to a 1e-10 FP64 relative
+residual.  Inputs are resident in HBM before the timed region.
+
+Prints one JSON line: wet-cell-steps/s (value), PCG iterations per step, the
+roofline of the dominant kernel (HIP events around every launch in the timed
+region) and a CPU baseline (the FP64 oracle on the host, a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per wet cell for one launch of each kernel, in elements of
+# the 3-D storage type: distinct state arrays read + written by the phase
+# (phase-private scratch, index arrays and 2-D arrays not counted).
+ARRAYS_PER_CELL = {
+    "exmom_x": 10, "exmom_y": 10,        # 9 reads + ex
+    "matmom_x": 7, "matmom_y": 7,        # ex, Av, rhop, hq, qpp + ag, ar
+    "matcon": 2,                         # uhpp, vhpp
+    "slayer_h": 6,                       # hp, hup, hvp -> h, hu, hv
+    "vel": 6,                            # agx, arx, agy, ary -> uh, vh
+    "continuity1": 5,                    # uh, uhpp, vh, vhpp -> wp
+    "exsal_hflux": 10,                   # salpp, uhpp, uh, vhpp, vh, hup, hvp, hpp -> fx_ew, fx_ns
+    "exsal_imsal": 11,                   # fluxes, salpp, hpp, hp, hupp, hvpp, wp, h, Dv -> sal
+    "vel2": 3,                           # -> h, hu, hv
+    "turbulence": 14,                    # hp, rhop, up, vp, Av, Dv, q2p, q2pp, q2lp, q2lpp -> q2, q2l, Av, Dv
+    "settrap_col": 24, "settrap2_col": 16, "save_col": 26,
+    "continuity2": 3,
+}
+TURB_FP64 = {"turbulence"}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--nx", type=int, default=1024)
+    ap.add_argument("--ny", type=int, default=512)
+    ap.add_argument("--layers", type=int, default=60)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precond", default="line", choices=["jacobi", "line"])
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-timers", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(deck, nsteps):
+    """The FP64 oracle (C++ restatement of the reference, one core) on the same
+    deck: init excluded, `nsteps` time steps timed."""
+    from oracle.oracle import Oracle, build
+    build()
+    o = Oracle(deck)
+    t0 = time.perf_counter()
+    for n in range(1, nsteps + 1):
+        o.step(n)
+    dt = time.perf_counter() - t0
+    return dt, o
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {a.gpus} is not supported "
+              "(the multi-GPU decomposition is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    from si3d_amd import lake
+    from si3d_amd.model import GpuLake
+
+    deck = lake.ellipse(nx=a.nx, ny=a.ny, dx=100.0, nlayers=a.layers, dt=60.0,
+                        nts=a.warmup + a.steps + 1, seed=1)
+    n_wet = lake.n_wet_cells(deck)
+    t_setup = time.perf_counter()
+    g = GpuLake(deck, precision=a.precision, preconditioner=a.precond, tol=a.tol)
+    t_setup = time.perf_counter() - t_setup
+    for n in range(1, a.warmup + 1):
+        g.step(n)
+    g.synchronize()
+    st0 = g.stats()
+    if not a.no_timers:
+        g.timers(True)
+    t0 = time.perf_counter()
+    n0 = a.warmup + 1
+    for n in range(n0, n0 + a.steps):
+        g.step(n)
+    g.synchronize()
+    elapsed = time.perf_counter() - t0
+    st1 = g.stats()
+    ktimes = {} if a.no_timers else g.kernel_times()
+    g.timers(False)
+
+    value = n_wet * a.steps / elapsed
+    iters = (st1["pcg_iters_total"] - st0["pcg_iters_total"]) / a.steps
+    elem = 8 if a.precision == "fp64" else 4
+
+    roofline = None
+    kernel_summary = {}
+    if ktimes:
+        best = None
+        for name, (avg_ms, cnt) in ktimes.items():
+            kernel_summary[name] = {"avg_ms": round(avg_ms, 5), "launches": cnt}
+            if name not in ARRAYS_PER_CELL:
+                continue
+            tot = avg_ms * cnt
+            if best is None or tot > best[1]:
+                best = (name, tot, avg_ms)
+        if best:
+            name, _, avg_ms = best
+            eb = 8 if name in TURB_FP64 else elem
+            bytes_launch = ARRAYS_PER_CELL[name] * eb * n_wet
+            achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+            roofline = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": None, "bytes_per_launch": bytes_launch,
+                        "avg_launch_ms": round(avg_ms, 5)}
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        dt, o = cpu_baseline(deck, a.cpu_steps)
+        cpu = {"value": round(n_wet * a.cpu_steps / dt, 1), "unit": "wet-cell-steps/s", "cores": 1,
+               "kind": "port",
+               "sample": f"FP64 oracle (C++ restatement, 1 thread), same lake, steps 1-{a.cpu_steps}, "
+                         f"{dt:.1f} s"}
+
+    out = {
+        "metric": "wet-cell-steps/sec and PCG iters/step at 1 MI355X, learned vs plain, same error",
+        "value": round(value, 1),
+        "unit": "wet-cell-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(1e3 * elapsed / a.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if a.precision == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"ellipse lake {a.nx}x{a.ny}x{a.layers}, dx=100 m, dt=60 s, TVD + "
+                               f"turbulence, plain PCG ({a.precond}) tol {a.tol:g}, {a.precision}",
+                   "n_wet": n_wet, "lm": g.lm, "km1": g.km1, "parallelism": "1 GPU"},
+        "pcg_iters_per_step": round(iters, 2),
+        "pcg_relres_last": st1["pcg_relres_last"],
+        "setup_s": round(t_setup, 2),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kernel_summary,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,146 @@
+/*
+ * si3d_gpu.h -- C-ABI of libsi3dgpu.so, the MI355X (gfx950) time-step core of
+ * pSi3D.
+ *
+ * The Fortran driver (si3d.f90) keeps reading the input files, computing the
+ * grid (bathy) and the initial state (init), and writing every output file.
+ * It hands the grid and state to the library once, then replaces its calls to
+ * fd / settrap / settrap2 / save with the entry points below, and pulls fields
+ * back with s3g_get_field_* before an output is due.  All pointers are host
+ * pointers owned by the caller; the library copies from / to them and never
+ * keeps them.  Arrays use the reference layout: 3-D fields are (km1, lm1) with k
+ * fastest, 2-D fields are (lm1), columns numbered l = 1..lm plus the dry
+ * sentinel column lm1 = lm+1 (si3d_types.f90:117-170, si3d_utils.f90:757-806).
+ *
+ * Return codes follow the reference's error convention (si3d_procedures.f90:
+ * 2002-2016): 0 ok, > 0 warning (1 = free-surface solve hit its iteration
+ * limit, like NSPCG ier=1), < 0 fatal (the Fortran shim prints
+ * s3g_last_error() and STOPs).
+ *
+ * One host thread calls the library (the OpenMP driver wraps each call in
+ * `!$omp masked` + `!$omp barrier`).  All device work of one context runs in
+ * order on one HIP stream.
+ */
+#ifndef SI3D_GPU_H
+#define SI3D_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define S3G_OK 0
+#define S3G_WARN_ITMAX 1
+#define S3G_ERR_ARG (-1)
+#define S3G_ERR_HIP (-2)
+#define S3G_ERR_UNSUPPORTED (-3)
+#define S3G_ERR_SOLVER (-6)
+
+/* Precision of the device state. */
+#define S3G_FP64 0  /* FP64 everywhere                                         */
+#define S3G_MIXED 1 /* FP32 stencils, column solves and SpMV; FP64 dots, true
+                       residual, refinement, zeta and continuity             */
+
+/* Free-surface preconditioners. */
+#define S3G_PREC_JACOBI 0
+#define S3G_PREC_LINE 1 /* exact tridiagonal solve along each N-S line of wet cells */
+
+typedef struct s3g_ctx s3g_ctx;
+
+/* Grid as left by bathy (si3d_utils.f90:595-892) and init
+ * (si3d_procedures.f90:67-134).  Index arrays are 1-based like the Fortran. */
+typedef struct {
+  int im1, jm1, km1, lm;
+  const int *l2i, *l2j;             /* [lm1]                                 */
+  const int *lEC, *lWC, *lNC, *lSC; /* [lm1]; lm1 where the neighbour is dry */
+  const int *kmz, *k1z, *k1u, *k1v; /* [lm1]                                 */
+  const float *zlevel;              /* [km1]                                 */
+  const float *hhs, *hhu, *hhv;     /* [lm1]                                 */
+} s3g_grid;
+
+/* Run parameters read by `input` (si3d_utils.f90:21-315). */
+typedef struct {
+  double dt, dx, dy;
+  int itrap, niter, ismooth, iturb, iadv, itrmom, ihd, ibc, isal, ifsurfbc;
+  double beta, Av0, Dv0, Ax0, Ay0, f, theta, cd, cw, wa, phi, dtsurfbc;
+  int n_surfbc;          /* records of si3d_surfbc_W.txt (ifsurfbc = 20)   */
+  const double *surfbc;  /* 3*n_surfbc: cdw, uair, vair per record         */
+  int precision;         /* S3G_FP64 or S3G_MIXED                          */
+  int preconditioner;    /* S3G_PREC_*                                     */
+  double pcg_tol;        /* relative FP64 true-residual tolerance          */
+  int pcg_maxit;
+} s3g_params;
+
+/* Input deck for the standalone driver: the content of si3d_inp.txt,
+ * si3d_bathy, si3d_layer.txt and si3d_init.txt (same meaning as above). */
+typedef struct {
+  int imm, jmm;          /* interior cells (si3d_bathy header)              */
+  int ibathyf, km1_file; /* ibathyf < 0: km1_file zlevel records are given  */
+  double ddz, dzmin, datadj, zetainit, zl;
+  const double *zlevel_file; /* [km1_file]                                  */
+  const int *bathy_dm;       /* imm*jmm depths in dm, index (i-2)*jmm+(j-2) */
+  const double *init_profile;/* [km1] active scalar per layer               */
+} s3g_deck;
+
+typedef struct {
+  long steps, fd_calls, solves;
+  long pcg_iters_total;      /* over all solves                              */
+  int pcg_iters_last;        /* last solve                                   */
+  double pcg_relres_last;    /* FP64 true relative residual of the last solve*/
+  double zeta_volume;        /* sum of zeta over wet columns (m)             */
+  int lm, km1;
+  long n_wet;                /* sum of wet layers at creation                */
+} s3g_stats;
+
+/* --- Drop-in entry points (replace fd/settrap/settrap2/save) ------------- */
+
+/* Replaces the state hand-over after init (si3d.f90:48-68). */
+int s3g_init(const s3g_grid *grid, const s3g_params *params, s3g_ctx **out);
+
+/* Copy one state array in / out (names as in si3d_types.f90: "uh", "sal",
+ * "q2p", "s", "k1z", ...).  *_f32 for REAL arrays, *_f64 for REAL*8 arrays
+ * (q2, q2l and their time levels) or for FP64 callers, *_i32 for indices. */
+int s3g_set_field_f32(s3g_ctx *c, const char *name, const float *host);
+int s3g_set_field_f64(s3g_ctx *c, const char *name, const double *host);
+int s3g_set_field_i32(s3g_ctx *c, const char *name, const int *host);
+int s3g_get_field_f32(s3g_ctx *c, const char *name, float *host);
+int s3g_get_field_f64(s3g_ctx *c, const char *name, double *host);
+int s3g_get_field_i32(s3g_ctx *c, const char *name, int *host);
+
+/* fd(n, ..., istep, lastiter, ..., iter, ..., thrs): si3d_procedures.f90:382-575 */
+int s3g_fd(s3g_ctx *c, int n, int istep, int iter, int lastiter, double thrs);
+/* settrap: si3d_procedures.f90:3418-3618 */
+int s3g_settrap(s3g_ctx *c);
+/* settrap2: si3d_procedures.f90:3944-4087 */
+int s3g_settrap2(s3g_ctx *c);
+/* save(istep): si3d_procedures.f90:3621-3941 */
+int s3g_save(s3g_ctx *c, int istep);
+
+/* One whole time step of the si3d.f90 loop (324-393) / nsteps of them. */
+int s3g_step(s3g_ctx *c, int n);
+int s3g_run(s3g_ctx *c, int n0, int nsteps);
+
+/* Run one phase of fd alone (phase-level checking): 0 surfbc, 1 horizontal
+ * mixing, 2 exmom x, 3 matmom x, 4 exmom y, 5 matmom y, 6 matcon, 7 free
+ * surface solve + slayer_h, 8 vel + continuity, 9 exsal, 10 imsal, 11 vel2,
+ * 12 smooth, 13 turbulence.  Phases 9 and 10 run fused in fd; phase 9 alone
+ * computes the explicit transport terms, phase 10 the implicit solve. */
+int s3g_phase(s3g_ctx *c, int phase, int n, int istep, double thrs);
+
+int s3g_synchronize(s3g_ctx *c);
+int s3g_get_stats(s3g_ctx *c, s3g_stats *out);
+/* Average device duration (ms) of each timed kernel class since the last
+ * reset, for roofline reporting: names[i] / avg_ms[i] / launches[i]. */
+int s3g_kernel_times(s3g_ctx *c, int max, char (*names)[32], double *avg_ms, long *launches);
+int s3g_reset_timers(s3g_ctx *c, int enable);
+void s3g_finalize(s3g_ctx *c);
+const char *s3g_last_error(void);
+
+/* --- Standalone driver (no Fortran host) --------------------------------- */
+
+/* Restates input/bathy/init on the host and calls s3g_init + s3g_set_field_*. */
+int s3g_create_from_deck(const s3g_deck *deck, const s3g_params *params, s3g_ctx **out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

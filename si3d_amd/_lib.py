@@ -1,0 +1,102 @@
+"""ctypes binding of libsi3dgpu.so (declared in include/si3d_gpu.h).
+
+The library is built in-tree by si3d_amd/build.py.  Loading fails loudly when
+it is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libsi3dgpu.so")
+
+S3G_FP64, S3G_MIXED = 0, 1
+S3G_PREC_JACOBI, S3G_PREC_LINE = 0, 1
+
+EXPORTS = [
+    "s3g_init", "s3g_set_field_f32", "s3g_set_field_f64", "s3g_set_field_i32",
+    "s3g_get_field_f32", "s3g_get_field_f64", "s3g_get_field_i32", "s3g_fd",
+    "s3g_settrap", "s3g_settrap2", "s3g_save", "s3g_step", "s3g_run", "s3g_phase",
+    "s3g_synchronize", "s3g_get_stats", "s3g_kernel_times", "s3g_reset_timers",
+    "s3g_finalize", "s3g_last_error", "s3g_create_from_deck",
+]
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+fp = C.POINTER(C.c_float)
+
+
+class S3gGrid(C.Structure):
+    _fields_ = [("im1", C.c_int), ("jm1", C.c_int), ("km1", C.c_int), ("lm", C.c_int),
+                ("l2i", ip), ("l2j", ip), ("lEC", ip), ("lWC", ip), ("lNC", ip), ("lSC", ip),
+                ("kmz", ip), ("k1z", ip), ("k1u", ip), ("k1v", ip),
+                ("zlevel", fp), ("hhs", fp), ("hhu", fp), ("hhv", fp)]
+
+
+class S3gParams(C.Structure):
+    _fields_ = [("dt", C.c_double), ("dx", C.c_double), ("dy", C.c_double),
+                ("itrap", C.c_int), ("niter", C.c_int), ("ismooth", C.c_int), ("iturb", C.c_int),
+                ("iadv", C.c_int), ("itrmom", C.c_int), ("ihd", C.c_int), ("ibc", C.c_int),
+                ("isal", C.c_int), ("ifsurfbc", C.c_int),
+                ("beta", C.c_double), ("Av0", C.c_double), ("Dv0", C.c_double), ("Ax0", C.c_double),
+                ("Ay0", C.c_double), ("f", C.c_double), ("theta", C.c_double), ("cd", C.c_double),
+                ("cw", C.c_double), ("wa", C.c_double), ("phi", C.c_double), ("dtsurfbc", C.c_double),
+                ("n_surfbc", C.c_int), ("surfbc", dp), ("precision", C.c_int),
+                ("preconditioner", C.c_int), ("pcg_tol", C.c_double), ("pcg_maxit", C.c_int)]
+
+
+class S3gDeck(C.Structure):
+    _fields_ = [("imm", C.c_int), ("jmm", C.c_int), ("ibathyf", C.c_int), ("km1_file", C.c_int),
+                ("ddz", C.c_double), ("dzmin", C.c_double), ("datadj", C.c_double),
+                ("zetainit", C.c_double), ("zl", C.c_double), ("zlevel_file", dp),
+                ("bathy_dm", ip), ("init_profile", dp)]
+
+
+class S3gStats(C.Structure):
+    _fields_ = [("steps", C.c_long), ("fd_calls", C.c_long), ("solves", C.c_long),
+                ("pcg_iters_total", C.c_long), ("pcg_iters_last", C.c_int),
+                ("pcg_relres_last", C.c_double), ("zeta_volume", C.c_double),
+                ("lm", C.c_int), ("km1", C.c_int), ("n_wet", C.c_long)]
+
+
+_lib = None
+
+
+def load():
+    """Load libsi3dgpu.so; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.s3g_init.argtypes = [C.POINTER(S3gGrid), C.POINTER(S3gParams), C.POINTER(vp)]
+    L.s3g_create_from_deck.argtypes = [C.POINTER(S3gDeck), C.POINTER(S3gParams), C.POINTER(vp)]
+    for n, t in (("f32", fp), ("f64", dp), ("i32", ip)):
+        getattr(L, f"s3g_set_field_{n}").argtypes = [vp, C.c_char_p, t]
+        getattr(L, f"s3g_get_field_{n}").argtypes = [vp, C.c_char_p, t]
+    L.s3g_fd.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
+    L.s3g_settrap.argtypes = [vp]
+    L.s3g_settrap2.argtypes = [vp]
+    L.s3g_save.argtypes = [vp, C.c_int]
+    L.s3g_step.argtypes = [vp, C.c_int]
+    L.s3g_run.argtypes = [vp, C.c_int, C.c_int]
+    L.s3g_phase.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_double]
+    L.s3g_synchronize.argtypes = [vp]
+    L.s3g_get_stats.argtypes = [vp, C.POINTER(S3gStats)]
+    L.s3g_kernel_times.argtypes = [vp, C.c_int, C.c_void_p, dp, C.POINTER(C.c_long)]
+    L.s3g_reset_timers.argtypes = [vp, C.c_int]
+    L.s3g_finalize.argtypes = [vp]
+    L.s3g_finalize.restype = None
+    L.s3g_last_error.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def check(rc, what="s3g"):
+    if rc < 0:
+        msg = load().s3g_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed ({rc}): {msg}")
+    return rc

@@ -1,0 +1,569 @@
+// s3g_host.hip -- host side of libsi3dgpu.so: the C-ABI, device allocation and
+// layout marshalling, the time-loop driver (si3d.f90:324-393), wind forcing,
+// timers, and the standalone deck path that restates input/bathy/init.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "s3g_internal.h"
+
+namespace s3g {
+
+static thread_local std::string g_err;
+void set_error(const std::string &m) { g_err = m; }
+
+int hip_check(hipError_t e, const char *what) {
+  if (e == hipSuccess) return 0;
+  set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return S3G_ERR_HIP;
+}
+
+template <typename T>
+static T *dalloc(s3g_ctx *c, size_t n) {
+  void *p = nullptr;
+  if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return nullptr;
+  hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T));
+  c->allocs.push_back(p);
+  return (T *)p;
+}
+
+// ---------------------------------------------------------------------------
+// Timers: an event pair around each timed launch, resolved lazily.
+void timer_begin(s3g_ctx *c, const char *name, int *slot) {
+  int ti = -1;
+  for (size_t i = 0; i < c->timers.size(); ++i)
+    if (c->timers[i].name == name) ti = (int)i;
+  if (ti < 0) {
+    c->timers.push_back(Timer{name, 0.0, 0});
+    ti = (int)c->timers.size() - 1;
+  }
+  if (c->pending.size() >= 4096) {
+    hipStreamSynchronize(c->stream);
+    for (auto &pr : c->pending) {
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, c->ev_pool[pr.second].first, c->ev_pool[pr.second].second);
+      c->timers[pr.first].total_ms += ms;
+      c->timers[pr.first].n += 1;
+    }
+    c->pending.clear();
+  }
+  const int ei = (int)c->pending.size();
+  while ((int)c->ev_pool.size() <= ei) {
+    hipEvent_t a, b;
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    c->ev_pool.push_back({a, b});
+  }
+  hipEventRecord(c->ev_pool[ei].first, c->stream);
+  c->pending.push_back({ti, ei});
+  *slot = ei;
+}
+
+void timer_end(s3g_ctx *c, int slot) {
+  if (slot >= 0) hipEventRecord(c->ev_pool[slot].second, c->stream);
+}
+
+static void timers_flush(s3g_ctx *c) {
+  hipStreamSynchronize(c->stream);
+  for (auto &pr : c->pending) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c->ev_pool[pr.second].first, c->ev_pool[pr.second].second);
+    c->timers[pr.first].total_ms += ms;
+    c->timers[pr.first].n += 1;
+  }
+  c->pending.clear();
+}
+
+// Parabolic interpolation of a uniform series, si3d_utils.f90:895-920.
+static double parab(double frstpt, double x, const std::vector<double> &fx1, double dx) {
+  // fx1 is 1-based (fx1[0] unused)
+  int m = (int)((x - frstpt) / dx);
+  double om = m;
+  double theta = (x - frstpt - om * dx) / dx;
+  if (m == 0) {
+    m = 2;
+    theta -= 1.0;
+  } else {
+    m = m + 1;
+  }
+  const int n = (int)fx1.size() - 1;
+  if (m + 1 > n) m = n - 1;  // beyond the last record: extrapolate from the end
+  return fx1[m] + 0.5 * theta * (fx1[m + 1] - fx1[m - 1] + theta * (fx1[m + 1] + fx1[m - 1] - 2.0 * fx1[m]));
+}
+
+// surfbc cases 0 and 20 (si3d_boundaryconditions.f90:4508-4515, 4650-4665): the
+// forcing is uniform in space, so it is kept as three scalars.
+void surfbc(s3g_ctx *c, int n, int istep) {
+  if (c->P.ifsurfbc == 0) {
+    c->uair = -c->P.wa * std::sin(3.1415926535897932 * c->P.phi / 180.);
+    c->vair = -c->P.wa * std::cos(3.1415926535897932 * c->P.phi / 180.);
+    c->cdw = c->P.cw;
+  } else if (c->P.ifsurfbc == 20) {
+    if (n > 1 && istep == 2) return;
+    const int np = (int)(c->surfbc.size() / 3);
+    if (np < 3) return;
+    std::vector<double> f1(np + 1), f2(np + 1), f3(np + 1);
+    for (int j = 0; j < np; ++j) f1[j + 1] = c->surfbc[3 * j], f2[j + 1] = c->surfbc[3 * j + 1], f3[j + 1] = c->surfbc[3 * j + 2];
+    const double dth = c->P.dtsurfbc / 3600.;
+    c->cdw = parab(0., c->thrs, f1, dth);
+    c->uair = parab(0., c->thrs, f2, dth);
+    c->vair = parab(0., c->thrs, f3, dth);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Field table: name -> device storage.
+enum Kind { K3R, K3D, K2D, KI };
+struct Field {
+  const char *name;
+  Kind kind;
+  void *ptr;
+};
+
+template <typename R>
+static std::vector<Field> field_table(Dev<R> &D) {
+  return {
+      {"h", K3R, D.h},         {"hp", K3R, D.hp},       {"hpp", K3R, D.hpp},     {"hu", K3R, D.hu},
+      {"hup", K3R, D.hup},     {"hupp", K3R, D.hupp},   {"hv", K3R, D.hv},       {"hvp", K3R, D.hvp},
+      {"hvpp", K3R, D.hvpp},   {"uh", K3R, D.uh},       {"uhp", K3R, D.uhp},     {"uhpp", K3R, D.uhpp},
+      {"u", K3R, D.u},         {"up", K3R, D.up},       {"upp", K3R, D.upp},     {"vh", K3R, D.vh},
+      {"vhp", K3R, D.vhp},     {"vhpp", K3R, D.vhpp},   {"v", K3R, D.v},         {"vp", K3R, D.vp},
+      {"vpp", K3R, D.vpp},     {"wp", K3R, D.wp},       {"sal", K3R, D.sal},     {"salp", K3R, D.salp},
+      {"salpp", K3R, D.salpp}, {"rhop", K3R, D.rhop},   {"Av", K3R, D.Av},       {"Dv", K3R, D.Dv},
+      {"kh", K3R, D.kh},       {"ex", K3R, D.ex},       {"agx", K3R, D.agx},     {"arx", K3R, D.arx},
+      {"agy", K3R, D.agy},     {"ary", K3R, D.ary},     {"bex", K3R, D.bex},     {"q2", K3D, D.q2},
+      {"q2p", K3D, D.q2p},     {"q2pp", K3D, D.q2pp},   {"q2l", K3D, D.q2l},     {"q2lp", K3D, D.q2lp},
+      {"q2lpp", K3D, D.q2lpp}, {"s", K2D, D.s},         {"sp", K2D, D.sp},       {"spp", K2D, D.spp},
+      {"sx", K2D, D.sx},       {"sy", K2D, D.sy},       {"dd", K2D, D.dd},       {"qq", K2D, D.qq},
+      {"rr", K2D, D.rr},       {"eagx", K2D, D.eagx},   {"earx", K2D, D.earx},   {"eagy", K2D, D.eagy},
+      {"eary", K2D, D.eary},   {"hhs", K2D, (void *)D.hhs}, {"hhu", K2D, (void *)D.hhu},
+      {"hhv", K2D, (void *)D.hhv}, {"kmz", KI, (void *)D.kmz}, {"k1z", KI, D.k1z}, {"k1u", KI, D.k1u},
+      {"k1v", KI, D.k1v},
+  };
+}
+
+static bool find_field(s3g_ctx *c, const char *name, Field &out) {
+  auto tab = (c->precision == S3G_FP64) ? field_table(c->d64) : field_table(c->d32);
+  for (auto &f : tab)
+    if (std::strcmp(f.name, name) == 0) {
+      out = f;
+      return true;
+    }
+  return false;
+}
+
+// Reference layout (km1, lm1) k-fastest  <->  device planes [k][c].
+template <typename HT, typename DT>
+static int upload3(s3g_ctx *c, DT *dst, const HT *src) {
+  const int km1 = c->km1, lm1 = c->lm + 1;
+  std::vector<DT> buf((size_t)c->KP * c->LP, DT(0));
+  for (int l = 0; l < lm1; ++l)
+    for (int k = 1; k <= km1; ++k) buf[(size_t)k * c->LP + l] = (DT)src[(size_t)l * km1 + (k - 1)];
+  return hip_check(hipMemcpy(dst, buf.data(), buf.size() * sizeof(DT), hipMemcpyHostToDevice), "upload3");
+}
+
+template <typename HT, typename DT>
+static int download3(s3g_ctx *c, HT *dst, const DT *src) {
+  const int km1 = c->km1, lm1 = c->lm + 1;
+  std::vector<DT> buf((size_t)c->KP * c->LP);
+  if (int e = hip_check(hipMemcpy(buf.data(), src, buf.size() * sizeof(DT), hipMemcpyDeviceToHost), "download3")) return e;
+  for (int l = 0; l < lm1; ++l)
+    for (int k = 1; k <= km1; ++k) dst[(size_t)l * km1 + (k - 1)] = (HT)buf[(size_t)k * c->LP + l];
+  return 0;
+}
+
+template <typename HT>
+static int set_field(s3g_ctx *c, const char *name, const HT *src) {
+  Field f;
+  if (!find_field(c, name, f)) {
+    set_error(std::string("unknown field ") + name);
+    return S3G_ERR_ARG;
+  }
+  const int lm1 = c->lm + 1;
+  switch (f.kind) {
+    case K3R:
+      if (c->precision == S3G_FP64) return upload3(c, (double *)f.ptr, src);
+      return upload3(c, (float *)f.ptr, src);
+    case K3D: return upload3(c, (double *)f.ptr, src);
+    case K2D: {
+      std::vector<double> b(c->LP, 0.0);
+      for (int l = 0; l < lm1; ++l) b[l] = (double)src[l];
+      return hip_check(hipMemcpy(f.ptr, b.data(), c->LP * sizeof(double), hipMemcpyHostToDevice), "set2");
+    }
+    case KI: {
+      std::vector<int> b(c->LP, c->km1);
+      for (int l = 0; l < lm1; ++l) b[l] = (int)src[l];
+      return hip_check(hipMemcpy(f.ptr, b.data(), c->LP * sizeof(int), hipMemcpyHostToDevice), "seti");
+    }
+  }
+  return S3G_ERR_ARG;
+}
+
+template <typename HT>
+static int get_field(s3g_ctx *c, const char *name, HT *dst) {
+  Field f;
+  if (!find_field(c, name, f)) {
+    set_error(std::string("unknown field ") + name);
+    return S3G_ERR_ARG;
+  }
+  if (int e = hip_check(hipStreamSynchronize(c->stream), "sync")) return e;
+  const int lm1 = c->lm + 1;
+  switch (f.kind) {
+    case K3R:
+      if (c->precision == S3G_FP64) return download3(c, dst, (const double *)f.ptr);
+      return download3(c, dst, (const float *)f.ptr);
+    case K3D: return download3(c, dst, (const double *)f.ptr);
+    case K2D: {
+      std::vector<double> b(c->LP);
+      if (int e = hip_check(hipMemcpy(b.data(), f.ptr, c->LP * sizeof(double), hipMemcpyDeviceToHost), "get2")) return e;
+      for (int l = 0; l < lm1; ++l) dst[l] = (HT)b[l];
+      return 0;
+    }
+    case KI: {
+      std::vector<int> b(c->LP);
+      if (int e = hip_check(hipMemcpy(b.data(), f.ptr, c->LP * sizeof(int), hipMemcpyDeviceToHost), "geti")) return e;
+      for (int l = 0; l < lm1; ++l) dst[l] = (HT)b[l];
+      return 0;
+    }
+  }
+  return S3G_ERR_ARG;
+}
+
+// Allocate the device state of one lake in storage type R.
+template <typename R>
+static int alloc_state(s3g_ctx *c, Dev<R> &D, const int *lE, const int *lW, const int *lN, const int *lS,
+                       const int *kmz, const int *k1z, const int *k1u, const int *k1v, const double *zl,
+                       const double *hhs, const double *hhu, const double *hhv) {
+  const size_t n3 = (size_t)c->KP * c->LP, n2 = c->LP;
+  D.lm = c->lm;
+  D.LP = c->LP;
+  D.km1 = c->km1;
+  D.km = c->km;
+  D.k1 = c->k1;
+  D.P = c->P;
+  auto up_i = [&](const int *src) -> int * {
+    int *d = dalloc<int>(c, n2);
+    if (d) hipMemcpy(d, src, n2 * sizeof(int), hipMemcpyHostToDevice);
+    return d;
+  };
+  auto up_d = [&](const double *src, size_t n) -> double * {
+    double *d = dalloc<double>(c, n);
+    if (d) hipMemcpy(d, src, n * sizeof(double), hipMemcpyHostToDevice);
+    return d;
+  };
+  D.lE = up_i(lE);
+  D.lW = up_i(lW);
+  D.lN = up_i(lN);
+  D.lS = up_i(lS);
+  D.kmz = up_i(kmz);
+  D.k1z = up_i(k1z);
+  D.k1u = up_i(k1u);
+  D.k1v = up_i(k1v);
+  D.zlevel = up_d(zl, c->KP);
+  D.hhs = up_d(hhs, n2);
+  D.hhu = up_d(hhu, n2);
+  D.hhv = up_d(hhv, n2);
+  double **d2[] = {&D.s, &D.sp, &D.spp, &D.sx, &D.sy, &D.dd, &D.qq, &D.rr, &D.eagx, &D.earx, &D.eagy, &D.eary};
+  for (double **p : d2)
+    if (!(*p = dalloc<double>(c, n2))) return S3G_ERR_HIP;
+  R **d3[] = {&D.h,    &D.hp,   &D.hpp,  &D.hu,  &D.hup,  &D.hupp, &D.hv,  &D.hvp,  &D.hvpp, &D.uh,
+              &D.uhp,  &D.uhpp, &D.u,    &D.up,  &D.upp,  &D.vh,   &D.vhp, &D.vhpp, &D.v,    &D.vp,
+              &D.vpp,  &D.wp,   &D.sal,  &D.salp, &D.salpp, &D.rhop, &D.Av, &D.Dv,   &D.kh,   &D.ex,
+              &D.agx,  &D.arx,  &D.agy,  &D.ary, &D.fx_ew, &D.fx_ns, &D.bex, &D.t0,  &D.t1};
+  for (R **p : d3)
+    if (!(*p = dalloc<R>(c, n3))) return S3G_ERR_HIP;
+  double **q3[] = {&D.q2, &D.q2p, &D.q2pp, &D.q2l, &D.q2lp, &D.q2lpp, &D.d0, &D.d1, &D.d2};
+  for (double **p : q3)
+    if (!(*p = dalloc<double>(c, c->P.iturb > 0 ? n3 : 1))) return S3G_ERR_HIP;
+  std::vector<double> one(n2, 1.0);
+  hipMemcpy(D.rr, one.data(), n2 * sizeof(double), hipMemcpyHostToDevice);
+  return hip_check(hipGetLastError(), "alloc");
+}
+
+}  // namespace s3g
+
+using namespace s3g;
+
+extern "C" {
+
+const char *s3g_last_error(void) { return g_err.c_str(); }
+
+int s3g_init(const s3g_grid *g, const s3g_params *p, s3g_ctx **out) {
+  if (!g || !p || !out || g->lm <= 0 || g->km1 < 3) {
+    set_error("s3g_init: invalid grid");
+    return S3G_ERR_ARG;
+  }
+  if (p->precision != S3G_FP64 && p->precision != S3G_MIXED) {
+    set_error("s3g_init: unknown precision");
+    return S3G_ERR_ARG;
+  }
+  if (p->itrmom < 1 || p->itrmom > 4) {
+    set_error("s3g_init: itrmom must be 1..4");
+    return S3G_ERR_UNSUPPORTED;
+  }
+  if (p->iturb < 0) {
+    set_error("s3g_init: iturb < 0 (GOTM) is not supported");
+    return S3G_ERR_UNSUPPORTED;
+  }
+  if (p->ifsurfbc != 0 && p->ifsurfbc != 20) {
+    set_error("s3g_init: only ifsurfbc = 0 or 20 is supported");
+    return S3G_ERR_UNSUPPORTED;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    set_error("s3g_init: no HIP device is visible");
+    return S3G_ERR_HIP;
+  }
+  s3g_ctx *c = new s3g_ctx();
+  c->precision = p->precision;
+  c->prec_kind = p->preconditioner;
+  c->im1 = g->im1;
+  c->jm1 = g->jm1;
+  c->km1 = g->km1;
+  c->km = g->km1 - 1;
+  c->k1 = 2;
+  c->lm = g->lm;
+  c->LP = ((g->lm + 1 + 255) / 256) * 256;
+  c->KP = g->km1 + 3;
+  c->pcg_tol = p->pcg_tol > 0 ? p->pcg_tol : 1e-10;
+  c->pcg_maxit = p->pcg_maxit > 0 ? p->pcg_maxit : 4000;
+  Params &P = c->P;
+  P.dt = p->dt;
+  P.dx = p->dx;
+  P.dy = p->dy;
+  P.twodt = 2.0 * P.dt;
+  P.dtdx = P.dt / P.dx;
+  P.dtdy = P.dt / P.dy;
+  P.gdtdx = 9.806 * P.dtdx;
+  P.gdtdy = 9.806 * P.dtdy;
+  P.gdt2dx2 = P.gdtdx * P.dtdx;
+  P.gdt2dy2 = P.gdtdy * P.dtdy;
+  P.twodx = 2.0 * P.dx;
+  P.twody = 2.0 * P.dy;
+  P.fourdx = 2.0 * P.twodx;
+  P.fourdy = 2.0 * P.twody;
+  P.dxdx = P.dx * P.dx;
+  P.dydy = P.dy * P.dy;
+  P.dxdy = P.dx * P.dy;
+  P.beta = p->beta;
+  P.beta2 = p->beta / 2.0;
+  P.itrap = p->itrap;
+  P.niter = p->niter;
+  P.ismooth = p->ismooth;
+  P.iturb = p->iturb;
+  P.iadv = p->iadv;
+  P.itrmom = p->itrmom;
+  P.ihd = p->ihd;
+  P.ibc = p->ibc;
+  P.isal = p->isal;
+  P.ifsurfbc = p->ifsurfbc;
+  P.Av0 = p->Av0;
+  P.Dv0 = p->Dv0;
+  P.Ax0 = p->Ax0;
+  P.Ay0 = p->Ay0;
+  P.f = p->f;
+  P.theta = p->theta;
+  P.cd = p->cd;
+  P.cw = p->cw;
+  P.wa = p->wa;
+  P.phi = p->phi;
+  P.dtsurfbc = p->dtsurfbc;
+  // closure constants, si3d_mixing.f90:641-645
+  const double A1 = 0.92, A2 = 0.74, B1 = 16.6, B2 = 10.1, C2 = 0.70, C3 = 0.20;
+  P.t_1 = A2 * (1. - 6. * A1 / B1);
+  P.t_2 = 3. * A2 * (6. * A1 + B2 * (1. - C3));
+  P.t_3 = std::pow(B1, -1. / 3.);
+  P.t_4 = 9. * A1 * (2. * A1 + A2 * (1. - C2));
+  P.t_5 = 9. * A1 * A2;
+  if (p->n_surfbc > 0 && p->surfbc) c->surfbc.assign(p->surfbc, p->surfbc + 3 * p->n_surfbc);
+  if (c->P.ifsurfbc == 20 && c->surfbc.size() >= 3) {
+    c->cdw = c->surfbc[0];
+    c->uair = c->surfbc[1];
+    c->vair = c->surfbc[2];
+  }
+  if (hip_check(hipStreamCreate(&c->stream), "stream") < 0) {
+    delete c;
+    return S3G_ERR_HIP;
+  }
+  hipHostMalloc((void **)&c->h_iters, sizeof(int) * 4, hipHostMallocDefault);
+  hipHostMalloc((void **)&c->h_scal, sizeof(double) * 4096, hipHostMallocDefault);
+  // geometry in device numbering (0-based columns, sentinel = lm)
+  const int lm = c->lm, LP = c->LP, S = lm;
+  std::vector<int> lE(LP, S), lW(LP, S), lN(LP, S), lS(LP, S), kmz(LP, c->km1), k1z(LP, c->km1),
+      k1u(LP, c->km1), k1v(LP, c->km1);
+  std::vector<double> hhs(LP, 0.0), hhu(LP, 0.0), hhv(LP, 0.0), zl(c->KP, 1e30);
+  long nwet = 0;
+  for (int l = 0; l < lm; ++l) {
+    auto nb = [&](const int *a) { int v = a[l] - 1; return (v < 0 || v >= lm) ? S : v; };
+    lE[l] = nb(g->lEC);
+    lW[l] = nb(g->lWC);
+    lN[l] = nb(g->lNC);
+    lS[l] = nb(g->lSC);
+    kmz[l] = g->kmz[l];
+    k1z[l] = g->k1z[l];
+    k1u[l] = g->k1u[l];
+    k1v[l] = g->k1v[l];
+    hhs[l] = g->hhs[l];
+    hhu[l] = g->hhu[l];
+    hhv[l] = g->hhv[l];
+    nwet += std::max(0, g->kmz[l] - g->k1z[l] + 1);
+  }
+  zl[0] = -100.0;
+  for (int k = 1; k <= c->km1; ++k) zl[k] = g->zlevel[k - 1];
+  c->n_wet = nwet;
+  int rc = (c->precision == S3G_FP64)
+               ? alloc_state(c, c->d64, lE.data(), lW.data(), lN.data(), lS.data(), kmz.data(), k1z.data(),
+                             k1u.data(), k1v.data(), zl.data(), hhs.data(), hhu.data(), hhv.data())
+               : alloc_state(c, c->d32, lE.data(), lW.data(), lN.data(), lS.data(), kmz.data(), k1z.data(),
+                             k1u.data(), k1v.data(), zl.data(), hhs.data(), hhu.data(), hhv.data());
+  Pcg &G = c->pcg;
+  const size_t n2 = LP;
+  double **pv[] = {&G.r, &G.z, &G.p0, &G.p1, &G.q, &G.x, &G.lc, &G.ld};
+  for (double **q : pv)
+    if (rc == 0 && !(*q = dalloc<double>(c, n2))) rc = S3G_ERR_HIP;
+  if (rc == 0) rc = pcg_setup(c, lN.data());
+  if (rc == 0 && !(G.part = dalloc<double>(c, 8 * (size_t)G.nblk + 64))) rc = S3G_ERR_HIP;
+  if (rc == 0 && !(G.iters = dalloc<int>(c, 4))) rc = S3G_ERR_HIP;
+  if (rc != 0) {
+    if (g_err.empty()) set_error("s3g_init: device allocation failed");
+    s3g_finalize(c);
+    return rc;
+  }
+  c->stats.lm = lm;
+  c->stats.km1 = c->km1;
+  c->stats.n_wet = nwet;
+  *out = c;
+  return hip_check(hipDeviceSynchronize(), "init");
+}
+
+int s3g_set_field_f32(s3g_ctx *c, const char *n, const float *h) { return set_field(c, n, h); }
+int s3g_set_field_f64(s3g_ctx *c, const char *n, const double *h) { return set_field(c, n, h); }
+int s3g_set_field_i32(s3g_ctx *c, const char *n, const int *h) { return set_field(c, n, h); }
+int s3g_get_field_f32(s3g_ctx *c, const char *n, float *h) { return get_field(c, n, h); }
+int s3g_get_field_f64(s3g_ctx *c, const char *n, double *h) { return get_field(c, n, h); }
+int s3g_get_field_i32(s3g_ctx *c, const char *n, int *h) { return get_field(c, n, h); }
+
+int s3g_fd(s3g_ctx *c, int n, int istep, int iter, int lastiter, double thrs) {
+  (void)iter;
+  if (!c || (istep != 1 && istep != 2)) return S3G_ERR_ARG;
+  c->thrs = thrs;
+  c->stats.fd_calls++;
+  return c->precision == S3G_FP64 ? s3g::fd<double>(c, n, istep, lastiter) : s3g::fd<float>(c, n, istep, lastiter);
+}
+
+int s3g_settrap(s3g_ctx *c) {
+  if (!c) return S3G_ERR_ARG;
+  return c->precision == S3G_FP64 ? s3g::settrap<double>(c) : s3g::settrap<float>(c);
+}
+
+int s3g_settrap2(s3g_ctx *c) {
+  if (!c) return S3G_ERR_ARG;
+  return c->precision == S3G_FP64 ? s3g::settrap2<double>(c) : s3g::settrap2<float>(c);
+}
+
+int s3g_save(s3g_ctx *c, int istep) {
+  if (!c || (istep != 1 && istep != 2)) return S3G_ERR_ARG;
+  return c->precision == S3G_FP64 ? s3g::save<double>(c, istep) : s3g::save<float>(c, istep);
+}
+
+int s3g_phase(s3g_ctx *c, int ph, int n, int istep, double thrs) {
+  if (!c || (istep != 1 && istep != 2)) return S3G_ERR_ARG;
+  c->thrs = thrs;
+  return c->precision == S3G_FP64 ? s3g::phase<double>(c, ph, n, istep) : s3g::phase<float>(c, ph, n, istep);
+}
+
+// One step of the si3d.f90 time loop (324-393).
+int s3g_step(s3g_ctx *c, int n) {
+  if (!c) return S3G_ERR_ARG;
+  const double thrs = (double)n * c->P.dt / 3600.;
+  int niter1 = c->P.niter, lastiter = 0, rc = 0, worst = 0;
+  auto acc = [&](int r) {
+    if (r < 0) return false;
+    worst = std::max(worst, r);
+    return true;
+  };
+  if (n == 1) {
+    niter1 = std::max(c->P.niter, 2);
+  } else {
+    const bool lf_only = (c->P.itrap == 0) || (n % std::max(c->P.itrap, 1) != 0);
+    if (lf_only) lastiter = 1;
+    if (!acc(rc = s3g_fd(c, n, 1, 0, lastiter, thrs))) return rc;
+    if (lf_only) {
+      if (!acc(rc = s3g_save(c, 1))) return rc;
+      c->stats.steps++;
+      return worst;
+    }
+    if (!acc(rc = s3g_settrap(c))) return rc;
+  }
+  for (int iter = 1; iter <= niter1; ++iter) {
+    if (iter == niter1) lastiter = 1;
+    if (!acc(rc = s3g_fd(c, n, 2, iter, lastiter, thrs))) return rc;
+    if (iter < niter1)
+      if (!acc(rc = s3g_settrap2(c))) return rc;
+  }
+  if (!acc(rc = s3g_save(c, 2))) return rc;
+  c->stats.steps++;
+  return worst;
+}
+
+int s3g_run(s3g_ctx *c, int n0, int nsteps) {
+  int worst = 0;
+  for (int n = n0; n < n0 + nsteps; ++n) {
+    int rc = s3g_step(c, n);
+    if (rc < 0) return rc;
+    worst = std::max(worst, rc);
+  }
+  return worst;
+}
+
+int s3g_synchronize(s3g_ctx *c) { return c ? hip_check(hipStreamSynchronize(c->stream), "sync") : S3G_ERR_ARG; }
+
+int s3g_get_stats(s3g_ctx *c, s3g_stats *out) {
+  if (!c || !out) return S3G_ERR_ARG;
+  std::vector<double> s(c->lm + 1);
+  if (int e = get_field(c, "s", s.data())) return e;
+  double v = 0.0;
+  for (int l = 0; l < c->lm; ++l) v += s[l];
+  c->stats.zeta_volume = v;
+  *out = c->stats;
+  return 0;
+}
+
+int s3g_reset_timers(s3g_ctx *c, int enable) {
+  if (!c) return S3G_ERR_ARG;
+  if (!c->pending.empty()) timers_flush(c);
+  c->timers.clear();
+  c->timing = enable != 0;
+  return 0;
+}
+
+int s3g_kernel_times(s3g_ctx *c, int max, char (*names)[32], double *avg_ms, long *launches) {
+  if (!c) return S3G_ERR_ARG;
+  timers_flush(c);
+  int n = std::min<int>(max, (int)c->timers.size());
+  for (int i = 0; i < n; ++i) {
+    std::snprintf(names[i], 32, "%s", c->timers[i].name.c_str());
+    avg_ms[i] = c->timers[i].n ? c->timers[i].total_ms / c->timers[i].n : 0.0;
+    launches[i] = c->timers[i].n;
+  }
+  return n;
+}
+
+void s3g_finalize(s3g_ctx *c) {
+  if (!c) return;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (void *p : c->allocs) hipFree(p);
+  for (auto &e : c->ev_pool) {
+    hipEventDestroy(e.first);
+    hipEventDestroy(e.second);
+  }
+  if (c->h_iters) hipHostFree(c->h_iters);
+  if (c->h_scal) hipHostFree(c->h_scal);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+}  // extern "C"

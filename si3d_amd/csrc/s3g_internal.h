@@ -1,0 +1,124 @@
+// s3g_internal.h -- device state and kernel entry points of libsi3dgpu.so.
+//
+// Device layout (HBM): every 3-D field is a stack of horizontal planes, one
+// plane per layer k (k = 0 .. km1+2, reference numbering, planes 0 and km1+1,
+// km1+2 are zero padding), each plane holding the wet columns in the
+// reference's l order (i outer, j inner, si3d_utils.f90:781-793) at index
+// c = l-1, followed by the dry sentinel column c = lm.  The plane stride LP is
+// lm+1 rounded up to 256 columns.  With j running fastest, north/south
+// neighbours are adjacent columns and east/west neighbours sit one row of wet
+// cells away, so a wavefront of 64 consecutive columns reads every neighbour
+// plane with coalesced runs.  One lane owns one column and walks it in k.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <string>
+#include <vector>
+
+#include "../../include/si3d_gpu.h"
+
+namespace s3g {
+
+constexpr int kBlock = 256;
+
+// Run parameters and derived constants (si3d_utils.f90:299-310).
+struct Params {
+  double dt, dx, dy, twodt, dtdx, dtdy, gdtdx, gdtdy, gdt2dx2, gdt2dy2, twodx, twody, fourdx,
+      fourdy, dxdx, dydy, dxdy, beta2;
+  int itrap, niter, ismooth, iturb, iadv, itrmom, ihd, ibc, isal, ifsurfbc;
+  double beta, Av0, Dv0, Ax0, Ay0, f, theta, cd, cw, wa, phi, dtsurfbc;
+  double t_1, t_2, t_3, t_4, t_5;  // turbulence closure constants (si3d_mixing.f90:641-645)
+};
+
+// Everything a kernel needs, passed by value.  R is the storage type of the
+// 3-D hydrodynamic fields (double, or float in the mixed-precision mode).  The
+// free surface, the 2-D solver arrays and the turbulence quantities are FP64
+// in every mode.
+template <typename R>
+struct Dev {
+  int lm, LP, km1, km, k1;
+  Params P;
+  double uair, vair, cdw;  // spatially uniform forcing (surfbc cases 0, 20)
+  const int *lE, *lW, *lN, *lS, *kmz;
+  int *k1z, *k1u, *k1v;
+  const double *zlevel;  // [km1+3], zlevel[km1+1] = +huge
+  const double *hhs, *hhu, *hhv;
+  double *s, *sp, *spp, *sx, *sy, *dd, *qq, *rr, *eagx, *earx, *eagy, *eary;
+  R *h, *hp, *hpp, *hu, *hup, *hupp, *hv, *hvp, *hvpp;
+  R *uh, *uhp, *uhpp, *u, *up, *upp, *vh, *vhp, *vhpp, *v, *vp, *vpp, *wp;
+  R *sal, *salp, *salpp, *rhop, *Av, *Dv, *kh, *ex, *agx, *arx, *agy, *ary;
+  R *fx_ew, *fx_ns, *bex, *t0, *t1;
+  double *q2, *q2p, *q2pp, *q2l, *q2lp, *q2lpp, *d0, *d1, *d2;
+};
+
+// Free-surface solver work space (FP64).
+struct Pcg {
+  int n, nblk;
+  double *r, *z, *p0, *p1, *q, *diag, *x;
+  double *part;    // partial sums: [4][nblk]
+  double *scal;    // device scalars
+  int *iters;      // device iteration counter
+  float *rf, *zf, *pf0, *pf1, *qf, *sxf, *syf, *rrf, *diagf;  // FP32 inner solve
+  // line preconditioner: tridiagonal factors along N-S lines
+  int nlines;
+  int *line_start;  // [nlines+1] first column of each line (lines are contiguous in l)
+  double *lc, *ld;  // modified super-diagonal and inverse pivots
+  float *lcf, *ldf;
+};
+
+struct Timer {
+  std::string name;
+  double total_ms = 0.0;
+  long n = 0;
+};
+
+}  // namespace s3g
+
+struct s3g_ctx {
+  int precision = S3G_FP64;
+  int prec_kind = S3G_PREC_JACOBI;
+  int im1 = 0, jm1 = 0, km1 = 0, km = 0, k1 = 2, lm = 0, LP = 0, KP = 0;
+  long n_wet = 0;
+  s3g::Params P{};
+  std::vector<double> surfbc;  // 3*n records
+  double pcg_tol = 1e-10;
+  int pcg_maxit = 4000;
+  double thrs = 0.0;
+  double uair = 0.0, vair = 0.0, cdw = 0.0;
+  hipStream_t stream = nullptr;
+  // device memory
+  std::vector<void *> allocs;
+  s3g::Dev<double> d64{};
+  s3g::Dev<float> d32{};
+  s3g::Pcg pcg{};
+  int *d_l2i = nullptr, *d_l2j = nullptr;
+  // statistics
+  s3g_stats stats{};
+  // timers
+  bool timing = false;
+  std::vector<s3g::Timer> timers;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
+  std::vector<std::pair<int, int>> pending;  // (timer index, event pair index)
+  int *h_iters = nullptr;                    // pinned
+  double *h_scal = nullptr;                  // pinned
+};
+
+namespace s3g {
+// kernels.hip
+template <typename R> int fd(s3g_ctx *c, int n, int istep, int lastiter);
+template <typename R> int phase(s3g_ctx *c, int ph, int n, int istep);
+template <typename R> int settrap(s3g_ctx *c);
+template <typename R> int settrap2(s3g_ctx *c);
+template <typename R> int save(s3g_ctx *c, int istep);
+int init_density(s3g_ctx *c);
+// pcg.hip
+template <typename R> int solve_free_surface(s3g_ctx *c, double tz);
+int pcg_setup(s3g_ctx *c, const int *lN0);
+// host
+void set_error(const std::string &msg);
+int hip_check(hipError_t e, const char *what);
+void timer_begin(s3g_ctx *c, const char *name, int *slot);
+void timer_end(s3g_ctx *c, int slot);
+void surfbc(s3g_ctx *c, int n, int istep);
+}  // namespace s3g

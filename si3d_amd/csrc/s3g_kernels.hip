@@ -1,0 +1,1286 @@
+// s3g_kernels.hip -- gfx950 kernels of the pSi3D time step (everything fd,
+// settrap, settrap2 and save do except the free-surface solve).
+//
+// One lane owns one wet column c and walks it in k; a 256-lane workgroup covers
+// 256 consecutive columns.  Column-local recurrences (Thomas sweeps, the
+// cumulative baroclinic term, continuity's bottom-up integration) stay in
+// registers; the forward-sweep factors go to a per-column scratch plane stack
+// in the same coalesced layout and are read back by the same lane.
+#include <cfloat>
+#include <cmath>
+
+#include "s3g_internal.h"
+
+namespace s3g {
+
+#define IX(k, c) ((size_t)(k) * (size_t)D.LP + (size_t)(c))
+
+constexpr double G = 9.806, RHOAIR = 1.3, KAPPAS = 0.41, PI = 3.1415926535897932;
+constexpr double SED_H = 0.5, AV_MOL = 1.3e-6, DV_MOL = 1.4e-7;
+constexpr double Q2_MIN = 2.e-6, Q2L_MIN = 6.80236723501e-9, GH_MAX = 2.9e-2, GH_MIN = -2.8e-1;
+constexpr double B_1 = 16.6, E_1 = 1.80, E_2 = 1.33, E_3 = 0.25, SQ = 0.20;
+
+__device__ __forceinline__ int col() { return blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
+__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+
+// UNESCO/Gill EOS with the fixed-point pressure iteration (si3d_utils.f90:3865-3949),
+// evaluated in FP64 in every precision mode.  salinity**(3/2) is salinity**1 in the
+// reference (integer exponent 3/2).
+__device__ double densty(double t, double s, double elev) {
+  double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t;
+  double densw = 999.842594 + 6.793952e-2 * t - 9.095290e-3 * t2 + 1.001685e-4 * t3 -
+                 1.120083e-6 * t4 + 6.536332e-9 * t5;
+  double densws = densw + s * (0.824493 - 4.0899e-3 * t + 7.6438e-5 * t2 - 8.2467e-7 * t3 + 5.3875e-9 * t4) +
+                  s * (-5.72466e-3 + 1.0227e-4 * t - 1.6546e-6 * t2) + 4.8314e-4 * s * s;
+  double kw = 19652.21 + 148.4206 * t - 2.327105 * t2 + 1.360477e-2 * t3 - 5.155288e-5 * t4;
+  double ks = kw + s * (54.6746 - 0.603459 * t + 1.09987e-2 * t2 - 6.1670e-5 * t3) +
+              s * (7.944e-2 + 1.6483e-2 * t - 5.3009e-4 * t2);
+  double kp1 = 3.239908 + 1.43713e-3 * t + 1.16092e-4 * t2 - 5.77905e-7 * t3 +
+               s * (2.2838e-3 - 1.0981e-5 * t - 1.6078e-6 * t2) + 1.91075e-4 * s;
+  double kp2 = (8.50935e-5 - 6.12293e-6 * t + 5.2787e-8 * t2) + s * (-9.9348e-7 + 2.0816e-8 * t + 9.1697e-10 * t2);
+  double guess = densws, rho = densws, delta = 1.0;
+  for (int it = 0; delta > 1e-6 && it < 10000; ++it) {
+    double p = 1e-5 * (guess * 9.806 * elev);
+    double k = ks + p * kp1 + p * p * kp2;
+    rho = densws / (1.0 - p / k);
+    delta = fabs(rho - guess);
+    guess = rho;
+  }
+  return rho;
+}
+
+template <typename R>
+__device__ __forceinline__ double zdens(const Dev<R> &D, int k, int c) {
+  double zl = D.zlevel[k];
+  return (zl == -100.0) ? 0.5 * (double)D.hp[IX(k, c)] : zl + 0.5 * (double)D.hp[IX(k, c)];
+}
+
+// ---------------------------------------------------------------------------
+// Horizontal eddy coefficient of the momentum stencil at column w, layer k
+// (exmom's haxpp / haypp, si3d_procedures.f90:632-690 and 966-1026), evaluated
+// on the fly.  X: x-momentum (u-faces), else y-momentum.  DX: the x-direction
+// coefficient (haxpp), else haypp.
+template <typename R, bool X, bool DX>
+__device__ __forceinline__ double hcoef(const Dev<R> &D, int k, int w) {
+  const int S = D.lm;
+  if (w == S) return 0.0;
+  const int wn = X ? D.lE[w] : D.lN[w];  // the face neighbour of w
+  if (wn == S) return 0.0;
+  const int k1q = X ? D.k1u[w] : D.k1v[w];
+  const int kmq = min(D.kmz[wn], D.kmz[w]);
+  if (k < k1q || k > kmq) return 0.0;
+  const R *hq = X ? D.hupp : D.hvpp;
+  const int other = DX ? D.lE[w] : D.lN[w];
+  double hmin = dmin((double)hq[IX(k, other)], (double)hq[IX(k, w)]);
+  if (D.P.ihd == 1) return (DX ? D.P.Ax0 : D.P.Ay0) * hmin;
+  if (D.P.ihd > 1) {
+    const bool single = (X && DX) || (!X && !DX);
+    if (single) return (double)D.kh[IX(k, other)] * hmin;
+    double k4 = ((double)D.kh[IX(k, D.lE[w])] + (double)D.kh[IX(k, w)] + (double)D.kh[IX(k, D.lN[w])] +
+                 (double)D.kh[IX(k, D.lE[D.lN[w]])]) * 0.25;
+    return k4 * hmin;
+  }
+  return 0.0;
+}
+
+// exmom: explicit advection, Coriolis and horizontal diffusion of the x (X) or
+// y momentum equation, si3d_procedures.f90:578-1173.
+template <typename R, bool X>
+__global__ void __launch_bounds__(kBlock) k_exmom(Dev<R> D, double twodt1) {
+  const int c = col();
+  if (c >= D.lm) return;
+  const int S = D.lm;
+  const int lE = D.lE[c], lW = D.lW[c], lN = D.lN[c], lS = D.lS[c];
+  const int ln = X ? lE : lN;
+  const int kmq = min(D.kmz[ln], D.kmz[c]);
+  const int k1q = X ? D.k1u[c] : D.k1v[c];
+  R *ex = D.ex;
+  if (ln == S) {
+    for (int k = 1; k <= D.km1; ++k) ex[IX(k, c)] = R(0);
+    return;
+  }
+  for (int k = 1; k < k1q; ++k) ex[IX(k, c)] = R(0);
+  for (int k = kmq + 1; k <= D.km1; ++k) ex[IX(k, c)] = R(0);
+  const R *q = X ? D.up : D.vp;
+  const R *qpp = X ? D.upp : D.vpp;
+  const R *hq = X ? D.hup : D.hvp;
+  const int l2 = X ? D.lS[lE] : D.lW[lN];  // SE corner (x) / NW corner (y)
+  const double f = D.P.f;
+  for (int k = k1q; k <= kmq; ++k) {
+    if ((double)D.hp[IX(k, c)] <= 0.0 || (double)D.hp[IX(k, ln)] <= 0.0) {
+      ex[IX(k, c)] = X ? D.uhpp[IX(k, c)] : D.vhpp[IX(k, c)];
+      continue;
+    }
+    double cor, uE, uW, vN, vS, wU, wD;
+    if (X) {
+      cor = 0.25 * f * ((double)D.vhp[IX(k, lE)] + (double)D.vhp[IX(k, c)] + (double)D.vhp[IX(k, l2)] + (double)D.vhp[IX(k, lS)]);
+      uE = (double)D.uhp[IX(k, lE)] + (double)D.uhp[IX(k, c)];
+      uW = (double)D.uhp[IX(k, c)] + (double)D.uhp[IX(k, lW)];
+      vN = (double)D.vhp[IX(k, lE)] + (double)D.vhp[IX(k, c)];
+      vS = (double)D.vhp[IX(k, l2)] + (double)D.vhp[IX(k, lS)];
+      wU = (double)D.wp[IX(k, lE)] + (double)D.wp[IX(k, c)];
+      wD = (double)D.wp[IX(k + 1, lE)] + (double)D.wp[IX(k + 1, c)];
+    } else {
+      cor = 0.25 * f * ((double)D.uhp[IX(k, lN)] + (double)D.uhp[IX(k, c)] + (double)D.uhp[IX(k, l2)] + (double)D.uhp[IX(k, lW)]);
+      uE = (double)D.uhp[IX(k, lN)] + (double)D.uhp[IX(k, c)];
+      uW = (double)D.uhp[IX(k, l2)] + (double)D.uhp[IX(k, lW)];
+      vN = (double)D.vhp[IX(k, lN)] + (double)D.vhp[IX(k, c)];
+      vS = (double)D.vhp[IX(k, c)] + (double)D.vhp[IX(k, lS)];
+      wU = (double)D.wp[IX(k, lN)] + (double)D.wp[IX(k, c)];
+      wD = (double)D.wp[IX(k + 1, lN)] + (double)D.wp[IX(k + 1, c)];
+    }
+    if (k == k1q) wU = 0.0;
+    if (k == kmq) wD = 0.0;
+    const double qc = q[IX(k, c)];
+    double adv = 0.0;
+    const int it = D.P.itrmom;
+    if (it == 1 || it == 3) {
+      double scE = (double)q[IX(k, lE)] + qc, scW = (double)q[IX(k, lW)] + qc;
+      double scN = (double)q[IX(k, lN)] + qc, scS = (double)q[IX(k, lS)] + qc;
+      adv = (uE * scE - uW * scW) / D.P.fourdx + (vN * scN - vS * scS) / D.P.fourdy;
+    } else {
+      const double pc = qpp[IX(k, c)];
+      adv = ((uE + fabs(uE)) * pc + (uE - fabs(uE)) * (double)qpp[IX(k, lE)] -
+             (uW + fabs(uW)) * (double)qpp[IX(k, lW)] - (uW - fabs(uW)) * pc) / D.P.fourdx +
+            ((vN + fabs(vN)) * pc + (vN - fabs(vN)) * (double)qpp[IX(k, lN)] -
+             (vS + fabs(vS)) * (double)qpp[IX(k, lS)] - (vS - fabs(vS)) * pc) / D.P.fourdy;
+    }
+    if (it == 1 || it == 4) {
+      // vertical weights th/th1 (680-689): th(k1q)=0, th(kmq+1)=0.5
+      const double hkm = hq[IX(k - 1, c)], hk = hq[IX(k, c)], hkp = hq[IX(k + 1, c)];
+      double th = (k == k1q) ? 0.0 : hkm / (hkm + hk);
+      double th1 = (k == k1q) ? 0.0 : 1.0 - th;
+      double thp = (k == kmq) ? 0.5 : hk / (hk + hkp);
+      double th1p = (k == kmq) ? 0.5 : 1.0 - thp;
+      adv += (wU * (th * qc + th1 * (double)q[IX(k - 1, c)]) - wD * (thp * (double)q[IX(k + 1, c)] + th1p * qc)) / 2.;
+    } else if (it == 2) {
+      const double pc = qpp[IX(k, c)];
+      adv += ((wU + fabs(wU)) * pc + (wU - fabs(wU)) * (double)qpp[IX(k - 1, c)]) / 4. -
+             ((wD + fabs(wD)) * (double)qpp[IX(k + 1, c)] + (wD - fabs(wD)) * pc) / 4.;
+    } else {  // it == 3
+      const double hkm = hq[IX(k - 1, c)], hk = hq[IX(k, c)], hkp = hq[IX(k + 1, c)];
+      double scU = (qc * hk + (double)q[IX(k - 1, c)] * hkm) / (hk + hkm);
+      double scD = (qc * hk + (double)q[IX(k + 1, c)] * hkp) / (hk + hkp);
+      adv += (wU * scU - wD * scD) / 2.;
+    }
+    double hd = 0.0;
+    if (D.P.ihd >= 1) {
+      const double pc = qpp[IX(k, c)];
+      const double axc = hcoef<R, X, true>(D, k, c), axw = hcoef<R, X, true>(D, k, lW);
+      const double ayc = hcoef<R, X, false>(D, k, c), ays = hcoef<R, X, false>(D, k, lS);
+      if (D.P.ihd == 1) {
+        hd = (ayc * ((double)qpp[IX(k, lN)] - pc) - ays * (pc - (double)qpp[IX(k, lS)])) / D.P.dydy +
+             (axc * ((double)qpp[IX(k, lE)] - pc) - axw * (pc - (double)qpp[IX(k, lW)])) / D.P.dxdx;
+      } else if (X) {
+        hd = 2. * (axc * ((double)D.upp[IX(k, lE)] - pc) - axw * (pc - (double)D.upp[IX(k, lW)])) / D.P.dxdx +
+             (ayc * ((double)D.upp[IX(k, lN)] - pc) - ays * (pc - (double)D.upp[IX(k, lS)])) / D.P.dydy +
+             (ayc * ((double)D.vpp[IX(k, lE)] - (double)D.vpp[IX(k, c)]) -
+              ays * ((double)D.vpp[IX(k, D.lE[lS])] - (double)D.vpp[IX(k, lS)])) / D.P.dxdy;
+      } else {
+        hd = 2. * (ayc * ((double)D.vpp[IX(k, lN)] - pc) - ays * (pc - (double)D.vpp[IX(k, lS)])) / D.P.dydy +
+             (axc * ((double)D.vpp[IX(k, lE)] - pc) - axw * (pc - (double)D.vpp[IX(k, lW)])) / D.P.dxdx +
+             (axc * ((double)D.upp[IX(k, lN)] - (double)D.upp[IX(k, c)]) -
+              axw * ((double)D.upp[IX(k, D.lN[lW])] - (double)D.upp[IX(k, lW)])) / D.P.dxdy;
+      }
+    }
+    if (X)
+      ex[IX(k, c)] = (R)((double)D.uhpp[IX(k, c)] - twodt1 * (adv * D.P.iadv - cor - hd));
+    else
+      ex[IX(k, c)] = (R)((double)D.vhpp[IX(k, c)] - twodt1 * (adv * D.P.iadv + cor - hd));
+  }
+}
+
+// bclnc_km: baroclinic term of the bottom layer on a horizontal plane,
+// si3d_procedures.f90:1633-1746.
+template <typename R, bool X>
+__device__ double bclnc_km(const Dev<R> &D, int c, int ln, int kb, double plain) {
+  const double eps = FLT_EPSILON;  // EPSILON of a default REAL
+  const double hq = X ? (double)D.hup[IX(kb, c)] : (double)D.hvp[IX(kb, c)];
+  const double hplus = hq + eps, hminus = hq - eps;
+  bool ce = false, cw = false;
+  double rn = D.rhop[IX(kb, ln)], rw = D.rhop[IX(kb, c)];
+  const double hpn = D.hp[IX(kb, ln)], hpc = D.hp[IX(kb, c)];
+  if (hpn > hplus || hpn < hminus) {
+    ce = true;
+    double h1 = D.hp[IX(kb - 1, ln)];
+    double dz1 = 0.5 * (h1 + hpn), dz2 = 0.5 * (h1 + hq);
+    double r1 = D.rhop[IX(kb - 1, ln)];
+    rn = r1 + (dz2 / dz1) * ((double)D.rhop[IX(kb, ln)] - r1);
+  }
+  if (hpc > hplus || hpc < hminus) {
+    cw = true;
+    double h1 = D.hp[IX(kb - 1, c)];
+    double dz1 = 0.5 * (h1 + hpc), dz2 = 0.5 * (h1 + hq);
+    double r1 = D.rhop[IX(kb - 1, c)];
+    rw = r1 + (dz2 / dz1) * ((double)D.rhop[IX(kb, c)] - r1);
+  }
+  if (ce || cw) return (X ? D.P.gdtdx : D.P.gdtdy) * hq * (rn - rw);
+  return plain;
+}
+
+// matmom: per-face tridiagonal momentum system and its two-RHS double sweep,
+// si3d_procedures.f90:1176-1630 with trid 3210-3289.  Pass A: column-mean
+// density; pass B: assemble each row and eliminate downward (c1 -> t0, d1/e1 ->
+// ag/ar in place); pass C: back-substitute upward and sum the column.
+template <typename R, bool X>
+__global__ void __launch_bounds__(kBlock) k_matmom(Dev<R> D, double tz) {
+  const int c = col();
+  if (c >= D.lm) return;
+  const int S = D.lm;
+  const int lE = D.lE[c], lW = D.lW[c], lN = D.lN[c], lS = D.lS[c];
+  const int ln = X ? lE : lN;
+  if (ln == S) return;
+  const int kmq = min(D.kmz[ln], D.kmz[c]);
+  const int k1q = X ? D.k1u[c] : D.k1v[c];
+  const int nwl = kmq - k1q + 1;
+  if (nwl < 1) return;
+  const double twodt1 = D.P.twodt * tz, theta = D.P.theta;
+  const R *hq = X ? D.hup : D.hvp;
+  const R *qpp = X ? D.upp : D.vpp;
+  R *AG = X ? D.agx : D.agy;
+  R *AR = X ? D.arx : D.ary;
+  const double gq = X ? D.P.gdtdx : D.P.gdtdy;
+
+  double rsum = 0.0;
+  for (int k = k1q; k <= kmq; ++k) rsum += 1000. + (double)D.rhop[IX(k, c)];
+  const double rhoq = rsum / nwl;
+  const double ws0 = rhoq * gq * (D.spp[ln] - D.spp[c]);
+
+  // wind stress at the top face (1348-1360 / 1541-1553)
+  double usurf, vsurf, wind_g;
+  if (X) {
+    const int lSE = D.lS[lE];
+    usurf = D.up[IX(k1q, c)];
+    vsurf = ((double)D.vp[IX(D.k1v[c], c)] + (double)D.vp[IX(D.k1v[lS], lS)] + (double)D.vp[IX(D.k1v[lE], lE)] +
+             (double)D.vp[IX(D.k1v[lSE], lSE)]) / 4.;
+  } else {
+    const int lNW = D.lW[lN];
+    vsurf = D.vp[IX(k1q, c)];
+    usurf = ((double)D.up[IX(D.k1u[c], c)] + (double)D.up[IX(D.k1u[lW], lW)] + (double)D.up[IX(D.k1u[lN], lN)] +
+             (double)D.up[IX(D.k1u[lNW], lNW)]) / 4.;
+  }
+  const double cwq = D.cdw * RHOAIR * sqrt((D.vair - vsurf) * (D.vair - vsurf) + (D.uair - usurf) * (D.uair - usurf));
+  wind_g = cwq / rhoq * (X ? D.uair : D.vair) * twodt1;
+  const double wind_a = cwq / rhoq / (double)hq[IX(k1q, c)] * twodt1;
+  // quadratic bottom stress (1364-1369 / 1557-1562)
+  double taub;
+  if (X) {
+    double vb = 0.25 * ((double)D.vhpp[IX(kmq, lE)] + (double)D.vhpp[IX(kmq, c)] + (double)D.vhpp[IX(kmq, lS)] +
+                        (double)D.vhpp[IX(kmq, D.lS[lE])]);
+    double ub = D.uhpp[IX(kmq, c)];
+    double hb = D.hup[IX(kmq, c)];
+    taub = D.P.cd * sqrt(ub * ub + vb * vb) / (hb * hb);
+  } else {
+    double ub = 0.25 * ((double)D.uhpp[IX(kmq, lN)] + (double)D.uhpp[IX(kmq, D.lW[lN])] + (double)D.uhpp[IX(kmq, lW)] +
+                        (double)D.uhpp[IX(kmq, c)]);
+    double vb = D.vhpp[IX(kmq, c)];
+    double hb = D.hvp[IX(kmq, c)];
+    taub = D.P.cd * sqrt(vb * vb + ub * ub) / (hb * hb);
+  }
+  const double bot_a = taub * twodt1;
+
+  double bcl = 0.0, hdr_prev = 0.0, c1p = 0.0, d1p = 0.0, e1p = 0.0;
+  for (int k = k1q; k <= kmq; ++k) {
+    const double hkm = hq[IX(k - 1, c)], hk = hq[IX(k, c)], hkp = hq[IX(k + 1, c)];
+    const double Avk = (k > k1q) ? 0.5 * ((double)D.Av[IX(k, ln)] + (double)D.Av[IX(k, c)]) : 0.0;
+    const double Avkp = (k < kmq) ? 0.5 * ((double)D.Av[IX(k + 1, ln)] + (double)D.Av[IX(k + 1, c)]) : 0.0;
+    if (D.P.ibc != 0) {
+      double hdr = gq * hk * ((double)D.rhop[IX(k, ln)] - (double)D.rhop[IX(k, c)]);
+      if (k == kmq && kmq > k1q) hdr = bclnc_km<R, X>(D, c, ln, kmq, hdr);
+      bcl = (k == k1q) ? hdr : bcl + (hdr_prev + hdr);
+      hdr_prev = hdr;
+    }
+    double vd = 0.0;
+    if (nwl > 1) {
+      double dk = (k > k1q) ? Avk * ((double)qpp[IX(k - 1, c)] - (double)qpp[IX(k, c)]) / (hkm + hk) : 0.0;
+      double dkp = (k < kmq) ? Avkp * ((double)qpp[IX(k, c)] - (double)qpp[IX(k + 1, c)]) / (hk + hkp) : 0.0;
+      vd = D.P.twodt * (dk - dkp) * 2. * (1. - theta);
+    }
+    const double hh = hk / rhoq;
+    double gg = (double)D.ex[IX(k, c)] - hh * (bcl + ws0) * tz + vd * tz;
+    double a1 = 0.0, a2 = 1.0, a3 = 0.0;
+    if (nwl > 1) {
+      a3 = (k < kmq) ? -twodt1 * Avkp / (hkp * (hk + hkp)) * 2. * theta : 0.0;
+      a1 = (k > k1q) ? -twodt1 * Avk / (hkm * (hkm + hk)) * 2. * theta : 0.0;
+      a2 = 1.0 - (hkm / hk) * a1 - (hkp / hk) * a3;
+    }
+    if (k == k1q) {
+      gg += wind_g;
+      a2 += wind_a;
+    }
+    if (k == kmq) a2 += bot_a;
+    double c1, d1, e1;
+    if (k == k1q) {
+      c1 = a3 / a2;
+      d1 = gg / a2;
+      e1 = hh / a2;
+    } else {
+      const double den = a2 - a1 * c1p;
+      c1 = a3 / den;
+      d1 = (gg - a1 * d1p) / den;
+      e1 = (hh - a1 * e1p) / den;
+    }
+    D.t0[IX(k, c)] = (R)c1;
+    AG[IX(k, c)] = (R)d1;
+    AR[IX(k, c)] = (R)e1;
+    c1p = c1;
+    d1p = d1;
+    e1p = e1;
+  }
+  double xg = d1p, xr = e1p, sg = xg, sr = xr;
+  AG[IX(kmq, c)] = (R)xg;
+  AR[IX(kmq, c)] = (R)xr;
+  for (int k = kmq - 1; k >= k1q; --k) {
+    const double c1 = D.t0[IX(k, c)];
+    xg = (double)AG[IX(k, c)] - c1 * xg;
+    xr = (double)AR[IX(k, c)] - c1 * xr;
+    AG[IX(k, c)] = (R)xg;
+    AR[IX(k, c)] = (R)xr;
+    sg += xg;
+    sr += xr;
+  }
+  if (X) {
+    D.eagx[c] = sg;
+    D.earx[c] = sr;
+  } else {
+    D.eagy[c] = sg;
+    D.eary[c] = sr;
+  }
+}
+
+// matcon: pentadiagonal free-surface coefficients, si3d_procedures.f90:1749-1867.
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_matcon(Dev<R> D, double tz) {
+  const int c = col();
+  if (c >= D.lm) return;
+  const int S = D.lm;
+  const int lE = D.lE[c], lW = D.lW[c], lN = D.lN[c], lS = D.lS[c];
+  const double cx = D.P.gdt2dx2 * tz * tz * 1000., cy = D.P.gdt2dy2 * tz * tz * 1000.;
+  const double sxc = (lE != S) ? cx * D.earx[c] : 0.0;
+  const double syc = (lN != S) ? cy * D.eary[c] : 0.0;
+  const double sxw = (lW != S) ? cx * D.earx[lW] : 0.0;  // E of lW is c: wet
+  const double sys = (lS != S) ? cy * D.eary[lS] : 0.0;
+  double sum = 0.0;
+  const int kms = D.kmz[c];
+  for (int k = D.k1; k <= kms; ++k)
+    sum += ((double)D.uhpp[IX(k, c)] - (double)D.uhpp[IX(k, lW)]) / D.P.dx +
+           ((double)D.vhpp[IX(k, c)] - (double)D.vhpp[IX(k, lS)]) / D.P.dy;
+  const double dd = D.P.dt * tz * sum;
+  const double eW = (lW != S) ? D.eagx[lW] : 0.0, eS = (lS != S) ? D.eagy[lS] : 0.0;
+  D.sx[c] = sxc;
+  D.sy[c] = syc;
+  D.dd[c] = dd;
+  D.qq[c] = D.spp[c] - D.P.dtdx * tz * (D.eagx[c] - eW) - D.P.dtdy * tz * (D.eagy[c] - eS) - dd;
+  D.rr[c] = 1.0 + sxc + sxw + syc + sys;
+}
+
+// slayer_h: new surface-layer thicknesses after the zeta solve (drying only),
+// si3d_procedures.f90:2070-2175, including the re-test of the same u-layer (2136).
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_slayer_h(Dev<R> D) {
+  const int c = col();
+  if (c > D.lm) return;
+  const int S = D.lm, km1 = D.km1;
+  if (c == S) {
+    for (int k = 0; k <= km1 + 1; ++k) {
+      D.h[IX(k, c)] = D.hp[IX(k, c)];
+      D.hu[IX(k, c)] = D.hup[IX(k, c)];
+      D.hv[IX(k, c)] = D.hvp[IX(k, c)];
+    }
+    return;
+  }
+  const int k1z = D.k1z[c], k1u = D.k1u[c], k1v = D.k1v[c];
+  for (int k = 1; k <= km1; ++k) {
+    if (k < k1z || k >= k1z + 2) D.h[IX(k, c)] = D.hp[IX(k, c)];
+    if (k < k1u || k >= k1u + 1) D.hu[IX(k, c)] = D.hup[IX(k, c)];
+    if (k < k1v || k >= k1v + 2) D.hv[IX(k, c)] = D.hvp[IX(k, c)];
+  }
+  const double sc = D.s[c];
+  int k = k1z;
+  double haux = dmin(D.zlevel[k + 1], D.hhs[c]) + sc;
+  if (haux <= 0.0) {
+    D.h[IX(k, c)] = R(0);
+    k = k + 1;
+    haux = dmin(D.zlevel[k + 1], D.hhs[c]) + sc;
+    D.h[IX(k, c)] = (haux <= 0.0) ? R(0) : (R)haux;
+  } else {
+    D.h[IX(k, c)] = (R)haux;
+  }
+  const int lE = D.lE[c], lN = D.lN[c];
+  if (lE != S) {
+    k = k1u;
+    haux = dmin(D.zlevel[k + 1], D.hhu[c]) + dmax(sc, D.s[lE]);
+    D.hu[IX(k, c)] = (haux <= 0.0) ? R(0) : (R)haux;
+  } else {
+    for (int kk = k1u; kk <= min(k1u + 1, km1); ++kk) D.hu[IX(kk, c)] = D.hup[IX(kk, c)];
+  }
+  if (lN != S) {
+    k = k1v;
+    haux = dmin(D.zlevel[k + 1], D.hhv[c]) + dmax(sc, D.s[lN]);
+    if (haux <= 0.0) {
+      D.hv[IX(k, c)] = R(0);
+      k = k + 1;
+      haux = dmin(D.zlevel[k + 1], D.hhv[c]) + dmax(sc, D.s[lN]);
+      D.hv[IX(k, c)] = (haux <= 0.0) ? R(0) : (R)haux;
+    } else {
+      D.hv[IX(k, c)] = (R)haux;
+    }
+  } else {
+    for (int kk = k1v; kk <= min(k1v + 1, km1); ++kk) D.hv[IX(kk, c)] = D.hvp[IX(kk, c)];
+  }
+}
+
+// vel: fluxes from the new zeta plus the top-cell drying merge,
+// si3d_procedures.f90:2430-2545.
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_vel(Dev<R> D, double tz) {
+  const int c = col();
+  if (c >= D.lm) return;
+  const int S = D.lm;
+  const int lE = D.lE[c], lN = D.lN[c];
+  const double sc = D.s[c];
+  if (lE != S) {
+    const int kmx = min(D.kmz[lE], D.kmz[c]), k0 = D.k1u[c];
+    const double cxx = D.P.gdtdx * tz * 1000. * (D.s[lE] - sc);
+    for (int k = k0; k <= kmx; ++k) D.uh[IX(k, c)] = (R)((double)D.agx[IX(k, c)] - cxx * (double)D.arx[IX(k, c)]);
+    if (k0 <= kmx && (double)D.hu[IX(k0, c)] <= 0.0) {
+      D.uh[IX(k0 + 1, c)] = D.uh[IX(k0, c)] + D.uh[IX(k0 + 1, c)];
+      D.uh[IX(k0, c)] = R(0);
+      D.k1u[c] = k0 + 1;
+    }
+  }
+  if (lN != S) {
+    const int kmy = min(D.kmz[lN], D.kmz[c]), k0 = D.k1v[c];
+    const double cyy = D.P.gdtdy * tz * 1000. * (D.s[lN] - sc);
+    for (int k = k0; k <= kmy; ++k) D.vh[IX(k, c)] = (R)((double)D.agy[IX(k, c)] - cyy * (double)D.ary[IX(k, c)]);
+    if (k0 <= kmy && (double)D.hv[IX(k0, c)] <= 0.0) {
+      D.vh[IX(k0 + 1, c)] = D.vh[IX(k0, c)] + D.vh[IX(k0 + 1, c)];
+      D.vh[IX(k0, c)] = R(0);
+      D.k1v[c] = k0 + 1;
+    }
+  }
+}
+
+// continuity: vertical velocity by bottom-up integration of the horizontal
+// flux divergence, si3d_procedures.f90:2635-2745 (case 1 from uh and uhpp, case
+// 2 from uhp), accumulated in FP64.  Case 1 also updates k1z for drying (vel,
+// 2547-2561).
+template <typename R, int IST>
+__global__ void __launch_bounds__(kBlock) k_continuity(Dev<R> D) {
+  const int c = col();
+  if (c >= D.lm) return;
+  const int kms = D.kmz[c], k1s = D.k1z[c], lW = D.lW[c], lS = D.lS[c];
+  if (k1s <= kms) {
+    auto div = [&](int k) -> double {
+      if (IST == 1)
+        return ((double)D.uh[IX(k, c)] - (double)D.uh[IX(k, lW)] + (double)D.uhpp[IX(k, c)] - (double)D.uhpp[IX(k, lW)]) / D.P.twodx +
+               ((double)D.vh[IX(k, c)] - (double)D.vh[IX(k, lS)] + (double)D.vhpp[IX(k, c)] - (double)D.vhpp[IX(k, lS)]) / D.P.twody;
+      return ((double)D.uhp[IX(k, c)] - (double)D.uhp[IX(k, lW)]) / D.P.dx +
+             ((double)D.vhp[IX(k, c)] - (double)D.vhp[IX(k, lS)]) / D.P.dy;
+    };
+    for (int k = 0; k <= D.km1 + 1; ++k)
+      if (k < k1s || k > kms) D.wp[IX(k, c)] = R(0);
+    double w = 0.0;
+    for (int k = kms; k >= k1s; --k) {
+      w -= div(k);
+      if (k > k1s) D.wp[IX(k, c)] = (R)w;
+    }
+    for (int k = D.k1; k <= k1s - 1; ++k) w -= div(k);
+    D.wp[IX(k1s, c)] = (R)w;
+  }
+  if (IST == 1) {
+    const int k0 = D.k1z[c];
+    if ((double)D.h[IX(k0, c)] <= 0.0) D.k1z[c] = min(k0 + 1, D.km1);
+  }
+}
+
+// Superbee limiter C_f (si3d_procedures.f90:2928) and the TVD face flux.
+__device__ __forceinline__ double tvd_flux(double vel, double s1, double s2, double s3, double s4,
+                                           double cfdef, double courant) {
+  double cf = cfdef;
+  const double d23 = s3 - s2;
+  if (d23 != 0.0) {
+    double ratio = (vel >= 0.0) ? (s2 - s1) / d23 : (s4 - s3) / d23;
+    cf = dmax(0.0, dmax(dmin(1.0, 2. * ratio), dmin(2.0, ratio)));
+  }
+  return vel / 2. * (s3 + s2) - ((1. - cf) * fabs(vel) + vel * vel * courant * cf) * d23 / 2.;
+}
+
+// exsal (part 1): east-west and north-south TVD fluxes at the faces of each
+// column, si3d_procedures.f90:2885-2975.  Only the wet range of each column is
+// rewritten; the arrays persist across calls like the reference's Bth2/Bth4.
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_exsal_hflux(Dev<R> D, double twodt1) {
+  const int c = col();
+  if (c >= D.lm) return;
+  const int kms = D.kmz[c], k1s = D.k1z[c];
+  const int lE = D.lE[c], lW = D.lW[c], lN = D.lN[c], lS = D.lS[c];
+  const int lEE = D.lE[lE], lNN = D.lN[lN];
+  for (int k = k1s; k <= kms; ++k) {
+    double fe = 0.0, fn = 0.0;
+    if ((double)D.hup[IX(k, c)] > 0.0) {
+      const double vel = ((double)D.uhpp[IX(k, c)] + (double)D.uh[IX(k, c)]) / 2.;
+      const double s2 = D.salpp[IX(k, c)], s3 = D.salpp[IX(k, lE)];
+      const double s1 = ((double)D.hpp[IX(k, lW)] <= 0.0) ? s2 : (double)D.salpp[IX(k, lW)];
+      const double s4 = ((double)D.hpp[IX(k, lEE)] <= 0.0) ? s3 : (double)D.salpp[IX(k, lEE)];
+      fe = tvd_flux(vel, s1, s2, s3, s4, 0.0, twodt1 / D.P.dx);
+    }
+    if ((double)D.hvp[IX(k, c)] > 0.0) {
+      const double vel = ((double)D.vhpp[IX(k, c)] + (double)D.vh[IX(k, c)]) / 2.;
+      const double s2 = D.salpp[IX(k, c)], s3 = D.salpp[IX(k, lN)];
+      const double s1 = ((double)D.hpp[IX(k, lS)] <= 0.0) ? s2 : (double)D.salpp[IX(k, lS)];
+      const double s4 = ((double)D.hpp[IX(k, lNN)] <= 0.0) ? s3 : (double)D.salpp[IX(k, lNN)];
+      fn = tvd_flux(vel, s1, s2, s3, s4, 0.0, twodt1 / D.P.dy);
+    }
+    D.fx_ew[IX(k, c)] = (R)fe;
+    D.fx_ns[IX(k, c)] = (R)fn;
+  }
+}
+
+// Vertical TVD flux through the top of cell k (si3d_procedures.f90:2999-3035).
+template <typename R>
+__device__ __forceinline__ double vflux(const Dev<R> &D, int c, int k, int k1s, double twodt1) {
+  if (!((double)D.hp[IX(k - 1, c)] > 0.0)) return 0.0;
+  const double vel = (k == k1s) ? 0.0 : (double)D.wp[IX(k, c)];
+  const double s2 = D.salpp[IX(k, c)], s3 = D.salpp[IX(k - 1, c)];
+  const double s4 = ((double)D.hpp[IX(k - 2, c)] <= 0.0) ? s3 : (double)D.salpp[IX(k - 2, c)];
+  const double s1 = ((double)D.hpp[IX(k + 1, c)] <= 0.0) ? s2 : (double)D.salpp[IX(k + 1, c)];
+  const double delz = ((double)D.hp[IX(k, c)] + (double)D.hp[IX(k - 1, c)]) / 2.;
+  return tvd_flux(vel, s1, s2, s3, s4, 1.0, twodt1 / delz);
+}
+
+// exsal (part 2) fused with imsal: flux divergence + horizontal diffusion form
+// the right-hand side (3038-3067), then the implicit vertical diffusion column
+// solve (imsal, 3079-3207, trid1 3292-3352) produces sal.  With DO_SOLVE false
+// only Bex is written (phase-level checking of exsal alone).
+template <typename R, bool DO_EX, bool DO_SOLVE>
+__global__ void __launch_bounds__(kBlock) k_exsal_imsal(Dev<R> D, double twodt1) {
+  const int c = col();
+  if (c >= D.lm) return;
+  const int S = D.lm;
+  const int kms = D.kmz[c], k1s = D.k1z[c];
+  const int lE = D.lE[c], lW = D.lW[c], lN = D.lN[c], lS = D.lS[c];
+  const int nwl = kms - k1s + 1;
+  if (DO_EX) {
+    for (int k = 1; k <= D.km1; ++k)
+      if (k < k1s || k > kms) D.bex[IX(k, c)] = R(0);
+    auto bh = [&](const R *hq, double a0, int w, int k) -> double {
+      return (w == S || k > D.kmz[w]) ? 0.0 : a0 * (double)hq[IX(k, w)];
+    };
+    double fud_k = (k1s <= kms) ? vflux(D, c, k1s, k1s, twodt1) : 0.0;
+    for (int k = k1s; k <= kms; ++k) {
+      const double fud_kp = (k + 1 <= kms) ? vflux(D, c, k + 1, k1s, twodt1) : 0.0;
+      const double sc = D.salpp[IX(k, c)];
+      double b = (double)D.hpp[IX(k, c)] * sc / twodt1 -
+                 ((double)D.fx_ew[IX(k, c)] - (lW == S ? 0.0 : (double)D.fx_ew[IX(k, lW)])) / D.P.dx -
+                 ((double)D.fx_ns[IX(k, c)] - (lS == S ? 0.0 : (double)D.fx_ns[IX(k, lS)])) / D.P.dy -
+                 (fud_k - fud_kp);
+      if (D.P.ihd > 0) {
+        b += (bh(D.hupp, D.P.Ax0, c, k) * ((double)D.salpp[IX(k, lE)] - sc) -
+              bh(D.hupp, D.P.Ax0, lW, k) * (sc - (double)D.salpp[IX(k, lW)])) / D.P.dxdx +
+             (bh(D.hvpp, D.P.Ay0, c, k) * ((double)D.salpp[IX(k, lN)] - sc) -
+              bh(D.hvpp, D.P.Ay0, lS, k) * (sc - (double)D.salpp[IX(k, lS)])) / D.P.dydy;
+      }
+      D.bex[IX(k, c)] = (R)b;
+      fud_k = fud_kp;
+    }
+  }
+  if (!DO_SOLVE || nwl < 1) return;
+  // imsal: layer thicknesses at time n with the surface-cell corrections (3124-3132)
+  auto hn = [&](int k) -> double {
+    if (k == k1s) return twodt1 * (double)D.wp[IX(k1s, c)] + (double)D.hpp[IX(k1s, c)];
+    if (k == k1s + 1 && (double)D.hpp[IX(k1s, c)] <= 0.0) return D.hpp[IX(k1s + 1, c)];
+    return D.h[IX(k, c)];
+  };
+  if (nwl == 1) {
+    double a = hn(k1s) / twodt1;
+    double v = (double)D.bex[IX(k1s, c)] / a;
+    if ((double)D.h[IX(k1s, c)] < 1.E-2) v = D.salpp[IX(k1s, c)];
+    D.sal[IX(k1s, c)] = (R)v;
+    return;
+  }
+  // forward e/f sweep: e -> t0, f -> sal (in place), then back substitution
+  double hk = hn(k1s), hkp = hn(k1s + 1);
+  double ep = 0.0, fp = 0.0;
+  for (int k = k1s; k <= kms; ++k) {
+    const double hkn = (k + 1 <= kms) ? hn(k + 1) : 0.0;  // not used when k == kms
+    const double a1 = (k > k1s) ? -(double)D.Dv[IX(k, c)] / (hn(k - 1) + hk) * 2. : 0.0;
+    const double a3 = (k < kms) ? -(double)D.Dv[IX(k + 1, c)] / (hk + hkp) * 2. : 0.0;
+    const double a2 = hk / twodt1 - a1 - a3;
+    const double d = D.bex[IX(k, c)];
+    if (k == k1s) {
+      ep = -a3 / a2;
+      fp = d / a2;
+    } else if (k < kms) {
+      const double den = a2 + a1 * ep;
+      ep = -a3 / den;
+      fp = (d - a1 * fp) / den;
+    } else {
+      fp = (d - a1 * fp) / (a2 + a1 * ep);  // bottom value
+      ep = 0.0;
+    }
+    D.t0[IX(k, c)] = (R)ep;
+    D.sal[IX(k, c)] = (R)fp;
+    hk = hkp;
+    hkp = (k + 2 <= kms) ? hn(k + 2) : 0.0;
+    (void)hkn;
+  }
+  double x = fp;
+  for (int k = kms - 1; k >= k1s; --k) {
+    x = (double)D.t0[IX(k, c)] * x + (double)D.sal[IX(k, c)];
+    D.sal[IX(k, c)] = (R)x;
+  }
+  for (int k = D.k1; k <= k1s - 1; ++k) D.sal[IX(k, c)] = (R)x;
+  D.sal[IX(kms + 1, c)] = D.sal[IX(kms, c)];
+}
+
+// vel2: layer_h (si3d_procedures.f90:2181-2231) then the top-cell flux split for
+// wetting (2570-2631).
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_vel2(Dev<R> D) {
+  const int c = col();
+  if (c >= D.lm) return;
+  const int S = D.lm;
+  const int lE = D.lE[c], lN = D.lN[c];
+  const double sc = D.s[c];
+  for (int k = D.k1; k <= D.kmz[c]; ++k) {
+    double v = dmin(D.zlevel[k + 1], D.hhs[c]) - dmax(D.zlevel[k], -sc);
+    D.h[IX(k, c)] = (v <= 0.0) ? R(0) : (R)v;
+  }
+  if (lE != S) {
+    const int kmx = min(D.kmz[c], D.kmz[lE]);
+    const double sm = dmax(sc, D.s[lE]);
+    for (int k = D.k1; k <= kmx; ++k) {
+      double v = dmin(D.zlevel[k + 1], D.hhu[c]) - dmax(D.zlevel[k], -sm);
+      D.hu[IX(k, c)] = (v <= 0.0) ? R(0) : (R)v;
+    }
+    const int k = D.k1u[c];
+    const double hm = D.hu[IX(k - 1, c)];
+    if (hm > 0.0) {
+      const double t = (double)D.uh[IX(k, c)] / ((double)D.hu[IX(k, c)] + hm);
+      D.uh[IX(k, c)] = (R)(t * (double)D.hu[IX(k, c)]);
+      D.uh[IX(k - 1, c)] = (R)(t * hm);
+    }
+  }
+  if (lN != S) {
+    const int kmy = min(D.kmz[c], D.kmz[lN]);
+    const double sm = dmax(sc, D.s[lN]);
+    for (int k = D.k1; k <= kmy; ++k) {
+      double v = dmin(D.zlevel[k + 1], D.hhv[c]) - dmax(D.zlevel[k], -sm);
+      D.hv[IX(k, c)] = (v <= 0.0) ? R(0) : (R)v;
+    }
+    const int k = D.k1v[c];
+    const double hm = D.hv[IX(k - 1, c)];
+    if (hm > 0.0) {
+      const double t = (double)D.vh[IX(k, c)] / ((double)D.hv[IX(k, c)] + hm);
+      D.vh[IX(k, c)] = (R)(t * (double)D.hv[IX(k, c)]);
+      D.vh[IX(k - 1, c)] = (R)(t * hm);
+    }
+  }
+}
+
+// UpdateHorizontalMixingCoefficients (Smagorinsky), si3d_mixing.f90:894-966.
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_hmix(Dev<R> D) {
+  const int c = col();
+  if (c >= D.lm) return;
+  const int lE = D.lE[c], lW = D.lW[c], lN = D.lN[c], lS = D.lS[c];
+  for (int k = D.k1z[c]; k <= D.kmz[c]; ++k) {
+    double khv;
+    if ((double)D.hp[IX(k, lW)] <= 0.0 || (double)D.hp[IX(k, lE)] <= 0.0 || (double)D.hp[IX(k, lS)] <= 0.0 ||
+        (double)D.hp[IX(k, lN)] <= 0.0) {
+      khv = D.P.Ax0;
+    } else {
+      double dudx = ((double)D.upp[IX(k, c)] - (double)D.upp[IX(k, lW)]) / D.P.dx;
+      double dvdy = ((double)D.vpp[IX(k, c)] - (double)D.vpp[IX(k, lS)]) / D.P.dy;
+      double dudy = (((double)D.upp[IX(k, lN)] + (double)D.upp[IX(k, D.lN[lW])]) -
+                     ((double)D.upp[IX(k, lS)] + (double)D.upp[IX(k, D.lS[lW])])) / D.P.fourdy;
+      double dvdx = (((double)D.vpp[IX(k, lE)] + (double)D.vpp[IX(k, D.lE[lS])]) -
+                     ((double)D.vpp[IX(k, lW)] + (double)D.vpp[IX(k, D.lW[lS])])) / D.P.fourdx;
+      khv = dmax(D.P.Ax0, 0.03 * D.P.dx * D.P.dy * sqrt(dudx * dudx + dvdy * dvdy + .5 * (dudy + dvdx) * (dudy + dvdx)));
+    }
+    D.kh[IX(k, c)] = (R)khv;
+  }
+}
+
+// Two-equation turbulence closure: SolveTKEandLengthScale (si3d_mixing.f90:
+// 301-551, tridT 554-612) and the eddy coefficients of UpdateMixingCoefficients
+// (245-290).  FP64 throughout, as the reference's REAL*8 turbulence state.
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_turb(Dev<R> D, int istep) {
+  const int m = col();
+  if (m >= D.lm) return;
+  const int kms = D.kmz[m], k1s = D.k1z[m], nwl = kms - k1s + 1;
+  if (nwl <= 1) return;
+  const double twodt1 = D.P.twodt / istep, r3 = 3. / twodt1;
+  const int lW = D.lW[m], lS = D.lS[m];
+  double *XL = D.d0, *SP = D.d1, *BP = D.d2;
+  double htot = 0.0, hbelow = 0.0;
+  for (int k = k1s; k <= kms; ++k) htot += D.hp[IX(k, m)];
+  for (int k = k1s + 1; k <= kms; ++k) hbelow += D.hp[IX(k, m)];
+  // pass 1: N^2, shear, limited length scale, productions
+  for (int k = k1s + 1; k <= kms; ++k) {
+    const double hkm = D.hp[IX(k - 1, m)], hk = D.hp[IX(k, m)];
+    const double dz = 0.5 * (hkm + hk);
+    const double rkm = D.rhop[IX(k - 1, m)], rk = D.rhop[IX(k, m)];
+    const double N2 = -G / (0.5 * (rkm + rk) + 1000.) * ((rkm - rk) / dz);
+    const double dudz = 0.5 * ((double)D.up[IX(k - 1, m)] + (double)D.up[IX(k - 1, lW)] - (double)D.up[IX(k, m)] - (double)D.up[IX(k, lW)]) / dz;
+    const double dvdz = 0.5 * ((double)D.vp[IX(k - 1, m)] + (double)D.vp[IX(k - 1, lS)] - (double)D.vp[IX(k, m)] - (double)D.vp[IX(k, lS)]) / dz;
+    const double q2pk = D.q2p[IX(k, m)];
+    double xl = D.q2lp[IX(k, m)] / q2pk;
+    if (N2 > 0.0) xl = dmin(xl, 0.53 * sqrt(q2pk / N2));
+    XL[IX(k, m)] = xl;
+    SP[IX(k, m)] = (double)D.Av[IX(k, m)] * (dudz * dudz + dvdz * dvdz);
+    BP[IX(k, m)] = -(double)D.Dv[IX(k, m)] * N2;
+  }
+  // boundary values (445-457)
+  const double b1_23 = pow(B_1, 2. / 3.);
+  double ub = 0.5 * ((double)D.u[IX(kms, m)] + (double)D.u[IX(kms, lW)]);
+  double vb = 0.5 * ((double)D.v[IX(kms, m)] + (double)D.v[IX(kms, lS)]);
+  const double ds_bot = 3. * b1_23 * (D.P.cd * (ub * ub + vb * vb)) / twodt1;
+  double us = 0.5 * ((double)D.up[IX(D.k1u[m], m)] + (double)D.up[IX(D.k1u[lW], lW)]);
+  double vs = 0.5 * ((double)D.vp[IX(D.k1v[m], m)] + (double)D.vp[IX(D.k1v[lS], lS)]);
+  const double ds_top = 3. * b1_23 *
+                        (D.cdw * RHOAIR * ((D.uair - us) * (D.uair - us) + (D.vair - vs) * (D.vair - vs)) / 1000.) / twodt1;
+  // pass 2: rows k1s..kms+1 of both systems, forward e/f sweeps together.
+  // Row k1s and row kms+1 are the Dirichlet rows (a1 = a3 = 0, a2 = 3/twodt1).
+  double e1 = 0.0, f1 = ds_top / r3, e2 = 0.0, f2 = 0.0;  // row k1s
+  D.q2[IX(k1s, m)] = f1;
+  D.q2l[IX(k1s, m)] = f2;
+  XL[IX(k1s, m)] = 0.0;  // xl(k1s) = 0 (e1 of row k1s)
+  SP[IX(k1s, m)] = 0.0;  // e2 of row k1s
+  double xlk = 0.0, dco_prev = 0.0, zfromt = 0.0;
+  {
+    // dco(k1s) = Sq*0.5*(xl(k1s)+xl(k1s+1))*sqrt(0.5*(q2p(k1s)+q2p(k1s+1)))
+    const double xl1 = (k1s + 1 <= kms) ? XL[IX(k1s + 1, m)] : 0.0;
+    dco_prev = SQ * 0.5 * (0.0 + xl1) * sqrt(0.5 * (D.q2p[IX(k1s, m)] + D.q2p[IX(k1s + 1, m)]));
+  }
+  for (int k = k1s + 1; k <= kms; ++k) {
+    xlk = XL[IX(k, m)];
+    const double xlp = (k + 1 <= kms) ? XL[IX(k + 1, m)] : 0.0;
+    const double q2pk = D.q2p[IX(k, m)];
+    const double dco = SQ * 0.5 * (xlk + xlp) * sqrt(0.5 * (q2pk + D.q2p[IX(k + 1, m)]));
+    const double hk = D.hp[IX(k, m)], hkm = D.hp[IX(k - 1, m)];
+    const double a3 = -dco / (hk * 0.5 * (hk + hkm));
+    const double a1 = -dco_prev / (hkm * 0.5 * (hk + hkm));
+    const double shp = SP[IX(k, m)], byp = BP[IX(k, m)];
+    const double sq = sqrt(q2pk);
+    // TKE row
+    double a2 = r3 - a1 - a3 + 2. * (sq / (B_1 * xlk));
+    double ds = 4. * q2pk / twodt1 - D.q2pp[IX(k, m)] / twodt1;
+    if (byp >= 0.0) {
+      ds += 2. * (shp + byp);
+    } else {
+      ds += 2. * shp;
+      a2 -= byp * 2. / q2pk;
+    }
+    // length-scale row (wall function of M&Y)
+    zfromt += hkm;
+    const double zfromb = hbelow;  // sum of hp(k..kms)
+    hbelow -= hk;
+    const double r1 = xlk / (KAPPAS * zfromb), r2 = xlk / (KAPPAS * zfromt);
+    double a2l = r3 - a1 - a3 + sq / (B_1 * xlk) * (1. + E_2 * r1 * r1 + E_3 * r2 * r2);
+    double dsl;
+    if (D.P.iturb == 1) {
+      if (byp >= 0) {
+        dsl = xlk * (E_1 * shp + E_1 * byp);
+      } else {
+        dsl = xlk * (E_1 * shp);
+        a2l -= E_1 * byp / q2pk;
+      }
+    } else {
+      if (byp >= 0) {
+        dsl = xlk * (E_1 * shp + 2.00 * byp);
+      } else {
+        dsl = xlk * (E_1 * shp);
+        a2l -= 5.06 * byp / q2pk;
+      }
+    }
+    dsl += 4 * D.q2lp[IX(k, m)] / twodt1 - D.q2lpp[IX(k, m)] / twodt1;
+    // forward sweeps
+    double den = a2 + a1 * e1;
+    e1 = -a3 / den;
+    f1 = (ds - a1 * f1) / den;
+    den = a2l + a1 * e2;
+    e2 = -a3 / den;
+    f2 = (dsl - a1 * f2) / den;
+    XL[IX(k, m)] = e1;  // xl(k) no longer needed
+    SP[IX(k, m)] = e2;
+    D.q2[IX(k, m)] = f1;
+    D.q2l[IX(k, m)] = f2;
+    dco_prev = dco;
+  }
+  // bottom Dirichlet row kms+1 (a1 = 0)
+  double x1 = ds_bot / r3, x2 = 0.0;
+  D.q2[IX(kms + 1, m)] = dmax(Q2_MIN, x1);
+  D.q2l[IX(kms + 1, m)] = dmax(Q2L_MIN, x2);
+  for (int k = kms; k >= k1s; --k) {
+    x1 = XL[IX(k, m)] * x1 + D.q2[IX(k, m)];
+    x2 = SP[IX(k, m)] * x2 + D.q2l[IX(k, m)];
+    D.q2[IX(k, m)] = dmax(Q2_MIN, x1);
+    D.q2l[IX(k, m)] = dmax(Q2L_MIN, x2);
+  }
+  for (int k = D.k1; k <= k1s - 1; ++k) {
+    D.q2[IX(k, m)] = D.q2[IX(k1s, m)];
+    D.q2l[IX(k, m)] = D.q2l[IX(k1s, m)];
+  }
+  // eddy coefficients from the p-level turbulence (245-290)
+  for (int k = k1s + 1; k <= kms; ++k) {
+    const double hkm = D.hp[IX(k - 1, m)], hk = D.hp[IX(k, m)];
+    const double dz = 0.5 * (hkm + hk);
+    const double rkm = D.rhop[IX(k - 1, m)], rk = D.rhop[IX(k, m)];
+    const double N2 = -G * ((rkm - rk) / dz) / (0.5 * (rkm + rk) + 1000.);
+    const double q2pk = D.q2p[IX(k, m)];
+    double xl = D.q2lp[IX(k, m)] / q2pk;
+    if (N2 > 0.0) xl = dmin(xl, 0.53 * sqrt(q2pk / N2));
+    double Gh = -N2 * xl * xl / q2pk;
+    Gh = dmax(GH_MIN, dmin(GH_MAX, Gh));
+    const double Sh = D.P.t_1 / (1. - Gh * D.P.t_2);
+    const double Sm = (D.P.t_3 + Sh * Gh * D.P.t_4) / (1. - D.P.t_5 * Gh);
+    const double Rt = sqrt(q2pk) * xl;
+    D.Av[IX(k, m)] = (R)(Rt * Sm + AV_MOL);
+    D.Dv[IX(k, m)] = (R)(Rt * Sh + DV_MOL);
+  }
+  if ((double)D.hp[IX(k1s, m)] < 1.E-2) {
+    D.Av[IX(k1s + 1, m)] = (R)dmax(D.Av[IX(k1s + 1, m)], 1.E-4);
+    D.Dv[IX(k1s + 1, m)] = (R)dmax(D.Dv[IX(k1s + 1, m)], 1.E-4);
+  }
+  for (int k = 1; k <= k1s; ++k) D.Av[IX(k, m)] = D.Dv[IX(k, m)] = R(0);
+  for (int k = kms + 1; k <= D.km1; ++k) D.Av[IX(k, m)] = D.Dv[IX(k, m)] = R(0);
+}
+
+// ---------------------------------------------------------------------------
+// Time-level shuffles.
+
+// 2-D zeta levels: settrap (spp := sp, sp := (s+spp)/2, 3447-3450), settrap2
+// (sp := (s+spp)/2, 3951-3962), smooth's Asselin filter (3366-3372).
+template <typename R, int MODE>
+__global__ void __launch_bounds__(kBlock) k_zeta_levels(Dev<R> D) {
+  const int c = col();
+  if (c > D.lm) return;
+  if (MODE == 0) {  // settrap
+    D.spp[c] = D.sp[c];
+    if (c < D.lm) D.sp[c] = 0.5 * (D.s[c] + D.spp[c]);
+  } else if (MODE == 1) {  // settrap2
+    D.sp[c] = 0.5 * (D.s[c] + D.spp[c]);
+  } else {  // smooth
+    if (c < D.lm) D.sp[c] = D.sp[c] + D.P.beta2 * (D.s[c] - 2. * D.sp[c] + D.spp[c]);
+  }
+}
+
+// layer_hp2 + TopLayerIndexp2 for one column (si3d_procedures.f90:2238-2293, 2368-2427).
+template <typename R>
+__device__ void layer_hp2_col(const Dev<R> &D, int c) {
+  const int S = D.lm;
+  const double spc = D.sp[c];
+  int k1z = D.km1, k1u = D.km1, k1v = D.km1;
+  for (int k = D.k1; k <= D.kmz[c]; ++k) {
+    double v = dmin(D.zlevel[k + 1], D.hhs[c]) - dmax(D.zlevel[k], -spc);
+    R hv = (v <= 0.0) ? R(0) : (R)v;
+    D.hp[IX(k, c)] = hv;
+    if (hv > R(0) && k1z == D.km1) k1z = k;
+  }
+  const int lE = D.lE[c], lN = D.lN[c];
+  if (lE != S) {
+    const double sm = dmax(spc, D.sp[lE]);
+    for (int k = D.k1; k <= min(D.kmz[c], D.kmz[lE]); ++k) {
+      double v = dmin(D.zlevel[k + 1], D.hhu[c]) - dmax(D.zlevel[k], -sm);
+      R hv = (v <= 0.0) ? R(0) : (R)v;
+      D.hup[IX(k, c)] = hv;
+      if (hv > R(0) && k1u == D.km1) k1u = k;
+    }
+  }
+  if (lN != S) {
+    const double sm = dmax(spc, D.sp[lN]);
+    for (int k = D.k1; k <= min(D.kmz[c], D.kmz[lN]); ++k) {
+      double v = dmin(D.zlevel[k + 1], D.hhv[c]) - dmax(D.zlevel[k], -sm);
+      R hv = (v <= 0.0) ? R(0) : (R)v;
+      D.hvp[IX(k, c)] = hv;
+      if (hv > R(0) && k1v == D.km1) k1v = k;
+    }
+  }
+  D.k1z[c] = k1z;
+  D.k1u[c] = k1u;
+  D.k1v[c] = k1v;
+}
+
+// TopLayerIndexp2 alone (used by save after hp := h).
+template <typename R>
+__device__ void toplayer_col(const Dev<R> &D, int c) {
+  const int S = D.lm;
+  int k1z = D.km1, k1u = D.km1, k1v = D.km1;
+  for (int k = D.k1; k <= D.kmz[c]; ++k)
+    if ((double)D.hp[IX(k, c)] > 0.0) {
+      k1z = k;
+      break;
+    }
+  if (D.lE[c] != S)
+    for (int k = D.k1; k <= min(D.kmz[c], D.kmz[D.lE[c]]); ++k)
+      if ((double)D.hup[IX(k, c)] > 0.0) {
+        k1u = k;
+        break;
+      }
+  if (D.lN[c] != S)
+    for (int k = D.k1; k <= min(D.kmz[c], D.kmz[D.lN[c]]); ++k)
+      if ((double)D.hvp[IX(k, c)] > 0.0) {
+        k1v = k;
+        break;
+      }
+  D.k1z[c] = k1z;
+  D.k1u[c] = k1u;
+  D.k1v[c] = k1v;
+}
+
+// u/v fluxes and velocities at n+1/2 with the top-cell corrections, shared by
+// settrap (3488-3543) and settrap2 (3984-4035).  UPD_PP: settrap also moves
+// the p level into pp first.
+template <typename R, bool X, bool UPD_PP>
+__device__ void half_level_flux(const Dev<R> &D, int c) {
+  const int ln = X ? D.lE[c] : D.lN[c];
+  if (ln == D.lm) return;
+  R *qh = X ? D.uh : D.vh;
+  R *qhp = X ? D.uhp : D.vhp;
+  R *qhpp = X ? D.uhpp : D.vhpp;
+  R *qp = X ? D.up : D.vp;
+  R *qpp = X ? D.upp : D.vpp;
+  const R *hq = X ? D.hup : D.hvp;
+  const R *hqn = X ? D.hu : D.hv;
+  const R *hqpp = X ? D.hupp : D.hvpp;
+  const int kmq = min(D.kmz[c], D.kmz[ln]);
+  for (int k = D.k1; k <= kmq; ++k) {
+    if (UPD_PP) {
+      qhpp[IX(k, c)] = qhp[IX(k, c)];
+      qpp[IX(k, c)] = qp[IX(k, c)];
+    }
+    const double hk = hq[IX(k, c)];
+    if (hk > 0.0) {
+      double f = 0.5 * ((double)qh[IX(k, c)] + (double)qhpp[IX(k, c)]);
+      qhp[IX(k, c)] = (R)f;
+      qp[IX(k, c)] = (R)(f / hk);
+    } else {
+      qhp[IX(k, c)] = R(0);
+      qp[IX(k, c)] = R(0);
+    }
+  }
+  const int k = X ? D.k1u[c] : D.k1v[c];
+  if ((double)hqn[IX(k - 1, c)] > 0.0) {
+    double f = (double)qhp[IX(k, c)] + (double)qh[IX(k - 1, c)] / 2.;
+    qhp[IX(k, c)] = (R)f;
+    qp[IX(k, c)] = (R)(f / (double)hq[IX(k, c)]);
+  }
+  if ((double)hqpp[IX(k - 1, c)] > 0.0) {
+    double f = (double)qhp[IX(k, c)] + (double)qhpp[IX(k - 1, c)] / 2.;
+    qhp[IX(k, c)] = (R)f;
+    qp[IX(k, c)] = (R)(f / (double)hq[IX(k, c)]);
+  }
+}
+
+// settrap (MODE 0) / settrap2 (MODE 1) column part: layer thicknesses at n+1/2,
+// top indices, scalar and density at n+1/2, fluxes at n+1/2, turbulence levels.
+template <typename R, int MODE>
+__global__ void __launch_bounds__(kBlock) k_settrap_col(Dev<R> D) {
+  const int c = col();
+  if (c > D.lm) return;
+  if (MODE == 0) {  // hpp := hp (whole column, sentinel included)
+    for (int k = 0; k <= D.km1 + 1; ++k) {
+      D.hpp[IX(k, c)] = D.hp[IX(k, c)];
+      D.hupp[IX(k, c)] = D.hup[IX(k, c)];
+      D.hvpp[IX(k, c)] = D.hvp[IX(k, c)];
+    }
+  }
+  if (c == D.lm) return;
+  layer_hp2_col(D, c);
+  for (int k = D.k1; k <= D.kmz[c]; ++k) {
+    if (MODE == 0) D.salpp[IX(k, c)] = D.salp[IX(k, c)];
+    double t = 0.5 * ((double)D.sal[IX(k, c)] + (double)D.salpp[IX(k, c)]);
+    D.salp[IX(k, c)] = (R)t;
+    D.rhop[IX(k, c)] = (R)(densty((double)(R)t, 0.00004, zdens(D, k, c)) - 1000.);
+  }
+  half_level_flux<R, true, MODE == 0>(D, c);
+  half_level_flux<R, false, MODE == 0>(D, c);
+  if (D.P.iturb > 0) {
+    for (int k = D.k1; k <= D.kmz[c]; ++k) {
+      if (MODE == 0) {
+        D.q2pp[IX(k, c)] = D.q2p[IX(k, c)];
+        D.q2lpp[IX(k, c)] = D.q2lp[IX(k, c)];
+      }
+      D.q2p[IX(k, c)] = 0.5 * (D.q2[IX(k, c)] + D.q2pp[IX(k, c)]);
+      D.q2lp[IX(k, c)] = 0.5 * (D.q2l[IX(k, c)] + D.q2lpp[IX(k, c)]);
+    }
+    const int k = D.k1z[c];
+    if ((double)D.hpp[IX(k, c)] <= 0.0) {
+      D.q2lp[IX(k + 1, c)] = D.q2lp[IX(k + 2, c)];
+      D.q2p[IX(k + 1, c)] = D.q2p[IX(k + 2, c)];
+    }
+  }
+}
+
+// save(istep) column part, si3d_procedures.f90:3621-3941 (+ save_2EqTvars).
+template <typename R, int ISTEP>
+__global__ void __launch_bounds__(kBlock) k_save_col(Dev<R> D) {
+  const int c = col();
+  if (c > D.lm) return;
+  if (ISTEP == 1) D.spp[c] = (c == D.lm) ? D.spp[c] : D.sp[c];
+  D.sp[c] = D.s[c];
+  for (int k = 0; k <= D.km1 + 1; ++k) {
+    if (ISTEP == 1) {
+      D.hpp[IX(k, c)] = D.hp[IX(k, c)];
+      D.hupp[IX(k, c)] = D.hup[IX(k, c)];
+      D.hvpp[IX(k, c)] = D.hvp[IX(k, c)];
+    }
+    D.hp[IX(k, c)] = D.h[IX(k, c)];
+    D.hup[IX(k, c)] = D.hu[IX(k, c)];
+    D.hvp[IX(k, c)] = D.hv[IX(k, c)];
+  }
+  if (D.P.iturb > 0) {
+    for (int k = 0; k <= D.km1 + 1; ++k) {
+      if (ISTEP == 1) {
+        D.q2pp[IX(k, c)] = D.q2p[IX(k, c)];
+        D.q2lpp[IX(k, c)] = D.q2lp[IX(k, c)];
+      }
+      D.q2p[IX(k, c)] = D.q2[IX(k, c)];
+      D.q2lp[IX(k, c)] = D.q2l[IX(k, c)];
+    }
+  }
+  if (c == D.lm) return;
+  toplayer_col(D, c);
+  for (int k = D.k1; k <= D.km; ++k) {
+    if (ISTEP == 1) {
+      D.salpp[IX(k, c)] = D.salp[IX(k, c)];
+      D.uhpp[IX(k, c)] = D.uhp[IX(k, c)];
+      D.vhpp[IX(k, c)] = D.vhp[IX(k, c)];
+      D.upp[IX(k, c)] = D.up[IX(k, c)];
+      D.vpp[IX(k, c)] = D.vp[IX(k, c)];
+    }
+    const R t = D.sal[IX(k, c)];
+    D.salp[IX(k, c)] = t;
+    D.rhop[IX(k, c)] = (R)(densty((double)t, 0.00004, zdens(D, k, c)) - 1000.);
+    const double hu = D.hup[IX(k, c)];
+    if (hu > 0.0) {
+      const R f = D.uh[IX(k, c)];
+      D.uhp[IX(k, c)] = f;
+      D.up[IX(k, c)] = D.u[IX(k, c)] = (R)((double)f / hu);
+    } else {
+      D.uhp[IX(k, c)] = D.up[IX(k, c)] = D.u[IX(k, c)] = R(0);
+    }
+    const double hv = D.hvp[IX(k, c)];
+    if (hv > 0.0) {
+      const R f = D.vh[IX(k, c)];
+      D.vhp[IX(k, c)] = f;
+      D.vp[IX(k, c)] = D.v[IX(k, c)] = (R)((double)f / hv);
+    } else {
+      D.vhp[IX(k, c)] = D.vp[IX(k, c)] = D.v[IX(k, c)] = R(0);
+    }
+  }
+  if (D.P.iturb > 0) {
+    const int k1s = D.k1z[c];
+    const bool guard = (ISTEP == 1) ? true : (D.km1 - k1s > 1);
+    if (guard && (double)D.hpp[IX(k1s, c)] <= 0.0) {
+      D.q2lp[IX(k1s + 1, c)] = D.q2lp[IX(k1s + 2, c)];
+      D.q2p[IX(k1s + 1, c)] = D.q2p[IX(k1s + 2, c)];
+    }
+  }
+}
+
+// smooth (Asselin filter on the leapfrog step), si3d_procedures.f90:3356-3415.
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_smooth_col(Dev<R> D) {
+  const int c = col();
+  if (c >= D.lm) return;
+  const int S = D.lm;
+  const double spc = D.sp[c];
+  for (int k = D.k1; k <= D.kmz[c]; ++k) {
+    double v = dmin(D.zlevel[k + 1], D.hhs[c]) - dmax(D.zlevel[k], -spc);
+    D.hp[IX(k, c)] = (v <= 0.0) ? R(0) : (R)v;
+  }
+  for (int dir = 0; dir < 2; ++dir) {
+    const bool X = (dir == 0);
+    const int ln = X ? D.lE[c] : D.lN[c];
+    if (ln == S) continue;
+    R *hq = X ? D.hup : D.hvp;
+    const double *hh = X ? D.hhu : D.hhv;
+    R *qh = X ? D.uh : D.vh;
+    R *qhp = X ? D.uhp : D.vhp;
+    R *qhpp = X ? D.uhpp : D.vhpp;
+    R *qp = X ? D.up : D.vp;
+    const double sm = dmax(spc, D.sp[ln]);
+    for (int k = D.k1; k <= min(D.kmz[c], D.kmz[ln]); ++k) {
+      double v = dmin(D.zlevel[k + 1], hh[c]) - dmax(D.zlevel[k], -sm);
+      hq[IX(k, c)] = (v <= 0.0) ? R(0) : (R)v;
+    }
+    for (int k = D.k1; k <= min(D.kmz[c], D.kmz[ln]); ++k) {
+      const double hk = hq[IX(k, c)];
+      if (hk <= 0.0) continue;
+      double f = (double)qhp[IX(k, c)] + D.P.beta2 * ((double)qh[IX(k, c)] - 2. * (double)qhp[IX(k, c)] + (double)qhpp[IX(k, c)]);
+      qhp[IX(k, c)] = (R)f;
+      qp[IX(k, c)] = (R)(f / hk);
+    }
+  }
+}
+
+// Density of the initial state at every column, sentinel included
+// (InitializeScalarFields, si3d_procedures.f90:368-377).
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_init_density(Dev<R> D) {
+  const int c = col();
+  if (c > D.lm) return;
+  for (int k = D.k1; k <= D.km1; ++k)
+    D.rhop[IX(k, c)] = (R)(densty((double)D.salp[IX(k, c)], 0.00004, zdens(D, k, c)) - 1000.);
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launch sequences.
+
+template <typename R> Dev<R> &dev(s3g_ctx *c);
+template <> Dev<double> &dev<double>(s3g_ctx *c) { return c->d64; }
+template <> Dev<float> &dev<float>(s3g_ctx *c) { return c->d32; }
+
+static inline dim3 grid_for(int n) { return dim3((n + kBlock - 1) / kBlock); }
+
+#define LAUNCH(name, kern, n, ...)                                          \
+  do {                                                                      \
+    int _slot = -1;                                                         \
+    if (c->timing) timer_begin(c, name, &_slot);                            \
+    hipLaunchKernelGGL(kern, grid_for(n), dim3(kBlock), 0, c->stream, __VA_ARGS__); \
+    if (c->timing) timer_end(c, _slot);                                     \
+  } while (0)
+
+template <typename R>
+static void sync_forcing(s3g_ctx *c) {
+  Dev<R> &D = dev<R>(c);
+  D.uair = c->uair;
+  D.vair = c->vair;
+  D.cdw = c->cdw;
+}
+
+template <typename R>
+int phase(s3g_ctx *c, int ph, int n, int istep) {
+  const double tz = 1.0 / istep, twodt1 = c->P.twodt * tz;
+  if (ph == 0) {
+    surfbc(c, n, istep);
+    return 0;
+  }
+  sync_forcing<R>(c);
+  Dev<R> &D = dev<R>(c);
+  const int L = c->lm;
+  switch (ph) {
+    case 1:
+      if (c->P.ihd > 1) LAUNCH("hmix", (k_hmix<R>), L, D);
+      break;
+    case 2: LAUNCH("exmom_x", (k_exmom<R, true>), L, D, twodt1); break;
+    case 3: LAUNCH("matmom_x", (k_matmom<R, true>), L, D, tz); break;
+    case 4: LAUNCH("exmom_y", (k_exmom<R, false>), L, D, twodt1); break;
+    case 5: LAUNCH("matmom_y", (k_matmom<R, false>), L, D, tz); break;
+    case 6: LAUNCH("matcon", (k_matcon<R>), L, D, tz); break;
+    case 7: {
+      int rc = solve_free_surface<R>(c, tz);
+      LAUNCH("slayer_h", (k_slayer_h<R>), L + 1, D);
+      return rc;
+    }
+    case 8:
+      LAUNCH("vel", (k_vel<R>), L, D, tz);
+      LAUNCH("continuity1", (k_continuity<R, 1>), L, D);
+      break;
+    case 9: LAUNCH("exsal_hflux", (k_exsal_hflux<R>), L, D, twodt1);
+            LAUNCH("exsal_div", (k_exsal_imsal<R, true, false>), L, D, twodt1); break;
+    case 10: LAUNCH("imsal", (k_exsal_imsal<R, false, true>), L, D, twodt1); break;
+    case 11: LAUNCH("vel2", (k_vel2<R>), L, D); break;
+    case 12:
+      LAUNCH("smooth_zeta", (k_zeta_levels<R, 2>), L + 1, D);
+      LAUNCH("smooth_col", (k_smooth_col<R>), L, D);
+      break;
+    case 13:
+      if (c->P.iturb >= 1) LAUNCH("turbulence", (k_turb<R>), L, D, istep);
+      break;
+    default: return S3G_ERR_ARG;
+  }
+  return hip_check(hipGetLastError(), "phase launch");
+}
+
+// fd, si3d_procedures.f90:382-575.
+template <typename R>
+int fd(s3g_ctx *c, int n, int istep, int lastiter) {
+  (void)lastiter;
+  const double tz = 1.0 / istep, twodt1 = c->P.twodt * tz;
+  surfbc(c, n, istep);
+  sync_forcing<R>(c);
+  Dev<R> &D = dev<R>(c);
+  const int L = c->lm;
+  if (c->P.ihd > 1) LAUNCH("hmix", (k_hmix<R>), L, D);
+  LAUNCH("exmom_x", (k_exmom<R, true>), L, D, twodt1);
+  LAUNCH("matmom_x", (k_matmom<R, true>), L, D, tz);
+  LAUNCH("exmom_y", (k_exmom<R, false>), L, D, twodt1);
+  LAUNCH("matmom_y", (k_matmom<R, false>), L, D, tz);
+  LAUNCH("matcon", (k_matcon<R>), L, D, tz);
+  int rc = solve_free_surface<R>(c, tz);
+  if (rc < 0) return rc;
+  LAUNCH("slayer_h", (k_slayer_h<R>), L + 1, D);
+  LAUNCH("vel", (k_vel<R>), L, D, tz);
+  LAUNCH("continuity1", (k_continuity<R, 1>), L, D);
+  if (c->P.isal != 0) {
+    LAUNCH("exsal_hflux", (k_exsal_hflux<R>), L, D, twodt1);
+    LAUNCH("exsal_imsal", (k_exsal_imsal<R, true, true>), L, D, twodt1);
+  }
+  LAUNCH("vel2", (k_vel2<R>), L, D);
+  if (c->P.ismooth >= 1 && istep == 1) {
+    LAUNCH("smooth_zeta", (k_zeta_levels<R, 2>), L + 1, D);
+    LAUNCH("smooth_col", (k_smooth_col<R>), L, D);
+  }
+  if (c->P.iturb >= 1) LAUNCH("turbulence", (k_turb<R>), L, D, istep);
+  int e = hip_check(hipGetLastError(), "fd launch");
+  return e < 0 ? e : rc;
+}
+
+template <typename R>
+int settrap(s3g_ctx *c) {
+  Dev<R> &D = dev<R>(c);
+  const int L = c->lm;
+  LAUNCH("settrap_zeta", (k_zeta_levels<R, 0>), L + 1, D);
+  LAUNCH("settrap_col", (k_settrap_col<R, 0>), L + 1, D);
+  LAUNCH("continuity2", (k_continuity<R, 2>), L, D);
+  return hip_check(hipGetLastError(), "settrap launch");
+}
+
+template <typename R>
+int settrap2(s3g_ctx *c) {
+  Dev<R> &D = dev<R>(c);
+  const int L = c->lm;
+  LAUNCH("settrap2_zeta", (k_zeta_levels<R, 1>), L + 1, D);
+  LAUNCH("settrap2_col", (k_settrap_col<R, 1>), L, D);
+  LAUNCH("continuity2", (k_continuity<R, 2>), L, D);
+  return hip_check(hipGetLastError(), "settrap2 launch");
+}
+
+template <typename R>
+int save(s3g_ctx *c, int istep) {
+  Dev<R> &D = dev<R>(c);
+  const int L = c->lm;
+  if (istep == 1)
+    LAUNCH("save_col", (k_save_col<R, 1>), L + 1, D);
+  else
+    LAUNCH("save_col", (k_save_col<R, 2>), L + 1, D);
+  LAUNCH("continuity2", (k_continuity<R, 2>), L, D);
+  return hip_check(hipGetLastError(), "save launch");
+}
+
+int init_density(s3g_ctx *c) {
+  if (c->precision == S3G_FP64)
+    hipLaunchKernelGGL(k_init_density<double>, grid_for(c->lm + 1), dim3(kBlock), 0, c->stream, c->d64);
+  else
+    hipLaunchKernelGGL(k_init_density<float>, grid_for(c->lm + 1), dim3(kBlock), 0, c->stream, c->d32);
+  return hip_check(hipStreamSynchronize(c->stream), "init_density");
+}
+
+template int fd<double>(s3g_ctx *, int, int, int);
+template int fd<float>(s3g_ctx *, int, int, int);
+template int phase<double>(s3g_ctx *, int, int, int);
+template int phase<float>(s3g_ctx *, int, int, int);
+template int settrap<double>(s3g_ctx *);
+template int settrap<float>(s3g_ctx *);
+template int settrap2<double>(s3g_ctx *);
+template int settrap2<float>(s3g_ctx *);
+template int save<double>(s3g_ctx *, int);
+template int save<float>(s3g_ctx *, int);
+
+}  // namespace s3g

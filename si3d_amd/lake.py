@@ -1,0 +1,174 @@
+"""Input decks for pSi3D lakes: the content of si3d_inp.txt, si3d_bathy,
+si3d_layer.txt, si3d_init.txt and si3d_surfbc_W.txt, held in memory.
+
+A deck is a plain dict.  Its keys follow the names the reference reads in
+`input` (si3d_utils.f90:21-315), `ZGridDimensions` (318-382), `bathy`
+(595-892), `InitializeScalarFields` (si3d_procedures.f90:292-363) and
+`surfbc0` case 20 (si3d_boundaryconditions.f90:4450-4484).  Depths are
+integer decimetres as in si3d_bathy.
+
+The generators build the synthetic lakes used for measurement:
+
+* rectangle  -- flat rectangular basin with a constant wind (uniform layers,
+  constant eddy viscosity); the 100x50x20 case is the CPU-runnable workload.
+* ellipse    -- elliptic basin, depth 100 m * (1 - r^2) plus a seeded Gaussian
+  random field, geometrically stretched layers, tanh thermocline, a
+  sinusoidal wind series, TVD transport and the two-equation turbulence
+  closure on.  1024x512 cells with 60 layers is the single-GPU workload.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+G_RHOAIR = 1.3  # si3d_types.f90:33
+
+_DEFAULTS = dict(
+    ibathyf=0, zlevel=None, dzmin=0.1, datadj=0.0, zetainit=0.0,
+    itrap=1, niter=2, ismooth=0, beta=0.05, iturb=0, Av0=3e-2, Dv0=3e-2,
+    iadv=1, itrmom=1, ihd=1, Ax0=1.0, Ay0=1.0, f=0.0, theta=1.0, ibc=0,
+    isal=0, cd=2.5e-3, ifsurfbc=0, dtsurfbc=600.0, cw=1.2e-3, wa=0.0,
+    phi=0.0, surfbc=None, pcg_tol=1e-10, pcg_maxit=4000,
+)
+
+
+def _deck(**kw):
+    d = dict(_DEFAULTS)
+    d.update(kw)
+    return d
+
+
+def km1_of(deck) -> int:
+    """Number of layer records km1 (ZGridDimensions, si3d_utils.f90:318-382)."""
+    if deck["ibathyf"] < 0:
+        return len(deck["zlevel"])
+    zl = np.float32(deck["zl"])
+    ddz = np.float32(deck["ddz"])
+    return int(math.ceil(float(zl / ddz))) + 2
+
+
+def rectangle(nx=100, ny=50, dx=50.0, depth=20.0, dz=1.0, tau=0.1, dt=30.0,
+              nts=1000, iadv=1, ihd=1, Ax0=1.0, cd=2.5e-3, Av0=3e-2,
+              temperature=20.0, phi=270.0):
+    """Flat rectangular basin, homogeneous density, uniform wind stress `tau`
+    (Pa) blowing toward +x (phi = 270 deg), closed boundaries."""
+    cw = 1.2e-3
+    wa = math.sqrt(tau / (cw * G_RHOAIR))
+    d = _deck(
+        name=f"rectangle_{nx}x{ny}x{int(round(depth / dz))}",
+        imm=nx, jmm=ny, dx=dx, dy=dx, ddz=dz, zl=depth, dt=dt, nts=nts,
+        bathy_dm=np.full((nx, ny), int(round(depth * 10)), dtype=np.int32),
+        iadv=iadv, ihd=ihd, Ax0=Ax0, Ay0=Ax0, cd=cd, Av0=Av0, Dv0=Av0,
+        iturb=0, isal=0, ibc=0, ifsurfbc=0, cw=cw, wa=wa, phi=phi,
+    )
+    d["init_profile"] = np.full(km1_of(d), temperature)
+    return d
+
+
+def stretched_levels(nlayers=60, top=0.5, bottom=5.0):
+    """zlevel records of si3d_layer.txt: two -100 sentinels, then the depths of
+    the layer interfaces for `nlayers` layers growing geometrically from `top`
+    to `bottom` metres (si3d_utils.f90:330-358)."""
+    r = (bottom / top) ** (1.0 / (nlayers - 1))
+    dz = top * r ** np.arange(nlayers)
+    z = np.concatenate([[-100.0, -100.0], np.cumsum(dz)])
+    # zlevel(k) is the depth of the top of layer k; the last record is the
+    # bottom of the deepest layer.
+    return np.round(z, 4)
+
+
+def gaussian_random_field(nx, ny, dx, corr_len, sigma, seed):
+    """Seeded Gaussian random field with Gaussian covariance of length
+    `corr_len` and standard deviation `sigma`, by FFT filtering of white noise."""
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal((nx, ny))
+    kx = np.fft.fftfreq(nx, d=dx) * 2 * np.pi
+    ky = np.fft.fftfreq(ny, d=dx) * 2 * np.pi
+    k2 = kx[:, None] ** 2 + ky[None, :] ** 2
+    f = np.real(np.fft.ifft2(np.fft.fft2(w) * np.exp(-k2 * corr_len ** 2 / 4.0)))
+    f = f - f.mean()
+    sd = f.std()
+    return f * (sigma / sd) if sd > 0 else f
+
+
+def ellipse_bathymetry(nx, ny, dx, depth=100.0, sigma=5.0, corr_len=2000.0,
+                       seed=0, min_depth=1.0, max_depth=None):
+    """Depths (dm, int32) of an elliptic basin spanning the box with a one-cell
+    margin: depth*(1-r^2) + GRF; cells shallower than `min_depth` are dry."""
+    i = np.arange(nx) + 0.5
+    j = np.arange(ny) + 0.5
+    a = nx / 2.0 - 1.0
+    b = ny / 2.0 - 1.0
+    X = (i[:, None] - nx / 2.0) / a
+    Y = (j[None, :] - ny / 2.0) / b
+    r2 = X ** 2 + Y ** 2
+    h = depth * (1.0 - r2) + gaussian_random_field(nx, ny, dx, corr_len, sigma, seed)
+    if max_depth is not None:
+        h = np.minimum(h, max_depth)
+    h = np.where((r2 < 1.0) & (h >= min_depth), h, -9.9)
+    return np.round(h * 10.0).astype(np.int32)
+
+
+def thermocline_profile(zlevel, t_top=24.0, t_bot=8.0, z_tc=15.0, width=2.0):
+    """Temperature per layer k=1..km1 for si3d_init.txt: t_bot + (t_top-t_bot)
+    * (1 + tanh((z_tc - z)/width))/2 at the layer centre."""
+    z = np.asarray(zlevel, dtype=np.float64)
+    km1 = len(z)
+    prof = np.empty(km1)
+    for k in range(km1):
+        top = 0.0 if z[k] < -99 else z[k]
+        bot = z[k + 1] if k + 1 < km1 else z[k] + 0.5
+        zc = 0.5 * (top + bot)
+        prof[k] = t_bot + (t_top - t_bot) * 0.5 * (1.0 + math.tanh((z_tc - zc) / width))
+    return prof
+
+
+def wind_series(tl, dtsbc=600.0, amp=0.2, period=86400.0, direction_deg=30.0,
+                cdw=1.2e-3, phase=0.0):
+    """si3d_surfbc_W.txt records (cdw, uair, vair) for a stress
+    amp*sin(2 pi t/period + phase) Pa blowing from `direction_deg`."""
+    n = int(tl / dtsbc) + 4
+    t = np.arange(n) * dtsbc
+    tau = amp * np.sin(2 * np.pi * t / period + phase)
+    U = np.sqrt(np.abs(tau) / (cdw * G_RHOAIR))
+    th = math.radians(direction_deg)
+    ua = np.sign(tau) * U * (-math.sin(th))
+    va = np.sign(tau) * U * (-math.cos(th))
+    return np.stack([np.full(n, cdw), ua, va], axis=1)
+
+
+def ellipse(nx=1024, ny=512, dx=100.0, nlayers=60, dt=60.0, nts=2000, seed=1,
+            wind_amp=0.2, wind_dir=30.0, f=1e-4, Ax0=1.0, depth=100.0,
+            top=0.5, bottom=5.0, iturb=1):
+    """Synthetic stratified elliptic lake with TVD transport and turbulence."""
+    z = stretched_levels(nlayers, top, bottom)
+    bathy = ellipse_bathymetry(nx, ny, dx, depth=depth, seed=seed,
+                               max_depth=float(z[-1]) - 0.5)
+    d = _deck(
+        name=f"ellipse_{nx}x{ny}x{nlayers}",
+        imm=nx, jmm=ny, dx=dx, dy=dx, ddz=1.0, zl=float(z[-1]), dt=dt, nts=nts,
+        ibathyf=-1, zlevel=z, bathy_dm=bathy,
+        iadv=1, itrmom=1, ihd=1, Ax0=Ax0, Ay0=Ax0, f=f, ibc=1, isal=1,
+        iturb=iturb, ifsurfbc=20, dtsurfbc=600.0,
+        surfbc=wind_series(nts * dt, 600.0, wind_amp, 86400.0, wind_dir),
+    )
+    d["init_profile"] = thermocline_profile(z)
+    return d
+
+
+def n_wet_cells(deck) -> int:
+    """N_wet = sum over wet columns of (kmz - k1 + 1) at t=0 for a level
+    surface (the per-column layer count of bathy, si3d_utils.f90:812-867)."""
+    h = deck["bathy_dm"].astype(np.float32) * np.float32(0.1) + np.float32(deck["datadj"])
+    wet = h > 0
+    if deck["ibathyf"] >= 0:
+        ddz = np.float32(deck["ddz"])
+        kz = np.floor(h / ddz).astype(np.int64)
+        kz = kz + ((h - kz.astype(np.float32) * ddz) > np.float32(deck["dzmin"]))
+        return int(kz[wet].sum())
+    z = np.asarray(deck["zlevel"], dtype=np.float32)
+    # layer count = index of the first level whose bottom reaches the depth
+    bottoms = z[2:]
+    kz = np.searchsorted(bottoms, h[wet], side="left") + 1
+    return int(kz.sum())

@@ -1,0 +1,180 @@
+"""Host-side driver of one lake on the GPU, mirroring the reference's call
+sequence: the si3d.f90 time loop (324-393) calling fd / settrap / settrap2 /
+save, with field access for outputs.
+
+Two ways to create a lake:
+
+* `GpuLake(deck)` -- standalone: the library restates input/bathy/init from
+  an in-memory deck (see si3d_amd.lake).
+* `GpuLake.from_state(grid, params, fields)` -- drop-in: the caller (the
+  Fortran driver, or a test holding the reference-layout arrays) hands over
+  the grid and every state array, exactly as si3d.f90 would after `init`.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import S3gDeck, S3gGrid, S3gParams, S3gStats, check
+
+_PREC = {"fp64": _lib.S3G_FP64, "mixed": _lib.S3G_MIXED}
+_PRECOND = {"jacobi": _lib.S3G_PREC_JACOBI, "line": _lib.S3G_PREC_LINE}
+INT_FIELDS = ("kmz", "k1z", "k1u", "k1v")
+TWO_D = ("s", "sp", "spp", "sx", "sy", "dd", "qq", "rr", "eagx", "earx", "eagy", "eary",
+         "hhs", "hhu", "hhv") + INT_FIELDS
+
+
+def make_params(deck, precision="fp64", preconditioner="jacobi", tol=None, maxit=None):
+    p = S3gParams()
+    for name, _ in S3gParams._fields_:
+        if name in ("surfbc", "n_surfbc", "precision", "preconditioner", "pcg_tol", "pcg_maxit"):
+            continue
+        if name in deck:
+            setattr(p, name, deck[name])
+    keep = []
+    if deck.get("surfbc") is not None:
+        sb = np.ascontiguousarray(deck["surfbc"], dtype=np.float64)
+        keep.append(sb)
+        p.surfbc = sb.ctypes.data_as(_lib.dp)
+        p.n_surfbc = sb.shape[0]
+    p.precision = _PREC[precision]
+    p.preconditioner = _PRECOND[preconditioner]
+    p.pcg_tol = tol if tol is not None else deck.get("pcg_tol", 1e-10)
+    p.pcg_maxit = maxit if maxit is not None else deck.get("pcg_maxit", 4000)
+    return p, keep
+
+
+class GpuLake:
+    def __init__(self, deck=None, precision="fp64", preconditioner="jacobi", tol=None,
+                 maxit=None, _handle=None, _dims=None):
+        self.L = _lib.load()
+        self.precision = precision
+        self._keep = []
+        if _handle is not None:
+            self.h = _handle
+            self.km1, self.lm = _dims
+        else:
+            p, keep = make_params(deck, precision, preconditioner, tol, maxit)
+            self._keep += keep
+            d = S3gDeck()
+            for name in ("imm", "jmm", "ibathyf", "ddz", "dzmin", "datadj", "zetainit", "zl"):
+                setattr(d, name, deck[name])
+            bathy = np.ascontiguousarray(deck["bathy_dm"], dtype=np.int32)
+            prof = np.ascontiguousarray(deck["init_profile"], dtype=np.float64)
+            self._keep += [bathy, prof]
+            d.bathy_dm = bathy.ctypes.data_as(_lib.ip)
+            d.init_profile = prof.ctypes.data_as(_lib.dp)
+            if deck["ibathyf"] < 0:
+                z = np.ascontiguousarray(deck["zlevel"], dtype=np.float64)
+                self._keep.append(z)
+                d.zlevel_file = z.ctypes.data_as(_lib.dp)
+                d.km1_file = len(z)
+            h = C.c_void_p()
+            check(self.L.s3g_create_from_deck(C.byref(d), C.byref(p), C.byref(h)), "s3g_create_from_deck")
+            self.h = h
+            st = self.stats()
+            self.km1, self.lm = st["km1"], st["lm"]
+        self.lm1 = self.lm + 1
+
+    @classmethod
+    def from_state(cls, grid, deck, fields, precision="fp64", preconditioner="jacobi", tol=None):
+        """Drop-in creation: `grid` holds the bathy/init index arrays (1-based,
+        reference layout), `fields` maps state-array names to reference-layout
+        arrays ((lm1, km1) for 3-D, (lm1,) for 2-D)."""
+        L = _lib.load()
+        g = S3gGrid()
+        keep = []
+        for name in ("im1", "jm1", "km1", "lm"):
+            setattr(g, name, int(grid[name]))
+        for name in ("l2i", "l2j", "lEC", "lWC", "lNC", "lSC", "kmz", "k1z", "k1u", "k1v"):
+            a = np.ascontiguousarray(grid[name], dtype=np.int32)
+            keep.append(a)
+            setattr(g, name, a.ctypes.data_as(_lib.ip))
+        for name in ("zlevel", "hhs", "hhu", "hhv"):
+            a = np.ascontiguousarray(grid[name], dtype=np.float32)
+            keep.append(a)
+            setattr(g, name, a.ctypes.data_as(_lib.fp))
+        p, k2 = make_params(deck, precision, preconditioner, tol)
+        keep += k2
+        h = C.c_void_p()
+        check(L.s3g_init(C.byref(g), C.byref(p), C.byref(h)), "s3g_init")
+        obj = cls(_handle=h, _dims=(int(grid["km1"]), int(grid["lm"])), precision=precision)
+        obj._keep = keep
+        for name, arr in fields.items():
+            obj.set(name, arr)
+        return obj
+
+    # -- lifecycle --------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.s3g_finalize(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- the reference's calls -------------------------------------------
+    def fd(self, n, istep, lastiter=1, thrs=None, iter=1):
+        if thrs is None:
+            thrs = 0.0
+        return check(self.L.s3g_fd(self.h, n, istep, iter, lastiter, thrs), "s3g_fd")
+
+    def settrap(self):
+        return check(self.L.s3g_settrap(self.h), "s3g_settrap")
+
+    def settrap2(self):
+        return check(self.L.s3g_settrap2(self.h), "s3g_settrap2")
+
+    def save(self, istep):
+        return check(self.L.s3g_save(self.h, istep), "s3g_save")
+
+    def step(self, n):
+        return check(self.L.s3g_step(self.h, n), "s3g_step")
+
+    def run(self, nsteps, start=1):
+        return check(self.L.s3g_run(self.h, start, nsteps), "s3g_run")
+
+    def phase(self, p, n, istep, thrs=0.0):
+        return check(self.L.s3g_phase(self.h, p, n, istep, thrs), "s3g_phase")
+
+    def synchronize(self):
+        return check(self.L.s3g_synchronize(self.h), "s3g_synchronize")
+
+    # -- fields -----------------------------------------------------------
+    def get(self, name, dtype=np.float64):
+        if name in INT_FIELDS:
+            out = np.empty(self.lm1, dtype=np.int32)
+            check(self.L.s3g_get_field_i32(self.h, name.encode(), out.ctypes.data_as(_lib.ip)), name)
+            return out
+        n = self.lm1 if name in TWO_D else self.lm1 * self.km1
+        out = np.empty(n, dtype=np.float64)
+        check(self.L.s3g_get_field_f64(self.h, name.encode(), out.ctypes.data_as(_lib.dp)), name)
+        return out if name in TWO_D else out.reshape(self.lm1, self.km1)
+
+    def set(self, name, values):
+        if name in INT_FIELDS:
+            v = np.ascontiguousarray(values, dtype=np.int32).ravel()
+            check(self.L.s3g_set_field_i32(self.h, name.encode(), v.ctypes.data_as(_lib.ip)), name)
+            return
+        v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        check(self.L.s3g_set_field_f64(self.h, name.encode(), v.ctypes.data_as(_lib.dp)), name)
+
+    def stats(self):
+        st = S3gStats()
+        check(self.L.s3g_get_stats(self.h, C.byref(st)), "s3g_get_stats")
+        return {name: getattr(st, name) for name, _ in S3gStats._fields_}
+
+    def timers(self, enable=True):
+        check(self.L.s3g_reset_timers(self.h, 1 if enable else 0), "s3g_reset_timers")
+
+    def kernel_times(self, maxn=64):
+        names = (C.c_char * 32 * maxn)()
+        avg = (C.c_double * maxn)()
+        cnt = (C.c_long * maxn)()
+        n = check(self.L.s3g_kernel_times(self.h, maxn, names, avg, cnt), "s3g_kernel_times")
+        return {bytes(names[i]).split(b"\0")[0].decode(): (avg[i], cnt[i]) for i in range(n)}

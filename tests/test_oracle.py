@@ -1,0 +1,227 @@
+"""Pins of the CPU oracle against what the mathematics fixes.
+
+The reference ships no tests, golden outputs or known-answer vectors, so each
+oracle routine is pinned here by a closed form, a library routine solving the
+same problem, or an exact invariant of the discretisation."""
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg
+import scipy.optimize
+import scipy.sparse
+import scipy.sparse.linalg
+
+from si3d_amd import lake
+from tests.helpers import small_ellipse, small_rectangle
+
+
+def _tridiag(n, rng, dominant=True):
+    a = rng.uniform(-1, 0, n)
+    c = rng.uniform(-1, 0, n)
+    a[0] = 0.0
+    c[-1] = 0.0
+    b = np.abs(a) + np.abs(c) + rng.uniform(0.1, 2.0, n) if dominant else rng.uniform(1, 2, n)
+    return a, b, c
+
+
+def _banded(a, b, c):
+    n = len(b)
+    ab = np.zeros((3, n))
+    ab[0, 1:] = c[:-1]
+    ab[1] = b
+    ab[2, :-1] = a[1:]
+    return ab
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 20, 61, 80])
+def test_trid_two_rhs_matches_banded_solver(oracle_lib, n):
+    """trid (si3d_procedures.f90:3210-3289) vs LAPACK banded solve."""
+    rng = np.random.default_rng(n)
+    a, b, c = _tridiag(n, rng)
+    g, r = rng.standard_normal(n), rng.standard_normal(n)
+    ag, ar = oracle_lib.trid(a, b, c, g, r)
+    ab = _banded(a, b, c)
+    np.testing.assert_allclose(ag, scipy.linalg.solve_banded((1, 1), ab, g), rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(ar, scipy.linalg.solve_banded((1, 1), ab, r), rtol=1e-12, atol=1e-13)
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 33, 81])
+def test_trid1_matches_banded_solver(oracle_lib, n):
+    """trid1 / tridT e-f sweeps (3292-3352, si3d_mixing.f90:554-612)."""
+    rng = np.random.default_rng(100 + n)
+    a, b, c = _tridiag(n, rng)
+    d = rng.standard_normal(n)
+    x = oracle_lib.trid1(a, b, c, d)
+    np.testing.assert_allclose(x, scipy.linalg.solve_banded((1, 1), _banded(a, b, c), d), rtol=1e-12, atol=1e-13)
+
+
+def test_eos_known_values(oracle_lib):
+    """UNESCO/Gill fresh-water density at the surface: 999.842594 at 0 C,
+    999.96675 at 5 C (Gill 1982, App. 3), maximum near 4 C."""
+    assert abs(oracle_lib.densty(0.0, 0.0, 0.0) - 999.842594) < 1e-9
+    assert abs(oracle_lib.densty(5.0, 0.0, 0.0) - 999.96675) < 5e-5
+    temps = np.linspace(0, 10, 1001)
+    rho = [oracle_lib.densty(t, 0.0, 0.0) for t in temps]
+    assert abs(temps[int(np.argmax(rho))] - 3.98) < 0.02
+
+
+def test_eos_pressure_fixed_point_matches_root(oracle_lib):
+    """The pressure iteration converges to the root of rho = rho0/(1-p(rho)/K(p))."""
+    t, s, z = 12.0, 4e-5, 80.0
+    rho0 = oracle_lib.densty(t, s, 0.0)
+
+    def K(p):
+        kw = 19652.21 + 148.4206 * t - 2.327105 * t**2 + 1.360477e-2 * t**3 - 5.155288e-5 * t**4
+        ks = kw + s * (54.6746 - 0.603459 * t + 1.09987e-2 * t**2 - 6.1670e-5 * t**3) + \
+            s * (7.944e-2 + 1.6483e-2 * t - 5.3009e-4 * t**2)
+        return ks + p * (3.239908 + 1.43713e-3 * t + 1.16092e-4 * t**2 - 5.77905e-7 * t**3) + \
+            p * s * (2.2838e-3 - 1.0981e-5 * t - 1.6078e-6 * t**2) + 1.91075e-4 * p * s + \
+            p**2 * (8.50935e-5 - 6.12293e-6 * t + 5.2787e-8 * t**2) + \
+            p**2 * s * (-9.9348e-7 + 2.0816e-8 * t + 9.1697e-10 * t**2)
+
+    f = lambda r: r - rho0 / (1.0 - 1e-5 * r * 9.806 * z / K(1e-5 * r * 9.806 * z))
+    root = scipy.optimize.brentq(f, 990.0, 1010.0, xtol=1e-12)
+    assert abs(oracle_lib.densty(t, s, z) - root) < 1e-6
+    assert oracle_lib.densty(t, s, z) > rho0  # compressibility raises density with depth
+
+
+def test_parab_exact_for_quadratics(oracle_lib):
+    """parab (si3d_utils.f90:895-920) interpolates a quadratic exactly."""
+    dx = 0.5
+    x = np.arange(12) * dx
+    fx = 3.0 - 2.0 * x + 0.75 * x**2
+    for t in [0.0, 0.1, 0.49, 0.5, 1.3, 2.71, 4.99]:
+        assert abs(oracle_lib.parab(0.0, t, fx, dx) - (3.0 - 2.0 * t + 0.75 * t**2)) < 1e-12
+
+
+def test_grid_indexing_hand_case(oracle_lib):
+    """bathy (si3d_utils.f90:595-892) on a 3x2 lake with one dry cell, uniform
+    layers of 1 m, dzmin 0.1: l is j-fastest within i; neighbours; kmz rule."""
+    dm = np.array([[45, 0], [12, 30], [20, 21]], dtype=np.int32)  # (i, j) interior, dm
+    d = lake.rectangle(nx=3, ny=2, depth=5.0)
+    d["bathy_dm"] = dm
+    o = oracle_lib.Oracle(d)
+    assert o.lm == 5
+    l2i = o.get("l2i")[:5].astype(int)
+    l2j = o.get("l2j")[:5].astype(int)
+    assert list(zip(l2i, l2j)) == [(2, 2), (3, 2), (3, 3), (4, 2), (4, 3)]
+    lEC = o.get("lEC")[:5].astype(int)
+    lWC = o.get("lWC")[:5].astype(int)
+    lNC = o.get("lNC")[:5].astype(int)
+    lSC = o.get("lSC")[:5].astype(int)
+    lm1 = 6
+    assert list(lEC) == [2, 4, 5, lm1, lm1]
+    assert list(lWC) == [lm1, 1, lm1, 2, 3]
+    assert list(lNC) == [lm1, 3, lm1, 5, lm1]
+    assert list(lSC) == [lm1, lm1, 2, lm1, 4]
+    for l in range(1, 6):
+        if lEC[l - 1] != lm1:
+            assert lWC[lEC[l - 1] - 1] == l
+    # depths 4.5, 1.2, 3.0, 2.0, 2.1 m; kmz = FLOOR(h/dz) + (remainder > dzmin) + 1
+    # in FP32: 21 dm * 0.1f = 2.1000001f, whose remainder 0.1000001 exceeds
+    # dzmin = 0.1f, so that column gets a thin third layer; 3.0 and 2.0 have
+    # remainder 0 and keep hhs = kmz*dz.
+    kmz = o.get("kmz")[:5].astype(int)
+    hhs = o.get("hhs")[:5]
+    assert list(kmz) == [6, 3, 4, 3, 4]
+    np.testing.assert_array_equal(hhs, np.float32([21 * np.float32(0.1) * 0 + 45 * np.float32(0.1),
+                                                   12 * np.float32(0.1), 3.0, 2.0,
+                                                   np.float32(21) * np.float32(0.1)]).astype(np.float64))
+    hhu = o.get("hhu")[:5]
+    np.testing.assert_allclose(hhu[:3], [min(hhs[0], hhs[1]), min(hhs[1], hhs[3]), min(hhs[2], hhs[4])], atol=0)
+
+
+def test_free_surface_solution_matches_direct_solve(oracle_lib):
+    """The assembled system (matcon, 1749-1867) is symmetric positive definite
+    and the oracle's PCG solution equals a sparse direct solve."""
+    d = small_ellipse()
+    d["pcg_tol"] = 1e-13
+    o = oracle_lib.Oracle(d)
+    o.step(1)
+    for ph in range(0, 7):
+        o.phase(ph, 2, 1, 2 * d["dt"] / 3600)
+    lm = o.lm
+    sx, sy, rr, qq = (o.get(n)[:lm] for n in ("sx", "sy", "rr", "qq"))
+    lE = o.get("lEC")[:lm].astype(int) - 1
+    lN = o.get("lNC")[:lm].astype(int) - 1
+    rows, cols, vals = list(range(lm)), list(range(lm)), list(rr)
+    for c in range(lm):
+        if lE[c] < lm:
+            rows += [c, lE[c]]; cols += [lE[c], c]; vals += [-sx[c], -sx[c]]
+        if lN[c] < lm:
+            rows += [c, lN[c]]; cols += [lN[c], c]; vals += [-sy[c], -sy[c]]
+    A = scipy.sparse.csr_matrix((vals, (rows, cols)), shape=(lm, lm))
+    assert np.all(sx >= 0) and np.all(sy >= 0)
+    offsum = np.asarray(abs(A).sum(axis=1)).ravel() - rr
+    assert np.all(rr > offsum)  # strictly diagonally dominant M-matrix
+    ref = scipy.sparse.linalg.spsolve(A.tocsc(), qq)
+    o.phase(7, 2, 1, 2 * d["dt"] / 3600)
+    s = o.get("s")[:lm]
+    assert np.max(np.abs(s - ref)) <= 1e-10 * max(np.max(np.abs(ref)), 1e-3)
+
+
+def test_volume_conserved_closed_basin(oracle_lib):
+    """sum(zeta) is invariant: with closed boundaries the column sums of A are 1
+    and the flux divergences in qq telescope (matcon 1822-1829)."""
+    d = small_ellipse(nts=30)
+    d["pcg_tol"] = 1e-13
+    o = oracle_lib.Oracle(d)
+    v0 = o.get("s")[:o.lm].sum()
+    for n in range(1, 31):
+        o.step(n)
+    v1 = o.get("s")[:o.lm].sum()
+    smax = np.max(np.abs(o.get("s")[:o.lm]))
+    assert smax > 1e-5
+    assert abs(v1 - v0) <= 1e-9 * o.lm * smax
+
+
+def test_uniform_temperature_stays_uniform(oracle_lib):
+    """Free-stream preservation of the TVD transport + implicit diffusion
+    (exsal/imsal): a uniform scalar stays uniform under a divergence-free flow."""
+    d = small_ellipse(nts=10)
+    d["init_profile"] = np.full(len(d["init_profile"]), 17.0)
+    o = oracle_lib.Oracle(d)
+    for n in range(1, 11):
+        o.step(n)
+    sal = o.get("sal")
+    kmz = o.get("kmz").astype(int)
+    k1z = o.get("k1z").astype(int)
+    vals = np.concatenate([sal[l, k1z[l] - 1:kmz[l]] for l in range(o.lm)])
+    assert np.max(np.abs(vals - 17.0)) < 1e-9
+    assert np.max(np.abs(o.get("uh"))) > 1e-4  # the flow is not trivial
+
+
+def test_wind_setup_momentum_balance(oracle_lib):
+    """Wind set-up in a closed channel with constant eddy viscosity, no
+    advection and no rotation.  Averaged over many seiche periods, the
+    depth-integrated momentum balance rho g H dzeta/dx = tau_s - tau_b holds
+    (pressure gradient, wind stress and quadratic bottom stress as applied in
+    matmom 1348-1369 and vel 2478-2483), and the slope lies between the
+    free-slip (tau/(rho g H)) and no-slip (3 tau/(2 rho g H)) limits."""
+    tau = 0.1
+    d = lake.rectangle(nx=40, ny=3, dx=250.0, depth=10.0, dz=1.0, tau=tau, dt=60.0, nts=3000,
+                       iadv=0, ihd=0, Av0=5e-3, cd=2.5e-3)
+    d["pcg_tol"] = 1e-12
+    o = oracle_lib.Oracle(d)
+    l2i = o.get("l2i")[:o.lm]
+    j2 = o.get("l2j")[:o.lm] == 3
+    x = (l2i[j2] - 2) * 250.0
+    mid = np.where(j2)[0][len(x) // 2]
+    kmz = int(o.get("kmz")[mid])
+    rho, g, H = 1000.0, 9.806, 10.0
+    slopes, taub, taus = [], [], []
+    for n in range(1, 3001):
+        o.step(n)
+        if n > 1000:
+            s = o.get("s")[:o.lm]
+            slopes.append(np.polyfit(x[5:-5], s[j2][5:-5], 1)[0])
+            u = o.get("u")
+            ub, us = u[mid, kmz - 1], u[mid, 1]
+            taub.append(rho * d["cd"] * abs(ub) * ub)
+            taus.append(1.3 * 1.2e-3 * abs(d["wa"] - us) * d["wa"])
+    slope = float(np.mean(slopes))
+    free_slip = tau / (rho * g * H)
+    assert free_slip * 0.95 < slope < 1.5 * free_slip
+    balance = rho * g * H * slope - (np.mean(taus) - np.mean(taub))
+    assert abs(balance) < 0.03 * tau, balance
