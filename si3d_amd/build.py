@@ -42,6 +42,28 @@ def _compile(src):
     return obj
 
 
+def build_fortran_binding(verbose=False):
+    """Compile the ISO_C_BINDING module with flang when the image has it (a
+    check that the Fortran side of the boundary stays consistent)."""
+    flang = None
+    for c in (os.environ.get("FLANG"), "/opt/rocm/lib/llvm/bin/flang", shutil.which("flang")):
+        if c and os.path.exists(c):
+            flang = c
+            break
+    if flang is None:
+        return None
+    out = os.path.join(OUT, "fortran")
+    os.makedirs(out, exist_ok=True)
+    src = os.path.join(HERE, "fortran", "si3d_gpu_mod.f90")
+    obj = os.path.join(out, "si3d_gpu_mod.o")
+    r = subprocess.run([flang, "-c", src, "-o", obj], cwd=out, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"flang failed on si3d_gpu_mod.f90\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print("built", obj)
+    return obj
+
+
 def build(verbose=False):
     os.makedirs(OUT, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=min(4, os.cpu_count() or 1)) as ex:
@@ -54,6 +76,7 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_fortran_binding(verbose)
     return LIB
 
 

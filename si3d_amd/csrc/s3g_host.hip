@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -319,6 +320,7 @@ int s3g_init(const s3g_grid *g, const s3g_params *p, s3g_ctx **out) {
   }
   s3g_ctx *c = new s3g_ctx();
   c->precision = p->precision;
+  c->debug = std::getenv("S3G_DEBUG") != nullptr;
   c->prec_kind = p->preconditioner;
   c->im1 = g->im1;
   c->jm1 = g->jm1;
@@ -424,6 +426,9 @@ int s3g_init(const s3g_grid *g, const s3g_params *p, s3g_ctx **out) {
   double **pv[] = {&G.r, &G.z, &G.p0, &G.p1, &G.q, &G.x, &G.lc, &G.ld};
   for (double **q : pv)
     if (rc == 0 && !(*q = dalloc<double>(c, n2))) rc = S3G_ERR_HIP;
+  float **pf[] = {&G.rf, &G.zf, &G.pf0, &G.pf1, &G.qf, &G.sxf, &G.syf, &G.rrf, &G.bf, &G.df, &G.lcf, &G.ldf};
+  for (float **q : pf)
+    if (rc == 0 && !(*q = dalloc<float>(c, c->precision == S3G_MIXED ? n2 : 1))) rc = S3G_ERR_HIP;
   if (rc == 0) rc = pcg_setup(c, lN.data());
   if (rc == 0 && !(G.part = dalloc<double>(c, 8 * (size_t)G.nblk + 64))) rc = S3G_ERR_HIP;
   if (rc == 0 && !(G.iters = dalloc<int>(c, 4))) rc = S3G_ERR_HIP;

@@ -59,7 +59,7 @@ struct Pcg {
   double *part;    // partial sums: [4][nblk]
   double *scal;    // device scalars
   int *iters;      // device iteration counter
-  float *rf, *zf, *pf0, *pf1, *qf, *sxf, *syf, *rrf, *diagf;  // FP32 inner solve
+  float *rf, *zf, *pf0, *pf1, *qf, *sxf, *syf, *rrf, *bf, *df;  // FP32 inner solve
   // line preconditioner: tridiagonal factors along N-S lines
   int nlines;
   int *line_start;  // [nlines+1] first column of each line (lines are contiguous in l)
@@ -97,6 +97,7 @@ struct s3g_ctx {
   s3g_stats stats{};
   // timers
   bool timing = false;
+  bool debug = false;  // S3G_DEBUG set: per-solve diagnostics on stderr
   std::vector<s3g::Timer> timers;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
   std::vector<std::pair<int, int>> pending;  // (timer index, event pair index)

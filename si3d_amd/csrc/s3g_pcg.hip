@@ -6,30 +6,32 @@
 // the upper ones (1954-1967); it is applied matrix-free from rr/sx/sy and the
 // neighbour index arrays.  The initial guess is sp (1970).
 //
-// Conjugate gradients with FP64 vectors and dots.  Every reduction is a
-// deterministic two-level sum: each workgroup writes one partial (64-lane
-// wavefront shuffles, then LDS across the four wavefronts), and the next kernel
-// re-reduces the partials in a fixed order, so all workgroups derive the same
-// alpha/beta and the same convergence decision without a host round trip.
-// Iterations are enqueued in chunks; once converged, the remaining kernels of a
-// chunk return at entry.  The stopping test is the FP64 relative residual
-// ||qq - A zeta|| <= tol ||qq||, confirmed by a true-residual recompute.
+// Conjugate gradients, templated on the vector type T.  Every reduction is
+// FP64 and deterministic: each workgroup writes one partial (64-lane wavefront
+// shuffles, then LDS across the four wavefronts) and the next kernel
+// re-reduces the partials in a fixed order, so every workgroup derives the
+// same alpha, beta and convergence decision without a host round trip.
+// Iterations are enqueued in chunks; once converged the remaining kernels of a
+// chunk return at entry.
+//
+// FP64 mode: one CG with T = double, stopped on ||r|| <= tol/2 ||qq|| and
+// confirmed by an FP64 true-residual recompute.
+// Mixed mode: iterative refinement.  The outer loop forms the FP64 true
+// residual r = qq - A zeta; an inner CG with T = float (FP32 matrix copy,
+// vectors and SpMV; FP64 dots) reduces it by `kInnerTol`; zeta += d in FP64.
 //
 // Preconditioners: Jacobi, or an exact tridiagonal solve along each north-south
 // line of wet cells (the wet cells of one i-row are consecutive in l, so a line
-// is a contiguous range of columns: one lane per line runs the Thomas sweep;
-// its factors are computed once per matrix).
-//
-// Mixed precision (S3G_MIXED): iterative refinement.  The outer loop computes
-// the true residual in FP64; an inner CG in FP32 (FP32 SpMV and vectors, FP64
-// dots) solves the correction to a loose tolerance; zeta is updated in FP64.
+// is a contiguous range of columns; one lane per line runs the Thomas sweep,
+// factors computed once per matrix).
 #include <cmath>
+#include <cstdio>
 
 #include "s3g_internal.h"
 
 namespace s3g {
 
-#define SIX(c) (c)
+constexpr double kInnerTol = 1e-5;  // inner FP32 reduction per refinement sweep
 
 __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -61,63 +63,44 @@ __device__ double reduce_partials(const double *part, int nblk, double *lds) {
   return r;
 }
 
-struct Mat {
+// Partial sums layout: [RZ0 | RZ1 | RR0 | RR1 | PQ | BB | TR] x nblk
+enum { P_RZ0 = 0, P_RZ1 = 1, P_RR0 = 2, P_RR1 = 3, P_PQ = 4, P_BB = 5, P_TR = 6, P_NUM = 7 };
+
+template <typename T>
+struct CgArgs {
   const int *lE, *lW, *lN, *lS;
-  const double *rr, *sx, *sy;
-  int n, S;
+  const T *rr, *sx, *sy;  // matrix (sx(S) = sy(S) = 0)
+  int n, nblk;
+  T *x, *r, *z, *p0, *p1, *q;
+  const T *b;
+  double *part;
+  int *iters;
+  double tol2;  // stop when (r,r) <= tol2 * (b,b)
+  int nlines;
+  const int *line_start;
+  const T *lc, *ld;
 };
 
 template <typename T>
-__device__ __forceinline__ double apply_A(const Mat &M, const T *v, int c) {
-  const int lW = M.lW[c], lS = M.lS[c], lN = M.lN[c], lE = M.lE[c];
-  double acc = M.rr[c] * (double)v[c];
-  acc -= M.sx[lW] * (double)v[lW];  // sx(S)=0 and v(S)=0 at the sentinel
-  acc -= M.sy[lS] * (double)v[lS];
-  acc -= M.sy[c] * (double)v[lN];
-  acc -= M.sx[c] * (double)v[lE];
-  return acc;
+__device__ __forceinline__ double Ax(const CgArgs<T> &A, const T *v, int c) {
+  const int lW = A.lW[c], lS = A.lS[c], lN = A.lN[c], lE = A.lE[c];
+  return (double)A.rr[c] * (double)v[c] - (double)A.sx[lW] * (double)v[lW] - (double)A.sy[lS] * (double)v[lS] -
+         (double)A.sy[c] * (double)v[lN] - (double)A.sx[c] * (double)v[lE];
 }
 
-// Partial sums layout in Pcg::part: [RZ0 | RZ1 | RR0 | RR1 | PQ | BB | TR] x nblk
-enum { P_RZ0 = 0, P_RZ1 = 1, P_RR0 = 2, P_RR1 = 3, P_PQ = 4, P_BB = 5, P_TR = 6, P_NUM = 7 };
-
-struct PcgArgs {
-  Mat M;
-  int nblk;
-  double *x, *r, *z, *p0, *p1, *q, *b;
-  const double *diag;
-  double *part;
-  int *iters;
-  double tol2;  // (0.5*tol)^2
-  // line preconditioner
-  int nlines;
-  const int *line_start;
-  const double *lc, *ld;
-};
-
-// z = M^-1 r for one column (Jacobi) -- the line preconditioner runs separately.
-__device__ __forceinline__ double jacobi(const PcgArgs &A, int c, double r) { return r / A.diag[c]; }
-
-__global__ void __launch_bounds__(kBlock) k_pcg_start(PcgArgs A, const double *sp) {
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c <= A.M.n; c += gridDim.x * blockDim.x) {
-    A.x[c] = (c < A.M.n) ? sp[c] : 0.0;
-    A.p0[c] = A.p1[c] = A.z[c] = A.r[c] = A.q[c] = 0.0;
-  }
-}
-
-// r = b - A x; z = M^-1 r (Jacobi); partials of (r,z), (r,r), (b,b).
-template <bool JACOBI>
-__global__ void __launch_bounds__(kBlock) k_pcg_residual(PcgArgs A) {
+// r = b - A x; Jacobi z; partials (r,z), (r,r), (b,b).
+template <typename T, bool JACOBI>
+__global__ void __launch_bounds__(kBlock) k_cg_residual(CgArgs<T> A) {
   __shared__ double lds[16];
   double rz = 0.0, rr = 0.0, bb = 0.0;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < A.M.n; c += gridDim.x * blockDim.x) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < A.n; c += gridDim.x * blockDim.x) {
     const double b = A.b[c];
-    const double r = b - apply_A(A.M, A.x, c);
-    A.r[c] = r;
+    const double r = b - Ax(A, A.x, c);
+    A.r[c] = (T)r;
     if (JACOBI) {
-      const double z = jacobi(A, c, r);
-      A.z[c] = z;
-      rz += r * z;
+      const double z = (double)(T)r / (double)A.rr[c];
+      A.z[c] = (T)z;
+      rz += (double)(T)r * (double)(T)z;
     }
     rr += r * r;
     bb += b * b;
@@ -126,54 +109,42 @@ __global__ void __launch_bounds__(kBlock) k_pcg_residual(PcgArgs A) {
   rr = block_sum(rr, lds);
   bb = block_sum(bb, lds);
   if (threadIdx.x == 0) {
-    A.part[P_RZ0 * A.nblk + blockIdx.x] = rz;
+    if (JACOBI) A.part[P_RZ0 * A.nblk + blockIdx.x] = rz;
     A.part[P_RR0 * A.nblk + blockIdx.x] = rr;
     A.part[P_BB * A.nblk + blockIdx.x] = bb;
-    A.part[P_TR * A.nblk + blockIdx.x] = rr;
   }
 }
 
-// Line preconditioner: z = T^-1 r with T the tridiagonal part of A along each
-// north-south line (diagonal rr, off-diagonals -sy).  Factors: ld = 1/pivot,
-// lc = modified super-diagonal.  One lane per line.  SLOT >= 0 also writes the
-// partial of (r,z) for that slot.
-template <int SLOT>
-__global__ void __launch_bounds__(kBlock) k_line_solve(PcgArgs A) {
+// Line preconditioner z = T^-1 r (tridiagonal part of A along each N-S line),
+// partial (r,z) into SLOT.  Blocks beyond the line count write zero partials.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_line_solve(CgArgs<T> A, int slot) {
   __shared__ double lds[16];
   double rz = 0.0;
   for (int ln = blockIdx.x * blockDim.x + threadIdx.x; ln < A.nlines; ln += gridDim.x * blockDim.x) {
     const int c0 = A.line_start[ln], c1 = A.line_start[ln + 1];
-    // forward: y(c) = (r(c) + sy(c-1) y(c-1)) * ld(c)
     double y = 0.0;
     for (int c = c0; c < c1; ++c) {
-      const double off = (c > c0) ? A.M.sy[c - 1] : 0.0;
-      y = (A.r[c] + off * y) * A.ld[c];
-      A.z[c] = y;
+      const double off = (c > c0) ? (double)A.sy[c - 1] : 0.0;
+      y = ((double)A.r[c] + off * y) * (double)A.ld[c];
+      A.z[c] = (T)y;
     }
-    // backward: z(c) = y(c) - lc(c) z(c+1)
     double zn = 0.0;
     for (int c = c1 - 1; c >= c0; --c) {
-      const double zc = (c < c1 - 1) ? A.z[c] - A.lc[c] * zn : A.z[c];
-      A.z[c] = zc;
+      const double zc = (c < c1 - 1) ? (double)A.z[c] - (double)A.lc[c] * zn : (double)A.z[c];
+      A.z[c] = (T)zc;
       zn = zc;
-      if (SLOT >= 0) rz += A.r[c] * zc;
+      rz += (double)A.r[c] * (double)(T)zc;
     }
   }
-  if (SLOT >= 0) {
-    rz = block_sum(rz, lds);
-    if (threadIdx.x == 0) A.part[SLOT * A.nblk + blockIdx.x] = rz;
-  }
+  rz = block_sum(rz, lds);
+  if (threadIdx.x == 0) A.part[slot * A.nblk + blockIdx.x] = rz;
 }
 
-// Zero the partials a line-solve block range does not cover.
-__global__ void k_zero_part(double *p, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0.0;
-}
-
-// Iteration i, step 1: beta from (r,z) partials, p_new = z + beta p_old at the
-// column and its neighbours, q = A p_new, partial (p,q).
-__global__ void __launch_bounds__(kBlock) k_pcg_dir(PcgArgs A, int i) {
+// Iteration i, step 1: beta, p_new = z + beta p_old at the column and its four
+// neighbours (double-buffered p), q = A p_new, partial (p,q).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_cg_dir(CgArgs<T> A, int i) {
   __shared__ double lds[16];
   const int cur = i & 1;
   const double rr = reduce_partials(A.part + (P_RR0 + cur) * A.nblk, A.nblk, lds);
@@ -182,44 +153,49 @@ __global__ void __launch_bounds__(kBlock) k_pcg_dir(PcgArgs A, int i) {
   const double rz = reduce_partials(A.part + (P_RZ0 + cur) * A.nblk, A.nblk, lds);
   double beta = 0.0;
   if (i > 0) beta = rz / reduce_partials(A.part + (P_RZ0 + (cur ^ 1)) * A.nblk, A.nblk, lds);
-  const double *pold = cur ? A.p0 : A.p1;
-  double *pnew = cur ? A.p1 : A.p0;
+  const T *pold = cur ? A.p0 : A.p1;
+  T *pnew = cur ? A.p1 : A.p0;
   double pq = 0.0;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < A.M.n; c += gridDim.x * blockDim.x) {
-    const int lW = A.M.lW[c], lS = A.M.lS[c], lN = A.M.lN[c], lE = A.M.lE[c];
-    auto pn = [&](int j) { return A.z[j] + beta * pold[j]; };  // z(S) = p(S) = 0
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < A.n; c += gridDim.x * blockDim.x) {
+    const int lW = A.lW[c], lS = A.lS[c], lN = A.lN[c], lE = A.lE[c];
+    auto pn = [&](int j) { return (double)(T)((double)A.z[j] + beta * (double)pold[j]); };  // z(S)=p(S)=0
     const double pc = pn(c);
-    pnew[c] = pc;
-    const double q = A.M.rr[c] * pc - A.M.sx[lW] * pn(lW) - A.M.sy[lS] * pn(lS) - A.M.sy[c] * pn(lN) -
-                     A.M.sx[c] * pn(lE);
-    A.q[c] = q;
-    pq += pc * q;
+    pnew[c] = (T)pc;
+    const double q = (double)A.rr[c] * pc - (double)A.sx[lW] * pn(lW) - (double)A.sy[lS] * pn(lS) -
+                     (double)A.sy[c] * pn(lN) - (double)A.sx[c] * pn(lE);
+    A.q[c] = (T)q;
+    pq += pc * (double)(T)q;
   }
   pq = block_sum(pq, lds);
   if (threadIdx.x == 0) A.part[P_PQ * A.nblk + blockIdx.x] = pq;
 }
 
-// Iteration i, step 2: alpha, x += alpha p, r -= alpha q, z = M^-1 r (Jacobi),
-// partials (r,z) and (r,r) for iteration i+1.
-template <bool JACOBI>
-__global__ void __launch_bounds__(kBlock) k_pcg_update(PcgArgs A, int i) {
+// Iteration i, step 2: alpha, x += alpha p, r -= alpha q, Jacobi z, partials
+// (r,z) and (r,r) for iteration i+1.
+template <typename T, bool JACOBI>
+__global__ void __launch_bounds__(kBlock) k_cg_update(CgArgs<T> A, int i) {
   __shared__ double lds[16];
   const int cur = i & 1, nxt = cur ^ 1;
   const double rr = reduce_partials(A.part + (P_RR0 + cur) * A.nblk, A.nblk, lds);
   const double bb = reduce_partials(A.part + P_BB * A.nblk, A.nblk, lds);
-  if (rr <= A.tol2 * bb) return;
+  if (rr <= A.tol2 * bb) {
+    // Converged: carry this residual partial into the slot iteration i+1 reads,
+    // so every later kernel of the chunk also sees convergence and returns.
+    if (threadIdx.x == 0) A.part[(P_RR0 + nxt) * A.nblk + blockIdx.x] = A.part[(P_RR0 + cur) * A.nblk + blockIdx.x];
+    return;
+  }
   const double rz = reduce_partials(A.part + (P_RZ0 + cur) * A.nblk, A.nblk, lds);
   const double pq = reduce_partials(A.part + P_PQ * A.nblk, A.nblk, lds);
   const double alpha = rz / pq;
-  const double *p = cur ? A.p1 : A.p0;
+  const T *p = cur ? A.p1 : A.p0;
   double rzn = 0.0, rrn = 0.0;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < A.M.n; c += gridDim.x * blockDim.x) {
-    A.x[c] += alpha * p[c];
-    const double r = A.r[c] - alpha * A.q[c];
-    A.r[c] = r;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < A.n; c += gridDim.x * blockDim.x) {
+    A.x[c] = (T)((double)A.x[c] + alpha * (double)p[c]);
+    const double r = (double)(T)((double)A.r[c] - alpha * (double)A.q[c]);
+    A.r[c] = (T)r;
     if (JACOBI) {
-      const double z = jacobi(A, c, r);
-      A.z[c] = z;
+      const double z = (double)(T)(r / (double)A.rr[c]);
+      A.z[c] = (T)z;
       rzn += r * z;
     }
     rrn += r * r;
@@ -233,32 +209,77 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update(PcgArgs A, int i) {
   }
 }
 
-// True residual partials: ||b - A x||^2 into slot TR.
-__global__ void __launch_bounds__(kBlock) k_pcg_true_residual(PcgArgs A) {
-  __shared__ double lds[16];
-  double rr = 0.0;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < A.M.n; c += gridDim.x * blockDim.x) {
-    const double r = A.b[c] - apply_A(A.M, A.x, c);
-    rr += r * r;
-  }
-  rr = block_sum(rr, lds);
-  if (threadIdx.x == 0) A.part[P_TR * A.nblk + blockIdx.x] = rr;
-}
-
-// Line factors: for each line, Thomas pivots of tridiag(-sy(c-1), rr(c), -sy(c)).
-__global__ void __launch_bounds__(kBlock) k_line_factor(PcgArgs A, double *lc, double *ld) {
+// Line factors of tridiag(-sy(c-1), rr(c), -sy(c)) per line: ld = 1/pivot,
+// lc = modified super-diagonal.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_line_factor(CgArgs<T> A, T *lc, T *ld) {
   for (int ln = blockIdx.x * blockDim.x + threadIdx.x; ln < A.nlines; ln += gridDim.x * blockDim.x) {
     const int c0 = A.line_start[ln], c1 = A.line_start[ln + 1];
     double cprev = 0.0;
     for (int c = c0; c < c1; ++c) {
-      const double a = (c > c0) ? -A.M.sy[c - 1] : 0.0;  // sub-diagonal
-      const double piv = A.M.rr[c] - a * cprev;
-      const double inv = 1.0 / piv;
-      ld[c] = inv;
-      const double csup = (c < c1 - 1) ? -A.M.sy[c] * inv : 0.0;
-      lc[c] = csup;
+      const double a = (c > c0) ? -(double)A.sy[c - 1] : 0.0;
+      const double inv = 1.0 / ((double)A.rr[c] - a * cprev);
+      ld[c] = (T)inv;
+      const double csup = (c < c1 - 1) ? -(double)A.sy[c] * inv : 0.0;
+      lc[c] = (T)csup;
       cprev = csup;
     }
+  }
+}
+
+// ---- FP64 outer level (zeta, qq, true residual) ---------------------------
+struct Outer {
+  const int *lE, *lW, *lN, *lS;
+  const double *rr, *sx, *sy, *b;
+  double *x;
+  int n, nblk;
+  double *part;
+};
+
+__global__ void __launch_bounds__(kBlock) k_start_guess(double *x, const double *sp, int n) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c <= n; c += gridDim.x * blockDim.x) x[c] = (c < n) ? sp[c] : 0.0;
+}
+
+// FP64 true residual; optionally stores it as the right-hand side (cast) of the
+// inner solve and clears the inner vectors.  Partials: TR = (r,r), BB = (b,b).
+template <typename T, bool STORE>
+__global__ void __launch_bounds__(kBlock) k_true_residual(Outer O, T *rin, T *d, T *p0, T *p1, T *z) {
+  __shared__ double lds[16];
+  double rr = 0.0, bb = 0.0;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < O.n; c += gridDim.x * blockDim.x) {
+    const int lW = O.lW[c], lS = O.lS[c], lN = O.lN[c], lE = O.lE[c];
+    const double b = O.b[c];
+    const double r = b - (O.rr[c] * O.x[c] - O.sx[lW] * O.x[lW] - O.sy[lS] * O.x[lS] - O.sy[c] * O.x[lN] -
+                          O.sx[c] * O.x[lE]);
+    if (STORE) {
+      rin[c] = (T)r;
+      d[c] = T(0);
+      p0[c] = p1[c] = z[c] = T(0);
+    }
+    rr += r * r;
+    bb += b * b;
+  }
+  rr = block_sum(rr, lds);
+  bb = block_sum(bb, lds);
+  if (threadIdx.x == 0) {
+    O.part[P_TR * O.nblk + blockIdx.x] = rr;
+    O.part[P_BB * O.nblk + blockIdx.x] = bb;
+  }
+}
+
+// x += d (FP64 accumulation of the inner correction).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_add_correction(double *x, const T *d, int n) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) x[c] += (double)d[c];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_cast_matrix(const double *rr, const double *sx, const double *sy, T *rrT,
+                                                        T *sxT, T *syT, int n) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c <= n; c += gridDim.x * blockDim.x) {
+    rrT[c] = (T)rr[c];
+    sxT[c] = (T)sx[c];
+    syT[c] = (T)sy[c];
   }
 }
 
@@ -270,109 +291,163 @@ __global__ void k_copy_to_s(double *s, const double *x, int n) {
 // ---------------------------------------------------------------------------
 
 int pcg_setup(s3g_ctx *c, const int *lN0) {
-  // lN0: 0-based north neighbour of each column (sentinel = lm).  North-south
-  // lines are maximal runs of consecutive columns c-1 -> c with lN(c-1) = c.
+  // North-south lines: maximal runs of consecutive columns c-1 -> c with
+  // lN(c-1) = c (lN0 is 0-based, sentinel = lm).
   Pcg &P = c->pcg;
   P.n = c->lm;
-  P.nblk = std::min((P.n + kBlock - 1) / kBlock, 1024);
-  if (P.nblk < 1) P.nblk = 1;
+  P.nblk = std::max(1, std::min((P.n + kBlock - 1) / kBlock, 1024));
   std::vector<int> starts;
   for (int cc = 0; cc < c->lm; ++cc)
     if (cc == 0 || lN0[cc - 1] != cc) starts.push_back(cc);
   starts.push_back(c->lm);
   P.nlines = (int)starts.size() - 1;
-  if (hip_check(hipMalloc(&P.line_start, starts.size() * sizeof(int)), "alloc lines") < 0) return S3G_ERR_HIP;
-  c->allocs.push_back(P.line_start);
+  void *p = nullptr;
+  if (hip_check(hipMalloc(&p, starts.size() * sizeof(int)), "alloc lines") < 0) return S3G_ERR_HIP;
+  c->allocs.push_back(p);
+  P.line_start = (int *)p;
   return hip_check(hipMemcpy(P.line_start, starts.data(), starts.size() * sizeof(int), hipMemcpyHostToDevice),
                    "upload lines");
+}
+
+static double host_sum(s3g_ctx *c, const double *dev_part, int n) {
+  hipMemcpyAsync(c->h_scal, dev_part, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return NAN;
+  double s = 0.0;
+  for (int k = 0; k < n; ++k) s += c->h_scal[k];
+  return s;
+}
+
+// Runs CG iterations on CgArgs A (x, r, b set up) until (r,r) <= tol2 (b,b) or
+// maxit; returns the number of completed iterations (< 0 on HIP error).
+template <typename T>
+static int run_cg(s3g_ctx *c, CgArgs<T> &A, bool line, int maxit, int hint) {
+  const dim3 g(A.nblk), b(kBlock);
+  const int nlb = std::min(A.nblk, std::max(1, (A.nlines + kBlock - 1) / kBlock));
+  auto zero_rz = [&](int slot) {
+    // the line solve writes partials only for its own (possibly fewer) blocks
+    hipMemsetAsync(A.part + slot * A.nblk, 0, A.nblk * sizeof(double), c->stream);
+  };
+  if (line) {
+    hipLaunchKernelGGL((k_cg_residual<T, false>), g, b, 0, c->stream, A);
+    zero_rz(P_RZ0);
+    hipLaunchKernelGGL((k_line_solve<T>), dim3(nlb), b, 0, c->stream, A, (int)P_RZ0);
+  } else {
+    hipLaunchKernelGGL((k_cg_residual<T, true>), g, b, 0, c->stream, A);
+  }
+  hipMemsetAsync(A.iters, 0, sizeof(int), c->stream);
+  int it = 0, slot = -1;
+  int chunk = std::max(8, std::min(hint + 4, 512));
+  while (it < maxit) {
+    const int nl = std::min(chunk, maxit - it);
+    if (c->timing) timer_begin(c, sizeof(T) == 8 ? "pcg_chunk_f64" : "pcg_chunk_f32", &slot);
+    for (int j = 0; j < nl; ++j) {
+      const int i = it + j;
+      hipLaunchKernelGGL((k_cg_dir<T>), g, b, 0, c->stream, A, i);
+      if (line) {
+        hipLaunchKernelGGL((k_cg_update<T, false>), g, b, 0, c->stream, A, i);
+        const int nxt = P_RZ0 + ((i + 1) & 1);
+        zero_rz(nxt);
+        hipLaunchKernelGGL((k_line_solve<T>), dim3(nlb), b, 0, c->stream, A, nxt);
+      } else {
+        hipLaunchKernelGGL((k_cg_update<T, true>), g, b, 0, c->stream, A, i);
+      }
+    }
+    if (c->timing) timer_end(c, slot);
+    it += nl;
+    hipMemcpyAsync(c->h_iters, A.iters, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (hip_check(hipStreamSynchronize(c->stream), "pcg sync") < 0) return -1;
+    if (c->h_iters[0] < it) break;  // fewer updates than launched: converged
+    chunk = 8;
+  }
+  return c->h_iters[0];
+}
+
+template <typename T>
+static CgArgs<T> cg_args(s3g_ctx *c, const int *lE, const int *lW, const int *lN, const int *lS, const T *rr,
+                         const T *sx, const T *sy, T *x, T *r, T *z, T *p0, T *p1, T *q, const T *b, const T *lc,
+                         const T *ld, double tol) {
+  Pcg &P = c->pcg;
+  CgArgs<T> A{};
+  A.lE = lE;
+  A.lW = lW;
+  A.lN = lN;
+  A.lS = lS;
+  A.rr = rr;
+  A.sx = sx;
+  A.sy = sy;
+  A.n = P.n;
+  A.nblk = P.nblk;
+  A.x = x;
+  A.r = r;
+  A.z = z;
+  A.p0 = p0;
+  A.p1 = p1;
+  A.q = q;
+  A.b = b;
+  A.part = P.part;
+  A.iters = P.iters;
+  A.tol2 = tol * tol;
+  A.nlines = P.nlines;
+  A.line_start = P.line_start;
+  A.lc = lc;
+  A.ld = ld;
+  return A;
 }
 
 template <typename R>
 int solve_free_surface(s3g_ctx *c, double tz) {
   (void)tz;
   Pcg &P = c->pcg;
-  Dev<R> &D = (sizeof(R) == 8) ? *reinterpret_cast<Dev<R> *>(&c->d64) : *reinterpret_cast<Dev<R> *>(&c->d32);
-  PcgArgs A{};
-  A.M = Mat{D.lE, D.lW, D.lN, D.lS, D.rr, D.sx, D.sy, P.n, c->lm};
-  A.nblk = P.nblk;
-  A.x = P.x;
-  A.r = P.r;
-  A.z = P.z;
-  A.p0 = P.p0;
-  A.p1 = P.p1;
-  A.q = P.q;
-  A.b = D.qq;
-  A.diag = D.rr;
-  A.part = P.part;
-  A.iters = P.iters;
-  const double tol = c->pcg_tol;
-  A.tol2 = (0.5 * tol) * (0.5 * tol);
-  A.nlines = P.nlines;
-  A.line_start = P.line_start;
-  A.lc = P.lc;
-  A.ld = P.ld;
+  Dev<R> &D = *reinterpret_cast<Dev<R> *>(sizeof(R) == 8 ? (void *)&c->d64 : (void *)&c->d32);
   const bool line = (c->prec_kind == S3G_PREC_LINE) && P.nlines > 0;
+  const double tol = c->pcg_tol;
   const dim3 g(P.nblk), b(kBlock);
-  const dim3 gl((P.nlines + kBlock - 1) / kBlock > 0 ? (P.nlines + kBlock - 1) / kBlock : 1);
-  int slot = -1;
-  hipLaunchKernelGGL(k_pcg_start, g, b, 0, c->stream, A, (const double *)D.sp);
-  if (line) hipLaunchKernelGGL(k_line_factor, gl, b, 0, c->stream, A, P.lc, P.ld);
-  int total = 0, rc = 0;
+  const dim3 gl(std::max(1, (P.nlines + kBlock - 1) / kBlock));
+  int rc = 0, total = 0;
   double relres = 0.0;
-  for (int restart = 0; restart < 3; ++restart) {
-    if (line) {
-      hipLaunchKernelGGL(k_pcg_residual<false>, g, b, 0, c->stream, A);
-      hipLaunchKernelGGL(k_zero_part, dim3((P.nblk + 255) / 256), b, 0, c->stream, A.part + P_RZ0 * P.nblk, P.nblk);
-      // line solve writes partials indexed by its own blocks (<= nblk of them)
-      hipLaunchKernelGGL(k_line_solve<P_RZ0>, dim3(std::min((int)gl.x, P.nblk)), b, 0, c->stream, A);
-    } else {
-      hipLaunchKernelGGL(k_pcg_residual<true>, g, b, 0, c->stream, A);
-    }
-    hipMemsetAsync(P.iters, 0, sizeof(int), c->stream);
-    int done = 0, it = 0;
-    int chunk = std::max(8, std::min(c->stats.pcg_iters_last + 4, 256));
-    while (!done && it < c->pcg_maxit) {
-      int nlaunch = std::min(chunk, c->pcg_maxit - it);
-      if (c->timing) timer_begin(c, "pcg_iter", &slot);
-      for (int j = 0; j < nlaunch; ++j) {
-        const int i = it + j;
-        hipLaunchKernelGGL(k_pcg_dir, g, b, 0, c->stream, A, i);
-        if (line) {
-          hipLaunchKernelGGL(k_pcg_update<false>, g, b, 0, c->stream, A, i);
-          const int nxt = (i + 1) & 1;
-          hipLaunchKernelGGL(k_zero_part, dim3((P.nblk + 255) / 256), b, 0, c->stream,
-                             A.part + (P_RZ0 + nxt) * P.nblk, P.nblk);
-          if (nxt)
-            hipLaunchKernelGGL(k_line_solve<P_RZ1>, dim3(std::min((int)gl.x, P.nblk)), b, 0, c->stream, A);
-          else
-            hipLaunchKernelGGL(k_line_solve<P_RZ0>, dim3(std::min((int)gl.x, P.nblk)), b, 0, c->stream, A);
-        } else {
-          hipLaunchKernelGGL(k_pcg_update<true>, g, b, 0, c->stream, A, i);
-        }
+  Outer O{D.lE, D.lW, D.lN, D.lS, D.rr, D.sx, D.sy, D.qq, P.x, P.n, P.nblk, P.part};
+  hipLaunchKernelGGL(k_start_guess, g, b, 0, c->stream, P.x, (const double *)D.sp, P.n);
+
+  if (c->precision == S3G_FP64) {
+    CgArgs<double> A = cg_args<double>(c, D.lE, D.lW, D.lN, D.lS, D.rr, D.sx, D.sy, P.x, P.r, P.z, P.p0, P.p1, P.q,
+                                       D.qq, P.lc, P.ld, 0.5 * tol);
+    if (line) hipLaunchKernelGGL((k_line_factor<double>), gl, b, 0, c->stream, A, P.lc, P.ld);
+    for (int restart = 0; restart < 3; ++restart) {
+      int it = run_cg(c, A, line, c->pcg_maxit - total, c->stats.pcg_iters_last);
+      if (it < 0) return S3G_ERR_HIP;
+      total += it;
+      hipLaunchKernelGGL((k_true_residual<double, false>), g, b, 0, c->stream, O, nullptr, nullptr, nullptr,
+                         nullptr, nullptr);
+      const double tr = host_sum(c, P.part + P_TR * P.nblk, P.nblk);
+      const double bb = host_sum(c, P.part + P_BB * P.nblk, P.nblk);
+      relres = bb > 0 ? std::sqrt(tr / bb) : std::sqrt(tr);
+      if (relres <= tol || !std::isfinite(relres)) break;
+      if (total >= c->pcg_maxit) {
+        rc = S3G_WARN_ITMAX;
+        break;
       }
-      if (c->timing) timer_end(c, slot);
-      it += nlaunch;
-      hipMemcpyAsync(c->h_iters, P.iters, sizeof(int), hipMemcpyDeviceToHost, c->stream);
-      if (hip_check(hipStreamSynchronize(c->stream), "pcg sync") < 0) return S3G_ERR_HIP;
-      done = (c->h_iters[0] < it);  // fewer completed updates than launched: converged
-      chunk = 8;
     }
-    total += c->h_iters[0];
-    hipLaunchKernelGGL(k_pcg_true_residual, g, b, 0, c->stream, A);
-    std::vector<double> h(2 * P.nblk);
-    hipMemcpyAsync(c->h_scal, A.part + P_TR * P.nblk, P.nblk * sizeof(double), hipMemcpyDeviceToHost, c->stream);
-    if (hip_check(hipStreamSynchronize(c->stream), "pcg residual") < 0) return S3G_ERR_HIP;
-    double tr = 0.0;
-    for (int k = 0; k < P.nblk; ++k) tr += c->h_scal[k];
-    hipMemcpyAsync(c->h_scal, A.part + P_BB * P.nblk, P.nblk * sizeof(double), hipMemcpyDeviceToHost, c->stream);
-    if (hip_check(hipStreamSynchronize(c->stream), "pcg bb") < 0) return S3G_ERR_HIP;
-    double bb = 0.0;
-    for (int k = 0; k < P.nblk; ++k) bb += c->h_scal[k];
-    relres = (bb > 0) ? std::sqrt(tr / bb) : std::sqrt(tr);
-    if (relres <= tol || !std::isfinite(relres)) break;
-    if (it >= c->pcg_maxit) {
-      rc = S3G_WARN_ITMAX;
-      break;
+  } else {
+    // iterative refinement with an FP32 inner CG
+    hipLaunchKernelGGL((k_cast_matrix<float>), g, b, 0, c->stream, (const double *)D.rr, (const double *)D.sx,
+                       (const double *)D.sy, P.rrf, P.sxf, P.syf, P.n);
+    CgArgs<float> A = cg_args<float>(c, D.lE, D.lW, D.lN, D.lS, P.rrf, P.sxf, P.syf, P.df, P.rf, P.zf, P.pf0, P.pf1,
+                                     P.qf, P.bf, P.lcf, P.ldf, kInnerTol);
+    if (line) hipLaunchKernelGGL((k_line_factor<float>), gl, b, 0, c->stream, A, P.lcf, P.ldf);
+    for (int sweep = 0; sweep < 12; ++sweep) {
+      hipLaunchKernelGGL((k_true_residual<float, true>), g, b, 0, c->stream, O, P.bf, P.df, P.pf0, P.pf1, P.zf);
+      const double tr = host_sum(c, P.part + P_TR * P.nblk, P.nblk);
+      const double bb = host_sum(c, P.part + P_BB * P.nblk, P.nblk);
+      relres = bb > 0 ? std::sqrt(tr / bb) : std::sqrt(tr);
+      if (relres <= tol || !std::isfinite(relres)) break;
+      if (total >= c->pcg_maxit) {
+        rc = S3G_WARN_ITMAX;
+        break;
+      }
+      int it = run_cg(c, A, line, c->pcg_maxit - total, c->stats.pcg_iters_last / 2);
+      if (it < 0) return S3G_ERR_HIP;
+      total += it;
+      hipLaunchKernelGGL((k_add_correction<float>), g, b, 0, c->stream, P.x, (const float *)P.df, P.n);
     }
   }
   if (!std::isfinite(relres)) {
@@ -380,6 +455,9 @@ int solve_free_surface(s3g_ctx *c, double tz) {
     return S3G_ERR_SOLVER;
   }
   hipLaunchKernelGGL(k_copy_to_s, dim3((P.n + 255) / 256), b, 0, c->stream, D.s, P.x, P.n);
+  if (c->debug)
+    std::fprintf(stderr, "[s3g] solve %ld: %s %s iters=%d relres=%.3e rc=%d\n", c->stats.solves,
+                 c->precision == S3G_FP64 ? "fp64" : "mixed", line ? "line" : "jacobi", total, relres, rc);
   c->stats.pcg_iters_last = total;
   c->stats.pcg_iters_total += total;
   c->stats.pcg_relres_last = relres;
