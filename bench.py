@@ -53,7 +53,7 @@ def parse():
     ap.add_argument("--ny", type=int, default=512)
     ap.add_argument("--layers", type=int, default=60)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
-    ap.add_argument("--precond", default="line", choices=["jacobi", "line"])
+    ap.add_argument("--precond", default="mg", choices=["jacobi", "line", "mg"])
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)

@@ -43,6 +43,7 @@ extern "C" {
 /* Free-surface preconditioners. */
 #define S3G_PREC_JACOBI 0
 #define S3G_PREC_LINE 1 /* exact tridiagonal solve along each N-S line of wet cells */
+#define S3G_PREC_MG 2   /* V(1,1) aggregation multigrid on 2x2 blocks of wet cells */
 
 typedef struct s3g_ctx s3g_ctx;
 

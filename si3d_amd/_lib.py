@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libsi3dgpu.so")
 
 S3G_FP64, S3G_MIXED = 0, 1
-S3G_PREC_JACOBI, S3G_PREC_LINE = 0, 1
+S3G_PREC_JACOBI, S3G_PREC_LINE, S3G_PREC_MG = 0, 1, 2
 
 EXPORTS = [
     "s3g_init", "s3g_set_field_f32", "s3g_set_field_f64", "s3g_set_field_i32",

@@ -134,7 +134,7 @@ static std::vector<Field> field_table(Dev<R> &D) {
       {"vhp", K3R, D.vhp},     {"vhpp", K3R, D.vhpp},   {"v", K3R, D.v},         {"vp", K3R, D.vp},
       {"vpp", K3R, D.vpp},     {"wp", K3R, D.wp},       {"sal", K3R, D.sal},     {"salp", K3R, D.salp},
       {"salpp", K3R, D.salpp}, {"rhop", K3R, D.rhop},   {"Av", K3R, D.Av},       {"Dv", K3R, D.Dv},
-      {"kh", K3R, D.kh},       {"ex", K3R, D.ex},       {"agx", K3R, D.agx},     {"arx", K3R, D.arx},
+      {"kh", K3R, D.kh},       {"ex", K3R, D.ex},       {"ey", K3R, D.ey},       {"agx", K3R, D.agx},     {"arx", K3R, D.arx},
       {"agy", K3R, D.agy},     {"ary", K3R, D.ary},     {"bex", K3R, D.bex},     {"q2", K3D, D.q2},
       {"q2p", K3D, D.q2p},     {"q2pp", K3D, D.q2pp},   {"q2l", K3D, D.q2l},     {"q2lp", K3D, D.q2lp},
       {"q2lpp", K3D, D.q2lpp}, {"s", K2D, D.s},         {"sp", K2D, D.sp},       {"spp", K2D, D.spp},
@@ -272,7 +272,7 @@ static int alloc_state(s3g_ctx *c, Dev<R> &D, const int *lE, const int *lW, cons
     if (!(*p = dalloc<double>(c, n2))) return S3G_ERR_HIP;
   R **d3[] = {&D.h,    &D.hp,   &D.hpp,  &D.hu,  &D.hup,  &D.hupp, &D.hv,  &D.hvp,  &D.hvpp, &D.uh,
               &D.uhp,  &D.uhpp, &D.u,    &D.up,  &D.upp,  &D.vh,   &D.vhp, &D.vhpp, &D.v,    &D.vp,
-              &D.vpp,  &D.wp,   &D.sal,  &D.salp, &D.salpp, &D.rhop, &D.Av, &D.Dv,   &D.kh,   &D.ex,
+              &D.vpp,  &D.wp,   &D.sal,  &D.salp, &D.salpp, &D.rhop, &D.Av, &D.Dv,   &D.kh,   &D.ex, &D.ey,
               &D.agx,  &D.arx,  &D.agy,  &D.ary, &D.fx_ew, &D.fx_ns, &D.bex, &D.t0,  &D.t1};
   for (R **p : d3)
     if (!(*p = dalloc<R>(c, n3))) return S3G_ERR_HIP;
@@ -429,7 +429,7 @@ int s3g_init(const s3g_grid *g, const s3g_params *p, s3g_ctx **out) {
   float **pf[] = {&G.rf, &G.zf, &G.pf0, &G.pf1, &G.qf, &G.sxf, &G.syf, &G.rrf, &G.bf, &G.df, &G.lcf, &G.ldf};
   for (float **q : pf)
     if (rc == 0 && !(*q = dalloc<float>(c, c->precision == S3G_MIXED ? n2 : 1))) rc = S3G_ERR_HIP;
-  if (rc == 0) rc = pcg_setup(c, lN.data());
+  if (rc == 0) rc = pcg_setup(c, lN.data(), g->l2i, g->l2j);
   if (rc == 0 && !(G.part = dalloc<double>(c, 8 * (size_t)G.nblk + 64))) rc = S3G_ERR_HIP;
   if (rc == 0 && !(G.iters = dalloc<int>(c, 4))) rc = S3G_ERR_HIP;
   if (rc != 0) {

@@ -47,9 +47,20 @@ struct Dev {
   double *s, *sp, *spp, *sx, *sy, *dd, *qq, *rr, *eagx, *earx, *eagy, *eary;
   R *h, *hp, *hpp, *hu, *hup, *hupp, *hv, *hvp, *hvpp;
   R *uh, *uhp, *uhpp, *u, *up, *upp, *vh, *vhp, *vhpp, *v, *vp, *vpp, *wp;
-  R *sal, *salp, *salpp, *rhop, *Av, *Dv, *kh, *ex, *agx, *arx, *agy, *ary;
+  R *sal, *salp, *salpp, *rhop, *Av, *Dv, *kh, *ex, *ey, *agx, *arx, *agy, *ary;
   R *fx_ew, *fx_ns, *bex, *t0, *t1;
   double *q2, *q2p, *q2pp, *q2l, *q2lp, *q2lpp, *d0, *d1, *d2;
+};
+
+// One level of the aggregation multigrid preconditioner (levels >= 1; level 0
+// is the lake's own wet-column grid).  Cells of level l+1 are 2x2 blocks of
+// level-l cells in (i,j), numbered i outer / j inner like the fine grid.
+struct MgLevel {
+  int n = 0;                          // cells; index n is the sentinel
+  int *lE = nullptr, *lW = nullptr, *lN = nullptr, *lS = nullptr;
+  int *ch = nullptr;                  // [4n] children on the finer level (finer sentinel if absent)
+  int *parent_of_fine = nullptr;      // [n_finer + 1] this level's cell of each finer cell
+  void *rr = nullptr, *sx = nullptr, *sy = nullptr, *b = nullptr, *x = nullptr;  // T (double or float)
 };
 
 // Free-surface solver work space (FP64).
@@ -65,6 +76,7 @@ struct Pcg {
   int *line_start;  // [nlines+1] first column of each line (lines are contiguous in l)
   double *lc, *ld;  // modified super-diagonal and inverse pivots
   float *lcf, *ldf;
+  std::vector<MgLevel> mg;  // mg[0] unused (the fine grid), mg[1..] coarse levels
 };
 
 struct Timer {
@@ -115,7 +127,7 @@ template <typename R> int save(s3g_ctx *c, int istep);
 int init_density(s3g_ctx *c);
 // pcg.hip
 template <typename R> int solve_free_surface(s3g_ctx *c, double tz);
-int pcg_setup(s3g_ctx *c, const int *lN0);
+int pcg_setup(s3g_ctx *c, const int *lN0, const int *l2i, const int *l2j);
 // host
 void set_error(const std::string &msg);
 int hip_check(hipError_t e, const char *what);

@@ -85,32 +85,37 @@ __device__ __forceinline__ double hcoef(const Dev<R> &D, int k, int w) {
 }
 
 // exmom: explicit advection, Coriolis and horizontal diffusion of the x (X) or
-// y momentum equation, si3d_procedures.f90:578-1173.
+// y momentum equation, si3d_procedures.f90:578-1173.  One lane per (column,
+// layer): the grid's y index is the layer (k = 2 + blockIdx.y).  ex can only
+// ever be non-zero on k1 <= k <= kmq (kmq = bottom of the face, fixed), so
+// layers below kmq are never touched (they stay zero from allocation) and the
+// zeroing of the reference (645-656, 821-827) reduces to k1 <= k < k1q.
 template <typename R, bool X>
 __global__ void __launch_bounds__(kBlock) k_exmom(Dev<R> D, double twodt1) {
   const int c = col();
   if (c >= D.lm) return;
+  const int k = 2 + blockIdx.y;
   const int S = D.lm;
-  const int lE = D.lE[c], lW = D.lW[c], lN = D.lN[c], lS = D.lS[c];
-  const int ln = X ? lE : lN;
+  const int lE = D.lE[c], ln = X ? lE : D.lN[c];
+  if (ln == S) return;  // no face: ex stays zero
   const int kmq = min(D.kmz[ln], D.kmz[c]);
+  if (k > kmq) return;
   const int k1q = X ? D.k1u[c] : D.k1v[c];
-  R *ex = D.ex;
-  if (ln == S) {
-    for (int k = 1; k <= D.km1; ++k) ex[IX(k, c)] = R(0);
+  R *ex = X ? D.ex : D.ey;
+  if (k < k1q) {
+    ex[IX(k, c)] = R(0);
     return;
   }
-  for (int k = 1; k < k1q; ++k) ex[IX(k, c)] = R(0);
-  for (int k = kmq + 1; k <= D.km1; ++k) ex[IX(k, c)] = R(0);
+  const int lW = D.lW[c], lN = D.lN[c], lS = D.lS[c];
   const R *q = X ? D.up : D.vp;
   const R *qpp = X ? D.upp : D.vpp;
   const R *hq = X ? D.hup : D.hvp;
   const int l2 = X ? D.lS[lE] : D.lW[lN];  // SE corner (x) / NW corner (y)
   const double f = D.P.f;
-  for (int k = k1q; k <= kmq; ++k) {
+  {
     if ((double)D.hp[IX(k, c)] <= 0.0 || (double)D.hp[IX(k, ln)] <= 0.0) {
       ex[IX(k, c)] = X ? D.uhpp[IX(k, c)] : D.vhpp[IX(k, c)];
-      continue;
+      return;
     }
     double cor, uE, uW, vN, vS, wU, wD;
     if (X) {
@@ -298,7 +303,7 @@ __global__ void __launch_bounds__(kBlock) k_matmom(Dev<R> D, double tz) {
       vd = D.P.twodt * (dk - dkp) * 2. * (1. - theta);
     }
     const double hh = hk / rhoq;
-    double gg = (double)D.ex[IX(k, c)] - hh * (bcl + ws0) * tz + vd * tz;
+    double gg = (double)(X ? D.ex : D.ey)[IX(k, c)] - hh * (bcl + ws0) * tz + vd * tz;
     double a1 = 0.0, a2 = 1.0, a3 = 0.0;
     if (nwl > 1) {
       a3 = (k < kmq) ? -twodt1 * Avkp / (hkp * (hk + hkp)) * 2. * theta : 0.0;
@@ -377,89 +382,103 @@ __global__ void __launch_bounds__(kBlock) k_matcon(Dev<R> D, double tz) {
 
 // slayer_h: new surface-layer thicknesses after the zeta solve (drying only),
 // si3d_procedures.f90:2070-2175, including the re-test of the same u-layer (2136).
+// One lane per (column, layer k = 1 + blockIdx.y).  Below kmz+1 (the sediment
+// cell) h, hp, hu, hup, hv, hvp are all zero, so only k <= kmz+1 is visited.
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_slayer_h(Dev<R> D) {
   const int c = col();
-  if (c > D.lm) return;
+  if (c >= D.lm) return;
+  const int k = 1 + blockIdx.y;
+  if (k > D.kmz[c] + 1) return;
   const int S = D.lm, km1 = D.km1;
-  if (c == S) {
-    for (int k = 0; k <= km1 + 1; ++k) {
-      D.h[IX(k, c)] = D.hp[IX(k, c)];
-      D.hu[IX(k, c)] = D.hup[IX(k, c)];
-      D.hv[IX(k, c)] = D.hvp[IX(k, c)];
-    }
-    return;
-  }
   const int k1z = D.k1z[c], k1u = D.k1u[c], k1v = D.k1v[c];
-  for (int k = 1; k <= km1; ++k) {
-    if (k < k1z || k >= k1z + 2) D.h[IX(k, c)] = D.hp[IX(k, c)];
-    if (k < k1u || k >= k1u + 1) D.hu[IX(k, c)] = D.hup[IX(k, c)];
-    if (k < k1v || k >= k1v + 2) D.hv[IX(k, c)] = D.hvp[IX(k, c)];
-  }
   const double sc = D.s[c];
-  int k = k1z;
-  double haux = dmin(D.zlevel[k + 1], D.hhs[c]) + sc;
-  if (haux <= 0.0) {
-    D.h[IX(k, c)] = R(0);
-    k = k + 1;
-    haux = dmin(D.zlevel[k + 1], D.hhs[c]) + sc;
-    D.h[IX(k, c)] = (haux <= 0.0) ? R(0) : (R)haux;
-  } else {
-    D.h[IX(k, c)] = (R)haux;
+  // zeta points: drying of the top cell moves the computation one layer down
+  {
+    const double haux0 = dmin(D.zlevel[k1z + 1], D.hhs[c]) + sc;
+    if (k < k1z || k >= k1z + 2) {
+      D.h[IX(k, c)] = D.hp[IX(k, c)];
+    } else if (k == k1z) {
+      D.h[IX(k, c)] = (haux0 <= 0.0) ? R(0) : (R)haux0;
+    } else if (haux0 <= 0.0) {  // k == k1z + 1 after the top cell dried
+      const double haux = dmin(D.zlevel[k + 1], D.hhs[c]) + sc;
+      D.h[IX(k, c)] = (haux <= 0.0) ? R(0) : (R)haux;
+    }
   }
   const int lE = D.lE[c], lN = D.lN[c];
-  if (lE != S) {
-    k = k1u;
-    haux = dmin(D.zlevel[k + 1], D.hhu[c]) + dmax(sc, D.s[lE]);
+  // u points: the reference re-tests the same layer after drying (2136)
+  if (k != k1u) {
+    D.hu[IX(k, c)] = D.hup[IX(k, c)];
+  } else if (lE != S) {
+    const double haux = dmin(D.zlevel[k + 1], D.hhu[c]) + dmax(sc, D.s[lE]);
     D.hu[IX(k, c)] = (haux <= 0.0) ? R(0) : (R)haux;
   } else {
-    for (int kk = k1u; kk <= min(k1u + 1, km1); ++kk) D.hu[IX(kk, c)] = D.hup[IX(kk, c)];
+    D.hu[IX(k, c)] = D.hup[IX(k, c)];
   }
-  if (lN != S) {
-    k = k1v;
-    haux = dmin(D.zlevel[k + 1], D.hhv[c]) + dmax(sc, D.s[lN]);
-    if (haux <= 0.0) {
-      D.hv[IX(k, c)] = R(0);
-      k = k + 1;
-      haux = dmin(D.zlevel[k + 1], D.hhv[c]) + dmax(sc, D.s[lN]);
-      D.hv[IX(k, c)] = (haux <= 0.0) ? R(0) : (R)haux;
-    } else {
-      D.hv[IX(k, c)] = (R)haux;
-    }
+  // v points
+  if (lN == S) {
+    if (k < k1v || k >= k1v + 2 || k <= min(k1v + 1, km1)) D.hv[IX(k, c)] = D.hvp[IX(k, c)];
   } else {
-    for (int kk = k1v; kk <= min(k1v + 1, km1); ++kk) D.hv[IX(kk, c)] = D.hvp[IX(kk, c)];
+    const double sm = dmax(sc, D.s[lN]);
+    const double haux0 = dmin(D.zlevel[k1v + 1], D.hhv[c]) + sm;
+    if (k < k1v || k >= k1v + 2) {
+      D.hv[IX(k, c)] = D.hvp[IX(k, c)];
+    } else if (k == k1v) {
+      D.hv[IX(k, c)] = (haux0 <= 0.0) ? R(0) : (R)haux0;
+    } else if (haux0 <= 0.0) {
+      const double haux = dmin(D.zlevel[k + 1], D.hhv[c]) + sm;
+      D.hv[IX(k, c)] = (haux <= 0.0) ? R(0) : (R)haux;
+    }
   }
+}
+
+// The sentinel column: h := hp, hu := hup, hv := hvp (2087-2089).
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_slayer_h_sentinel(Dev<R> D) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > D.km1 + 1) return;
+  const int c = D.lm;
+  D.h[IX(k, c)] = D.hp[IX(k, c)];
+  D.hu[IX(k, c)] = D.hup[IX(k, c)];
+  D.hv[IX(k, c)] = D.hvp[IX(k, c)];
 }
 
 // vel: fluxes from the new zeta plus the top-cell drying merge,
 // si3d_procedures.f90:2430-2545.
+// One lane per (column, layer k = 2 + blockIdx.y).  The drying merge moves the
+// top flux one layer down: the lane of layer k0+1 adds the top flux, the lane
+// of k0 zeroes it.
+template <typename R, bool X>
+__device__ __forceinline__ void vel_face(const Dev<R> &D, int c, int k, double tz) {
+  const int ln = X ? D.lE[c] : D.lN[c];
+  if (ln == D.lm) return;
+  const int kmq = min(D.kmz[ln], D.kmz[c]);
+  const int k0 = X ? D.k1u[c] : D.k1v[c];
+  if (k < k0 || k > kmq) return;
+  const R *ag = X ? D.agx : D.agy;
+  const R *ar = X ? D.arx : D.ary;
+  R *qh = X ? D.uh : D.vh;
+  const R *hq = X ? D.hu : D.hv;
+  const double cq = (X ? D.P.gdtdx : D.P.gdtdy) * tz * 1000. * (D.s[ln] - D.s[c]);
+  const bool dry_top = (double)hq[IX(k0, c)] <= 0.0;
+  R v = (R)((double)ag[IX(k, c)] - cq * (double)ar[IX(k, c)]);
+  if (dry_top && k == k0) {
+    // a single wet layer: the merged flux lands one layer below the face
+    if (k0 + 1 > kmq) qh[IX(k0 + 1, c)] = v + qh[IX(k0 + 1, c)];
+    qh[IX(k, c)] = R(0);
+    return;  // the top index moves in k_continuity<R,1>, after every lane has read it
+  }
+  if (dry_top && k == k0 + 1) v = (R)((double)ag[IX(k0, c)] - cq * (double)ar[IX(k0, c)]) + v;
+  qh[IX(k, c)] = v;
+}
+
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_vel(Dev<R> D, double tz) {
   const int c = col();
   if (c >= D.lm) return;
-  const int S = D.lm;
-  const int lE = D.lE[c], lN = D.lN[c];
-  const double sc = D.s[c];
-  if (lE != S) {
-    const int kmx = min(D.kmz[lE], D.kmz[c]), k0 = D.k1u[c];
-    const double cxx = D.P.gdtdx * tz * 1000. * (D.s[lE] - sc);
-    for (int k = k0; k <= kmx; ++k) D.uh[IX(k, c)] = (R)((double)D.agx[IX(k, c)] - cxx * (double)D.arx[IX(k, c)]);
-    if (k0 <= kmx && (double)D.hu[IX(k0, c)] <= 0.0) {
-      D.uh[IX(k0 + 1, c)] = D.uh[IX(k0, c)] + D.uh[IX(k0 + 1, c)];
-      D.uh[IX(k0, c)] = R(0);
-      D.k1u[c] = k0 + 1;
-    }
-  }
-  if (lN != S) {
-    const int kmy = min(D.kmz[lN], D.kmz[c]), k0 = D.k1v[c];
-    const double cyy = D.P.gdtdy * tz * 1000. * (D.s[lN] - sc);
-    for (int k = k0; k <= kmy; ++k) D.vh[IX(k, c)] = (R)((double)D.agy[IX(k, c)] - cyy * (double)D.ary[IX(k, c)]);
-    if (k0 <= kmy && (double)D.hv[IX(k0, c)] <= 0.0) {
-      D.vh[IX(k0 + 1, c)] = D.vh[IX(k0, c)] + D.vh[IX(k0 + 1, c)];
-      D.vh[IX(k0, c)] = R(0);
-      D.k1v[c] = k0 + 1;
-    }
-  }
+  const int k = 2 + blockIdx.y;
+  vel_face<R, true>(D, c, k, tz);
+  vel_face<R, false>(D, c, k, tz);
 }
 
 // continuity: vertical velocity by bottom-up integration of the horizontal
@@ -490,6 +509,17 @@ __global__ void __launch_bounds__(kBlock) k_continuity(Dev<R> D) {
     D.wp[IX(k1s, c)] = (R)w;
   }
   if (IST == 1) {
+    // top indices after drying (vel 2490-2497, 2535-2541, 2547-2561)
+    const int S = D.lm;
+    const int lE = D.lE[c], lN = D.lN[c];
+    if (lE != S) {
+      const int k0 = D.k1u[c];
+      if (k0 <= min(D.kmz[lE], kms) && (double)D.hu[IX(k0, c)] <= 0.0) D.k1u[c] = k0 + 1;
+    }
+    if (lN != S) {
+      const int k0 = D.k1v[c];
+      if (k0 <= min(D.kmz[lN], kms) && (double)D.hv[IX(k0, c)] <= 0.0) D.k1v[c] = k0 + 1;
+    }
     const int k0 = D.k1z[c];
     if ((double)D.h[IX(k0, c)] <= 0.0) D.k1z[c] = min(k0 + 1, D.km1);
   }
@@ -514,28 +544,29 @@ template <typename R>
 __global__ void __launch_bounds__(kBlock) k_exsal_hflux(Dev<R> D, double twodt1) {
   const int c = col();
   if (c >= D.lm) return;
+  const int k = 2 + blockIdx.y;
   const int kms = D.kmz[c], k1s = D.k1z[c];
+  if (k < k1s || k > kms) return;
   const int lE = D.lE[c], lW = D.lW[c], lN = D.lN[c], lS = D.lS[c];
-  const int lEE = D.lE[lE], lNN = D.lN[lN];
-  for (int k = k1s; k <= kms; ++k) {
-    double fe = 0.0, fn = 0.0;
-    if ((double)D.hup[IX(k, c)] > 0.0) {
-      const double vel = ((double)D.uhpp[IX(k, c)] + (double)D.uh[IX(k, c)]) / 2.;
-      const double s2 = D.salpp[IX(k, c)], s3 = D.salpp[IX(k, lE)];
-      const double s1 = ((double)D.hpp[IX(k, lW)] <= 0.0) ? s2 : (double)D.salpp[IX(k, lW)];
-      const double s4 = ((double)D.hpp[IX(k, lEE)] <= 0.0) ? s3 : (double)D.salpp[IX(k, lEE)];
-      fe = tvd_flux(vel, s1, s2, s3, s4, 0.0, twodt1 / D.P.dx);
-    }
-    if ((double)D.hvp[IX(k, c)] > 0.0) {
-      const double vel = ((double)D.vhpp[IX(k, c)] + (double)D.vh[IX(k, c)]) / 2.;
-      const double s2 = D.salpp[IX(k, c)], s3 = D.salpp[IX(k, lN)];
-      const double s1 = ((double)D.hpp[IX(k, lS)] <= 0.0) ? s2 : (double)D.salpp[IX(k, lS)];
-      const double s4 = ((double)D.hpp[IX(k, lNN)] <= 0.0) ? s3 : (double)D.salpp[IX(k, lNN)];
-      fn = tvd_flux(vel, s1, s2, s3, s4, 0.0, twodt1 / D.P.dy);
-    }
-    D.fx_ew[IX(k, c)] = (R)fe;
-    D.fx_ns[IX(k, c)] = (R)fn;
+  double fe = 0.0, fn = 0.0;
+  if ((double)D.hup[IX(k, c)] > 0.0) {
+    const int lEE = D.lE[lE];
+    const double vel = ((double)D.uhpp[IX(k, c)] + (double)D.uh[IX(k, c)]) / 2.;
+    const double s2 = D.salpp[IX(k, c)], s3 = D.salpp[IX(k, lE)];
+    const double s1 = ((double)D.hpp[IX(k, lW)] <= 0.0) ? s2 : (double)D.salpp[IX(k, lW)];
+    const double s4 = ((double)D.hpp[IX(k, lEE)] <= 0.0) ? s3 : (double)D.salpp[IX(k, lEE)];
+    fe = tvd_flux(vel, s1, s2, s3, s4, 0.0, twodt1 / D.P.dx);
   }
+  if ((double)D.hvp[IX(k, c)] > 0.0) {
+    const int lNN = D.lN[lN];
+    const double vel = ((double)D.vhpp[IX(k, c)] + (double)D.vh[IX(k, c)]) / 2.;
+    const double s2 = D.salpp[IX(k, c)], s3 = D.salpp[IX(k, lN)];
+    const double s1 = ((double)D.hpp[IX(k, lS)] <= 0.0) ? s2 : (double)D.salpp[IX(k, lS)];
+    const double s4 = ((double)D.hpp[IX(k, lNN)] <= 0.0) ? s3 : (double)D.salpp[IX(k, lNN)];
+    fn = tvd_flux(vel, s1, s2, s3, s4, 0.0, twodt1 / D.P.dy);
+  }
+  D.fx_ew[IX(k, c)] = (R)fe;
+  D.fx_ns[IX(k, c)] = (R)fn;
 }
 
 // Vertical TVD flux through the top of cell k (si3d_procedures.f90:2999-3035).
@@ -636,48 +667,48 @@ __global__ void __launch_bounds__(kBlock) k_exsal_imsal(Dev<R> D, double twodt1)
 }
 
 // vel2: layer_h (si3d_procedures.f90:2181-2231) then the top-cell flux split for
-// wetting (2570-2631).
+// wetting (2570-2631).  One lane per (column, layer k = 2 + blockIdx.y); the
+// lane of the top face layer performs the split for both layers it touches,
+// evaluating layer_h for the layer above itself.
+template <typename R, bool X>
+__device__ __forceinline__ void vel2_face(const Dev<R> &D, int c, int k, double sc) {
+  const int ln = X ? D.lE[c] : D.lN[c];
+  if (ln == D.lm) return;
+  const int kmq = min(D.kmz[c], D.kmz[ln]);
+  const double sm = dmax(sc, D.s[ln]);
+  const double hh = X ? D.hhu[c] : D.hhv[c];
+  auto hlay = [&](int kk) {
+    double v = dmin(D.zlevel[kk + 1], hh) - dmax(D.zlevel[kk], -sm);
+    return (v <= 0.0) ? 0.0 : v;
+  };
+  R *hq = X ? D.hu : D.hv;
+  R *qh = X ? D.uh : D.vh;
+  if (k <= kmq) hq[IX(k, c)] = (R)hlay(k);
+  const int kt = X ? D.k1u[c] : D.k1v[c];
+  if (k != kt) return;
+  // thickness of the layer above the top face after layer_h (kt-1 >= k1 is recomputed,
+  // layer 1 keeps its value)
+  const double hm = (kt - 1 >= D.k1 && kt - 1 <= kmq) ? (double)(R)hlay(kt - 1) : (double)hq[IX(kt - 1, c)];
+  if (hm > 0.0) {
+    const double hk = (kt <= kmq) ? (double)(R)hlay(kt) : (double)hq[IX(kt, c)];
+    const double t = (double)qh[IX(kt, c)] / (hk + hm);
+    qh[IX(kt, c)] = (R)(t * hk);
+    qh[IX(kt - 1, c)] = (R)(t * hm);
+  }
+}
+
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_vel2(Dev<R> D) {
   const int c = col();
   if (c >= D.lm) return;
-  const int S = D.lm;
-  const int lE = D.lE[c], lN = D.lN[c];
+  const int k = 2 + blockIdx.y;
   const double sc = D.s[c];
-  for (int k = D.k1; k <= D.kmz[c]; ++k) {
+  if (k <= D.kmz[c]) {
     double v = dmin(D.zlevel[k + 1], D.hhs[c]) - dmax(D.zlevel[k], -sc);
     D.h[IX(k, c)] = (v <= 0.0) ? R(0) : (R)v;
   }
-  if (lE != S) {
-    const int kmx = min(D.kmz[c], D.kmz[lE]);
-    const double sm = dmax(sc, D.s[lE]);
-    for (int k = D.k1; k <= kmx; ++k) {
-      double v = dmin(D.zlevel[k + 1], D.hhu[c]) - dmax(D.zlevel[k], -sm);
-      D.hu[IX(k, c)] = (v <= 0.0) ? R(0) : (R)v;
-    }
-    const int k = D.k1u[c];
-    const double hm = D.hu[IX(k - 1, c)];
-    if (hm > 0.0) {
-      const double t = (double)D.uh[IX(k, c)] / ((double)D.hu[IX(k, c)] + hm);
-      D.uh[IX(k, c)] = (R)(t * (double)D.hu[IX(k, c)]);
-      D.uh[IX(k - 1, c)] = (R)(t * hm);
-    }
-  }
-  if (lN != S) {
-    const int kmy = min(D.kmz[c], D.kmz[lN]);
-    const double sm = dmax(sc, D.s[lN]);
-    for (int k = D.k1; k <= kmy; ++k) {
-      double v = dmin(D.zlevel[k + 1], D.hhv[c]) - dmax(D.zlevel[k], -sm);
-      D.hv[IX(k, c)] = (v <= 0.0) ? R(0) : (R)v;
-    }
-    const int k = D.k1v[c];
-    const double hm = D.hv[IX(k - 1, c)];
-    if (hm > 0.0) {
-      const double t = (double)D.vh[IX(k, c)] / ((double)D.hv[IX(k, c)] + hm);
-      D.vh[IX(k, c)] = (R)(t * (double)D.hv[IX(k, c)]);
-      D.vh[IX(k - 1, c)] = (R)(t * hm);
-    }
-  }
+  vel2_face<R, true>(D, c, k, sc);
+  vel2_face<R, false>(D, c, k, sc);
 }
 
 // UpdateHorizontalMixingCoefficients (Smagorinsky), si3d_mixing.f90:894-966.
@@ -1146,6 +1177,14 @@ static inline dim3 grid_for(int n) { return dim3((n + kBlock - 1) / kBlock); }
     if (c->timing) timer_end(c, _slot);                                     \
   } while (0)
 
+#define LAUNCH2(name, kern, n, nk, ...)                                     \
+  do {                                                                      \
+    int _slot = -1;                                                         \
+    if (c->timing) timer_begin(c, name, &_slot);                            \
+    hipLaunchKernelGGL(kern, dim3((n + kBlock - 1) / kBlock, nk), dim3(kBlock), 0, c->stream, __VA_ARGS__); \
+    if (c->timing) timer_end(c, _slot);                                     \
+  } while (0)
+
 template <typename R>
 static void sync_forcing(s3g_ctx *c) {
   Dev<R> &D = dev<R>(c);
@@ -1168,24 +1207,25 @@ int phase(s3g_ctx *c, int ph, int n, int istep) {
     case 1:
       if (c->P.ihd > 1) LAUNCH("hmix", (k_hmix<R>), L, D);
       break;
-    case 2: LAUNCH("exmom_x", (k_exmom<R, true>), L, D, twodt1); break;
+    case 2: LAUNCH2("exmom_x", (k_exmom<R, true>), L, c->km - 1, D, twodt1); break;
     case 3: LAUNCH("matmom_x", (k_matmom<R, true>), L, D, tz); break;
-    case 4: LAUNCH("exmom_y", (k_exmom<R, false>), L, D, twodt1); break;
+    case 4: LAUNCH2("exmom_y", (k_exmom<R, false>), L, c->km - 1, D, twodt1); break;
     case 5: LAUNCH("matmom_y", (k_matmom<R, false>), L, D, tz); break;
     case 6: LAUNCH("matcon", (k_matcon<R>), L, D, tz); break;
     case 7: {
       int rc = solve_free_surface<R>(c, tz);
-      LAUNCH("slayer_h", (k_slayer_h<R>), L + 1, D);
+      LAUNCH2("slayer_h", (k_slayer_h<R>), L, c->km1, D);
+      LAUNCH("slayer_h_sentinel", (k_slayer_h_sentinel<R>), c->km1 + 2, D);
       return rc;
     }
     case 8:
-      LAUNCH("vel", (k_vel<R>), L, D, tz);
+      LAUNCH2("vel", (k_vel<R>), L, c->km - 1, D, tz);
       LAUNCH("continuity1", (k_continuity<R, 1>), L, D);
       break;
-    case 9: LAUNCH("exsal_hflux", (k_exsal_hflux<R>), L, D, twodt1);
+    case 9: LAUNCH2("exsal_hflux", (k_exsal_hflux<R>), L, c->km - 1, D, twodt1);
             LAUNCH("exsal_div", (k_exsal_imsal<R, true, false>), L, D, twodt1); break;
     case 10: LAUNCH("imsal", (k_exsal_imsal<R, false, true>), L, D, twodt1); break;
-    case 11: LAUNCH("vel2", (k_vel2<R>), L, D); break;
+    case 11: LAUNCH2("vel2", (k_vel2<R>), L, c->km - 1, D); break;
     case 12:
       LAUNCH("smooth_zeta", (k_zeta_levels<R, 2>), L + 1, D);
       LAUNCH("smooth_col", (k_smooth_col<R>), L, D);
@@ -1208,21 +1248,22 @@ int fd(s3g_ctx *c, int n, int istep, int lastiter) {
   Dev<R> &D = dev<R>(c);
   const int L = c->lm;
   if (c->P.ihd > 1) LAUNCH("hmix", (k_hmix<R>), L, D);
-  LAUNCH("exmom_x", (k_exmom<R, true>), L, D, twodt1);
+  LAUNCH2("exmom_x", (k_exmom<R, true>), L, c->km - 1, D, twodt1);
   LAUNCH("matmom_x", (k_matmom<R, true>), L, D, tz);
-  LAUNCH("exmom_y", (k_exmom<R, false>), L, D, twodt1);
+  LAUNCH2("exmom_y", (k_exmom<R, false>), L, c->km - 1, D, twodt1);
   LAUNCH("matmom_y", (k_matmom<R, false>), L, D, tz);
   LAUNCH("matcon", (k_matcon<R>), L, D, tz);
   int rc = solve_free_surface<R>(c, tz);
   if (rc < 0) return rc;
-  LAUNCH("slayer_h", (k_slayer_h<R>), L + 1, D);
-  LAUNCH("vel", (k_vel<R>), L, D, tz);
+  LAUNCH2("slayer_h", (k_slayer_h<R>), L, c->km1, D);
+  LAUNCH("slayer_h_sentinel", (k_slayer_h_sentinel<R>), c->km1 + 2, D);
+  LAUNCH2("vel", (k_vel<R>), L, c->km - 1, D, tz);
   LAUNCH("continuity1", (k_continuity<R, 1>), L, D);
   if (c->P.isal != 0) {
-    LAUNCH("exsal_hflux", (k_exsal_hflux<R>), L, D, twodt1);
+    LAUNCH2("exsal_hflux", (k_exsal_hflux<R>), L, c->km - 1, D, twodt1);
     LAUNCH("exsal_imsal", (k_exsal_imsal<R, true, true>), L, D, twodt1);
   }
-  LAUNCH("vel2", (k_vel2<R>), L, D);
+  LAUNCH2("vel2", (k_vel2<R>), L, c->km - 1, D);
   if (c->P.ismooth >= 1 && istep == 1) {
     LAUNCH("smooth_zeta", (k_zeta_levels<R, 2>), L + 1, D);
     LAUNCH("smooth_col", (k_smooth_col<R>), L, D);

@@ -25,7 +25,10 @@
 // is a contiguous range of columns; one lane per line runs the Thomas sweep,
 // factors computed once per matrix).
 #include <cmath>
+#include <algorithm>
 #include <cstdio>
+#include <utility>
+#include <vector>
 
 #include "s3g_internal.h"
 
@@ -227,6 +230,135 @@ __global__ void __launch_bounds__(kBlock) k_line_factor(CgArgs<T> A, T *lc, T *l
   }
 }
 
+
+// ---- aggregation multigrid (preconditioner S3G_PREC_MG) --------------------
+// V(1,1) cycle with damped Jacobi smoothing (omega = kMgOmega) on levels of 2x2
+// aggregates; Galerkin coarse operators A_c = P^T A P with piecewise-constant P
+// stay 5-point: diag = sum of the children's diagonals minus twice the
+// couplings inside the aggregate, off-diagonals = sum of the couplings that
+// leave it eastward / northward.  The coarsest level (<= 1024 cells) is
+// smoothed kMgCoarseSweeps times inside one workgroup.  Every step is a fixed
+// linear operator and pre/post smoothing are identical, so the preconditioner
+// is symmetric positive definite and plain CG applies.
+constexpr double kMgOmega = 0.8;
+constexpr int kMgCoarseSweeps = 10;
+constexpr int kMgCoarseMax = 1024;
+
+template <typename T>
+struct MgPair {  // a fine level f and the next coarser level c
+  int nf, nc;
+  const int *fE, *fW, *fN, *fS;
+  const T *frr, *fsx, *fsy, *fb;
+  T *fx;
+  const int *ch, *parent;
+  T *crr, *csx, *csy, *cb;
+  const T *cx;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_mg_galerkin(MgPair<T> M) {
+  for (int C = blockIdx.x * blockDim.x + threadIdx.x; C < M.nc; C += gridDim.x * blockDim.x) {
+    double rr = 0.0, inner = 0.0, sx = 0.0, sy = 0.0;
+    for (int s = 0; s < 4; ++s) {
+      const int f = M.ch[4 * C + s];
+      if (f == M.nf) continue;
+      rr += (double)M.frr[f];
+      const int e = M.fE[f], n = M.fN[f];
+      if (e != M.nf) {
+        if (M.parent[e] == C) inner += (double)M.fsx[f];
+        else sx += (double)M.fsx[f];
+      }
+      if (n != M.nf) {
+        if (M.parent[n] == C) inner += (double)M.fsy[f];
+        else sy += (double)M.fsy[f];
+      }
+    }
+    M.crr[C] = (T)(rr - 2.0 * inner);
+    M.csx[C] = (T)sx;
+    M.csy[C] = (T)sy;
+  }
+}
+
+// Pre-smoothing from a zero guess (x = omega D^-1 b), residual, restriction.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_mg_down(MgPair<T> M) {
+  for (int C = blockIdx.x * blockDim.x + threadIdx.x; C < M.nc; C += gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int s = 0; s < 4; ++s) {
+      const int f = M.ch[4 * C + s];
+      if (f == M.nf) continue;
+      auto xs = [&](int j) { return kMgOmega * (double)M.fb[j] / (double)M.frr[j]; };  // fb(S)=0, frr(S)=1
+      const int w = M.fW[f], so = M.fS[f], n = M.fN[f], e = M.fE[f];
+      const double ax = (double)M.frr[f] * xs(f) - (double)M.fsx[w] * xs(w) - (double)M.fsy[so] * xs(so) -
+                        (double)M.fsy[f] * xs(n) - (double)M.fsx[f] * xs(e);
+      acc += (double)M.fb[f] - ax;
+    }
+    M.cb[C] = (T)acc;
+  }
+}
+
+// Prolongation + post-smoothing: y = omega D^-1 b + P x_c, x = y + omega D^-1 (b - A y).
+// With SLOT >= 0 (finest level) also writes the partial of (b, x) = (r, z).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_mg_up(MgPair<T> M, double *part, int nblk, int slot) {
+  __shared__ double lds[16];
+  double rz = 0.0;
+  for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < M.nf; f += gridDim.x * blockDim.x) {
+    auto ys = [&](int j) {  // j = sentinel -> 0 (fb = 0, parent -> coarse sentinel with cx = 0)
+      return kMgOmega * (double)M.fb[j] / (double)M.frr[j] + (double)M.cx[M.parent[j]];
+    };
+    const int w = M.fW[f], so = M.fS[f], n = M.fN[f], e = M.fE[f];
+    const double yf = ys(f);
+    const double ay = (double)M.frr[f] * yf - (double)M.fsx[w] * ys(w) - (double)M.fsy[so] * ys(so) -
+                      (double)M.fsy[f] * ys(n) - (double)M.fsx[f] * ys(e);
+    const double x = (double)(T)(yf + kMgOmega * ((double)M.fb[f] - ay) / (double)M.frr[f]);
+    M.fx[f] = (T)x;
+    rz += (double)M.fb[f] * x;
+  }
+  if (slot >= 0) {
+    rz = block_sum(rz, lds);
+    if (threadIdx.x == 0) part[slot * nblk + blockIdx.x] = rz;
+  }
+}
+
+// Coarsest level: kMgCoarseSweeps damped Jacobi sweeps from zero, one workgroup.
+template <typename T>
+__global__ void __launch_bounds__(kMgCoarseMax) k_mg_coarsest(int n, const int *lE, const int *lW, const int *lN,
+                                                              const int *lS, const T *rr, const T *sx, const T *sy,
+                                                              const T *b, T *xo) {
+  __shared__ double xs[kMgCoarseMax + 1];
+  const int c = threadIdx.x;
+  const bool act = c < n;
+  if (c == 0) xs[n] = 0.0;
+  double bc = 0, rc = 1, sxc = 0, syc = 0, sxw = 0, sys = 0;
+  int w = n, so = n, no = n, e = n;
+  if (act) {
+    bc = b[c];
+    rc = rr[c];
+    sxc = sx[c];
+    syc = sy[c];
+    w = lW[c];
+    so = lS[c];
+    no = lN[c];
+    e = lE[c];
+    sxw = sx[w];
+    sys = sy[so];
+    xs[c] = kMgOmega * bc / rc;
+  }
+  __syncthreads();
+  for (int it = 1; it < kMgCoarseSweeps; ++it) {
+    double xn = 0.0;
+    if (act) {
+      const double ax = rc * xs[c] - sxw * xs[w] - sys * xs[so] - syc * xs[no] - sxc * xs[e];
+      xn = xs[c] + kMgOmega * (bc - ax) / rc;
+    }
+    __syncthreads();
+    if (act) xs[c] = xn;
+    __syncthreads();
+  }
+  if (act) xo[c] = (T)xs[c];
+}
+
 // ---- FP64 outer level (zeta, qq, true residual) ---------------------------
 struct Outer {
   const int *lE, *lW, *lN, *lS;
@@ -290,7 +422,96 @@ __global__ void k_copy_to_s(double *s, const double *x, int n) {
 
 // ---------------------------------------------------------------------------
 
-int pcg_setup(s3g_ctx *c, const int *lN0) {
+template <typename T>
+static T *mg_alloc(s3g_ctx *c, size_t n, double fill_last) {
+  void *p = nullptr;
+  if (hipMalloc(&p, (n + 1) * sizeof(T)) != hipSuccess) return nullptr;
+  c->allocs.push_back(p);
+  std::vector<T> h(n + 1, T(0));
+  h[n] = (T)fill_last;
+  hipMemcpy(p, h.data(), (n + 1) * sizeof(T), hipMemcpyHostToDevice);
+  return (T *)p;
+}
+
+static int *upload_ints(s3g_ctx *c, const std::vector<int> &v) {
+  void *p = nullptr;
+  if (hipMalloc(&p, std::max<size_t>(v.size(), 1) * sizeof(int)) != hipSuccess) return nullptr;
+  c->allocs.push_back(p);
+  hipMemcpy(p, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice);
+  return (int *)p;
+}
+
+// Hierarchy of 2x2 aggregates built once from the (i,j) position of each wet
+// column; ordering i outer / j inner on every level.
+static int mg_setup(s3g_ctx *c, const int *l2i, const int *l2j, bool fp32) {
+  Pcg &P = c->pcg;
+  P.mg.clear();
+  P.mg.emplace_back();  // level 0 = the lake grid
+  std::vector<int> ci(c->lm), cj(c->lm);
+  for (int k = 0; k < c->lm; ++k) ci[k] = l2i[k] - 2, cj[k] = l2j[k] - 2;
+  int nf = c->lm;
+  while ((nf > kMgCoarseMax || P.mg.size() == 1) && P.mg.size() < 24) {
+    std::vector<std::pair<long, int>> key(nf);
+    for (int f = 0; f < nf; ++f) key[f] = {((long)(ci[f] >> 1) << 32) | (long)(cj[f] >> 1), f};
+    std::sort(key.begin(), key.end());
+    std::vector<int> parent(nf + 1), ni, nj;
+    std::vector<int> ch;
+    int nc = 0;
+    for (int a = 0; a < nf;) {
+      int b = a;
+      while (b < nf && key[b].first == key[a].first) ++b;
+      ni.push_back((int)(key[a].first >> 32));
+      nj.push_back((int)(key[a].first & 0xffffffffL));
+      for (int s2 = 0; s2 < 4; ++s2) ch.push_back(a + s2 < b ? key[a + s2].second : nf);
+      for (int q = a; q < b; ++q) parent[key[q].second] = nc;
+      ++nc;
+      a = b;
+    }
+    parent[nf] = nc;
+    // coarse neighbours through a dense (I,J) table
+    int imax = 0, jmax = 0;
+    for (int C = 0; C < nc; ++C) imax = std::max(imax, ni[C]), jmax = std::max(jmax, nj[C]);
+    std::vector<int> tab((size_t)(imax + 3) * (jmax + 3), nc);
+    auto T_ = [&](int I, int J) -> int & { return tab[(size_t)(I + 1) * (jmax + 3) + (J + 1)]; };
+    for (int C = 0; C < nc; ++C) T_(ni[C], nj[C]) = C;
+    std::vector<int> E(nc + 1, nc), W(nc + 1, nc), N(nc + 1, nc), S(nc + 1, nc);
+    for (int C = 0; C < nc; ++C) {
+      E[C] = T_(ni[C] + 1, nj[C]);
+      W[C] = T_(ni[C] - 1, nj[C]);
+      N[C] = T_(ni[C], nj[C] + 1);
+      S[C] = T_(ni[C], nj[C] - 1);
+    }
+    MgLevel L;
+    L.n = nc;
+    L.lE = upload_ints(c, E);
+    L.lW = upload_ints(c, W);
+    L.lN = upload_ints(c, N);
+    L.lS = upload_ints(c, S);
+    L.ch = upload_ints(c, ch);
+    L.parent_of_fine = upload_ints(c, parent);
+    if (fp32) {
+      L.rr = mg_alloc<float>(c, nc, 1.0);
+      L.sx = mg_alloc<float>(c, nc, 0.0);
+      L.sy = mg_alloc<float>(c, nc, 0.0);
+      L.b = mg_alloc<float>(c, nc, 0.0);
+      L.x = mg_alloc<float>(c, nc, 0.0);
+    } else {
+      L.rr = mg_alloc<double>(c, nc, 1.0);
+      L.sx = mg_alloc<double>(c, nc, 0.0);
+      L.sy = mg_alloc<double>(c, nc, 0.0);
+      L.b = mg_alloc<double>(c, nc, 0.0);
+      L.x = mg_alloc<double>(c, nc, 0.0);
+    }
+    if (!L.lE || !L.ch || !L.parent_of_fine || !L.rr || !L.x) return S3G_ERR_HIP;
+    P.mg.push_back(L);
+    ci = ni;
+    cj = nj;
+    nf = nc;
+  }
+  return hip_check(hipGetLastError(), "mg setup");
+}
+
+int pcg_setup(s3g_ctx *c, const int *lN0, const int *l2i, const int *l2j) {
   // North-south lines: maximal runs of consecutive columns c-1 -> c with
   // lN(c-1) = c (lN0 is 0-based, sentinel = lm).
   Pcg &P = c->pcg;
@@ -301,12 +522,10 @@ int pcg_setup(s3g_ctx *c, const int *lN0) {
     if (cc == 0 || lN0[cc - 1] != cc) starts.push_back(cc);
   starts.push_back(c->lm);
   P.nlines = (int)starts.size() - 1;
-  void *p = nullptr;
-  if (hip_check(hipMalloc(&p, starts.size() * sizeof(int)), "alloc lines") < 0) return S3G_ERR_HIP;
-  c->allocs.push_back(p);
-  P.line_start = (int *)p;
-  return hip_check(hipMemcpy(P.line_start, starts.data(), starts.size() * sizeof(int), hipMemcpyHostToDevice),
-                   "upload lines");
+  P.line_start = upload_ints(c, starts);
+  if (!P.line_start) return S3G_ERR_HIP;
+  if (c->prec_kind == S3G_PREC_MG) return mg_setup(c, l2i, l2j, c->precision == S3G_MIXED);
+  return 0;
 }
 
 static double host_sum(s3g_ctx *c, const double *dev_part, int n) {
@@ -317,37 +536,103 @@ static double host_sum(s3g_ctx *c, const double *dev_part, int n) {
   return s;
 }
 
+// z = M^-1 r for the line and multigrid preconditioners, with the partial of
+// (r, z) written into `slot` (Jacobi is fused into the CG kernels instead).
+template <typename T>
+static void apply_prec(s3g_ctx *c, CgArgs<T> &A, int slot) {
+  const dim3 b(kBlock);
+  if (c->prec_kind == S3G_PREC_LINE) {
+    const int nlb = std::min(A.nblk, std::max(1, (A.nlines + kBlock - 1) / kBlock));
+    // the line solve writes partials only for its own (possibly fewer) blocks
+    hipMemsetAsync(A.part + slot * A.nblk, 0, A.nblk * sizeof(double), c->stream);
+    hipLaunchKernelGGL((k_line_solve<T>), dim3(nlb), b, 0, c->stream, A, slot);
+    return;
+  }
+  // multigrid V-cycle: level 0 is the CG system itself
+  std::vector<MgLevel> &Lv = c->pcg.mg;
+  const int nl = (int)Lv.size();
+  auto pair = [&](int l) {  // level l (fine) and l+1 (coarse)
+    MgPair<T> M{};
+    if (l == 0) {
+      M.nf = A.n;
+      M.fE = A.lE; M.fW = A.lW; M.fN = A.lN; M.fS = A.lS;
+      M.frr = A.rr; M.fsx = A.sx; M.fsy = A.sy; M.fb = A.r; M.fx = A.z;
+    } else {
+      M.nf = Lv[l].n;
+      M.fE = Lv[l].lE; M.fW = Lv[l].lW; M.fN = Lv[l].lN; M.fS = Lv[l].lS;
+      M.frr = (const T *)Lv[l].rr; M.fsx = (const T *)Lv[l].sx; M.fsy = (const T *)Lv[l].sy;
+      M.fb = (const T *)Lv[l].b; M.fx = (T *)Lv[l].x;
+    }
+    M.nc = Lv[l + 1].n;
+    M.ch = Lv[l + 1].ch;
+    M.parent = Lv[l + 1].parent_of_fine;
+    M.crr = (T *)Lv[l + 1].rr; M.csx = (T *)Lv[l + 1].sx; M.csy = (T *)Lv[l + 1].sy;
+    M.cb = (T *)Lv[l + 1].b; M.cx = (const T *)Lv[l + 1].x;
+    return M;
+  };
+  auto grid = [&](int n) { return dim3(std::max(1, std::min((n + kBlock - 1) / kBlock, 2048))); };
+  for (int l = 0; l + 1 < nl; ++l) {
+    MgPair<T> M = pair(l);
+    hipLaunchKernelGGL((k_mg_down<T>), grid(M.nc), b, 0, c->stream, M);
+  }
+  const MgLevel &Lc = Lv[nl - 1];
+  hipLaunchKernelGGL((k_mg_coarsest<T>), dim3(1), dim3(kMgCoarseMax), 0, c->stream, Lc.n, Lc.lE, Lc.lW, Lc.lN, Lc.lS,
+                     (const T *)Lc.rr, (const T *)Lc.sx, (const T *)Lc.sy, (const T *)Lc.b, (T *)Lc.x);
+  for (int l = nl - 2; l >= 0; --l) {
+    MgPair<T> M = pair(l);
+    if (l == 0)
+      hipLaunchKernelGGL((k_mg_up<T>), dim3(A.nblk), b, 0, c->stream, M, A.part, A.nblk, slot);
+    else
+      hipLaunchKernelGGL((k_mg_up<T>), grid(M.nf), b, 0, c->stream, M, A.part, A.nblk, -1);
+  }
+}
+
+// Galerkin coarse operators for the current matrix (once per solve).
+template <typename T>
+static void mg_build(s3g_ctx *c, CgArgs<T> &A) {
+  std::vector<MgLevel> &Lv = c->pcg.mg;
+  for (int l = 0; l + 1 < (int)Lv.size(); ++l) {
+    MgPair<T> M{};
+    if (l == 0) {
+      M.nf = A.n; M.fE = A.lE; M.fW = A.lW; M.fN = A.lN; M.fS = A.lS;
+      M.frr = A.rr; M.fsx = A.sx; M.fsy = A.sy;
+    } else {
+      M.nf = Lv[l].n; M.fE = Lv[l].lE; M.fW = Lv[l].lW; M.fN = Lv[l].lN; M.fS = Lv[l].lS;
+      M.frr = (const T *)Lv[l].rr; M.fsx = (const T *)Lv[l].sx; M.fsy = (const T *)Lv[l].sy;
+    }
+    M.nc = Lv[l + 1].n;
+    M.ch = Lv[l + 1].ch;
+    M.parent = Lv[l + 1].parent_of_fine;
+    M.crr = (T *)Lv[l + 1].rr; M.csx = (T *)Lv[l + 1].sx; M.csy = (T *)Lv[l + 1].sy;
+    hipLaunchKernelGGL((k_mg_galerkin<T>), dim3(std::max(1, std::min((M.nc + kBlock - 1) / kBlock, 2048))),
+                       dim3(kBlock), 0, c->stream, M);
+  }
+}
+
 // Runs CG iterations on CgArgs A (x, r, b set up) until (r,r) <= tol2 (b,b) or
 // maxit; returns the number of completed iterations (< 0 on HIP error).
 template <typename T>
-static int run_cg(s3g_ctx *c, CgArgs<T> &A, bool line, int maxit, int hint) {
+static int run_cg(s3g_ctx *c, CgArgs<T> &A, int maxit, int hint) {
   const dim3 g(A.nblk), b(kBlock);
-  const int nlb = std::min(A.nblk, std::max(1, (A.nlines + kBlock - 1) / kBlock));
-  auto zero_rz = [&](int slot) {
-    // the line solve writes partials only for its own (possibly fewer) blocks
-    hipMemsetAsync(A.part + slot * A.nblk, 0, A.nblk * sizeof(double), c->stream);
-  };
-  if (line) {
+  const bool jac = (c->prec_kind == S3G_PREC_JACOBI) || (c->prec_kind == S3G_PREC_LINE && A.nlines == 0);
+  if (!jac) {
     hipLaunchKernelGGL((k_cg_residual<T, false>), g, b, 0, c->stream, A);
-    zero_rz(P_RZ0);
-    hipLaunchKernelGGL((k_line_solve<T>), dim3(nlb), b, 0, c->stream, A, (int)P_RZ0);
+    apply_prec(c, A, P_RZ0);
   } else {
     hipLaunchKernelGGL((k_cg_residual<T, true>), g, b, 0, c->stream, A);
   }
   hipMemsetAsync(A.iters, 0, sizeof(int), c->stream);
   int it = 0, slot = -1;
-  int chunk = std::max(8, std::min(hint + 4, 512));
+  int chunk = std::max(4, std::min(hint + 2, 512));
   while (it < maxit) {
     const int nl = std::min(chunk, maxit - it);
     if (c->timing) timer_begin(c, sizeof(T) == 8 ? "pcg_chunk_f64" : "pcg_chunk_f32", &slot);
     for (int j = 0; j < nl; ++j) {
       const int i = it + j;
       hipLaunchKernelGGL((k_cg_dir<T>), g, b, 0, c->stream, A, i);
-      if (line) {
+      if (!jac) {
         hipLaunchKernelGGL((k_cg_update<T, false>), g, b, 0, c->stream, A, i);
-        const int nxt = P_RZ0 + ((i + 1) & 1);
-        zero_rz(nxt);
-        hipLaunchKernelGGL((k_line_solve<T>), dim3(nlb), b, 0, c->stream, A, nxt);
+        apply_prec(c, A, P_RZ0 + ((i + 1) & 1));
       } else {
         hipLaunchKernelGGL((k_cg_update<T, true>), g, b, 0, c->stream, A, i);
       }
@@ -357,7 +642,7 @@ static int run_cg(s3g_ctx *c, CgArgs<T> &A, bool line, int maxit, int hint) {
     hipMemcpyAsync(c->h_iters, A.iters, sizeof(int), hipMemcpyDeviceToHost, c->stream);
     if (hip_check(hipStreamSynchronize(c->stream), "pcg sync") < 0) return -1;
     if (c->h_iters[0] < it) break;  // fewer updates than launched: converged
-    chunk = 8;
+    chunk = 4;
   }
   return c->h_iters[0];
 }
@@ -406,14 +691,18 @@ int solve_free_surface(s3g_ctx *c, double tz) {
   int rc = 0, total = 0;
   double relres = 0.0;
   Outer O{D.lE, D.lW, D.lN, D.lS, D.rr, D.sx, D.sy, D.qq, P.x, P.n, P.nblk, P.part};
-  hipLaunchKernelGGL(k_start_guess, g, b, 0, c->stream, P.x, (const double *)D.sp, P.n);
+  // Initial guess: the latest free surface s.  The reference starts from sp
+  // (1970); s is the same field on leapfrog steps and the previous estimate of
+  // zeta^{n+1} on trapezoidal iterations, so it only shortens the iteration.
+  hipLaunchKernelGGL(k_start_guess, g, b, 0, c->stream, P.x, (const double *)D.s, P.n);
 
   if (c->precision == S3G_FP64) {
     CgArgs<double> A = cg_args<double>(c, D.lE, D.lW, D.lN, D.lS, D.rr, D.sx, D.sy, P.x, P.r, P.z, P.p0, P.p1, P.q,
                                        D.qq, P.lc, P.ld, 0.5 * tol);
     if (line) hipLaunchKernelGGL((k_line_factor<double>), gl, b, 0, c->stream, A, P.lc, P.ld);
+    if (c->prec_kind == S3G_PREC_MG) mg_build(c, A);
     for (int restart = 0; restart < 3; ++restart) {
-      int it = run_cg(c, A, line, c->pcg_maxit - total, c->stats.pcg_iters_last);
+      int it = run_cg(c, A, c->pcg_maxit - total, c->stats.pcg_iters_last);
       if (it < 0) return S3G_ERR_HIP;
       total += it;
       hipLaunchKernelGGL((k_true_residual<double, false>), g, b, 0, c->stream, O, nullptr, nullptr, nullptr,
@@ -434,6 +723,7 @@ int solve_free_surface(s3g_ctx *c, double tz) {
     CgArgs<float> A = cg_args<float>(c, D.lE, D.lW, D.lN, D.lS, P.rrf, P.sxf, P.syf, P.df, P.rf, P.zf, P.pf0, P.pf1,
                                      P.qf, P.bf, P.lcf, P.ldf, kInnerTol);
     if (line) hipLaunchKernelGGL((k_line_factor<float>), gl, b, 0, c->stream, A, P.lcf, P.ldf);
+    if (c->prec_kind == S3G_PREC_MG) mg_build(c, A);
     for (int sweep = 0; sweep < 12; ++sweep) {
       hipLaunchKernelGGL((k_true_residual<float, true>), g, b, 0, c->stream, O, P.bf, P.df, P.pf0, P.pf1, P.zf);
       const double tr = host_sum(c, P.part + P_TR * P.nblk, P.nblk);
@@ -444,7 +734,7 @@ int solve_free_surface(s3g_ctx *c, double tz) {
         rc = S3G_WARN_ITMAX;
         break;
       }
-      int it = run_cg(c, A, line, c->pcg_maxit - total, c->stats.pcg_iters_last / 2);
+      int it = run_cg(c, A, c->pcg_maxit - total, c->stats.pcg_iters_last / 2);
       if (it < 0) return S3G_ERR_HIP;
       total += it;
       hipLaunchKernelGGL((k_add_correction<float>), g, b, 0, c->stream, P.x, (const float *)P.df, P.n);
@@ -457,7 +747,8 @@ int solve_free_surface(s3g_ctx *c, double tz) {
   hipLaunchKernelGGL(k_copy_to_s, dim3((P.n + 255) / 256), b, 0, c->stream, D.s, P.x, P.n);
   if (c->debug)
     std::fprintf(stderr, "[s3g] solve %ld: %s %s iters=%d relres=%.3e rc=%d\n", c->stats.solves,
-                 c->precision == S3G_FP64 ? "fp64" : "mixed", line ? "line" : "jacobi", total, relres, rc);
+                 c->precision == S3G_FP64 ? "fp64" : "mixed",
+                 c->prec_kind == S3G_PREC_MG ? "mg" : (line ? "line" : "jacobi"), total, relres, rc);
   c->stats.pcg_iters_last = total;
   c->stats.pcg_iters_total += total;
   c->stats.pcg_relres_last = relres;

@@ -18,7 +18,7 @@ MODULE si3d_gpu
    IMPLICIT NONE
 
    INTEGER(C_INT), PARAMETER :: S3G_FP64 = 0, S3G_MIXED = 1
-   INTEGER(C_INT), PARAMETER :: S3G_PREC_JACOBI = 0, S3G_PREC_LINE = 1
+   INTEGER(C_INT), PARAMETER :: S3G_PREC_JACOBI = 0, S3G_PREC_LINE = 1, S3G_PREC_MG = 2
 
    TYPE, BIND(C) :: s3g_grid
       INTEGER(C_INT) :: im1, jm1, km1, lm

@@ -20,7 +20,7 @@ from . import _lib
 from ._lib import S3gDeck, S3gGrid, S3gParams, S3gStats, check
 
 _PREC = {"fp64": _lib.S3G_FP64, "mixed": _lib.S3G_MIXED}
-_PRECOND = {"jacobi": _lib.S3G_PREC_JACOBI, "line": _lib.S3G_PREC_LINE}
+_PRECOND = {"jacobi": _lib.S3G_PREC_JACOBI, "line": _lib.S3G_PREC_LINE, "mg": _lib.S3G_PREC_MG}
 INT_FIELDS = ("kmz", "k1z", "k1u", "k1v")
 TWO_D = ("s", "sp", "spp", "sx", "sy", "dd", "qq", "rr", "eagx", "earx", "eagy", "eary",
          "hhs", "hhu", "hhv") + INT_FIELDS

@@ -92,3 +92,36 @@ def test_line_preconditioner_same_solution(oracle_lib):
         o.step(n)
         g.step(n)
     compare(g, o, FIELDS_2D + ["uh", "vh", "sal"], FIELD_TOL, "line preconditioner")
+
+
+@pytest.mark.parametrize("nx,ny", [(40, 24), (160, 96)])
+def test_multigrid_preconditioner_same_solution(oracle_lib, nx, ny):
+    """The multigrid-preconditioned CG reaches the same free surface and flow
+    (one coarse level on the small lake, a four-level hierarchy on the larger)."""
+    deck = small_ellipse(nx=nx, ny=ny)
+    o, g = _pair(deck, oracle_lib, precond="mg")
+    for n in range(1, 4):
+        o.step(n)
+        g.step(n)
+    compare(g, o, FIELDS_2D + ["uh", "vh", "wp", "sal", "q2"], FIELD_TOL, f"multigrid {nx}x{ny}")
+    assert g.stats()["pcg_relres_last"] <= TOL_SOLVE
+
+
+@pytest.mark.parametrize("precond", ["jacobi", "mg"])
+def test_mixed_precision_error_vs_fp64_oracle(oracle_lib, precond):
+    """Mixed mode (FP32 state, stencils, column solves and inner SpMV; FP64
+    zeta, dots, true residual and refinement) stays close to the FP64 oracle."""
+    from si3d_amd.model import GpuLake
+    deck = small_ellipse()
+    deck["pcg_tol"] = 1e-12
+    o = oracle_lib.Oracle(deck)
+    g = GpuLake(deck, precision="mixed", preconditioner=precond, tol=1e-12)
+    for n in range(1, 6):
+        o.step(n)
+        g.step(n)
+    errs = {n: relerr(g.get(n), o.get(n)) for n in ["s", "uh", "vh", "wp", "sal", "q2"]}
+    print("mixed-precision relative errors:", errs, flush=True)
+    for name in INTS:
+        np.testing.assert_array_equal(g.get(name), o.get(name).astype(np.int32), err_msg=name)
+    assert errs["s"] < 1e-4 and errs["sal"] < 1e-5 and errs["uh"] < 1e-3, errs
+    assert g.stats()["pcg_relres_last"] <= 1e-12
