@@ -423,10 +423,10 @@ int s3g_init(const s3g_grid *g, const s3g_params *p, s3g_ctx **out) {
                              k1u.data(), k1v.data(), zl.data(), hhs.data(), hhu.data(), hhv.data());
   Pcg &G = c->pcg;
   const size_t n2 = LP;
-  double **pv[] = {&G.r, &G.z, &G.p0, &G.p1, &G.q, &G.x, &G.lc, &G.ld};
+  double **pv[] = {&G.r, &G.r1, &G.z, &G.p0, &G.p1, &G.q, &G.x, &G.lc, &G.ld};
   for (double **q : pv)
     if (rc == 0 && !(*q = dalloc<double>(c, n2))) rc = S3G_ERR_HIP;
-  float **pf[] = {&G.rf, &G.zf, &G.pf0, &G.pf1, &G.qf, &G.sxf, &G.syf, &G.rrf, &G.bf, &G.df, &G.lcf, &G.ldf};
+  float **pf[] = {&G.rf, &G.rf1, &G.zf, &G.pf0, &G.pf1, &G.qf, &G.sxf, &G.syf, &G.rrf, &G.bf, &G.df, &G.lcf, &G.ldf};
   for (float **q : pf)
     if (rc == 0 && !(*q = dalloc<float>(c, c->precision == S3G_MIXED ? n2 : 1))) rc = S3G_ERR_HIP;
   if (rc == 0) rc = pcg_setup(c, lN.data(), g->l2i, g->l2j);

@@ -66,11 +66,11 @@ struct MgLevel {
 // Free-surface solver work space (FP64).
 struct Pcg {
   int n, nblk;
-  double *r, *z, *p0, *p1, *q, *diag, *x;
+  double *r, *r1, *z, *p0, *p1, *q, *diag, *x;  // r / r1: residual of even / odd iterations
   double *part;    // partial sums: [4][nblk]
   double *scal;    // device scalars
   int *iters;      // device iteration counter
-  float *rf, *zf, *pf0, *pf1, *qf, *sxf, *syf, *rrf, *bf, *df;  // FP32 inner solve
+  float *rf, *rf1, *zf, *pf0, *pf1, *qf, *sxf, *syf, *rrf, *bf, *df;  // FP32 inner solve
   // line preconditioner: tridiagonal factors along N-S lines
   int nlines;
   int *line_start;  // [nlines+1] first column of each line (lines are contiguous in l)

@@ -54,16 +54,31 @@ __device__ double block_sum(double v, double *lds) {
   return s;
 }
 
-// Deterministic reduction of nblk partials; every block gets the same value.
-__device__ double reduce_partials(const double *part, int nblk, double *lds) {
-  double s = 0.0;
-  for (int i = threadIdx.x; i < nblk; i += blockDim.x) s += part[i];
-  s = block_sum(s, lds);
-  if (threadIdx.x == 0) lds[8] = s;
+// N reductions of nblk partials at once (slots[q] * nblk + i); every lane of
+// every block gets the same N values (fixed summation order).
+template <int N>
+__device__ __forceinline__ void reduce_slots(const double *part, const int (&slots)[N], int nblk, double *lds,
+                                             double (&out)[N]) {
+  double s[N];
+#pragma unroll
+  for (int q = 0; q < N; ++q) s[q] = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x)
+#pragma unroll
+    for (int q = 0; q < N; ++q) s[q] += part[slots[q] * nblk + i];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int q = 0; q < N; ++q) {
+    s[q] = wave_sum(s[q]);
+    if (lane == 0) lds[w * N + q] = s[q];
+  }
   __syncthreads();
-  double r = lds[8];
+#pragma unroll
+  for (int q = 0; q < N; ++q) {
+    double t = 0.0;
+    for (int v = 0; v < nw; ++v) t += lds[v * N + q];
+    out[q] = t;
+  }
   __syncthreads();
-  return r;
 }
 
 // Partial sums layout: [RZ0 | RZ1 | RR0 | RR1 | PQ | BB | TR] x nblk
@@ -74,7 +89,7 @@ struct CgArgs {
   const int *lE, *lW, *lN, *lS;
   const T *rr, *sx, *sy;  // matrix (sx(S) = sy(S) = 0)
   int n, nblk;
-  T *x, *r, *z, *p0, *p1, *q;
+  T *x, *r, *r1, *z, *p0, *p1, *q;  // r / r1: residual entering even / odd iterations
   const T *b;
   double *part;
   int *iters;
@@ -115,13 +130,15 @@ __global__ void __launch_bounds__(kBlock) k_cg_residual(CgArgs<T> A) {
     if (JACOBI) A.part[P_RZ0 * A.nblk + blockIdx.x] = rz;
     A.part[P_RR0 * A.nblk + blockIdx.x] = rr;
     A.part[P_BB * A.nblk + blockIdx.x] = bb;
+    if (blockIdx.x == 0) A.iters[0] = 0;
   }
 }
 
 // Line preconditioner z = T^-1 r (tridiagonal part of A along each N-S line),
-// partial (r,z) into SLOT.  Blocks beyond the line count write zero partials.
+// partial (r,z) into SLOT.  Launched with nblk blocks: blocks beyond the line
+// count write zero partials.
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_line_solve(CgArgs<T> A, int slot) {
+__global__ void __launch_bounds__(kBlock) k_line_solve(CgArgs<T> A, const T *r, int slot) {
   __shared__ double lds[16];
   double rz = 0.0;
   for (int ln = blockIdx.x * blockDim.x + threadIdx.x; ln < A.nlines; ln += gridDim.x * blockDim.x) {
@@ -129,7 +146,7 @@ __global__ void __launch_bounds__(kBlock) k_line_solve(CgArgs<T> A, int slot) {
     double y = 0.0;
     for (int c = c0; c < c1; ++c) {
       const double off = (c > c0) ? (double)A.sy[c - 1] : 0.0;
-      y = ((double)A.r[c] + off * y) * (double)A.ld[c];
+      y = ((double)r[c] + off * y) * (double)A.ld[c];
       A.z[c] = (T)y;
     }
     double zn = 0.0;
@@ -137,7 +154,7 @@ __global__ void __launch_bounds__(kBlock) k_line_solve(CgArgs<T> A, int slot) {
       const double zc = (c < c1 - 1) ? (double)A.z[c] - (double)A.lc[c] * zn : (double)A.z[c];
       A.z[c] = (T)zc;
       zn = zc;
-      rz += (double)A.r[c] * (double)(T)zc;
+      rz += (double)r[c] * (double)(T)zc;
     }
   }
   rz = block_sum(rz, lds);
@@ -148,14 +165,12 @@ __global__ void __launch_bounds__(kBlock) k_line_solve(CgArgs<T> A, int slot) {
 // neighbours (double-buffered p), q = A p_new, partial (p,q).
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_cg_dir(CgArgs<T> A, int i) {
-  __shared__ double lds[16];
+  __shared__ double lds[32];
   const int cur = i & 1;
-  const double rr = reduce_partials(A.part + (P_RR0 + cur) * A.nblk, A.nblk, lds);
-  const double bb = reduce_partials(A.part + P_BB * A.nblk, A.nblk, lds);
-  if (rr <= A.tol2 * bb) return;  // converged: every block takes the same branch
-  const double rz = reduce_partials(A.part + (P_RZ0 + cur) * A.nblk, A.nblk, lds);
-  double beta = 0.0;
-  if (i > 0) beta = rz / reduce_partials(A.part + (P_RZ0 + (cur ^ 1)) * A.nblk, A.nblk, lds);
+  double v[4];
+  reduce_slots<4>(A.part, {P_RR0 + cur, P_BB, P_RZ0 + cur, P_RZ0 + (cur ^ 1)}, A.nblk, lds, v);
+  if (v[0] <= A.tol2 * v[1]) return;  // converged: every block takes the same branch
+  const double beta = (i > 0) ? v[2] / v[3] : 0.0;
   const T *pold = cur ? A.p0 : A.p1;
   T *pnew = cur ? A.p1 : A.p0;
   double pq = 0.0;
@@ -177,25 +192,25 @@ __global__ void __launch_bounds__(kBlock) k_cg_dir(CgArgs<T> A, int i) {
 // (r,z) and (r,r) for iteration i+1.
 template <typename T, bool JACOBI>
 __global__ void __launch_bounds__(kBlock) k_cg_update(CgArgs<T> A, int i) {
-  __shared__ double lds[16];
+  __shared__ double lds[32];
   const int cur = i & 1, nxt = cur ^ 1;
-  const double rr = reduce_partials(A.part + (P_RR0 + cur) * A.nblk, A.nblk, lds);
-  const double bb = reduce_partials(A.part + P_BB * A.nblk, A.nblk, lds);
-  if (rr <= A.tol2 * bb) {
+  double v[4];
+  reduce_slots<4>(A.part, {P_RR0 + cur, P_BB, P_RZ0 + cur, P_PQ}, A.nblk, lds, v);
+  if (v[0] <= A.tol2 * v[1]) {
     // Converged: carry this residual partial into the slot iteration i+1 reads,
     // so every later kernel of the chunk also sees convergence and returns.
     if (threadIdx.x == 0) A.part[(P_RR0 + nxt) * A.nblk + blockIdx.x] = A.part[(P_RR0 + cur) * A.nblk + blockIdx.x];
     return;
   }
-  const double rz = reduce_partials(A.part + (P_RZ0 + cur) * A.nblk, A.nblk, lds);
-  const double pq = reduce_partials(A.part + P_PQ * A.nblk, A.nblk, lds);
-  const double alpha = rz / pq;
+  const double alpha = v[2] / v[3];
   const T *p = cur ? A.p1 : A.p0;
+  const T *ro = cur ? A.r1 : A.r;
+  T *rn = cur ? A.r : A.r1;
   double rzn = 0.0, rrn = 0.0;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < A.n; c += gridDim.x * blockDim.x) {
     A.x[c] = (T)((double)A.x[c] + alpha * (double)p[c]);
-    const double r = (double)(T)((double)A.r[c] - alpha * (double)A.q[c]);
-    A.r[c] = (T)r;
+    const double r = (double)(T)((double)ro[c] - alpha * (double)A.q[c]);
+    rn[c] = (T)r;
     if (JACOBI) {
       const double z = (double)(T)(r / (double)A.rr[c]);
       A.z[c] = (T)z;
@@ -321,6 +336,51 @@ __global__ void __launch_bounds__(kBlock) k_mg_up(MgPair<T> M, double *part, int
   }
 }
 
+// CG iteration i, step 2, fused with the first restriction of the V-cycle:
+// alpha, x += alpha p and r_new = r - alpha q for the children of each level-1
+// aggregate, (r_new, r_new) partial, and the restricted residual of the
+// pre-smoothed level 0 (as k_mg_down, with r_new at the neighbours recomputed
+// from the previous residual buffer).  Launched with nblk blocks.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_cg_update_mg(CgArgs<T> A, MgPair<T> M, int i) {
+  __shared__ double lds[32];
+  const int cur = i & 1, nxt = cur ^ 1;
+  double v[4];
+  reduce_slots<4>(A.part, {P_RR0 + cur, P_BB, P_RZ0 + cur, P_PQ}, A.nblk, lds, v);
+  if (v[0] <= A.tol2 * v[1]) {
+    if (threadIdx.x == 0) A.part[(P_RR0 + nxt) * A.nblk + blockIdx.x] = A.part[(P_RR0 + cur) * A.nblk + blockIdx.x];
+    return;
+  }
+  const double alpha = v[2] / v[3];
+  const T *p = cur ? A.p1 : A.p0;
+  const T *ro = cur ? A.r1 : A.r;
+  T *rn = cur ? A.r : A.r1;
+  auto rnew = [&](int j) { return (double)(T)((double)ro[j] - alpha * (double)A.q[j]); };  // r(S) = q(S) = 0
+  double rrn = 0.0;
+  for (int C = blockIdx.x * blockDim.x + threadIdx.x; C < M.nc; C += gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int s2 = 0; s2 < 4; ++s2) {
+      const int f = M.ch[4 * C + s2];
+      if (f == M.nf) continue;
+      A.x[f] = (T)((double)A.x[f] + alpha * (double)p[f]);
+      const double rf = rnew(f);
+      rn[f] = (T)rf;
+      rrn += rf * rf;
+      auto xs = [&](int j) { return kMgOmega * rnew(j) / (double)M.frr[j]; };
+      const int w = M.fW[f], so = M.fS[f], n = M.fN[f], e = M.fE[f];
+      const double ax = (double)M.frr[f] * (kMgOmega * rf / (double)M.frr[f]) - (double)M.fsx[w] * xs(w) -
+                        (double)M.fsy[so] * xs(so) - (double)M.fsy[f] * xs(n) - (double)M.fsx[f] * xs(e);
+      acc += rf - ax;
+    }
+    M.cb[C] = (T)acc;
+  }
+  rrn = block_sum(rrn, lds);
+  if (threadIdx.x == 0) {
+    A.part[(P_RR0 + nxt) * A.nblk + blockIdx.x] = rrn;
+    if (blockIdx.x == 0) A.iters[0] = i + 1;
+  }
+}
+
 // Coarsest level: kMgCoarseSweeps damped Jacobi sweeps from zero, one workgroup.
 template <typename T>
 __global__ void __launch_bounds__(kMgCoarseMax) k_mg_coarsest(int n, const int *lE, const int *lW, const int *lN,
@@ -357,6 +417,87 @@ __global__ void __launch_bounds__(kMgCoarseMax) k_mg_coarsest(int n, const int *
     __syncthreads();
   }
   if (act) xo[c] = (T)xs[c];
+}
+
+
+// The small levels of the hierarchy (each at most kMgSmall cells) in one
+// workgroup of kMgCoarseMax lanes:
+// restriction sweeps, the coarsest smoothing and the prolongation sweeps, with
+// a workgroup barrier between levels.  Same arithmetic as k_mg_down /
+// k_mg_coarsest / k_mg_up.
+constexpr int kMgSmall = 2048;
+constexpr int kMgMaxLevels = 12;
+
+template <typename T>
+struct MgChain {
+  int nlev;                       // levels in the chain (first = the finest handled here)
+  MgPair<T> pr[kMgMaxLevels];     // pr[i]: level i (fine) -> level i+1 (coarse), i < nlev-1
+  int nc;                         // coarsest size
+  const int *cE, *cW, *cN, *cS;
+  const T *crr, *csx, *csy, *cb;
+  T *cx;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_mg_chain(MgChain<T> Ch) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int i = 0; i + 1 < Ch.nlev; ++i) {
+    const MgPair<T> &M = Ch.pr[i];
+    for (int C = tid; C < M.nc; C += nt) {
+      double acc = 0.0;
+      for (int s2 = 0; s2 < 4; ++s2) {
+        const int f = M.ch[4 * C + s2];
+        if (f == M.nf) continue;
+        auto xs = [&](int j) { return kMgOmega * (double)M.fb[j] / (double)M.frr[j]; };
+        const int w = M.fW[f], so = M.fS[f], n = M.fN[f], e = M.fE[f];
+        const double ax = (double)M.frr[f] * xs(f) - (double)M.fsx[w] * xs(w) - (double)M.fsy[so] * xs(so) -
+                          (double)M.fsy[f] * xs(n) - (double)M.fsx[f] * xs(e);
+        acc += (double)M.fb[f] - ax;
+      }
+      M.cb[C] = (T)acc;
+    }
+    __syncthreads();
+  }
+  // coarsest (at most blockDim.x cells): damped Jacobi sweeps in LDS, as k_mg_coarsest
+  {
+    __shared__ double xs[kMgCoarseMax + 1];
+    const int n = Ch.nc, cc = tid;
+    const bool act = cc < n;
+    if (cc == 0) xs[n] = 0.0;
+    double bc = 0, rc = 1, sxc = 0, syc = 0, sxw = 0, sys = 0;
+    int w = n, so = n, no = n, e = n;
+    if (act) {
+      bc = Ch.cb[cc]; rc = Ch.crr[cc]; sxc = Ch.csx[cc]; syc = Ch.csy[cc];
+      w = Ch.cW[cc]; so = Ch.cS[cc]; no = Ch.cN[cc]; e = Ch.cE[cc];
+      sxw = Ch.csx[w]; sys = Ch.csy[so];
+      xs[cc] = kMgOmega * bc / rc;
+    }
+    __syncthreads();
+    for (int it = 1; it < kMgCoarseSweeps; ++it) {
+      double xn = 0.0;
+      if (act) {
+        const double ax = rc * xs[cc] - sxw * xs[w] - sys * xs[so] - syc * xs[no] - sxc * xs[e];
+        xn = xs[cc] + kMgOmega * (bc - ax) / rc;
+      }
+      __syncthreads();
+      if (act) xs[cc] = xn;
+      __syncthreads();
+    }
+    if (act) Ch.cx[cc] = (T)xs[cc];
+    __syncthreads();
+  }
+  for (int i = Ch.nlev - 2; i >= 0; --i) {
+    const MgPair<T> &M = Ch.pr[i];
+    for (int f = tid; f < M.nf; f += nt) {
+      auto ys = [&](int j) { return kMgOmega * (double)M.fb[j] / (double)M.frr[j] + (double)M.cx[M.parent[j]]; };
+      const int w = M.fW[f], so = M.fS[f], n = M.fN[f], e = M.fE[f];
+      const double yf = ys(f);
+      const double ay = (double)M.frr[f] * yf - (double)M.fsx[w] * ys(w) - (double)M.fsy[so] * ys(so) -
+                        (double)M.fsy[f] * ys(n) - (double)M.fsx[f] * ys(e);
+      M.fx[f] = (T)(yf + kMgOmega * ((double)M.fb[f] - ay) / (double)M.frr[f]);
+    }
+    __syncthreads();
+  }
 }
 
 // ---- FP64 outer level (zeta, qq, true residual) ---------------------------
@@ -508,6 +649,10 @@ static int mg_setup(s3g_ctx *c, const int *l2i, const int *l2j, bool fp32) {
     cj = nj;
     nf = nc;
   }
+  if (nf > kMgCoarseMax) {
+    set_error("multigrid: coarsest level has " + std::to_string(nf) + " cells");
+    return S3G_ERR_UNSUPPORTED;
+  }
   return hip_check(hipGetLastError(), "mg setup");
 }
 
@@ -516,7 +661,7 @@ int pcg_setup(s3g_ctx *c, const int *lN0, const int *l2i, const int *l2j) {
   // lN(c-1) = c (lN0 is 0-based, sentinel = lm).
   Pcg &P = c->pcg;
   P.n = c->lm;
-  P.nblk = std::max(1, std::min((P.n + kBlock - 1) / kBlock, 1024));
+  P.nblk = std::max(1, std::min((P.n + kBlock - 1) / kBlock, 512));
   std::vector<int> starts;
   for (int cc = 0; cc < c->lm; ++cc)
     if (cc == 0 || lN0[cc - 1] != cc) starts.push_back(cc);
@@ -538,14 +683,13 @@ static double host_sum(s3g_ctx *c, const double *dev_part, int n) {
 
 // z = M^-1 r for the line and multigrid preconditioners, with the partial of
 // (r, z) written into `slot` (Jacobi is fused into the CG kernels instead).
+// `rc` is the residual buffer; with `down0_done` the level-0 restriction has
+// already run (fused into k_cg_update_mg).
 template <typename T>
-static void apply_prec(s3g_ctx *c, CgArgs<T> &A, int slot) {
+static void apply_prec(s3g_ctx *c, CgArgs<T> &A, const T *rc, int slot, bool down0_done) {
   const dim3 b(kBlock);
   if (c->prec_kind == S3G_PREC_LINE) {
-    const int nlb = std::min(A.nblk, std::max(1, (A.nlines + kBlock - 1) / kBlock));
-    // the line solve writes partials only for its own (possibly fewer) blocks
-    hipMemsetAsync(A.part + slot * A.nblk, 0, A.nblk * sizeof(double), c->stream);
-    hipLaunchKernelGGL((k_line_solve<T>), dim3(nlb), b, 0, c->stream, A, slot);
+    hipLaunchKernelGGL((k_line_solve<T>), dim3(A.nblk), b, 0, c->stream, A, rc, slot);
     return;
   }
   // multigrid V-cycle: level 0 is the CG system itself
@@ -556,7 +700,7 @@ static void apply_prec(s3g_ctx *c, CgArgs<T> &A, int slot) {
     if (l == 0) {
       M.nf = A.n;
       M.fE = A.lE; M.fW = A.lW; M.fN = A.lN; M.fS = A.lS;
-      M.frr = A.rr; M.fsx = A.sx; M.fsy = A.sy; M.fb = A.r; M.fx = A.z;
+      M.frr = A.rr; M.fsx = A.sx; M.fsy = A.sy; M.fb = rc; M.fx = A.z;
     } else {
       M.nf = Lv[l].n;
       M.fE = Lv[l].lE; M.fW = Lv[l].lW; M.fN = Lv[l].lN; M.fS = Lv[l].lS;
@@ -571,14 +715,28 @@ static void apply_prec(s3g_ctx *c, CgArgs<T> &A, int slot) {
     return M;
   };
   auto grid = [&](int n) { return dim3(std::max(1, std::min((n + kBlock - 1) / kBlock, 2048))); };
-  for (int l = 0; l + 1 < nl; ++l) {
+  // levels 0 .. lc-1 run one grid-wide kernel per level; lc .. nl-1 (each at
+  // most kMgSmall cells) run in one workgroup.  lc >= 1: level 0 is the CG
+  // system, whose prolongation also writes the (r,z) partials.
+  int lc = 1;
+  while (lc < nl - 1 && Lv[lc].n > kMgSmall) ++lc;
+  if (nl - lc > kMgMaxLevels) lc = nl - kMgMaxLevels;
+  for (int l = down0_done ? 1 : 0; l < lc; ++l) {
     MgPair<T> M = pair(l);
     hipLaunchKernelGGL((k_mg_down<T>), grid(M.nc), b, 0, c->stream, M);
   }
-  const MgLevel &Lc = Lv[nl - 1];
-  hipLaunchKernelGGL((k_mg_coarsest<T>), dim3(1), dim3(kMgCoarseMax), 0, c->stream, Lc.n, Lc.lE, Lc.lW, Lc.lN, Lc.lS,
-                     (const T *)Lc.rr, (const T *)Lc.sx, (const T *)Lc.sy, (const T *)Lc.b, (T *)Lc.x);
-  for (int l = nl - 2; l >= 0; --l) {
+  {
+    MgChain<T> Ch{};
+    Ch.nlev = nl - lc;
+    for (int i = 0; i + 1 < Ch.nlev; ++i) Ch.pr[i] = pair(lc + i);
+    const MgLevel &Lc = Lv[nl - 1];
+    Ch.nc = Lc.n;
+    Ch.cE = Lc.lE; Ch.cW = Lc.lW; Ch.cN = Lc.lN; Ch.cS = Lc.lS;
+    Ch.crr = (const T *)Lc.rr; Ch.csx = (const T *)Lc.sx; Ch.csy = (const T *)Lc.sy; Ch.cb = (const T *)Lc.b;
+    Ch.cx = (T *)Lc.x;
+    hipLaunchKernelGGL((k_mg_chain<T>), dim3(1), dim3(1024), 0, c->stream, Ch);
+  }
+  for (int l = lc - 1; l >= 0; --l) {
     MgPair<T> M = pair(l);
     if (l == 0)
       hipLaunchKernelGGL((k_mg_up<T>), dim3(A.nblk), b, 0, c->stream, M, A.part, A.nblk, slot);
@@ -615,13 +773,22 @@ template <typename T>
 static int run_cg(s3g_ctx *c, CgArgs<T> &A, int maxit, int hint) {
   const dim3 g(A.nblk), b(kBlock);
   const bool jac = (c->prec_kind == S3G_PREC_JACOBI) || (c->prec_kind == S3G_PREC_LINE && A.nlines == 0);
+  const bool mg = c->prec_kind == S3G_PREC_MG;
   if (!jac) {
     hipLaunchKernelGGL((k_cg_residual<T, false>), g, b, 0, c->stream, A);
-    apply_prec(c, A, P_RZ0);
+    apply_prec(c, A, (const T *)A.r, P_RZ0, false);
   } else {
     hipLaunchKernelGGL((k_cg_residual<T, true>), g, b, 0, c->stream, A);
   }
-  hipMemsetAsync(A.iters, 0, sizeof(int), c->stream);
+  MgPair<T> M0{};
+  if (mg) {  // level 0 -> 1 of the hierarchy; fb is recomputed inside the fused kernel
+    const MgLevel &L1 = c->pcg.mg[1];
+    M0.nf = A.n;
+    M0.fE = A.lE; M0.fW = A.lW; M0.fN = A.lN; M0.fS = A.lS;
+    M0.frr = A.rr; M0.fsx = A.sx; M0.fsy = A.sy;
+    M0.nc = L1.n; M0.ch = L1.ch; M0.parent = L1.parent_of_fine;
+    M0.cb = (T *)L1.b;
+  }
   int it = 0, slot = -1;
   int chunk = std::max(4, std::min(hint + 2, 512));
   while (it < maxit) {
@@ -629,10 +796,14 @@ static int run_cg(s3g_ctx *c, CgArgs<T> &A, int maxit, int hint) {
     if (c->timing) timer_begin(c, sizeof(T) == 8 ? "pcg_chunk_f64" : "pcg_chunk_f32", &slot);
     for (int j = 0; j < nl; ++j) {
       const int i = it + j;
+      const T *rnext = ((i + 1) & 1) ? A.r1 : A.r;
       hipLaunchKernelGGL((k_cg_dir<T>), g, b, 0, c->stream, A, i);
-      if (!jac) {
+      if (mg) {
+        hipLaunchKernelGGL((k_cg_update_mg<T>), g, b, 0, c->stream, A, M0, i);
+        apply_prec(c, A, rnext, P_RZ0 + ((i + 1) & 1), true);
+      } else if (!jac) {
         hipLaunchKernelGGL((k_cg_update<T, false>), g, b, 0, c->stream, A, i);
-        apply_prec(c, A, P_RZ0 + ((i + 1) & 1));
+        apply_prec(c, A, rnext, P_RZ0 + ((i + 1) & 1), false);
       } else {
         hipLaunchKernelGGL((k_cg_update<T, true>), g, b, 0, c->stream, A, i);
       }
@@ -649,7 +820,8 @@ static int run_cg(s3g_ctx *c, CgArgs<T> &A, int maxit, int hint) {
 
 template <typename T>
 static CgArgs<T> cg_args(s3g_ctx *c, const int *lE, const int *lW, const int *lN, const int *lS, const T *rr,
-                         const T *sx, const T *sy, T *x, T *r, T *z, T *p0, T *p1, T *q, const T *b, const T *lc,
+                         const T *sx, const T *sy, T *x, T *r, T *r1, T *z, T *p0, T *p1, T *q, const T *b,
+                         const T *lc,
                          const T *ld, double tol) {
   Pcg &P = c->pcg;
   CgArgs<T> A{};
@@ -664,6 +836,7 @@ static CgArgs<T> cg_args(s3g_ctx *c, const int *lE, const int *lW, const int *lN
   A.nblk = P.nblk;
   A.x = x;
   A.r = r;
+  A.r1 = r1;
   A.z = z;
   A.p0 = p0;
   A.p1 = p1;
@@ -697,7 +870,7 @@ int solve_free_surface(s3g_ctx *c, double tz) {
   hipLaunchKernelGGL(k_start_guess, g, b, 0, c->stream, P.x, (const double *)D.s, P.n);
 
   if (c->precision == S3G_FP64) {
-    CgArgs<double> A = cg_args<double>(c, D.lE, D.lW, D.lN, D.lS, D.rr, D.sx, D.sy, P.x, P.r, P.z, P.p0, P.p1, P.q,
+    CgArgs<double> A = cg_args<double>(c, D.lE, D.lW, D.lN, D.lS, D.rr, D.sx, D.sy, P.x, P.r, P.r1, P.z, P.p0, P.p1, P.q,
                                        D.qq, P.lc, P.ld, 0.5 * tol);
     if (line) hipLaunchKernelGGL((k_line_factor<double>), gl, b, 0, c->stream, A, P.lc, P.ld);
     if (c->prec_kind == S3G_PREC_MG) mg_build(c, A);
@@ -720,7 +893,7 @@ int solve_free_surface(s3g_ctx *c, double tz) {
     // iterative refinement with an FP32 inner CG
     hipLaunchKernelGGL((k_cast_matrix<float>), g, b, 0, c->stream, (const double *)D.rr, (const double *)D.sx,
                        (const double *)D.sy, P.rrf, P.sxf, P.syf, P.n);
-    CgArgs<float> A = cg_args<float>(c, D.lE, D.lW, D.lN, D.lS, P.rrf, P.sxf, P.syf, P.df, P.rf, P.zf, P.pf0, P.pf1,
+    CgArgs<float> A = cg_args<float>(c, D.lE, D.lW, D.lN, D.lS, P.rrf, P.sxf, P.syf, P.df, P.rf, P.rf1, P.zf, P.pf0, P.pf1,
                                      P.qf, P.bf, P.lcf, P.ldf, kInnerTol);
     if (line) hipLaunchKernelGGL((k_line_factor<float>), gl, b, 0, c->stream, A, P.lcf, P.ldf);
     if (c->prec_kind == S3G_PREC_MG) mg_build(c, A);
