@@ -24,24 +24,27 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes per wet cell for one launch of each kernel, in elements of
-# the 3-D storage type: distinct state arrays read + written by the phase
-# (phase-private scratch, index arrays and 2-D arrays not counted).
+# Algorithmic bytes per wet cell for one launch of each kernel: distinct state
+# arrays read plus distinct arrays written that a later phase consumes, counted
+# once per wet cell (phase-private scratch, index arrays and 2-D arrays not
+# counted), as (arrays of the storage type, FP64 arrays).  q2/q2l and their
+# time levels are FP64 in every precision mode.
 ARRAYS_PER_CELL = {
-    "exmom_x": 10, "exmom_y": 10,        # 9 reads + ex
-    "matmom_x": 7, "matmom_y": 7,        # ex, Av, rhop, hq, qpp + ag, ar
-    "matcon": 2,                         # uhpp, vhpp
-    "slayer_h": 6,                       # hp, hup, hvp -> h, hu, hv
-    "vel": 6,                            # agx, arx, agy, ary -> uh, vh
-    "continuity1": 5,                    # uh, uhpp, vh, vhpp -> wp
-    "exsal_hflux": 10,                   # salpp, uhpp, uh, vhpp, vh, hup, hvp, hpp -> fx_ew, fx_ns
-    "exsal_imsal": 11,                   # fluxes, salpp, hpp, hp, hupp, hvpp, wp, h, Dv -> sal
-    "vel2": 3,                           # -> h, hu, hv
-    "turbulence": 14,                    # hp, rhop, up, vp, Av, Dv, q2p, q2pp, q2lp, q2lpp -> q2, q2l, Av, Dv
-    "settrap_col": 24, "settrap2_col": 16, "save_col": 26,
-    "continuity2": 3,
+    "exmom_x": (10, 0), "exmom_y": (10, 0),      # uhp vhp wp up upp uhpp hup hupp hp -> ex
+    "matmom_x": (7, 0), "matmom_y": (7, 0),      # ex Av rhop hup upp uhpp -> agx arx
+    "matcon": (2, 0),                            # uhpp vhpp
+    "slayer_h": (6, 0),                          # hp hup hvp -> h hu hv
+    "vel": (8, 0),                               # agx arx agy ary hu hv -> uh vh
+    "continuity1": (5, 0),                       # uh uhpp vh vhpp -> wp
+    "exsal_hflux": (10, 0),                      # salpp uhpp uh vhpp vh hup hvp hpp -> fx_ew fx_ns
+    "exsal_imsal": (11, 0),                      # fx_ew fx_ns salpp hpp hp hupp hvpp wp h Dv -> sal
+    "vel2": (3, 0),                              # -> h hu hv
+    "turbulence": (8, 6),                        # hp rhop up vp Av Dv -> Av Dv; q2p q2pp q2lp q2lpp -> q2 q2l
+    "settrap_cell": (28, 8),                     # 11 read, 17 written; 4 + 4 FP64
+    "settrap2_cell": (15, 6),                    # 6 read, 9 written; 4 + 2 FP64
+    "save_cell": (17, 4),                        # trapezoidal save: 6 read, 11 written; 2 + 2 FP64
+    "continuity2": (3, 0),                       # uhp vhp -> wp
 }
-TURB_FP64 = {"turbulence"}
 
 
 def parse():
@@ -122,8 +125,8 @@ def main():
                 best = (name, tot, avg_ms)
         if best:
             name, _, avg_ms = best
-            eb = 8 if name in TURB_FP64 else elem
-            bytes_launch = ARRAYS_PER_CELL[name] * eb * n_wet
+            n_r, n_f64 = ARRAYS_PER_CELL[name]
+            bytes_launch = (n_r * elem + n_f64 * 8) * n_wet
             achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
             roofline = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -159,6 +162,7 @@ def main():
         "setup_s": round(t_setup, 2),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "tiles": g.tiles(),
         "kernels": kernel_summary,
     }
     print(json.dumps(out), flush=True)

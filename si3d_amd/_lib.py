@@ -18,7 +18,7 @@ EXPORTS = [
     "s3g_init", "s3g_set_field_f32", "s3g_set_field_f64", "s3g_set_field_i32",
     "s3g_get_field_f32", "s3g_get_field_f64", "s3g_get_field_i32", "s3g_fd",
     "s3g_settrap", "s3g_settrap2", "s3g_save", "s3g_step", "s3g_run", "s3g_phase",
-    "s3g_synchronize", "s3g_get_stats", "s3g_kernel_times", "s3g_reset_timers",
+    "s3g_synchronize", "s3g_get_stats", "s3g_kernel_times", "s3g_reset_timers", "s3g_get_tiles",
     "s3g_finalize", "s3g_last_error", "s3g_create_from_deck",
 ]
 
@@ -88,6 +88,7 @@ def load():
     L.s3g_get_stats.argtypes = [vp, C.POINTER(S3gStats)]
     L.s3g_kernel_times.argtypes = [vp, C.c_int, C.c_void_p, dp, C.POINTER(C.c_long)]
     L.s3g_reset_timers.argtypes = [vp, C.c_int]
+    L.s3g_get_tiles.argtypes = [vp, C.c_int, ip]
     L.s3g_finalize.argtypes = [vp]
     L.s3g_finalize.restype = None
     L.s3g_last_error.restype = C.c_char_p

@@ -321,6 +321,21 @@ int s3g_init(const s3g_grid *g, const s3g_params *p, s3g_ctx **out) {
   s3g_ctx *c = new s3g_ctx();
   c->precision = p->precision;
   c->debug = std::getenv("S3G_DEBUG") != nullptr;
+  {
+    const char *t = std::getenv("S3G_TUNE");
+    c->tune = !(t && t[0] == '0');
+    // S3G_TILES="w0,w1,...": fixed tile words (chunk | order << 8), no tuning
+    if (const char *w = std::getenv("S3G_TILES")) {
+      c->tune = false;
+      for (int i = 0; i < s3g::TILE_NUM && *w; ++i) {
+        char *end = nullptr;
+        const long v = std::strtol(w, &end, 10);
+        if (end == w) break;
+        if ((v & 255) >= 1) c->kc[i] = (int)v;
+        w = (*end == ',') ? end + 1 : end;
+      }
+    }
+  }
   c->prec_kind = p->preconditioner;
   c->im1 = g->im1;
   c->jm1 = g->jm1;
@@ -543,6 +558,13 @@ int s3g_reset_timers(s3g_ctx *c, int enable) {
   c->timers.clear();
   c->timing = enable != 0;
   return 0;
+}
+
+int s3g_get_tiles(s3g_ctx *c, int max, int *kc) {
+  if (!c || !kc) return S3G_ERR_ARG;
+  const int n = std::min<int>(max, s3g::TILE_NUM);
+  for (int i = 0; i < n; ++i) kc[i] = c->kc[i];
+  return n;
 }
 
 int s3g_kernel_times(s3g_ctx *c, int max, char (*names)[32], double *avg_ms, long *launches) {

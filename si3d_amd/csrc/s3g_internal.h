@@ -87,6 +87,14 @@ struct Timer {
 
 }  // namespace s3g
 
+namespace s3g {
+// Per-cell kernels with a layer-chunk size; the first TILE_TUNED are autotuned.
+enum TileId {
+  TILE_EXMOM_X, TILE_EXMOM_Y, TILE_SLAYER_H, TILE_VEL, TILE_HFLUX, TILE_VEL2, TILE_TUNED,
+  TILE_SETTRAP = TILE_TUNED, TILE_SAVE, TILE_NUM
+};
+}
+
 struct s3g_ctx {
   int precision = S3G_FP64;
   int prec_kind = S3G_PREC_JACOBI;
@@ -115,6 +123,10 @@ struct s3g_ctx {
   std::vector<std::pair<int, int>> pending;  // (timer index, event pair index)
   int *h_iters = nullptr;                    // pinned
   double *h_scal = nullptr;                  // pinned
+  // layer-chunk size of each per-cell kernel (autotuned at the first fd call
+  // unless S3G_TUNE=0)
+  int kc[s3g::TILE_NUM] = {1, 1, 1, 1, 1, 1, 1, 1};
+  bool tune = true, tuned = false;
 };
 
 namespace s3g {
@@ -125,6 +137,7 @@ template <typename R> int settrap(s3g_ctx *c);
 template <typename R> int settrap2(s3g_ctx *c);
 template <typename R> int save(s3g_ctx *c, int istep);
 int init_density(s3g_ctx *c);
+template <typename R> int autotune(s3g_ctx *c);
 // pcg.hip
 template <typename R> int solve_free_surface(s3g_ctx *c, double tz);
 int pcg_setup(s3g_ctx *c, const int *lN0, const int *l2i, const int *l2j);

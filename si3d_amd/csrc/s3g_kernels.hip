@@ -8,6 +8,8 @@
 // in the same coalesced layout and are read back by the same lane.
 #include <cfloat>
 #include <cmath>
+#include <cstdio>
+#include <vector>
 
 #include "s3g_internal.h"
 
@@ -23,6 +25,45 @@ constexpr double B_1 = 16.6, E_1 = 1.80, E_2 = 1.33, E_3 = 0.25, SQ = 0.20;
 __device__ __forceinline__ int col() { return blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
 __device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+
+// Per-cell kernels: a lane owns one column and a chunk of `kc` consecutive
+// layers (column-invariant index loads are shared by the chunk).  The tile
+// word packs kc (low 8 bits) and the order of the 1-D grid of (column block,
+// layer chunk) tiles: order 0 runs layer chunk by layer chunk (all column
+// blocks of a chunk are in flight together, so the E/W neighbour rows of a
+// plane are read by concurrently running workgroups); order 1 remaps the grid
+// so that each of the 8 XCDs (workgroups are dispatched round-robin over them)
+// runs a contiguous range of column blocks with all their layer chunks.  The
+// autotuner picks both per kernel; results do not depend on them.
+// Each kernel has a chunked instantiation (CH) and a single-layer one used for
+// chunk size 1: a loop lets the compiler hoist the column's address arithmetic,
+// which can cost more registers (occupancy) than it saves.
+struct CellTile {
+  int c, k0, k1;  // column, layers [k0, k1)
+};
+
+__device__ __forceinline__ bool cell_tile(int lm, int k_first, int nk, int tile, CellTile &t) {
+  const int kc = tile & 255;
+  const int nkc = (nk + kc - 1) / kc;
+  const int nbx = (lm + kBlock - 1) / kBlock;
+  const int pid = blockIdx.x;
+  int bx, kq;
+  if ((tile >> 8) == 0) {
+    kq = pid / nbx;
+    bx = pid - kq * nbx;
+  } else {
+    const int total = nbx * nkc;
+    const int xcd = pid & 7, idx = pid >> 3;
+    const int per = total >> 3, rem = total & 7;
+    const int logical = (xcd < rem) ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+    bx = logical / nkc;
+    kq = logical - bx * nkc;
+  }
+  t.c = bx * kBlock + threadIdx.x;
+  t.k0 = k_first + kq * kc;
+  t.k1 = min(t.k0 + kc, k_first + nk);
+  return t.c < lm;
+}
 
 // UNESCO/Gill EOS with the fixed-point pressure iteration (si3d_utils.f90:3865-3949),
 // evaluated in FP64 in every precision mode.  salinity**(3/2) is salinity**1 in the
@@ -86,15 +127,12 @@ __device__ __forceinline__ double hcoef(const Dev<R> &D, int k, int w) {
 
 // exmom: explicit advection, Coriolis and horizontal diffusion of the x (X) or
 // y momentum equation, si3d_procedures.f90:578-1173.  One lane per (column,
-// layer): the grid's y index is the layer (k = 2 + blockIdx.y).  ex can only
+// layer, k = 2..km in chunks, see cell_tile).  ex can only
 // ever be non-zero on k1 <= k <= kmq (kmq = bottom of the face, fixed), so
 // layers below kmq are never touched (they stay zero from allocation) and the
 // zeroing of the reference (645-656, 821-827) reduces to k1 <= k < k1q.
 template <typename R, bool X>
-__global__ void __launch_bounds__(kBlock) k_exmom(Dev<R> D, double twodt1) {
-  const int c = col();
-  if (c >= D.lm) return;
-  const int k = 2 + blockIdx.y;
+__device__ __forceinline__ void exmom_cell(const Dev<R> &D, int c, int k, double twodt1) {
   const int S = D.lm;
   const int lE = D.lE[c], ln = X ? lE : D.lN[c];
   if (ln == S) return;  // no face: ex stays zero
@@ -194,6 +232,13 @@ __global__ void __launch_bounds__(kBlock) k_exmom(Dev<R> D, double twodt1) {
     else
       ex[IX(k, c)] = (R)((double)D.vhpp[IX(k, c)] - twodt1 * (adv * D.P.iadv + cor - hd));
   }
+}
+
+template <typename R, bool X, bool CH>
+__global__ void __launch_bounds__(kBlock) k_exmom(Dev<R> D, double twodt1, int nk, int tile) {
+  CellTile t;
+  if (!cell_tile(D.lm, 2, nk, tile, t)) return;
+  for (int k = t.k0; k < (CH ? t.k1 : min(t.k1, t.k0 + 1)); ++k) exmom_cell<R, X>(D, t.c, k, twodt1);
 }
 
 // bclnc_km: baroclinic term of the bottom layer on a horizontal plane,
@@ -382,13 +427,10 @@ __global__ void __launch_bounds__(kBlock) k_matcon(Dev<R> D, double tz) {
 
 // slayer_h: new surface-layer thicknesses after the zeta solve (drying only),
 // si3d_procedures.f90:2070-2175, including the re-test of the same u-layer (2136).
-// One lane per (column, layer k = 1 + blockIdx.y).  Below kmz+1 (the sediment
-// cell) h, hp, hu, hup, hv, hvp are all zero, so only k <= kmz+1 is visited.
+// One lane per (column, layer k = 1..km1).  Below kmz+1 (the sediment cell) h,
+// hp, hu, hup, hv, hvp are all zero, so only k <= kmz+1 is visited.
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_slayer_h(Dev<R> D) {
-  const int c = col();
-  if (c >= D.lm) return;
-  const int k = 1 + blockIdx.y;
+__device__ __forceinline__ void slayer_h_cell(const Dev<R> &D, int c, int k) {
   if (k > D.kmz[c] + 1) return;
   const int S = D.lm, km1 = D.km1;
   const int k1z = D.k1z[c], k1u = D.k1u[c], k1v = D.k1v[c];
@@ -432,6 +474,13 @@ __global__ void __launch_bounds__(kBlock) k_slayer_h(Dev<R> D) {
   }
 }
 
+template <typename R, bool CH>
+__global__ void __launch_bounds__(kBlock) k_slayer_h(Dev<R> D, int nk, int tile) {
+  CellTile t;
+  if (!cell_tile(D.lm, 1, nk, tile, t)) return;
+  for (int k = t.k0; k < (CH ? t.k1 : min(t.k1, t.k0 + 1)); ++k) slayer_h_cell<R>(D, t.c, k);
+}
+
 // The sentinel column: h := hp, hu := hup, hv := hvp (2087-2089).
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_slayer_h_sentinel(Dev<R> D) {
@@ -445,7 +494,7 @@ __global__ void __launch_bounds__(kBlock) k_slayer_h_sentinel(Dev<R> D) {
 
 // vel: fluxes from the new zeta plus the top-cell drying merge,
 // si3d_procedures.f90:2430-2545.
-// One lane per (column, layer k = 2 + blockIdx.y).  The drying merge moves the
+// One lane per (column, layer k = 2..km).  The drying merge moves the
 // top flux one layer down: the lane of layer k0+1 adds the top flux, the lane
 // of k0 zeroes it.
 template <typename R, bool X>
@@ -472,13 +521,14 @@ __device__ __forceinline__ void vel_face(const Dev<R> &D, int c, int k, double t
   qh[IX(k, c)] = v;
 }
 
-template <typename R>
-__global__ void __launch_bounds__(kBlock) k_vel(Dev<R> D, double tz) {
-  const int c = col();
-  if (c >= D.lm) return;
-  const int k = 2 + blockIdx.y;
-  vel_face<R, true>(D, c, k, tz);
-  vel_face<R, false>(D, c, k, tz);
+template <typename R, bool CH>
+__global__ void __launch_bounds__(kBlock) k_vel(Dev<R> D, double tz, int nk, int tile) {
+  CellTile t;
+  if (!cell_tile(D.lm, 2, nk, tile, t)) return;
+  for (int k = t.k0; k < (CH ? t.k1 : min(t.k1, t.k0 + 1)); ++k) {
+    vel_face<R, true>(D, t.c, k, tz);
+    vel_face<R, false>(D, t.c, k, tz);
+  }
 }
 
 // continuity: vertical velocity by bottom-up integration of the horizontal
@@ -541,10 +591,7 @@ __device__ __forceinline__ double tvd_flux(double vel, double s1, double s2, dou
 // column, si3d_procedures.f90:2885-2975.  Only the wet range of each column is
 // rewritten; the arrays persist across calls like the reference's Bth2/Bth4.
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_exsal_hflux(Dev<R> D, double twodt1) {
-  const int c = col();
-  if (c >= D.lm) return;
-  const int k = 2 + blockIdx.y;
+__device__ __forceinline__ void exsal_hflux_cell(const Dev<R> &D, int c, int k, double twodt1) {
   const int kms = D.kmz[c], k1s = D.k1z[c];
   if (k < k1s || k > kms) return;
   const int lE = D.lE[c], lW = D.lW[c], lN = D.lN[c], lS = D.lS[c];
@@ -567,6 +614,13 @@ __global__ void __launch_bounds__(kBlock) k_exsal_hflux(Dev<R> D, double twodt1)
   }
   D.fx_ew[IX(k, c)] = (R)fe;
   D.fx_ns[IX(k, c)] = (R)fn;
+}
+
+template <typename R, bool CH>
+__global__ void __launch_bounds__(kBlock) k_exsal_hflux(Dev<R> D, double twodt1, int nk, int tile) {
+  CellTile t;
+  if (!cell_tile(D.lm, 2, nk, tile, t)) return;
+  for (int k = t.k0; k < (CH ? t.k1 : min(t.k1, t.k0 + 1)); ++k) exsal_hflux_cell<R>(D, t.c, k, twodt1);
 }
 
 // Vertical TVD flux through the top of cell k (si3d_procedures.f90:2999-3035).
@@ -667,7 +721,7 @@ __global__ void __launch_bounds__(kBlock) k_exsal_imsal(Dev<R> D, double twodt1)
 }
 
 // vel2: layer_h (si3d_procedures.f90:2181-2231) then the top-cell flux split for
-// wetting (2570-2631).  One lane per (column, layer k = 2 + blockIdx.y); the
+// wetting (2570-2631).  One lane per (column, layer k = 2..km); the
 // lane of the top face layer performs the split for both layers it touches,
 // evaluating layer_h for the layer above itself.
 template <typename R, bool X>
@@ -697,18 +751,20 @@ __device__ __forceinline__ void vel2_face(const Dev<R> &D, int c, int k, double 
   }
 }
 
-template <typename R>
-__global__ void __launch_bounds__(kBlock) k_vel2(Dev<R> D) {
-  const int c = col();
-  if (c >= D.lm) return;
-  const int k = 2 + blockIdx.y;
+template <typename R, bool CH>
+__global__ void __launch_bounds__(kBlock) k_vel2(Dev<R> D, int nk, int tile) {
+  CellTile t;
+  if (!cell_tile(D.lm, 2, nk, tile, t)) return;
+  const int c = t.c;
   const double sc = D.s[c];
-  if (k <= D.kmz[c]) {
-    double v = dmin(D.zlevel[k + 1], D.hhs[c]) - dmax(D.zlevel[k], -sc);
-    D.h[IX(k, c)] = (v <= 0.0) ? R(0) : (R)v;
+  for (int k = t.k0; k < (CH ? t.k1 : min(t.k1, t.k0 + 1)); ++k) {
+    if (k <= D.kmz[c]) {
+      double v = dmin(D.zlevel[k + 1], D.hhs[c]) - dmax(D.zlevel[k], -sc);
+      D.h[IX(k, c)] = (v <= 0.0) ? R(0) : (R)v;
+    }
+    vel2_face<R, true>(D, c, k, sc);
+    vel2_face<R, false>(D, c, k, sc);
   }
-  vel2_face<R, true>(D, c, k, sc);
-  vel2_face<R, false>(D, c, k, sc);
 }
 
 // UpdateHorizontalMixingCoefficients (Smagorinsky), si3d_mixing.f90:894-966.
@@ -900,101 +956,56 @@ __global__ void __launch_bounds__(kBlock) k_zeta_levels(Dev<R> D) {
   }
 }
 
-// layer_hp2 + TopLayerIndexp2 for one column (si3d_procedures.f90:2238-2293, 2368-2427).
-template <typename R>
-__device__ void layer_hp2_col(const Dev<R> &D, int c) {
-  const int S = D.lm;
-  const double spc = D.sp[c];
-  int k1z = D.km1, k1u = D.km1, k1v = D.km1;
-  for (int k = D.k1; k <= D.kmz[c]; ++k) {
-    double v = dmin(D.zlevel[k + 1], D.hhs[c]) - dmax(D.zlevel[k], -spc);
-    R hv = (v <= 0.0) ? R(0) : (R)v;
-    D.hp[IX(k, c)] = hv;
-    if (hv > R(0) && k1z == D.km1) k1z = k;
-  }
-  const int lE = D.lE[c], lN = D.lN[c];
-  if (lE != S) {
-    const double sm = dmax(spc, D.sp[lE]);
-    for (int k = D.k1; k <= min(D.kmz[c], D.kmz[lE]); ++k) {
-      double v = dmin(D.zlevel[k + 1], D.hhu[c]) - dmax(D.zlevel[k], -sm);
-      R hv = (v <= 0.0) ? R(0) : (R)v;
-      D.hup[IX(k, c)] = hv;
-      if (hv > R(0) && k1u == D.km1) k1u = k;
-    }
-  }
-  if (lN != S) {
-    const double sm = dmax(spc, D.sp[lN]);
-    for (int k = D.k1; k <= min(D.kmz[c], D.kmz[lN]); ++k) {
-      double v = dmin(D.zlevel[k + 1], D.hhv[c]) - dmax(D.zlevel[k], -sm);
-      R hv = (v <= 0.0) ? R(0) : (R)v;
-      D.hvp[IX(k, c)] = hv;
-      if (hv > R(0) && k1v == D.km1) k1v = k;
-    }
-  }
-  D.k1z[c] = k1z;
-  D.k1u[c] = k1u;
-  D.k1v[c] = k1v;
+// Time-level shuffles as per-cell kernels (every (column, layer) lane is
+// independent) followed by small per-column kernels for what needs the whole
+// column: the top-layer indices and the corrections at the top cell.
+
+// Layer thickness of a zeta-, u- or v-cell at level k for surface elevation
+// `eta` and depth `hh` (layer_hp2 / layer_h, si3d_procedures.f90:2181-2293).
+__device__ __forceinline__ double cell_h(const double *zlevel, int k, double hh, double eta) {
+  const double v = dmin(zlevel[k + 1], hh) - dmax(zlevel[k], -eta);
+  return v;
 }
 
-// TopLayerIndexp2 alone (used by save after hp := h).
-template <typename R>
-__device__ void toplayer_col(const Dev<R> &D, int c) {
-  const int S = D.lm;
-  int k1z = D.km1, k1u = D.km1, k1v = D.km1;
-  for (int k = D.k1; k <= D.kmz[c]; ++k)
-    if ((double)D.hp[IX(k, c)] > 0.0) {
-      k1z = k;
-      break;
-    }
-  if (D.lE[c] != S)
-    for (int k = D.k1; k <= min(D.kmz[c], D.kmz[D.lE[c]]); ++k)
-      if ((double)D.hup[IX(k, c)] > 0.0) {
-        k1u = k;
-        break;
-      }
-  if (D.lN[c] != S)
-    for (int k = D.k1; k <= min(D.kmz[c], D.kmz[D.lN[c]]); ++k)
-      if ((double)D.hvp[IX(k, c)] > 0.0) {
-        k1v = k;
-        break;
-      }
-  D.k1z[c] = k1z;
-  D.k1u[c] = k1u;
-  D.k1v[c] = k1v;
-}
-
-// u/v fluxes and velocities at n+1/2 with the top-cell corrections, shared by
-// settrap (3488-3543) and settrap2 (3984-4035).  UPD_PP: settrap also moves
-// the p level into pp first.
+// Pass 1 of the u/v fluxes at n+1/2 (settrap 3488-3531, settrap2 3984-4023)
+// for one face cell.  UPD_PP: settrap moves the p level into pp first.
 template <typename R, bool X, bool UPD_PP>
-__device__ void half_level_flux(const Dev<R> &D, int c) {
+__device__ __forceinline__ void half_level_flux_cell(const Dev<R> &D, int c, int k) {
   const int ln = X ? D.lE[c] : D.lN[c];
-  if (ln == D.lm) return;
-  R *qh = X ? D.uh : D.vh;
+  if (ln == D.lm || k < D.k1 || k > min(D.kmz[c], D.kmz[ln])) return;
   R *qhp = X ? D.uhp : D.vhp;
   R *qhpp = X ? D.uhpp : D.vhpp;
   R *qp = X ? D.up : D.vp;
   R *qpp = X ? D.upp : D.vpp;
+  const R *qh = X ? D.uh : D.vh;
+  const R *hq = X ? D.hup : D.hvp;
+  if (UPD_PP) {
+    qhpp[IX(k, c)] = qhp[IX(k, c)];
+    qpp[IX(k, c)] = qp[IX(k, c)];
+  }
+  const double hk = hq[IX(k, c)];
+  if (hk > 0.0) {
+    const double f = 0.5 * ((double)qh[IX(k, c)] + (double)qhpp[IX(k, c)]);
+    qhp[IX(k, c)] = (R)f;
+    qp[IX(k, c)] = (R)(f / hk);
+  } else {
+    qhp[IX(k, c)] = R(0);
+    qp[IX(k, c)] = R(0);
+  }
+}
+
+// Top-cell corrections of the n+1/2 fluxes (3532-3543, 4024-4035).
+template <typename R, bool X>
+__device__ __forceinline__ void half_level_flux_top(const Dev<R> &D, int c, int k) {
+  const int ln = X ? D.lE[c] : D.lN[c];
+  if (ln == D.lm) return;
+  R *qhp = X ? D.uhp : D.vhp;
+  R *qp = X ? D.up : D.vp;
+  const R *qh = X ? D.uh : D.vh;
+  const R *qhpp = X ? D.uhpp : D.vhpp;
   const R *hq = X ? D.hup : D.hvp;
   const R *hqn = X ? D.hu : D.hv;
   const R *hqpp = X ? D.hupp : D.hvpp;
-  const int kmq = min(D.kmz[c], D.kmz[ln]);
-  for (int k = D.k1; k <= kmq; ++k) {
-    if (UPD_PP) {
-      qhpp[IX(k, c)] = qhp[IX(k, c)];
-      qpp[IX(k, c)] = qp[IX(k, c)];
-    }
-    const double hk = hq[IX(k, c)];
-    if (hk > 0.0) {
-      double f = 0.5 * ((double)qh[IX(k, c)] + (double)qhpp[IX(k, c)]);
-      qhp[IX(k, c)] = (R)f;
-      qp[IX(k, c)] = (R)(f / hk);
-    } else {
-      qhp[IX(k, c)] = R(0);
-      qp[IX(k, c)] = R(0);
-    }
-  }
-  const int k = X ? D.k1u[c] : D.k1v[c];
   if ((double)hqn[IX(k - 1, c)] > 0.0) {
     double f = (double)qhp[IX(k, c)] + (double)qh[IX(k - 1, c)] / 2.;
     qhp[IX(k, c)] = (R)f;
@@ -1007,31 +1018,76 @@ __device__ void half_level_flux(const Dev<R> &D, int c) {
   }
 }
 
-// settrap (MODE 0) / settrap2 (MODE 1) column part: layer thicknesses at n+1/2,
-// top indices, scalar and density at n+1/2, fluxes at n+1/2, turbulence levels.
-template <typename R, int MODE>
-__global__ void __launch_bounds__(kBlock) k_settrap_col(Dev<R> D) {
-  const int c = col();
-  if (c > D.lm) return;
-  if (MODE == 0) {  // hpp := hp (whole column, sentinel included)
-    for (int k = 0; k <= D.km1 + 1; ++k) {
+// TopLayerIndexp2 (2368-2427): first wet layer of the zeta-, u- and v-cells of
+// column c from hp / hup / hvp, km1 when there is none.
+template <typename R>
+__device__ __forceinline__ void top_layer_indices(const Dev<R> &D, int c) {
+  const int S = D.lm;
+  int k1z = D.km1, k1u = D.km1, k1v = D.km1;
+  for (int k = D.k1; k <= D.kmz[c]; ++k)
+    if ((double)D.hp[IX(k, c)] > 0.0) {
+      k1z = k;
+      break;
+    }
+  const int lE = D.lE[c], lN = D.lN[c];
+  if (lE != S)
+    for (int k = D.k1; k <= min(D.kmz[c], D.kmz[lE]); ++k)
+      if ((double)D.hup[IX(k, c)] > 0.0) {
+        k1u = k;
+        break;
+      }
+  if (lN != S)
+    for (int k = D.k1; k <= min(D.kmz[c], D.kmz[lN]); ++k)
+      if ((double)D.hvp[IX(k, c)] > 0.0) {
+        k1v = k;
+        break;
+      }
+  D.k1z[c] = k1z;
+  D.k1u[c] = k1u;
+  D.k1v[c] = k1v;
+}
+
+// settrap (MODE 0, 3418-3618) / settrap2 (MODE 1, 3944-4087), per cell
+// (c = 0..lm, k = 0..km1+1): pp := p for the thicknesses (settrap), layer_hp2's
+// thicknesses from the n+1/2 zeta (2238-2293), scalar and density at n+1/2
+// (3471-3486), pass 1 of the fluxes, turbulence levels (settrap_2EqTVars).
+template <typename R, int MODE, bool CH>
+__global__ void __launch_bounds__(kBlock) k_settrap_cell(Dev<R> D, int nk, int tile) {
+  CellTile t;
+  if (!cell_tile(D.lm + (MODE == 0 ? 1 : 0), 0, nk, tile, t)) return;
+  const int c = t.c, S = D.lm;
+  const double spc = (c < S) ? D.sp[c] : 0.0;
+  for (int k = t.k0; k < (CH ? t.k1 : min(t.k1, t.k0 + 1)); ++k) {
+    if (MODE == 0) {
       D.hpp[IX(k, c)] = D.hp[IX(k, c)];
       D.hupp[IX(k, c)] = D.hup[IX(k, c)];
       D.hvpp[IX(k, c)] = D.hvp[IX(k, c)];
     }
-  }
-  if (c == D.lm) return;
-  layer_hp2_col(D, c);
-  for (int k = D.k1; k <= D.kmz[c]; ++k) {
-    if (MODE == 0) D.salpp[IX(k, c)] = D.salp[IX(k, c)];
-    double t = 0.5 * ((double)D.sal[IX(k, c)] + (double)D.salpp[IX(k, c)]);
-    D.salp[IX(k, c)] = (R)t;
-    D.rhop[IX(k, c)] = (R)(densty((double)(R)t, 0.00004, zdens(D, k, c)) - 1000.);
-  }
-  half_level_flux<R, true, MODE == 0>(D, c);
-  half_level_flux<R, false, MODE == 0>(D, c);
-  if (D.P.iturb > 0) {
-    for (int k = D.k1; k <= D.kmz[c]; ++k) {
+    if (c == S) continue;
+    const int kmz = D.kmz[c];
+    const bool wet = k >= D.k1 && k <= kmz;
+    if (wet) {
+      const double v = cell_h(D.zlevel, k, D.hhs[c], spc);
+      D.hp[IX(k, c)] = (v <= 0.0) ? R(0) : (R)v;
+    }
+    const int lE = D.lE[c], lN = D.lN[c];
+    if (lE != S && k >= D.k1 && k <= min(kmz, D.kmz[lE])) {
+      const double v = cell_h(D.zlevel, k, D.hhu[c], dmax(spc, D.sp[lE]));
+      D.hup[IX(k, c)] = (v <= 0.0) ? R(0) : (R)v;
+    }
+    if (lN != S && k >= D.k1 && k <= min(kmz, D.kmz[lN])) {
+      const double v = cell_h(D.zlevel, k, D.hhv[c], dmax(spc, D.sp[lN]));
+      D.hvp[IX(k, c)] = (v <= 0.0) ? R(0) : (R)v;
+    }
+    if (wet) {
+      if (MODE == 0) D.salpp[IX(k, c)] = D.salp[IX(k, c)];
+      const double tm = 0.5 * ((double)D.sal[IX(k, c)] + (double)D.salpp[IX(k, c)]);
+      D.salp[IX(k, c)] = (R)tm;
+      D.rhop[IX(k, c)] = (R)(densty((double)(R)tm, 0.00004, zdens(D, k, c)) - 1000.);
+    }
+    half_level_flux_cell<R, true, MODE == 0>(D, c, k);
+    half_level_flux_cell<R, false, MODE == 0>(D, c, k);
+    if (D.P.iturb > 0 && wet) {
       if (MODE == 0) {
         D.q2pp[IX(k, c)] = D.q2p[IX(k, c)];
         D.q2lpp[IX(k, c)] = D.q2lp[IX(k, c)];
@@ -1039,6 +1095,19 @@ __global__ void __launch_bounds__(kBlock) k_settrap_col(Dev<R> D) {
       D.q2p[IX(k, c)] = 0.5 * (D.q2[IX(k, c)] + D.q2pp[IX(k, c)]);
       D.q2lp[IX(k, c)] = 0.5 * (D.q2l[IX(k, c)] + D.q2lpp[IX(k, c)]);
     }
+  }
+}
+
+// settrap / settrap2 per column, after k_settrap_cell: top-layer indices, the
+// top-cell flux corrections and the turbulence fix of a dry top cell.
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_settrap_top(Dev<R> D) {
+  const int c = col();
+  if (c >= D.lm) return;
+  top_layer_indices(D, c);
+  half_level_flux_top<R, true>(D, c, D.k1u[c]);
+  half_level_flux_top<R, false>(D, c, D.k1v[c]);
+  if (D.P.iturb > 0) {
     const int k = D.k1z[c];
     if ((double)D.hpp[IX(k, c)] <= 0.0) {
       D.q2lp[IX(k + 1, c)] = D.q2lp[IX(k + 2, c)];
@@ -1047,14 +1116,14 @@ __global__ void __launch_bounds__(kBlock) k_settrap_col(Dev<R> D) {
   }
 }
 
-// save(istep) column part, si3d_procedures.f90:3621-3941 (+ save_2EqTvars).
-template <typename R, int ISTEP>
-__global__ void __launch_bounds__(kBlock) k_save_col(Dev<R> D) {
-  const int c = col();
-  if (c > D.lm) return;
-  if (ISTEP == 1) D.spp[c] = (c == D.lm) ? D.spp[c] : D.sp[c];
-  D.sp[c] = D.s[c];
-  for (int k = 0; k <= D.km1 + 1; ++k) {
+// save(istep), si3d_procedures.f90:3621-3941 (+ save_2EqTvars), per cell
+// (c = 0..lm, k = 0..km1+1; the reference loops whole columns, 3678-3703).
+template <typename R, int ISTEP, bool CH>
+__global__ void __launch_bounds__(kBlock) k_save_cell(Dev<R> D, int nk, int tile) {
+  CellTile t;
+  if (!cell_tile(D.lm + 1, 0, nk, tile, t)) return;
+  const int c = t.c;
+  for (int k = t.k0; k < (CH ? t.k1 : min(t.k1, t.k0 + 1)); ++k) {
     if (ISTEP == 1) {
       D.hpp[IX(k, c)] = D.hp[IX(k, c)];
       D.hupp[IX(k, c)] = D.hup[IX(k, c)];
@@ -1063,9 +1132,7 @@ __global__ void __launch_bounds__(kBlock) k_save_col(Dev<R> D) {
     D.hp[IX(k, c)] = D.h[IX(k, c)];
     D.hup[IX(k, c)] = D.hu[IX(k, c)];
     D.hvp[IX(k, c)] = D.hv[IX(k, c)];
-  }
-  if (D.P.iturb > 0) {
-    for (int k = 0; k <= D.km1 + 1; ++k) {
+    if (D.P.iturb > 0) {
       if (ISTEP == 1) {
         D.q2pp[IX(k, c)] = D.q2p[IX(k, c)];
         D.q2lpp[IX(k, c)] = D.q2lp[IX(k, c)];
@@ -1073,10 +1140,7 @@ __global__ void __launch_bounds__(kBlock) k_save_col(Dev<R> D) {
       D.q2p[IX(k, c)] = D.q2[IX(k, c)];
       D.q2lp[IX(k, c)] = D.q2l[IX(k, c)];
     }
-  }
-  if (c == D.lm) return;
-  toplayer_col(D, c);
-  for (int k = D.k1; k <= D.km; ++k) {
+    if (c == D.lm || k < D.k1 || k > D.km) continue;
     if (ISTEP == 1) {
       D.salpp[IX(k, c)] = D.salp[IX(k, c)];
       D.uhpp[IX(k, c)] = D.uhp[IX(k, c)];
@@ -1084,9 +1148,9 @@ __global__ void __launch_bounds__(kBlock) k_save_col(Dev<R> D) {
       D.upp[IX(k, c)] = D.up[IX(k, c)];
       D.vpp[IX(k, c)] = D.vp[IX(k, c)];
     }
-    const R t = D.sal[IX(k, c)];
-    D.salp[IX(k, c)] = t;
-    D.rhop[IX(k, c)] = (R)(densty((double)t, 0.00004, zdens(D, k, c)) - 1000.);
+    const R tm = D.sal[IX(k, c)];
+    D.salp[IX(k, c)] = tm;
+    D.rhop[IX(k, c)] = (R)(densty((double)tm, 0.00004, zdens(D, k, c)) - 1000.);
     const double hu = D.hup[IX(k, c)];
     if (hu > 0.0) {
       const R f = D.uh[IX(k, c)];
@@ -1104,6 +1168,18 @@ __global__ void __launch_bounds__(kBlock) k_save_col(Dev<R> D) {
       D.vhp[IX(k, c)] = D.vp[IX(k, c)] = D.v[IX(k, c)] = R(0);
     }
   }
+}
+
+// save per column, after k_save_cell: zeta levels, top-layer indices from the
+// new hp, and the turbulence fix of a dry top cell.
+template <typename R, int ISTEP>
+__global__ void __launch_bounds__(kBlock) k_save_top(Dev<R> D) {
+  const int c = col();
+  if (c > D.lm) return;
+  if (ISTEP == 1) D.spp[c] = (c == D.lm) ? D.spp[c] : D.sp[c];
+  D.sp[c] = D.s[c];
+  if (c == D.lm) return;
+  top_layer_indices(D, c);
   if (D.P.iturb > 0) {
     const int k1s = D.k1z[c];
     const bool guard = (ISTEP == 1) ? true : (D.km1 - k1s > 1);
@@ -1177,11 +1253,19 @@ static inline dim3 grid_for(int n) { return dim3((n + kBlock - 1) / kBlock); }
     if (c->timing) timer_end(c, _slot);                                     \
   } while (0)
 
-#define LAUNCH2(name, kern, n, nk, ...)                                     \
+// Per-cell kernel over layers k_first .. k_first+nk-1 in tiles described by the
+// tuned word c->kc[tid] (chunk size | order << 8); the kernel takes (..., nk,
+// tile) as its last arguments.
+#define LAUNCHC(name, kern1, kernc, tid, n, nk, ...)                        \
   do {                                                                      \
     int _slot = -1;                                                         \
+    const int _tile = c->kc[tid], _kc = _tile & 255;                        \
+    const int _nb = ((n) + kBlock - 1) / kBlock * (((nk) + _kc - 1) / _kc); \
     if (c->timing) timer_begin(c, name, &_slot);                            \
-    hipLaunchKernelGGL(kern, dim3((n + kBlock - 1) / kBlock, nk), dim3(kBlock), 0, c->stream, __VA_ARGS__); \
+    if (_kc == 1)                                                           \
+      hipLaunchKernelGGL(kern1, dim3(_nb), dim3(kBlock), 0, c->stream, __VA_ARGS__, (int)(nk), _tile); \
+    else                                                                    \
+      hipLaunchKernelGGL(kernc, dim3(_nb), dim3(kBlock), 0, c->stream, __VA_ARGS__, (int)(nk), _tile); \
     if (c->timing) timer_end(c, _slot);                                     \
   } while (0)
 
@@ -1207,25 +1291,25 @@ int phase(s3g_ctx *c, int ph, int n, int istep) {
     case 1:
       if (c->P.ihd > 1) LAUNCH("hmix", (k_hmix<R>), L, D);
       break;
-    case 2: LAUNCH2("exmom_x", (k_exmom<R, true>), L, c->km - 1, D, twodt1); break;
+    case 2: LAUNCHC("exmom_x", (k_exmom<R, true, false>), (k_exmom<R, true, true>), TILE_EXMOM_X, L, c->km - 1, D, twodt1); break;
     case 3: LAUNCH("matmom_x", (k_matmom<R, true>), L, D, tz); break;
-    case 4: LAUNCH2("exmom_y", (k_exmom<R, false>), L, c->km - 1, D, twodt1); break;
+    case 4: LAUNCHC("exmom_y", (k_exmom<R, false, false>), (k_exmom<R, false, true>), TILE_EXMOM_Y, L, c->km - 1, D, twodt1); break;
     case 5: LAUNCH("matmom_y", (k_matmom<R, false>), L, D, tz); break;
     case 6: LAUNCH("matcon", (k_matcon<R>), L, D, tz); break;
     case 7: {
       int rc = solve_free_surface<R>(c, tz);
-      LAUNCH2("slayer_h", (k_slayer_h<R>), L, c->km1, D);
+      LAUNCHC("slayer_h", (k_slayer_h<R, false>), (k_slayer_h<R, true>), TILE_SLAYER_H, L, c->km1, D);
       LAUNCH("slayer_h_sentinel", (k_slayer_h_sentinel<R>), c->km1 + 2, D);
       return rc;
     }
     case 8:
-      LAUNCH2("vel", (k_vel<R>), L, c->km - 1, D, tz);
+      LAUNCHC("vel", (k_vel<R, false>), (k_vel<R, true>), TILE_VEL, L, c->km - 1, D, tz);
       LAUNCH("continuity1", (k_continuity<R, 1>), L, D);
       break;
-    case 9: LAUNCH2("exsal_hflux", (k_exsal_hflux<R>), L, c->km - 1, D, twodt1);
+    case 9: LAUNCHC("exsal_hflux", (k_exsal_hflux<R, false>), (k_exsal_hflux<R, true>), TILE_HFLUX, L, c->km - 1, D, twodt1);
             LAUNCH("exsal_div", (k_exsal_imsal<R, true, false>), L, D, twodt1); break;
     case 10: LAUNCH("imsal", (k_exsal_imsal<R, false, true>), L, D, twodt1); break;
-    case 11: LAUNCH2("vel2", (k_vel2<R>), L, c->km - 1, D); break;
+    case 11: LAUNCHC("vel2", (k_vel2<R, false>), (k_vel2<R, true>), TILE_VEL2, L, c->km - 1, D); break;
     case 12:
       LAUNCH("smooth_zeta", (k_zeta_levels<R, 2>), L + 1, D);
       LAUNCH("smooth_col", (k_smooth_col<R>), L, D);
@@ -1239,31 +1323,111 @@ int phase(s3g_ctx *c, int ph, int n, int istep) {
 }
 
 // fd, si3d_procedures.f90:382-575.
+// Tile autotuner: for each per-cell kernel, time every layer-chunk size on the
+// current state (one untimed launch, then the best of three) and keep the
+// fastest.  The kernel's outputs are saved before and restored after, so
+// tuning leaves the state exactly as it found it.
+template <typename R>
+int autotune(s3g_ctx *c) {
+  Dev<R> &D = dev<R>(c);
+  const int L = c->lm;
+  const double tz = 1.0, twodt1 = c->P.twodt;
+  const size_t n3 = (size_t)c->KP * c->LP;
+  const int cand[] = {1, 2, 4, 8, 16, 256 + 1, 256 + 2, 256 + 4, 256 + 8};
+  const bool timing = c->timing;
+  c->timing = false;
+  R *save = nullptr;
+  if (hipMalloc(&save, 5 * n3 * sizeof(R)) != hipSuccess) {
+    c->timing = timing;
+    set_error("autotune: no device memory for the output snapshot");
+    return S3G_ERR_HIP;
+  }
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  for (int tid = 0; tid < TILE_TUNED; ++tid) {
+    if (tid == TILE_HFLUX && c->P.isal == 0) continue;
+    std::vector<R *> outs;
+    switch (tid) {
+      case TILE_EXMOM_X: outs = {D.ex}; break;
+      case TILE_EXMOM_Y: outs = {D.ey}; break;
+      case TILE_SLAYER_H: outs = {D.h, D.hu, D.hv}; break;
+      case TILE_VEL: outs = {D.uh, D.vh}; break;
+      case TILE_HFLUX: outs = {D.fx_ew, D.fx_ns}; break;
+      default: outs = {D.h, D.hu, D.hv, D.uh, D.vh}; break;
+    }
+    for (size_t i = 0; i < outs.size(); ++i)
+      hipMemcpyAsync(save + i * n3, outs[i], n3 * sizeof(R), hipMemcpyDeviceToDevice, c->stream);
+    auto launch = [&]() {
+      switch (tid) {
+        case TILE_EXMOM_X: LAUNCHC("", (k_exmom<R, true, false>), (k_exmom<R, true, true>), TILE_EXMOM_X, L, c->km - 1, D, twodt1); break;
+        case TILE_EXMOM_Y: LAUNCHC("", (k_exmom<R, false, false>), (k_exmom<R, false, true>), TILE_EXMOM_Y, L, c->km - 1, D, twodt1); break;
+        case TILE_SLAYER_H: LAUNCHC("", (k_slayer_h<R, false>), (k_slayer_h<R, true>), TILE_SLAYER_H, L, c->km1, D); break;
+        case TILE_VEL: LAUNCHC("", (k_vel<R, false>), (k_vel<R, true>), TILE_VEL, L, c->km - 1, D, tz); break;
+        case TILE_HFLUX: LAUNCHC("", (k_exsal_hflux<R, false>), (k_exsal_hflux<R, true>), TILE_HFLUX, L, c->km - 1, D, twodt1); break;
+        default: LAUNCHC("", (k_vel2<R, false>), (k_vel2<R, true>), TILE_VEL2, L, c->km - 1, D); break;
+      }
+    };
+    int best_kc = c->kc[tid];
+    float best = 1e30f;
+    for (int kc : cand) {
+      c->kc[tid] = kc;
+      launch();
+      for (int rep = 0; rep < 3; ++rep) {
+        hipEventRecord(e0, c->stream);
+        launch();
+        hipEventRecord(e1, c->stream);
+        hipEventSynchronize(e1);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) best = ms, best_kc = kc;
+      }
+    }
+    c->kc[tid] = best_kc;
+    for (size_t i = 0; i < outs.size(); ++i)
+      hipMemcpyAsync(outs[i], save + i * n3, n3 * sizeof(R), hipMemcpyDeviceToDevice, c->stream);
+    if (c->debug) std::fprintf(stderr, "[s3g] tile %d: kc=%d order=%d (%.3f ms)\n", tid, best_kc & 255, best_kc >> 8, best);
+  }
+  const int e = hip_check(hipStreamSynchronize(c->stream), "autotune");
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(save);
+  c->timing = timing;
+  return e;
+}
+
 template <typename R>
 int fd(s3g_ctx *c, int n, int istep, int lastiter) {
   (void)lastiter;
+  if (!c->tuned) {
+    c->tuned = true;
+    if (c->tune) {
+      const int e = autotune<R>(c);
+      if (e < 0) return e;
+    }
+  }
   const double tz = 1.0 / istep, twodt1 = c->P.twodt * tz;
   surfbc(c, n, istep);
   sync_forcing<R>(c);
   Dev<R> &D = dev<R>(c);
   const int L = c->lm;
   if (c->P.ihd > 1) LAUNCH("hmix", (k_hmix<R>), L, D);
-  LAUNCH2("exmom_x", (k_exmom<R, true>), L, c->km - 1, D, twodt1);
+  LAUNCHC("exmom_x", (k_exmom<R, true, false>), (k_exmom<R, true, true>), TILE_EXMOM_X, L, c->km - 1, D, twodt1);
   LAUNCH("matmom_x", (k_matmom<R, true>), L, D, tz);
-  LAUNCH2("exmom_y", (k_exmom<R, false>), L, c->km - 1, D, twodt1);
+  LAUNCHC("exmom_y", (k_exmom<R, false, false>), (k_exmom<R, false, true>), TILE_EXMOM_Y, L, c->km - 1, D, twodt1);
   LAUNCH("matmom_y", (k_matmom<R, false>), L, D, tz);
   LAUNCH("matcon", (k_matcon<R>), L, D, tz);
   int rc = solve_free_surface<R>(c, tz);
   if (rc < 0) return rc;
-  LAUNCH2("slayer_h", (k_slayer_h<R>), L, c->km1, D);
+  LAUNCHC("slayer_h", (k_slayer_h<R, false>), (k_slayer_h<R, true>), TILE_SLAYER_H, L, c->km1, D);
   LAUNCH("slayer_h_sentinel", (k_slayer_h_sentinel<R>), c->km1 + 2, D);
-  LAUNCH2("vel", (k_vel<R>), L, c->km - 1, D, tz);
+  LAUNCHC("vel", (k_vel<R, false>), (k_vel<R, true>), TILE_VEL, L, c->km - 1, D, tz);
   LAUNCH("continuity1", (k_continuity<R, 1>), L, D);
   if (c->P.isal != 0) {
-    LAUNCH2("exsal_hflux", (k_exsal_hflux<R>), L, c->km - 1, D, twodt1);
+    LAUNCHC("exsal_hflux", (k_exsal_hflux<R, false>), (k_exsal_hflux<R, true>), TILE_HFLUX, L, c->km - 1, D, twodt1);
     LAUNCH("exsal_imsal", (k_exsal_imsal<R, true, true>), L, D, twodt1);
   }
-  LAUNCH2("vel2", (k_vel2<R>), L, c->km - 1, D);
+  LAUNCHC("vel2", (k_vel2<R, false>), (k_vel2<R, true>), TILE_VEL2, L, c->km - 1, D);
   if (c->P.ismooth >= 1 && istep == 1) {
     LAUNCH("smooth_zeta", (k_zeta_levels<R, 2>), L + 1, D);
     LAUNCH("smooth_col", (k_smooth_col<R>), L, D);
@@ -1278,7 +1442,8 @@ int settrap(s3g_ctx *c) {
   Dev<R> &D = dev<R>(c);
   const int L = c->lm;
   LAUNCH("settrap_zeta", (k_zeta_levels<R, 0>), L + 1, D);
-  LAUNCH("settrap_col", (k_settrap_col<R, 0>), L + 1, D);
+  LAUNCHC("settrap_cell", (k_settrap_cell<R, 0, false>), (k_settrap_cell<R, 0, true>), TILE_SETTRAP, L + 1, c->km1 + 2, D);
+  LAUNCH("settrap_top", (k_settrap_top<R>), L, D);
   LAUNCH("continuity2", (k_continuity<R, 2>), L, D);
   return hip_check(hipGetLastError(), "settrap launch");
 }
@@ -1288,7 +1453,8 @@ int settrap2(s3g_ctx *c) {
   Dev<R> &D = dev<R>(c);
   const int L = c->lm;
   LAUNCH("settrap2_zeta", (k_zeta_levels<R, 1>), L + 1, D);
-  LAUNCH("settrap2_col", (k_settrap_col<R, 1>), L, D);
+  LAUNCHC("settrap2_cell", (k_settrap_cell<R, 1, false>), (k_settrap_cell<R, 1, true>), TILE_SETTRAP, L, c->km1 + 2, D);
+  LAUNCH("settrap_top", (k_settrap_top<R>), L, D);
   LAUNCH("continuity2", (k_continuity<R, 2>), L, D);
   return hip_check(hipGetLastError(), "settrap2 launch");
 }
@@ -1297,10 +1463,13 @@ template <typename R>
 int save(s3g_ctx *c, int istep) {
   Dev<R> &D = dev<R>(c);
   const int L = c->lm;
-  if (istep == 1)
-    LAUNCH("save_col", (k_save_col<R, 1>), L + 1, D);
-  else
-    LAUNCH("save_col", (k_save_col<R, 2>), L + 1, D);
+  if (istep == 1) {
+    LAUNCHC("save_cell", (k_save_cell<R, 1, false>), (k_save_cell<R, 1, true>), TILE_SAVE, L + 1, c->km1 + 2, D);
+    LAUNCH("save_top", (k_save_top<R, 1>), L + 1, D);
+  } else {
+    LAUNCHC("save_cell", (k_save_cell<R, 2, false>), (k_save_cell<R, 2, true>), TILE_SAVE, L + 1, c->km1 + 2, D);
+    LAUNCH("save_top", (k_save_top<R, 2>), L + 1, D);
+  }
   LAUNCH("continuity2", (k_continuity<R, 2>), L, D);
   return hip_check(hipGetLastError(), "save launch");
 }

@@ -172,6 +172,13 @@ class GpuLake:
     def timers(self, enable=True):
         check(self.L.s3g_reset_timers(self.h, 1 if enable else 0), "s3g_reset_timers")
 
+    TILE_NAMES = ("exmom_x", "exmom_y", "slayer_h", "vel", "exsal_hflux", "vel2", "settrap", "save")
+
+    def tiles(self):
+        kc = (C.c_int * 16)()
+        n = check(self.L.s3g_get_tiles(self.h, 16, kc), "s3g_get_tiles")
+        return {self.TILE_NAMES[i]: [kc[i] & 255, kc[i] >> 8] for i in range(n)}
+
     def kernel_times(self, maxn=64):
         names = (C.c_char * 32 * maxn)()
         avg = (C.c_double * maxn)()
