@@ -142,6 +142,47 @@ int s3g_get_tiles(s3g_ctx *c, int max, int *kc);
 void s3g_finalize(s3g_ctx *c);
 const char *s3g_last_error(void);
 
+/* --- Learned initial guess of the free-surface solve ---------------------- */
+/* No counterpart in the reference (its solve is NSPCG, si3d_procedures.f90:
+ * 1870-2065).  A U-Net maps the normalised residual of the plain initial
+ * guess, r0/sigma (r0 = qq - A zeta_guess, sigma = rms r0), the coupling
+ * field (rr-1)/rr and the wet mask to (zeta - zeta_guess)/sigma.  The CG
+ * that follows still stops at the same FP64 true-residual tolerance. */
+
+typedef struct {
+  int pairs, capacity;   /* training pairs recorded / ring capacity          */
+  long train_steps;      /* Adam steps taken                                  */
+  double last_loss;      /* mean masked square error of the last step         */
+  int height, width;     /* dense box of the network (multiples of 8)         */
+  int mode;              /* 0 plain guess, 1 learned guess                    */
+  long guess_solves;     /* solves with the learned guess since the mode was set */
+  double guess_error_ratio; /* mean ||x - x_learned||_A / ||x - x_plain||_A over
+                              those solves (x: the converged solution)        */
+} s3g_learn_stats;
+
+/* Allocate the network (He-initialised from `seed`) and a device ring of
+ * `capacity` training pairs; from now on every free-surface solve records
+ * (r0/sigma, coupling, (zeta-guess)/sigma) while recording is on. */
+int s3g_learn_setup(s3g_ctx *c, int capacity, unsigned seed);
+int s3g_learn_record(s3g_ctx *c, int on);
+/* `steps` Adam steps (learning rate lr) on minibatches of `batch` pairs drawn
+ * from the ring, forward and backward on the GPU; *loss_out = mean loss of
+ * the last step. */
+int s3g_learn_train(s3g_ctx *c, int steps, int batch, double lr, double *loss_out);
+/* 0: plain initial guess (zeta of the latest solve); 1: plus the network's
+ * correction. */
+int s3g_learn_mode(s3g_ctx *c, int mode);
+long s3g_learn_num_weights(void);
+int s3g_learn_get_weights(s3g_ctx *c, float *host, long n);
+int s3g_learn_set_weights(s3g_ctx *c, const float *host, long n);
+int s3g_learn_info(s3g_ctx *c, s3g_learn_stats *out);
+/* Forward + backward of the network with the given weights on one dense
+ * H x W input (x3: 3 planes of H*W), loss = sum mask (out-target)^2 / sum
+ * mask; writes out (H*W), the weight gradients (same order as the weights)
+ * and the loss.  For gradient checks. */
+int s3g_unet_check(int H, int W, const float *weights, const float *x3, const float *target, const float *mask,
+                   float *out, float *grads, double *loss);
+
 /* --- Standalone driver (no Fortran host) --------------------------------- */
 
 /* Restates input/bathy/init on the host and calls s3g_init + s3g_set_field_*. */

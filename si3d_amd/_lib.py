@@ -20,6 +20,8 @@ EXPORTS = [
     "s3g_settrap", "s3g_settrap2", "s3g_save", "s3g_step", "s3g_run", "s3g_phase",
     "s3g_synchronize", "s3g_get_stats", "s3g_kernel_times", "s3g_reset_timers", "s3g_get_tiles",
     "s3g_finalize", "s3g_last_error", "s3g_create_from_deck",
+    "s3g_learn_setup", "s3g_learn_record", "s3g_learn_train", "s3g_learn_mode", "s3g_learn_num_weights",
+    "s3g_learn_get_weights", "s3g_learn_set_weights", "s3g_learn_info", "s3g_unet_check",
 ]
 
 dp = C.POINTER(C.c_double)
@@ -60,6 +62,12 @@ class S3gStats(C.Structure):
                 ("lm", C.c_int), ("km1", C.c_int), ("n_wet", C.c_long)]
 
 
+class S3gLearnStats(C.Structure):
+    _fields_ = [("pairs", C.c_int), ("capacity", C.c_int), ("train_steps", C.c_long),
+                ("last_loss", C.c_double), ("height", C.c_int), ("width", C.c_int), ("mode", C.c_int),
+                ("guess_solves", C.c_long), ("guess_error_ratio", C.c_double)]
+
+
 _lib = None
 
 
@@ -89,6 +97,16 @@ def load():
     L.s3g_kernel_times.argtypes = [vp, C.c_int, C.c_void_p, dp, C.POINTER(C.c_long)]
     L.s3g_reset_timers.argtypes = [vp, C.c_int]
     L.s3g_get_tiles.argtypes = [vp, C.c_int, ip]
+    L.s3g_learn_setup.argtypes = [vp, C.c_int, C.c_uint]
+    L.s3g_learn_record.argtypes = [vp, C.c_int]
+    L.s3g_learn_train.argtypes = [vp, C.c_int, C.c_int, C.c_double, dp]
+    L.s3g_learn_mode.argtypes = [vp, C.c_int]
+    L.s3g_learn_num_weights.restype = C.c_long
+    L.s3g_learn_num_weights.argtypes = []
+    L.s3g_learn_get_weights.argtypes = [vp, fp, C.c_long]
+    L.s3g_learn_set_weights.argtypes = [vp, fp, C.c_long]
+    L.s3g_learn_info.argtypes = [vp, C.POINTER(S3gLearnStats)]
+    L.s3g_unet_check.argtypes = [C.c_int, C.c_int, fp, fp, fp, fp, fp, fp, dp]
     L.s3g_finalize.argtypes = [vp]
     L.s3g_finalize.restype = None
     L.s3g_last_error.restype = C.c_char_p

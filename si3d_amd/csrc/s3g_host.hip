@@ -444,6 +444,8 @@ int s3g_init(const s3g_grid *g, const s3g_params *p, s3g_ctx **out) {
   float **pf[] = {&G.rf, &G.rf1, &G.zf, &G.pf0, &G.pf1, &G.qf, &G.sxf, &G.syf, &G.rrf, &G.bf, &G.df, &G.lcf, &G.ldf};
   for (float **q : pf)
     if (rc == 0 && !(*q = dalloc<float>(c, c->precision == S3G_MIXED ? n2 : 1))) rc = S3G_ERR_HIP;
+  c->l2i.assign(g->l2i, g->l2i + c->lm);
+  c->l2j.assign(g->l2j, g->l2j + c->lm);
   if (rc == 0) rc = pcg_setup(c, lN.data(), g->l2i, g->l2j);
   if (rc == 0 && !(G.part = dalloc<double>(c, 8 * (size_t)G.nblk + 64))) rc = S3G_ERR_HIP;
   if (rc == 0 && !(G.iters = dalloc<int>(c, 4))) rc = S3G_ERR_HIP;
@@ -582,6 +584,7 @@ int s3g_kernel_times(s3g_ctx *c, int max, char (*names)[32], double *avg_ms, lon
 void s3g_finalize(s3g_ctx *c) {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
+  s3g::learn_destroy(c);
   for (void *p : c->allocs) hipFree(p);
   for (auto &e : c->ev_pool) {
     hipEventDestroy(e.first);

@@ -79,6 +79,18 @@ struct Pcg {
   std::vector<MgLevel> mg;  // mg[0] unused (the fine grid), mg[1..] coarse levels
 };
 
+// FP64 view of the free-surface system for the outer solver level (true
+// residual, refinement, learned initial guess).
+struct Outer {
+  const int *lE, *lW, *lN, *lS;
+  const double *rr, *sx, *sy, *b;
+  double *x;
+  int n, nblk;
+  double *part;
+};
+
+struct Learner;  // s3g_unet.hip
+
 struct Timer {
   std::string name;
   double total_ms = 0.0;
@@ -112,7 +124,7 @@ struct s3g_ctx {
   s3g::Dev<double> d64{};
   s3g::Dev<float> d32{};
   s3g::Pcg pcg{};
-  int *d_l2i = nullptr, *d_l2j = nullptr;
+  std::vector<int> l2i, l2j;  // (i, j) of each wet column (host copy)
   // statistics
   s3g_stats stats{};
   // timers
@@ -127,6 +139,9 @@ struct s3g_ctx {
   // unless S3G_TUNE=0)
   int kc[s3g::TILE_NUM] = {1, 1, 1, 1, 1, 1, 1, 1};
   bool tune = true, tuned = false;
+  // learned initial guess (s3g_learn_setup)
+  s3g::Learner *learn = nullptr;
+  std::vector<void *> learn_allocs;
 };
 
 namespace s3g {
@@ -140,6 +155,10 @@ int init_density(s3g_ctx *c);
 template <typename R> int autotune(s3g_ctx *c);
 // pcg.hip
 template <typename R> int solve_free_surface(s3g_ctx *c, double tz);
+// unet.hip
+int learn_before_solve(s3g_ctx *c, const void *outer, double *x);
+int learn_after_solve(s3g_ctx *c, const void *outer);
+void learn_destroy(s3g_ctx *c);
 int pcg_setup(s3g_ctx *c, const int *lN0, const int *l2i, const int *l2j);
 // host
 void set_error(const std::string &msg);

@@ -501,14 +501,6 @@ __global__ void __launch_bounds__(1024) k_mg_chain(MgChain<T> Ch) {
 }
 
 // ---- FP64 outer level (zeta, qq, true residual) ---------------------------
-struct Outer {
-  const int *lE, *lW, *lN, *lS;
-  const double *rr, *sx, *sy, *b;
-  double *x;
-  int n, nblk;
-  double *part;
-};
-
 __global__ void __launch_bounds__(kBlock) k_start_guess(double *x, const double *sp, int n) {
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c <= n; c += gridDim.x * blockDim.x) x[c] = (c < n) ? sp[c] : 0.0;
 }
@@ -868,6 +860,7 @@ int solve_free_surface(s3g_ctx *c, double tz) {
   // (1970); s is the same field on leapfrog steps and the previous estimate of
   // zeta^{n+1} on trapezoidal iterations, so it only shortens the iteration.
   hipLaunchKernelGGL(k_start_guess, g, b, 0, c->stream, P.x, (const double *)D.s, P.n);
+  if (c->learn && learn_before_solve(c, &O, P.x) < 0) return S3G_ERR_HIP;
 
   if (c->precision == S3G_FP64) {
     CgArgs<double> A = cg_args<double>(c, D.lE, D.lW, D.lN, D.lS, D.rr, D.sx, D.sy, P.x, P.r, P.r1, P.z, P.p0, P.p1, P.q,
@@ -917,6 +910,7 @@ int solve_free_surface(s3g_ctx *c, double tz) {
     set_error("free-surface solve produced a non-finite residual");
     return S3G_ERR_SOLVER;
   }
+  if (c->learn && learn_after_solve(c, &O) < 0) return S3G_ERR_HIP;
   hipLaunchKernelGGL(k_copy_to_s, dim3((P.n + 255) / 256), b, 0, c->stream, D.s, P.x, P.n);
   if (c->debug)
     std::fprintf(stderr, "[s3g] solve %ld: %s %s iters=%d relres=%.3e rc=%d\n", c->stats.solves,

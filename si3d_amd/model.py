@@ -17,7 +17,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import S3gDeck, S3gGrid, S3gParams, S3gStats, check
+from ._lib import S3gDeck, S3gGrid, S3gLearnStats, S3gParams, S3gStats, check
 
 _PREC = {"fp64": _lib.S3G_FP64, "mixed": _lib.S3G_MIXED}
 _PRECOND = {"jacobi": _lib.S3G_PREC_JACOBI, "line": _lib.S3G_PREC_LINE, "mg": _lib.S3G_PREC_MG}
@@ -172,6 +172,35 @@ class GpuLake:
     def timers(self, enable=True):
         check(self.L.s3g_reset_timers(self.h, 1 if enable else 0), "s3g_reset_timers")
 
+    # -- learned initial guess ---------------------------------------------
+    def learn_setup(self, capacity=64, seed=1):
+        return check(self.L.s3g_learn_setup(self.h, capacity, seed), "s3g_learn_setup")
+
+    def learn_record(self, on=True):
+        return check(self.L.s3g_learn_record(self.h, 1 if on else 0), "s3g_learn_record")
+
+    def learn_train(self, steps, batch=4, lr=1e-3):
+        loss = C.c_double()
+        check(self.L.s3g_learn_train(self.h, steps, batch, lr, C.byref(loss)), "s3g_learn_train")
+        return loss.value
+
+    def learn_mode(self, mode):
+        return check(self.L.s3g_learn_mode(self.h, mode), "s3g_learn_mode")
+
+    def learn_weights(self, values=None):
+        n = self.L.s3g_learn_num_weights()
+        if values is None:
+            out = np.empty(n, dtype=np.float32)
+            check(self.L.s3g_learn_get_weights(self.h, out.ctypes.data_as(_lib.fp), n), "s3g_learn_get_weights")
+            return out
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        check(self.L.s3g_learn_set_weights(self.h, v.ctypes.data_as(_lib.fp), n), "s3g_learn_set_weights")
+
+    def learn_info(self):
+        st = S3gLearnStats()
+        check(self.L.s3g_learn_info(self.h, C.byref(st)), "s3g_learn_info")
+        return {name: getattr(st, name) for name, _ in S3gLearnStats._fields_}
+
     TILE_NAMES = ("exmom_x", "exmom_y", "slayer_h", "vel", "exsal_hflux", "vel2", "settrap", "save")
 
     def tiles(self):
@@ -185,3 +214,21 @@ class GpuLake:
         cnt = (C.c_long * maxn)()
         n = check(self.L.s3g_kernel_times(self.h, maxn, names, avg, cnt), "s3g_kernel_times")
         return {bytes(names[i]).split(b"\0")[0].decode(): (avg[i], cnt[i]) for i in range(n)}
+
+
+def unet_check(weights, x3, target, mask):
+    """Forward + backward of the library's U-Net on one dense input (x3: 3 x H x
+    W); returns (out H x W, weight gradients, loss)."""
+    L = _lib.load()
+    _, H, W = x3.shape
+    w = np.ascontiguousarray(weights, dtype=np.float32)
+    x = np.ascontiguousarray(x3, dtype=np.float32)
+    t = np.ascontiguousarray(target, dtype=np.float32)
+    m = np.ascontiguousarray(mask, dtype=np.float32)
+    out = np.empty((H, W), dtype=np.float32)
+    grads = np.empty(L.s3g_learn_num_weights(), dtype=np.float32)
+    loss = C.c_double()
+    check(L.s3g_unet_check(H, W, w.ctypes.data_as(_lib.fp), x.ctypes.data_as(_lib.fp), t.ctypes.data_as(_lib.fp),
+                           m.ctypes.data_as(_lib.fp), out.ctypes.data_as(_lib.fp), grads.ctypes.data_as(_lib.fp),
+                           C.byref(loss)), "s3g_unet_check")
+    return out, grads, loss.value

@@ -1,0 +1,79 @@
+"""The learned initial guess on the GPU: the network's forward pass, loss and
+every weight gradient against the CPU oracle (oracle/unet_oracle.py, same
+bf16 rounding points), and what the trained network gains over the plain
+initial guess on a lake, at the same converged error."""
+import numpy as np
+import pytest
+
+from oracle import unet_oracle as U
+from tests.helpers import relerr, small_ellipse
+
+pytestmark = pytest.mark.gpu
+
+# bf16 activations on both sides; only the FP32 summation order differs, which
+# can flip the rounding of single bf16 values (2^-8 relative).
+OUT_TOL = 1e-2
+GRAD_TOL = 3e-2
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+@pytest.mark.parametrize("H,W,seed", [(16, 24, 7), (32, 8, 8)])
+def test_unet_forward_backward_match_oracle(H, W, seed):
+    from si3d_amd.model import unet_check
+    rng = np.random.default_rng(seed)
+    wv = U.init_weights(seed + 4)
+    x3 = rng.normal(size=(3, H, W)).astype(np.float32)
+    target = rng.normal(size=(H, W)).astype(np.float32)
+    mask = (rng.random((H, W)) > 0.3).astype(np.float32)
+    print(f"unet check H={H} W={W} seed={seed}", flush=True)
+    out, g, loss = unet_check(wv, x3, target, mask)
+    o_ref, l_ref, g_ref = U.forward_backward(wv, x3, target, mask, bf16=True)
+    assert _rel(out, o_ref) < OUT_TOL
+    assert abs(loss - l_ref) <= OUT_TOL * abs(l_ref)
+    o = 0
+    for i, (ci, co) in enumerate(U.CFG):
+        for what, n in (("w", 9 * ci * co), ("b", co)):
+            e = _rel(g[o:o + n], g_ref[o:o + n])
+            assert e < GRAD_TOL, (i, what, e)
+            o += n
+    assert _rel(g[o:], g_ref[o:]) < GRAD_TOL
+
+
+def test_learned_guess_gain_same_error():
+    """Two identical lakes run 20 steps; lake A records the 60 solves and
+    trains its network on them; both then run 10 more steps, A with the
+    learned initial guess.  The fields agree to the solver tolerance (same
+    error), and A's learned guess starts closer to the solution in the energy
+    norm that CG minimises."""
+    from si3d_amd.model import GpuLake
+    deck = small_ellipse(nts=40)
+    A = GpuLake(deck, preconditioner="mg", tol=1e-10)
+    B = GpuLake(deck, preconditioner="mg", tol=1e-10)
+    A.learn_setup(capacity=64, seed=3)
+    for n in range(1, 21):
+        A.step(n)
+        B.step(n)
+    info = A.learn_info()
+    assert info["pairs"] == 59  # 2 solves on step 1, 3 on each later step
+    loss0 = A.learn_train(1, batch=4, lr=0.0)
+    loss1 = A.learn_train(400, batch=4, lr=1e-3)
+    print(f"energy-norm loss before {loss0:.4e} after {loss1:.4e}", flush=True)
+    assert loss1 < loss0
+    A.learn_record(False)
+    A.learn_mode(1)
+    ia0, ib0 = A.stats()["pcg_iters_total"], B.stats()["pcg_iters_total"]
+    for n in range(21, 31):
+        A.step(n)
+        B.step(n)
+    ia = A.stats()["pcg_iters_total"] - ia0
+    ib = B.stats()["pcg_iters_total"] - ib0
+    info = A.learn_info()
+    print(f"PCG iterations over 10 steps: learned {ia}, plain {ib}; "
+          f"energy-norm error ratio of the learned guess {info['guess_error_ratio']:.3f}", flush=True)
+    assert info["guess_solves"] == 30
+    assert info["guess_error_ratio"] < 1.0
+    for name in ("s", "uh", "vh", "sal"):
+        assert relerr(A.get(name), B.get(name)) < 1e-6, name
