@@ -47,6 +47,17 @@ ARRAYS_PER_CELL = {
 }
 
 
+# HBM bytes per wet cell per launch measured with rocprofv3 FETCH_SIZE /
+# WRITE_SIZE (scaled by a 512 MiB device copy) on the default workload in FP64:
+# profiles/hbm_traffic_ellipse_1024x512x60_fp64.md.  Reported as
+# roofline.traffic for that workload only.
+TRAFFIC_FP64_1024x512x60 = {
+    "turbulence": 337.2, "exsal_imsal": 226.7, "exmom_x": 156.6, "exmom_y": 217.4, "matmom_x": 152.8,
+    "matmom_y": 128.7, "settrap_cell": 445.9, "settrap2_cell": 281.0, "save_cell": 294.5, "exsal_hflux": 138.2,
+    "continuity1": 107.6, "vel": 75.3, "vel2": 64.9, "slayer_h": 62.7, "continuity2": 49.1, "matcon": 29.0,
+}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,6 +72,10 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-timers", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-initial-guess leg")
+    ap.add_argument("--train-steps", type=int, default=300)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (calibrates HBM byte counters) and exit")
     return ap.parse_args()
 
 
@@ -77,8 +92,25 @@ def cpu_baseline(deck, nsteps):
     return dt, o
 
 
+def copy_probe(mib):
+    """A plain device-to-device copy of known size, for scaling the FETCH_SIZE /
+    WRITE_SIZE counters of the profiled kernels."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_mib": mib, "bytes_read_per_copy": n * 4, "bytes_written_per_copy": n * 4}))
+
+
 def main():
     a = parse()
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {a.gpus} is not supported "
               "(the multi-GPU decomposition is described in DESIGN.md)", file=sys.stderr)
@@ -92,9 +124,13 @@ def main():
     t_setup = time.perf_counter()
     g = GpuLake(deck, precision=a.precision, preconditioner=a.precond, tol=a.tol)
     t_setup = time.perf_counter() - t_setup
+    if not a.no_learned:
+        g.learn_setup(capacity=256, seed=1)  # the warmup solves record training pairs
     for n in range(1, a.warmup + 1):
         g.step(n)
     g.synchronize()
+    if not a.no_learned:
+        g.learn_record(False)
     st0 = g.stats()
     if not a.no_timers:
         g.timers(True)
@@ -110,6 +146,34 @@ def main():
 
     value = n_wet * a.steps / elapsed
     iters = (st1["pcg_iters_total"] - st0["pcg_iters_total"]) / a.steps
+
+    learned = None
+    if not a.no_learned:
+        # learned initial guess: train on the recorded pairs (untimed), then the
+        # same number of steps from where the plain leg stopped
+        t_tr = time.perf_counter()
+        loss0 = g.learn_train(1, batch=4, lr=0.0)
+        loss1 = g.learn_train(a.train_steps, batch=4, lr=1e-3)
+        t_tr = time.perf_counter() - t_tr
+        g.learn_mode(1)
+        g.synchronize()
+        sl0 = g.stats()
+        t1 = time.perf_counter()
+        n1 = n0 + a.steps
+        for n in range(n1, n1 + a.steps):
+            g.step(n)
+        g.synchronize()
+        el1 = time.perf_counter() - t1
+        sl1 = g.stats()
+        info = g.learn_info()
+        learned = {"value": round(n_wet * a.steps / el1, 1), "unit": "wet-cell-steps/s",
+                   "ms_per_step": round(1e3 * el1 / a.steps, 4),
+                   "pcg_iters_per_step": round((sl1["pcg_iters_total"] - sl0["pcg_iters_total"]) / a.steps, 2),
+                   "pcg_relres_last": sl1["pcg_relres_last"],
+                   "guess_error_ratio": round(info["guess_error_ratio"], 4),
+                   "pairs": info["pairs"], "train_steps": info["train_steps"],
+                   "train_s": round(t_tr, 2), "loss_before": loss0, "loss_after": loss1,
+                   "net": f"U-Net 4 levels, 32 base channels, bf16 MFMA, {info['height']}x{info['width']}"}
     elem = 8 if a.precision == "fp64" else 4
 
     roofline = None
@@ -128,9 +192,12 @@ def main():
             n_r, n_f64 = ARRAYS_PER_CELL[name]
             bytes_launch = (n_r * elem + n_f64 * 8) * n_wet
             achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+            traffic = None
+            if (a.nx, a.ny, a.layers, a.precision) == (1024, 512, 60, "fp64") and name in TRAFFIC_FP64_1024x512x60:
+                traffic = int(TRAFFIC_FP64_1024x512x60[name] * n_wet)
             roofline = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                        "traffic": None, "bytes_per_launch": bytes_launch,
+                        "traffic": traffic, "bytes_per_launch": bytes_launch,
                         "avg_launch_ms": round(avg_ms, 5)}
 
     cpu = None
@@ -162,6 +229,7 @@ def main():
         "setup_s": round(t_setup, 2),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "learned": learned,
         "tiles": g.tiles(),
         "kernels": kernel_summary,
     }
