@@ -802,25 +802,43 @@ __global__ void __launch_bounds__(kBlock) k_turb(Dev<R> D, int istep) {
   if (nwl <= 1) return;
   const double twodt1 = D.P.twodt / istep, r3 = 3. / twodt1;
   const int lW = D.lW[m], lS = D.lS[m];
-  double *XL = D.d0, *SP = D.d1, *BP = D.d2;
-  double htot = 0.0, hbelow = 0.0;
-  for (int k = k1s; k <= kms; ++k) htot += D.hp[IX(k, m)];
+  double *E1 = D.d0, *E2 = D.d1;  // forward-sweep factors of the two systems
+  double hbelow = 0.0;
   for (int k = k1s + 1; k <= kms; ++k) hbelow += D.hp[IX(k, m)];
-  // pass 1: N^2, shear, limited length scale, productions
-  for (int k = k1s + 1; k <= kms; ++k) {
+  // Interface quantities of level k (k1s < k <= kms) from the p-level fields:
+  // the limited length scale, shear and buoyancy production (the TKE source
+  // terms, 411-440) and the new eddy coefficients (245-290).  Av(k)/Dv(k) are
+  // read for the productions before the new values replace them.
+  struct Lev {
+    double xl, sp, bp, av, dv;
+  };
+  auto level = [&](int k) -> Lev {
+    Lev L;
     const double hkm = D.hp[IX(k - 1, m)], hk = D.hp[IX(k, m)];
     const double dz = 0.5 * (hkm + hk);
     const double rkm = D.rhop[IX(k - 1, m)], rk = D.rhop[IX(k, m)];
     const double N2 = -G / (0.5 * (rkm + rk) + 1000.) * ((rkm - rk) / dz);
     const double dudz = 0.5 * ((double)D.up[IX(k - 1, m)] + (double)D.up[IX(k - 1, lW)] - (double)D.up[IX(k, m)] - (double)D.up[IX(k, lW)]) / dz;
     const double dvdz = 0.5 * ((double)D.vp[IX(k - 1, m)] + (double)D.vp[IX(k - 1, lS)] - (double)D.vp[IX(k, m)] - (double)D.vp[IX(k, lS)]) / dz;
-    const double q2pk = D.q2p[IX(k, m)];
-    double xl = D.q2lp[IX(k, m)] / q2pk;
+    const double q2pk = D.q2p[IX(k, m)], q2lpk = D.q2lp[IX(k, m)];
+    double xl = q2lpk / q2pk;
     if (N2 > 0.0) xl = dmin(xl, 0.53 * sqrt(q2pk / N2));
-    XL[IX(k, m)] = xl;
-    SP[IX(k, m)] = (double)D.Av[IX(k, m)] * (dudz * dudz + dvdz * dvdz);
-    BP[IX(k, m)] = -(double)D.Dv[IX(k, m)] * N2;
-  }
+    L.xl = xl;
+    L.sp = (double)D.Av[IX(k, m)] * (dudz * dudz + dvdz * dvdz);
+    L.bp = -(double)D.Dv[IX(k, m)] * N2;
+    // eddy coefficients (UpdateMixingCoefficients evaluates N^2 in this order)
+    const double N2b = -G * ((rkm - rk) / dz) / (0.5 * (rkm + rk) + 1000.);
+    double xlb = q2lpk / q2pk;
+    if (N2b > 0.0) xlb = dmin(xlb, 0.53 * sqrt(q2pk / N2b));
+    double Gh = -N2b * xlb * xlb / q2pk;
+    Gh = dmax(GH_MIN, dmin(GH_MAX, Gh));
+    const double Sh = D.P.t_1 / (1. - Gh * D.P.t_2);
+    const double Sm = (D.P.t_3 + Sh * Gh * D.P.t_4) / (1. - D.P.t_5 * Gh);
+    const double Rt = sqrt(q2pk) * xlb;
+    L.av = Rt * Sm + AV_MOL;
+    L.dv = Rt * Sh + DV_MOL;
+    return L;
+  };
   // boundary values (445-457)
   const double b1_23 = pow(B_1, 2. / 3.);
   double ub = 0.5 * ((double)D.u[IX(kms, m)] + (double)D.u[IX(kms, lW)]);
@@ -830,28 +848,32 @@ __global__ void __launch_bounds__(kBlock) k_turb(Dev<R> D, int istep) {
   double vs = 0.5 * ((double)D.vp[IX(D.k1v[m], m)] + (double)D.vp[IX(D.k1v[lS], lS)]);
   const double ds_top = 3. * b1_23 *
                         (D.cdw * RHOAIR * ((D.uair - us) * (D.uair - us) + (D.vair - vs) * (D.vair - vs)) / 1000.) / twodt1;
-  // pass 2: rows k1s..kms+1 of both systems, forward e/f sweeps together.
-  // Row k1s and row kms+1 are the Dirichlet rows (a1 = a3 = 0, a2 = 3/twodt1).
+  const bool thin_top = (double)D.hp[IX(k1s, m)] < 1.E-2;
+  // One forward sweep over rows k1s..kms of both systems (row k1s and row
+  // kms+1 are the Dirichlet rows: a1 = a3 = 0, a2 = 3/twodt1), with the
+  // interface quantities of level k+1 computed one level ahead.
   double e1 = 0.0, f1 = ds_top / r3, e2 = 0.0, f2 = 0.0;  // row k1s
   D.q2[IX(k1s, m)] = f1;
   D.q2l[IX(k1s, m)] = f2;
-  XL[IX(k1s, m)] = 0.0;  // xl(k1s) = 0 (e1 of row k1s)
-  SP[IX(k1s, m)] = 0.0;  // e2 of row k1s
-  double xlk = 0.0, dco_prev = 0.0, zfromt = 0.0;
-  {
-    // dco(k1s) = Sq*0.5*(xl(k1s)+xl(k1s+1))*sqrt(0.5*(q2p(k1s)+q2p(k1s+1)))
-    const double xl1 = (k1s + 1 <= kms) ? XL[IX(k1s + 1, m)] : 0.0;
-    dco_prev = SQ * 0.5 * (0.0 + xl1) * sqrt(0.5 * (D.q2p[IX(k1s, m)] + D.q2p[IX(k1s + 1, m)]));
-  }
+  E1[IX(k1s, m)] = 0.0;
+  E2[IX(k1s, m)] = 0.0;
+  Lev cur = level(k1s + 1);
+  double q2p_next = D.q2p[IX(k1s + 1, m)];
+  double dco_prev = SQ * 0.5 * (0.0 + cur.xl) * sqrt(0.5 * (D.q2p[IX(k1s, m)] + q2p_next));
+  double zfromt = 0.0;
+  double hkm = D.hp[IX(k1s, m)];
   for (int k = k1s + 1; k <= kms; ++k) {
-    xlk = XL[IX(k, m)];
-    const double xlp = (k + 1 <= kms) ? XL[IX(k + 1, m)] : 0.0;
-    const double q2pk = D.q2p[IX(k, m)];
-    const double dco = SQ * 0.5 * (xlk + xlp) * sqrt(0.5 * (q2pk + D.q2p[IX(k + 1, m)]));
-    const double hk = D.hp[IX(k, m)], hkm = D.hp[IX(k - 1, m)];
+    const bool more = k + 1 <= kms;
+    Lev nxt{};
+    if (more) nxt = level(k + 1);
+    const double xlk = cur.xl, xlp = more ? nxt.xl : 0.0;
+    const double q2pk = q2p_next;
+    q2p_next = D.q2p[IX(k + 1, m)];
+    const double dco = SQ * 0.5 * (xlk + xlp) * sqrt(0.5 * (q2pk + q2p_next));
+    const double hk = D.hp[IX(k, m)];
     const double a3 = -dco / (hk * 0.5 * (hk + hkm));
     const double a1 = -dco_prev / (hkm * 0.5 * (hk + hkm));
-    const double shp = SP[IX(k, m)], byp = BP[IX(k, m)];
+    const double shp = cur.sp, byp = cur.bp;
     const double sq = sqrt(q2pk);
     // TKE row
     double a2 = r3 - a1 - a3 + 2. * (sq / (B_1 * xlk));
@@ -892,46 +914,32 @@ __global__ void __launch_bounds__(kBlock) k_turb(Dev<R> D, int istep) {
     den = a2l + a1 * e2;
     e2 = -a3 / den;
     f2 = (dsl - a1 * f2) / den;
-    XL[IX(k, m)] = e1;  // xl(k) no longer needed
-    SP[IX(k, m)] = e2;
+    E1[IX(k, m)] = e1;
+    E2[IX(k, m)] = e2;
     D.q2[IX(k, m)] = f1;
     D.q2l[IX(k, m)] = f2;
+    // new eddy coefficients of interface k (the surface minimum, 281-284)
+    double av = cur.av, dv = cur.dv;
+    if (k == k1s + 1 && thin_top) av = dmax((double)(R)av, 1.E-4), dv = dmax((double)(R)dv, 1.E-4);
+    D.Av[IX(k, m)] = (R)av;
+    D.Dv[IX(k, m)] = (R)dv;
     dco_prev = dco;
+    hkm = hk;
+    cur = nxt;
   }
-  // bottom Dirichlet row kms+1 (a1 = 0)
+  // bottom Dirichlet row kms+1 (a1 = 0) and back substitution
   double x1 = ds_bot / r3, x2 = 0.0;
   D.q2[IX(kms + 1, m)] = dmax(Q2_MIN, x1);
   D.q2l[IX(kms + 1, m)] = dmax(Q2L_MIN, x2);
   for (int k = kms; k >= k1s; --k) {
-    x1 = XL[IX(k, m)] * x1 + D.q2[IX(k, m)];
-    x2 = SP[IX(k, m)] * x2 + D.q2l[IX(k, m)];
+    x1 = E1[IX(k, m)] * x1 + D.q2[IX(k, m)];
+    x2 = E2[IX(k, m)] * x2 + D.q2l[IX(k, m)];
     D.q2[IX(k, m)] = dmax(Q2_MIN, x1);
     D.q2l[IX(k, m)] = dmax(Q2L_MIN, x2);
   }
   for (int k = D.k1; k <= k1s - 1; ++k) {
     D.q2[IX(k, m)] = D.q2[IX(k1s, m)];
     D.q2l[IX(k, m)] = D.q2l[IX(k1s, m)];
-  }
-  // eddy coefficients from the p-level turbulence (245-290)
-  for (int k = k1s + 1; k <= kms; ++k) {
-    const double hkm = D.hp[IX(k - 1, m)], hk = D.hp[IX(k, m)];
-    const double dz = 0.5 * (hkm + hk);
-    const double rkm = D.rhop[IX(k - 1, m)], rk = D.rhop[IX(k, m)];
-    const double N2 = -G * ((rkm - rk) / dz) / (0.5 * (rkm + rk) + 1000.);
-    const double q2pk = D.q2p[IX(k, m)];
-    double xl = D.q2lp[IX(k, m)] / q2pk;
-    if (N2 > 0.0) xl = dmin(xl, 0.53 * sqrt(q2pk / N2));
-    double Gh = -N2 * xl * xl / q2pk;
-    Gh = dmax(GH_MIN, dmin(GH_MAX, Gh));
-    const double Sh = D.P.t_1 / (1. - Gh * D.P.t_2);
-    const double Sm = (D.P.t_3 + Sh * Gh * D.P.t_4) / (1. - D.P.t_5 * Gh);
-    const double Rt = sqrt(q2pk) * xl;
-    D.Av[IX(k, m)] = (R)(Rt * Sm + AV_MOL);
-    D.Dv[IX(k, m)] = (R)(Rt * Sh + DV_MOL);
-  }
-  if ((double)D.hp[IX(k1s, m)] < 1.E-2) {
-    D.Av[IX(k1s + 1, m)] = (R)dmax(D.Av[IX(k1s + 1, m)], 1.E-4);
-    D.Dv[IX(k1s + 1, m)] = (R)dmax(D.Dv[IX(k1s + 1, m)], 1.E-4);
   }
   for (int k = 1; k <= k1s; ++k) D.Av[IX(k, m)] = D.Dv[IX(k, m)] = R(0);
   for (int k = kms + 1; k <= D.km1; ++k) D.Av[IX(k, m)] = D.Dv[IX(k, m)] = R(0);
