@@ -31,9 +31,12 @@ __device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : 
 // word packs kc (low 8 bits) and the order of the 1-D grid of (column block,
 // layer chunk) tiles: order 0 runs layer chunk by layer chunk (all column
 // blocks of a chunk are in flight together, so the E/W neighbour rows of a
-// plane are read by concurrently running workgroups); order 1 remaps the grid
-// so that each of the 8 XCDs (workgroups are dispatched round-robin over them)
-// runs a contiguous range of column blocks with all their layer chunks.  The
+// plane are read by concurrently running workgroups); orders 1-3 remap the
+// grid so that each of the 8 XCDs (workgroups are dispatched round-robin over
+// them) runs a contiguous range of column blocks with all their layer chunks,
+// either block by block (1) or in panels of 8 (2) or 16 (3) column blocks
+// walked chunk by chunk, so that a workgroup's east/west neighbour rows were
+// fetched into the same XCD's L2 by a workgroup a few slots earlier.  The
 // autotuner picks both per kernel; results do not depend on them.
 // Each kernel has a chunked instantiation (CH) and a single-layer one used for
 // chunk size 1: a loop lets the compiler hoist the column's address arithmetic,
@@ -43,12 +46,12 @@ struct CellTile {
 };
 
 __device__ __forceinline__ bool cell_tile(int lm, int k_first, int nk, int tile, CellTile &t) {
-  const int kc = tile & 255;
+  const int kc = tile & 255, order = tile >> 8;
   const int nkc = (nk + kc - 1) / kc;
   const int nbx = (lm + kBlock - 1) / kBlock;
   const int pid = blockIdx.x;
   int bx, kq;
-  if ((tile >> 8) == 0) {
+  if (order == 0) {
     kq = pid / nbx;
     bx = pid - kq * nbx;
   } else {
@@ -56,8 +59,18 @@ __device__ __forceinline__ bool cell_tile(int lm, int k_first, int nk, int tile,
     const int xcd = pid & 7, idx = pid >> 3;
     const int per = total >> 3, rem = total & 7;
     const int logical = (xcd < rem) ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
-    bx = logical / nkc;
-    kq = logical - bx * nkc;
+    if (order == 1) {
+      bx = logical / nkc;
+      kq = logical - bx * nkc;
+    } else {
+      // panels of P column blocks, walked layer chunk by layer chunk
+      const int P = (order == 2) ? 8 : 16;
+      const int panel = logical / (P * nkc);
+      const int r = logical - panel * P * nkc;
+      const int Pp = min(P, nbx - panel * P);
+      kq = r / Pp;
+      bx = panel * P + (r - kq * Pp);
+    }
   }
   t.c = bx * kBlock + threadIdx.x;
   t.k0 = k_first + kq * kc;
@@ -647,31 +660,46 @@ __global__ void __launch_bounds__(kBlock) k_exsal_imsal(Dev<R> D, double twodt1)
   const int kms = D.kmz[c], k1s = D.k1z[c];
   const int lE = D.lE[c], lW = D.lW[c], lN = D.lN[c], lS = D.lS[c];
   const int nwl = kms - k1s + 1;
-  if (DO_EX) {
+  // Right-hand side of layer k (3038-3067); fud_k is the vertical flux through
+  // the top of k, carried from the previous layer, fud_kp the one below.
+  auto bh = [&](const R *hq, double a0, int w, int k) -> double {
+    return (w == S || k > D.kmz[w]) ? 0.0 : a0 * (double)hq[IX(k, w)];
+  };
+  auto rhs = [&](int k, double fud_k, double fud_kp) -> double {
+    const double sc = D.salpp[IX(k, c)];
+    double b = (double)D.hpp[IX(k, c)] * sc / twodt1 -
+               ((double)D.fx_ew[IX(k, c)] - (lW == S ? 0.0 : (double)D.fx_ew[IX(k, lW)])) / D.P.dx -
+               ((double)D.fx_ns[IX(k, c)] - (lS == S ? 0.0 : (double)D.fx_ns[IX(k, lS)])) / D.P.dy -
+               (fud_k - fud_kp);
+    if (D.P.ihd > 0) {
+      b += (bh(D.hupp, D.P.Ax0, c, k) * ((double)D.salpp[IX(k, lE)] - sc) -
+            bh(D.hupp, D.P.Ax0, lW, k) * (sc - (double)D.salpp[IX(k, lW)])) / D.P.dxdx +
+           (bh(D.hvpp, D.P.Ay0, c, k) * ((double)D.salpp[IX(k, lN)] - sc) -
+            bh(D.hvpp, D.P.Ay0, lS, k) * (sc - (double)D.salpp[IX(k, lS)])) / D.P.dydy;
+    }
+    return (double)(R)b;  // the reference stores Bex in the state precision
+  };
+  if (DO_EX && !DO_SOLVE) {  // exsal alone: Bex for every layer of the column
     for (int k = 1; k <= D.km1; ++k)
       if (k < k1s || k > kms) D.bex[IX(k, c)] = R(0);
-    auto bh = [&](const R *hq, double a0, int w, int k) -> double {
-      return (w == S || k > D.kmz[w]) ? 0.0 : a0 * (double)hq[IX(k, w)];
-    };
     double fud_k = (k1s <= kms) ? vflux(D, c, k1s, k1s, twodt1) : 0.0;
     for (int k = k1s; k <= kms; ++k) {
       const double fud_kp = (k + 1 <= kms) ? vflux(D, c, k + 1, k1s, twodt1) : 0.0;
-      const double sc = D.salpp[IX(k, c)];
-      double b = (double)D.hpp[IX(k, c)] * sc / twodt1 -
-                 ((double)D.fx_ew[IX(k, c)] - (lW == S ? 0.0 : (double)D.fx_ew[IX(k, lW)])) / D.P.dx -
-                 ((double)D.fx_ns[IX(k, c)] - (lS == S ? 0.0 : (double)D.fx_ns[IX(k, lS)])) / D.P.dy -
-                 (fud_k - fud_kp);
-      if (D.P.ihd > 0) {
-        b += (bh(D.hupp, D.P.Ax0, c, k) * ((double)D.salpp[IX(k, lE)] - sc) -
-              bh(D.hupp, D.P.Ax0, lW, k) * (sc - (double)D.salpp[IX(k, lW)])) / D.P.dxdx +
-             (bh(D.hvpp, D.P.Ay0, c, k) * ((double)D.salpp[IX(k, lN)] - sc) -
-              bh(D.hvpp, D.P.Ay0, lS, k) * (sc - (double)D.salpp[IX(k, lS)])) / D.P.dydy;
-      }
-      D.bex[IX(k, c)] = (R)b;
+      D.bex[IX(k, c)] = (R)rhs(k, fud_k, fud_kp);
       fud_k = fud_kp;
     }
+    return;
   }
-  if (!DO_SOLVE || nwl < 1) return;
+  if (nwl < 1) return;
+  // Bex of layer k: computed in the sweep (fused) or read back (imsal alone)
+  double fud = (DO_EX && k1s <= kms) ? vflux(D, c, k1s, k1s, twodt1) : 0.0;
+  auto bex = [&](int k) -> double {
+    if (!DO_EX) return (double)D.bex[IX(k, c)];
+    const double fud_kp = (k + 1 <= kms) ? vflux(D, c, k + 1, k1s, twodt1) : 0.0;
+    const double b = rhs(k, fud, fud_kp);
+    fud = fud_kp;
+    return b;
+  };
   // imsal: layer thicknesses at time n with the surface-cell corrections (3124-3132)
   auto hn = [&](int k) -> double {
     if (k == k1s) return twodt1 * (double)D.wp[IX(k1s, c)] + (double)D.hpp[IX(k1s, c)];
@@ -680,20 +708,19 @@ __global__ void __launch_bounds__(kBlock) k_exsal_imsal(Dev<R> D, double twodt1)
   };
   if (nwl == 1) {
     double a = hn(k1s) / twodt1;
-    double v = (double)D.bex[IX(k1s, c)] / a;
+    double v = bex(k1s) / a;
     if ((double)D.h[IX(k1s, c)] < 1.E-2) v = D.salpp[IX(k1s, c)];
     D.sal[IX(k1s, c)] = (R)v;
     return;
   }
   // forward e/f sweep: e -> t0, f -> sal (in place), then back substitution
-  double hk = hn(k1s), hkp = hn(k1s + 1);
+  double hk = hn(k1s), hkp = hn(k1s + 1), hkm = 0.0;
   double ep = 0.0, fp = 0.0;
   for (int k = k1s; k <= kms; ++k) {
-    const double hkn = (k + 1 <= kms) ? hn(k + 1) : 0.0;  // not used when k == kms
-    const double a1 = (k > k1s) ? -(double)D.Dv[IX(k, c)] / (hn(k - 1) + hk) * 2. : 0.0;
+    const double a1 = (k > k1s) ? -(double)D.Dv[IX(k, c)] / (hkm + hk) * 2. : 0.0;
     const double a3 = (k < kms) ? -(double)D.Dv[IX(k + 1, c)] / (hk + hkp) * 2. : 0.0;
     const double a2 = hk / twodt1 - a1 - a3;
-    const double d = D.bex[IX(k, c)];
+    const double d = bex(k);
     if (k == k1s) {
       ep = -a3 / a2;
       fp = d / a2;
@@ -707,9 +734,9 @@ __global__ void __launch_bounds__(kBlock) k_exsal_imsal(Dev<R> D, double twodt1)
     }
     D.t0[IX(k, c)] = (R)ep;
     D.sal[IX(k, c)] = (R)fp;
+    hkm = hk;
     hk = hkp;
     hkp = (k + 2 <= kms) ? hn(k + 2) : 0.0;
-    (void)hkn;
   }
   double x = fp;
   for (int k = kms - 1; k >= k1s; --k) {
@@ -1341,7 +1368,7 @@ int autotune(s3g_ctx *c) {
   const int L = c->lm;
   const double tz = 1.0, twodt1 = c->P.twodt;
   const size_t n3 = (size_t)c->KP * c->LP;
-  const int cand[] = {1, 2, 4, 8, 16, 256 + 1, 256 + 2, 256 + 4, 256 + 8};
+  const int cand[] = {1, 2, 4, 8, 16, 256 + 1, 256 + 4, 512 + 1, 512 + 2, 512 + 4, 768 + 1, 768 + 2, 768 + 4};
   const bool timing = c->timing;
   c->timing = false;
   R *save = nullptr;
