@@ -188,6 +188,21 @@ int s3g_unet_check(int H, int W, const float *weights, const float *x3, const fl
 /* Restates input/bathy/init on the host and calls s3g_init + s3g_set_field_*. */
 int s3g_create_from_deck(const s3g_deck *deck, const s3g_params *params, s3g_ctx **out);
 
+/* --- Ensemble of lakes ---------------------------------------------------- */
+/* No counterpart in the reference (one lake per run).  nlakes independent
+ * lakes of uneven size and layer count, each from its own deck and forcing
+ * (params[i] may differ from params[0] only in ifsurfbc, cw, wa, phi,
+ * dtsurfbc, n_surfbc, surfbc), advanced together: every phase is one launch
+ * over the columns of all lakes, and the free-surface solve is one CG on the
+ * block-diagonal system (its tolerance applies to the whole ensemble).
+ * Fields use the same layout as for one lake with the lakes' columns one
+ * after the other (lake i owns columns col_offset[i] .. col_offset[i+1]-1)
+ * and km1 = the largest km1 of the lakes. */
+int s3g_create_ensemble(int nlakes, const s3g_deck *decks, const s3g_params *params, s3g_ctx **out);
+/* Number of lakes; with max >= that number also col_offset[0..nlakes] and
+ * each lake's km1. */
+int s3g_ensemble_info(s3g_ctx *c, int max, int *col_offset, int *km1);
+
 #ifdef __cplusplus
 }
 #endif

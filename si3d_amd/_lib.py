@@ -22,6 +22,7 @@ EXPORTS = [
     "s3g_finalize", "s3g_last_error", "s3g_create_from_deck",
     "s3g_learn_setup", "s3g_learn_record", "s3g_learn_train", "s3g_learn_mode", "s3g_learn_num_weights",
     "s3g_learn_get_weights", "s3g_learn_set_weights", "s3g_learn_info", "s3g_unet_check",
+    "s3g_create_ensemble", "s3g_ensemble_info",
 ]
 
 dp = C.POINTER(C.c_double)
@@ -107,6 +108,8 @@ def load():
     L.s3g_learn_set_weights.argtypes = [vp, fp, C.c_long]
     L.s3g_learn_info.argtypes = [vp, C.POINTER(S3gLearnStats)]
     L.s3g_unet_check.argtypes = [C.c_int, C.c_int, fp, fp, fp, fp, fp, fp, dp]
+    L.s3g_create_ensemble.argtypes = [C.c_int, C.POINTER(S3gDeck), C.POINTER(S3gParams), C.POINTER(vp)]
+    L.s3g_ensemble_info.argtypes = [vp, C.c_int, ip, ip]
     L.s3g_finalize.argtypes = [vp]
     L.s3g_finalize.restype = None
     L.s3g_last_error.restype = C.c_char_p

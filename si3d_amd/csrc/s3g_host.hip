@@ -97,7 +97,30 @@ static double parab(double frstpt, double x, const std::vector<double> &fx1, dou
 
 // surfbc cases 0 and 20 (si3d_boundaryconditions.f90:4508-4515, 4650-4665): the
 // forcing is uniform in space, so it is kept as three scalars.
+static void forcing_of(const s3g_ctx::LakeForcing &F, double thrs, int n, int istep, double *out) {
+  if (F.ifsurfbc == 0) {
+    out[0] = -F.wa * std::sin(3.1415926535897932 * F.phi / 180.);
+    out[1] = -F.wa * std::cos(3.1415926535897932 * F.phi / 180.);
+    out[2] = F.cw;
+  } else if (F.ifsurfbc == 20) {
+    if (n > 1 && istep == 2) return;
+    const int np = (int)(F.surfbc.size() / 3);
+    if (np < 3) return;
+    std::vector<double> f1(np + 1), f2(np + 1), f3(np + 1);
+    for (int j = 0; j < np; ++j) f1[j + 1] = F.surfbc[3 * j], f2[j + 1] = F.surfbc[3 * j + 1], f3[j + 1] = F.surfbc[3 * j + 2];
+    const double dth = F.dtsurfbc / 3600.;
+    out[2] = parab(0., thrs, f1, dth);
+    out[0] = parab(0., thrs, f2, dth);
+    out[1] = parab(0., thrs, f3, dth);
+  }
+}
+
 void surfbc(s3g_ctx *c, int n, int istep) {
+  if (c->nlakes > 1) {
+    for (int i = 0; i < c->nlakes; ++i) forcing_of(c->lake_forcing[i], c->thrs, n, istep, &c->lake_force[3 * i]);
+    set_forcing(c);
+    return;
+  }
   if (c->P.ifsurfbc == 0) {
     c->uair = -c->P.wa * std::sin(3.1415926535897932 * c->P.phi / 180.);
     c->vair = -c->P.wa * std::cos(3.1415926535897932 * c->P.phi / 180.);
@@ -237,7 +260,8 @@ static int get_field(s3g_ctx *c, const char *name, HT *dst) {
 template <typename R>
 static int alloc_state(s3g_ctx *c, Dev<R> &D, const int *lE, const int *lW, const int *lN, const int *lS,
                        const int *kmz, const int *k1z, const int *k1u, const int *k1v, const double *zl,
-                       const double *hhs, const double *hhu, const double *hhv) {
+                       const double *hhs, const double *hhu, const double *hhv, const int *lake_of,
+                       const int *lake_km1) {
   const size_t n3 = (size_t)c->KP * c->LP, n2 = c->LP;
   D.lm = c->lm;
   D.LP = c->LP;
@@ -263,7 +287,20 @@ static int alloc_state(s3g_ctx *c, Dev<R> &D, const int *lE, const int *lW, cons
   D.k1z = up_i(k1z);
   D.k1u = up_i(k1u);
   D.k1v = up_i(k1v);
-  D.zlevel = up_d(zl, c->KP);
+  D.zlevel = up_d(zl, (size_t)c->KP * c->nlakes);
+  D.KPz = c->KP;
+  D.lake = nullptr;
+  D.lake_km1 = nullptr;
+  D.force = nullptr;
+  if (c->nlakes > 1) {
+    D.lake = up_i(lake_of);
+    int *k = dalloc<int>(c, c->nlakes);
+    if (k) hipMemcpy(k, lake_km1, c->nlakes * sizeof(int), hipMemcpyHostToDevice);
+    D.lake_km1 = k;
+    c->d_force = dalloc<double>(c, 3 * (size_t)c->nlakes);
+    D.force = c->d_force;
+    if (!D.lake || !D.lake_km1 || !D.force) return S3G_ERR_HIP;
+  }
   D.hhs = up_d(hhs, n2);
   D.hhu = up_d(hhu, n2);
   D.hhv = up_d(hhv, n2);
@@ -293,6 +330,13 @@ extern "C" {
 const char *s3g_last_error(void) { return g_err.c_str(); }
 
 int s3g_init(const s3g_grid *g, const s3g_params *p, s3g_ctx **out) {
+  return s3g::init_ctx(g, p, 1, nullptr, nullptr, nullptr, out);
+}
+
+}  // extern "C"
+
+int s3g::init_ctx(const s3g_grid *g, const s3g_params *p, int nlakes, const double *zl_lakes, const int *lake_of,
+                  const int *lake_km1, s3g_ctx **out) {
   if (!g || !p || !out || g->lm <= 0 || g->km1 < 3) {
     set_error("s3g_init: invalid grid");
     return S3G_ERR_ARG;
@@ -319,6 +363,7 @@ int s3g_init(const s3g_grid *g, const s3g_params *p, s3g_ctx **out) {
     return S3G_ERR_HIP;
   }
   s3g_ctx *c = new s3g_ctx();
+  c->nlakes = nlakes;
   c->precision = p->precision;
   c->debug = std::getenv("S3G_DEBUG") != nullptr;
   {
@@ -412,6 +457,7 @@ int s3g_init(const s3g_grid *g, const s3g_params *p, s3g_ctx **out) {
   std::vector<int> lE(LP, S), lW(LP, S), lN(LP, S), lS(LP, S), kmz(LP, c->km1), k1z(LP, c->km1),
       k1u(LP, c->km1), k1v(LP, c->km1);
   std::vector<double> hhs(LP, 0.0), hhu(LP, 0.0), hhv(LP, 0.0), zl(c->KP, 1e30);
+  std::vector<int> lake(LP, 0);
   long nwet = 0;
   for (int l = 0; l < lm; ++l) {
     auto nb = [&](const int *a) { int v = a[l] - 1; return (v < 0 || v >= lm) ? S : v; };
@@ -428,14 +474,21 @@ int s3g_init(const s3g_grid *g, const s3g_params *p, s3g_ctx **out) {
     hhv[l] = g->hhv[l];
     nwet += std::max(0, g->kmz[l] - g->k1z[l] + 1);
   }
-  zl[0] = -100.0;
-  for (int k = 1; k <= c->km1; ++k) zl[k] = g->zlevel[k - 1];
+  if (zl_lakes) {
+    zl.assign(zl_lakes, zl_lakes + (size_t)c->KP * nlakes);
+    for (int l = 0; l <= lm; ++l) lake[l] = lake_of[l];
+  } else {
+    zl[0] = -100.0;
+    for (int k = 1; k <= c->km1; ++k) zl[k] = g->zlevel[k - 1];
+  }
   c->n_wet = nwet;
   int rc = (c->precision == S3G_FP64)
                ? alloc_state(c, c->d64, lE.data(), lW.data(), lN.data(), lS.data(), kmz.data(), k1z.data(),
-                             k1u.data(), k1v.data(), zl.data(), hhs.data(), hhu.data(), hhv.data())
+                             k1u.data(), k1v.data(), zl.data(), hhs.data(), hhu.data(), hhv.data(), lake.data(),
+                             lake_km1)
                : alloc_state(c, c->d32, lE.data(), lW.data(), lN.data(), lS.data(), kmz.data(), k1z.data(),
-                             k1u.data(), k1v.data(), zl.data(), hhs.data(), hhu.data(), hhv.data());
+                             k1u.data(), k1v.data(), zl.data(), hhs.data(), hhu.data(), hhv.data(), lake.data(),
+                             lake_km1);
   Pcg &G = c->pcg;
   const size_t n2 = LP;
   double **pv[] = {&G.r, &G.r1, &G.z, &G.p0, &G.p1, &G.q, &G.x, &G.lc, &G.ld};
@@ -460,6 +513,8 @@ int s3g_init(const s3g_grid *g, const s3g_params *p, s3g_ctx **out) {
   *out = c;
   return hip_check(hipDeviceSynchronize(), "init");
 }
+
+extern "C" {
 
 int s3g_set_field_f32(s3g_ctx *c, const char *n, const float *h) { return set_field(c, n, h); }
 int s3g_set_field_f64(s3g_ctx *c, const char *n, const double *h) { return set_field(c, n, h); }

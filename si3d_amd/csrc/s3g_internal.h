@@ -42,7 +42,12 @@ struct Dev {
   double uair, vair, cdw;  // spatially uniform forcing (surfbc cases 0, 20)
   const int *lE, *lW, *lN, *lS, *kmz;
   int *k1z, *k1u, *k1v;
-  const double *zlevel;  // [km1+3], zlevel[km1+1] = +huge
+  const double *zlevel;  // [km1+3], zlevel[km1+1] = +huge; [nlakes][KPz] for an ensemble
+  // ensemble of lakes (null for one lake): lake of each column (sentinel: 0),
+  // each lake's km1, and its forcing (uair, vair, cdw) per lake
+  const int *lake, *lake_km1;
+  int KPz;
+  const double *force;
   const double *hhs, *hhu, *hhv;
   double *s, *sp, *spp, *sx, *sy, *dd, *qq, *rr, *eagx, *earx, *eagy, *eary;
   R *h, *hp, *hpp, *hu, *hup, *hupp, *hv, *hvp, *hvpp;
@@ -139,6 +144,18 @@ struct s3g_ctx {
   // unless S3G_TUNE=0)
   int kc[s3g::TILE_NUM] = {1, 1, 1, 1, 1, 1, 1, 1};
   bool tune = true, tuned = false;
+  // ensemble of lakes (s3g_create_ensemble): column offset of each lake
+  // ([nlakes+1]), each lake's km1 and forcing parameters
+  int nlakes = 1;
+  std::vector<int> lake_off, lake_km1;
+  struct LakeForcing {
+    int ifsurfbc = 0;
+    double cw = 0, wa = 0, phi = 0, dtsurfbc = 0;
+    std::vector<double> surfbc;  // 3 per record
+  };
+  std::vector<LakeForcing> lake_forcing;
+  std::vector<double> lake_force;  // 3 per lake: uair, vair, cdw
+  double *d_force = nullptr;
   // learned initial guess (s3g_learn_setup)
   s3g::Learner *learn = nullptr;
   std::vector<void *> learn_allocs;
@@ -166,4 +183,9 @@ int hip_check(hipError_t e, const char *what);
 void timer_begin(s3g_ctx *c, const char *name, int *slot);
 void timer_end(s3g_ctx *c, int slot);
 void surfbc(s3g_ctx *c, int n, int istep);
+// host.hip: context creation shared by s3g_init and s3g_create_ensemble;
+// zl_lakes (nlakes x KP) and lake_of (lm+1) are null for one lake
+int init_ctx(const s3g_grid *g, const s3g_params *p, int nlakes, const double *zl_lakes, const int *lake_of,
+             const int *lake_km1, s3g_ctx **out);
+int set_forcing(s3g_ctx *c);
 }  // namespace s3g

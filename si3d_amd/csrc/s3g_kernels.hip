@@ -6,6 +6,7 @@
 // cumulative baroclinic term, continuity's bottom-up integration) stay in
 // registers; the forward-sweep factors go to a per-column scratch plane stack
 // in the same coalesced layout and are read back by the same lane.
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -104,9 +105,30 @@ __device__ double densty(double t, double s, double elev) {
   return rho;
 }
 
+// Per-lake data of an ensemble context (D.lake != null); one lake otherwise.
+template <typename R>
+__device__ __forceinline__ const double *zlv(const Dev<R> &D, int c) {
+  return D.lake ? D.zlevel + (size_t)D.lake[c] * D.KPz : D.zlevel;
+}
+template <typename R>
+__device__ __forceinline__ int km1c(const Dev<R> &D, int c) {
+  return D.lake ? D.lake_km1[D.lake[c]] : D.km1;
+}
+struct Wind {
+  double uair, vair, cdw;
+};
+template <typename R>
+__device__ __forceinline__ Wind wind(const Dev<R> &D, int c) {
+  if (D.force) {
+    const double *f = D.force + 3 * D.lake[c];
+    return {f[0], f[1], f[2]};
+  }
+  return {D.uair, D.vair, D.cdw};
+}
+
 template <typename R>
 __device__ __forceinline__ double zdens(const Dev<R> &D, int k, int c) {
-  double zl = D.zlevel[k];
+  double zl = zlv(D, c)[k];
   return (zl == -100.0) ? 0.5 * (double)D.hp[IX(k, c)] : zl + 0.5 * (double)D.hp[IX(k, c)];
 }
 
@@ -323,8 +345,9 @@ __global__ void __launch_bounds__(kBlock) k_matmom(Dev<R> D, double tz) {
     usurf = ((double)D.up[IX(D.k1u[c], c)] + (double)D.up[IX(D.k1u[lW], lW)] + (double)D.up[IX(D.k1u[lN], lN)] +
              (double)D.up[IX(D.k1u[lNW], lNW)]) / 4.;
   }
-  const double cwq = D.cdw * RHOAIR * sqrt((D.vair - vsurf) * (D.vair - vsurf) + (D.uair - usurf) * (D.uair - usurf));
-  wind_g = cwq / rhoq * (X ? D.uair : D.vair) * twodt1;
+  const Wind wd = wind(D, c);
+  const double cwq = wd.cdw * RHOAIR * sqrt((wd.vair - vsurf) * (wd.vair - vsurf) + (wd.uair - usurf) * (wd.uair - usurf));
+  wind_g = cwq / rhoq * (X ? wd.uair : wd.vair) * twodt1;
   const double wind_a = cwq / rhoq / (double)hq[IX(k1q, c)] * twodt1;
   // quadratic bottom stress (1364-1369 / 1557-1562)
   double taub;
@@ -445,18 +468,18 @@ __global__ void __launch_bounds__(kBlock) k_matcon(Dev<R> D, double tz) {
 template <typename R>
 __device__ __forceinline__ void slayer_h_cell(const Dev<R> &D, int c, int k) {
   if (k > D.kmz[c] + 1) return;
-  const int S = D.lm, km1 = D.km1;
+  const int S = D.lm, km1 = km1c(D, c);
   const int k1z = D.k1z[c], k1u = D.k1u[c], k1v = D.k1v[c];
   const double sc = D.s[c];
   // zeta points: drying of the top cell moves the computation one layer down
   {
-    const double haux0 = dmin(D.zlevel[k1z + 1], D.hhs[c]) + sc;
+    const double haux0 = dmin(zlv(D, c)[k1z + 1], D.hhs[c]) + sc;
     if (k < k1z || k >= k1z + 2) {
       D.h[IX(k, c)] = D.hp[IX(k, c)];
     } else if (k == k1z) {
       D.h[IX(k, c)] = (haux0 <= 0.0) ? R(0) : (R)haux0;
     } else if (haux0 <= 0.0) {  // k == k1z + 1 after the top cell dried
-      const double haux = dmin(D.zlevel[k + 1], D.hhs[c]) + sc;
+      const double haux = dmin(zlv(D, c)[k + 1], D.hhs[c]) + sc;
       D.h[IX(k, c)] = (haux <= 0.0) ? R(0) : (R)haux;
     }
   }
@@ -465,7 +488,7 @@ __device__ __forceinline__ void slayer_h_cell(const Dev<R> &D, int c, int k) {
   if (k != k1u) {
     D.hu[IX(k, c)] = D.hup[IX(k, c)];
   } else if (lE != S) {
-    const double haux = dmin(D.zlevel[k + 1], D.hhu[c]) + dmax(sc, D.s[lE]);
+    const double haux = dmin(zlv(D, c)[k + 1], D.hhu[c]) + dmax(sc, D.s[lE]);
     D.hu[IX(k, c)] = (haux <= 0.0) ? R(0) : (R)haux;
   } else {
     D.hu[IX(k, c)] = D.hup[IX(k, c)];
@@ -475,13 +498,13 @@ __device__ __forceinline__ void slayer_h_cell(const Dev<R> &D, int c, int k) {
     if (k < k1v || k >= k1v + 2 || k <= min(k1v + 1, km1)) D.hv[IX(k, c)] = D.hvp[IX(k, c)];
   } else {
     const double sm = dmax(sc, D.s[lN]);
-    const double haux0 = dmin(D.zlevel[k1v + 1], D.hhv[c]) + sm;
+    const double haux0 = dmin(zlv(D, c)[k1v + 1], D.hhv[c]) + sm;
     if (k < k1v || k >= k1v + 2) {
       D.hv[IX(k, c)] = D.hvp[IX(k, c)];
     } else if (k == k1v) {
       D.hv[IX(k, c)] = (haux0 <= 0.0) ? R(0) : (R)haux0;
     } else if (haux0 <= 0.0) {
-      const double haux = dmin(D.zlevel[k + 1], D.hhv[c]) + sm;
+      const double haux = dmin(zlv(D, c)[k + 1], D.hhv[c]) + sm;
       D.hv[IX(k, c)] = (haux <= 0.0) ? R(0) : (R)haux;
     }
   }
@@ -584,7 +607,7 @@ __global__ void __launch_bounds__(kBlock) k_continuity(Dev<R> D) {
       if (k0 <= min(D.kmz[lN], kms) && (double)D.hv[IX(k0, c)] <= 0.0) D.k1v[c] = k0 + 1;
     }
     const int k0 = D.k1z[c];
-    if ((double)D.h[IX(k0, c)] <= 0.0) D.k1z[c] = min(k0 + 1, D.km1);
+    if ((double)D.h[IX(k0, c)] <= 0.0) D.k1z[c] = min(k0 + 1, km1c(D, c));
   }
 }
 
@@ -759,7 +782,8 @@ __device__ __forceinline__ void vel2_face(const Dev<R> &D, int c, int k, double 
   const double sm = dmax(sc, D.s[ln]);
   const double hh = X ? D.hhu[c] : D.hhv[c];
   auto hlay = [&](int kk) {
-    double v = dmin(D.zlevel[kk + 1], hh) - dmax(D.zlevel[kk], -sm);
+    const double *zl = zlv(D, c);
+    double v = dmin(zl[kk + 1], hh) - dmax(zl[kk], -sm);
     return (v <= 0.0) ? 0.0 : v;
   };
   R *hq = X ? D.hu : D.hv;
@@ -786,7 +810,7 @@ __global__ void __launch_bounds__(kBlock) k_vel2(Dev<R> D, int nk, int tile) {
   const double sc = D.s[c];
   for (int k = t.k0; k < (CH ? t.k1 : min(t.k1, t.k0 + 1)); ++k) {
     if (k <= D.kmz[c]) {
-      double v = dmin(D.zlevel[k + 1], D.hhs[c]) - dmax(D.zlevel[k], -sc);
+      double v = dmin(zlv(D, c)[k + 1], D.hhs[c]) - dmax(zlv(D, c)[k], -sc);
       D.h[IX(k, c)] = (v <= 0.0) ? R(0) : (R)v;
     }
     vel2_face<R, true>(D, c, k, sc);
@@ -871,10 +895,11 @@ __global__ void __launch_bounds__(kBlock) k_turb(Dev<R> D, int istep) {
   double ub = 0.5 * ((double)D.u[IX(kms, m)] + (double)D.u[IX(kms, lW)]);
   double vb = 0.5 * ((double)D.v[IX(kms, m)] + (double)D.v[IX(kms, lS)]);
   const double ds_bot = 3. * b1_23 * (D.P.cd * (ub * ub + vb * vb)) / twodt1;
+  const Wind wd = wind(D, m);
   double us = 0.5 * ((double)D.up[IX(D.k1u[m], m)] + (double)D.up[IX(D.k1u[lW], lW)]);
   double vs = 0.5 * ((double)D.vp[IX(D.k1v[m], m)] + (double)D.vp[IX(D.k1v[lS], lS)]);
   const double ds_top = 3. * b1_23 *
-                        (D.cdw * RHOAIR * ((D.uair - us) * (D.uair - us) + (D.vair - vs) * (D.vair - vs)) / 1000.) / twodt1;
+                        (wd.cdw * RHOAIR * ((wd.uair - us) * (wd.uair - us) + (wd.vair - vs) * (wd.vair - vs)) / 1000.) / twodt1;
   const bool thin_top = (double)D.hp[IX(k1s, m)] < 1.E-2;
   // One forward sweep over rows k1s..kms of both systems (row k1s and row
   // kms+1 are the Dirichlet rows: a1 = a3 = 0, a2 = 3/twodt1), with the
@@ -1058,7 +1083,8 @@ __device__ __forceinline__ void half_level_flux_top(const Dev<R> &D, int c, int 
 template <typename R>
 __device__ __forceinline__ void top_layer_indices(const Dev<R> &D, int c) {
   const int S = D.lm;
-  int k1z = D.km1, k1u = D.km1, k1v = D.km1;
+  const int km1 = km1c(D, c);
+  int k1z = km1, k1u = km1, k1v = km1;
   for (int k = D.k1; k <= D.kmz[c]; ++k)
     if ((double)D.hp[IX(k, c)] > 0.0) {
       k1z = k;
@@ -1102,16 +1128,16 @@ __global__ void __launch_bounds__(kBlock) k_settrap_cell(Dev<R> D, int nk, int t
     const int kmz = D.kmz[c];
     const bool wet = k >= D.k1 && k <= kmz;
     if (wet) {
-      const double v = cell_h(D.zlevel, k, D.hhs[c], spc);
+      const double v = cell_h(zlv(D, c), k, D.hhs[c], spc);
       D.hp[IX(k, c)] = (v <= 0.0) ? R(0) : (R)v;
     }
     const int lE = D.lE[c], lN = D.lN[c];
     if (lE != S && k >= D.k1 && k <= min(kmz, D.kmz[lE])) {
-      const double v = cell_h(D.zlevel, k, D.hhu[c], dmax(spc, D.sp[lE]));
+      const double v = cell_h(zlv(D, c), k, D.hhu[c], dmax(spc, D.sp[lE]));
       D.hup[IX(k, c)] = (v <= 0.0) ? R(0) : (R)v;
     }
     if (lN != S && k >= D.k1 && k <= min(kmz, D.kmz[lN])) {
-      const double v = cell_h(D.zlevel, k, D.hhv[c], dmax(spc, D.sp[lN]));
+      const double v = cell_h(zlv(D, c), k, D.hhv[c], dmax(spc, D.sp[lN]));
       D.hvp[IX(k, c)] = (v <= 0.0) ? R(0) : (R)v;
     }
     if (wet) {
@@ -1175,7 +1201,7 @@ __global__ void __launch_bounds__(kBlock) k_save_cell(Dev<R> D, int nk, int tile
       D.q2p[IX(k, c)] = D.q2[IX(k, c)];
       D.q2lp[IX(k, c)] = D.q2l[IX(k, c)];
     }
-    if (c == D.lm || k < D.k1 || k > D.km) continue;
+    if (c == D.lm || k < D.k1 || k > km1c(D, c) - 1) continue;
     if (ISTEP == 1) {
       D.salpp[IX(k, c)] = D.salp[IX(k, c)];
       D.uhpp[IX(k, c)] = D.uhp[IX(k, c)];
@@ -1217,7 +1243,7 @@ __global__ void __launch_bounds__(kBlock) k_save_top(Dev<R> D) {
   top_layer_indices(D, c);
   if (D.P.iturb > 0) {
     const int k1s = D.k1z[c];
-    const bool guard = (ISTEP == 1) ? true : (D.km1 - k1s > 1);
+    const bool guard = (ISTEP == 1) ? true : (km1c(D, c) - k1s > 1);
     if (guard && (double)D.hpp[IX(k1s, c)] <= 0.0) {
       D.q2lp[IX(k1s + 1, c)] = D.q2lp[IX(k1s + 2, c)];
       D.q2p[IX(k1s + 1, c)] = D.q2p[IX(k1s + 2, c)];
@@ -1233,7 +1259,7 @@ __global__ void __launch_bounds__(kBlock) k_smooth_col(Dev<R> D) {
   const int S = D.lm;
   const double spc = D.sp[c];
   for (int k = D.k1; k <= D.kmz[c]; ++k) {
-    double v = dmin(D.zlevel[k + 1], D.hhs[c]) - dmax(D.zlevel[k], -spc);
+    double v = dmin(zlv(D, c)[k + 1], D.hhs[c]) - dmax(zlv(D, c)[k], -spc);
     D.hp[IX(k, c)] = (v <= 0.0) ? R(0) : (R)v;
   }
   for (int dir = 0; dir < 2; ++dir) {
@@ -1248,7 +1274,7 @@ __global__ void __launch_bounds__(kBlock) k_smooth_col(Dev<R> D) {
     R *qp = X ? D.up : D.vp;
     const double sm = dmax(spc, D.sp[ln]);
     for (int k = D.k1; k <= min(D.kmz[c], D.kmz[ln]); ++k) {
-      double v = dmin(D.zlevel[k + 1], hh[c]) - dmax(D.zlevel[k], -sm);
+      double v = dmin(zlv(D, c)[k + 1], hh[c]) - dmax(zlv(D, c)[k], -sm);
       hq[IX(k, c)] = (v <= 0.0) ? R(0) : (R)v;
     }
     for (int k = D.k1; k <= min(D.kmz[c], D.kmz[ln]); ++k) {
@@ -1267,7 +1293,8 @@ template <typename R>
 __global__ void __launch_bounds__(kBlock) k_init_density(Dev<R> D) {
   const int c = col();
   if (c > D.lm) return;
-  for (int k = D.k1; k <= D.km1; ++k)
+  const int km1 = km1c(D, c);
+  for (int k = D.k1; k <= km1; ++k)
     D.rhop[IX(k, c)] = (R)(densty((double)D.salp[IX(k, c)], 0.00004, zdens(D, k, c)) - 1000.);
 }
 
@@ -1507,6 +1534,26 @@ int save(s3g_ctx *c, int istep) {
   }
   LAUNCH("continuity2", (k_continuity<R, 2>), L, D);
   return hip_check(hipGetLastError(), "save launch");
+}
+
+// Per-lake forcing of an ensemble (uair, vair, cdw per lake), passed by value
+// in chunks of 64 lakes.
+struct ForceChunk {
+  double v[3 * 64];
+};
+__global__ void k_set_forcing(ForceChunk f, int first, int n, double *force) {
+  const int i = threadIdx.x;
+  if (i < 3 * n) force[3 * first + i] = f.v[i];
+}
+
+int set_forcing(s3g_ctx *c) {
+  for (int first = 0; first < c->nlakes; first += 64) {
+    const int n = std::min(64, c->nlakes - first);
+    ForceChunk f{};
+    for (int i = 0; i < 3 * n; ++i) f.v[i] = c->lake_force[3 * first + i];
+    hipLaunchKernelGGL(k_set_forcing, dim3(1), dim3(192), 0, c->stream, f, first, n, c->d_force);
+  }
+  return hip_check(hipGetLastError(), "set_forcing");
 }
 
 int init_density(s3g_ctx *c) {

@@ -157,6 +157,24 @@ def ellipse(nx=1024, ny=512, dx=100.0, nlayers=60, dt=60.0, nts=2000, seed=1,
     return d
 
 
+def ensemble(n=64, seed=7, side=(128, 768), layers=(20, 80), wind_amp=(0.05, 0.3), dt=60.0, nts=500,
+             dx=100.0):
+    """Decks of an ensemble of `n` independent elliptic lakes: square boxes with
+    sides uniform in `side`, layer counts uniform in `layers`, wind amplitude
+    uniform in `wind_amp` (Pa) and direction uniform in [0, 360), independent
+    bathymetry seeds."""
+    rng = np.random.default_rng(seed)
+    decks = []
+    for _ in range(n):
+        s_ = int(rng.integers(side[0], side[1] + 1))
+        nl = int(rng.integers(layers[0], layers[1] + 1))
+        amp = float(rng.uniform(*wind_amp))
+        direc = float(rng.uniform(0.0, 360.0))
+        decks.append(ellipse(nx=s_, ny=s_, dx=dx, nlayers=nl, dt=dt, nts=nts,
+                             seed=int(rng.integers(1, 2 ** 31 - 1)), wind_amp=amp, wind_dir=direc))
+    return decks
+
+
 def n_wet_cells(deck) -> int:
     """N_wet = sum over wet columns of (kmz - k1 + 1) at t=0 for a level
     surface (the per-column layer count of bathy, si3d_utils.f90:812-867)."""

@@ -46,6 +46,23 @@ def make_params(deck, precision="fp64", preconditioner="jacobi", tol=None, maxit
     return p, keep
 
 
+def _deck_struct(deck, keep):
+    d = S3gDeck()
+    for name in ("imm", "jmm", "ibathyf", "ddz", "dzmin", "datadj", "zetainit", "zl"):
+        setattr(d, name, deck[name])
+    bathy = np.ascontiguousarray(deck["bathy_dm"], dtype=np.int32)
+    prof = np.ascontiguousarray(deck["init_profile"], dtype=np.float64)
+    keep += [bathy, prof]
+    d.bathy_dm = bathy.ctypes.data_as(_lib.ip)
+    d.init_profile = prof.ctypes.data_as(_lib.dp)
+    if deck["ibathyf"] < 0:
+        z = np.ascontiguousarray(deck["zlevel"], dtype=np.float64)
+        keep.append(z)
+        d.zlevel_file = z.ctypes.data_as(_lib.dp)
+        d.km1_file = len(z)
+    return d
+
+
 class GpuLake:
     def __init__(self, deck=None, precision="fp64", preconditioner="jacobi", tol=None,
                  maxit=None, _handle=None, _dims=None):
@@ -58,19 +75,7 @@ class GpuLake:
         else:
             p, keep = make_params(deck, precision, preconditioner, tol, maxit)
             self._keep += keep
-            d = S3gDeck()
-            for name in ("imm", "jmm", "ibathyf", "ddz", "dzmin", "datadj", "zetainit", "zl"):
-                setattr(d, name, deck[name])
-            bathy = np.ascontiguousarray(deck["bathy_dm"], dtype=np.int32)
-            prof = np.ascontiguousarray(deck["init_profile"], dtype=np.float64)
-            self._keep += [bathy, prof]
-            d.bathy_dm = bathy.ctypes.data_as(_lib.ip)
-            d.init_profile = prof.ctypes.data_as(_lib.dp)
-            if deck["ibathyf"] < 0:
-                z = np.ascontiguousarray(deck["zlevel"], dtype=np.float64)
-                self._keep.append(z)
-                d.zlevel_file = z.ctypes.data_as(_lib.dp)
-                d.km1_file = len(z)
+            d = _deck_struct(deck, self._keep)
             h = C.c_void_p()
             check(self.L.s3g_create_from_deck(C.byref(d), C.byref(p), C.byref(h)), "s3g_create_from_deck")
             self.h = h
@@ -232,3 +237,40 @@ def unet_check(weights, x3, target, mask):
                            m.ctypes.data_as(_lib.fp), out.ctypes.data_as(_lib.fp), grads.ctypes.data_as(_lib.fp),
                            C.byref(loss)), "s3g_unet_check")
     return out, grads, loss.value
+
+
+class GpuEnsemble(GpuLake):
+    """Independent lakes advanced together (s3g_create_ensemble): one launch per
+    phase over the columns of all lakes.  `lake_get(name, i)` returns lake i's
+    part of a field in the layout of a single lake (its columns, its km1
+    levels; no sentinel column)."""
+
+    def __init__(self, decks, precision="fp64", preconditioner="mg", tol=None, maxit=None):
+        L = _lib.load()
+        n = len(decks)
+        keep = []
+        darr = (S3gDeck * n)()
+        parr = (S3gParams * n)()
+        for i, dk in enumerate(decks):
+            darr[i] = _deck_struct(dk, keep)
+            p, k2 = make_params(dk, precision, preconditioner, tol, maxit)
+            keep += k2
+            parr[i] = p
+        h = C.c_void_p()
+        check(L.s3g_create_ensemble(n, darr, parr, C.byref(h)), "s3g_create_ensemble")
+        st = S3gStats()
+        check(L.s3g_get_stats(h, C.byref(st)), "s3g_get_stats")
+        super().__init__(_handle=h, _dims=(st.km1, st.lm), precision=precision)
+        self._keep = keep + [darr, parr]
+        off = np.zeros(n + 1, dtype=np.int32)
+        km1 = np.zeros(n, dtype=np.int32)
+        check(L.s3g_ensemble_info(h, n, off.ctypes.data_as(_lib.ip), km1.ctypes.data_as(_lib.ip)),
+              "s3g_ensemble_info")
+        self.offsets, self.lake_km1, self.nlakes = off, km1, n
+
+    def lake_get(self, name, i, dtype=np.float64):
+        a = self.get(name, dtype)
+        sl = slice(int(self.offsets[i]), int(self.offsets[i + 1]))
+        if a.ndim == 1:
+            return a[sl]
+        return a[sl, :int(self.lake_km1[i])]
