@@ -38,6 +38,8 @@ ARRAYS_PER_CELL = {
     "continuity1": (5, 0),                       # uh uhpp vh vhpp -> wp
     "exsal_hflux": (10, 0),                      # salpp uhpp uh vhpp vh hup hvp hpp -> fx_ew fx_ns
     "exsal_imsal": (11, 0),                      # fx_ew fx_ns salpp hpp hp hupp hvpp wp h Dv -> sal
+    "exsal_rhs": (10, 0),                        # fx_ew fx_ns salpp hpp hp hupp hvpp wp -> bex
+    "imsal": (5, 0),                             # bex Dv h (hpp, wp at the top) -> sal
     "vel2": (3, 0),                              # -> h hu hv
     "turbulence": (8, 6),                        # hp rhop up vp Av Dv -> Av Dv; q2p q2pp q2lp q2lpp -> q2 q2l
     "settrap_cell": (28, 8),                     # 11 read, 17 written; 4 + 4 FP64
@@ -74,6 +76,9 @@ def parse():
     ap.add_argument("--no-timers", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-initial-guess leg")
     ap.add_argument("--train-steps", type=int, default=300)
+    ap.add_argument("--ensemble", type=int, default=0, metavar="N",
+                    help="advance N lakes of uneven size (sides 128-768, 20-80 layers, random wind) together")
+    ap.add_argument("--side-max", type=int, default=768)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (calibrates HBM byte counters) and exit")
     return ap.parse_args()
@@ -116,13 +121,25 @@ def main():
               "(the multi-GPU decomposition is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
     from si3d_amd import lake
-    from si3d_amd.model import GpuLake
+    from si3d_amd.model import GpuEnsemble, GpuLake
 
-    deck = lake.ellipse(nx=a.nx, ny=a.ny, dx=100.0, nlayers=a.layers, dt=60.0,
-                        nts=a.warmup + a.steps + 1, seed=1)
-    n_wet = lake.n_wet_cells(deck)
+    nts = 2 * (a.warmup + a.steps) + 2
     t_setup = time.perf_counter()
-    g = GpuLake(deck, precision=a.precision, preconditioner=a.precond, tol=a.tol)
+    if a.ensemble:
+        decks = lake.ensemble(a.ensemble, seed=7, side=(128, a.side_max), nts=nts)
+        deck = decks[0]
+        n_wet = sum(lake.n_wet_cells(d) for d in decks)
+        g = GpuEnsemble(decks, precision=a.precision, preconditioner=a.precond, tol=a.tol)
+        a.no_learned = True
+        a.no_cpu_baseline = True
+        workload = (f"ensemble of {a.ensemble} elliptic lakes, square sides 128-{a.side_max}, 20-80 layers, "
+                    f"dx=100 m, dt=60 s, wind 0.05-0.3 Pa any direction, {a.precision}")
+    else:
+        deck = lake.ellipse(nx=a.nx, ny=a.ny, dx=100.0, nlayers=a.layers, dt=60.0, nts=nts, seed=1)
+        n_wet = lake.n_wet_cells(deck)
+        g = GpuLake(deck, precision=a.precision, preconditioner=a.precond, tol=a.tol)
+        workload = (f"ellipse lake {a.nx}x{a.ny}x{a.layers}, dx=100 m, dt=60 s, TVD + "
+                    f"turbulence, plain PCG ({a.precond}) tol {a.tol:g}, {a.precision}")
     t_setup = time.perf_counter() - t_setup
     if not a.no_learned:
         g.learn_setup(capacity=256, seed=1)  # the warmup solves record training pairs
@@ -193,7 +210,8 @@ def main():
             bytes_launch = (n_r * elem + n_f64 * 8) * n_wet
             achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
             traffic = None
-            if (a.nx, a.ny, a.layers, a.precision) == (1024, 512, 60, "fp64") and name in TRAFFIC_FP64_1024x512x60:
+            if (not a.ensemble and (a.nx, a.ny, a.layers, a.precision) == (1024, 512, 60, "fp64")
+                    and name in TRAFFIC_FP64_1024x512x60):
                 traffic = int(TRAFFIC_FP64_1024x512x60[name] * n_wet)
             roofline = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -221,8 +239,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if a.precision == "fp64" else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"ellipse lake {a.nx}x{a.ny}x{a.layers}, dx=100 m, dt=60 s, TVD + "
-                               f"turbulence, plain PCG ({a.precond}) tol {a.tol:g}, {a.precision}",
+        "config": {"workload": workload,
                    "n_wet": n_wet, "lm": g.lm, "km1": g.km1, "parallelism": "1 GPU"},
         "pcg_iters_per_step": round(iters, 2),
         "pcg_relres_last": st1["pcg_relres_last"],

@@ -199,11 +199,23 @@ static int download3(s3g_ctx *c, HT *dst, const DT *src) {
   return 0;
 }
 
+// Fields that the configuration does not allocate (turbulence state with
+// iturb = 0, Smagorinsky coefficients with ihd <= 1) cannot be accessed.
+static bool allocated(const s3g_ctx *c, const Field &f) {
+  if (!f.ptr) return false;
+  if (f.kind == K3D) return c->P.iturb > 0;
+  return true;
+}
+
 template <typename HT>
 static int set_field(s3g_ctx *c, const char *name, const HT *src) {
   Field f;
   if (!find_field(c, name, f)) {
     set_error(std::string("unknown field ") + name);
+    return S3G_ERR_ARG;
+  }
+  if (!allocated(c, f)) {
+    set_error(std::string(name) + " is not held in this configuration");
     return S3G_ERR_ARG;
   }
   const int lm1 = c->lm + 1;
@@ -231,6 +243,10 @@ static int get_field(s3g_ctx *c, const char *name, HT *dst) {
   Field f;
   if (!find_field(c, name, f)) {
     set_error(std::string("unknown field ") + name);
+    return S3G_ERR_ARG;
+  }
+  if (!allocated(c, f)) {
+    set_error(std::string(name) + " is not held in this configuration");
     return S3G_ERR_ARG;
   }
   if (int e = hip_check(hipStreamSynchronize(c->stream), "sync")) return e;
@@ -309,11 +325,16 @@ static int alloc_state(s3g_ctx *c, Dev<R> &D, const int *lE, const int *lW, cons
     if (!(*p = dalloc<double>(c, n2))) return S3G_ERR_HIP;
   R **d3[] = {&D.h,    &D.hp,   &D.hpp,  &D.hu,  &D.hup,  &D.hupp, &D.hv,  &D.hvp,  &D.hvpp, &D.uh,
               &D.uhp,  &D.uhpp, &D.u,    &D.up,  &D.upp,  &D.vh,   &D.vhp, &D.vhpp, &D.v,    &D.vp,
-              &D.vpp,  &D.wp,   &D.sal,  &D.salp, &D.salpp, &D.rhop, &D.Av, &D.Dv,   &D.kh,   &D.ex, &D.ey,
-              &D.agx,  &D.arx,  &D.agy,  &D.ary, &D.fx_ew, &D.fx_ns, &D.bex, &D.t0,  &D.t1};
+              &D.vpp,  &D.wp,   &D.sal,  &D.salp, &D.salpp, &D.rhop, &D.Av, &D.Dv,   &D.ex, &D.ey,
+              &D.agx,  &D.arx,  &D.agy,  &D.ary, &D.fx_ew, &D.fx_ns, &D.bex, &D.t0};
   for (R **p : d3)
     if (!(*p = dalloc<R>(c, n3))) return S3G_ERR_HIP;
-  double **q3[] = {&D.q2, &D.q2p, &D.q2pp, &D.q2l, &D.q2lp, &D.q2lpp, &D.d0, &D.d1, &D.d2};
+  // Smagorinsky coefficients only for ihd > 1, turbulence state only for iturb > 0
+  D.kh = nullptr;
+  D.t1 = nullptr;
+  D.d2 = nullptr;
+  if (c->P.ihd > 1 && !(D.kh = dalloc<R>(c, n3))) return S3G_ERR_HIP;
+  double **q3[] = {&D.q2, &D.q2p, &D.q2pp, &D.q2l, &D.q2lp, &D.q2lpp, &D.d0, &D.d1};
   for (double **p : q3)
     if (!(*p = dalloc<double>(c, c->P.iturb > 0 ? n3 : 1))) return S3G_ERR_HIP;
   std::vector<double> one(n2, 1.0);

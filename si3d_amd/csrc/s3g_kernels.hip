@@ -770,6 +770,40 @@ __global__ void __launch_bounds__(kBlock) k_exsal_imsal(Dev<R> D, double twodt1)
   D.sal[IX(kms + 1, c)] = D.sal[IX(kms, c)];
 }
 
+// exsal (part 2) per cell: Bex of layer k (3038-3067), with the vertical TVD
+// fluxes through its top and bottom faces; the column solve (imsal) then reads
+// it back.  One lane per (column, layer) keeps the flux arithmetic off the
+// column sweep's serial path.
+template <typename R, bool CH>
+__global__ void __launch_bounds__(kBlock) k_exsal_rhs(Dev<R> D, double twodt1, int nk, int tile) {
+  CellTile t;
+  if (!cell_tile(D.lm, 2, nk, tile, t)) return;
+  const int c = t.c, S = D.lm;
+  const int kms = D.kmz[c], k1s = D.k1z[c];
+  const int lE = D.lE[c], lW = D.lW[c], lN = D.lN[c], lS = D.lS[c];
+  const int kb = max(t.k0, k1s), ke = CH ? min(t.k1, kms + 1) : min(min(t.k1, t.k0 + 1), kms + 1);
+  double fud_kp = (kb < ke) ? vflux(D, c, kb, k1s, twodt1) : 0.0;  // carried down the chunk
+  for (int k = kb; k < ke; ++k) {
+    const double fud_k = fud_kp;
+    fud_kp = (k + 1 <= kms) ? vflux(D, c, k + 1, k1s, twodt1) : 0.0;
+    auto bh = [&](const R *hq, double a0, int w) -> double {
+      return (w == S || k > D.kmz[w]) ? 0.0 : a0 * (double)hq[IX(k, w)];
+    };
+    const double sc = D.salpp[IX(k, c)];
+    double b = (double)D.hpp[IX(k, c)] * sc / twodt1 -
+               ((double)D.fx_ew[IX(k, c)] - (lW == S ? 0.0 : (double)D.fx_ew[IX(k, lW)])) / D.P.dx -
+               ((double)D.fx_ns[IX(k, c)] - (lS == S ? 0.0 : (double)D.fx_ns[IX(k, lS)])) / D.P.dy -
+               (fud_k - fud_kp);
+    if (D.P.ihd > 0) {
+      b += (bh(D.hupp, D.P.Ax0, c) * ((double)D.salpp[IX(k, lE)] - sc) -
+            bh(D.hupp, D.P.Ax0, lW) * (sc - (double)D.salpp[IX(k, lW)])) / D.P.dxdx +
+           (bh(D.hvpp, D.P.Ay0, c) * ((double)D.salpp[IX(k, lN)] - sc) -
+            bh(D.hvpp, D.P.Ay0, lS) * (sc - (double)D.salpp[IX(k, lS)])) / D.P.dydy;
+    }
+    D.bex[IX(k, c)] = (R)b;
+  }
+}
+
 // vel2: layer_h (si3d_procedures.f90:2181-2231) then the top-cell flux split for
 // wetting (2570-2631).  One lane per (column, layer k = 2..km); the
 // lane of the top face layer performs the split for both layers it touches,
@@ -1408,7 +1442,7 @@ int autotune(s3g_ctx *c) {
   hipEventCreate(&e0);
   hipEventCreate(&e1);
   for (int tid = 0; tid < TILE_TUNED; ++tid) {
-    if (tid == TILE_HFLUX && c->P.isal == 0) continue;
+    if ((tid == TILE_HFLUX || tid == TILE_RHS) && c->P.isal == 0) continue;
     std::vector<R *> outs;
     switch (tid) {
       case TILE_EXMOM_X: outs = {D.ex}; break;
@@ -1416,6 +1450,7 @@ int autotune(s3g_ctx *c) {
       case TILE_SLAYER_H: outs = {D.h, D.hu, D.hv}; break;
       case TILE_VEL: outs = {D.uh, D.vh}; break;
       case TILE_HFLUX: outs = {D.fx_ew, D.fx_ns}; break;
+      case TILE_RHS: outs = {D.bex}; break;
       default: outs = {D.h, D.hu, D.hv, D.uh, D.vh}; break;
     }
     for (size_t i = 0; i < outs.size(); ++i)
@@ -1427,6 +1462,7 @@ int autotune(s3g_ctx *c) {
         case TILE_SLAYER_H: LAUNCHC("", (k_slayer_h<R, false>), (k_slayer_h<R, true>), TILE_SLAYER_H, L, c->km1, D); break;
         case TILE_VEL: LAUNCHC("", (k_vel<R, false>), (k_vel<R, true>), TILE_VEL, L, c->km - 1, D, tz); break;
         case TILE_HFLUX: LAUNCHC("", (k_exsal_hflux<R, false>), (k_exsal_hflux<R, true>), TILE_HFLUX, L, c->km - 1, D, twodt1); break;
+        case TILE_RHS: LAUNCHC("", (k_exsal_rhs<R, false>), (k_exsal_rhs<R, true>), TILE_RHS, L, c->km - 1, D, twodt1); break;
         default: LAUNCHC("", (k_vel2<R, false>), (k_vel2<R, true>), TILE_VEL2, L, c->km - 1, D); break;
       }
     };
@@ -1449,6 +1485,33 @@ int autotune(s3g_ctx *c) {
     for (size_t i = 0; i < outs.size(); ++i)
       hipMemcpyAsync(outs[i], save + i * n3, n3 * sizeof(R), hipMemcpyDeviceToDevice, c->stream);
     if (c->debug) std::fprintf(stderr, "[s3g] tile %d: kc=%d order=%d (%.3f ms)\n", tid, best_kc & 255, best_kc >> 8, best);
+  }
+  // exsal: per-cell right-hand side + column solve, or the fused column sweep
+  if (c->P.isal != 0) {
+    std::vector<R *> outs = {D.sal, D.t0, D.bex};
+    for (size_t i = 0; i < outs.size(); ++i)
+      hipMemcpyAsync(save + i * n3, outs[i], n3 * sizeof(R), hipMemcpyDeviceToDevice, c->stream);
+    float best[2] = {1e30f, 1e30f};
+    for (int v = 0; v < 2; ++v) {
+      for (int rep = 0; rep < 4; ++rep) {
+        hipEventRecord(e0, c->stream);
+        if (v == 0) {
+          LAUNCHC("", (k_exsal_rhs<R, false>), (k_exsal_rhs<R, true>), TILE_RHS, L, c->km - 1, D, twodt1);
+          LAUNCH("", (k_exsal_imsal<R, false, true>), L, D, twodt1);
+        } else {
+          LAUNCH("", (k_exsal_imsal<R, true, true>), L, D, twodt1);
+        }
+        hipEventRecord(e1, c->stream);
+        hipEventSynchronize(e1);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (rep > 0) best[v] = std::min(best[v], ms);
+      }
+    }
+    c->exsal_split = best[0] <= best[1];
+    for (size_t i = 0; i < outs.size(); ++i)
+      hipMemcpyAsync(outs[i], save + i * n3, n3 * sizeof(R), hipMemcpyDeviceToDevice, c->stream);
+    if (c->debug) std::fprintf(stderr, "[s3g] exsal: split %.3f ms, fused %.3f ms\n", best[0], best[1]);
   }
   const int e = hip_check(hipStreamSynchronize(c->stream), "autotune");
   hipEventDestroy(e0);
@@ -1487,7 +1550,12 @@ int fd(s3g_ctx *c, int n, int istep, int lastiter) {
   LAUNCH("continuity1", (k_continuity<R, 1>), L, D);
   if (c->P.isal != 0) {
     LAUNCHC("exsal_hflux", (k_exsal_hflux<R, false>), (k_exsal_hflux<R, true>), TILE_HFLUX, L, c->km - 1, D, twodt1);
-    LAUNCH("exsal_imsal", (k_exsal_imsal<R, true, true>), L, D, twodt1);
+    if (c->exsal_split) {
+      LAUNCHC("exsal_rhs", (k_exsal_rhs<R, false>), (k_exsal_rhs<R, true>), TILE_RHS, L, c->km - 1, D, twodt1);
+      LAUNCH("imsal", (k_exsal_imsal<R, false, true>), L, D, twodt1);
+    } else {
+      LAUNCH("exsal_imsal", (k_exsal_imsal<R, true, true>), L, D, twodt1);
+    }
   }
   LAUNCHC("vel2", (k_vel2<R, false>), (k_vel2<R, true>), TILE_VEL2, L, c->km - 1, D);
   if (c->P.ismooth >= 1 && istep == 1) {
