@@ -42,6 +42,7 @@ ARRAYS_PER_CELL = {
     "imsal": (5, 0),                             # bex Dv h (hpp, wp at the top) -> sal
     "vel2": (3, 0),                              # -> h hu hv
     "turbulence": (8, 6),                        # hp rhop up vp Av Dv -> Av Dv; q2p q2pp q2lp q2lpp -> q2 q2l
+    "turb_cell": (8, 2),                         # hp rhop up vp Av Dv -> Av Dv; q2p q2lp
     "settrap_cell": (28, 8),                     # 11 read, 17 written; 4 + 4 FP64
     "settrap2_cell": (15, 6),                    # 6 read, 9 written; 4 + 2 FP64
     "save_cell": (17, 4),                        # trapezoidal save: 6 read, 11 written; 2 + 2 FP64

@@ -134,7 +134,8 @@ int s3g_get_stats(s3g_ctx *c, s3g_stats *out);
 int s3g_kernel_times(s3g_ctx *c, int max, char (*names)[32], double *avg_ms, long *launches);
 int s3g_reset_timers(s3g_ctx *c, int enable);
 /* Tile words of the per-cell kernels (exmom x, exmom y, slayer_h, vel, exsal
- * fluxes, vel2, exsal right-hand side, settrap, save): layer-chunk size | grid order << 8, as chosen
+ * fluxes, vel2, exsal right-hand side, turbulence interface terms, settrap,
+ * save): layer-chunk size | grid order << 8, as chosen
  * by the autotuner at the first fd call (S3G_TUNE=0 in the environment keeps
  * the defaults; S3G_TILES="w0,w1,..." fixes them).  Returns the number
  * written. */

@@ -332,9 +332,8 @@ static int alloc_state(s3g_ctx *c, Dev<R> &D, const int *lE, const int *lW, cons
   // Smagorinsky coefficients only for ihd > 1, turbulence state only for iturb > 0
   D.kh = nullptr;
   D.t1 = nullptr;
-  D.d2 = nullptr;
   if (c->P.ihd > 1 && !(D.kh = dalloc<R>(c, n3))) return S3G_ERR_HIP;
-  double **q3[] = {&D.q2, &D.q2p, &D.q2pp, &D.q2l, &D.q2lp, &D.q2lpp, &D.d0, &D.d1};
+  double **q3[] = {&D.q2, &D.q2p, &D.q2pp, &D.q2l, &D.q2lp, &D.q2lpp, &D.d0, &D.d1, &D.d2};
   for (double **p : q3)
     if (!(*p = dalloc<double>(c, c->P.iturb > 0 ? n3 : 1))) return S3G_ERR_HIP;
   std::vector<double> one(n2, 1.0);

@@ -107,7 +107,7 @@ struct Timer {
 namespace s3g {
 // Per-cell kernels with a layer-chunk size; the first TILE_TUNED are autotuned.
 enum TileId {
-  TILE_EXMOM_X, TILE_EXMOM_Y, TILE_SLAYER_H, TILE_VEL, TILE_HFLUX, TILE_VEL2, TILE_RHS, TILE_TUNED,
+  TILE_EXMOM_X, TILE_EXMOM_Y, TILE_SLAYER_H, TILE_VEL, TILE_HFLUX, TILE_VEL2, TILE_RHS, TILE_TURB, TILE_TUNED,
   TILE_SETTRAP = TILE_TUNED, TILE_SAVE, TILE_NUM
 };
 }
@@ -142,10 +142,11 @@ struct s3g_ctx {
   double *h_scal = nullptr;                  // pinned
   // layer-chunk size of each per-cell kernel (autotuned at the first fd call
   // unless S3G_TUNE=0)
-  int kc[s3g::TILE_NUM] = {1, 1, 1, 1, 1, 1, 1, 1, 1};
+  int kc[s3g::TILE_NUM] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
   // exsal: right-hand side per cell + column solve (true), or fused into the
   // column sweep (false); chosen by the autotuner
   bool exsal_split = true;
+  bool turb_split = true;  // turbulence closure: per-cell interface terms + sweep
   bool tune = true, tuned = false;
   // ensemble of lakes (s3g_create_ensemble): column offset of each lake
   // ([nlakes+1]), each lake's km1 and forcing parameters

@@ -876,10 +876,80 @@ __global__ void __launch_bounds__(kBlock) k_hmix(Dev<R> D) {
   }
 }
 
+// Interface quantities of level k (k1s < k <= kms) of column m from the
+// p-level fields: the limited length scale, shear and buoyancy production (the
+// TKE source terms, si3d_mixing.f90:411-440) and the new eddy coefficients
+// (245-290).  Av(k)/Dv(k) are read for the productions before the new values
+// replace them.
+struct TurbLev {
+  double xl, sp, bp, av, dv;
+};
+
+template <typename R>
+__device__ __forceinline__ TurbLev turb_level(const Dev<R> &D, int m, int k, int lW, int lS) {
+  TurbLev L;
+  const double hkm = D.hp[IX(k - 1, m)], hk = D.hp[IX(k, m)];
+  const double dz = 0.5 * (hkm + hk);
+  const double rkm = D.rhop[IX(k - 1, m)], rk = D.rhop[IX(k, m)];
+  const double N2 = -G / (0.5 * (rkm + rk) + 1000.) * ((rkm - rk) / dz);
+  const double dudz = 0.5 * ((double)D.up[IX(k - 1, m)] + (double)D.up[IX(k - 1, lW)] - (double)D.up[IX(k, m)] - (double)D.up[IX(k, lW)]) / dz;
+  const double dvdz = 0.5 * ((double)D.vp[IX(k - 1, m)] + (double)D.vp[IX(k - 1, lS)] - (double)D.vp[IX(k, m)] - (double)D.vp[IX(k, lS)]) / dz;
+  const double q2pk = D.q2p[IX(k, m)], q2lpk = D.q2lp[IX(k, m)];
+  double xl = q2lpk / q2pk;
+  if (N2 > 0.0) xl = dmin(xl, 0.53 * sqrt(q2pk / N2));
+  L.xl = xl;
+  L.sp = (double)D.Av[IX(k, m)] * (dudz * dudz + dvdz * dvdz);
+  L.bp = -(double)D.Dv[IX(k, m)] * N2;
+  // eddy coefficients (UpdateMixingCoefficients evaluates N^2 in this order)
+  const double N2b = -G * ((rkm - rk) / dz) / (0.5 * (rkm + rk) + 1000.);
+  double xlb = q2lpk / q2pk;
+  if (N2b > 0.0) xlb = dmin(xlb, 0.53 * sqrt(q2pk / N2b));
+  double Gh = -N2b * xlb * xlb / q2pk;
+  Gh = dmax(GH_MIN, dmin(GH_MAX, Gh));
+  const double Sh = D.P.t_1 / (1. - Gh * D.P.t_2);
+  const double Sm = (D.P.t_3 + Sh * Gh * D.P.t_4) / (1. - D.P.t_5 * Gh);
+  const double Rt = sqrt(q2pk) * xlb;
+  L.av = Rt * Sm + AV_MOL;
+  L.dv = Rt * Sh + DV_MOL;
+  return L;
+}
+
+// Per cell (split variant): the interface quantities of level k into the
+// FP64 scratch planes (length scale -> d0, shear production -> d1, buoyancy
+// production -> d2) and the new Av/Dv in place; Av/Dv are zero outside
+// k1s < k <= kms.  Columns with at most one wet layer are left alone.
+template <typename R, bool CH>
+__global__ void __launch_bounds__(kBlock) k_turb_cell(Dev<R> D, int nk, int tile) {
+  CellTile t;
+  if (!cell_tile(D.lm, 1, nk, tile, t)) return;
+  const int m = t.c;
+  const int kms = D.kmz[m], k1s = D.k1z[m];
+  if (kms - k1s + 1 <= 1) return;
+  const int lW = D.lW[m], lS = D.lS[m];
+  for (int k = t.k0; k < (CH ? t.k1 : min(t.k1, t.k0 + 1)); ++k) {
+    if (k <= k1s || k > kms) {
+      D.Av[IX(k, m)] = D.Dv[IX(k, m)] = R(0);
+      continue;
+    }
+    const TurbLev L = turb_level(D, m, k, lW, lS);
+    D.d0[IX(k, m)] = L.xl;
+    D.d1[IX(k, m)] = L.sp;
+    D.d2[IX(k, m)] = L.bp;
+    double av = L.av, dv = L.dv;
+    if (k == k1s + 1 && (double)D.hp[IX(k1s, m)] < 1.E-2) av = dmax((double)(R)av, 1.E-4), dv = dmax((double)(R)dv, 1.E-4);
+    D.Av[IX(k, m)] = (R)av;
+    D.Dv[IX(k, m)] = (R)dv;
+  }
+}
+
 // Two-equation turbulence closure: SolveTKEandLengthScale (si3d_mixing.f90:
 // 301-551, tridT 554-612) and the eddy coefficients of UpdateMixingCoefficients
 // (245-290).  FP64 throughout, as the reference's REAL*8 turbulence state.
-template <typename R>
+//
+// SPLIT: the interface quantities and Av/Dv come from k_turb_cell (the sweep
+// reads them from the scratch planes and writes its factors over them);
+// otherwise they are computed one level ahead inside the sweep.
+template <typename R, bool SPLIT>
 __global__ void __launch_bounds__(kBlock) k_turb(Dev<R> D, int istep) {
   const int m = col();
   if (m >= D.lm) return;
@@ -890,39 +960,15 @@ __global__ void __launch_bounds__(kBlock) k_turb(Dev<R> D, int istep) {
   double *E1 = D.d0, *E2 = D.d1;  // forward-sweep factors of the two systems
   double hbelow = 0.0;
   for (int k = k1s + 1; k <= kms; ++k) hbelow += D.hp[IX(k, m)];
-  // Interface quantities of level k (k1s < k <= kms) from the p-level fields:
-  // the limited length scale, shear and buoyancy production (the TKE source
-  // terms, 411-440) and the new eddy coefficients (245-290).  Av(k)/Dv(k) are
-  // read for the productions before the new values replace them.
-  struct Lev {
-    double xl, sp, bp, av, dv;
-  };
-  auto level = [&](int k) -> Lev {
-    Lev L;
-    const double hkm = D.hp[IX(k - 1, m)], hk = D.hp[IX(k, m)];
-    const double dz = 0.5 * (hkm + hk);
-    const double rkm = D.rhop[IX(k - 1, m)], rk = D.rhop[IX(k, m)];
-    const double N2 = -G / (0.5 * (rkm + rk) + 1000.) * ((rkm - rk) / dz);
-    const double dudz = 0.5 * ((double)D.up[IX(k - 1, m)] + (double)D.up[IX(k - 1, lW)] - (double)D.up[IX(k, m)] - (double)D.up[IX(k, lW)]) / dz;
-    const double dvdz = 0.5 * ((double)D.vp[IX(k - 1, m)] + (double)D.vp[IX(k - 1, lS)] - (double)D.vp[IX(k, m)] - (double)D.vp[IX(k, lS)]) / dz;
-    const double q2pk = D.q2p[IX(k, m)], q2lpk = D.q2lp[IX(k, m)];
-    double xl = q2lpk / q2pk;
-    if (N2 > 0.0) xl = dmin(xl, 0.53 * sqrt(q2pk / N2));
-    L.xl = xl;
-    L.sp = (double)D.Av[IX(k, m)] * (dudz * dudz + dvdz * dvdz);
-    L.bp = -(double)D.Dv[IX(k, m)] * N2;
-    // eddy coefficients (UpdateMixingCoefficients evaluates N^2 in this order)
-    const double N2b = -G * ((rkm - rk) / dz) / (0.5 * (rkm + rk) + 1000.);
-    double xlb = q2lpk / q2pk;
-    if (N2b > 0.0) xlb = dmin(xlb, 0.53 * sqrt(q2pk / N2b));
-    double Gh = -N2b * xlb * xlb / q2pk;
-    Gh = dmax(GH_MIN, dmin(GH_MAX, Gh));
-    const double Sh = D.P.t_1 / (1. - Gh * D.P.t_2);
-    const double Sm = (D.P.t_3 + Sh * Gh * D.P.t_4) / (1. - D.P.t_5 * Gh);
-    const double Rt = sqrt(q2pk) * xlb;
-    L.av = Rt * Sm + AV_MOL;
-    L.dv = Rt * Sh + DV_MOL;
-    return L;
+  auto level = [&](int k) -> TurbLev {
+    if (SPLIT) {
+      TurbLev L{};
+      L.xl = D.d0[IX(k, m)];
+      L.sp = D.d1[IX(k, m)];
+      L.bp = D.d2[IX(k, m)];
+      return L;
+    }
+    return turb_level(D, m, k, lW, lS);
   };
   // boundary values (445-457)
   const double b1_23 = pow(B_1, 2. / 3.);
@@ -943,14 +989,14 @@ __global__ void __launch_bounds__(kBlock) k_turb(Dev<R> D, int istep) {
   D.q2l[IX(k1s, m)] = f2;
   E1[IX(k1s, m)] = 0.0;
   E2[IX(k1s, m)] = 0.0;
-  Lev cur = level(k1s + 1);
+  TurbLev cur = level(k1s + 1);
   double q2p_next = D.q2p[IX(k1s + 1, m)];
   double dco_prev = SQ * 0.5 * (0.0 + cur.xl) * sqrt(0.5 * (D.q2p[IX(k1s, m)] + q2p_next));
   double zfromt = 0.0;
   double hkm = D.hp[IX(k1s, m)];
   for (int k = k1s + 1; k <= kms; ++k) {
     const bool more = k + 1 <= kms;
-    Lev nxt{};
+    TurbLev nxt{};
     if (more) nxt = level(k + 1);
     const double xlk = cur.xl, xlp = more ? nxt.xl : 0.0;
     const double q2pk = q2p_next;
@@ -1005,10 +1051,12 @@ __global__ void __launch_bounds__(kBlock) k_turb(Dev<R> D, int istep) {
     D.q2[IX(k, m)] = f1;
     D.q2l[IX(k, m)] = f2;
     // new eddy coefficients of interface k (the surface minimum, 281-284)
-    double av = cur.av, dv = cur.dv;
-    if (k == k1s + 1 && thin_top) av = dmax((double)(R)av, 1.E-4), dv = dmax((double)(R)dv, 1.E-4);
-    D.Av[IX(k, m)] = (R)av;
-    D.Dv[IX(k, m)] = (R)dv;
+    if (!SPLIT) {
+      double av = cur.av, dv = cur.dv;
+      if (k == k1s + 1 && thin_top) av = dmax((double)(R)av, 1.E-4), dv = dmax((double)(R)dv, 1.E-4);
+      D.Av[IX(k, m)] = (R)av;
+      D.Dv[IX(k, m)] = (R)dv;
+    }
     dco_prev = dco;
     hkm = hk;
     cur = nxt;
@@ -1027,8 +1075,10 @@ __global__ void __launch_bounds__(kBlock) k_turb(Dev<R> D, int istep) {
     D.q2[IX(k, m)] = D.q2[IX(k1s, m)];
     D.q2l[IX(k, m)] = D.q2l[IX(k1s, m)];
   }
-  for (int k = 1; k <= k1s; ++k) D.Av[IX(k, m)] = D.Dv[IX(k, m)] = R(0);
-  for (int k = kms + 1; k <= D.km1; ++k) D.Av[IX(k, m)] = D.Dv[IX(k, m)] = R(0);
+  if (!SPLIT) {
+    for (int k = 1; k <= k1s; ++k) D.Av[IX(k, m)] = D.Dv[IX(k, m)] = R(0);
+    for (int k = kms + 1; k <= D.km1; ++k) D.Av[IX(k, m)] = D.Dv[IX(k, m)] = R(0);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1374,6 +1424,18 @@ static void sync_forcing(s3g_ctx *c) {
 }
 
 template <typename R>
+static void turbulence(s3g_ctx *c, int istep) {
+  Dev<R> &D = dev<R>(c);
+  const int L = c->lm;
+  if (c->turb_split) {
+    LAUNCHC("turb_cell", (k_turb_cell<R, false>), (k_turb_cell<R, true>), TILE_TURB, L, c->km1, D);
+    LAUNCH("turbulence", (k_turb<R, true>), L, D, istep);
+  } else {
+    LAUNCH("turbulence", (k_turb<R, false>), L, D, istep);
+  }
+}
+
+template <typename R>
 int phase(s3g_ctx *c, int ph, int n, int istep) {
   const double tz = 1.0 / istep, twodt1 = c->P.twodt * tz;
   if (ph == 0) {
@@ -1411,7 +1473,7 @@ int phase(s3g_ctx *c, int ph, int n, int istep) {
       LAUNCH("smooth_col", (k_smooth_col<R>), L, D);
       break;
     case 13:
-      if (c->P.iturb >= 1) LAUNCH("turbulence", (k_turb<R>), L, D, istep);
+      if (c->P.iturb >= 1) turbulence<R>(c, istep);
       break;
     default: return S3G_ERR_ARG;
   }
@@ -1419,9 +1481,11 @@ int phase(s3g_ctx *c, int ph, int n, int istep) {
 }
 
 // fd, si3d_procedures.f90:382-575.
-// Tile autotuner: for each per-cell kernel, time every layer-chunk size on the
-// current state (one untimed launch, then the best of three) and keep the
-// fastest.  The kernel's outputs are saved before and restored after, so
+// Autotuner: for each per-cell kernel, time every tile (layer-chunk size and
+// grid order) on the current state (one untimed launch, then the best of
+// three) and keep the fastest; then choose between the split (per-cell +
+// column sweep) and fused forms of exsal/imsal and of the turbulence closure.
+// The outputs of every timed launch are saved before and restored after, so
 // tuning leaves the state exactly as it found it.
 template <typename R>
 int autotune(s3g_ctx *c) {
@@ -1432,8 +1496,9 @@ int autotune(s3g_ctx *c) {
   const int cand[] = {1, 2, 4, 8, 16, 256 + 1, 256 + 4, 512 + 1, 512 + 2, 512 + 4, 768 + 1, 768 + 2, 768 + 4};
   const bool timing = c->timing;
   c->timing = false;
-  R *save = nullptr;
-  if (hipMalloc(&save, 5 * n3 * sizeof(R)) != hipSuccess) {
+  char *save = nullptr;
+  const size_t save_bytes = 6 * n3 * sizeof(double);
+  if (hipMalloc(&save, save_bytes) != hipSuccess) {
     c->timing = timing;
     set_error("autotune: no device memory for the output snapshot");
     return S3G_ERR_HIP;
@@ -1441,20 +1506,50 @@ int autotune(s3g_ctx *c) {
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
+  struct Out {
+    void *p;
+    size_t bytes;
+  };
+  const size_t bR = n3 * sizeof(R), bD = n3 * sizeof(double);
+  auto snapshot = [&](const std::vector<Out> &outs, bool restore) {
+    size_t o = 0;
+    for (const Out &x : outs) {
+      if (restore)
+        hipMemcpyAsync(x.p, save + o, x.bytes, hipMemcpyDeviceToDevice, c->stream);
+      else
+        hipMemcpyAsync(save + o, x.p, x.bytes, hipMemcpyDeviceToDevice, c->stream);
+      o += x.bytes;
+    }
+  };
+  auto time_it = [&](auto &&launch, int reps) {
+    float best = 1e30f;
+    launch();
+    for (int rep = 0; rep < reps; ++rep) {
+      hipEventRecord(e0, c->stream);
+      launch();
+      hipEventRecord(e1, c->stream);
+      hipEventSynchronize(e1);
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, e0, e1);
+      best = std::min(best, ms);
+    }
+    return best;
+  };
   for (int tid = 0; tid < TILE_TUNED; ++tid) {
     if ((tid == TILE_HFLUX || tid == TILE_RHS) && c->P.isal == 0) continue;
-    std::vector<R *> outs;
+    if (tid == TILE_TURB && c->P.iturb < 1) continue;
+    std::vector<Out> outs;
     switch (tid) {
-      case TILE_EXMOM_X: outs = {D.ex}; break;
-      case TILE_EXMOM_Y: outs = {D.ey}; break;
-      case TILE_SLAYER_H: outs = {D.h, D.hu, D.hv}; break;
-      case TILE_VEL: outs = {D.uh, D.vh}; break;
-      case TILE_HFLUX: outs = {D.fx_ew, D.fx_ns}; break;
-      case TILE_RHS: outs = {D.bex}; break;
-      default: outs = {D.h, D.hu, D.hv, D.uh, D.vh}; break;
+      case TILE_EXMOM_X: outs = {{D.ex, bR}}; break;
+      case TILE_EXMOM_Y: outs = {{D.ey, bR}}; break;
+      case TILE_SLAYER_H: outs = {{D.h, bR}, {D.hu, bR}, {D.hv, bR}}; break;
+      case TILE_VEL: outs = {{D.uh, bR}, {D.vh, bR}}; break;
+      case TILE_HFLUX: outs = {{D.fx_ew, bR}, {D.fx_ns, bR}}; break;
+      case TILE_RHS: outs = {{D.bex, bR}}; break;
+      case TILE_TURB: outs = {{D.Av, bR}, {D.Dv, bR}}; break;  // d0..d2 are scratch
+      default: outs = {{D.h, bR}, {D.hu, bR}, {D.hv, bR}, {D.uh, bR}, {D.vh, bR}}; break;
     }
-    for (size_t i = 0; i < outs.size(); ++i)
-      hipMemcpyAsync(save + i * n3, outs[i], n3 * sizeof(R), hipMemcpyDeviceToDevice, c->stream);
+    snapshot(outs, false);
     auto launch = [&]() {
       switch (tid) {
         case TILE_EXMOM_X: LAUNCHC("", (k_exmom<R, true, false>), (k_exmom<R, true, true>), TILE_EXMOM_X, L, c->km - 1, D, twodt1); break;
@@ -1463,6 +1558,7 @@ int autotune(s3g_ctx *c) {
         case TILE_VEL: LAUNCHC("", (k_vel<R, false>), (k_vel<R, true>), TILE_VEL, L, c->km - 1, D, tz); break;
         case TILE_HFLUX: LAUNCHC("", (k_exsal_hflux<R, false>), (k_exsal_hflux<R, true>), TILE_HFLUX, L, c->km - 1, D, twodt1); break;
         case TILE_RHS: LAUNCHC("", (k_exsal_rhs<R, false>), (k_exsal_rhs<R, true>), TILE_RHS, L, c->km - 1, D, twodt1); break;
+        case TILE_TURB: LAUNCHC("", (k_turb_cell<R, false>), (k_turb_cell<R, true>), TILE_TURB, L, c->km1, D); break;
         default: LAUNCHC("", (k_vel2<R, false>), (k_vel2<R, true>), TILE_VEL2, L, c->km - 1, D); break;
       }
     };
@@ -1470,48 +1566,42 @@ int autotune(s3g_ctx *c) {
     float best = 1e30f;
     for (int kc : cand) {
       c->kc[tid] = kc;
-      launch();
-      for (int rep = 0; rep < 3; ++rep) {
-        hipEventRecord(e0, c->stream);
-        launch();
-        hipEventRecord(e1, c->stream);
-        hipEventSynchronize(e1);
-        float ms = 0.f;
-        hipEventElapsedTime(&ms, e0, e1);
-        if (ms < best) best = ms, best_kc = kc;
-      }
+      const float ms = time_it(launch, 3);
+      if (ms < best) best = ms, best_kc = kc;
     }
     c->kc[tid] = best_kc;
-    for (size_t i = 0; i < outs.size(); ++i)
-      hipMemcpyAsync(outs[i], save + i * n3, n3 * sizeof(R), hipMemcpyDeviceToDevice, c->stream);
+    snapshot(outs, true);
     if (c->debug) std::fprintf(stderr, "[s3g] tile %d: kc=%d order=%d (%.3f ms)\n", tid, best_kc & 255, best_kc >> 8, best);
   }
-  // exsal: per-cell right-hand side + column solve, or the fused column sweep
+  // split or fused exsal/imsal
   if (c->P.isal != 0) {
-    std::vector<R *> outs = {D.sal, D.t0, D.bex};
-    for (size_t i = 0; i < outs.size(); ++i)
-      hipMemcpyAsync(save + i * n3, outs[i], n3 * sizeof(R), hipMemcpyDeviceToDevice, c->stream);
-    float best[2] = {1e30f, 1e30f};
-    for (int v = 0; v < 2; ++v) {
-      for (int rep = 0; rep < 4; ++rep) {
-        hipEventRecord(e0, c->stream);
-        if (v == 0) {
-          LAUNCHC("", (k_exsal_rhs<R, false>), (k_exsal_rhs<R, true>), TILE_RHS, L, c->km - 1, D, twodt1);
-          LAUNCH("", (k_exsal_imsal<R, false, true>), L, D, twodt1);
-        } else {
-          LAUNCH("", (k_exsal_imsal<R, true, true>), L, D, twodt1);
-        }
-        hipEventRecord(e1, c->stream);
-        hipEventSynchronize(e1);
-        float ms = 0.f;
-        hipEventElapsedTime(&ms, e0, e1);
-        if (rep > 0) best[v] = std::min(best[v], ms);
-      }
-    }
-    c->exsal_split = best[0] <= best[1];
-    for (size_t i = 0; i < outs.size(); ++i)
-      hipMemcpyAsync(outs[i], save + i * n3, n3 * sizeof(R), hipMemcpyDeviceToDevice, c->stream);
-    if (c->debug) std::fprintf(stderr, "[s3g] exsal: split %.3f ms, fused %.3f ms\n", best[0], best[1]);
+    const std::vector<Out> outs = {{D.sal, bR}, {D.t0, bR}, {D.bex, bR}};
+    snapshot(outs, false);
+    const float split = time_it([&]() {
+      LAUNCHC("", (k_exsal_rhs<R, false>), (k_exsal_rhs<R, true>), TILE_RHS, L, c->km - 1, D, twodt1);
+      LAUNCH("", (k_exsal_imsal<R, false, true>), L, D, twodt1);
+    }, 3);
+    const float fused = time_it([&]() { LAUNCH("", (k_exsal_imsal<R, true, true>), L, D, twodt1); }, 3);
+    c->exsal_split = split <= fused;
+    snapshot(outs, true);
+    if (c->debug) std::fprintf(stderr, "[s3g] exsal: split %.3f ms, fused %.3f ms\n", split, fused);
+  }
+  // split or fused turbulence closure
+  if (c->P.iturb >= 1) {
+    const std::vector<Out> outs = {{D.q2, bD}, {D.q2l, bD}, {D.Av, bR}, {D.Dv, bR}};
+    snapshot(outs, false);
+    const float split = time_it([&]() {
+      LAUNCHC("", (k_turb_cell<R, false>), (k_turb_cell<R, true>), TILE_TURB, L, c->km1, D);
+      LAUNCH("", (k_turb<R, true>), L, D, 1);
+      snapshot(outs, true);  // the closure updates Av/Dv in place: every run starts from the saved state
+    }, 3);
+    const float fused = time_it([&]() {
+      LAUNCH("", (k_turb<R, false>), L, D, 1);
+      snapshot(outs, true);
+    }, 3);
+    c->turb_split = split <= fused;
+    snapshot(outs, true);
+    if (c->debug) std::fprintf(stderr, "[s3g] turbulence: split %.3f ms, fused %.3f ms (each with the state restore)\n", split, fused);
   }
   const int e = hip_check(hipStreamSynchronize(c->stream), "autotune");
   hipEventDestroy(e0);
@@ -1520,6 +1610,7 @@ int autotune(s3g_ctx *c) {
   c->timing = timing;
   return e;
 }
+
 
 template <typename R>
 int fd(s3g_ctx *c, int n, int istep, int lastiter) {
@@ -1562,7 +1653,7 @@ int fd(s3g_ctx *c, int n, int istep, int lastiter) {
     LAUNCH("smooth_zeta", (k_zeta_levels<R, 2>), L + 1, D);
     LAUNCH("smooth_col", (k_smooth_col<R>), L, D);
   }
-  if (c->P.iturb >= 1) LAUNCH("turbulence", (k_turb<R>), L, D, istep);
+  if (c->P.iturb >= 1) turbulence<R>(c, istep);
   int e = hip_check(hipGetLastError(), "fd launch");
   return e < 0 ? e : rc;
 }

@@ -206,7 +206,7 @@ class GpuLake:
         check(self.L.s3g_learn_info(self.h, C.byref(st)), "s3g_learn_info")
         return {name: getattr(st, name) for name, _ in S3gLearnStats._fields_}
 
-    TILE_NAMES = ("exmom_x", "exmom_y", "slayer_h", "vel", "exsal_hflux", "vel2", "exsal_rhs", "settrap", "save")
+    TILE_NAMES = ("exmom_x", "exmom_y", "slayer_h", "vel", "exsal_hflux", "vel2", "exsal_rhs", "turb_cell", "settrap", "save")
 
     def tiles(self):
         kc = (C.c_int * 16)()
