@@ -69,6 +69,9 @@ typedef struct {
   int preconditioner;    /* S3G_PREC_*                                     */
   double pcg_tol;        /* relative FP64 true-residual tolerance          */
   int pcg_maxit;
+  int ntr;               /* conservative tracers (ecomod = 0; exTracer/ImTracer,
+                            si3d_procedures.f90:4090-4484), fields "tracer1".. and
+                            "tracerpp1".. */
 } s3g_params;
 
 /* Input deck for the standalone driver: the content of si3d_inp.txt,
@@ -80,6 +83,8 @@ typedef struct {
   const double *zlevel_file; /* [km1_file]                                  */
   const int *bathy_dm;       /* imm*jmm depths in dm, index (i-2)*jmm+(j-2) */
   const double *init_profile;/* [km1] active scalar per layer               */
+  const double *tracer_profiles; /* [ntr*km1] initial tracers (si3d_init.txt
+                                    columns 2.., si3d_procedures.f90:330-356) */
 } s3g_deck;
 
 typedef struct {

@@ -30,6 +30,7 @@ class OrcDeck(C.Structure):
         ("zlevel_file", C.POINTER(C.c_double)), ("bathy_dm", C.POINTER(C.c_int)),
         ("init_profile", C.POINTER(C.c_double)), ("n_surfbc", C.c_int),
         ("surfbc", C.POINTER(C.c_double)), ("pcg_tol", C.c_double), ("pcg_maxit", C.c_int),
+        ("ntr", C.c_int), ("tracer_profiles", C.POINTER(C.c_double)),
     ]
 
 
@@ -84,7 +85,8 @@ class Oracle:
         self._keep = []
         d = OrcDeck()
         for name, _ in OrcDeck._fields_:
-            if name in ("zlevel_file", "bathy_dm", "init_profile", "surfbc", "km1_file", "n_surfbc"):
+            if name in ("zlevel_file", "bathy_dm", "init_profile", "surfbc", "km1_file", "n_surfbc", "ntr",
+                        "tracer_profiles"):
                 continue
             if name in deck:
                 setattr(d, name, deck[name])
@@ -104,6 +106,11 @@ class Oracle:
             self._keep.append(sb)
             d.surfbc = _dp(sb)
             d.n_surfbc = sb.shape[0]
+        if deck.get("tracer_profiles") is not None:
+            tp = np.ascontiguousarray(deck["tracer_profiles"], dtype=np.float64)
+            self._keep.append(tp)
+            d.ntr = tp.shape[0]
+            d.tracer_profiles = _dp(tp)
         self.deck = deck
         self.h = L.orc_create(C.byref(d))
         if not self.h:

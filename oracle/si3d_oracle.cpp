@@ -152,6 +152,10 @@ struct OrcLake {
   F3 ex, agx, arx, agy, ary, haxpp, haypp, th, th1;
   F3 fx_ew, fx_ns, fx_ud, bex;  // exsal fluxes (Bth2, Bth4, Bth3) and Bex
   F3 q2, q2p, q2pp, q2l, q2lp, q2lpp;
+  // ---- conservative tracers (ecomod = 0): tracer and tracerpp per tracer
+  int ntr = 0;
+  std::vector<double> tprof;  // ntr*km1 initial profiles
+  std::vector<F3> tracer, tracerpp;
   // ---- solver statistics
   int last_iters = 0;
   double last_relres = 0.0;
@@ -183,7 +187,9 @@ struct OrcLake {
   void vel();
   void continuity(int ist);
   void exsal();
+  void exscalar(const F3 &cpp, bool is_tracer);
   void imsal();
+  void imtracer(int nt);
   void vel2();
   void smooth();
   void turbulence(int istep);
@@ -409,6 +415,17 @@ void OrcLake::init_state() {
     for (int k = 1; k <= km1; ++k) salp(k, l) = profile[k];
   sal = salp;
   salpp = salp;
+  // tracers (si3d_procedures.f90:330-356, ecomod = 0): profile column nn+1 of si3d_init.txt
+  ntr = d.tracer_profiles ? std::max(0, d.ntr) : 0;
+  tprof.assign(d.tracer_profiles, d.tracer_profiles + (size_t)ntr * km1);
+  d.tracer_profiles = nullptr;
+  tracer.assign(ntr, F3());
+  for (int nt = 0; nt < ntr; ++nt) {
+    tracer[nt].init(km1, lm1);
+    for (int l = 1; l <= lm1; ++l)
+      for (int k = 1; k <= km1; ++k) tracer[nt](k, l) = tprof[(size_t)nt * km1 + (k - 1)];
+  }
+  tracerpp = tracer;
   for (int l = 1; l <= lm1; ++l)
     for (int k = k1; k <= km1; ++k) rhop(k, l) = densty_s(salp(k, l), 0.00004, zdens(k, l)) - 1000.0;
 
@@ -1078,7 +1095,13 @@ void OrcLake::continuity(int ist) {
 }
 
 // exsal, si3d_procedures.f90:2827-3076 (superbee TVD; no open boundaries).
-void OrcLake::exsal() {
+void OrcLake::exsal() { exscalar(salpp, false); }
+
+// The explicit transport terms of a scalar: exsal (salinity/temperature, cpp =
+// salpp) and exTracer (si3d_procedures.f90:4090-4357, cpp = tracerpp of one
+// tracer, which also sets Bex of the sediment layer, 4342).  Both write the
+// same flux buffers (the reference's thread buffers Bth2/Bth4/Bth3).
+void OrcLake::exscalar(const F3 &salpp, bool is_tracer) {
   const double twodt1 = twodt * tz;
   auto superbee = [](double ratio) {
     return std::max(0.0, std::max(std::min(1.0, 2. * ratio), std::min(2.0, ratio)));
@@ -1157,6 +1180,7 @@ void OrcLake::exsal() {
       if (d.ihd > 0) b += hd;
       bex(k, l) = b;
     }
+    if (is_tracer) bex(kms + 1, l) = hpp(kms + 1, l) * salpp(kms + 1, l) / twodt1;
   }
 }
 
@@ -1188,6 +1212,41 @@ void OrcLake::imsal() {
       for (int k = k1; k <= k1s - 1; ++k) sal(k, l) = x[k1s];
       sal(kms + 1, l) = sal(kms, l);
     }
+  }
+}
+
+// ImTracer, si3d_procedures.f90:4360-4484 (sourcesink = 0 as srcsnk00,
+// no point sources): the imsal matrix without the thin-layer clamp, the
+// sediment layer from its own Bex, negative values clipped to zero.
+void OrcLake::imtracer(int nt) {
+  F3 &tr = tracer[nt];
+  const double twodt1 = twodt * tz;
+  std::vector<double> hn(km1 + 3), a1(km1 + 3), a2(km1 + 3), a3(km1 + 3), ds(km1 + 3), x(km1 + 3);
+  for (int l = 1; l <= lm; ++l) {
+    int kms = kmz[l], k1s = k1z[l], nwl = kms - k1s + 1;
+    for (int k = k1s + 1; k <= kms; ++k) hn[k] = h(k, l);
+    hn[k1s] = twodt1 * wp(k1s, l) + hpp(k1s, l);
+    if (hpp(k1s, l) <= ZERO) hn[k1s + 1] = hpp(k1s + 1, l);
+    if (nwl == 1) {
+      tr(k1s, l) = bex(k1s, l) / (hn[k1s] / twodt1);
+    } else if (nwl >= 2) {
+      for (int k = k1s; k <= kms - 1; ++k) a3[k] = -Dv(k + 1, l) / (hn[k] + hn[k + 1]) * 2.;
+      a3[kms] = 0.0;
+      for (int k = k1s + 1; k <= kms; ++k) a1[k] = -Dv(k, l) / (hn[k - 1] + hn[k]) * 2.;
+      a1[k1s] = 0.0;
+      for (int k = k1s; k <= kms; ++k) {
+        a2[k] = hn[k] / twodt1 - a1[k] - a3[k];
+        ds[k] = bex(k, l);
+      }
+      trid_ef(nwl, &a1[k1s], &a2[k1s], &a3[k1s], &ds[k1s], &x[k1s]);
+      for (int k = k1s; k <= kms; ++k) tr(k, l) = x[k];
+      for (int k = k1; k <= k1s - 1; ++k) tr(k, l) = x[k1s];
+      tr(k1 - 1, l) = x[k1s];
+    }
+    // sediment layer (4449-4452)
+    tr(kms + 1, l) = bex(kms + 1, l) / (hpp(kms + 1, l) / twodt1);
+    for (int k = k1 - 1; k <= kms + 1; ++k)
+      if (tr(k, l) < 0.0) tr(k, l) = 0.0;
   }
 }
 
@@ -1372,9 +1431,9 @@ void OrcLake::turbulence(int istep) {
   }
 }
 
-// fd supervisor, si3d_procedures.f90:382-575 (no open boundaries, point sources or tracers).
+// fd supervisor, si3d_procedures.f90:382-575 (no open boundaries or point
+// sources; conservative tracers only, ecomod = 0).
 int OrcLake::fd(int n, int istep, int lastiter) {
-  (void)lastiter;
   tz = 1.0 / istep;
   surfbc(n, istep);
   hmix();
@@ -1389,6 +1448,12 @@ int OrcLake::fd(int n, int istep, int lastiter) {
     exsal();
     imsal();
   }
+  // non-active scalars on the last iteration only (521-547)
+  if (ntr > 0 && d.niter > 0 && lastiter == 1)
+    for (int nt = 0; nt < ntr; ++nt) {
+      exscalar(tracerpp[nt], true);
+      imtracer(nt);
+    }
   vel2();
   if (d.ismooth >= 1 && istep == 1) smooth();
   turbulence(istep);
@@ -1550,6 +1615,7 @@ void OrcLake::settrap2() {
 
 // save, si3d_procedures.f90:3621-3941 (+ save_2EqTvars, si3d_mixing.f90:819-889).
 void OrcLake::save(int istep) {
+  if (ntr > 0) tracerpp = tracer;  // 3772-3774, 3927-3929
   if (istep == 1) {
     spp[lm1] = spp[lm1];
     for (int l = 1; l <= lm; ++l) {
@@ -1708,6 +1774,10 @@ int orc_phase(OrcLake *o, int phase, int n, int istep, double thrs) {
 
 static const F3 *field3(const OrcLake *o, const std::string &n) {
   OrcLake *m = const_cast<OrcLake *>(o);
+  for (int nt = 0; nt < m->ntr; ++nt) {
+    if (n == "tracer" + std::to_string(nt + 1)) return &m->tracer[nt];
+    if (n == "tracerpp" + std::to_string(nt + 1)) return &m->tracerpp[nt];
+  }
 #define F(x) if (n == #x) return &m->x;
   F(h) F(hp) F(hpp) F(hu) F(hup) F(hupp) F(hv) F(hvp) F(hvpp) F(uh) F(uhp) F(uhpp) F(u) F(up) F(upp)
   F(vh) F(vhp) F(vhpp) F(v) F(vp) F(vpp) F(wp) F(sal) F(salp) F(salpp) F(rhop) F(Av) F(Dv) F(kh)

@@ -35,6 +35,8 @@ typedef struct {
   const double *surfbc;       /* n_surfbc*3: cdw, uair, vair per record        */
   double pcg_tol;             /* relative true-residual tolerance              */
   int pcg_maxit;
+  int ntr;                    /* conservative tracers (ecomod = 0)             */
+  const double *tracer_profiles; /* ntr*km1: profile of tracer nn at nn*km1+k-1 */
 } OrcDeck;
 
 typedef struct OrcLake OrcLake;

@@ -46,14 +46,15 @@ class S3gParams(C.Structure):
                 ("Ay0", C.c_double), ("f", C.c_double), ("theta", C.c_double), ("cd", C.c_double),
                 ("cw", C.c_double), ("wa", C.c_double), ("phi", C.c_double), ("dtsurfbc", C.c_double),
                 ("n_surfbc", C.c_int), ("surfbc", dp), ("precision", C.c_int),
-                ("preconditioner", C.c_int), ("pcg_tol", C.c_double), ("pcg_maxit", C.c_int)]
+                ("preconditioner", C.c_int), ("pcg_tol", C.c_double), ("pcg_maxit", C.c_int),
+                ("ntr", C.c_int)]
 
 
 class S3gDeck(C.Structure):
     _fields_ = [("imm", C.c_int), ("jmm", C.c_int), ("ibathyf", C.c_int), ("km1_file", C.c_int),
                 ("ddz", C.c_double), ("dzmin", C.c_double), ("datadj", C.c_double),
                 ("zetainit", C.c_double), ("zl", C.c_double), ("zlevel_file", dp),
-                ("bathy_dm", ip), ("init_profile", dp)]
+                ("bathy_dm", ip), ("init_profile", dp), ("tracer_profiles", dp)]
 
 
 class S3gStats(C.Structure):

@@ -171,6 +171,7 @@ bool build_host_lake(const s3g_deck *d, HostLake &L) {
 struct HostFields {
   std::vector<float> s, hp, hup, hvp;
   std::vector<double> sal, Av, Dv, q2, q2l;
+  std::vector<std::vector<double>> tr;  // per tracer
 };
 
 void lake_fields(const HostLake &L, const s3g_deck *d, const s3g_params *p, HostFields &F) {
@@ -182,6 +183,11 @@ void lake_fields(const HostLake &L, const s3g_deck *d, const s3g_params *p, Host
   F.sal.assign((size_t)km1 * lm1, 0.0);
   for (int ll = 0; ll < lm1; ++ll)
     for (int k = 0; k < km1; ++k) F.sal[(size_t)ll * km1 + k] = d->init_profile[k];
+  // tracers: profile column nn+1 of si3d_init.txt on every column (330-356)
+  F.tr.assign(std::max(0, p->ntr), std::vector<double>((size_t)km1 * lm1, 0.0));
+  for (int nt = 0; nt < p->ntr && d->tracer_profiles; ++nt)
+    for (int ll = 0; ll < lm1; ++ll)
+      for (int k = 0; k < km1; ++k) F.tr[nt][(size_t)ll * km1 + k] = d->tracer_profiles[(size_t)nt * km1 + k];
   F.Av.assign((size_t)km1 * lm1, 0.0);
   F.Dv.assign((size_t)km1 * lm1, 0.0);
   if (p->iturb == 0) {
@@ -222,6 +228,10 @@ int upload_fields(s3g_ctx *c, const HostFields &F, int iturb) {
   } else {
     for (const char *n : {"q2", "q2p", "q2pp"}) rc |= s3g_set_field_f64(c, n, F.q2.data());
     for (const char *n : {"q2l", "q2lp", "q2lpp"}) rc |= s3g_set_field_f64(c, n, F.q2l.data());
+  }
+  for (size_t nt = 0; nt < F.tr.size(); ++nt) {
+    rc |= s3g_set_field_f64(c, ("tracer" + std::to_string(nt + 1)).c_str(), F.tr[nt].data());
+    rc |= s3g_set_field_f64(c, ("tracerpp" + std::to_string(nt + 1)).c_str(), F.tr[nt].data());
   }
   if (rc == 0) rc = s3g::init_density(c);
   return rc;
@@ -299,7 +309,7 @@ extern "C" int s3g_create_ensemble(int nlakes, const s3g_deck *decks, const s3g_
                       q.beta == p0.beta && q.Av0 == p0.Av0 && q.Dv0 == p0.Dv0 && q.Ax0 == p0.Ax0 &&
                       q.Ay0 == p0.Ay0 && q.f == p0.f && q.theta == p0.theta && q.cd == p0.cd &&
                       q.precision == p0.precision && q.preconditioner == p0.preconditioner &&
-                      q.pcg_tol == p0.pcg_tol && q.pcg_maxit == p0.pcg_maxit;
+                      q.pcg_tol == p0.pcg_tol && q.pcg_maxit == p0.pcg_maxit && q.ntr == p0.ntr;
     if (!same) {
       s3g::set_error("s3g_create_ensemble: lakes may differ only in their decks and forcing");
       return S3G_ERR_UNSUPPORTED;
@@ -347,6 +357,7 @@ extern "C" int s3g_create_ensemble(int nlakes, const s3g_deck *decks, const s3g_
     GF.q2.assign((size_t)km1 * lm1, 0.0);
     GF.q2l.assign((size_t)km1 * lm1, 0.0);
   }
+  GF.tr.assign(std::max(0, p0.ntr), std::vector<double>((size_t)km1 * lm1, 0.0));
   for (int i = 0; i < nlakes; ++i) {
     const HostLake &L = Ls[i];
     const int li1 = L.lm + 1, o = off[i];
@@ -390,6 +401,7 @@ extern "C" int s3g_create_ensemble(int nlakes, const s3g_deck *decks, const s3g_
           GF.q2[dst] = F.q2[src];
           GF.q2l[dst] = F.q2l[src];
         }
+        for (size_t nt = 0; nt < GF.tr.size(); ++nt) GF.tr[nt][dst] = F.tr[nt][src];
       }
     }
   }

@@ -148,8 +148,8 @@ struct Field {
 };
 
 template <typename R>
-static std::vector<Field> field_table(Dev<R> &D) {
-  return {
+static std::vector<Field> field_table(s3g_ctx *c, Dev<R> &D) {
+  std::vector<Field> t = {
       {"h", K3R, D.h},         {"hp", K3R, D.hp},       {"hpp", K3R, D.hpp},     {"hu", K3R, D.hu},
       {"hup", K3R, D.hup},     {"hupp", K3R, D.hupp},   {"hv", K3R, D.hv},       {"hvp", K3R, D.hvp},
       {"hvpp", K3R, D.hvpp},   {"uh", K3R, D.uh},       {"uhp", K3R, D.uhp},     {"uhpp", K3R, D.uhpp},
@@ -167,10 +167,16 @@ static std::vector<Field> field_table(Dev<R> &D) {
       {"hhv", K2D, (void *)D.hhv}, {"kmz", KI, (void *)D.kmz}, {"k1z", KI, D.k1z}, {"k1u", KI, D.k1u},
       {"k1v", KI, D.k1v},
   };
+  const size_t n3 = (size_t)c->KP * c->LP;
+  for (int nt = 0; nt < c->ntr; ++nt) {
+    t.push_back({c->tracer_names[2 * nt].c_str(), K3R, D.tr + nt * n3});
+    t.push_back({c->tracer_names[2 * nt + 1].c_str(), K3R, D.trpp + nt * n3});
+  }
+  return t;
 }
 
 static bool find_field(s3g_ctx *c, const char *name, Field &out) {
-  auto tab = (c->precision == S3G_FP64) ? field_table(c->d64) : field_table(c->d32);
+  auto tab = (c->precision == S3G_FP64) ? field_table(c, c->d64) : field_table(c, c->d32);
   for (auto &f : tab)
     if (std::strcmp(f.name, name) == 0) {
       out = f;
@@ -333,6 +339,12 @@ static int alloc_state(s3g_ctx *c, Dev<R> &D, const int *lE, const int *lW, cons
   D.kh = nullptr;
   D.t1 = nullptr;
   if (c->P.ihd > 1 && !(D.kh = dalloc<R>(c, n3))) return S3G_ERR_HIP;
+  D.ntr = c->ntr;
+  D.tr = D.trpp = nullptr;
+  if (c->ntr > 0) {
+    if (!(D.tr = dalloc<R>(c, (size_t)c->ntr * n3)) || !(D.trpp = dalloc<R>(c, (size_t)c->ntr * n3)))
+      return S3G_ERR_HIP;
+  }
   double **q3[] = {&D.q2, &D.q2p, &D.q2pp, &D.q2l, &D.q2lp, &D.q2lpp, &D.d0, &D.d1, &D.d2};
   for (double **p : q3)
     if (!(*p = dalloc<double>(c, c->P.iturb > 0 ? n3 : 1))) return S3G_ERR_HIP;
@@ -382,8 +394,17 @@ int s3g::init_ctx(const s3g_grid *g, const s3g_params *p, int nlakes, const doub
     set_error("s3g_init: no HIP device is visible");
     return S3G_ERR_HIP;
   }
+  if (p->ntr < 0 || p->ntr > 64) {
+    set_error("s3g_init: ntr must be 0..64");
+    return S3G_ERR_ARG;
+  }
   s3g_ctx *c = new s3g_ctx();
   c->nlakes = nlakes;
+  c->ntr = p->ntr;
+  for (int nt = 1; nt <= c->ntr; ++nt) {
+    c->tracer_names.push_back("tracer" + std::to_string(nt));
+    c->tracer_names.push_back("tracerpp" + std::to_string(nt));
+  }
   c->precision = p->precision;
   c->debug = std::getenv("S3G_DEBUG") != nullptr;
   {

@@ -54,6 +54,8 @@ struct Dev {
   R *uh, *uhp, *uhpp, *u, *up, *upp, *vh, *vhp, *vhpp, *v, *vp, *vpp, *wp;
   R *sal, *salp, *salpp, *rhop, *Av, *Dv, *kh, *ex, *ey, *agx, *arx, *agy, *ary;
   R *fx_ew, *fx_ns, *bex, *t0, *t1;
+  R *tr, *trpp;  // ntr tracers, each a full plane stack, one after the other
+  int ntr;
   double *q2, *q2p, *q2pp, *q2l, *q2lp, *q2lpp, *d0, *d1, *d2;
 };
 
@@ -150,6 +152,8 @@ struct s3g_ctx {
   bool tune = true, tuned = false;
   // ensemble of lakes (s3g_create_ensemble): column offset of each lake
   // ([nlakes+1]), each lake's km1 and forcing parameters
+  int ntr = 0;                           // conservative tracers
+  std::vector<std::string> tracer_names;  // "tracer1", "tracerpp1", ...
   int nlakes = 1;
   std::vector<int> lake_off, lake_km1;
   struct LakeForcing {

@@ -42,6 +42,14 @@ __device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : 
 // Each kernel has a chunked instantiation (CH) and a single-layer one used for
 // chunk size 1: a loop lets the compiler hoist the column's address arithmetic,
 // which can cost more registers (occupancy) than it saves.
+// The scalar a transport kernel advances: salinity/temperature (salpp -> sal)
+// or one tracer (tracerpp -> tracer).
+template <typename R>
+struct Scal {
+  const R *cpp;
+  R *cn;
+};
+
 struct CellTile {
   int c, k0, k1;  // column, layers [k0, k1)
 };
@@ -627,7 +635,7 @@ __device__ __forceinline__ double tvd_flux(double vel, double s1, double s2, dou
 // column, si3d_procedures.f90:2885-2975.  Only the wet range of each column is
 // rewritten; the arrays persist across calls like the reference's Bth2/Bth4.
 template <typename R>
-__device__ __forceinline__ void exsal_hflux_cell(const Dev<R> &D, int c, int k, double twodt1) {
+__device__ __forceinline__ void exsal_hflux_cell(const Dev<R> &D, int c, int k, double twodt1, const R *cpp) {
   const int kms = D.kmz[c], k1s = D.k1z[c];
   if (k < k1s || k > kms) return;
   const int lE = D.lE[c], lW = D.lW[c], lN = D.lN[c], lS = D.lS[c];
@@ -635,17 +643,17 @@ __device__ __forceinline__ void exsal_hflux_cell(const Dev<R> &D, int c, int k, 
   if ((double)D.hup[IX(k, c)] > 0.0) {
     const int lEE = D.lE[lE];
     const double vel = ((double)D.uhpp[IX(k, c)] + (double)D.uh[IX(k, c)]) / 2.;
-    const double s2 = D.salpp[IX(k, c)], s3 = D.salpp[IX(k, lE)];
-    const double s1 = ((double)D.hpp[IX(k, lW)] <= 0.0) ? s2 : (double)D.salpp[IX(k, lW)];
-    const double s4 = ((double)D.hpp[IX(k, lEE)] <= 0.0) ? s3 : (double)D.salpp[IX(k, lEE)];
+    const double s2 = cpp[IX(k, c)], s3 = cpp[IX(k, lE)];
+    const double s1 = ((double)D.hpp[IX(k, lW)] <= 0.0) ? s2 : (double)cpp[IX(k, lW)];
+    const double s4 = ((double)D.hpp[IX(k, lEE)] <= 0.0) ? s3 : (double)cpp[IX(k, lEE)];
     fe = tvd_flux(vel, s1, s2, s3, s4, 0.0, twodt1 / D.P.dx);
   }
   if ((double)D.hvp[IX(k, c)] > 0.0) {
     const int lNN = D.lN[lN];
     const double vel = ((double)D.vhpp[IX(k, c)] + (double)D.vh[IX(k, c)]) / 2.;
-    const double s2 = D.salpp[IX(k, c)], s3 = D.salpp[IX(k, lN)];
-    const double s1 = ((double)D.hpp[IX(k, lS)] <= 0.0) ? s2 : (double)D.salpp[IX(k, lS)];
-    const double s4 = ((double)D.hpp[IX(k, lNN)] <= 0.0) ? s3 : (double)D.salpp[IX(k, lNN)];
+    const double s2 = cpp[IX(k, c)], s3 = cpp[IX(k, lN)];
+    const double s1 = ((double)D.hpp[IX(k, lS)] <= 0.0) ? s2 : (double)cpp[IX(k, lS)];
+    const double s4 = ((double)D.hpp[IX(k, lNN)] <= 0.0) ? s3 : (double)cpp[IX(k, lNN)];
     fn = tvd_flux(vel, s1, s2, s3, s4, 0.0, twodt1 / D.P.dy);
   }
   D.fx_ew[IX(k, c)] = (R)fe;
@@ -653,20 +661,20 @@ __device__ __forceinline__ void exsal_hflux_cell(const Dev<R> &D, int c, int k, 
 }
 
 template <typename R, bool CH>
-__global__ void __launch_bounds__(kBlock) k_exsal_hflux(Dev<R> D, double twodt1, int nk, int tile) {
+__global__ void __launch_bounds__(kBlock) k_exsal_hflux(Dev<R> D, Scal<R> sv, double twodt1, int nk, int tile) {
   CellTile t;
   if (!cell_tile(D.lm, 2, nk, tile, t)) return;
-  for (int k = t.k0; k < (CH ? t.k1 : min(t.k1, t.k0 + 1)); ++k) exsal_hflux_cell<R>(D, t.c, k, twodt1);
+  for (int k = t.k0; k < (CH ? t.k1 : min(t.k1, t.k0 + 1)); ++k) exsal_hflux_cell<R>(D, t.c, k, twodt1, sv.cpp);
 }
 
 // Vertical TVD flux through the top of cell k (si3d_procedures.f90:2999-3035).
 template <typename R>
-__device__ __forceinline__ double vflux(const Dev<R> &D, int c, int k, int k1s, double twodt1) {
+__device__ __forceinline__ double vflux(const Dev<R> &D, int c, int k, int k1s, double twodt1, const R *cpp) {
   if (!((double)D.hp[IX(k - 1, c)] > 0.0)) return 0.0;
   const double vel = (k == k1s) ? 0.0 : (double)D.wp[IX(k, c)];
-  const double s2 = D.salpp[IX(k, c)], s3 = D.salpp[IX(k - 1, c)];
-  const double s4 = ((double)D.hpp[IX(k - 2, c)] <= 0.0) ? s3 : (double)D.salpp[IX(k - 2, c)];
-  const double s1 = ((double)D.hpp[IX(k + 1, c)] <= 0.0) ? s2 : (double)D.salpp[IX(k + 1, c)];
+  const double s2 = cpp[IX(k, c)], s3 = cpp[IX(k - 1, c)];
+  const double s4 = ((double)D.hpp[IX(k - 2, c)] <= 0.0) ? s3 : (double)cpp[IX(k - 2, c)];
+  const double s1 = ((double)D.hpp[IX(k + 1, c)] <= 0.0) ? s2 : (double)cpp[IX(k + 1, c)];
   const double delz = ((double)D.hp[IX(k, c)] + (double)D.hp[IX(k - 1, c)]) / 2.;
   return tvd_flux(vel, s1, s2, s3, s4, 1.0, twodt1 / delz);
 }
@@ -675,8 +683,8 @@ __device__ __forceinline__ double vflux(const Dev<R> &D, int c, int k, int k1s, 
 // the right-hand side (3038-3067), then the implicit vertical diffusion column
 // solve (imsal, 3079-3207, trid1 3292-3352) produces sal.  With DO_SOLVE false
 // only Bex is written (phase-level checking of exsal alone).
-template <typename R, bool DO_EX, bool DO_SOLVE>
-__global__ void __launch_bounds__(kBlock) k_exsal_imsal(Dev<R> D, double twodt1) {
+template <typename R, bool DO_EX, bool DO_SOLVE, bool TR>
+__global__ void __launch_bounds__(kBlock) k_exsal_imsal(Dev<R> D, Scal<R> sv, double twodt1) {
   const int c = col();
   if (c >= D.lm) return;
   const int S = D.lm;
@@ -689,36 +697,44 @@ __global__ void __launch_bounds__(kBlock) k_exsal_imsal(Dev<R> D, double twodt1)
     return (w == S || k > D.kmz[w]) ? 0.0 : a0 * (double)hq[IX(k, w)];
   };
   auto rhs = [&](int k, double fud_k, double fud_kp) -> double {
-    const double sc = D.salpp[IX(k, c)];
+    const double sc = sv.cpp[IX(k, c)];
     double b = (double)D.hpp[IX(k, c)] * sc / twodt1 -
                ((double)D.fx_ew[IX(k, c)] - (lW == S ? 0.0 : (double)D.fx_ew[IX(k, lW)])) / D.P.dx -
                ((double)D.fx_ns[IX(k, c)] - (lS == S ? 0.0 : (double)D.fx_ns[IX(k, lS)])) / D.P.dy -
                (fud_k - fud_kp);
     if (D.P.ihd > 0) {
-      b += (bh(D.hupp, D.P.Ax0, c, k) * ((double)D.salpp[IX(k, lE)] - sc) -
-            bh(D.hupp, D.P.Ax0, lW, k) * (sc - (double)D.salpp[IX(k, lW)])) / D.P.dxdx +
-           (bh(D.hvpp, D.P.Ay0, c, k) * ((double)D.salpp[IX(k, lN)] - sc) -
-            bh(D.hvpp, D.P.Ay0, lS, k) * (sc - (double)D.salpp[IX(k, lS)])) / D.P.dydy;
+      b += (bh(D.hupp, D.P.Ax0, c, k) * ((double)sv.cpp[IX(k, lE)] - sc) -
+            bh(D.hupp, D.P.Ax0, lW, k) * (sc - (double)sv.cpp[IX(k, lW)])) / D.P.dxdx +
+           (bh(D.hvpp, D.P.Ay0, c, k) * ((double)sv.cpp[IX(k, lN)] - sc) -
+            bh(D.hvpp, D.P.Ay0, lS, k) * (sc - (double)sv.cpp[IX(k, lS)])) / D.P.dydy;
     }
     return (double)(R)b;  // the reference stores Bex in the state precision
   };
   if (DO_EX && !DO_SOLVE) {  // exsal alone: Bex for every layer of the column
     for (int k = 1; k <= D.km1; ++k)
       if (k < k1s || k > kms) D.bex[IX(k, c)] = R(0);
-    double fud_k = (k1s <= kms) ? vflux(D, c, k1s, k1s, twodt1) : 0.0;
+    double fud_k = (k1s <= kms) ? vflux(D, c, k1s, k1s, twodt1, sv.cpp) : 0.0;
     for (int k = k1s; k <= kms; ++k) {
-      const double fud_kp = (k + 1 <= kms) ? vflux(D, c, k + 1, k1s, twodt1) : 0.0;
+      const double fud_kp = (k + 1 <= kms) ? vflux(D, c, k + 1, k1s, twodt1, sv.cpp) : 0.0;
       D.bex[IX(k, c)] = (R)rhs(k, fud_k, fud_kp);
       fud_k = fud_kp;
     }
     return;
   }
-  if (nwl < 1) return;
+  if (nwl < 1) {
+    if (TR) {
+      const double bsed = (double)(R)((double)D.hpp[IX(kms + 1, c)] * (double)sv.cpp[IX(kms + 1, c)] / twodt1);
+      sv.cn[IX(kms + 1, c)] = (R)(bsed / ((double)D.hpp[IX(kms + 1, c)] / twodt1));
+      for (int k = D.k1 - 1; k <= kms + 1; ++k)
+        if ((double)sv.cn[IX(k, c)] < 0.0) sv.cn[IX(k, c)] = R(0);
+    }
+    return;
+  }
   // Bex of layer k: computed in the sweep (fused) or read back (imsal alone)
-  double fud = (DO_EX && k1s <= kms) ? vflux(D, c, k1s, k1s, twodt1) : 0.0;
+  double fud = (DO_EX && k1s <= kms) ? vflux(D, c, k1s, k1s, twodt1, sv.cpp) : 0.0;
   auto bex = [&](int k) -> double {
     if (!DO_EX) return (double)D.bex[IX(k, c)];
-    const double fud_kp = (k + 1 <= kms) ? vflux(D, c, k + 1, k1s, twodt1) : 0.0;
+    const double fud_kp = (k + 1 <= kms) ? vflux(D, c, k + 1, k1s, twodt1, sv.cpp) : 0.0;
     const double b = rhs(k, fud, fud_kp);
     fud = fud_kp;
     return b;
@@ -729,11 +745,20 @@ __global__ void __launch_bounds__(kBlock) k_exsal_imsal(Dev<R> D, double twodt1)
     if (k == k1s + 1 && (double)D.hpp[IX(k1s, c)] <= 0.0) return D.hpp[IX(k1s + 1, c)];
     return D.h[IX(k, c)];
   };
+  // tracers: Bex of the sediment layer (exTracer 4342) and the tail of ImTracer
+  // (4449-4457): the sediment value from its own Bex, negatives clipped
+  auto tracer_tail = [&]() {
+    const double bsed = (double)(R)((double)D.hpp[IX(kms + 1, c)] * (double)sv.cpp[IX(kms + 1, c)] / twodt1);
+    sv.cn[IX(kms + 1, c)] = (R)(bsed / ((double)D.hpp[IX(kms + 1, c)] / twodt1));
+    for (int k = D.k1 - 1; k <= kms + 1; ++k)
+      if ((double)sv.cn[IX(k, c)] < 0.0) sv.cn[IX(k, c)] = R(0);
+  };
   if (nwl == 1) {
     double a = hn(k1s) / twodt1;
     double v = bex(k1s) / a;
-    if ((double)D.h[IX(k1s, c)] < 1.E-2) v = D.salpp[IX(k1s, c)];
-    D.sal[IX(k1s, c)] = (R)v;
+    if (!TR && (double)D.h[IX(k1s, c)] < 1.E-2) v = sv.cpp[IX(k1s, c)];
+    sv.cn[IX(k1s, c)] = (R)v;
+    if (TR) tracer_tail();
     return;
   }
   // forward e/f sweep: e -> t0, f -> sal (in place), then back substitution
@@ -756,52 +781,59 @@ __global__ void __launch_bounds__(kBlock) k_exsal_imsal(Dev<R> D, double twodt1)
       ep = 0.0;
     }
     D.t0[IX(k, c)] = (R)ep;
-    D.sal[IX(k, c)] = (R)fp;
+    sv.cn[IX(k, c)] = (R)fp;
     hkm = hk;
     hk = hkp;
     hkp = (k + 2 <= kms) ? hn(k + 2) : 0.0;
   }
   double x = fp;
   for (int k = kms - 1; k >= k1s; --k) {
-    x = (double)D.t0[IX(k, c)] * x + (double)D.sal[IX(k, c)];
-    D.sal[IX(k, c)] = (R)x;
+    x = (double)D.t0[IX(k, c)] * x + (double)sv.cn[IX(k, c)];
+    sv.cn[IX(k, c)] = (R)x;
   }
-  for (int k = D.k1; k <= k1s - 1; ++k) D.sal[IX(k, c)] = (R)x;
-  D.sal[IX(kms + 1, c)] = D.sal[IX(kms, c)];
+  for (int k = D.k1; k <= k1s - 1; ++k) sv.cn[IX(k, c)] = (R)x;
+  if (TR) {
+    sv.cn[IX(D.k1 - 1, c)] = (R)x;
+    tracer_tail();
+  } else {
+    sv.cn[IX(kms + 1, c)] = sv.cn[IX(kms, c)];
+  }
 }
 
 // exsal (part 2) per cell: Bex of layer k (3038-3067), with the vertical TVD
 // fluxes through its top and bottom faces; the column solve (imsal) then reads
 // it back.  One lane per (column, layer) keeps the flux arithmetic off the
 // column sweep's serial path.
-template <typename R, bool CH>
-__global__ void __launch_bounds__(kBlock) k_exsal_rhs(Dev<R> D, double twodt1, int nk, int tile) {
+template <typename R, bool CH, bool TR>
+__global__ void __launch_bounds__(kBlock) k_exsal_rhs(Dev<R> D, Scal<R> sv, double twodt1, int nk, int tile) {
   CellTile t;
   if (!cell_tile(D.lm, 2, nk, tile, t)) return;
   const int c = t.c, S = D.lm;
   const int kms = D.kmz[c], k1s = D.k1z[c];
   const int lE = D.lE[c], lW = D.lW[c], lN = D.lN[c], lS = D.lS[c];
   const int kb = max(t.k0, k1s), ke = CH ? min(t.k1, kms + 1) : min(min(t.k1, t.k0 + 1), kms + 1);
-  double fud_kp = (kb < ke) ? vflux(D, c, kb, k1s, twodt1) : 0.0;  // carried down the chunk
+  double fud_kp = (kb < ke) ? vflux(D, c, kb, k1s, twodt1, sv.cpp) : 0.0;  // carried down the chunk
   for (int k = kb; k < ke; ++k) {
     const double fud_k = fud_kp;
-    fud_kp = (k + 1 <= kms) ? vflux(D, c, k + 1, k1s, twodt1) : 0.0;
+    fud_kp = (k + 1 <= kms) ? vflux(D, c, k + 1, k1s, twodt1, sv.cpp) : 0.0;
     auto bh = [&](const R *hq, double a0, int w) -> double {
       return (w == S || k > D.kmz[w]) ? 0.0 : a0 * (double)hq[IX(k, w)];
     };
-    const double sc = D.salpp[IX(k, c)];
+    const double sc = sv.cpp[IX(k, c)];
     double b = (double)D.hpp[IX(k, c)] * sc / twodt1 -
                ((double)D.fx_ew[IX(k, c)] - (lW == S ? 0.0 : (double)D.fx_ew[IX(k, lW)])) / D.P.dx -
                ((double)D.fx_ns[IX(k, c)] - (lS == S ? 0.0 : (double)D.fx_ns[IX(k, lS)])) / D.P.dy -
                (fud_k - fud_kp);
     if (D.P.ihd > 0) {
-      b += (bh(D.hupp, D.P.Ax0, c) * ((double)D.salpp[IX(k, lE)] - sc) -
-            bh(D.hupp, D.P.Ax0, lW) * (sc - (double)D.salpp[IX(k, lW)])) / D.P.dxdx +
-           (bh(D.hvpp, D.P.Ay0, c) * ((double)D.salpp[IX(k, lN)] - sc) -
-            bh(D.hvpp, D.P.Ay0, lS) * (sc - (double)D.salpp[IX(k, lS)])) / D.P.dydy;
+      b += (bh(D.hupp, D.P.Ax0, c) * ((double)sv.cpp[IX(k, lE)] - sc) -
+            bh(D.hupp, D.P.Ax0, lW) * (sc - (double)sv.cpp[IX(k, lW)])) / D.P.dxdx +
+           (bh(D.hvpp, D.P.Ay0, c) * ((double)sv.cpp[IX(k, lN)] - sc) -
+            bh(D.hvpp, D.P.Ay0, lS) * (sc - (double)sv.cpp[IX(k, lS)])) / D.P.dydy;
     }
     D.bex[IX(k, c)] = (R)b;
   }
+  if (TR && kms >= t.k0 && kms < t.k1)  // Bex of the sediment layer (exTracer 4342)
+    D.bex[IX(kms + 1, c)] = (R)((double)D.hpp[IX(kms + 1, c)] * (double)sv.cpp[IX(kms + 1, c)] / twodt1);
 }
 
 // vel2: layer_h (si3d_procedures.f90:2181-2231) then the top-cell flux split for
@@ -1424,6 +1456,30 @@ static void sync_forcing(s3g_ctx *c) {
 }
 
 template <typename R>
+static Scal<R> sal_of(Dev<R> &D) {
+  return Scal<R>{D.salpp, D.sal};
+}
+
+// Explicit + implicit transport of one scalar (exsal/imsal or exTracer/
+// ImTracer): face fluxes, then the right-hand side per cell + column solve,
+// or both fused in the column sweep (autotuned).
+template <typename R, bool TR>
+static void transport(s3g_ctx *c, Scal<R> S, double twodt1) {
+  Dev<R> &D = dev<R>(c);
+  const int L = c->lm;
+  const bool tr = TR;
+  LAUNCHC(tr ? "tracer_hflux" : "exsal_hflux", (k_exsal_hflux<R, false>), (k_exsal_hflux<R, true>), TILE_HFLUX, L,
+          c->km - 1, D, S, twodt1);
+  if (c->exsal_split) {
+    LAUNCHC(tr ? "tracer_rhs" : "exsal_rhs", (k_exsal_rhs<R, false, TR>), (k_exsal_rhs<R, true, TR>), TILE_RHS, L,
+            c->km - 1, D, S, twodt1);
+    LAUNCH(tr ? "imtracer" : "imsal", (k_exsal_imsal<R, false, true, TR>), L, D, S, twodt1);
+  } else {
+    LAUNCH(tr ? "tracer_imsal" : "exsal_imsal", (k_exsal_imsal<R, true, true, TR>), L, D, S, twodt1);
+  }
+}
+
+template <typename R>
 static void turbulence(s3g_ctx *c, int istep) {
   Dev<R> &D = dev<R>(c);
   const int L = c->lm;
@@ -1464,9 +1520,9 @@ int phase(s3g_ctx *c, int ph, int n, int istep) {
       LAUNCHC("vel", (k_vel<R, false>), (k_vel<R, true>), TILE_VEL, L, c->km - 1, D, tz);
       LAUNCH("continuity1", (k_continuity<R, 1>), L, D);
       break;
-    case 9: LAUNCHC("exsal_hflux", (k_exsal_hflux<R, false>), (k_exsal_hflux<R, true>), TILE_HFLUX, L, c->km - 1, D, twodt1);
-            LAUNCH("exsal_div", (k_exsal_imsal<R, true, false>), L, D, twodt1); break;
-    case 10: LAUNCH("imsal", (k_exsal_imsal<R, false, true>), L, D, twodt1); break;
+    case 9: LAUNCHC("exsal_hflux", (k_exsal_hflux<R, false>), (k_exsal_hflux<R, true>), TILE_HFLUX, L, c->km - 1, D, sal_of(D), twodt1);
+            LAUNCH("exsal_div", (k_exsal_imsal<R, true, false, false>), L, D, sal_of(D), twodt1); break;
+    case 10: LAUNCH("imsal", (k_exsal_imsal<R, false, true, false>), L, D, sal_of(D), twodt1); break;
     case 11: LAUNCHC("vel2", (k_vel2<R, false>), (k_vel2<R, true>), TILE_VEL2, L, c->km - 1, D); break;
     case 12:
       LAUNCH("smooth_zeta", (k_zeta_levels<R, 2>), L + 1, D);
@@ -1556,8 +1612,8 @@ int autotune(s3g_ctx *c) {
         case TILE_EXMOM_Y: LAUNCHC("", (k_exmom<R, false, false>), (k_exmom<R, false, true>), TILE_EXMOM_Y, L, c->km - 1, D, twodt1); break;
         case TILE_SLAYER_H: LAUNCHC("", (k_slayer_h<R, false>), (k_slayer_h<R, true>), TILE_SLAYER_H, L, c->km1, D); break;
         case TILE_VEL: LAUNCHC("", (k_vel<R, false>), (k_vel<R, true>), TILE_VEL, L, c->km - 1, D, tz); break;
-        case TILE_HFLUX: LAUNCHC("", (k_exsal_hflux<R, false>), (k_exsal_hflux<R, true>), TILE_HFLUX, L, c->km - 1, D, twodt1); break;
-        case TILE_RHS: LAUNCHC("", (k_exsal_rhs<R, false>), (k_exsal_rhs<R, true>), TILE_RHS, L, c->km - 1, D, twodt1); break;
+        case TILE_HFLUX: LAUNCHC("", (k_exsal_hflux<R, false>), (k_exsal_hflux<R, true>), TILE_HFLUX, L, c->km - 1, D, sal_of(D), twodt1); break;
+        case TILE_RHS: LAUNCHC("", (k_exsal_rhs<R, false, false>), (k_exsal_rhs<R, true, false>), TILE_RHS, L, c->km - 1, D, sal_of(D), twodt1); break;
         case TILE_TURB: LAUNCHC("", (k_turb_cell<R, false>), (k_turb_cell<R, true>), TILE_TURB, L, c->km1, D); break;
         default: LAUNCHC("", (k_vel2<R, false>), (k_vel2<R, true>), TILE_VEL2, L, c->km - 1, D); break;
       }
@@ -1578,10 +1634,10 @@ int autotune(s3g_ctx *c) {
     const std::vector<Out> outs = {{D.sal, bR}, {D.t0, bR}, {D.bex, bR}};
     snapshot(outs, false);
     const float split = time_it([&]() {
-      LAUNCHC("", (k_exsal_rhs<R, false>), (k_exsal_rhs<R, true>), TILE_RHS, L, c->km - 1, D, twodt1);
-      LAUNCH("", (k_exsal_imsal<R, false, true>), L, D, twodt1);
+      LAUNCHC("", (k_exsal_rhs<R, false, false>), (k_exsal_rhs<R, true, false>), TILE_RHS, L, c->km - 1, D, sal_of(D), twodt1);
+      LAUNCH("", (k_exsal_imsal<R, false, true, false>), L, D, sal_of(D), twodt1);
     }, 3);
-    const float fused = time_it([&]() { LAUNCH("", (k_exsal_imsal<R, true, true>), L, D, twodt1); }, 3);
+    const float fused = time_it([&]() { LAUNCH("", (k_exsal_imsal<R, true, true, false>), L, D, sal_of(D), twodt1); }, 3);
     c->exsal_split = split <= fused;
     snapshot(outs, true);
     if (c->debug) std::fprintf(stderr, "[s3g] exsal: split %.3f ms, fused %.3f ms\n", split, fused);
@@ -1614,7 +1670,6 @@ int autotune(s3g_ctx *c) {
 
 template <typename R>
 int fd(s3g_ctx *c, int n, int istep, int lastiter) {
-  (void)lastiter;
   if (!c->tuned) {
     c->tuned = true;
     if (c->tune) {
@@ -1640,13 +1695,13 @@ int fd(s3g_ctx *c, int n, int istep, int lastiter) {
   LAUNCHC("vel", (k_vel<R, false>), (k_vel<R, true>), TILE_VEL, L, c->km - 1, D, tz);
   LAUNCH("continuity1", (k_continuity<R, 1>), L, D);
   if (c->P.isal != 0) {
-    LAUNCHC("exsal_hflux", (k_exsal_hflux<R, false>), (k_exsal_hflux<R, true>), TILE_HFLUX, L, c->km - 1, D, twodt1);
-    if (c->exsal_split) {
-      LAUNCHC("exsal_rhs", (k_exsal_rhs<R, false>), (k_exsal_rhs<R, true>), TILE_RHS, L, c->km - 1, D, twodt1);
-      LAUNCH("imsal", (k_exsal_imsal<R, false, true>), L, D, twodt1);
-    } else {
-      LAUNCH("exsal_imsal", (k_exsal_imsal<R, true, true>), L, D, twodt1);
-    }
+    transport<R, false>(c, sal_of(D), twodt1);
+  }
+  // non-active scalars on the last iteration only (521-547); sources and
+  // sinks are zero (srcsnk00)
+  if (c->ntr > 0 && c->P.niter > 0 && lastiter == 1) {
+    const size_t n3 = (size_t)c->KP * c->LP;
+    for (int nt = 0; nt < c->ntr; ++nt) transport<R, true>(c, Scal<R>{D.trpp + nt * n3, D.tr + nt * n3}, twodt1);
   }
   LAUNCHC("vel2", (k_vel2<R, false>), (k_vel2<R, true>), TILE_VEL2, L, c->km - 1, D);
   if (c->P.ismooth >= 1 && istep == 1) {
@@ -1692,6 +1747,8 @@ int save(s3g_ctx *c, int istep) {
     LAUNCH("save_top", (k_save_top<R, 2>), L + 1, D);
   }
   LAUNCH("continuity2", (k_continuity<R, 2>), L, D);
+  if (c->ntr > 0)  // tracerpp = tracer (3772-3774, 3927-3929)
+    hipMemcpyAsync(D.trpp, D.tr, (size_t)c->ntr * c->KP * c->LP * sizeof(R), hipMemcpyDeviceToDevice, c->stream);
   return hip_check(hipGetLastError(), "save launch");
 }
 

@@ -35,6 +35,7 @@ MODULE si3d_gpu
       INTEGER(C_INT) :: precision, preconditioner
       REAL(C_DOUBLE) :: pcg_tol
       INTEGER(C_INT) :: pcg_maxit
+      INTEGER(C_INT) :: ntr
    END TYPE s3g_params
 
    INTERFACE

@@ -29,7 +29,7 @@ TWO_D = ("s", "sp", "spp", "sx", "sy", "dd", "qq", "rr", "eagx", "earx", "eagy",
 def make_params(deck, precision="fp64", preconditioner="jacobi", tol=None, maxit=None):
     p = S3gParams()
     for name, _ in S3gParams._fields_:
-        if name in ("surfbc", "n_surfbc", "precision", "preconditioner", "pcg_tol", "pcg_maxit"):
+        if name in ("surfbc", "n_surfbc", "precision", "preconditioner", "pcg_tol", "pcg_maxit", "ntr"):
             continue
         if name in deck:
             setattr(p, name, deck[name])
@@ -39,6 +39,8 @@ def make_params(deck, precision="fp64", preconditioner="jacobi", tol=None, maxit
         keep.append(sb)
         p.surfbc = sb.ctypes.data_as(_lib.dp)
         p.n_surfbc = sb.shape[0]
+    if deck.get("tracer_profiles") is not None:
+        p.ntr = int(np.asarray(deck["tracer_profiles"]).shape[0])
     p.precision = _PREC[precision]
     p.preconditioner = _PRECOND[preconditioner]
     p.pcg_tol = tol if tol is not None else deck.get("pcg_tol", 1e-10)
@@ -60,6 +62,10 @@ def _deck_struct(deck, keep):
         keep.append(z)
         d.zlevel_file = z.ctypes.data_as(_lib.dp)
         d.km1_file = len(z)
+    if deck.get("tracer_profiles") is not None:
+        tp = np.ascontiguousarray(deck["tracer_profiles"], dtype=np.float64)
+        keep.append(tp)
+        d.tracer_profiles = tp.ctypes.data_as(_lib.dp)
     return d
 
 
