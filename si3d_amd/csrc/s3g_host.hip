@@ -410,6 +410,10 @@ int s3g::init_ctx(const s3g_grid *g, const s3g_params *p, int nlakes, const doub
   {
     const char *t = std::getenv("S3G_TUNE");
     c->tune = !(t && t[0] == '0');
+    if (const char *m = std::getenv("S3G_MG")) {
+      int a = 0, b = 0;
+      if (std::sscanf(m, "%d,%d", &a, &b) == 2 && a >= 0 && b >= 1) c->mg_small = a, c->mg_sweeps = b;
+    }
     // S3G_TILES="w0,w1,...": fixed tile words (chunk | order << 8), no tuning
     if (const char *w = std::getenv("S3G_TILES")) {
       c->tune = false;

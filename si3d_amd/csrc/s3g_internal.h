@@ -148,6 +148,9 @@ struct s3g_ctx {
   // exsal: right-hand side per cell + column solve (true), or fused into the
   // column sweep (false); chosen by the autotuner
   bool exsal_split = true;
+  // multigrid: levels of at most mg_small cells run in one workgroup;
+  // Jacobi sweeps on the coarsest level (S3G_MG="small,sweeps")
+  int mg_small = 2048, mg_sweeps = 10;
   bool turb_split = true;  // turbulence closure: per-cell interface terms + sweep
   bool tune = true, tuned = false;
   // ensemble of lakes (s3g_create_ensemble): column offset of each lake

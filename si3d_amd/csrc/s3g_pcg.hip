@@ -252,11 +252,11 @@ __global__ void __launch_bounds__(kBlock) k_line_factor(CgArgs<T> A, T *lc, T *l
 // stay 5-point: diag = sum of the children's diagonals minus twice the
 // couplings inside the aggregate, off-diagonals = sum of the couplings that
 // leave it eastward / northward.  The coarsest level (<= 1024 cells) is
-// smoothed kMgCoarseSweeps times inside one workgroup.  Every step is a fixed
+// smoothed by a few damped Jacobi sweeps (s3g_ctx::mg_sweeps) inside one
+// workgroup.  Every step is a fixed
 // linear operator and pre/post smoothing are identical, so the preconditioner
 // is symmetric positive definite and plain CG applies.
 constexpr double kMgOmega = 0.8;
-constexpr int kMgCoarseSweeps = 10;
 constexpr int kMgCoarseMax = 1024;
 
 template <typename T>
@@ -381,51 +381,11 @@ __global__ void __launch_bounds__(kBlock) k_cg_update_mg(CgArgs<T> A, MgPair<T> 
   }
 }
 
-// Coarsest level: kMgCoarseSweeps damped Jacobi sweeps from zero, one workgroup.
-template <typename T>
-__global__ void __launch_bounds__(kMgCoarseMax) k_mg_coarsest(int n, const int *lE, const int *lW, const int *lN,
-                                                              const int *lS, const T *rr, const T *sx, const T *sy,
-                                                              const T *b, T *xo) {
-  __shared__ double xs[kMgCoarseMax + 1];
-  const int c = threadIdx.x;
-  const bool act = c < n;
-  if (c == 0) xs[n] = 0.0;
-  double bc = 0, rc = 1, sxc = 0, syc = 0, sxw = 0, sys = 0;
-  int w = n, so = n, no = n, e = n;
-  if (act) {
-    bc = b[c];
-    rc = rr[c];
-    sxc = sx[c];
-    syc = sy[c];
-    w = lW[c];
-    so = lS[c];
-    no = lN[c];
-    e = lE[c];
-    sxw = sx[w];
-    sys = sy[so];
-    xs[c] = kMgOmega * bc / rc;
-  }
-  __syncthreads();
-  for (int it = 1; it < kMgCoarseSweeps; ++it) {
-    double xn = 0.0;
-    if (act) {
-      const double ax = rc * xs[c] - sxw * xs[w] - sys * xs[so] - syc * xs[no] - sxc * xs[e];
-      xn = xs[c] + kMgOmega * (bc - ax) / rc;
-    }
-    __syncthreads();
-    if (act) xs[c] = xn;
-    __syncthreads();
-  }
-  if (act) xo[c] = (T)xs[c];
-}
-
-
-// The small levels of the hierarchy (each at most kMgSmall cells) in one
+// The small levels of the hierarchy (each at most s3g_ctx::mg_small cells) in one
 // workgroup of kMgCoarseMax lanes:
 // restriction sweeps, the coarsest smoothing and the prolongation sweeps, with
 // a workgroup barrier between levels.  Same arithmetic as k_mg_down /
-// k_mg_coarsest / k_mg_up.
-constexpr int kMgSmall = 2048;
+// k_mg_up.
 constexpr int kMgMaxLevels = 12;
 
 template <typename T>
@@ -433,6 +393,7 @@ struct MgChain {
   int nlev;                       // levels in the chain (first = the finest handled here)
   MgPair<T> pr[kMgMaxLevels];     // pr[i]: level i (fine) -> level i+1 (coarse), i < nlev-1
   int nc;                         // coarsest size
+  int sweeps;                     // damped Jacobi sweeps on the coarsest level
   const int *cE, *cW, *cN, *cS;
   const T *crr, *csx, *csy, *cb;
   T *cx;
@@ -458,7 +419,7 @@ __global__ void __launch_bounds__(1024) k_mg_chain(MgChain<T> Ch) {
     }
     __syncthreads();
   }
-  // coarsest (at most blockDim.x cells): damped Jacobi sweeps in LDS, as k_mg_coarsest
+  // coarsest (at most blockDim.x cells): damped Jacobi sweeps in LDS
   {
     __shared__ double xs[kMgCoarseMax + 1];
     const int n = Ch.nc, cc = tid;
@@ -473,7 +434,7 @@ __global__ void __launch_bounds__(1024) k_mg_chain(MgChain<T> Ch) {
       xs[cc] = kMgOmega * bc / rc;
     }
     __syncthreads();
-    for (int it = 1; it < kMgCoarseSweeps; ++it) {
+    for (int it = 1; it < Ch.sweeps; ++it) {
       double xn = 0.0;
       if (act) {
         const double ax = rc * xs[cc] - sxw * xs[w] - sys * xs[so] - syc * xs[no] - sxc * xs[e];
@@ -708,10 +669,10 @@ static void apply_prec(s3g_ctx *c, CgArgs<T> &A, const T *rc, int slot, bool dow
   };
   auto grid = [&](int n) { return dim3(std::max(1, std::min((n + kBlock - 1) / kBlock, 2048))); };
   // levels 0 .. lc-1 run one grid-wide kernel per level; lc .. nl-1 (each at
-  // most kMgSmall cells) run in one workgroup.  lc >= 1: level 0 is the CG
+  // most mg_small cells) run in one workgroup.  lc >= 1: level 0 is the CG
   // system, whose prolongation also writes the (r,z) partials.
   int lc = 1;
-  while (lc < nl - 1 && Lv[lc].n > kMgSmall) ++lc;
+  while (lc < nl - 1 && Lv[lc].n > c->mg_small) ++lc;
   if (nl - lc > kMgMaxLevels) lc = nl - kMgMaxLevels;
   for (int l = down0_done ? 1 : 0; l < lc; ++l) {
     MgPair<T> M = pair(l);
@@ -723,6 +684,7 @@ static void apply_prec(s3g_ctx *c, CgArgs<T> &A, const T *rc, int slot, bool dow
     for (int i = 0; i + 1 < Ch.nlev; ++i) Ch.pr[i] = pair(lc + i);
     const MgLevel &Lc = Lv[nl - 1];
     Ch.nc = Lc.n;
+    Ch.sweeps = c->mg_sweeps;
     Ch.cE = Lc.lE; Ch.cW = Lc.lW; Ch.cN = Lc.lN; Ch.cS = Lc.lS;
     Ch.crr = (const T *)Lc.rr; Ch.csx = (const T *)Lc.sx; Ch.csy = (const T *)Lc.sy; Ch.cb = (const T *)Lc.b;
     Ch.cx = (T *)Lc.x;
