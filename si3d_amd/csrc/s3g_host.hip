@@ -412,8 +412,12 @@ int s3g::init_ctx(const s3g_grid *g, const s3g_params *p, int nlakes, const doub
     c->tune = !(t && t[0] == '0');
     if (const char *m = std::getenv("S3G_MG")) {
       int a = 0, b = 0;
-      if (std::sscanf(m, "%d,%d", &a, &b) == 2 && a >= 0 && b >= 1) c->mg_small = a, c->mg_sweeps = b;
+      double cs = 1.0;
+      const int got = std::sscanf(m, "%d,%d,%lf", &a, &b, &cs);
+      if (got >= 2 && a >= 0 && b >= 1) c->mg_small = a, c->mg_sweeps = b;
+      if (got == 3 && cs > 0.0 && cs <= 1.0) c->mg_cs = cs;
     }
+    if (const char *gs = std::getenv("S3G_GUESS")) c->pcg_guess = (gs[0] == '1') ? 1 : 0;
     // S3G_TILES="w0,w1,...": fixed tile words (chunk | order << 8), no tuning
     if (const char *w = std::getenv("S3G_TILES")) {
       c->tune = false;

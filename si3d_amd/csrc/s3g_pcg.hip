@@ -18,9 +18,10 @@
 // confirmed by an FP64 true-residual recompute.
 // Mixed mode: iterative refinement.  The outer loop forms the FP64 true
 // residual r = qq - A zeta; an inner CG with T = float (FP32 matrix copy,
-// vectors and SpMV; FP64 dots) reduces it by `kInnerTol`; zeta += d in FP64.
+// vectors and SpMV; FP64 dots) reduces it by `kInnerTol`, or on the last sweep
+// only as far as the outer tolerance needs; zeta += d in FP64.
 //
-// Preconditioners: Jacobi, or an exact tridiagonal solve along each north-south
+// Preconditioners: multigrid (below), Jacobi, or an exact tridiagonal solve along each north-south
 // line of wet cells (the wet cells of one i-row are consecutive in l, so a line
 // is a contiguous range of columns; one lane per line runs the Thomas sweep,
 // factors computed once per matrix).
@@ -34,7 +35,7 @@
 
 namespace s3g {
 
-constexpr double kInnerTol = 1e-5;  // inner FP32 reduction per refinement sweep
+constexpr double kInnerTol = 1e-5;  // largest inner FP32 reduction per refinement sweep
 
 __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -248,14 +249,17 @@ __global__ void __launch_bounds__(kBlock) k_line_factor(CgArgs<T> A, T *lc, T *l
 
 // ---- aggregation multigrid (preconditioner S3G_PREC_MG) --------------------
 // V(1,1) cycle with damped Jacobi smoothing (omega = kMgOmega) on levels of 2x2
-// aggregates; Galerkin coarse operators A_c = P^T A P with piecewise-constant P
-// stay 5-point: diag = sum of the children's diagonals minus twice the
-// couplings inside the aggregate, off-diagonals = sum of the couplings that
-// leave it eastward / northward.  The coarsest level (<= 1024 cells) is
-// smoothed by a few damped Jacobi sweeps (s3g_ctx::mg_sweeps) inside one
-// workgroup.  Every step is a fixed
-// linear operator and pre/post smoothing are identical, so the preconditioner
-// is symmetric positive definite and plain CG applies.
+// aggregates with piecewise-constant transfers.  The coarse operators stay
+// 5-point: diagonal = the children's mass plus the couplings that leave the
+// aggregate, off-diagonals = the couplings that leave it eastward / northward,
+// each scaled by s3g_ctx::mg_cs.  The Galerkin product P^T A P (mg_cs = 1)
+// counts two fine faces per coarse face and so doubles the coarse diffusion;
+// scaling it back towards the coarse grid's own discretisation (mg_cs = 1/2)
+// restores the coarse-grid correction of smooth errors.  The coarsest level
+// (<= 1024 cells) is smoothed by a few damped Jacobi sweeps
+// (s3g_ctx::mg_sweeps) inside one workgroup.  Every step is a fixed linear
+// operator, every coarse operator is SPD and pre/post smoothing are identical,
+// so the preconditioner is symmetric positive definite and plain CG applies.
 constexpr double kMgOmega = 0.8;
 constexpr int kMgCoarseMax = 1024;
 
@@ -270,15 +274,19 @@ struct MgPair {  // a fine level f and the next coarser level c
   const T *cx;
 };
 
+// cs scales the couplings that leave the aggregate (diagonal = children's
+// mass + the scaled outward couplings): cs = 1 is the Galerkin operator,
+// cs = 1/2 the 5-point operator of the coarse grid's own discretisation (a
+// coarse face crosses two fine faces).  Either keeps A_c SPD.
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_mg_galerkin(MgPair<T> M) {
+__global__ void __launch_bounds__(kBlock) k_mg_galerkin(MgPair<T> M, double cs) {
   for (int C = blockIdx.x * blockDim.x + threadIdx.x; C < M.nc; C += gridDim.x * blockDim.x) {
-    double rr = 0.0, inner = 0.0, sx = 0.0, sy = 0.0;
+    double rr = 0.0, inner = 0.0, sx = 0.0, sy = 0.0, ext = 0.0;
     for (int s = 0; s < 4; ++s) {
       const int f = M.ch[4 * C + s];
       if (f == M.nf) continue;
       rr += (double)M.frr[f];
-      const int e = M.fE[f], n = M.fN[f];
+      const int e = M.fE[f], n = M.fN[f], w = M.fW[f], so = M.fS[f];
       if (e != M.nf) {
         if (M.parent[e] == C) inner += (double)M.fsx[f];
         else sx += (double)M.fsx[f];
@@ -287,10 +295,13 @@ __global__ void __launch_bounds__(kBlock) k_mg_galerkin(MgPair<T> M) {
         if (M.parent[n] == C) inner += (double)M.fsy[f];
         else sy += (double)M.fsy[f];
       }
+      if (w != M.nf && M.parent[w] != C) ext += (double)M.fsx[w];
+      if (so != M.nf && M.parent[so] != C) ext += (double)M.fsy[so];
     }
-    M.crr[C] = (T)(rr - 2.0 * inner);
-    M.csx[C] = (T)sx;
-    M.csy[C] = (T)sy;
+    ext += sx + sy;
+    M.crr[C] = (T)(rr - 2.0 * inner - (1.0 - cs) * ext);
+    M.csx[C] = (T)(cs * sx);
+    M.csy[C] = (T)(cs * sy);
   }
 }
 
@@ -462,8 +473,12 @@ __global__ void __launch_bounds__(1024) k_mg_chain(MgChain<T> Ch) {
 }
 
 // ---- FP64 outer level (zeta, qq, true residual) ---------------------------
-__global__ void __launch_bounds__(kBlock) k_start_guess(double *x, const double *sp, int n) {
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c <= n; c += gridDim.x * blockDim.x) x[c] = (c < n) ? sp[c] : 0.0;
+// x = s, or with `extrap` the linear extrapolation 2 sp - spp of the two
+// latest levels (sp = zeta^n, spp = zeta^{n-1} before a leapfrog solve).
+__global__ void __launch_bounds__(kBlock) k_start_guess(double *x, const double *s, const double *sp,
+                                                        const double *spp, int n, int extrap) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c <= n; c += gridDim.x * blockDim.x)
+    x[c] = (c < n) ? (extrap ? 2.0 * sp[c] - spp[c] : s[c]) : 0.0;
 }
 
 // FP64 true residual; optionally stores it as the right-hand side (cast) of the
@@ -717,7 +732,7 @@ static void mg_build(s3g_ctx *c, CgArgs<T> &A) {
     M.parent = Lv[l + 1].parent_of_fine;
     M.crr = (T *)Lv[l + 1].rr; M.csx = (T *)Lv[l + 1].sx; M.csy = (T *)Lv[l + 1].sy;
     hipLaunchKernelGGL((k_mg_galerkin<T>), dim3(std::max(1, std::min((M.nc + kBlock - 1) / kBlock, 2048))),
-                       dim3(kBlock), 0, c->stream, M);
+                       dim3(kBlock), 0, c->stream, M, c->mg_cs);
   }
 }
 
@@ -808,7 +823,6 @@ static CgArgs<T> cg_args(s3g_ctx *c, const int *lE, const int *lW, const int *lN
 
 template <typename R>
 int solve_free_surface(s3g_ctx *c, double tz) {
-  (void)tz;
   Pcg &P = c->pcg;
   Dev<R> &D = *reinterpret_cast<Dev<R> *>(sizeof(R) == 8 ? (void *)&c->d64 : (void *)&c->d32);
   const bool line = (c->prec_kind == S3G_PREC_LINE) && P.nlines > 0;
@@ -821,7 +835,9 @@ int solve_free_surface(s3g_ctx *c, double tz) {
   // Initial guess: the latest free surface s.  The reference starts from sp
   // (1970); s is the same field on leapfrog steps and the previous estimate of
   // zeta^{n+1} on trapezoidal iterations, so it only shortens the iteration.
-  hipLaunchKernelGGL(k_start_guess, g, b, 0, c->stream, P.x, (const double *)D.s, P.n);
+  const int extrap = (c->pcg_guess == 1 && tz == 1.0) ? 1 : 0;
+  hipLaunchKernelGGL(k_start_guess, g, b, 0, c->stream, P.x, (const double *)D.s, (const double *)D.sp,
+                     (const double *)D.spp, P.n, extrap);
   if (c->learn && learn_before_solve(c, &O, P.x) < 0) return S3G_ERR_HIP;
 
   if (c->precision == S3G_FP64) {
@@ -862,6 +878,10 @@ int solve_free_surface(s3g_ctx *c, double tz) {
         rc = S3G_WARN_ITMAX;
         break;
       }
+      // the inner solve only needs to close the gap to tol (with a margin),
+      // and FP32 holds about 1e-5 of reduction per sweep
+      const double t_in = std::min(0.5, std::max(kInnerTol, 0.3 * tol / relres));
+      A.tol2 = t_in * t_in;
       int it = run_cg(c, A, c->pcg_maxit - total, c->stats.pcg_iters_last / 2);
       if (it < 0) return S3G_ERR_HIP;
       total += it;
