@@ -113,6 +113,25 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_mib": mib, "bytes_read_per_copy": n * 4, "bytes_written_per_copy": n * 4}))
 
 
+def copy_bandwidth(mib=1024, reps=10):
+    """Achievable HBM bandwidth on this device: a plain device-to-device copy of
+    `mib` MiB (read + write bytes / time), timed with events."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return 2 * n * 4 / (ms * 1e-3) / 1e9
+
+
 def main():
     a = parse()
     if a.copy_probe:
@@ -218,7 +237,8 @@ def main():
             roofline = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                         "traffic": traffic, "bytes_per_launch": bytes_launch,
-                        "avg_launch_ms": round(avg_ms, 5)}
+                        "avg_launch_ms": round(avg_ms, 5),
+                        "copy_GBs": round(copy_bandwidth(), 1)}
 
     cpu = None
     if not a.no_cpu_baseline:

@@ -63,8 +63,14 @@ typedef struct {
   double dt, dx, dy;
   int itrap, niter, ismooth, iturb, iadv, itrmom, ihd, ibc, isal, ifsurfbc;
   double beta, Av0, Dv0, Ax0, Ay0, f, theta, cd, cw, wa, phi, dtsurfbc;
-  int n_surfbc;          /* records of si3d_surfbc_W.txt (ifsurfbc = 20)   */
-  const double *surfbc;  /* 3*n_surfbc: cdw, uair, vair per record         */
+  int n_surfbc;          /* records of si3d_surfbc_W.txt / si3d_surfbc.txt */
+  const double *surfbc;  /* nv*n_surfbc values, nv per record in the order
+                            of the file (si3d_boundaryconditions.f90:
+                            4160-4300, 4450-4484):
+                            ifsurfbc 20: cdw uair vair (nv = 3);
+                            1: eta Qsw Qn cdw uair vair (nv = 6);
+                            2: eta Qsw Ta Pa Rh Cc cdw uair vair (nv = 9);
+                            3: eta Qsw Ta Pa Rh Qlw cdw uair vair (nv = 9) */
   int precision;         /* S3G_FP64 or S3G_MIXED                          */
   int preconditioner;    /* S3G_PREC_*                                     */
   double pcg_tol;        /* relative FP64 true-residual tolerance          */

@@ -152,6 +152,7 @@ struct OrcLake {
   F3 ex, agx, arx, agy, ary, haxpp, haypp, th, th1;
   F3 fx_ew, fx_ns, fx_ud, bex;  // exsal fluxes (Bth2, Bth4, Bth3) and Bex
   F3 q2, q2p, q2pp, q2l, q2lp, q2lpp;
+  F3 heat, qswfr;  // HeatSource and QswFr (surface heat budget, ifsurfbc 1-3)
   // ---- conservative tracers (ecomod = 0): tracer and tracerpp per tracer
   int ntr = 0;
   std::vector<double> tprof;  // ntr*km1 initial profiles
@@ -174,6 +175,10 @@ struct OrcLake {
 
   // phases of fd
   void surfbc(int n, int istep);
+  int sbc_nvar() const { return d.ifsurfbc == 20 ? 3 : d.ifsurfbc == 1 ? 6 : (d.ifsurfbc == 2 || d.ifsurfbc == 3) ? 9 : 0; }
+  double solar_fr(double eta, double depth) const;
+  void distribute_qsw(double eta);
+  void distribute_heat(const double *v);
   void hmix();
   void exmom(int ieq);
   void matmom(int ieq);
@@ -206,7 +211,7 @@ bool OrcLake::build(const OrcDeck *deck) {
   if (deck->ibathyf < 0 && deck->zlevel_file)
     zfile.assign(deck->zlevel_file, deck->zlevel_file + deck->km1_file);
   d.zlevel_file = nullptr;
-  if (deck->n_surfbc > 0 && deck->surfbc) sbc.assign(deck->surfbc, deck->surfbc + 3 * deck->n_surfbc);
+  if (deck->n_surfbc > 0 && deck->surfbc) sbc.assign(deck->surfbc, deck->surfbc + (size_t)sbc_nvar() * deck->n_surfbc);
   d.surfbc = nullptr;
 
   dt = d.dt;
@@ -344,7 +349,7 @@ bool OrcLake::build(const OrcDeck *deck) {
                 &uhpp, &u,   &up,   &upp,  &vh,   &vhp,  &vhpp, &v,    &vp,     &vpp,  &wp,
                 &sal,  &salp, &salpp, &rhop, &Av,  &Dv,   &kh,   &ex,   &agx,    &arx,  &agy,
                 &ary,  &haxpp, &haypp, &th, &th1, &fx_ew, &fx_ns, &fx_ud, &bex, &q2, &q2p,
-                &q2pp, &q2l,  &q2lp, &q2lpp};
+                &q2pp, &q2l,  &q2lp, &q2lpp, &heat, &qswfr};
   for (F3 *f : all3) f->init(km1, lm1);
   std::vector<double> *all2[] = {&s,    &sp,   &spp,  &sx,   &sy,   &dd,   &qq,  &rr,
                                  &eagx, &earx, &eagy, &eary, &uair, &vair, &cdw};
@@ -469,25 +474,127 @@ void OrcLake::init_state() {
     q2lp = q2l;
     q2lpp = q2l;
   }
-  if (d.ifsurfbc == 20 && !sbc.empty()) {
-    for (int l = 0; l <= lm1; ++l) cdw[l] = sbc[0], uair[l] = sbc[1], vair[l] = sbc[2];
+  const int nv = sbc_nvar();
+  if (nv > 0 && (int)sbc.size() >= nv) {  // momentum flux terms at t = 0 (surfbc0)
+    for (int l = 0; l <= lm1; ++l) cdw[l] = sbc[nv - 3], uair[l] = sbc[nv - 2], vair[l] = sbc[nv - 1];
   }
 }
 
-// surfbc cases 0 and 20, si3d_boundaryconditions.f90:4508-4515, 4650-4665.
+// surfbc, si3d_boundaryconditions.f90:4490-4669: cases 0 (constant wind),
+// 1-3 (spatially uniform heat budget) and 20 (wind series).  Series values are
+// interpolated in time with parab; the series cases return early on
+// trapezoidal steps after step 1.
 void OrcLake::surfbc(int n, int istep) {
   if (d.ifsurfbc == 0) {
     double ua = -d.wa * std::sin(PI * d.phi / 180.), va = -d.wa * std::cos(PI * d.phi / 180.);
     for (int l = 1; l <= lm; ++l) uair[l] = ua, vair[l] = va, cdw[l] = d.cw;
-  } else if (d.ifsurfbc == 20) {
-    if (n > 1 && istep == 2) return;
-    int np = (int)(sbc.size() / 3);
-    std::vector<double> f1(np + 2), f2(np + 2), f3(np + 2);
-    for (int j = 0; j < np; ++j) f1[j + 1] = sbc[3 * j], f2[j + 1] = sbc[3 * j + 1], f3[j + 1] = sbc[3 * j + 2];
-    double dth = d.dtsurfbc / 3600.;
-    double c = parab(0., thrs, f1.data(), dth), ua = parab(0., thrs, f2.data(), dth),
-           va = parab(0., thrs, f3.data(), dth);
-    for (int l = 0; l <= lm1; ++l) cdw[l] = c, uair[l] = ua, vair[l] = va;
+    return;
+  }
+  const int nv = sbc_nvar();
+  if (nv == 0) return;
+  if (n > 1 && istep == 2) return;
+  int np = (int)(sbc.size() / nv);
+  double dth = d.dtsurfbc / 3600.;
+  std::vector<double> f(np + 2), v(nv);
+  for (int q = 0; q < nv; ++q) {
+    for (int j = 0; j < np; ++j) f[j + 1] = sbc[(size_t)nv * j + q];
+    v[q] = parab(0., thrs, f.data(), dth);
+  }
+  for (int l = 0; l <= lm1; ++l) cdw[l] = v[nv - 3], uair[l] = v[nv - 2], vair[l] = v[nv - 1];
+  if (d.ifsurfbc >= 1 && d.ifsurfbc <= 3) {
+    distribute_qsw(v[0]);
+    distribute_heat(v.data());
+  }
+}
+
+// SolarFr, si3d_boundaryconditions.f90:4786-4816 (Henderson-Sellers eq. 2.25).
+double OrcLake::solar_fr(double eta, double depth) const {
+  const double zA = 0.60;
+  if (eta >= 0.1) {
+    double beta = 0.265 * std::log(eta) + 0.614;
+    return (depth < zA) ? (1. - beta * depth / zA) : (1. - beta) * std::exp(-eta * (depth - zA));
+  }
+  return std::exp(-eta * depth);
+}
+
+// DistributeQswH, si3d_boundaryconditions.f90:4672-4727.
+void OrcLake::distribute_qsw(double eta) {
+  std::vector<double> zfromt(km1 + 3);
+  for (int l = 1; l <= lm; ++l) {
+    int kms = kmz[l], k1s = k1z[l], nwl = kms - k1s + 1;
+    if (nwl == 1) {
+      qswfr(k1s, l) = 1.;
+    } else if (nwl >= 2) {
+      zfromt[k1s] = 0.0;
+      for (int k = k1s + 1; k <= kms + 1; ++k) {
+        zfromt[k] = zfromt[k - 1] + hp(k - 1, l);
+        qswfr(k - 1, l) = solar_fr(eta, zfromt[k - 1]) - solar_fr(eta, zfromt[k]);
+      }
+      double sum = 0.0;
+      for (int k = k1s; k <= kms; ++k) sum += qswfr(k, l);
+      double rem = 1. - sum;
+      if (rem > 1.e-6)
+        for (int k = k1s; k <= kms; ++k) qswfr(k, l) += rem / nwl;
+    }
+  }
+}
+
+static double sat_vapor_pressure(double T) { return 2.1718e10 * std::exp(-4157. / (T - 33.91)); }
+static double latent_heat_vaporization(double T) { return 1.91846e6 * std::pow(T / (T - 33.91), 2.); }
+
+// DistributeMomentumHeatSourcesH, si3d_boundaryconditions.f90:5144-5485,
+// cases 1-3.  v holds the interpolated series values of one record.
+void OrcLake::distribute_heat(const double *v) {
+  const double range[6] = {0.000, 2.2000, 5.0000, 8.0000, 25.00, 50.0};
+  const double a_h[5] = {0.000, 0.9270, 1.1500, 1.1700, 1.652}, b_h[5] = {1.185, 0.0521, 0.0100, 0.0075, -0.017};
+  const double c_h[5] = {0.000, 0.0000, 0.0000, -4.5E-4, 0.000}, p_h[5] = {-0.157, 1.0000, 1.0000, 1.0000, 1.000};
+  const double a_e[5] = {0.000, 0.9690, 1.1800, 1.1960, 1.680}, b_e[5] = {1.230, 0.0521, 0.0100, 0.0080, -0.016};
+  const double c_e[5] = {0.000, 0.0000, 0.0000, -4.0E-4, 0.000}, p_e[5] = {-0.160, 1.0000, 1.0000, 1.0000, 1.000};
+  const double EmWater = 0.97, StephanBoltzman = 5.6697e-8, Al = 0.03, Cp = 4181.6, SpecificHeatAir = 1012.0;
+  const int mode = d.ifsurfbc;
+  const double Qsw = v[1];
+  for (int l = 1; l <= lm; ++l)
+    for (int k = 1; k <= km1; ++k) heat(k, l) = 0.0;
+  double Qn = 0, Ta = 0, Pa = 0, Rh = 0, Cc = 0, Qlw = 0, ea = 0, Qbra = 0;
+  if (mode == 1) {
+    Qn = v[2];
+  } else {
+    Ta = v[2], Pa = v[3], Rh = v[4];
+    if (mode == 2) Cc = v[5]; else Qlw = v[5];
+    ea = sat_vapor_pressure(Ta + 273.) * Rh;
+    if (mode == 2) {
+      double EmAir = 0.642 * std::pow(ea / (Ta + 273.), 0.14285714);
+      EmAir = EmAir * (1. + 0.17 * std::pow(Cc, 2.));
+      double Qbri = EmAir * StephanBoltzman * std::pow(Ta + 273., 4.);
+      Qbra = Qbri * (1. - Al);
+    } else {
+      Qbra = Qlw * (1. - Al);
+    }
+  }
+  for (int l = 1; l <= lm; ++l) {
+    int k1s = k1z[l], kms = kmz[l];
+    if (mode == 1) {
+      heat(k1s, l) = heat(k1s, l) + (Qn - Qsw);
+    } else {
+      double ws = std::sqrt(std::pow(uair[l], 2.) + std::pow(vair[l], 2.));
+      int irg;
+      if (ws > range[0] && ws <= range[1]) irg = 0;
+      else if (ws > range[1] && ws <= range[2]) irg = 1;
+      else if (ws > range[2] && ws <= range[3]) irg = 2;
+      else if (ws > range[3] && ws <= range[4]) irg = 3;
+      else irg = 4;
+      double coeffH = (a_h[irg] + b_h[irg] * std::pow(ws, p_h[irg]) + c_h[irg] * std::pow(ws - 8., 2.)) * 1.e-3;
+      double coeffE = (a_e[irg] + b_e[irg] * std::pow(ws, p_e[irg]) + c_e[irg] * std::pow(ws - 8., 2.)) * 1.e-3;
+      double esw = sat_vapor_pressure(salp(k1s, l) + 273.);
+      double Lv = latent_heat_vaporization(salp(k1s, l) + 273.);
+      heat(k1s, l) = Qbra - EmWater * StephanBoltzman * std::pow(salp(k1s, l) + 273., 4.);
+      heat(k1s, l) = heat(k1s, l) - RHOAIR * Lv * coeffE * (esw - ea) * 0.622 / Pa * ws -
+                     RHOAIR * SpecificHeatAir * coeffH * (salp(k1s, l) - Ta) * ws;
+    }
+    for (int k = k1s; k <= kms; ++k) {
+      heat(k, l) = (heat(k, l) + Qsw * qswfr(k, l)) / ((rhop(k, l) + 1000.) * Cp);
+      if (Qsw < -100.) heat(k, l) = 0.0;
+    }
   }
 }
 
@@ -1184,7 +1291,8 @@ void OrcLake::exscalar(const F3 &salpp, bool is_tracer) {
   }
 }
 
-// imsal, si3d_procedures.f90:3079-3207 (HeatSource = 0, no point sources).
+// imsal, si3d_procedures.f90:3079-3207 (no point sources); HeatSource is
+// zero unless the surface heat budget (ifsurfbc 1-3) sets it.
 void OrcLake::imsal() {
   const double twodt1 = twodt * tz;
   std::vector<double> hn(km1 + 3), a1(km1 + 3), a2(km1 + 3), a3(km1 + 3), ds(km1 + 3), x(km1 + 3);
@@ -1196,7 +1304,7 @@ void OrcLake::imsal() {
     if (hpp(k1s, l) <= ZERO) hn[k1s + 1] = hpp(k1s + 1, l);
     if (nwl == 1) {
       double a = hn[k1s] / twodt1;
-      sal(k1s, l) = bex(k1s, l) / a;
+      sal(k1s, l) = (bex(k1s, l) + heat(k1s, l)) / a;
       if (h(k1s, l) < 1.E-2) sal(k1s, l) = salpp(k1s, l);
     } else {
       for (int k = k1s; k <= kms - 1; ++k) a3[k] = -Dv(k + 1, l) / (hn[k] + hn[k + 1]) * 2.;
@@ -1205,7 +1313,7 @@ void OrcLake::imsal() {
       a1[k1s] = 0.0;
       for (int k = k1s; k <= kms; ++k) {
         a2[k] = hn[k] / twodt1 - a1[k] - a3[k];
-        ds[k] = bex(k, l);
+        ds[k] = bex(k, l) + heat(k, l);
       }
       trid_ef(nwl, &a1[k1s], &a2[k1s], &a3[k1s], &ds[k1s], &x[k1s]);
       for (int k = k1s; k <= kms; ++k) sal(k, l) = x[k];
@@ -1782,6 +1890,8 @@ static const F3 *field3(const OrcLake *o, const std::string &n) {
   F(h) F(hp) F(hpp) F(hu) F(hup) F(hupp) F(hv) F(hvp) F(hvpp) F(uh) F(uhp) F(uhpp) F(u) F(up) F(upp)
   F(vh) F(vhp) F(vhpp) F(v) F(vp) F(vpp) F(wp) F(sal) F(salp) F(salpp) F(rhop) F(Av) F(Dv) F(kh)
   F(ex) F(agx) F(arx) F(agy) F(ary) F(q2) F(q2p) F(q2pp) F(q2l) F(q2lp) F(q2lpp) F(bex)
+  if (n == "HeatSource") return &m->heat;
+  if (n == "QswFr") return &m->qswfr;
 #undef F
   return nullptr;
 }

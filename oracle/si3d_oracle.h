@@ -20,7 +20,8 @@ extern "C" {
 #endif
 
 /* Input deck: the content of si3d_inp.txt, si3d_bathy, si3d_layer.txt,
- * si3d_init.txt and si3d_surfbc_W.txt for the options this path supports. */
+ * si3d_init.txt and si3d_surfbc_W.txt / si3d_surfbc.txt for the options this
+ * path supports. */
 typedef struct {
   int imm, jmm;          /* interior cells in x / y (si3d_bathy header)        */
   int ibathyf;           /* >=0 uniform layers of idz, <0 si3d_layer.txt       */
@@ -31,8 +32,11 @@ typedef struct {
   const double *zlevel_file;  /* km1_file values (ibathyf < 0)                 */
   const int *bathy_dm;        /* imm*jmm depths in dm, index (i-2)*jmm+(j-2)   */
   const double *init_profile; /* km1 values: active scalar per layer k=1..km1  */
-  int n_surfbc;               /* records in the wind series (ifsurfbc = 20)    */
-  const double *surfbc;       /* n_surfbc*3: cdw, uair, vair per record        */
+  int n_surfbc;               /* records of the surface series                 */
+  const double *surfbc;       /* n_surfbc*nv: ifsurfbc 20: cdw uair vair;
+                                 1: eta Qsw Qn cdw uair vair; 2: eta Qsw Ta Pa
+                                 Rh Cc cdw uair vair; 3: eta Qsw Ta Pa Rh Qlw
+                                 cdw uair vair                                 */
   double pcg_tol;             /* relative true-residual tolerance              */
   int pcg_maxit;
   int ntr;                    /* conservative tracers (ecomod = 0)             */

@@ -95,8 +95,87 @@ static double parab(double frstpt, double x, const std::vector<double> &fx1, dou
   return fx1[m] + 0.5 * theta * (fx1[m + 1] - fx1[m - 1] + theta * (fx1[m + 1] + fx1[m - 1] - 2.0 * fx1[m]));
 }
 
-// surfbc cases 0 and 20 (si3d_boundaryconditions.f90:4508-4515, 4650-4665): the
-// forcing is uniform in space, so it is kept as three scalars.
+// Values per record of the surface boundary series: ifsurfbc 20: cdw uair
+// vair; 1: eta Qsw Qn cdw uair vair; 2: eta Qsw Ta Pa Rh Cc cdw uair vair;
+// 3: eta Qsw Ta Pa Rh Qlw cdw uair vair (si3d_boundaryconditions.f90:4160-4300,
+// 4450-4484; nvSurfbcW/P/R, si3d_types.f90:173-175).
+int surfbc_nvar(int ifsurfbc) {
+  switch (ifsurfbc) {
+    case 20: return 3;
+    case 1: return 6;
+    case 2: case 3: return 9;
+    default: return 0;
+  }
+}
+
+// Every variable of the series at time thrs (hours); false with fewer than
+// three records (parab needs three).
+static bool interp_series(const std::vector<double> &sb, int nvar, double dtsurfbc, double thrs, double *v) {
+  const int np = nvar > 0 ? (int)(sb.size() / nvar) : 0;
+  if (np < 3) return false;
+  std::vector<double> f(np + 1);
+  for (int q = 0; q < nvar; ++q) {
+    for (int j = 0; j < np; ++j) f[j + 1] = sb[(size_t)nvar * j + q];
+    v[q] = parab(0., thrs, f, dtsurfbc / 3600.);
+  }
+  return true;
+}
+
+// Saturated vapour pressure (Pa) at T (K), si3d_boundaryconditions.f90:5488-5496.
+static double sat_vapor_pressure(double T) { return 2.1718e10 * std::exp(-4157. / (T - 33.91)); }
+
+// Bulk transfer coefficients of heat (H) and vapour (E) for neutral
+// conditions as a function of wind speed, Kondo (1975), the five wind ranges
+// and constants of si3d_boundaryconditions.f90:5166-5178, 5235-5243.
+static void kondo_coefficients(double ws, double &cE, double &cH) {
+  static const double lim[6] = {0.0, 2.2, 5.0, 8.0, 25.0, 50.0};
+  static const double ah[5] = {0.000, 0.9270, 1.1500, 1.1700, 1.652}, bh[5] = {1.185, 0.0521, 0.0100, 0.0075, -0.017},
+                      ch[5] = {0.000, 0.0000, 0.0000, -4.5E-4, 0.000}, ph[5] = {-0.157, 1., 1., 1., 1.};
+  static const double ae[5] = {0.000, 0.9690, 1.1800, 1.1960, 1.680}, be[5] = {1.230, 0.0521, 0.0100, 0.0080, -0.016},
+                      ce[5] = {0.000, 0.0000, 0.0000, -4.0E-4, 0.000}, pe[5] = {-0.160, 1., 1., 1., 1.};
+  int r = 4;
+  for (int q = 0; q < 4; ++q)
+    if (ws > lim[q] && ws <= lim[q + 1]) {
+      r = q;
+      break;
+    }
+  cH = (ah[r] + bh[r] * std::pow(ws, ph[r]) + ch[r] * (ws - 8.) * (ws - 8.)) * 1.e-3;
+  cE = (ae[r] + be[r] * std::pow(ws, pe[r]) + ce[r] * (ws - 8.) * (ws - 8.)) * 1.e-3;
+}
+
+// The column-independent part of the heat budget (ifsurfbc 1-3,
+// si3d_boundaryconditions.f90:5182-5320) from the interpolated series values.
+static s3g::HeatForce heat_force_of(int mode, const double *v) {
+  constexpr double kStefanBoltzmann = 5.6697e-8, kAlbedoLw = 0.03;
+  s3g::HeatForce H{};
+  H.mode = mode;
+  H.eta = v[0];
+  H.Qsw = v[1];
+  const double uair = v[surfbc_nvar(mode) - 2], vair = v[surfbc_nvar(mode) - 1];
+  if (mode == 1) {
+    H.Qn = v[2];
+    return H;
+  }
+  H.Ta = v[2];
+  H.Pa = v[3];
+  const double Rh = v[4];
+  const double TaK = H.Ta + 273.;
+  H.ea = sat_vapor_pressure(TaK) * Rh;
+  if (mode == 2) {  // longwave from air temperature, humidity and cloud cover
+    const double Cc = v[5];
+    const double em_air = 0.642 * std::pow(H.ea / TaK, 0.14285714) * (1. + 0.17 * Cc * Cc);
+    H.Qbra = em_air * kStefanBoltzmann * TaK * TaK * TaK * TaK * (1. - kAlbedoLw);
+  } else {  // measured incoming longwave
+    H.Qbra = v[5] * (1. - kAlbedoLw);
+  }
+  H.ws = std::sqrt(uair * uair + vair * vair);
+  kondo_coefficients(H.ws, H.coeffE, H.coeffH);
+  return H;
+}
+
+// surfbc cases 0, 1-3 and 20 (si3d_boundaryconditions.f90:4490-4669): the
+// forcing is uniform in space, so wind is kept as three scalars; the heat
+// budget cases also flag HeatSource for recomputation on the device.
 static void forcing_of(const s3g_ctx::LakeForcing &F, double thrs, int n, int istep, double *out) {
   if (F.ifsurfbc == 0) {
     out[0] = -F.wa * std::sin(3.1415926535897932 * F.phi / 180.);
@@ -104,14 +183,11 @@ static void forcing_of(const s3g_ctx::LakeForcing &F, double thrs, int n, int is
     out[2] = F.cw;
   } else if (F.ifsurfbc == 20) {
     if (n > 1 && istep == 2) return;
-    const int np = (int)(F.surfbc.size() / 3);
-    if (np < 3) return;
-    std::vector<double> f1(np + 1), f2(np + 1), f3(np + 1);
-    for (int j = 0; j < np; ++j) f1[j + 1] = F.surfbc[3 * j], f2[j + 1] = F.surfbc[3 * j + 1], f3[j + 1] = F.surfbc[3 * j + 2];
-    const double dth = F.dtsurfbc / 3600.;
-    out[2] = parab(0., thrs, f1, dth);
-    out[0] = parab(0., thrs, f2, dth);
-    out[1] = parab(0., thrs, f3, dth);
+    double v[3];
+    if (!interp_series(F.surfbc, 3, F.dtsurfbc, thrs, v)) return;
+    out[2] = v[0];
+    out[0] = v[1];
+    out[1] = v[2];
   }
 }
 
@@ -121,20 +197,23 @@ void surfbc(s3g_ctx *c, int n, int istep) {
     set_forcing(c);
     return;
   }
-  if (c->P.ifsurfbc == 0) {
+  const int mode = c->P.ifsurfbc;
+  if (mode == 0) {
     c->uair = -c->P.wa * std::sin(3.1415926535897932 * c->P.phi / 180.);
     c->vair = -c->P.wa * std::cos(3.1415926535897932 * c->P.phi / 180.);
     c->cdw = c->P.cw;
-  } else if (c->P.ifsurfbc == 20) {
-    if (n > 1 && istep == 2) return;
-    const int np = (int)(c->surfbc.size() / 3);
-    if (np < 3) return;
-    std::vector<double> f1(np + 1), f2(np + 1), f3(np + 1);
-    for (int j = 0; j < np; ++j) f1[j + 1] = c->surfbc[3 * j], f2[j + 1] = c->surfbc[3 * j + 1], f3[j + 1] = c->surfbc[3 * j + 2];
-    const double dth = c->P.dtsurfbc / 3600.;
-    c->cdw = parab(0., c->thrs, f1, dth);
-    c->uair = parab(0., c->thrs, f2, dth);
-    c->vair = parab(0., c->thrs, f3, dth);
+    return;
+  }
+  if (n > 1 && istep == 2) return;  // only on leapfrog steps (and step 1)
+  const int nv = surfbc_nvar(mode);
+  double v[9];
+  if (!interp_series(c->surfbc, nv, c->P.dtsurfbc, c->thrs, v)) return;
+  c->cdw = v[nv - 3];
+  c->uair = v[nv - 2];
+  c->vair = v[nv - 1];
+  if (mode >= 1 && mode <= 3) {
+    c->heat_force = heat_force_of(mode, v);
+    c->heat_due = true;
   }
 }
 
@@ -165,7 +244,7 @@ static std::vector<Field> field_table(s3g_ctx *c, Dev<R> &D) {
       {"rr", K2D, D.rr},       {"eagx", K2D, D.eagx},   {"earx", K2D, D.earx},   {"eagy", K2D, D.eagy},
       {"eary", K2D, D.eary},   {"hhs", K2D, (void *)D.hhs}, {"hhu", K2D, (void *)D.hhu},
       {"hhv", K2D, (void *)D.hhv}, {"kmz", KI, (void *)D.kmz}, {"k1z", KI, D.k1z}, {"k1u", KI, D.k1u},
-      {"k1v", KI, D.k1v},
+      {"k1v", KI, D.k1v},      {"HeatSource", K3R, D.heat},
   };
   const size_t n3 = (size_t)c->KP * c->LP;
   for (int nt = 0; nt < c->ntr; ++nt) {
@@ -339,6 +418,8 @@ static int alloc_state(s3g_ctx *c, Dev<R> &D, const int *lE, const int *lW, cons
   D.kh = nullptr;
   D.t1 = nullptr;
   if (c->P.ihd > 1 && !(D.kh = dalloc<R>(c, n3))) return S3G_ERR_HIP;
+  D.heat = nullptr;
+  if (c->P.ifsurfbc >= 1 && c->P.ifsurfbc <= 3 && !(D.heat = dalloc<R>(c, n3))) return S3G_ERR_HIP;
   D.ntr = c->ntr;
   D.tr = D.trpp = nullptr;
   if (c->ntr > 0) {
@@ -385,8 +466,13 @@ int s3g::init_ctx(const s3g_grid *g, const s3g_params *p, int nlakes, const doub
     set_error("s3g_init: iturb < 0 (GOTM) is not supported");
     return S3G_ERR_UNSUPPORTED;
   }
-  if (p->ifsurfbc != 0 && p->ifsurfbc != 20) {
-    set_error("s3g_init: only ifsurfbc = 0 or 20 is supported");
+  if (p->ifsurfbc != 0 && p->ifsurfbc != 20 && (p->ifsurfbc < 1 || p->ifsurfbc > 3)) {
+    set_error("s3g_init: ifsurfbc must be 0, 1, 2, 3 or 20 (the Barnes-interpolated met stations of 10/11 are "
+              "not supported)");
+    return S3G_ERR_UNSUPPORTED;
+  }
+  if (nlakes > 1 && p->ifsurfbc >= 1 && p->ifsurfbc <= 3) {
+    set_error("s3g_create_ensemble: the surface heat budget (ifsurfbc 1-3) is not supported in an ensemble");
     return S3G_ERR_UNSUPPORTED;
   }
   int ndev = 0;
@@ -489,11 +575,12 @@ int s3g::init_ctx(const s3g_grid *g, const s3g_params *p, int nlakes, const doub
   P.t_3 = std::pow(B1, -1. / 3.);
   P.t_4 = 9. * A1 * (2. * A1 + A2 * (1. - C2));
   P.t_5 = 9. * A1 * A2;
-  if (p->n_surfbc > 0 && p->surfbc) c->surfbc.assign(p->surfbc, p->surfbc + 3 * p->n_surfbc);
-  if (c->P.ifsurfbc == 20 && c->surfbc.size() >= 3) {
-    c->cdw = c->surfbc[0];
-    c->uair = c->surfbc[1];
-    c->vair = c->surfbc[2];
+  const int nv = surfbc_nvar(p->ifsurfbc);
+  if (p->n_surfbc > 0 && p->surfbc && nv > 0) c->surfbc.assign(p->surfbc, p->surfbc + (size_t)nv * p->n_surfbc);
+  if (nv > 0 && (int)c->surfbc.size() >= nv) {  // momentum flux terms at t = 0 (surfbc0)
+    c->cdw = c->surfbc[nv - 3];
+    c->uair = c->surfbc[nv - 2];
+    c->vair = c->surfbc[nv - 1];
   }
   if (hip_check(hipStreamCreate(&c->stream), "stream") < 0) {
     delete c;

@@ -31,6 +31,21 @@ struct Params {
   double t_1, t_2, t_3, t_4, t_5;  // turbulence closure constants (si3d_mixing.f90:641-645)
 };
 
+// Spatially uniform surface heat-budget forcing of one surfbc call (ifsurfbc
+// 1-3, si3d_boundaryconditions.f90:4523-4598, 5160-5320): what does not depend
+// on the column, evaluated on the host.
+struct HeatForce {
+  int mode;       // ifsurfbc
+  double eta;     // light attenuation coefficient (1/m)
+  double Qsw;     // shortwave flux at the surface (W/m2); < -100 means no data
+  double Qn;      // net heat flux (mode 1)
+  double Qbra;    // absorbed atmospheric longwave (modes 2, 3)
+  double ea;      // air vapour pressure
+  double Ta, Pa;  // air temperature (C), pressure (Pa)
+  double ws;      // wind speed
+  double coeffE, coeffH;  // bulk transfer coefficients (Kondo 1975)
+};
+
 // Everything a kernel needs, passed by value.  R is the storage type of the
 // 3-D hydrodynamic fields (double, or float in the mixed-precision mode).  The
 // free surface, the 2-D solver arrays and the turbulence quantities are FP64
@@ -56,6 +71,7 @@ struct Dev {
   R *fx_ew, *fx_ns, *bex, *t0, *t1;
   R *tr, *trpp;  // ntr tracers, each a full plane stack, one after the other
   int ntr;
+  R *heat;  // HeatSource (surface heat budget, ifsurfbc 1-3), null otherwise
   double *q2, *q2p, *q2pp, *q2l, *q2lp, *q2lpp, *d0, *d1, *d2;
 };
 
@@ -125,6 +141,9 @@ struct s3g_ctx {
   int pcg_maxit = 4000;
   double thrs = 0.0;
   double uair = 0.0, vair = 0.0, cdw = 0.0;
+  // surface heat budget: set by surfbc when HeatSource must be recomputed
+  bool heat_due = false;
+  s3g::HeatForce heat_force{};
   hipStream_t stream = nullptr;
   // device memory
   std::vector<void *> allocs;

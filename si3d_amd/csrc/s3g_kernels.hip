@@ -48,6 +48,7 @@ template <typename R>
 struct Scal {
   const R *cpp;
   R *cn;
+  const R *src;  // source added to the right-hand side (HeatSource), or null
 };
 
 struct CellTile {
@@ -730,13 +731,19 @@ __global__ void __launch_bounds__(kBlock) k_exsal_imsal(Dev<R> D, Scal<R> sv, do
     }
     return;
   }
-  // Bex of layer k: computed in the sweep (fused) or read back (imsal alone)
+  // Bex of layer k: computed in the sweep (fused) or read back (imsal alone),
+  // plus the heat source (dsal = Bex + HeatSource, 3140, 3167)
   double fud = (DO_EX && k1s <= kms) ? vflux(D, c, k1s, k1s, twodt1, sv.cpp) : 0.0;
   auto bex = [&](int k) -> double {
-    if (!DO_EX) return (double)D.bex[IX(k, c)];
-    const double fud_kp = (k + 1 <= kms) ? vflux(D, c, k + 1, k1s, twodt1, sv.cpp) : 0.0;
-    const double b = rhs(k, fud, fud_kp);
-    fud = fud_kp;
+    double b;
+    if (!DO_EX) {
+      b = (double)D.bex[IX(k, c)];
+    } else {
+      const double fud_kp = (k + 1 <= kms) ? vflux(D, c, k + 1, k1s, twodt1, sv.cpp) : 0.0;
+      b = rhs(k, fud, fud_kp);
+      fud = fud_kp;
+    }
+    if (!TR && sv.src) b += (double)sv.src[IX(k, c)];
     return b;
   };
   // imsal: layer thicknesses at time n with the surface-cell corrections (3124-3132)
@@ -881,6 +888,71 @@ __global__ void __launch_bounds__(kBlock) k_vel2(Dev<R> D, int nk, int tile) {
     }
     vel2_face<R, true>(D, c, k, sc);
     vel2_face<R, false>(D, c, k, sc);
+  }
+}
+
+// Fraction of the surface shortwave flux still present at depth z below the
+// free surface, Henderson-Sellers (Engineering Limnology, eq. 2.25) as
+// SolarFr (si3d_boundaryconditions.f90:4786-4816).
+__device__ __forceinline__ double solar_fraction(double eta, double z) {
+  if (eta >= 0.1) {
+    const double beta = 0.265 * log(eta) + 0.614;
+    return (z < 0.60) ? 1. - beta * z / 0.60 : (1. - beta) * exp(-eta * (z - 0.60));
+  }
+  return exp(-eta * z);
+}
+
+// Surface heat budget per column (ifsurfbc 1-3): the shortwave fraction of
+// each wet layer (DistributeQswH, si3d_boundaryconditions.f90:4672-4727) and
+// HeatSource in temperature units (DistributeMomentumHeatSourcesH, 5144-5485):
+// the non-penetrative fluxes enter the surface layer, shortwave is spread over
+// the column.  HeatSource is zero above and below the wet layers.
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_heat_source(Dev<R> D, HeatForce H) {
+  const int c = col();
+  if (c >= D.lm) return;
+  constexpr double kStefanBoltzmann = 5.6697e-8, kEmWater = 0.97, kCp = 4181.6, kCpAir = 1012.0;
+  const int kms = D.kmz[c], k1s = D.k1z[c], nwl = kms - k1s + 1;
+  for (int k = 1; k <= D.km1; ++k) D.heat[IX(k, c)] = R(0);
+  double surf;
+  if (H.mode == 1) {
+    surf = H.Qn - H.Qsw;
+  } else {
+    const double ts = D.salp[IX(k1s, c)];
+    const double T = ts + 273.;
+    const double esw = 2.1718e10 * exp(-4157. / (T - 33.91));
+    const double lv = 1.91846e6 * (T / (T - 33.91)) * (T / (T - 33.91));
+    surf = H.Qbra - kEmWater * kStefanBoltzmann * T * T * T * T -
+           RHOAIR * lv * H.coeffE * (esw - H.ea) * 0.622 / H.Pa * H.ws -
+           RHOAIR * kCpAir * H.coeffH * (ts - H.Ta) * H.ws;
+  }
+  if (nwl < 1) {  // the surface term alone, as the reference leaves it
+    D.heat[IX(k1s, c)] = (R)surf;
+    return;
+  }
+  // shortwave fractions, with what reaches the bottom shared evenly (remFr)
+  double rem = 0.0;
+  if (nwl >= 2) {
+    double z = 0.0, sum = 0.0;
+    for (int k = k1s; k <= kms; ++k) {
+      const double zn = z + (double)D.hp[IX(k, c)];
+      sum += solar_fraction(H.eta, z) - solar_fraction(H.eta, zn);
+      z = zn;
+    }
+    rem = 1. - sum;
+    rem = (rem > 1.e-6) ? rem / nwl : 0.0;
+  }
+  double z = 0.0;
+  for (int k = k1s; k <= kms; ++k) {
+    double fr = 1.0;
+    if (nwl >= 2) {
+      const double zn = z + (double)D.hp[IX(k, c)];
+      fr = solar_fraction(H.eta, z) - solar_fraction(H.eta, zn) + rem;
+      z = zn;
+    }
+    double hs = ((k == k1s ? surf : 0.0) + H.Qsw * fr) / (((double)D.rhop[IX(k, c)] + 1000.) * kCp);
+    if (H.Qsw < -100.) hs = 0.0;
+    D.heat[IX(k, c)] = (R)hs;
   }
 }
 
@@ -1457,7 +1529,7 @@ static void sync_forcing(s3g_ctx *c) {
 
 template <typename R>
 static Scal<R> sal_of(Dev<R> &D) {
-  return Scal<R>{D.salpp, D.sal};
+  return Scal<R>{D.salpp, D.sal, D.heat};
 }
 
 // Explicit + implicit transport of one scalar (exsal/imsal or exTracer/
@@ -1491,11 +1563,22 @@ static void turbulence(s3g_ctx *c, int istep) {
   }
 }
 
+// surfbc on the host, then HeatSource on the device when the heat budget is
+// due (ifsurfbc 1-3 on leapfrog steps and step 1).
+template <typename R>
+static void surface_forcing(s3g_ctx *c, int n, int istep) {
+  surfbc(c, n, istep);
+  if (c->heat_due) {
+    c->heat_due = false;
+    LAUNCH("heat_source", (k_heat_source<R>), c->lm, dev<R>(c), c->heat_force);
+  }
+}
+
 template <typename R>
 int phase(s3g_ctx *c, int ph, int n, int istep) {
   const double tz = 1.0 / istep, twodt1 = c->P.twodt * tz;
   if (ph == 0) {
-    surfbc(c, n, istep);
+    surface_forcing<R>(c, n, istep);
     return 0;
   }
   sync_forcing<R>(c);
@@ -1678,7 +1761,7 @@ int fd(s3g_ctx *c, int n, int istep, int lastiter) {
     }
   }
   const double tz = 1.0 / istep, twodt1 = c->P.twodt * tz;
-  surfbc(c, n, istep);
+  surface_forcing<R>(c, n, istep);
   sync_forcing<R>(c);
   Dev<R> &D = dev<R>(c);
   const int L = c->lm;
@@ -1701,7 +1784,7 @@ int fd(s3g_ctx *c, int n, int istep, int lastiter) {
   // sinks are zero (srcsnk00)
   if (c->ntr > 0 && c->P.niter > 0 && lastiter == 1) {
     const size_t n3 = (size_t)c->KP * c->LP;
-    for (int nt = 0; nt < c->ntr; ++nt) transport<R, true>(c, Scal<R>{D.trpp + nt * n3, D.tr + nt * n3}, twodt1);
+    for (int nt = 0; nt < c->ntr; ++nt) transport<R, true>(c, Scal<R>{D.trpp + nt * n3, D.tr + nt * n3, nullptr}, twodt1);
   }
   LAUNCHC("vel2", (k_vel2<R, false>), (k_vel2<R, true>), TILE_VEL2, L, c->km - 1, D);
   if (c->P.ismooth >= 1 && istep == 1) {

@@ -138,6 +138,34 @@ def wind_series(tl, dtsbc=600.0, amp=0.2, period=86400.0, direction_deg=30.0,
     return np.stack([np.full(n, cdw), ua, va], axis=1)
 
 
+def heat_budget(deck, mode, eta=0.5, qsw_max=800.0, q_night=-150.0, period=86400.0):
+    """The deck with a spatially uniform surface heat budget (ifsurfbc = mode,
+    1, 2 or 3, si3d_surfbc.txt records) over its wind series: a diurnal
+    shortwave cycle max(0, qsw_max cos(2 pi t/period)) (t = 0 at noon), attenuation `eta`
+    (1/m); mode 1 adds a net flux Qn = Qsw + q_night, modes 2 and 3 an air
+    temperature of 20 +- 5 C, 101325 Pa, 70 % humidity, and 30 % cloud cover
+    (2) or 330 W/m2 of incoming longwave (3)."""
+    if mode not in (1, 2, 3):
+        raise ValueError("heat budget mode must be 1, 2 or 3")
+    w = np.asarray(deck["surfbc"], dtype=np.float64)
+    if w.ndim != 2 or w.shape[1] != 3:
+        raise ValueError("heat_budget needs a deck with a wind series (ifsurfbc = 20)")
+    n = w.shape[0]
+    t = np.arange(n) * deck["dtsurfbc"]
+    ph = 2 * np.pi * t / period
+    qsw = np.maximum(0.0, qsw_max * np.cos(ph))
+    if mode == 1:
+        cols = [np.full(n, eta), qsw, qsw + q_night]
+    else:
+        ta = 20.0 + 5.0 * np.cos(ph - 0.5)
+        last = np.full(n, 0.3) if mode == 2 else np.full(n, 330.0)
+        cols = [np.full(n, eta), qsw, ta, np.full(n, 101325.0), np.full(n, 0.7), last]
+    d = dict(deck)
+    d["ifsurfbc"] = mode
+    d["surfbc"] = np.concatenate([np.stack(cols, axis=1), w], axis=1)
+    return d
+
+
 def ellipse(nx=1024, ny=512, dx=100.0, nlayers=60, dt=60.0, nts=2000, seed=1,
             wind_amp=0.2, wind_dir=30.0, f=1e-4, Ax0=1.0, depth=100.0,
             top=0.5, bottom=5.0, iturb=1):

@@ -265,3 +265,143 @@ def test_tracers_conserve_mass_bounds_and_uniformity(oracle_lib):
     assert t1.min() >= 2.0 - 1e-12 and t1.max() <= 10.0 + 1e-12
     assert np.max(np.abs(_wet_values(o, "tracer2") - 5.0)) < 1e-12
     assert np.max(np.abs(t1 - t0)) > 1e-3
+
+
+# ---- surface heat budget (ifsurfbc 1-3) ------------------------------------
+CP_WATER = 4181.6
+SIGMA = 5.6697e-8
+
+
+def _heat_lake(mode, nts=12, uniform=None, **kw):
+    d = lake.heat_budget(small_ellipse(nts=nts), mode, **kw)
+    if uniform is not None:
+        d["init_profile"] = np.full(len(d["init_profile"]), uniform)
+    return d
+
+
+def _column_energy(o):
+    """Per wet column: sum_k HeatSource (rhop + 1000) Cp, i.e. the heat flux
+    (W/m2) the column receives."""
+    hs, rho = o.get("HeatSource"), o.get("rhop")
+    kmz = o.get("kmz").astype(int)
+    k1z = o.get("k1z").astype(int)
+    return np.array([float((hs[l, k1z[l] - 1:kmz[l]] * (rho[l, k1z[l] - 1:kmz[l]] + 1000.0)).sum()) * CP_WATER
+                     for l in range(o.lm)])
+
+
+def test_heat_budget_mode1_column_receives_the_net_flux(oracle_lib):
+    """ifsurfbc = 1: non-penetrative part (Qn - Qsw) in the surface layer plus
+    shortwave spread over the column by fractions that sum to one, so each
+    column receives exactly Qn; the surface layer absorbs the most per metre."""
+    d = _heat_lake(1)
+    o = oracle_lib.Oracle(d)
+    o.step(1)
+    thrs = 2 * d["dt"] / 3600.0
+    o.phase(0, 2, 1, thrs)  # surfbc of step 2's leapfrog call alone: rhop is the one it used
+    sb = np.asarray(d["surfbc"])
+    t = np.arange(sb.shape[0]) * d["dtsurfbc"] / 3600.0
+    qsw = float(np.interp(thrs, t, sb[:, 1]))
+    qn = float(np.interp(thrs, t, sb[:, 2]))
+    assert qsw > 100.0  # daytime: shortwave present
+    e = _column_energy(o)
+    # Qn from the parabolic interpolation of the series; linear interpolation
+    # of the sampled cosine differs by the curvature term only
+    # columns whose bottom absorbs less than 1e-6 of the shortwave keep it
+    # (remFr is only shared out above 1e-6)
+    assert np.ptp(e) <= 1e-6 * qsw + 1e-9 * abs(qn)
+    assert abs(e.max() - qn) < 0.02 * abs(qsw)
+    fr = o.get("QswFr")
+    kmz = o.get("kmz").astype(int)
+    k1z = o.get("k1z").astype(int)
+    deep = [l for l in range(o.lm) if kmz[l] - k1z[l] >= 5]
+    for l in deep[:50]:
+        f = fr[l, k1z[l] - 1:kmz[l]]
+        assert abs(f.sum() - 1.0) < 1e-6
+        assert np.all(f > 0)
+        hp = o.get("hp")[l, k1z[l] - 1:kmz[l]]
+        assert np.argmax(f / hp) == 0  # the surface layer absorbs the most per metre
+
+
+def test_heat_budget_solar_fraction_closed_form(oracle_lib):
+    """eta < 0.1: the fraction of layer k is exp(-eta z_top) - exp(-eta z_bottom)
+    (plus the even share of what reaches the bottom)."""
+    d = _heat_lake(1, eta=0.05)
+    o = oracle_lib.Oracle(d)
+    o.step(1)
+    o.phase(0, 2, 1, 2 * d["dt"] / 3600.0)
+    fr, hp = o.get("QswFr"), o.get("hp")
+    kmz = o.get("kmz").astype(int)
+    k1z = o.get("k1z").astype(int)
+    for l in range(0, o.lm, 37):
+        k0, k1 = k1z[l] - 1, kmz[l]
+        n = k1 - k0
+        if n < 2:
+            continue
+        z = np.concatenate([[0.0], np.cumsum(hp[l, k0:k1])])
+        f = np.exp(-0.05 * z[:-1]) - np.exp(-0.05 * z[1:])
+        rem = 1.0 - f.sum()
+        f = f + (rem / n if rem > 1e-6 else 0.0)
+        np.testing.assert_allclose(fr[l, k0:k1], f, rtol=1e-12, atol=1e-15)
+
+
+def test_heat_budget_equilibrium_surface_flux_vanishes(oracle_lib):
+    """ifsurfbc = 3 with water and air at the same temperature, saturated air
+    and incoming longwave equal to the water's emission: sensible, latent and
+    longwave terms cancel, so with no sunshine HeatSource is zero everywhere;
+    one degree of air-water contrast then gives the Kondo (1975) neutral
+    sensible-heat transfer, rho_air c_p,air C_H U (C_H about 1.1e-3 at 3 m/s)."""
+    ts = 15.0
+    d = _heat_lake(3, uniform=ts, qsw_max=0.0)
+    sb = np.array(d["surfbc"])
+    sb[:, 2] = ts  # Ta
+    sb[:, 4] = 1.0  # Rh
+    sb[:, 5] = 0.97 * SIGMA * (ts + 273.0) ** 4 / (1.0 - 0.03)  # Qlw
+    ws = 3.0
+    sb[:, 6:9] = [1.2e-3, ws, 0.0]
+    d["surfbc"] = sb
+    o = oracle_lib.Oracle(d)
+    o.phase(0, 1, 2, d["dt"] / 3600.0)
+    assert np.max(np.abs(o.get("HeatSource"))) < 1e-12
+    sb2 = sb.copy()
+    sb2[:, 2] = ts - 1.0
+    sb2[:, 4] = 1.0 * (2.1718e10 * np.exp(-4157.0 / (ts + 273.0 - 33.91))) / (
+        2.1718e10 * np.exp(-4157.0 / (ts - 1.0 + 273.0 - 33.91)))  # same vapour pressure as before
+    d["surfbc"] = sb2
+    o = oracle_lib.Oracle(d)
+    o.phase(0, 1, 2, d["dt"] / 3600.0)
+    e = _column_energy(o)
+    sens = -e[0]  # W/m2 lost by the warmer water, longwave of the air unchanged
+    ch = sens / (1.3 * 1012.0 * 1.0 * ws)
+    assert 0.9e-3 < ch < 1.3e-3
+    assert np.max(np.abs(e - e[0])) < 1e-9 * abs(e[0])
+
+
+def test_heat_budget_saturation_pressure_and_latent_heat_textbook():
+    """The reference's vapour-pressure and latent-heat fits against standard
+    table values (WMO: e_s(10 C) = 1228 Pa, e_s(20 C) = 2339 Pa;
+    L_v(20 C) = 2.454e6 J/kg)."""
+    es = lambda t: 2.1718e10 * math.exp(-4157.0 / (t + 273.0 - 33.91))
+    lv = lambda t: 1.91846e6 * ((t + 273.0) / (t + 273.0 - 33.91)) ** 2
+    assert abs(es(10.0) / 1228.0 - 1) < 0.02
+    assert abs(es(20.0) / 2339.0 - 1) < 0.02
+    assert abs(lv(20.0) / 2.454e6 - 1) < 0.005
+
+
+@pytest.mark.parametrize("mode", [1, 2, 3])
+def test_heat_budget_heat_content_follows_the_sources(oracle_lib, mode):
+    """Closed basin: after each step sum(h * T) changes by dt * sum(HeatSource)
+    (the trapezoidal iteration that sets T uses twodt1 = dt and the sources of
+    the step's leapfrog call), so the sources reach imsal unchanged."""
+    d = _heat_lake(mode)
+    o = oracle_lib.Oracle(d)
+    o.step(1)
+    for n in range(2, 7):
+        h0 = _tracer_mass(o, "sal")
+        o.step(n)
+        hs = o.get("HeatSource")
+        kmz = o.get("kmz").astype(int)
+        k1z = o.get("k1z").astype(int)
+        src = sum(float(hs[l, k1z[l] - 1:kmz[l]].sum()) for l in range(o.lm))
+        dh = _tracer_mass(o, "sal") - h0
+        assert abs(src) > 0
+        assert abs(dh - d["dt"] * src) <= 1e-9 * abs(h0) + 1e-6 * abs(d["dt"] * src), (n, dh, d["dt"] * src)
