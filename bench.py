@@ -114,22 +114,37 @@ def copy_probe(mib):
 
 
 def copy_bandwidth(mib=1024, reps=10):
-    """Achievable HBM bandwidth on this device: a plain device-to-device copy of
-    `mib` MiB (read + write bytes / time), timed with events."""
-    import torch
-    n = mib * (1 << 20) // 4
-    src = torch.ones(n, dtype=torch.float32, device="cuda")
-    dst = torch.empty_like(src)
-    dst.copy_(src)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        dst.copy_(src)
-    e1.record()
-    e1.synchronize()
-    ms = e0.elapsed_time(e1) / reps
-    del src, dst
-    return 2 * n * 4 / (ms * 1e-3) / 1e9
+    """Achievable HBM bandwidth on this device: a device-to-device copy of `mib`
+    MiB (read + write bytes / time), timed with HIP events, through the HIP
+    runtime the library itself uses (a second runtime in the process, such as
+    torch's, would not see the device)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    nbytes = mib << 20
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    ok = (hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) == 0
+          and hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)) == 0
+          and hip.hipEventCreate(ctypes.byref(e0)) == 0 and hip.hipEventCreate(ctypes.byref(e1)) == 0)
+    ms = ctypes.c_float(0.0)
+    if ok:
+        hip.hipMemset(src, 0, ctypes.c_size_t(nbytes))
+        hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+        hip.hipEventRecord(e0, None)
+        for _ in range(reps):
+            hip.hipMemcpyAsync(dst, src, ctypes.c_size_t(nbytes), 3, None)
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+    for p_ in (src, dst):
+        if p_.value:
+            hip.hipFree(p_)
+    for e in (e0, e1):
+        if e.value:
+            hip.hipEventDestroy(e)
+    if not ok or ms.value <= 0:
+        return None
+    return 2 * nbytes * reps / (ms.value * 1e-3) / 1e9
 
 
 def main():
@@ -238,7 +253,9 @@ def main():
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                         "traffic": traffic, "bytes_per_launch": bytes_launch,
                         "avg_launch_ms": round(avg_ms, 5),
-                        "copy_GBs": round(copy_bandwidth(), 1)}
+                        "copy_GBs": None}
+            cb = copy_bandwidth()
+            roofline["copy_GBs"] = round(cb, 1) if cb else None
 
     cpu = None
     if not a.no_cpu_baseline:
