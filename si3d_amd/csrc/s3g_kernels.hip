@@ -376,10 +376,13 @@ __global__ void __launch_bounds__(kBlock) k_matmom(Dev<R> D, double tz) {
   const double bot_a = taub * twodt1;
 
   double bcl = 0.0, hdr_prev = 0.0, c1p = 0.0, d1p = 0.0, e1p = 0.0;
+  // thicknesses, face viscosity and qpp of levels k-1, k (k+1) carried down the sweep
+  double hkm = hq[IX(k1q - 1, c)], hk = hq[IX(k1q, c)], Avk = 0.0;
+  double qkm = 0.0, qk = (nwl > 1) ? (double)qpp[IX(k1q, c)] : 0.0;
   for (int k = k1q; k <= kmq; ++k) {
-    const double hkm = hq[IX(k - 1, c)], hk = hq[IX(k, c)], hkp = hq[IX(k + 1, c)];
-    const double Avk = (k > k1q) ? 0.5 * ((double)D.Av[IX(k, ln)] + (double)D.Av[IX(k, c)]) : 0.0;
+    const double hkp = hq[IX(k + 1, c)];
     const double Avkp = (k < kmq) ? 0.5 * ((double)D.Av[IX(k + 1, ln)] + (double)D.Av[IX(k + 1, c)]) : 0.0;
+    const double qkp = (nwl > 1 && k < kmq) ? (double)qpp[IX(k + 1, c)] : 0.0;
     if (D.P.ibc != 0) {
       double hdr = gq * hk * ((double)D.rhop[IX(k, ln)] - (double)D.rhop[IX(k, c)]);
       if (k == kmq && kmq > k1q) hdr = bclnc_km<R, X>(D, c, ln, kmq, hdr);
@@ -388,8 +391,8 @@ __global__ void __launch_bounds__(kBlock) k_matmom(Dev<R> D, double tz) {
     }
     double vd = 0.0;
     if (nwl > 1) {
-      double dk = (k > k1q) ? Avk * ((double)qpp[IX(k - 1, c)] - (double)qpp[IX(k, c)]) / (hkm + hk) : 0.0;
-      double dkp = (k < kmq) ? Avkp * ((double)qpp[IX(k, c)] - (double)qpp[IX(k + 1, c)]) / (hk + hkp) : 0.0;
+      double dk = (k > k1q) ? Avk * (qkm - qk) / (hkm + hk) : 0.0;
+      double dkp = (k < kmq) ? Avkp * (qk - qkp) / (hk + hkp) : 0.0;
       vd = D.P.twodt * (dk - dkp) * 2. * (1. - theta);
     }
     const double hh = hk / rhoq;
@@ -422,6 +425,11 @@ __global__ void __launch_bounds__(kBlock) k_matmom(Dev<R> D, double tz) {
     c1p = c1;
     d1p = d1;
     e1p = e1;
+    hkm = hk;
+    hk = hkp;
+    Avk = Avkp;
+    qkm = qk;
+    qk = qkp;
   }
   double xg = d1p, xr = e1p, sg = xg, sr = xr;
   AG[IX(kmq, c)] = (R)xg;
@@ -980,25 +988,40 @@ __global__ void __launch_bounds__(kBlock) k_hmix(Dev<R> D) {
   }
 }
 
-// Interface quantities of level k (k1s < k <= kms) of column m from the
-// p-level fields: the limited length scale, shear and buoyancy production (the
-// TKE source terms, si3d_mixing.f90:411-440) and the new eddy coefficients
-// (245-290).  Av(k)/Dv(k) are read for the productions before the new values
-// replace them.
-struct TurbLev {
-  double xl, sp, bp, av, dv;
+// The p-level values of level k of column m that the interface terms of
+// levels k and k+1 share: hp, rhop, up at m and its west face, vp at m and
+// its south face.  The column sweep loads each level once and carries it.
+struct TurbRaw {
+  double h, rho, um, uw, vm, vs;
 };
 
 template <typename R>
-__device__ __forceinline__ TurbLev turb_level(const Dev<R> &D, int m, int k, int lW, int lS) {
+__device__ __forceinline__ TurbRaw turb_raw(const Dev<R> &D, int m, int k, int lW, int lS) {
+  return TurbRaw{(double)D.hp[IX(k, m)], (double)D.rhop[IX(k, m)], (double)D.up[IX(k, m)],
+                 (double)D.up[IX(k, lW)], (double)D.vp[IX(k, m)],  (double)D.vp[IX(k, lS)]};
+}
+
+// Interface quantities of level k (k1s < k <= kms) of column m from the
+// p-level fields (a = level k-1, b = level k): the limited length scale, shear
+// and buoyancy production (the TKE source terms, si3d_mixing.f90:411-440) and
+// the new eddy coefficients (245-290), plus q2p(k) and q2lp(k).  Av(k)/Dv(k)
+// are read for the productions before the new values replace them.
+struct TurbLev {
+  double xl, sp, bp, av, dv, q2p, q2lp;
+};
+
+template <typename R>
+__device__ __forceinline__ TurbLev turb_level(const Dev<R> &D, int m, int k, const TurbRaw &a, const TurbRaw &b) {
   TurbLev L;
-  const double hkm = D.hp[IX(k - 1, m)], hk = D.hp[IX(k, m)];
+  const double hkm = a.h, hk = b.h;
   const double dz = 0.5 * (hkm + hk);
-  const double rkm = D.rhop[IX(k - 1, m)], rk = D.rhop[IX(k, m)];
+  const double rkm = a.rho, rk = b.rho;
   const double N2 = -G / (0.5 * (rkm + rk) + 1000.) * ((rkm - rk) / dz);
-  const double dudz = 0.5 * ((double)D.up[IX(k - 1, m)] + (double)D.up[IX(k - 1, lW)] - (double)D.up[IX(k, m)] - (double)D.up[IX(k, lW)]) / dz;
-  const double dvdz = 0.5 * ((double)D.vp[IX(k - 1, m)] + (double)D.vp[IX(k - 1, lS)] - (double)D.vp[IX(k, m)] - (double)D.vp[IX(k, lS)]) / dz;
+  const double dudz = 0.5 * (a.um + a.uw - b.um - b.uw) / dz;
+  const double dvdz = 0.5 * (a.vm + a.vs - b.vm - b.vs) / dz;
   const double q2pk = D.q2p[IX(k, m)], q2lpk = D.q2lp[IX(k, m)];
+  L.q2p = q2pk;
+  L.q2lp = q2lpk;
   double xl = q2lpk / q2pk;
   if (N2 > 0.0) xl = dmin(xl, 0.53 * sqrt(q2pk / N2));
   L.xl = xl;
@@ -1035,7 +1058,7 @@ __global__ void __launch_bounds__(kBlock) k_turb_cell(Dev<R> D, int nk, int tile
       D.Av[IX(k, m)] = D.Dv[IX(k, m)] = R(0);
       continue;
     }
-    const TurbLev L = turb_level(D, m, k, lW, lS);
+    const TurbLev L = turb_level(D, m, k, turb_raw(D, m, k - 1, lW, lS), turb_raw(D, m, k, lW, lS));
     D.d0[IX(k, m)] = L.xl;
     D.d1[IX(k, m)] = L.sp;
     D.d2[IX(k, m)] = L.bp;
@@ -1064,15 +1087,19 @@ __global__ void __launch_bounds__(kBlock) k_turb(Dev<R> D, int istep) {
   double *E1 = D.d0, *E2 = D.d1;  // forward-sweep factors of the two systems
   double hbelow = 0.0;
   for (int k = k1s + 1; k <= kms; ++k) hbelow += D.hp[IX(k, m)];
-  auto level = [&](int k) -> TurbLev {
+  // interface quantities of level k: read back (split) or computed from the
+  // carried raw values of level k-1 (a) and the new ones of level k (b)
+  auto level = [&](int k, const TurbRaw &a, const TurbRaw &b) -> TurbLev {
     if (SPLIT) {
       TurbLev L{};
       L.xl = D.d0[IX(k, m)];
       L.sp = D.d1[IX(k, m)];
       L.bp = D.d2[IX(k, m)];
+      L.q2p = D.q2p[IX(k, m)];
+      L.q2lp = D.q2lp[IX(k, m)];
       return L;
     }
-    return turb_level(D, m, k, lW, lS);
+    return turb_level(D, m, k, a, b);
   };
   // boundary values (445-457)
   const double b1_23 = pow(B_1, 2. / 3.);
@@ -1093,20 +1120,25 @@ __global__ void __launch_bounds__(kBlock) k_turb(Dev<R> D, int istep) {
   D.q2l[IX(k1s, m)] = f2;
   E1[IX(k1s, m)] = 0.0;
   E2[IX(k1s, m)] = 0.0;
-  TurbLev cur = level(k1s + 1);
-  double q2p_next = D.q2p[IX(k1s + 1, m)];
-  double dco_prev = SQ * 0.5 * (0.0 + cur.xl) * sqrt(0.5 * (D.q2p[IX(k1s, m)] + q2p_next));
+  TurbRaw rcur{};  // raw values of level k (unused when split)
+  if (!SPLIT) rcur = turb_raw(D, m, k1s + 1, lW, lS);
+  TurbLev cur = level(k1s + 1, SPLIT ? rcur : turb_raw(D, m, k1s, lW, lS), rcur);
+  double dco_prev = SQ * 0.5 * (0.0 + cur.xl) * sqrt(0.5 * (D.q2p[IX(k1s, m)] + cur.q2p));
   double zfromt = 0.0;
   double hkm = D.hp[IX(k1s, m)];
   for (int k = k1s + 1; k <= kms; ++k) {
     const bool more = k + 1 <= kms;
+    TurbRaw rnxt{};
     TurbLev nxt{};
-    if (more) nxt = level(k + 1);
+    if (more) {
+      if (!SPLIT) rnxt = turb_raw(D, m, k + 1, lW, lS);
+      nxt = level(k + 1, rcur, rnxt);
+    }
     const double xlk = cur.xl, xlp = more ? nxt.xl : 0.0;
-    const double q2pk = q2p_next;
-    q2p_next = D.q2p[IX(k + 1, m)];
+    const double q2pk = cur.q2p;
+    const double q2p_next = more ? nxt.q2p : D.q2p[IX(k + 1, m)];
     const double dco = SQ * 0.5 * (xlk + xlp) * sqrt(0.5 * (q2pk + q2p_next));
-    const double hk = D.hp[IX(k, m)];
+    const double hk = SPLIT ? (double)D.hp[IX(k, m)] : rcur.h;
     const double a3 = -dco / (hk * 0.5 * (hk + hkm));
     const double a1 = -dco_prev / (hkm * 0.5 * (hk + hkm));
     const double shp = cur.sp, byp = cur.bp;
@@ -1142,7 +1174,7 @@ __global__ void __launch_bounds__(kBlock) k_turb(Dev<R> D, int istep) {
         a2l -= 5.06 * byp / q2pk;
       }
     }
-    dsl += 4 * D.q2lp[IX(k, m)] / twodt1 - D.q2lpp[IX(k, m)] / twodt1;
+    dsl += 4 * cur.q2lp / twodt1 - D.q2lpp[IX(k, m)] / twodt1;
     // forward sweeps
     double den = a2 + a1 * e1;
     e1 = -a3 / den;
@@ -1164,6 +1196,7 @@ __global__ void __launch_bounds__(kBlock) k_turb(Dev<R> D, int istep) {
     dco_prev = dco;
     hkm = hk;
     cur = nxt;
+    rcur = rnxt;
   }
   // bottom Dirichlet row kms+1 (a1 = 0) and back substitution
   double x1 = ds_bot / r3, x2 = 0.0;
