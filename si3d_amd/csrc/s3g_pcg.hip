@@ -641,12 +641,19 @@ int pcg_setup(s3g_ctx *c, const int *lN0, const int *l2i, const int *l2j) {
   return 0;
 }
 
-static double host_sum(s3g_ctx *c, const double *dev_part, int n) {
-  hipMemcpyAsync(c->h_scal, dev_part, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
-  if (hipStreamSynchronize(c->stream) != hipSuccess) return NAN;
-  double s = 0.0;
-  for (int k = 0; k < n; ++k) s += c->h_scal[k];
-  return s;
+// (b,b) and (r,r) of the FP64 true residual from their partials (adjacent
+// slots P_BB, P_TR) with one copy and one synchronisation; with `pending` the
+// copy was already enqueued (behind the last CG chunk) and only the sync and
+// the sums remain.
+static bool true_residual_sums(s3g_ctx *c, double &bb, double &tr, bool pending) {
+  const int nb = c->pcg.nblk;
+  if (!pending)
+    hipMemcpyAsync(c->h_scal, c->pcg.part + P_BB * nb, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return false;
+  bb = tr = 0.0;
+  for (int k = 0; k < nb; ++k) bb += c->h_scal[k];
+  for (int k = 0; k < nb; ++k) tr += c->h_scal[nb + k];
+  return true;
 }
 
 // z = M^-1 r for the line and multigrid preconditioners, with the partial of
@@ -738,8 +745,11 @@ static void mg_build(s3g_ctx *c, CgArgs<T> &A) {
 
 // Runs CG iterations on CgArgs A (x, r, b set up) until (r,r) <= tol2 (b,b) or
 // maxit; returns the number of completed iterations (< 0 on HIP error).
-template <typename T>
-static int run_cg(s3g_ctx *c, CgArgs<T> &A, int maxit, int hint) {
+// `tail`, if given, is enqueued behind every chunk of iterations, before the
+// iteration count is copied back (the FP64 solve computes its true residual
+// there, so a converged solve needs a single synchronisation).
+template <typename T, typename Tail>
+static int run_cg(s3g_ctx *c, CgArgs<T> &A, int maxit, int hint, Tail tail) {
   const dim3 g(A.nblk), b(kBlock);
   const bool jac = (c->prec_kind == S3G_PREC_JACOBI) || (c->prec_kind == S3G_PREC_LINE && A.nlines == 0);
   const bool mg = c->prec_kind == S3G_PREC_MG;
@@ -779,6 +789,7 @@ static int run_cg(s3g_ctx *c, CgArgs<T> &A, int maxit, int hint) {
     }
     if (c->timing) timer_end(c, slot);
     it += nl;
+    tail();
     hipMemcpyAsync(c->h_iters, A.iters, sizeof(int), hipMemcpyDeviceToHost, c->stream);
     if (hip_check(hipStreamSynchronize(c->stream), "pcg sync") < 0) return -1;
     if (c->h_iters[0] < it) break;  // fewer updates than launched: converged
@@ -845,14 +856,18 @@ int solve_free_surface(s3g_ctx *c, double tz) {
                                        D.qq, P.lc, P.ld, 0.5 * tol);
     if (line) hipLaunchKernelGGL((k_line_factor<double>), gl, b, 0, c->stream, A, P.lc, P.ld);
     if (c->prec_kind == S3G_PREC_MG) mg_build(c, A);
-    for (int restart = 0; restart < 3; ++restart) {
-      int it = run_cg(c, A, c->pcg_maxit - total, c->stats.pcg_iters_last);
-      if (it < 0) return S3G_ERR_HIP;
-      total += it;
+    auto true_res = [&]() {  // behind each chunk: FP64 true residual, partials to the host
       hipLaunchKernelGGL((k_true_residual<double, false>), g, b, 0, c->stream, O, nullptr, nullptr, nullptr,
                          nullptr, nullptr);
-      const double tr = host_sum(c, P.part + P_TR * P.nblk, P.nblk);
-      const double bb = host_sum(c, P.part + P_BB * P.nblk, P.nblk);
+      hipMemcpyAsync(c->h_scal, P.part + P_BB * P.nblk, 2 * P.nblk * sizeof(double), hipMemcpyDeviceToHost,
+                     c->stream);
+    };
+    for (int restart = 0; restart < 3; ++restart) {
+      int it = run_cg(c, A, c->pcg_maxit - total, c->stats.pcg_iters_last, true_res);
+      if (it < 0) return S3G_ERR_HIP;
+      total += it;
+      double tr = 0.0, bb = 0.0;
+      if (!true_residual_sums(c, bb, tr, true)) return S3G_ERR_HIP;
       relres = bb > 0 ? std::sqrt(tr / bb) : std::sqrt(tr);
       if (relres <= tol || !std::isfinite(relres)) break;
       if (total >= c->pcg_maxit) {
@@ -870,8 +885,8 @@ int solve_free_surface(s3g_ctx *c, double tz) {
     if (c->prec_kind == S3G_PREC_MG) mg_build(c, A);
     for (int sweep = 0; sweep < 12; ++sweep) {
       hipLaunchKernelGGL((k_true_residual<float, true>), g, b, 0, c->stream, O, P.bf, P.df, P.pf0, P.pf1, P.zf);
-      const double tr = host_sum(c, P.part + P_TR * P.nblk, P.nblk);
-      const double bb = host_sum(c, P.part + P_BB * P.nblk, P.nblk);
+      double tr = 0.0, bb = 0.0;
+      if (!true_residual_sums(c, bb, tr, false)) return S3G_ERR_HIP;
       relres = bb > 0 ? std::sqrt(tr / bb) : std::sqrt(tr);
       if (relres <= tol || !std::isfinite(relres)) break;
       if (total >= c->pcg_maxit) {
@@ -882,7 +897,7 @@ int solve_free_surface(s3g_ctx *c, double tz) {
       // and FP32 holds about 1e-5 of reduction per sweep
       const double t_in = std::min(0.5, std::max(kInnerTol, 0.3 * tol / relres));
       A.tol2 = t_in * t_in;
-      int it = run_cg(c, A, c->pcg_maxit - total, c->stats.pcg_iters_last / 2);
+      int it = run_cg(c, A, c->pcg_maxit - total, c->stats.pcg_iters_last / 2, [] {});
       if (it < 0) return S3G_ERR_HIP;
       total += it;
       hipLaunchKernelGGL((k_add_correction<float>), g, b, 0, c->stream, P.x, (const float *)P.df, P.n);
