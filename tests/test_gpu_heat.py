@@ -55,3 +55,26 @@ def test_heat_content_follows_sources_full_size(precision, tol):
         print(f"step {n}: d(sum h T) {h1 - h0:.6e}, dt sum(HeatSource) {d['dt'] * src:.6e}", flush=True)
         assert abs(src) > 0
         assert abs((h1 - h0) - d["dt"] * src) <= tol * abs(h0) + 1e-4 * abs(d["dt"] * src)
+
+
+def test_heat_budget_from_the_reference_met_file(oracle_lib):
+    """ifsurfbc = 3 driven by the first 48 records of the reference's Lake Tahoe
+    met series (si3d_surfbc.txt, 10-minute records: attenuation, shortwave,
+    air temperature, pressure, humidity, longwave, drag, wind), read with the
+    reference's record layout: GPU = oracle."""
+    import os
+    from si3d_amd import inputs
+    from si3d_amd.model import GpuLake
+    path = os.path.join(os.path.dirname(__file__), "golden", "psi3d", "sample_files", "si3d_surfbc.txt")
+    d = small_ellipse(nts=10)
+    d.update(ifsurfbc=3, dtsurfbc=600.0, surfbc=inputs.read_surfbc(path, 9), pcg_tol=1e-12)
+    print("case: small ellipse, Lake Tahoe met series (ifsurfbc 3), 6 steps", flush=True)
+    o = oracle_lib.Oracle(d)
+    g = GpuLake(d, precision="fp64", preconditioner="mg", tol=1e-12)
+    for n in range(1, 7):
+        o.step(n)
+        g.step(n)
+    for name in ("HeatSource", "sal", "s", "uh", "vh"):
+        e = relerr(g.get(name), o.get(name))
+        assert e < 1e-8, (name, e)
+    assert np.max(np.abs(g.get("HeatSource"))) > 0
