@@ -1,6 +1,6 @@
-"""Readers of pSi3D's own input files into a deck (the dict `GpuLake`,
-`s3g_create_from_deck` and the oracle take), so that an existing case runs on
-this path from its files:
+"""Readers of pSi3D's own input files into a deck (the dict `GpuLake` and
+`s3g_create_from_deck` take), so that an existing case runs on this path from
+its files:
 
 * si3d_inp.txt      -- `input` (si3d_utils.f90:21-315)
 * si3d_bathy        -- `bathy` (si3d_utils.f90:595-640): depths in dm
