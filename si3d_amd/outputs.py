@@ -1,0 +1,261 @@
+"""Writers of pSi3D's node time files (`outt`, si3d_utils.f90:1020-1202) and
+horizontal plane files (`outh`, 2142-2334) for the Python driver, from fields
+copied back from the device (`GpuLake.get`, the library's sync_to_host).
+
+With the Fortran driver these files come from the reference's own routines
+after `s3g_get_field_*`; these writers give the same records to a run driven
+from Python:
+
+* tf<i>_<j>.txt: the title, a run line, the node/run parameters and two
+  column-heading lines, then per output step one record of time (h), step,
+  zeta (cm) and, for every layer k1..kmz+1, level depth (m), u, v, w (cm/s),
+  Av, Dv (cm2/s) and the scalar (FORMAT 4: 1X,F10.4,I10,2PF9.2, then
+  0PF9.2,2(2PF10.2),2PF9.4,2(4PF15.7),0PE15.5 per layer, continued on records
+  that start with 30X).  Values of layers without water are -99 in output
+  units, as the reference initialises them.
+* plane_<k>: Fortran unformatted sequential records (4-byte length before and
+  after each record): the frame count, the wet-cell count of the plane, then
+  one record per frame: step, year, month, day (int32), hour (float32) and,
+  per wet cell of the plane, (i, j, u, v, w, scalar, Av, s) on the first frame
+  and (u, v, w, scalar, Av, s) with u/v/w averaged to the cell centre on the
+  later ones (float32).
+
+The run line of a time file carries the wall-clock date of the run, as the
+reference's does; the model date of the plane records is the start date plus
+n dt in exact calendar arithmetic (the reference steps its date in single
+precision, which can differ in the last digit of the hundredths of an hour
+after long runs).
+"""
+from __future__ import annotations
+
+import datetime
+import math
+import os
+
+import numpy as np
+
+from . import lake
+
+
+# ---- Fortran edit descriptors -------------------------------------------------
+def fortran_f(x, w, d, scale=0):
+    """Fw.d output of x with scale factor kP (x * 10**k), right-justified;
+    asterisks when it does not fit."""
+    v = float(x) * (10.0 ** scale)
+    s = f"{v:.{d}f}"
+    if len(s) > w and s.startswith("0."):
+        s = s[1:]
+    elif len(s) > w and s.startswith("-0."):
+        s = "-" + s[2:]
+    return s.rjust(w) if len(s) <= w else "*" * w
+
+
+def fortran_e(x, w, d):
+    """Ew.d output with 0P: 0.ddddE+ee (mantissa in [0.1, 1))."""
+    v = float(x)
+    if v == 0.0 or not math.isfinite(v):
+        s = f"0.{'0' * d}E+00" if v == 0.0 else ("NaN" if math.isnan(v) else ("-Inf" if v < 0 else "Inf"))
+        return s.rjust(w)
+    e = math.floor(math.log10(abs(v))) + 1
+    m = round(abs(v) / 10.0 ** e, d)
+    if m >= 1.0:
+        m, e = m / 10.0, e + 1
+    if m < 0.1:  # log10 rounding at an exact power of ten
+        m, e = m * 10.0, e - 1
+    mant = f"{m:.{d}f}"
+    exp = f"E{'+' if e >= 0 else '-'}{abs(e):02d}" if abs(e) <= 99 else f"{'+' if e >= 0 else '-'}{abs(e):03d}"
+    s = ("-" if v < 0 else "") + mant + exp
+    if len(s) > w and s.startswith("-0."):  # the optional leading zero goes first
+        s = "-" + s[2:]
+    return s.rjust(w) if len(s) <= w else "*" * w
+
+
+def _layer_fields(z, u, v, w, av, dv, sc):
+    return (fortran_f(z, 9, 2) + fortran_f(u, 10, 2, 2) + fortran_f(v, 10, 2, 2) + fortran_f(w, 9, 4, 2) +
+            fortran_f(av, 15, 7, 4) + fortran_f(dv, 15, 7, 4) + fortran_e(sc, 15, 5))
+
+
+# ---- grid helpers ---------------------------------------------------------------
+def wet_columns(deck):
+    """(i, j) of every wet column in the reference's l order (i outer, j inner;
+    si3d_utils.f90:781-793), 1-based, from the FP32 depths of `bathy`."""
+    h = deck["bathy_dm"].astype(np.float32) * np.float32(0.1) + np.float32(deck["datadj"])
+    ii, jj = np.nonzero(h > 0)
+    order = np.lexsort((jj, ii))
+    return ii[order] + 2, jj[order] + 2
+
+
+def zlevels(deck):
+    """zlevel(1..km1) as ZGridDimensions defines it (si3d_utils.f90:318-382)."""
+    if deck["ibathyf"] < 0:
+        return np.asarray(deck["zlevel"], dtype=np.float32).astype(np.float64)
+    km1 = lake.km1_of(deck)
+    z = np.zeros(km1, dtype=np.float32)
+    dz = np.float32(deck["ddz"])
+    for k in range(2, km1):  # zlevel(k1) = 0, then + ddz in single precision
+        z[k] = z[k - 1] + dz
+    z[0] = z[1] = -100.0
+    return z.astype(np.float64)
+
+
+# ---- time files ------------------------------------------------------------------
+class TimeFiles:
+    """tf<i>_<j>.txt for the nodes `nodes` [(i, j), ...] of a GpuLake run."""
+
+    def __init__(self, g, deck, nodes, directory=".", title=None, sed_h=0.5):
+        self.g, self.deck = g, deck
+        self.dir = directory
+        ii, jj = wet_columns(deck)
+        self.l_of = {(int(i), int(j)): l for l, (i, j) in enumerate(zip(ii, jj))}
+        self.nodes = []
+        for (i, j) in nodes:
+            if (i, j) not in self.l_of:
+                raise ValueError(f"time-file node ({i}, {j}) is not a wet column")
+            self.nodes.append((i, j, self.l_of[(i, j)]))
+        self.z = zlevels(deck)
+        self.sed_h = sed_h
+        self.title = title if title is not None else deck.get("name", "si3d run")
+        self.files = {}
+
+    def _name(self, i, j):
+        return os.path.join(self.dir, f"tf{i}_{j}.txt")
+
+    def open(self):
+        kmz = self.g.get("kmz")
+        hhs = self.g.get("hhs")
+        now = datetime.datetime.now()
+        d = self.deck
+        for i, j, l in self.nodes:
+            f = open(self._name(i, j), "w")
+            f.write(f"{self.title}\n")
+            f.write(f"Run number = {now:%Y%m%d}{now:%H%M},  Start date of run:  "
+                    f"{d.get('imon0', 1):2d}/{d.get('iday0', 1):2d}/{d.get('iyr0', 2000):4d} at "
+                    f"{d.get('ihr0', 0):04d} hours\n")
+            km = int(kmz[l]) - 2 + 1
+            dtf = fortran_f(d["dt"], 5, 2) if d["dt"] >= 0.01 and d["ddz"] >= 0.01 else fortran_f(d["dt"], 5, 4)
+            dzf = fortran_f(d["ddz"], 5, 2) if d["dt"] >= 0.01 and d["ddz"] >= 0.01 else fortran_f(d["ddz"], 5, 4)
+            f.write(f"i ={i:4d}  j ={j:4d}  km ={km:4d}   dt ={dtf} sec  hhs ={fortran_f(hhs[l], 6, 3)} m"
+                    f"   dz ={dzf} m\n")
+            f.write(f"iexplt ={0:2d}  itrap ={int(d['itrap']):2d}  cd = {fortran_f(d['cd'], 7, 4)}  "
+                    f"ismooth ={int(d['ismooth']):2d}  beta ={fortran_f(d['beta'], 6, 3)} niter ={int(d['niter']):2d}\n")
+            f.write(f"iextrp ={0:2d}  f ={fortran_f(d['f'], 7, 4)}  tramp={fortran_f(0.0, 9, 1)}  "
+                    f"iupwind ={0:2d}\n")
+            f.write(" " + "   time     " + "  step     " + "  zeta " + "   depth   " + "    u       " +
+                    "  v      " + " w       " + "   Av       " + "        Dv      " + "  scalar    " +
+                    " Tracers-> " + "\n")
+            f.write(" " + "    hrs     " + "   no      " + "   cm       " + " m    " + "   cm/s   " +
+                    "   cm/s   " + "  cm/s      " + " cm2/s      " + "     cm2/s     " + "  oC        " +
+                    "  g/l   -> " + "\n")
+            self.files[(i, j)] = f
+
+    def write(self, n, thrs):
+        """One output record per node at step n, time thrs (hours)."""
+        if not self.files:
+            self.open()
+        g = self.g
+        s, u, v, wp, av, dv, sal, h = (g.get(x) for x in ("s", "u", "v", "wp", "Av", "Dv", "sal", "h"))
+        kmz = g.get("kmz")
+        for i, j, l in self.nodes:
+            rows = []
+            for k in range(2, int(kmz[l]) + 2):  # k = k1 .. kmz+1 (1-based)
+                kk = k - 1
+                if h[l, kk] <= 0.0:
+                    rows.append((0.0, -99e-2, -99e-2, -99e-2, -99e-4, -99e-4, -99.0))
+                    continue
+                zexp = self.z[k] if k < len(self.z) else self.z[k - 1]
+                if k == int(kmz[l]) + 1:
+                    zexp = self.z[k - 1] + self.sed_h
+                rows.append((zexp, u[l, kk], v[l, kk], wp[l, kk], 0.5 * (av[l, kk] + av[l, kk + 1]),
+                             0.5 * (dv[l, kk] + dv[l, kk + 1]), sal[l, kk]))
+            head = " " + fortran_f(thrs, 10, 4) + f"{n:10d}" + fortran_f(s[l], 9, 2, 2)
+            out = [head + _layer_fields(*rows[0])]
+            out += [" " * 30 + _layer_fields(*r) for r in rows[1:]]
+            self.files[(i, j)].write("\n".join(out) + "\n")
+
+    def close(self):
+        for f in self.files.values():
+            f.close()
+        self.files = {}
+
+
+# ---- plane files -------------------------------------------------------------------
+def _record(f, *parts):
+    payload = b"".join(np.ascontiguousarray(p).tobytes() for p in parts)
+    n = np.array([len(payload)], dtype=np.int32).tobytes()
+    f.write(n + payload + n)
+
+
+class PlaneFiles:
+    """plane_<k> files for the layers `planes` of a GpuLake run (outh)."""
+
+    def __init__(self, g, deck, planes, nts, iop, itspfh=0, directory=".", start=(2000, 1, 1, 0)):
+        self.g, self.deck = g, deck
+        self.planes = list(planes)
+        self.km = lake.km1_of(deck) - 1
+        for kp in self.planes:
+            if kp < 2:
+                raise ValueError("ERROR: Output plane is not in DOMAIN")
+        self.n_frames = int((nts - itspfh) / max(iop, 1))
+        self.dir = directory
+        self.start = datetime.datetime(start[0], start[1], start[2]) + datetime.timedelta(hours=start[3] / 100.0)
+        ii, jj = wet_columns(deck)
+        self.ij = (ii, jj)
+        self.files = []
+
+    def _cells(self, kp, kmz):
+        lm = len(self.ij[0])
+        if kp > self.km:
+            return np.arange(lm), kmz[:lm].astype(int)
+        sel = np.nonzero(kmz[:lm] >= kp)[0]
+        return sel, np.full(len(sel), kp)
+
+    def _stamp(self, n):
+        t = self.start + datetime.timedelta(seconds=n * self.deck["dt"])
+        hrs = (t - datetime.datetime(t.year, t.month, t.day)).total_seconds() / 3600.0
+        return (np.array([n, t.year, t.month, t.day], dtype=np.int32), np.array([round(hrs * 100)], dtype=np.float32))
+
+    def write(self, n):
+        g = self.g
+        kmz = g.get("kmz")
+        up, vp, wp, salp, hp, av, s = (g.get(x) for x in ("up", "vp", "wp", "salp", "hp", "Av", "s"))
+        first = not self.files
+        if first:
+            for kp in self.planes:
+                name = os.path.join(self.dir, f"plane_{kp}")
+                f = open(name, "wb")
+                _record(f, np.array([self.n_frames], dtype=np.int32))
+                _record(f, np.array([len(self._cells(kp, kmz)[0])], dtype=np.int32))
+                self.files.append(f)
+        ints, hour = self._stamp(n)
+        for f, kp in zip(self.files, self.planes):
+            cells, kb = self._cells(kp, kmz)
+            kk = kb - 1
+            sal = salp[cells, kk].copy()
+            sal[hp[cells, kk] <= 0.0] = -99.0
+            avm = 0.5 * (av[cells, kk] + av[cells, kk + 1])
+            if first:
+                cols = [self.ij[0][cells].astype(np.float64), self.ij[1][cells].astype(np.float64),
+                        up[cells, kk], vp[cells, kk], wp[cells, kk], sal, avm, s[cells]]
+            else:
+                west, south = self._neighbours()
+                cols = [(up[cells, kk] + up[west[cells], kk]) / 2.0, (vp[cells, kk] + vp[south[cells], kk]) / 2.0,
+                        (wp[cells, kk] + wp[cells, kk + 1]) / 2.0, sal, avm, s[cells]]
+            data = np.stack(cols, axis=1).astype(np.float32)  # (ipoints, nvar), written row by row
+            _record(f, ints, hour, data)
+
+    def _neighbours(self):
+        """West and south neighbour (l index, dry -> the sentinel lm) of each wet
+        column, from the (i, j) positions."""
+        if not hasattr(self, "_nb"):
+            ii, jj = self.ij
+            lm = len(ii)
+            pos = {(int(i), int(j)): l for l, (i, j) in enumerate(zip(ii, jj))}
+            west = np.array([pos.get((int(i) - 1, int(j)), lm) for i, j in zip(ii, jj)] + [lm])
+            south = np.array([pos.get((int(i), int(j) - 1), lm) for i, j in zip(ii, jj)] + [lm])
+            self._nb = (west, south)
+        return self._nb
+
+    def close(self):
+        for f in self.files:
+            f.close()
+        self.files = []

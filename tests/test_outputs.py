@@ -1,0 +1,106 @@
+"""Writers of the node time files and plane files (si3d_amd/outputs.py): the
+Fortran edit descriptors of their records, and what a run's files hold, read
+back field by field (the writers take any object with the library's `get`
+interface; the oracle provides one on the CPU)."""
+import numpy as np
+import pytest
+
+from si3d_amd import lake, outputs as O
+from tests.helpers import small_ellipse
+
+
+@pytest.mark.parametrize("args,expected", [
+    ((12.3456, 10, 4), "   12.3456"), ((0.012345, 9, 2, 2), "     1.23"), ((-0.5, 9, 2), "    -0.50"),
+    ((1e-5, 15, 7, 4), "      0.1000000"), ((123456.0, 5, 2), "*****"), ((-99e-2, 10, 2, 2), "    -99.00")])
+def test_fortran_f(args, expected):
+    assert O.fortran_f(*args) == expected
+
+
+@pytest.mark.parametrize("x,expected", [
+    (12.345, "    0.12345E+02"), (-0.00123456, "   -0.12346E-02"), (0.0, "    0.00000E+00"),
+    (1.0, "    0.10000E+01"), (0.999999, "    0.10000E+01"), (1e-120, "    0.10000-119"), (-99.0, "   -0.99000E+02")])
+def test_fortran_e(x, expected):
+    assert O.fortran_e(x, 15, 5) == expected
+
+
+def _run(oracle_lib, nsteps=3):
+    d = small_ellipse(nts=10)
+    o = oracle_lib.Oracle(d)
+    for n in range(1, nsteps + 1):
+        o.step(n)
+    return d, o
+
+
+def test_time_file_records(oracle_lib, tmp_path):
+    d, o = _run(oracle_lib)
+    ii, jj = O.wet_columns(d)
+    node = (int(ii[150]), int(jj[150]))
+    tf = O.TimeFiles(o, d, [node], directory=str(tmp_path), title="test lake")
+    tf.write(3, 3 * d["dt"] / 3600.0)
+    tf.close()
+    lines = open(tmp_path / f"tf{node[0]}_{node[1]}.txt").read().splitlines()
+    assert lines[0] == "test lake" and lines[2].startswith(f"i ={node[0]:4d}  j ={node[1]:4d}")
+    rec = lines[7:]
+    l = 150
+    kmz = int(o.get("kmz")[l])
+    assert len(rec) == kmz  # layers k1 .. kmz+1, one record each
+    assert float(rec[0][1:11]) == pytest.approx(3 * d["dt"] / 3600.0, abs=1e-4)
+    assert int(rec[0][11:21]) == 3
+    assert float(rec[0][21:30]) == pytest.approx(100 * o.get("s")[l], abs=0.006)
+    u, av, sal, h = o.get("u"), o.get("Av"), o.get("sal"), o.get("h")
+    for r, k in zip(rec, range(2, kmz + 2)):
+        body = r[30:]
+        assert r.startswith(" ") and (r is rec[0] or r[:30] == " " * 30)
+        fields = [body[0:9], body[9:19], body[19:29], body[29:38], body[38:53], body[53:68], body[68:83]]
+        if h[l, k - 1] <= 0:
+            assert float(fields[1]) == -99.0
+            continue
+        assert float(fields[1]) == pytest.approx(100 * u[l, k - 1], abs=0.006)
+        assert float(fields[4]) == pytest.approx(1e4 * 0.5 * (av[l, k - 1] + av[l, k]), abs=6e-8)
+        assert float(fields[6]) == pytest.approx(sal[l, k - 1], rel=5e-5)  # E15.5: five digits
+
+
+def _records(path):
+    raw = open(path, "rb").read()
+    out, p = [], 0
+    while p < len(raw):
+        n = int(np.frombuffer(raw[p:p + 4], np.int32)[0])
+        out.append(raw[p + 4:p + 4 + n])
+        assert int(np.frombuffer(raw[p + 4 + n:p + 8 + n], np.int32)[0]) == n
+        p += n + 8
+    return out
+
+
+def test_plane_file_records(oracle_lib, tmp_path):
+    d, o = _run(oracle_lib)
+    kp = 4
+    pf = O.PlaneFiles(o, d, [kp], nts=10, iop=1, directory=str(tmp_path), start=(2020, 5, 5, 1200))
+    pf.write(0)
+    pf.write(3)
+    pf.close()
+    recs = _records(tmp_path / f"plane_{kp}")
+    assert len(recs) == 4
+    assert np.frombuffer(recs[0], np.int32)[0] == 10
+    kmz = o.get("kmz")[:o.lm].astype(int)
+    cells = np.nonzero(kmz >= kp)[0]
+    assert np.frombuffer(recs[1], np.int32)[0] == len(cells)
+    first = recs[2]
+    ints = np.frombuffer(first[:16], np.int32)
+    assert list(ints) == [0, 2020, 5, 5]
+    assert np.frombuffer(first[16:20], np.float32)[0] == 1200.0
+    data = np.frombuffer(first[20:], np.float32).reshape(len(cells), 8)
+    ii, jj = O.wet_columns(d)
+    np.testing.assert_array_equal(data[:, 0], ii[cells])
+    np.testing.assert_array_equal(data[:, 1], jj[cells])
+    np.testing.assert_array_equal(data[:, 2], np.float32(o.get("up")[cells, kp - 1]))
+    np.testing.assert_array_equal(data[:, 7], np.float32(o.get("s")[cells]))
+    later = recs[3]
+    ints = np.frombuffer(later[:16], np.int32)
+    assert list(ints) == [3, 2020, 5, 5]
+    hour = 12.0 + 3 * d["dt"] / 3600.0
+    assert np.frombuffer(later[16:20], np.float32)[0] == np.float32(round(hour * 100))
+    data = np.frombuffer(later[20:], np.float32).reshape(len(cells), 6)
+    sal = o.get("salp")[cells, kp - 1]
+    np.testing.assert_array_equal(data[:, 3], np.float32(np.where(o.get("hp")[cells, kp - 1] <= 0, -99.0, sal)))
+    wp = o.get("wp")
+    np.testing.assert_allclose(data[:, 2], (wp[cells, kp - 1] + wp[cells, kp]) / 2, rtol=1e-6, atol=1e-12)
