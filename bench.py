@@ -19,6 +19,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -81,6 +83,10 @@ def parse():
     ap.add_argument("--ensemble", type=int, default=0, metavar="N",
                     help="advance N lakes of uneven size (sides 128-768, 20-80 layers, random wind) together")
     ap.add_argument("--side-max", type=int, default=768)
+    ap.add_argument("--tracers", type=int, default=0, metavar="N",
+                    help="advance N conservative tracers as well (step profiles)")
+    ap.add_argument("--surfbc", type=int, default=20, choices=[1, 2, 3, 20],
+                    help="surface forcing: 20 wind series (default), 1-3 heat budget over the same wind")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (calibrates HBM byte counters) and exit")
     return ap.parse_args()
@@ -172,10 +178,20 @@ def main():
                     f"dx=100 m, dt=60 s, wind 0.05-0.3 Pa any direction, {a.precision}")
     else:
         deck = lake.ellipse(nx=a.nx, ny=a.ny, dx=100.0, nlayers=a.layers, dt=60.0, nts=nts, seed=1)
+        if a.surfbc != 20:
+            deck = lake.heat_budget(deck, a.surfbc)
+        if a.tracers:
+            km1 = len(deck["init_profile"])
+            deck["tracer_profiles"] = np.stack([np.where(np.arange(km1) < km1 // (2 + q), 10.0, 2.0)
+                                                for q in range(a.tracers)])
         n_wet = lake.n_wet_cells(deck)
         g = GpuLake(deck, precision=a.precision, preconditioner=a.precond, tol=a.tol)
         workload = (f"ellipse lake {a.nx}x{a.ny}x{a.layers}, dx=100 m, dt=60 s, TVD + "
                     f"turbulence, plain PCG ({a.precond}) tol {a.tol:g}, {a.precision}")
+        if a.surfbc != 20:
+            workload += f", surface heat budget ifsurfbc={a.surfbc}"
+        if a.tracers:
+            workload += f", {a.tracers} tracers"
     t_setup = time.perf_counter() - t_setup
     if not a.no_learned:
         g.learn_setup(capacity=256, seed=1)  # the warmup solves record training pairs
