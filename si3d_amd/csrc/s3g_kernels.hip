@@ -1675,6 +1675,12 @@ int autotune(s3g_ctx *c) {
     set_error("autotune: no device memory for the output snapshot");
     return S3G_ERR_HIP;
   }
+  // Each timed run starts from cold caches, as in a step where the inputs were
+  // written by earlier kernels (timing back-to-back repeats would let the 256 MB
+  // Infinity Cache hold the inputs and favour candidates that re-read them).
+  void *scrub = nullptr;
+  const size_t scrub_bytes = (size_t)512 << 20;
+  if (hipMalloc(&scrub, scrub_bytes) != hipSuccess) scrub = nullptr;
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
@@ -1697,6 +1703,7 @@ int autotune(s3g_ctx *c) {
     float best = 1e30f;
     launch();
     for (int rep = 0; rep < reps; ++rep) {
+      if (scrub) hipMemsetAsync(scrub, rep, scrub_bytes, c->stream);
       hipEventRecord(e0, c->stream);
       launch();
       hipEventRecord(e1, c->stream);
@@ -1779,6 +1786,7 @@ int autotune(s3g_ctx *c) {
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   hipFree(save);
+  if (scrub) hipFree(scrub);
   c->timing = timing;
   return e;
 }
