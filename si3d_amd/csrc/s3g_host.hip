@@ -427,8 +427,10 @@ static int alloc_state(s3g_ctx *c, Dev<R> &D, const int *lE, const int *lW, cons
       return S3G_ERR_HIP;
   }
   double **q3[] = {&D.q2, &D.q2p, &D.q2pp, &D.q2l, &D.q2lp, &D.q2lpp, &D.d0, &D.d1, &D.d2};
-  for (double **p : q3)
-    if (!(*p = dalloc<double>(c, c->P.iturb > 0 ? n3 : 1))) return S3G_ERR_HIP;
+  for (double **p : q3) {  // d0, d1 also hold the fused turbulence sweep's factor pairs: 2 n3
+    const size_t n = (c->P.iturb > 0 ? n3 : 1) * ((p == &D.d0 || p == &D.d1) ? 2 : 1);
+    if (!(*p = dalloc<double>(c, n))) return S3G_ERR_HIP;
+  }
   std::vector<double> one(n2, 1.0);
   hipMemcpy(D.rr, one.data(), n2 * sizeof(double), hipMemcpyHostToDevice);
   return hip_check(hipGetLastError(), "alloc");

@@ -1084,7 +1084,11 @@ __global__ void __launch_bounds__(kBlock) k_turb(Dev<R> D, int istep) {
   if (nwl <= 1) return;
   const double twodt1 = D.P.twodt / istep, r3 = 3. / twodt1;
   const int lW = D.lW[m], lS = D.lS[m];
-  double *E1 = D.d0, *E2 = D.d1;  // forward-sweep factors of the two systems
+  // forward-sweep factors of the two systems: split, E1/E2 in d0/d1 and the
+  // right-hand sides in q2/q2l; fused, (e1, e2) and (f1, f2) as pairs in d0 and
+  // d1 (2 n3 doubles each), one 16-byte access per level and direction
+  double *E1 = D.d0, *E2 = D.d1;
+  double2 *EP = reinterpret_cast<double2 *>(D.d0), *FP = reinterpret_cast<double2 *>(D.d1);
   double hbelow = 0.0;
   for (int k = k1s + 1; k <= kms; ++k) hbelow += D.hp[IX(k, m)];
   // interface quantities of level k: read back (split) or computed from the
@@ -1116,10 +1120,15 @@ __global__ void __launch_bounds__(kBlock) k_turb(Dev<R> D, int istep) {
   // kms+1 are the Dirichlet rows: a1 = a3 = 0, a2 = 3/twodt1), with the
   // interface quantities of level k+1 computed one level ahead.
   double e1 = 0.0, f1 = ds_top / r3, e2 = 0.0, f2 = 0.0;  // row k1s
-  D.q2[IX(k1s, m)] = f1;
-  D.q2l[IX(k1s, m)] = f2;
-  E1[IX(k1s, m)] = 0.0;
-  E2[IX(k1s, m)] = 0.0;
+  if (SPLIT) {
+    D.q2[IX(k1s, m)] = f1;
+    D.q2l[IX(k1s, m)] = f2;
+    E1[IX(k1s, m)] = 0.0;
+    E2[IX(k1s, m)] = 0.0;
+  } else {
+    EP[IX(k1s, m)] = make_double2(0.0, 0.0);
+    FP[IX(k1s, m)] = make_double2(f1, f2);
+  }
   TurbRaw rcur{};  // raw values of level k (unused when split)
   if (!SPLIT) rcur = turb_raw(D, m, k1s + 1, lW, lS);
   TurbLev cur = level(k1s + 1, SPLIT ? rcur : turb_raw(D, m, k1s, lW, lS), rcur);
@@ -1182,10 +1191,15 @@ __global__ void __launch_bounds__(kBlock) k_turb(Dev<R> D, int istep) {
     den = a2l + a1 * e2;
     e2 = -a3 / den;
     f2 = (dsl - a1 * f2) / den;
-    E1[IX(k, m)] = e1;
-    E2[IX(k, m)] = e2;
-    D.q2[IX(k, m)] = f1;
-    D.q2l[IX(k, m)] = f2;
+    if (SPLIT) {
+      E1[IX(k, m)] = e1;
+      E2[IX(k, m)] = e2;
+      D.q2[IX(k, m)] = f1;
+      D.q2l[IX(k, m)] = f2;
+    } else {
+      EP[IX(k, m)] = make_double2(e1, e2);
+      FP[IX(k, m)] = make_double2(f1, f2);
+    }
     // new eddy coefficients of interface k (the surface minimum, 281-284)
     if (!SPLIT) {
       double av = cur.av, dv = cur.dv;
@@ -1203,8 +1217,14 @@ __global__ void __launch_bounds__(kBlock) k_turb(Dev<R> D, int istep) {
   D.q2[IX(kms + 1, m)] = dmax(Q2_MIN, x1);
   D.q2l[IX(kms + 1, m)] = dmax(Q2L_MIN, x2);
   for (int k = kms; k >= k1s; --k) {
-    x1 = E1[IX(k, m)] * x1 + D.q2[IX(k, m)];
-    x2 = E2[IX(k, m)] * x2 + D.q2l[IX(k, m)];
+    if (SPLIT) {
+      x1 = E1[IX(k, m)] * x1 + D.q2[IX(k, m)];
+      x2 = E2[IX(k, m)] * x2 + D.q2l[IX(k, m)];
+    } else {
+      const double2 e = EP[IX(k, m)], f = FP[IX(k, m)];
+      x1 = e.x * x1 + f.x;
+      x2 = e.y * x2 + f.y;
+    }
     D.q2[IX(k, m)] = dmax(Q2_MIN, x1);
     D.q2l[IX(k, m)] = dmax(Q2L_MIN, x2);
   }
