@@ -144,6 +144,37 @@ MODULE si3d_gpu
          IMPORT :: C_PTR
          TYPE(C_PTR) :: s3g_last_error
       END FUNCTION s3g_last_error
+
+      ! learned initial guess of the free-surface solve (no counterpart in the reference)
+      FUNCTION s3g_learn_setup(ctx, capacity, seed) BIND(C, NAME="s3g_learn_setup")
+         IMPORT :: C_INT, C_PTR
+         TYPE(C_PTR), VALUE :: ctx
+         INTEGER(C_INT), VALUE :: capacity, seed
+         INTEGER(C_INT) :: s3g_learn_setup
+      END FUNCTION s3g_learn_setup
+
+      FUNCTION s3g_learn_record(ctx, on) BIND(C, NAME="s3g_learn_record")
+         IMPORT :: C_INT, C_PTR
+         TYPE(C_PTR), VALUE :: ctx
+         INTEGER(C_INT), VALUE :: on
+         INTEGER(C_INT) :: s3g_learn_record
+      END FUNCTION s3g_learn_record
+
+      FUNCTION s3g_learn_train(ctx, steps, batch, lr, loss) BIND(C, NAME="s3g_learn_train")
+         IMPORT :: C_INT, C_PTR, C_DOUBLE
+         TYPE(C_PTR), VALUE :: ctx
+         INTEGER(C_INT), VALUE :: steps, batch
+         REAL(C_DOUBLE), VALUE :: lr
+         REAL(C_DOUBLE) :: loss
+         INTEGER(C_INT) :: s3g_learn_train
+      END FUNCTION s3g_learn_train
+
+      FUNCTION s3g_learn_mode(ctx, mode) BIND(C, NAME="s3g_learn_mode")
+         IMPORT :: C_INT, C_PTR
+         TYPE(C_PTR), VALUE :: ctx
+         INTEGER(C_INT), VALUE :: mode
+         INTEGER(C_INT) :: s3g_learn_mode
+      END FUNCTION s3g_learn_mode
    END INTERFACE
 
 CONTAINS
