@@ -54,3 +54,11 @@ def test_ensemble_rejects_mismatched_time_steps():
     decks[1]["dt"] = 30.0
     with pytest.raises(RuntimeError, match="differ only"):
         GpuEnsemble(decks)
+
+
+def test_ensemble_rejects_heat_budget():
+    from si3d_amd.model import GpuEnsemble
+    decks = _decks()
+    decks[2] = lake.heat_budget(decks[2], 1)
+    with pytest.raises(RuntimeError, match="ifsurfbc"):
+        GpuEnsemble(decks)

@@ -104,3 +104,26 @@ def test_c0_rectangle_200_steps_matches_oracle(oracle_lib):
     print(errs, flush=True)
     for n, e in errs.items():
         assert e < 1e-7, (n, e)
+
+
+def test_c3_2048_volume_and_free_stream():
+    """The 2048x2048x60 lake (dx = 50 m, N_wet 1.25e8, the largest single-GPU
+    configuration): basin volume conserved and a uniform temperature kept
+    uniform over 6 steps, FP64."""
+    from si3d_amd.model import GpuLake
+    deck = lake.ellipse(nx=2048, ny=2048, dx=50.0, nlayers=60, dt=60.0, nts=10, seed=2)
+    deck["init_profile"] = np.full(len(deck["init_profile"]), 12.0)
+    print("case: 2048x2048x60 volume and uniform temperature", flush=True)
+    g = GpuLake(deck, precision="fp64", preconditioner="mg", tol=1e-12)
+    v0 = g.get("s")[:g.lm].sum()
+    for n in range(1, 7):
+        g.step(n)
+    s = g.get("s")[:g.lm]
+    assert np.max(np.abs(s)) > 1e-6
+    assert abs(s.sum() - v0) <= 1e-9 * g.lm * np.max(np.abs(s))
+    sal = g.get("sal")
+    kmz = g.get("kmz")[:g.lm].astype(int)
+    k1z = g.get("k1z")[:g.lm].astype(int)
+    kk = np.arange(sal.shape[1])[None, :] + 1
+    wet = (kk >= k1z[:, None]) & (kk <= kmz[:, None])
+    assert np.max(np.abs(sal[:g.lm][wet] - 12.0)) < 1e-9 * 12.0
