@@ -13,6 +13,8 @@
 // layer indices come out bit-identical to the Fortran driver's.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -358,9 +360,11 @@ extern "C" int s3g_create_ensemble(int nlakes, const s3g_deck *decks, const s3g_
     GF.q2l.assign((size_t)km1 * lm1, 0.0);
   }
   GF.tr.assign(std::max(0, p0.ntr), std::vector<double>((size_t)km1 * lm1, 0.0));
+  const bool verbose = std::getenv("S3G_DEBUG") != nullptr;
   for (int i = 0; i < nlakes; ++i) {
     const HostLake &L = Ls[i];
     const int li1 = L.lm + 1, o = off[i];
+    if (verbose) std::fprintf(stderr, "[s3g] ensemble: lake %d of %d (%d columns)\n", i + 1, nlakes, L.lm);
     lake_km1[i] = L.km1;
     zl[(size_t)i * KP] = -100.0;
     for (int k = 1; k <= L.km1; ++k) zl[(size_t)i * KP + k] = L.zlevel[k - 1];

@@ -1703,11 +1703,14 @@ int autotune(s3g_ctx *c) {
   const bool timing = c->timing;
   c->timing = false;
   char *save = nullptr;
-  const size_t save_bytes = 6 * n3 * sizeof(double);
-  if (hipMalloc(&save, save_bytes) != hipSuccess) {
+  // the largest set of outputs saved at once: five state arrays, or q2, q2l,
+  // Av, Dv
+  const size_t save_bytes = std::max(5 * sizeof(R), 2 * sizeof(double) + 2 * sizeof(R)) * n3;
+  if (hipMalloc(&save, save_bytes) != hipSuccess) {  // no room: keep the default tiles
+    (void)hipGetLastError();
     c->timing = timing;
-    set_error("autotune: no device memory for the output snapshot");
-    return S3G_ERR_HIP;
+    if (c->debug) std::fprintf(stderr, "[s3g] autotune skipped: no device memory for the output snapshot\n");
+    return 0;
   }
   // Each timed run starts from cold caches, as in a step where the inputs were
   // written by earlier kernels (timing back-to-back repeats would let the 256 MB
