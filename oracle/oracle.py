@@ -31,6 +31,8 @@ class OrcDeck(C.Structure):
         ("init_profile", C.POINTER(C.c_double)), ("n_surfbc", C.c_int),
         ("surfbc", C.POINTER(C.c_double)), ("pcg_tol", C.c_double), ("pcg_maxit", C.c_int),
         ("ntr", C.c_int), ("tracer_profiles", C.POINTER(C.c_double)),
+        ("nopen", C.c_int), ("dtsec_openbc", C.c_double), ("openbc", C.POINTER(C.c_int)),
+        ("openbc_npts", C.c_int), ("openbc_values", C.POINTER(C.c_double)),
     ]
 
 
@@ -86,7 +88,7 @@ class Oracle:
         d = OrcDeck()
         for name, _ in OrcDeck._fields_:
             if name in ("zlevel_file", "bathy_dm", "init_profile", "surfbc", "km1_file", "n_surfbc", "ntr",
-                        "tracer_profiles"):
+                        "tracer_profiles", "nopen", "dtsec_openbc", "openbc", "openbc_npts", "openbc_values"):
                 continue
             if name in deck:
                 setattr(d, name, deck[name])
@@ -111,6 +113,17 @@ class Oracle:
             self._keep.append(tp)
             d.ntr = tp.shape[0]
             d.tracer_profiles = _dp(tp)
+        ob = deck.get("openbc")
+        if ob is not None:
+            spec = np.ascontiguousarray(np.stack([ob["iside"], ob["itype"], ob["isbc"], ob["jsbc"], ob["iebc"],
+                                                  ob["jebc"]], axis=1), dtype=np.int32)
+            vals = np.ascontiguousarray(ob["values"], dtype=np.float64)  # (nopen, npts, ntr+2)
+            self._keep += [spec, vals]
+            d.nopen = spec.shape[0]
+            d.dtsec_openbc = float(ob["dtsec"])
+            d.openbc = spec.ctypes.data_as(C.POINTER(C.c_int))
+            d.openbc_npts = vals.shape[1]
+            d.openbc_values = _dp(vals)
         self.deck = deck
         self.h = L.orc_create(C.byref(d))
         if not self.h:

@@ -157,6 +157,22 @@ struct OrcLake {
   int ntr = 0;
   std::vector<double> tprof;  // ntr*km1 initial profiles
   std::vector<F3> tracer, tracerpp;
+  // ---- open boundaries, types 1 (water surface elevation) and 2 (free-surface
+  // flow) (si3d_boundaryconditions.f90).  The boundary transports and
+  // thicknesses keep the reference's storage (si3d_utils.f90:441-455):
+  // uhWB/huWB, uhEB/huEB over (k, j) and vhNB/hvNB, vhSB/hvSB over (k, i),
+  // each at n+1, n and n-1; here bq/bh[side][level] with side = iside-1
+  // (0 W, 1 N, 2 E, 3 S) and level 0 = n+1, 1 = n, 2 = n-1.
+  struct Obc {
+    int side, type, is, js, ie, je;
+  };
+  std::vector<Obc> obc;
+  int obc_np = 0, obc_nv = 0;
+  double obc_dtsec = 0.0;
+  std::vector<double> obc_vals;  // [nopen][nv] series of obc_np values, 1-based
+  std::vector<double> bq[4][3], bh[4][3];
+  double &BQ(int sd, int lev, int k, int a) { return bq[sd][lev][(size_t)a * (km1 + 2) + k]; }
+  double &BH(int sd, int lev, int k, int a) { return bh[sd][lev][(size_t)a * (km1 + 2) + k]; }
   // ---- solver statistics
   int last_iters = 0;
   double last_relres = 0.0;
@@ -199,6 +215,17 @@ struct OrcLake {
   void smooth();
   void turbulence(int istep);
   int fd(int n, int istep, int lastiter);
+  // open boundaries
+  double obc_value(int nn, int q) const;
+  bool openbc0();
+  void openbc_uvh();
+  void flow_section(const Obc &b, double qbc, bool first);
+  void mod_qq();
+  void mod_exmom(int ieq);
+  void mod_vel();
+  void openbc_scalar(F3 &c, int q);
+  void mod_exscalar(const F3 &cpp, int q);
+  void obc_shift(int mode);
   void settrap();
   void settrap2();
   void save(int istep);
@@ -406,7 +433,7 @@ bool OrcLake::build(const OrcDeck *deck) {
   h = hp;
   for (l = 0; l <= lm1; ++l) rr[l] = 1.0;
   init_state();
-  return true;
+  return openbc0();
 }
 
 // InitializeScalarFields (si3d_procedures.f90:184-379, InitProc=4) and
@@ -763,6 +790,7 @@ void OrcLake::exmom(int ieq) {
         ex(k, l) = vhpp(k, l) - twodt1 * (adv * d.iadv + cor - hd);
     }
   }
+  if (!obc.empty()) mod_exmom(ieq);  // MODexmom4openBCX / Y, 957 / 1165
 }
 
 // bclnc_km, si3d_procedures.f90:1633-1746.
@@ -919,6 +947,7 @@ void OrcLake::matcon() {
     qq[l] = spp[l] - dtdx1 * (eagx[l] - eW) - dtdy1 * (eagy[l] - eS) - dd[l];
     rr[l] = 1 + sx[l] + sx[lW] + sy[l] + sy[lS];
   }
+  if (!obc.empty()) mod_qq();  // MODqqddrr4openBC, 1859
 }
 
 // SolverSparse (si3d_procedures.f90:1870-2065): A zeta = qq with ELLPACK rows
@@ -1199,6 +1228,7 @@ void OrcLake::continuity(int ist) {
     for (int k = kms; k >= k1s; --k) wp(k, l) = wp(k + 1, l) - div(k);
     for (int k = k1; k <= k1s - 1; ++k) wp(k1s, l) = wp(k1s, l) - div(k);
   }
+  if (!obc.empty()) mod_vel();  // MODvel4openBC, 2821
 }
 
 // exsal, si3d_procedures.f90:2827-3076 (superbee TVD; no open boundaries).
@@ -1288,6 +1318,13 @@ void OrcLake::exscalar(const F3 &salpp, bool is_tracer) {
       bex(k, l) = b;
     }
     if (is_tracer) bex(kms + 1, l) = hpp(kms + 1, l) * salpp(kms + 1, l) / twodt1;
+  }
+  // MODexsal4openbc (3070) / MODexTracer4openbc (4350)
+  if (!obc.empty()) {
+    int q = 1;
+    for (int nt = 0; nt < ntr; ++nt)
+      if (&salpp == &tracerpp[nt]) q = 2 + nt;
+    mod_exscalar(salpp, q);
   }
 }
 
@@ -1543,6 +1580,7 @@ void OrcLake::turbulence(int istep) {
 // sources; conservative tracers only, ecomod = 0).
 int OrcLake::fd(int n, int istep, int lastiter) {
   tz = 1.0 / istep;
+  openbc_uvh();  // 423
   surfbc(n, istep);
   hmix();
   exmom(1);
@@ -1551,16 +1589,19 @@ int OrcLake::fd(int n, int istep, int lastiter) {
   matmom(2);
   matcon();
   int rc = solver();
+  openbc_uvh();  // 482
   vel();
   if (d.isal != 0) {
     exsal();
     imsal();
+    openbc_scalar(sal, 1);  // openbcSCA, 515
   }
   // non-active scalars on the last iteration only (521-547)
   if (ntr > 0 && d.niter > 0 && lastiter == 1)
     for (int nt = 0; nt < ntr; ++nt) {
       exscalar(tracerpp[nt], true);
       imtracer(nt);
+      openbc_scalar(tracer[nt], 2 + nt);  // openbcTracer, 544
     }
   vel2();
   if (d.ismooth >= 1 && istep == 1) smooth();
@@ -1634,6 +1675,7 @@ void OrcLake::settrap() {
     }
   }
   continuity(2);
+  obc_shift(0);  // 3558-3604
   if (d.iturb > 0) {
     for (int m = 1; m <= lm; ++m) {
       for (int k = k1; k <= kmz[m]; ++k) {
@@ -1772,6 +1814,7 @@ void OrcLake::save(int istep) {
     }
   }
   continuity(2);
+  obc_shift(istep == 1 ? 1 : 2);  // 3722-3768 (istep 2), 3872-3918 (istep 1)
   if (d.iturb > 0) {
     for (int m = 1; m <= lm1; ++m) {
       for (int k = 0; k <= km1 + 1; ++k) {
@@ -1790,6 +1833,361 @@ void OrcLake::save(int istep) {
         q2p(k1s + 1, m) = q2p(k1s + 2, m);
       }
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Open boundaries (si3d_boundaryconditions.f90), types 1 and 2 on the four
+// sides; one sweep over the boundaries in input order, as the reference's
+// single-thread loops do (si3d.f90:196-246 give each boundary its full range
+// with one thread).
+
+// Boundary value q of boundary nn at thrs: parab over records dtsecOpenBC
+// apart (938-939, 3187-3188).  Past the last record the end parabola is
+// extended (the reference reads beyond the series there).
+double OrcLake::obc_value(int nn, int q) const {
+  const double *fx = &obc_vals[((size_t)nn * obc_nv + q) * (obc_np + 1)];
+  const double dxs = obc_dtsec / 3600.;
+  int m = (int)(thrs / dxs);
+  double theta = (thrs - (double)m * dxs) / dxs;
+  if (m == 0) {
+    m = 2;
+    theta -= 1.0;
+  } else {
+    m = m + 1;
+  }
+  if (m + 1 > obc_np) m = obc_np - 1;
+  return fx[m] + 0.5 * theta * (fx[m + 1] - fx[m - 1] + theta * (fx[m + 1] + fx[m - 1] - 2.0 * fx[m]));
+}
+
+// Thickness and transport of the boundary cells of a flow boundary for flow
+// qbc spread uniformly over the section (openbc0 420-647, openbcUVH 976-1176).
+// The free surface of the section is taken from the column one past its end
+// (the Fortran loop index after the loop); the south side sums hvSB where the
+// reference sums hvNB (1170), which divides by zero without a north boundary.
+void OrcLake::flow_section(const Obc &b, double qbc, bool first) {
+  const int sd = b.side;
+  const bool we = (sd == 0 || sd == 2);
+  double hbc = -1.e6;
+  for (int a = (we ? b.js : b.is); a <= (we ? b.je : b.ie); ++a) {
+    int l = we ? IJ(b.is, a) : IJ(a, b.js);
+    if (hhs[l] > hbc) hbc = hhs[l];
+  }
+  int lpast = we ? IJ(b.is, b.je + 1) : IJ(b.ie + 1, b.js);
+  double sbc = std::max(s[lpast], -hbc + d.dzmin);
+  double areatot = 0.0;
+  const int a0 = we ? b.js : b.is, a1 = we ? b.je : b.ie;
+  // W/E zero the thickness first; N/S do so only in openbc0 (1089, 1146)
+  if (we || first)
+    for (int a = a0; a <= a1; ++a)
+      for (int k = 1; k <= km1; ++k) BH(sd, 0, k, a) = ZERO;
+  for (int a = a0; a <= a1; ++a) {
+    int l = we ? IJ(b.is, a) : IJ(a, b.js);
+    for (int k = k1; k <= kmz[l]; ++k) {
+      double hh = std::min(zlevel[k + 1], hhs[l]) - std::max(zlevel[k], -sbc);
+      if (hh <= HMIN) hh = ZERO;
+      BH(sd, 0, k, a) = hh;
+      areatot = areatot + hh * dy;  // dy on every side (440-647, 990-1170)
+    }
+  }
+  double uavg = qbc / areatot;
+  for (int a = a0; a <= a1; ++a)
+    for (int k = 1; k <= km1; ++k) BQ(sd, 0, k, a) = 0.0;
+  for (int a = a0; a <= a1; ++a) {
+    int l = we ? IJ(b.is, a) : IJ(a, b.js);
+    int kf = (first && sd == 1) ? k1z[l] : k1;  // openbc0 north: from k1z (600)
+    for (int k = kf; k <= kmz[l]; ++k) BQ(sd, 0, k, a) = uavg * BH(sd, 0, k, a);
+  }
+}
+
+// openbc0, si3d_boundaryconditions.f90:23-897 for types 1 and 2: checks that
+// every boundary cell is wet with a dry outer neighbour, then the t = 0
+// values (sp of water-level boundaries, the flow section of flow boundaries).
+bool OrcLake::openbc0() {
+  const int nopen = std::max(0, d.nopen);
+  obc.clear();
+  if (nopen == 0 || !d.openbc) return true;
+  obc_np = d.openbc_npts;
+  obc_nv = ntr + 2;
+  obc_dtsec = d.dtsec_openbc;
+  if (obc_np < 3 || !d.openbc_values || obc_dtsec <= 0.0) return false;
+  obc_vals.assign((size_t)nopen * obc_nv * (obc_np + 1), 0.0);
+  for (int nn = 0; nn < nopen; ++nn)
+    for (int r = 0; r < obc_np; ++r)
+      for (int q = 0; q < obc_nv; ++q)
+        obc_vals[((size_t)nn * obc_nv + q) * (obc_np + 1) + r + 1] =
+            d.openbc_values[((size_t)nn * obc_np + r) * obc_nv + q];
+  const size_t na = (size_t)(std::max(im1, jm1) + 2) * (km1 + 2);
+  for (int sd = 0; sd < 4; ++sd)
+    for (int lev = 0; lev < 3; ++lev) bq[sd][lev].assign(na, 0.0), bh[sd][lev].assign(na, 0.0);
+  for (int nn = 0; nn < nopen; ++nn) {
+    const int *o6 = d.openbc + 6 * nn;
+    Obc b{o6[0] - 1, o6[1], o6[2], o6[3], o6[4], o6[5]};
+    if (b.side < 0 || b.side > 3 || (b.type != 1 && b.type != 2)) return false;
+    const bool we = (b.side == 0 || b.side == 2);
+    // one column (W/E) or one row (N/S); the range runs along it
+    if (we ? (b.ie < b.is || b.je < b.js) : (b.je < b.js || b.ie < b.is)) return false;
+    if (we ? (b.is < 1 || b.is > im1 || b.js < 1 || b.je > jm1) : (b.js < 1 || b.js > jm1 || b.is < 1 || b.ie > im1))
+      return false;
+    for (int a = (we ? b.js : b.is); a <= (we ? b.je : b.ie); ++a) {
+      int i = we ? b.is : a, j = we ? a : b.js;
+      int io = i + (b.side == 0 ? -1 : b.side == 2 ? 1 : 0), jo = j + (b.side == 1 ? 1 : b.side == 3 ? -1 : 0);
+      if (!M2(i, j) || M2(io, jo)) return false;  // "not a bdry. point"
+      int l = IJ(i, j);
+      if (b.type == 1 && kmz[l] - k1z[l] + 1 < 1) return false;  // dry point on boundary
+    }
+    obc.push_back(b);
+  }
+  // t = 0 values (337-647); the n+1 arrays start at zero (334-339)
+  for (auto &b : obc) {
+    const bool we = (b.side == 0 || b.side == 2);
+    const double v0 = obc_vals[((size_t)(&b - &obc[0]) * obc_nv) * (obc_np + 1) + 1];
+    if (b.type == 1) {
+      for (int a = (we ? b.js : b.is); a <= (we ? b.je : b.ie); ++a) sp[we ? IJ(b.is, a) : IJ(a, b.js)] = v0;
+    } else {
+      flow_section(b, v0, true);
+    }
+  }
+  return true;
+}
+
+// openbcUVH, si3d_boundaryconditions.f90:900-1398 (types 1 and 2).
+void OrcLake::openbc_uvh() {
+  for (size_t nn = 0; nn < obc.size(); ++nn) {
+    const Obc &b = obc[nn];
+    const bool we = (b.side == 0 || b.side == 2);
+    double v = obc_value((int)nn, 0);
+    if (b.type == 1) {
+      for (int a = (we ? b.js : b.is); a <= (we ? b.je : b.ie); ++a) s[we ? IJ(b.is, a) : IJ(a, b.js)] = v;
+    } else {
+      flow_section(b, v, false);
+    }
+  }
+}
+
+// MODqqddrr4openBC, si3d_boundaryconditions.f90:1508-1690 (types 1 and 2):
+// a water-level boundary moves its coupling to the known zeta into qq of the
+// inner neighbour and drops it from the matrix; a flow boundary adds the
+// boundary inflow at n+1 and n-1 to qq of the boundary cell.
+void OrcLake::mod_qq() {
+  const double dtdx1 = dtdx * tz, dtdy1 = dtdy * tz;
+  for (const Obc &b : obc) {
+    const bool we = (b.side == 0 || b.side == 2);
+    const int a0 = we ? b.js : b.is, a1 = we ? b.je : b.ie;
+    if (b.type == 1) {
+      for (int a = a0; a <= a1; ++a) {
+        int l = we ? IJ(b.is, a) : IJ(a, b.js);
+        int li = b.side == 0 ? lEC[l] : b.side == 1 ? lSC[l] : b.side == 2 ? lWC[l] : lNC[l];  // inner neighbour
+        double cpl = b.side == 0 ? sx[l] : b.side == 1 ? sy[li] : b.side == 2 ? sx[li] : sy[l];
+        if (li != lm1) qq[li] = qq[li] + cpl * s[l];
+      }
+      for (int a = a0; a <= a1; ++a) {
+        int l = we ? IJ(b.is, a) : IJ(a, b.js);
+        if (b.side == 0) sx[l] = 0.0;
+        else if (b.side == 1) { if (lSC[l] != lm1) sy[lSC[l]] = 0.0; }
+        else if (b.side == 2) { if (lWC[l] != lm1) sx[lWC[l]] = 0.0; }
+        else sy[l] = 0.0;
+      }
+    } else {
+      for (int a = a0; a <= a1; ++a) {
+        int l = we ? IJ(b.is, a) : IJ(a, b.js);
+        double s0 = 0.0, s2 = 0.0;
+        for (int k = k1; k <= kmz[l]; ++k) s0 += BQ(b.side, 0, k, a);
+        for (int k = k1; k <= kmz[l]; ++k) s2 += BQ(b.side, 2, k, a);
+        switch (b.side) {
+          case 0: qq[l] = qq[l] + dtdx1 * s0 + dtdx1 * s2; break;
+          case 1: qq[l] = qq[l] - dtdy1 * s0 - dtdy1 * s2; break;
+          case 2: qq[l] = qq[l] - dtdx1 * s0 - dtdx1 * s2; break;
+          default: qq[l] = qq[l] + dtdy1 * s0 + dtdy1 * s2; break;
+        }
+      }
+    }
+  }
+}
+
+// MODexmom4openBCX / MODexmom4openBCY, si3d_boundaryconditions.f90:1801-2936
+// (types 1 and 2): the explicit term of the first momentum face inside the
+// boundary is recomputed from upwind advection only, with the boundary
+// transport as the outer neighbour of a flow boundary.  West and east faces
+// in x, north and south faces in y; the east water-level face keeps ex = 0
+// (its assignment is commented out in the reference, 2070).
+void OrcLake::mod_exmom(int ieq) {
+  const double twodt1 = twodt * tz;
+  const bool x = (ieq == 1);
+  auto P = [](double a) { return a + std::fabs(a); };
+  auto M = [](double a) { return a - std::fabs(a); };
+  for (const Obc &b : obc) {
+    if (x != (b.side == 0 || b.side == 2)) continue;
+    const int a0 = x ? b.js : b.is, a1 = x ? b.je : b.ie;
+    for (int a = a0; a <= a1; ++a) {
+      int lb = x ? IJ(b.is, a) : IJ(a, b.js);
+      // the face column: the boundary cell (W, S) or its inner neighbour (E, N)
+      int l = (b.side == 0 || b.side == 3) ? lb : (b.side == 2 ? IJ(b.is - 1, a) : IJ(a, b.js - 1));
+      if (l == lm1) continue;  // no wet cell inside (the reference would write the sentinel column)
+      if (b.type == 2) {
+        if (x && !M2(l2i[l] + 1, l2j[l])) continue;
+        if (!x && !M2(l2i[l], l2j[l] + 1)) continue;
+      }
+      int kmq = x ? std::min(kmz[lEC[l]], kmz[l]) : std::min(kmz[lNC[l]], kmz[l]);
+      int k1q = x ? k1u[l] : k1v[l];
+      if (kmq - k1q + 1 < 1) continue;
+      for (int k = 1; k <= km1; ++k) ex(k, l) = 0.0;
+      if (b.type == 1 && b.side == 2) continue;
+      const int lE = lEC[l], lW = lWC[l], lN = lNC[l], lS = lSC[l];
+      for (int k = k1q; k <= kmq; ++k) {
+        double bq0 = 0.0, bdry = 0.0;
+        if (b.type == 2) {
+          bq0 = (BQ(b.side, 0, k, a) + BQ(b.side, 2, k, a)) / 2.;
+          bdry = (BH(b.side, 2, k, a) > ZERO) ? BQ(b.side, 2, k, a) / BH(b.side, 2, k, a) : 0.0;
+        }
+        double adv;
+        if (x) {
+          double uE, uW;
+          double vN = vhp(k, lE) + vhp(k, l), vS = vhp(k, lSC[lE]) + vhp(k, lS);
+          double wU = wp(k, lE) + wp(k, l), wD = wp(k + 1, lE) + wp(k + 1, l);
+          double hx;
+          if (b.side == 0) {
+            uE = uhp(k, lE) + uhp(k, l);
+            uW = uhp(k, l) + (b.type == 2 ? bq0 : uhp(k, l));
+            double uout = (b.type == 2) ? bdry : upp(k, l);
+            hx = P(uE) * upp(k, l) + M(uE) * upp(k, lE) - P(uW) * uout - M(uW) * upp(k, l);
+          } else {  // east flow boundary
+            uE = uhp(k, l) + bq0;
+            uW = uhp(k, lW) + uhp(k, l);
+            hx = P(uE) * upp(k, l) + M(uE) * bdry - P(uW) * upp(k, lW) - M(uW) * upp(k, l);
+          }
+          adv = hx / fourdx + (P(vN) * upp(k, l) + M(vN) * upp(k, lN) - P(vS) * upp(k, lS) - M(vS) * upp(k, l)) / fourdy;
+          adv = adv + (P(wU) * upp(k, l) + M(wU) * upp(k - 1, l)) / 4. - (P(wD) * upp(k + 1, l) + M(wD) * upp(k, l)) / 4.;
+          ex(k, l) = uhpp(k, l) - twodt1 * (adv * d.iadv);
+        } else {
+          double uE = uhp(k, lN) + uhp(k, l), uW = uhp(k, lWC[lN]) + uhp(k, lW);
+          double wU = wp(k, lN) + wp(k, l), wD = wp(k + 1, lN) + wp(k + 1, l);
+          double vN, vS, hy;
+          if (b.side == 1) {  // face south of a north boundary
+            vN = vhp(k, l) + (b.type == 2 ? bq0 : vhp(k, l));
+            vS = vhp(k, l) + vhp(k, lS);
+            double vout = (b.type == 2) ? bdry : vpp(k, l);
+            hy = P(vN) * vpp(k, l) + M(vN) * vout - P(vS) * vpp(k, lS) - M(vS) * vpp(k, l);
+          } else {  // face north of a south boundary
+            vN = vhp(k, lN) + vhp(k, l);
+            vS = vhp(k, l) + (b.type == 2 ? bq0 : vhp(k, l));
+            double vout = (b.type == 2) ? bdry : vpp(k, l);
+            hy = P(vN) * vpp(k, l) + M(vN) * vpp(k, lN) - P(vS) * vout - M(vS) * vpp(k, l);
+          }
+          adv = (P(uE) * vpp(k, l) + M(uE) * vpp(k, lE) - P(uW) * vpp(k, lW) - M(uW) * vpp(k, l)) / fourdx + hy / fourdy;
+          adv = adv + (P(wU) * vpp(k, l) + M(wU) * vpp(k - 1, l)) / 4. - (P(wD) * vpp(k + 1, l) + M(wD) * vpp(k, l)) / 4.;
+          ex(k, l) = vhpp(k, l) - twodt1 * (adv * d.iadv);
+        }
+      }
+    }
+  }
+}
+
+// MODvel4openBC, si3d_boundaryconditions.f90:2941-3154 (types 1 and 2), at
+// the end of every continuity call.
+void OrcLake::mod_vel() {
+  for (const Obc &b : obc) {
+    const bool we = (b.side == 0 || b.side == 2);
+    for (int a = (we ? b.js : b.is); a <= (we ? b.je : b.ie); ++a) {
+      int l = we ? IJ(b.is, a) : IJ(a, b.js);
+      if (b.type == 1) {
+        F3 &t = we ? vh : uh;
+        int ln = b.side == 0 ? lEC[l] : b.side == 1 ? lSC[l] : b.side == 2 ? lWC[l] : lNC[l];
+        for (int k = 1; k <= km1; ++k) t(k, l) = 0.0;
+        for (int k = 1; k <= km1; ++k) wp(k, l) = wp(k, ln);
+      } else {
+        int kms = kmz[l];
+        const double den = we ? twodx : twody;
+        wp(kms + 1, l) = 0.0;
+        double dru = 0.0;
+        for (int k = kms; k >= k1; --k) {
+          double q0 = BQ(b.side, 0, k, a), q2 = BQ(b.side, 2, k, a);
+          if (b.side == 0 || b.side == 3)
+            dru = dru - (-q0 - q2) / den;
+          else
+            dru = dru - (q0 + q2) / den;
+          wp(k, l) = wp(k, l) + dru;
+        }
+      }
+    }
+  }
+}
+
+// openbcSCA / openbcTracer, si3d_boundaryconditions.f90:3157-3311 and
+// 3661-3806: on a water-level boundary the scalar takes the boundary value
+// where the flow enters and the inner neighbour's value elsewhere.
+void OrcLake::openbc_scalar(F3 &c, int q) {
+  for (size_t nn = 0; nn < obc.size(); ++nn) {
+    const Obc &b = obc[nn];
+    if (b.type != 1) continue;
+    const double sbc = obc_value((int)nn, q);
+    const bool we = (b.side == 0 || b.side == 2);
+    for (int a = (we ? b.js : b.is); a <= (we ? b.je : b.ie); ++a) {
+      int l = we ? IJ(b.is, a) : IJ(a, b.js);
+      for (int k = k1z[l]; k <= kmz[l]; ++k) {
+        bool in;
+        int ln;
+        switch (b.side) {
+          case 0: in = uh(k, l) > 0.0; ln = lEC[l]; break;
+          case 1: in = vh(k, lSC[l]) < 0.0; ln = lSC[l]; break;
+          case 2: in = uh(k, lWC[l]) < 0.0; ln = lWC[l]; break;
+          default: in = vh(k, l) > 0.0; ln = lNC[l]; break;
+        }
+        c(k, l) = (in && sbc > 0.0) ? sbc : c(k, ln);
+      }
+    }
+  }
+}
+
+// MODexsal4openbc / MODexTracer4openbc, si3d_boundaryconditions.f90:3314-3658
+// and 3809-4137 (type 2): the advective flux through the boundary face, with
+// the boundary value on inflow and the cell's own value on outflow.
+void OrcLake::mod_exscalar(const F3 &cpp, int q) {
+  for (size_t nn = 0; nn < obc.size(); ++nn) {
+    const Obc &b = obc[nn];
+    if (b.type != 2) continue;
+    const double sbc = obc_value((int)nn, q);
+    const bool we = (b.side == 0 || b.side == 2);
+    for (int a = (we ? b.js : b.is); a <= (we ? b.je : b.ie); ++a) {
+      int l = we ? IJ(b.is, a) : IJ(a, b.js);
+      for (int k = k1z[l]; k <= kmz[l]; ++k) {
+        double f = BQ(b.side, 0, k, a) + BQ(b.side, 2, k, a);
+        switch (b.side) {
+          case 0: bex(k, l) = bex(k, l) + f * ((f >= 0.0 && sbc > 0.0) ? sbc : cpp(k, l)) / twodx; break;
+          case 1: bex(k, l) = bex(k, l) - f * ((f < 0.0 && sbc > 0.0) ? sbc : cpp(k, l)) / twody; break;
+          case 2: bex(k, l) = bex(k, l) - f * ((f < 0.0 && sbc > 0.0) ? sbc : cpp(k, l)) / twodx; break;
+          default: bex(k, l) = bex(k, l) + f * ((f > 0.0 && sbc > 0.0) ? sbc : cpp(k, l)) / twody; break;
+        }
+      }
+    }
+  }
+}
+
+// Time levels of the boundary arrays: settrap (mode 0, 3558-3604), save with
+// istep 1 (mode 1, 3872-3918) and istep 2 (mode 2, 3722-3768).  Each boundary
+// shifts both arrays of its orientation (W and E, or N and S) over its range,
+// so overlapping ranges shift twice, as in the reference.
+void OrcLake::obc_shift(int mode) {
+  for (const Obc &b : obc) {
+    const bool we = (b.side == 0 || b.side == 2);
+    const int sides[2] = {we ? 2 : 1, we ? 0 : 3};  // E and W, or N and S (3569-3572, 3581-3584)
+    for (int a = (we ? b.js : b.is); a <= (we ? b.je : b.ie); ++a)
+      for (int t = 0; t < 2; ++t) {
+        const int sd = sides[t];
+        for (int k = 1; k <= km1; ++k) {
+          if (mode <= 1) {
+            BQ(sd, 2, k, a) = BQ(sd, 1, k, a);
+            BH(sd, 2, k, a) = BH(sd, 1, k, a);
+          }
+          if (mode == 0) {
+            BQ(sd, 1, k, a) = (BQ(sd, 2, k, a) + BQ(sd, 0, k, a)) / 2.;
+            BH(sd, 1, k, a) = (BH(sd, 1, k, a) + BH(sd, 0, k, a)) / 2.;
+          } else {
+            BQ(sd, 1, k, a) = BQ(sd, 0, k, a);
+            BH(sd, 1, k, a) = BH(sd, 0, k, a);
+          }
+        }
+      }
   }
 }
 
@@ -1861,17 +2259,21 @@ int orc_phase(OrcLake *o, int phase, int n, int istep, double thrs) {
   o->thrs = thrs;
   o->tz = 1.0 / istep;
   switch (phase) {
-    case 0: o->surfbc(n, istep); break;
+    case 0: o->openbc_uvh(); o->surfbc(n, istep); break;
     case 1: o->hmix(); break;
     case 2: o->exmom(1); break;
     case 3: o->matmom(1); break;
     case 4: o->exmom(2); break;
     case 5: o->matmom(2); break;
     case 6: o->matcon(); break;
-    case 7: return o->solver();
+    case 7: {
+      int rc = o->solver();
+      o->openbc_uvh();
+      return rc;
+    }
     case 8: o->vel(); break;
     case 9: o->exsal(); break;
-    case 10: o->imsal(); break;
+    case 10: o->imsal(); o->openbc_scalar(o->sal, 1); break;
     case 11: o->vel2(); break;
     case 12: o->smooth(); break;
     case 13: o->turbulence(istep); break;

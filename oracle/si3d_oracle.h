@@ -41,6 +41,14 @@ typedef struct {
   int pcg_maxit;
   int ntr;                    /* conservative tracers (ecomod = 0)             */
   const double *tracer_profiles; /* ntr*km1: profile of tracer nn at nn*km1+k-1 */
+  int nopen;                  /* open boundaries (types 1 and 2)              */
+  double dtsec_openbc;        /* seconds between boundary records             */
+  const int *openbc;          /* nopen*6: iside itype isbc jsbc iebc jebc     */
+  int openbc_npts;            /* records per boundary                         */
+  const double *openbc_values;/* nopen*npts*(ntr+2): boundary nn, record r,
+                                 value q at (nn*npts + r)*(ntr+2) + q; q = 0
+                                 zeta (m) or flow (m3/s), 1 active scalar,
+                                 2.. tracers (openbcNN.txt)                   */
 } OrcDeck;
 
 typedef struct OrcLake OrcLake;

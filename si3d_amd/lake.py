@@ -15,6 +15,9 @@ The generators build the synthetic lakes used for measurement:
   random field, geometrically stretched layers, tanh thermocline, a
   sinusoidal wind series, TVD transport and the two-equation turbulence
   closure on.  1024x512 cells with 60 layers is the single-GPU workload.
+* channel    -- rectangular reach with a cross-channel depth profile, for
+  open boundaries (`open_boundary` adds one: the nopen block of si3d_inp.txt
+  and its openbcNN.txt series).
 """
 from __future__ import annotations
 
@@ -218,3 +221,83 @@ def n_wet_cells(deck) -> int:
     bottoms = z[2:]
     kz = np.searchsorted(bottoms, h[wet], side="left") + 1
     return int(kz.sum())
+
+
+SIDES = {"W": 1, "N": 2, "E": 3, "S": 4}
+
+
+def channel(nx=30, ny=12, dx=50.0, depth=8.0, shallow=4.0, dz=1.0, dt=30.0, nts=200, iturb=0,
+            isal=1, temperature=20.0, ntr=0):
+    """Rectangular reach, deepest along its axis (x) and shoaling to `shallow`
+    at the banks, no wind, no Coriolis, uniform temperature; `ntr` tracers
+    start at zero."""
+    jj = np.arange(ny) - 0.5 * (ny - 1)
+    prof = depth - (depth - shallow) * (jj / (0.5 * ny)) ** 2
+    bathy = np.repeat(np.round(prof * 10.0).astype(np.int32)[None, :], nx, axis=0)
+    d = _deck(
+        name=f"channel_{nx}x{ny}", imm=nx, jmm=ny, dx=dx, dy=dx, ddz=dz, zl=depth, dt=dt, nts=nts,
+        bathy_dm=bathy, iadv=1, itrmom=1, ihd=1, Ax0=1.0, Ay0=1.0, f=0.0, ibc=1 if isal else 0, isal=isal,
+        iturb=iturb, Av0=1e-3, Dv0=1e-4, ifsurfbc=0, wa=0.0,
+    )
+    d["init_profile"] = np.full(km1_of(d), temperature)
+    if ntr:
+        d["tracer_profiles"] = np.zeros((ntr, km1_of(d)))
+    return d
+
+
+def open_boundary(deck, side, btype, first, last, series, dtsec=600.0):
+    """Deck with one more open boundary (si3d_utils.f90:123-144,
+    si3d_boundaryconditions.f90:23-897): on the outer interior column (W at
+    i = 2, E at i = imm+1) or row (S at j = 2, N at j = jmm+1) of the grid,
+    cells first..last along it (grid indices as in si3d_inp.txt).  btype 1
+    prescribes the water surface elevation (m), 2 the flow (m3/s, positive
+    toward +x / +y).  series: records x (2 + ntr) values (elevation or flow,
+    active scalar, tracers) dtsec seconds apart, as in openbcNN.txt."""
+    sd = SIDES[side] if isinstance(side, str) else int(side)
+    series = np.atleast_2d(np.asarray(series, dtype=np.float64))
+    if sd in (1, 3):
+        i = 2 if sd == 1 else deck["imm"] + 1
+        spec = (sd, btype, i, first, i, last)
+    else:
+        j = 2 if sd == 4 else deck["jmm"] + 1
+        spec = (sd, btype, first, j, last, j)
+    d = dict(deck)
+    ob = dict(d.get("openbc") or dict(dtsec=dtsec, iside=[], itype=[], isbc=[], jsbc=[], iebc=[], jebc=[],
+                                      values=np.zeros((0,) + series.shape)))
+    if ob["values"].shape[1:] != series.shape or ob["dtsec"] != dtsec:
+        if ob["values"].shape[0]:
+            raise ValueError("open boundaries share one record count, value count and dtsecopenbc")
+    for key, v in zip(["iside", "itype", "isbc", "jsbc", "iebc", "jebc"], spec):
+        ob[key] = list(ob[key]) + [v]
+    ob["values"] = np.concatenate([ob["values"].reshape((-1,) + series.shape), series[None]], axis=0)
+    ob["dtsec"] = dtsec
+    d["openbc"] = ob
+    return d
+
+
+def steady_series(value, scalar=0.0, tracers=(), npts=200):
+    """Boundary series holding every value constant."""
+    row = [value, scalar] + list(tracers)
+    return np.tile(np.asarray(row, dtype=np.float64), (npts, 1))
+
+
+def mirror_x(deck):
+    """The same lake reflected in x (i -> imm+3-i)."""
+    d = dict(deck)
+    d["bathy_dm"] = np.ascontiguousarray(deck["bathy_dm"][::-1, :])
+    return d
+
+
+def mirror_y(deck):
+    """The same lake reflected in y (j -> jmm+3-j)."""
+    d = dict(deck)
+    d["bathy_dm"] = np.ascontiguousarray(deck["bathy_dm"][:, ::-1])
+    return d
+
+
+def transpose_xy(deck):
+    """The same lake with x and y exchanged (needs dx = dy)."""
+    d = dict(deck)
+    d["bathy_dm"] = np.ascontiguousarray(deck["bathy_dm"].T)
+    d["imm"], d["jmm"] = deck["jmm"], deck["imm"]
+    return d
