@@ -154,6 +154,32 @@ int s3g_get_tiles(s3g_ctx *c, int max, int *kc);
 void s3g_finalize(s3g_ctx *c);
 const char *s3g_last_error(void);
 
+/* --- Open boundaries ------------------------------------------------------ */
+/* Replaces openbc0 (si3d_boundaryconditions.f90:23-897) for boundary types 1
+ * (water surface elevation) and 2 (free-surface flow); the library then runs
+ * openbcUVH, MODexmom4openBCX/Y, MODqqddrr4openBC, MODvel4openBC,
+ * MODexsal4openbc / MODexTracer4openbc and openbcSCA / openbcTracer inside
+ * s3g_fd and shifts the boundary time levels in s3g_settrap / s3g_save.
+ * Call once after the initial state is set and before the first s3g_fd.
+ * The fields are the nopen block of si3d_inp.txt (si3d_utils.f90:123-144);
+ * values holds the openbcNN.txt records: boundary nn, record r, value q at
+ * (nn*npts + r)*nvar + q with q = 0 the elevation (m) or flow (m3/s,
+ * positive toward +x / +y), 1 the active scalar, 2.. the tracers (nvar =
+ * ntr + 2).  Types 3 (submerged flow) and 4 (nested grids) return
+ * S3G_ERR_UNSUPPORTED; a cell that is not a wet edge cell of its side, or a
+ * dry cell on an elevation boundary, S3G_ERR_ARG (the reference STOPs). */
+typedef struct {
+  int nopen;
+  double dtsec;              /* dtsecOpenBC: seconds between records       */
+  double dzmin;              /* dzmin of si3d_inp.txt                      */
+  const int *iside;          /* [nopen] 1 W, 2 N, 3 E, 4 S                  */
+  const int *itype;          /* [nopen] 1 elevation, 2 flow                 */
+  const int *isbc, *jsbc, *iebc, *jebc; /* [nopen] first / last cell (i, j) */
+  int npts, nvar;            /* records per boundary, values per record     */
+  const double *values;      /* [nopen*npts*nvar]                           */
+} s3g_openbc;
+int s3g_set_open_boundaries(s3g_ctx *c, const s3g_openbc *bc);
+
 /* --- Learned initial guess of the free-surface solve ---------------------- */
 /* No counterpart in the reference (its solve is NSPCG, si3d_procedures.f90:
  * 1870-2065).  A U-Net maps the normalised residual of the plain initial

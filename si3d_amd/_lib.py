@@ -57,6 +57,12 @@ class S3gDeck(C.Structure):
                 ("bathy_dm", ip), ("init_profile", dp), ("tracer_profiles", dp)]
 
 
+class S3gOpenBc(C.Structure):
+    _fields_ = [("nopen", C.c_int), ("dtsec", C.c_double), ("dzmin", C.c_double),
+                ("iside", ip), ("itype", ip), ("isbc", ip), ("jsbc", ip), ("iebc", ip), ("jebc", ip),
+                ("npts", C.c_int), ("nvar", C.c_int), ("values", dp)]
+
+
 class S3gStats(C.Structure):
     _fields_ = [("steps", C.c_long), ("fd_calls", C.c_long), ("solves", C.c_long),
                 ("pcg_iters_total", C.c_long), ("pcg_iters_last", C.c_int),
@@ -111,6 +117,7 @@ def load():
     L.s3g_unet_check.argtypes = [C.c_int, C.c_int, fp, fp, fp, fp, fp, fp, dp]
     L.s3g_create_ensemble.argtypes = [C.c_int, C.POINTER(S3gDeck), C.POINTER(S3gParams), C.POINTER(vp)]
     L.s3g_ensemble_info.argtypes = [vp, C.c_int, ip, ip]
+    L.s3g_set_open_boundaries.argtypes = [vp, C.POINTER(S3gOpenBc)]
     L.s3g_finalize.argtypes = [vp]
     L.s3g_finalize.restype = None
     L.s3g_last_error.restype = C.c_char_p

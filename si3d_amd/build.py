@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT, "libsi3dgpu.so")
-SOURCES = ["s3g_kernels.hip", "s3g_pcg.hip", "s3g_host.hip", "s3g_unet.hip", "s3g_deck.hip"]
+SOURCES = ["s3g_kernels.hip", "s3g_pcg.hip", "s3g_host.hip", "s3g_unet.hip", "s3g_deck.hip", "s3g_openbc.hip"]
 ARCH = os.environ.get("SI3D_OFFLOAD_ARCH", "gfx950")
 
 

@@ -123,6 +123,38 @@ struct Timer {
 }  // namespace s3g
 
 namespace s3g {
+// Open boundaries (types 1 and 2).  The boundary transports and thicknesses
+// keep the reference's storage (si3d_utils.f90:441-455): one row of km1+2
+// layers per j for the west and east sides and per i for the north and south
+// sides, at time levels n+1, n and n-1: array (side*3 + level) of bq / bh, row
+// a at a*KB.
+constexpr int kMaxOpen = 20;  // the nopen fields of si3d_inp.txt hold 20 values
+struct OpenBc {
+  int n = 0;
+  std::vector<int> side, type;        // side 0 W, 1 N, 2 E, 3 S
+  std::vector<int> first, count;      // cells of boundary nn: cells[first .. first+count)
+  int np = 0, nv = 0;                 // records, values per record (2 + ntr)
+  double dtsec = 0.0, dzmin = 0.0;
+  std::vector<double> vals;           // [n][nv][np+1], 1-based series
+  int KB = 0, NA = 0;                 // row length, rows per array
+  int *cells = nullptr, *along = nullptr;  // column and row index of each boundary cell
+  double *bq = nullptr, *bh = nullptr;     // [12][NA*KB]
+  bool any_flow = false;
+};
+
+// Everything an open-boundary kernel needs, by value; val and sc hold the
+// boundary values of one call (elevation or flow, and the scalar).
+struct ObcArgs {
+  int n, KB, SZ;
+  int side[kMaxOpen], type[kMaxOpen], first[kMaxOpen], count[kMaxOpen];
+  double val[kMaxOpen], sc[kMaxOpen];
+  const int *cells, *along;
+  double *bq, *bh;
+  double dzmin;
+};
+}  // namespace s3g
+
+namespace s3g {
 // Per-cell kernels with a layer-chunk size; the first TILE_TUNED are autotuned.
 enum TileId {
   TILE_EXMOM_X, TILE_EXMOM_Y, TILE_SLAYER_H, TILE_VEL, TILE_HFLUX, TILE_VEL2, TILE_RHS, TILE_TURB, TILE_TUNED,
@@ -193,6 +225,8 @@ struct s3g_ctx {
   std::vector<LakeForcing> lake_forcing;
   std::vector<double> lake_force;  // 3 per lake: uair, vair, cdw
   double *d_force = nullptr;
+  // open boundaries (s3g_set_open_boundaries)
+  s3g::OpenBc obc;
   // learned initial guess (s3g_learn_setup)
   s3g::Learner *learn = nullptr;
   std::vector<void *> learn_allocs;
@@ -209,6 +243,15 @@ int init_density(s3g_ctx *c);
 template <typename R> int autotune(s3g_ctx *c);
 // pcg.hip
 template <typename R> int solve_free_surface(s3g_ctx *c, double tz);
+// openbc.hip: the open-boundary steps of fd / settrap / save (no-ops without
+// open boundaries); q selects the scalar series (1 active scalar, 2+ tracers)
+template <typename R> void obc_uvh(s3g_ctx *c);
+template <typename R> void obc_exmom(s3g_ctx *c, bool x, double twodt1);
+template <typename R> void obc_qq(s3g_ctx *c, double tz);
+template <typename R> void obc_vel(s3g_ctx *c);
+template <typename R> void obc_exscalar(s3g_ctx *c, const R *cpp, int q);
+template <typename R> void obc_scalar(s3g_ctx *c, R *cn, int q);
+void obc_shift(s3g_ctx *c, int mode);
 // unet.hip
 int learn_before_solve(s3g_ctx *c, const void *outer, double *x);
 int learn_after_solve(s3g_ctx *c, const void *outer);

@@ -1602,20 +1602,24 @@ static Scal<R> sal_of(Dev<R> &D) {
 // Explicit + implicit transport of one scalar (exsal/imsal or exTracer/
 // ImTracer): face fluxes, then the right-hand side per cell + column solve,
 // or both fused in the column sweep (autotuned).
+// With open flow boundaries the boundary fluxes join Bex between the two
+// (q: the boundary series of this scalar, 1 active scalar, 2.. tracers).
 template <typename R, bool TR>
-static void transport(s3g_ctx *c, Scal<R> S, double twodt1) {
+static void transport(s3g_ctx *c, Scal<R> S, double twodt1, int q) {
   Dev<R> &D = dev<R>(c);
   const int L = c->lm;
   const bool tr = TR;
   LAUNCHC(tr ? "tracer_hflux" : "exsal_hflux", (k_exsal_hflux<R, false>), (k_exsal_hflux<R, true>), TILE_HFLUX, L,
           c->km - 1, D, S, twodt1);
-  if (c->exsal_split) {
+  if (c->exsal_split || c->obc.any_flow) {
     LAUNCHC(tr ? "tracer_rhs" : "exsal_rhs", (k_exsal_rhs<R, false, TR>), (k_exsal_rhs<R, true, TR>), TILE_RHS, L,
             c->km - 1, D, S, twodt1);
+    obc_exscalar<R>(c, S.cpp, q);
     LAUNCH(tr ? "imtracer" : "imsal", (k_exsal_imsal<R, false, true, TR>), L, D, S, twodt1);
   } else {
     LAUNCH(tr ? "tracer_imsal" : "exsal_imsal", (k_exsal_imsal<R, true, true, TR>), L, D, S, twodt1);
   }
+  obc_scalar<R>(c, S.cn, q);
 }
 
 template <typename R>
@@ -1645,6 +1649,7 @@ template <typename R>
 int phase(s3g_ctx *c, int ph, int n, int istep) {
   const double tz = 1.0 / istep, twodt1 = c->P.twodt * tz;
   if (ph == 0) {
+    obc_uvh<R>(c);
     surface_forcing<R>(c, n, istep);
     return 0;
   }
@@ -1655,24 +1660,40 @@ int phase(s3g_ctx *c, int ph, int n, int istep) {
     case 1:
       if (c->P.ihd > 1) LAUNCH("hmix", (k_hmix<R>), L, D);
       break;
-    case 2: LAUNCHC("exmom_x", (k_exmom1<R, true>), (k_exmom<R, true, true>), TILE_EXMOM_X, L, c->km - 1, D, twodt1); break;
+    case 2:
+      LAUNCHC("exmom_x", (k_exmom1<R, true>), (k_exmom<R, true, true>), TILE_EXMOM_X, L, c->km - 1, D, twodt1);
+      obc_exmom<R>(c, true, twodt1);
+      break;
     case 3: LAUNCH("matmom_x", (k_matmom<R, true>), L, D, tz); break;
-    case 4: LAUNCHC("exmom_y", (k_exmom1<R, false>), (k_exmom<R, false, true>), TILE_EXMOM_Y, L, c->km - 1, D, twodt1); break;
+    case 4:
+      LAUNCHC("exmom_y", (k_exmom1<R, false>), (k_exmom<R, false, true>), TILE_EXMOM_Y, L, c->km - 1, D, twodt1);
+      obc_exmom<R>(c, false, twodt1);
+      break;
     case 5: LAUNCH("matmom_y", (k_matmom<R, false>), L, D, tz); break;
-    case 6: LAUNCH("matcon", (k_matcon<R>), L, D, tz); break;
+    case 6:
+      LAUNCH("matcon", (k_matcon<R>), L, D, tz);
+      obc_qq<R>(c, tz);
+      break;
     case 7: {
       int rc = solve_free_surface<R>(c, tz);
       LAUNCHC("slayer_h", (k_slayer_h<R, false>), (k_slayer_h<R, true>), TILE_SLAYER_H, L, c->km1, D);
       LAUNCH("slayer_h_sentinel", (k_slayer_h_sentinel<R>), c->km1 + 2, D);
+      obc_uvh<R>(c);
       return rc;
     }
     case 8:
       LAUNCHC("vel", (k_vel<R, false>), (k_vel<R, true>), TILE_VEL, L, c->km - 1, D, tz);
       LAUNCH("continuity1", (k_continuity<R, 1>), L, D);
+      obc_vel<R>(c);
       break;
     case 9: LAUNCHC("exsal_hflux", (k_exsal_hflux<R, false>), (k_exsal_hflux<R, true>), TILE_HFLUX, L, c->km - 1, D, sal_of(D), twodt1);
-            LAUNCH("exsal_div", (k_exsal_imsal<R, true, false, false>), L, D, sal_of(D), twodt1); break;
-    case 10: LAUNCH("imsal", (k_exsal_imsal<R, false, true, false>), L, D, sal_of(D), twodt1); break;
+            LAUNCH("exsal_div", (k_exsal_imsal<R, true, false, false>), L, D, sal_of(D), twodt1);
+            obc_exscalar<R>(c, D.salpp, 1);
+            break;
+    case 10:
+      LAUNCH("imsal", (k_exsal_imsal<R, false, true, false>), L, D, sal_of(D), twodt1);
+      obc_scalar<R>(c, D.sal, 1);
+      break;
     case 11: LAUNCHC("vel2", (k_vel2<R, false>), (k_vel2<R, true>), TILE_VEL2, L, c->km - 1, D); break;
     case 12:
       LAUNCH("smooth_zeta", (k_zeta_levels<R, 2>), L + 1, D);
@@ -1839,30 +1860,37 @@ int fd(s3g_ctx *c, int n, int istep, int lastiter) {
     }
   }
   const double tz = 1.0 / istep, twodt1 = c->P.twodt * tz;
+  obc_uvh<R>(c);  // openbcUVH (423)
   surface_forcing<R>(c, n, istep);
   sync_forcing<R>(c);
   Dev<R> &D = dev<R>(c);
   const int L = c->lm;
   if (c->P.ihd > 1) LAUNCH("hmix", (k_hmix<R>), L, D);
   LAUNCHC("exmom_x", (k_exmom1<R, true>), (k_exmom<R, true, true>), TILE_EXMOM_X, L, c->km - 1, D, twodt1);
+  obc_exmom<R>(c, true, twodt1);
   LAUNCH("matmom_x", (k_matmom<R, true>), L, D, tz);
   LAUNCHC("exmom_y", (k_exmom1<R, false>), (k_exmom<R, false, true>), TILE_EXMOM_Y, L, c->km - 1, D, twodt1);
+  obc_exmom<R>(c, false, twodt1);
   LAUNCH("matmom_y", (k_matmom<R, false>), L, D, tz);
   LAUNCH("matcon", (k_matcon<R>), L, D, tz);
+  obc_qq<R>(c, tz);
   int rc = solve_free_surface<R>(c, tz);
   if (rc < 0) return rc;
   LAUNCHC("slayer_h", (k_slayer_h<R, false>), (k_slayer_h<R, true>), TILE_SLAYER_H, L, c->km1, D);
   LAUNCH("slayer_h_sentinel", (k_slayer_h_sentinel<R>), c->km1 + 2, D);
+  obc_uvh<R>(c);  // openbcUVH after the solve (482)
   LAUNCHC("vel", (k_vel<R, false>), (k_vel<R, true>), TILE_VEL, L, c->km - 1, D, tz);
   LAUNCH("continuity1", (k_continuity<R, 1>), L, D);
+  obc_vel<R>(c);
   if (c->P.isal != 0) {
-    transport<R, false>(c, sal_of(D), twodt1);
+    transport<R, false>(c, sal_of(D), twodt1, 1);
   }
   // non-active scalars on the last iteration only (521-547); sources and
   // sinks are zero (srcsnk00)
   if (c->ntr > 0 && c->P.niter > 0 && lastiter == 1) {
     const size_t n3 = (size_t)c->KP * c->LP;
-    for (int nt = 0; nt < c->ntr; ++nt) transport<R, true>(c, Scal<R>{D.trpp + nt * n3, D.tr + nt * n3, nullptr}, twodt1);
+    for (int nt = 0; nt < c->ntr; ++nt)
+      transport<R, true>(c, Scal<R>{D.trpp + nt * n3, D.tr + nt * n3, nullptr}, twodt1, 2 + nt);
   }
   LAUNCHC("vel2", (k_vel2<R, false>), (k_vel2<R, true>), TILE_VEL2, L, c->km - 1, D);
   if (c->P.ismooth >= 1 && istep == 1) {
@@ -1882,6 +1910,8 @@ int settrap(s3g_ctx *c) {
   LAUNCHC("settrap_cell", (k_settrap_cell<R, 0, false>), (k_settrap_cell<R, 0, true>), TILE_SETTRAP, L + 1, c->km1 + 2, D);
   LAUNCH("settrap_top", (k_settrap_top<R>), L, D);
   LAUNCH("continuity2", (k_continuity<R, 2>), L, D);
+  obc_vel<R>(c);
+  obc_shift(c, 0);
   return hip_check(hipGetLastError(), "settrap launch");
 }
 
@@ -1893,6 +1923,7 @@ int settrap2(s3g_ctx *c) {
   LAUNCHC("settrap2_cell", (k_settrap_cell<R, 1, false>), (k_settrap_cell<R, 1, true>), TILE_SETTRAP, L, c->km1 + 2, D);
   LAUNCH("settrap_top", (k_settrap_top<R>), L, D);
   LAUNCH("continuity2", (k_continuity<R, 2>), L, D);
+  obc_vel<R>(c);
   return hip_check(hipGetLastError(), "settrap2 launch");
 }
 
@@ -1908,6 +1939,8 @@ int save(s3g_ctx *c, int istep) {
     LAUNCH("save_top", (k_save_top<R, 2>), L + 1, D);
   }
   LAUNCH("continuity2", (k_continuity<R, 2>), L, D);
+  obc_vel<R>(c);
+  obc_shift(c, istep == 1 ? 1 : 2);
   if (c->ntr > 0)  // tracerpp = tracer (3772-3774, 3927-3929)
     hipMemcpyAsync(D.trpp, D.tr, (size_t)c->ntr * c->KP * c->LP * sizeof(R), hipMemcpyDeviceToDevice, c->stream);
   return hip_check(hipGetLastError(), "save launch");

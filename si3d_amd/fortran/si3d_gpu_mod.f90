@@ -38,6 +38,15 @@ MODULE si3d_gpu
       INTEGER(C_INT) :: ntr
    END TYPE s3g_params
 
+   ! nopen block of si3d_inp.txt and the openbcNN.txt records (openbc0)
+   TYPE, BIND(C) :: s3g_openbc
+      INTEGER(C_INT) :: nopen
+      REAL(C_DOUBLE) :: dtsec, dzmin
+      TYPE(C_PTR)    :: iside, itype, isbc, jsbc, iebc, jebc
+      INTEGER(C_INT) :: npts, nvar
+      TYPE(C_PTR)    :: values
+   END TYPE s3g_openbc
+
    INTERFACE
       FUNCTION s3g_init(grid, params, ctx) BIND(C, NAME="s3g_init")
          IMPORT :: C_INT, C_PTR, s3g_grid, s3g_params
@@ -102,6 +111,13 @@ MODULE si3d_gpu
          REAL(C_DOUBLE), VALUE :: thrs
          INTEGER(C_INT) :: s3g_fd
       END FUNCTION s3g_fd
+
+      FUNCTION s3g_set_open_boundaries(ctx, bc) BIND(C, NAME="s3g_set_open_boundaries")
+         IMPORT :: C_INT, C_PTR, s3g_openbc
+         TYPE(C_PTR), VALUE :: ctx
+         TYPE(s3g_openbc), INTENT(IN) :: bc
+         INTEGER(C_INT) :: s3g_set_open_boundaries
+      END FUNCTION s3g_set_open_boundaries
 
       FUNCTION s3g_settrap(ctx) BIND(C, NAME="s3g_settrap")
          IMPORT :: C_INT, C_PTR

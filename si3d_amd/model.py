@@ -87,7 +87,26 @@ class GpuLake:
             self.h = h
             st = self.stats()
             self.km1, self.lm = st["km1"], st["lm"]
+            if deck.get("openbc") is not None:
+                self.set_open_boundaries(deck["openbc"], deck.get("dzmin", 0.1))
         self.lm1 = self.lm + 1
+
+    def set_open_boundaries(self, ob, dzmin):
+        """openbc0: the nopen block of si3d_inp.txt (`ob` as built by
+        lake.open_boundary or read by inputs.read_case) and its series."""
+        vals = np.ascontiguousarray(ob["values"], dtype=np.float64)  # (nopen, npts, 2+ntr)
+        b = _lib.S3gOpenBc()
+        b.nopen = vals.shape[0]
+        b.dtsec = float(ob["dtsec"])
+        b.dzmin = float(dzmin)
+        for key in ("iside", "itype", "isbc", "jsbc", "iebc", "jebc"):
+            a = np.ascontiguousarray(ob[key], dtype=np.int32)
+            self._keep.append(a)
+            setattr(b, key, a.ctypes.data_as(_lib.ip))
+        b.npts, b.nvar = vals.shape[1], vals.shape[2]
+        self._keep.append(vals)
+        b.values = vals.ctypes.data_as(_lib.dp)
+        check(self.L.s3g_set_open_boundaries(self.h, C.byref(b)), "s3g_set_open_boundaries")
 
     @classmethod
     def from_state(cls, grid, deck, fields, precision="fp64", preconditioner="jacobi", tol=None):
