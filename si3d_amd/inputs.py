@@ -330,16 +330,40 @@ def read_surfbc(path, nvar):
 _SURFBC_NVAR = {1: 6, 2: 9, 3: 9, 20: 3}
 
 
+def read_openbc(path, nvar):
+    """openbcNN.txt (si3d_boundaryconditions.f90:55-86): five header records,
+    the record count in (10X,I7), then records of nvar = ntr + 2 values in
+    (10X, nvar G11.2): elevation or flow, active scalar, tracers."""
+    F = _File(path)
+    F.skip(5)
+    r = F.record()
+    r.skip(10)
+    npts = r.int(7)
+    out = np.zeros((npts, nvar))
+    for j in range(npts):
+        r = F.record()
+        r.skip(10)
+        for q in range(nvar):
+            out[j, q] = r.real(11)
+    return out
+
+
+def openbc_name(nn):
+    """File of open boundary nn (1-based): openbc01.txt .. openbc99.txt."""
+    return f"openbc{nn:02d}.txt"
+
+
 def read_case(directory, bathy_file="si3d_bathy", missing_icw=False):
     """A deck from a pSi3D case directory (si3d_inp.txt, the bathymetry, and
     si3d_layer.txt / si3d_init.txt / the surface series as the options ask).
-    Raises InputError for options this path does not run (open boundaries,
-    point sources, water-quality or tracer-cloud models, met-station
-    interpolation, GOTM)."""
+    Open boundaries of types 1 and 2 come with their openbcNN.txt series.
+    Raises InputError for options this path does not run (submerged-flow and
+    nested-grid boundaries, point sources, water-quality or tracer-cloud
+    models, met-station interpolation, GOTM)."""
     p = read_inp(os.path.join(directory, "si3d_inp.txt"), missing_icw=missing_icw)
     why = []
-    if p["nopen"] > 0:
-        why.append("open boundaries (nopen > 0)")
+    if p["nopen"] > 0 and any(t not in (1, 2) for t in p["itype"]):
+        why.append("open boundaries of type 3 or 4 (submerged flow, nested grids)")
     if p["iopss"] > 0:
         why.append("point sources and sinks (iopss > 0)")
     if p["ntr"] > 0 and p["ecomod"] != 0:
@@ -370,6 +394,13 @@ def read_case(directory, bathy_file="si3d_bathy", missing_icw=False):
     if p["ifsurfbc"] in _SURFBC_NVAR:
         name = "si3d_surfbc_W.txt" if p["ifsurfbc"] == 20 else "si3d_surfbc.txt"
         d["surfbc"] = read_surfbc(os.path.join(directory, name), _SURFBC_NVAR[p["ifsurfbc"]])
+    if p["nopen"] > 0:
+        series = [read_openbc(os.path.join(directory, openbc_name(nn + 1)), p["ntr"] + 2)
+                  for nn in range(p["nopen"])]
+        if len({a.shape[0] for a in series}) != 1:  # varsOpenBC is sized by the first file (74-77)
+            raise InputError("openbcNN.txt files must hold the same number of records")
+        d["openbc"] = dict(dtsec=p["dtsecopenbc"], values=np.stack(series),
+                           **{k: list(p[k]) for k in ("iside", "itype", "isbc", "jsbc", "iebc", "jebc")})
     d["inp"] = p
     return d
 
@@ -393,7 +424,8 @@ def _section(title):
 
 def write_inp(deck, path, title="si3d case written by si3d_amd", ntr=0, start=(2000, 1, 1, 0)):
     """si3d_inp.txt in the record layout `input` reads (si3d_utils.f90:21-315):
-    output sections off, no open boundaries, nesting output, point sources."""
+    output sections, nesting output and point sources off; the open
+    boundaries of deck["openbc"], if any."""
     imm, jmm = int(deck["imm"]), int(deck["jmm"])
     xl, yl = imm * deck["dx"], jmm * deck["dy"]  # im = nint(xl/dx) + 1 = imm + 1
     L = ["*******************************************************\n", f"{title}\n",
@@ -421,8 +453,14 @@ def write_inp(deck, path, title="si3d case written by si3d_amd", ntr=0, start=(2
     L += _section("Output specifications for 3D space files")
     L += [_line("ipxml", "0"), _line("itspf", "0"), _line("iTurbVars", "0")]
     L += _section("Open boundary conditions specifications")
-    L += [_line("nopen", "0")] + [f" {k:<12}!\n" for k in ("dtsecopenbc", "iside", "itype", "isbc", "jsbc",
-                                                            "iebc", "jebc")]
+    ob = deck.get("openbc")
+    if ob is not None and len(ob["iside"]) > 0:
+        L += [_line("nopen", str(len(ob["iside"]))), _line("dtsecopenbc", _fmt_real(ob["dtsec"]))]
+        L += [_line(k, ",".join(str(int(v)) for v in ob[k])) for k in ("iside", "itype", "isbc", "jsbc", "iebc",
+                                                                       "jebc")]
+    else:
+        L += [_line("nopen", "0")] + [f" {k:<12}!\n" for k in ("dtsecopenbc", "iside", "itype", "isbc", "jsbc",
+                                                                "iebc", "jebc")]
     L += _section("Open boundary conditions for nesting procedures")
     L += [_line("nxBDO", "0")] + [f" {k:<12}!\n" for k in ("iob", "xxb", "iside", "isbc", "jsbc", "iebc",
                                                             "jebc")]
@@ -485,6 +523,16 @@ def write_surfbc(series, dtsurfbc, path):
         f.writelines(L)
 
 
+def write_openbc(series, dtsec, path):
+    s = np.asarray(series, dtype=np.float64)
+    L = ["Open boundary condition file for si3d model\n", "written by si3d_amd\n", "-\n", "-\n", "-\n",
+         f"   npts = {s.shape[0]:7d}\n"]
+    for j in range(s.shape[0]):
+        L.append(f"{j * dtsec / 3600.0:10.4f}" + "".join(f"{v:11.4e}" for v in s[j]) + "\n")
+    with open(path, "w") as f:
+        f.writelines(L)
+
+
 def write_case(deck, directory):
     """The deck as a pSi3D case directory (si3d_inp.txt, si3d_bathy, and
     si3d_layer.txt / si3d_init.txt / the surface series as needed), readable
@@ -499,3 +547,6 @@ def write_case(deck, directory):
     if deck.get("surfbc") is not None:
         name = "si3d_surfbc_W.txt" if deck["ifsurfbc"] == 20 else "si3d_surfbc.txt"
         write_surfbc(deck["surfbc"], deck["dtsurfbc"], os.path.join(directory, name))
+    if deck.get("openbc") is not None:
+        for nn, series in enumerate(np.asarray(deck["openbc"]["values"])):
+            write_openbc(series, deck["openbc"]["dtsec"], os.path.join(directory, openbc_name(nn + 1)))

@@ -80,12 +80,25 @@ def test_clear_lake_case_is_read_and_refused():
         I.read_case(os.path.join(G, "CLWaterQuality"))
 
 
-@pytest.mark.parametrize("which", ["ellipse", "rectangle", "heat3"])
+def _river():
+    """Channel with a tidal water level at the west end, a river leaving
+    through part of the east end and a tributary entering from the south."""
+    d = lake.channel(nx=30, ny=12, ntr=1)
+    t = np.arange(50) * 600.0
+    d = lake.open_boundary(d, "W", 1, 2, 13, np.stack([0.02 + 0.05 * np.sin(t / 1146.0), np.full(50, 18.0),
+                                                         np.full(50, 1.0)], axis=1))
+    d = lake.open_boundary(d, "E", 2, 4, 11, lake.steady_series(5.0, 20.0, [0.0], npts=50))
+    return lake.open_boundary(d, "S", 2, 3, 8, lake.steady_series(4.0, 16.5, [0.5], npts=50))
+
+
+@pytest.mark.parametrize("which", ["ellipse", "rectangle", "heat3", "river"])
 def test_write_read_round_trip(tmp_path, which):
     if which == "ellipse":
         d = lake.ellipse(nx=40, ny=24, dx=100.0, nlayers=12, nts=20, top=1.0, bottom=12.0, depth=40.0, seed=3)
     elif which == "rectangle":
         d = lake.rectangle(nx=100, ny=50, dx=50.0, depth=20.0, dz=1.0, tau=0.1, dt=30.0, nts=210)
+    elif which == "river":
+        d = _river()
     else:
         d = lake.heat_budget(lake.ellipse(nx=40, ny=24, dx=100.0, nlayers=12, nts=20, top=1.0, bottom=12.0,
                                           depth=40.0, seed=3), 3)
@@ -101,7 +114,37 @@ def test_write_read_round_trip(tmp_path, which):
     np.testing.assert_allclose(e["init_profile"], d["init_profile"], atol=5e-6)  # G11.5
     if d.get("surfbc") is not None:
         np.testing.assert_allclose(e["surfbc"], d["surfbc"], rtol=5e-5, atol=1e-12)  # G11.4
+    if d.get("openbc") is not None:
+        for k in ("iside", "itype", "isbc", "jsbc", "iebc", "jebc", "dtsec"):
+            assert e["openbc"][k] == d["openbc"][k], k
+        np.testing.assert_allclose(e["openbc"]["values"], d["openbc"]["values"], rtol=5e-5, atol=1e-12)
+    else:
+        assert e.get("openbc") is None
     assert lake.km1_of(e) == lake.km1_of(d)
+
+
+def test_river_case_from_files_runs_like_the_deck(oracle_lib, tmp_path):
+    """Open boundaries read back from si3d_inp.txt and openbcNN.txt drive the
+    oracle like the deck they were written from (up to the files' 5 digits)."""
+    d = _river()
+    I.write_case(d, str(tmp_path))
+    assert sorted(f for f in os.listdir(tmp_path) if f.startswith("openbc")) == [
+        "openbc01.txt", "openbc02.txt", "openbc03.txt"]
+    e = I.read_case(str(tmp_path))
+    a, b = oracle_lib.Oracle(d), oracle_lib.Oracle(e)
+    for n in range(1, 6):
+        a.step(n)
+        b.step(n)
+    for name in ("s", "uh", "sal", "tracer1"):
+        x, y = a.get(name), b.get(name)
+        assert np.max(np.abs(x - y)) <= 1e-4 * np.max(np.abs(x)), name
+
+
+def test_open_boundary_types_3_and_4_are_refused(tmp_path):
+    d = lake.open_boundary(lake.channel(), "W", 3, 2, 13, lake.steady_series(5.0, 20.0))
+    I.write_case(d, str(tmp_path))
+    with pytest.raises(I.InputError, match="type 3 or 4"):
+        I.read_case(str(tmp_path))
 
 
 def test_case_from_files_runs_like_the_deck(oracle_lib, tmp_path):
