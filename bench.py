@@ -87,6 +87,9 @@ def parse():
                     help="advance N conservative tracers as well (step profiles)")
     ap.add_argument("--surfbc", type=int, default=20, choices=[1, 2, 3, 20],
                     help="surface forcing: 20 wind series (default), 1-3 heat budget over the same wind")
+    ap.add_argument("--river", type=float, default=0.0, metavar="Q",
+                    help="a channel of the same size with flow Q (m3/s) entering at the west end and leaving at "
+                         "the east end (open boundaries of type 2) instead of the elliptic lake")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (calibrates HBM byte counters) and exit")
     return ap.parse_args()
@@ -176,6 +179,20 @@ def main():
         a.no_cpu_baseline = True
         workload = (f"ensemble of {a.ensemble} elliptic lakes, square sides 128-{a.side_max}, 20-80 layers, "
                     f"dx=100 m, dt=60 s, wind 0.05-0.3 Pa any direction, {a.precision}")
+    elif a.river:
+        deck = lake.channel(nx=a.nx, ny=a.ny, dx=100.0, depth=float(a.layers), shallow=a.layers / 3.0, dz=1.0,
+                            dt=60.0, nts=nts, iturb=1, isal=1, temperature=20.0, ntr=a.tracers)
+        tr = [1.0] * a.tracers
+        deck = lake.open_boundary(deck, "W", 2, 2, a.ny + 1, lake.steady_series(a.river, 12.0, tr))
+        deck = lake.open_boundary(deck, "E", 2, 2, a.ny + 1, lake.steady_series(a.river, 12.0, tr))
+        n_wet = lake.n_wet_cells(deck)
+        g = GpuLake(deck, precision=a.precision, preconditioner=a.precond, tol=a.tol)
+        a.no_learned = True
+        workload = (f"channel {a.nx}x{a.ny}x{a.layers}, dx=100 m, dt=60 s, {a.river:g} m3/s through open "
+                    f"boundaries at both ends (12 C into 20 C), TVD + turbulence, plain PCG ({a.precond}) "
+                    f"tol {a.tol:g}, {a.precision}")
+        if a.tracers:
+            workload += f", {a.tracers} tracers"
     else:
         deck = lake.ellipse(nx=a.nx, ny=a.ny, dx=100.0, nlayers=a.layers, dt=60.0, nts=nts, seed=1)
         if a.surfbc != 20:
@@ -262,7 +279,8 @@ def main():
             bytes_launch = (n_r * elem + n_f64 * 8) * n_wet
             achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
             traffic = None
-            if (not a.ensemble and (a.nx, a.ny, a.layers, a.precision) == (1024, 512, 60, "fp64")
+            if (not a.ensemble and not a.river and not a.tracers and a.surfbc == 20
+                    and (a.nx, a.ny, a.layers, a.precision) == (1024, 512, 60, "fp64")
                     and name in TRAFFIC_FP64_1024x512x60):
                 traffic = int(TRAFFIC_FP64_1024x512x60[name] * n_wet)
             roofline = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1),

@@ -139,6 +139,8 @@ struct OpenBc {
   int KB = 0, NA = 0;                 // row length, rows per array
   int *cells = nullptr, *along = nullptr;  // column and row index of each boundary cell
   double *bq = nullptr, *bh = nullptr;     // [12][NA*KB]
+  std::vector<double> hbc;                 // depth of the deepest cell of each section
+  double *part = nullptr;                  // per-workgroup partial sums of a section's area
   bool any_flow = false;
 };
 
@@ -147,9 +149,9 @@ struct OpenBc {
 struct ObcArgs {
   int n, KB, SZ;
   int side[kMaxOpen], type[kMaxOpen], first[kMaxOpen], count[kMaxOpen];
-  double val[kMaxOpen], sc[kMaxOpen];
+  double val[kMaxOpen], sc[kMaxOpen], hbc[kMaxOpen];
   const int *cells, *along;
-  double *bq, *bh;
+  double *bq, *bh, *part;
   double dzmin;
 };
 }  // namespace s3g
