@@ -127,3 +127,43 @@ def test_c3_2048_volume_and_free_stream():
     kk = np.arange(sal.shape[1])[None, :] + 1
     wet = (kk >= k1z[:, None]) & (kk <= kmz[:, None])
     assert np.max(np.abs(sal[:g.lm][wet] - 12.0)) < 1e-9 * 12.0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_river_1024_volume_and_heat_closed_forms(precision):
+    """1024x512x60 channel, 5000 m3/s entering at the west end at the lake's
+    temperature: after n steps the volume has grown by Q dt (n - 1/2), the
+    temperature is still uniform and the heat content has grown with the
+    volume; then the same flow leaving at the east end holds the volume."""
+    from si3d_amd.model import GpuLake
+    Q, T = 5000.0, 20.0
+    base = lake.channel(nx=1024, ny=512, dx=100.0, depth=60.0, shallow=20.0, dz=1.0, dt=60.0, nts=30, iturb=1,
+                        temperature=T)
+    d = lake.open_boundary(base, "W", 2, 2, 513, lake.steady_series(Q, T))
+    print("case: 1024x512x60 river inflow", precision, flush=True)
+    g = GpuLake(d, precision=precision, preconditioner="mg", tol=1e-12)
+    area = d["dx"] * d["dy"]
+    v0 = g.get("s")[:g.lm].sum() * area
+    h0 = float((g.get("h")[:g.lm] * g.get("sal")[:g.lm]).sum()) * area
+    for n in range(1, 11):
+        g.step(n)
+    dv = g.get("s")[:g.lm].sum() * area - v0
+    expect = Q * d["dt"] * 9.5
+    sal = g.get("sal")[:g.lm]
+    wet = g.get("h")[:g.lm] > 0
+    dh = float((g.get("h")[:g.lm] * sal).sum()) * area - h0
+    print("volume", dv / expect - 1, "uniformity", np.max(np.abs(sal[wet] - T)), "heat", dh / (T * expect) - 1,
+          flush=True)
+    tol = 1e-8 if precision == "fp64" else 1e-5
+    assert abs(dv / expect - 1) < tol
+    assert np.max(np.abs(sal[wet] - T)) < (1e-10 if precision == "fp64" else 1e-4)
+    if precision == "fp64":  # in FP32 storage sum(h T) (~5e12) resolves a 6e7 change to ~1% only
+        assert abs(dh / (T * expect) - 1) < 1e-7
+    del g
+    d = lake.open_boundary(d, "E", 2, 2, 513, lake.steady_series(Q, T))
+    print("case: 1024x512x60 river through", precision, flush=True)
+    g = GpuLake(d, precision=precision, preconditioner="mg", tol=1e-12)
+    v0 = g.get("s")[:g.lm].sum() * area
+    for n in range(1, 11):
+        g.step(n)
+    assert abs(g.get("s")[:g.lm].sum() * area - v0) < tol * Q * d["dt"] * 10
