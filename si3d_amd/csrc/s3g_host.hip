@@ -506,6 +506,7 @@ int s3g::init_ctx(const s3g_grid *g, const s3g_params *p, int nlakes, const doub
       if (got == 3 && cs > 0.0 && cs <= 1.0) c->mg_cs = cs;
     }
     if (const char *gs = std::getenv("S3G_GUESS")) c->pcg_guess = (gs[0] == '1') ? 1 : 0;
+    if (const char *gr = std::getenv("S3G_GRAPH")) c->cg_graph = (gr[0] != '0');
     // S3G_TILES="w0,w1,...": fixed tile words (chunk | order << 8), no tuning
     if (const char *w = std::getenv("S3G_TILES")) {
       c->tune = false;
@@ -640,6 +641,7 @@ int s3g::init_ctx(const s3g_grid *g, const s3g_params *p, int nlakes, const doub
   if (rc == 0) rc = pcg_setup(c, lN.data(), g->l2i, g->l2j);
   if (rc == 0 && !(G.part = dalloc<double>(c, 8 * (size_t)G.nblk + 64))) rc = S3G_ERR_HIP;
   if (rc == 0 && !(G.iters = dalloc<int>(c, 4))) rc = S3G_ERR_HIP;
+  if (rc == 0 && !(G.tol2 = dalloc<double>(c, 1))) rc = S3G_ERR_HIP;
   if (rc != 0) {
     if (g_err.empty()) set_error("s3g_init: device allocation failed");
     s3g_finalize(c);
@@ -777,6 +779,8 @@ int s3g_kernel_times(s3g_ctx *c, int max, char (*names)[32], double *avg_ms, lon
 void s3g_finalize(s3g_ctx *c) {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
+  for (auto &e : c->cg_exec)
+    if (e) hipGraphExecDestroy(e);
   s3g::learn_destroy(c);
   for (void *p : c->allocs) hipFree(p);
   for (auto &e : c->ev_pool) {

@@ -91,6 +91,7 @@ struct Pcg {
   int n, nblk;
   double *r, *r1, *z, *p0, *p1, *q, *diag, *x;  // r / r1: residual of even / odd iterations
   double *part;    // partial sums: [4][nblk]
+  double *tol2 = nullptr;  // squared relative tolerance of the running CG
   double *scal;    // device scalars
   int *iters;      // device iteration counter
   float *rf, *rf1, *zf, *pf0, *pf1, *qf, *sxf, *syf, *rrf, *bf, *df;  // FP32 inner solve
@@ -211,6 +212,12 @@ struct s3g_ctx {
   // initial guess of the leapfrog solve (istep 1): 0 = the latest zeta,
   // 1 = extrapolated from the two latest time levels (S3G_GUESS)
   int pcg_guess = 0;
+  // CG iterations i >= 2 replayed in pairs from a captured graph of two
+  // iterations (per storage type: [0] FP64, [1] FP32), rebuilt when the
+  // kernel arguments change (S3G_GRAPH=0 launches every kernel instead)
+  bool cg_graph = true;
+  hipGraphExec_t cg_exec[2] = {nullptr, nullptr};
+  std::vector<char> cg_key[2];
   bool turb_split = true;  // turbulence closure: per-cell interface terms + sweep
   bool tune = true, tuned = false;
   // ensemble of lakes (s3g_create_ensemble): column offset of each lake

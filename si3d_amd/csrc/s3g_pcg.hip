@@ -27,6 +27,7 @@
 // factors computed once per matrix).
 #include <cmath>
 #include <algorithm>
+#include <cstring>
 #include <cstdio>
 #include <utility>
 #include <vector>
@@ -94,7 +95,8 @@ struct CgArgs {
   const T *b;
   double *part;
   int *iters;
-  double tol2;  // stop when (r,r) <= tol2 * (b,b)
+  double tol2;          // stop when (r,r) <= tol2 * (b,b); the kernels read it from
+  const double *tol2p;  // this device copy, set at the start of every CG run
   int nlines;
   const int *line_start;
   const T *lc, *ld;
@@ -170,7 +172,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_dir(CgArgs<T> A, int i) {
   const int cur = i & 1;
   double v[4];
   reduce_slots<4>(A.part, {P_RR0 + cur, P_BB, P_RZ0 + cur, P_RZ0 + (cur ^ 1)}, A.nblk, lds, v);
-  if (v[0] <= A.tol2 * v[1]) return;  // converged: every block takes the same branch
+  if (v[0] <= *A.tol2p * v[1]) return;  // converged: every block takes the same branch
   const double beta = (i > 0) ? v[2] / v[3] : 0.0;
   const T *pold = cur ? A.p0 : A.p1;
   T *pnew = cur ? A.p1 : A.p0;
@@ -197,7 +199,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(CgArgs<T> A, int i) {
   const int cur = i & 1, nxt = cur ^ 1;
   double v[4];
   reduce_slots<4>(A.part, {P_RR0 + cur, P_BB, P_RZ0 + cur, P_PQ}, A.nblk, lds, v);
-  if (v[0] <= A.tol2 * v[1]) {
+  if (v[0] <= *A.tol2p * v[1]) {
     // Converged: carry this residual partial into the slot iteration i+1 reads,
     // so every later kernel of the chunk also sees convergence and returns.
     if (threadIdx.x == 0) A.part[(P_RR0 + nxt) * A.nblk + blockIdx.x] = A.part[(P_RR0 + cur) * A.nblk + blockIdx.x];
@@ -224,7 +226,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(CgArgs<T> A, int i) {
   if (threadIdx.x == 0) {
     if (JACOBI) A.part[(P_RZ0 + nxt) * A.nblk + blockIdx.x] = rzn;
     A.part[(P_RR0 + nxt) * A.nblk + blockIdx.x] = rrn;
-    if (blockIdx.x == 0) A.iters[0] = i + 1;
+    if (blockIdx.x == 0) A.iters[0] += 1;  // iterations since the run's first residual
   }
 }
 
@@ -358,7 +360,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_update_mg(CgArgs<T> A, MgPair<T> 
   const int cur = i & 1, nxt = cur ^ 1;
   double v[4];
   reduce_slots<4>(A.part, {P_RR0 + cur, P_BB, P_RZ0 + cur, P_PQ}, A.nblk, lds, v);
-  if (v[0] <= A.tol2 * v[1]) {
+  if (v[0] <= *A.tol2p * v[1]) {
     if (threadIdx.x == 0) A.part[(P_RR0 + nxt) * A.nblk + blockIdx.x] = A.part[(P_RR0 + cur) * A.nblk + blockIdx.x];
     return;
   }
@@ -388,7 +390,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_update_mg(CgArgs<T> A, MgPair<T> 
   rrn = block_sum(rrn, lds);
   if (threadIdx.x == 0) {
     A.part[(P_RR0 + nxt) * A.nblk + blockIdx.x] = rrn;
-    if (blockIdx.x == 0) A.iters[0] = i + 1;
+    if (blockIdx.x == 0) A.iters[0] += 1;  // iterations since the run's first residual
   }
 }
 
@@ -743,6 +745,8 @@ static void mg_build(s3g_ctx *c, CgArgs<T> &A) {
   }
 }
 
+__global__ void k_set_scalar(double *p, double v) { *p = v; }
+
 // Runs CG iterations on CgArgs A (x, r, b set up) until (r,r) <= tol2 (b,b) or
 // maxit; returns the number of completed iterations (< 0 on HIP error).
 // `tail`, if given, is enqueued behind every chunk of iterations, before the
@@ -752,6 +756,7 @@ template <typename T, typename Tail>
 static int run_cg(s3g_ctx *c, CgArgs<T> &A, int maxit, int hint, Tail tail) {
   const dim3 g(A.nblk), b(kBlock);
   const bool jac = (c->prec_kind == S3G_PREC_JACOBI) || (c->prec_kind == S3G_PREC_LINE && A.nlines == 0);
+  hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c->stream, c->pcg.tol2, A.tol2);
   const bool mg = c->prec_kind == S3G_PREC_MG;
   if (!jac) {
     hipLaunchKernelGGL((k_cg_residual<T, false>), g, b, 0, c->stream, A);
@@ -768,23 +773,67 @@ static int run_cg(s3g_ctx *c, CgArgs<T> &A, int maxit, int hint, Tail tail) {
     M0.nc = L1.n; M0.ch = L1.ch; M0.parent = L1.parent_of_fine;
     M0.cb = (T *)L1.b;
   }
+  auto iteration = [&](int i) {
+    const T *rnext = ((i + 1) & 1) ? A.r1 : A.r;
+    hipLaunchKernelGGL((k_cg_dir<T>), g, b, 0, c->stream, A, i);
+    if (mg) {
+      hipLaunchKernelGGL((k_cg_update_mg<T>), g, b, 0, c->stream, A, M0, i);
+      apply_prec(c, A, rnext, P_RZ0 + ((i + 1) & 1), true);
+    } else if (!jac) {
+      hipLaunchKernelGGL((k_cg_update<T, false>), g, b, 0, c->stream, A, i);
+      apply_prec(c, A, rnext, P_RZ0 + ((i + 1) & 1), false);
+    } else {
+      hipLaunchKernelGGL((k_cg_update<T, true>), g, b, 0, c->stream, A, i);
+    }
+  };
+  // The kernels depend on the iteration index only through i & 1 and i > 0,
+  // so iterations (2m, 2m+1), m >= 1, are one launch of a graph of
+  // iterations 2 and 3.  The graph is captured on first use and again when
+  // the arguments change (a new tolerance of the mixed solve's inner CG).
+  hipGraphExec_t pair = nullptr;
+  if (c->cg_graph) {
+    const int gi = sizeof(T) == 8 ? 0 : 1;
+    std::vector<char> key(sizeof(A) + sizeof(M0));
+    CgArgs<T> Ak = A;
+    Ak.tol2 = 0.0;  // read from the device copy: not part of the graph
+    std::memcpy(key.data(), &Ak, sizeof(Ak));
+    std::memcpy(key.data() + sizeof(A), &M0, sizeof(M0));
+    if (!c->cg_exec[gi] || key != c->cg_key[gi]) {
+      if (c->cg_exec[gi]) hipGraphExecDestroy(c->cg_exec[gi]);
+      c->cg_exec[gi] = nullptr;
+      hipGraph_t gr = nullptr;
+      bool ok = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+      if (ok) {
+        iteration(2);
+        iteration(3);
+        ok = hipStreamEndCapture(c->stream, &gr) == hipSuccess && gr;
+      }
+      if (ok) ok = hipGraphInstantiate(&c->cg_exec[gi], gr, nullptr, nullptr, 0) == hipSuccess;
+      if (gr) hipGraphDestroy(gr);
+      if (!ok) {  // launch kernel by kernel from now on
+        (void)hipGetLastError();
+        c->cg_exec[gi] = nullptr;
+        c->cg_graph = false;
+        if (c->debug) std::fprintf(stderr, "[s3g] CG graph capture failed; launching iterations one by one\n");
+      }
+      c->cg_key[gi] = key;
+    }
+    pair = c->cg_exec[gi];
+  }
   int it = 0, slot = -1;
   int chunk = std::max(4, std::min(hint + 2, 512));
   while (it < maxit) {
     const int nl = std::min(chunk, maxit - it);
     if (c->timing) timer_begin(c, sizeof(T) == 8 ? "pcg_chunk_f64" : "pcg_chunk_f32", &slot);
-    for (int j = 0; j < nl; ++j) {
+    int j = 0;
+    while (j < nl) {
       const int i = it + j;
-      const T *rnext = ((i + 1) & 1) ? A.r1 : A.r;
-      hipLaunchKernelGGL((k_cg_dir<T>), g, b, 0, c->stream, A, i);
-      if (mg) {
-        hipLaunchKernelGGL((k_cg_update_mg<T>), g, b, 0, c->stream, A, M0, i);
-        apply_prec(c, A, rnext, P_RZ0 + ((i + 1) & 1), true);
-      } else if (!jac) {
-        hipLaunchKernelGGL((k_cg_update<T, false>), g, b, 0, c->stream, A, i);
-        apply_prec(c, A, rnext, P_RZ0 + ((i + 1) & 1), false);
+      if (pair && i >= 2 && (i & 1) == 0 && j + 1 < nl) {
+        hipGraphLaunch(pair, c->stream);
+        j += 2;
       } else {
-        hipLaunchKernelGGL((k_cg_update<T, true>), g, b, 0, c->stream, A, i);
+        iteration(i);
+        j += 1;
       }
     }
     if (c->timing) timer_end(c, slot);
@@ -825,6 +874,7 @@ static CgArgs<T> cg_args(s3g_ctx *c, const int *lE, const int *lW, const int *lN
   A.part = P.part;
   A.iters = P.iters;
   A.tol2 = tol * tol;
+  A.tol2p = P.tol2;
   A.nlines = P.nlines;
   A.line_start = P.line_start;
   A.lc = lc;
