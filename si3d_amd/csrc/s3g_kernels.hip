@@ -1373,7 +1373,9 @@ __global__ void __launch_bounds__(kBlock) k_settrap_cell(Dev<R> D, int nk, int t
   if (!cell_tile(D.lm + (MODE == 0 ? 1 : 0), 0, nk, tile, t)) return;
   const int c = t.c, S = D.lm;
   const double spc = (c < S) ? D.sp[c] : 0.0;
+  const int kbot = (c < S) ? D.kmz[c] + 1 : D.km1 + 2;  // below the sediment cell nothing ever changes
   for (int k = t.k0; k < (CH ? t.k1 : min(t.k1, t.k0 + 1)); ++k) {
+    if (k > kbot) break;
     if (MODE == 0) {
       D.hpp[IX(k, c)] = D.hp[IX(k, c)];
       D.hupp[IX(k, c)] = D.hup[IX(k, c)];
@@ -1439,7 +1441,11 @@ __global__ void __launch_bounds__(kBlock) k_save_cell(Dev<R> D, int nk, int tile
   CellTile t;
   if (!cell_tile(D.lm + 1, 0, nk, tile, t)) return;
   const int c = t.c;
+  // Below a column's sediment cell every array holds its initial value and
+  // the copies are no-ops (the reference still loops whole columns).
+  const int kbot = (c < D.lm) ? D.kmz[c] + 1 : D.km1 + 2;
   for (int k = t.k0; k < (CH ? t.k1 : min(t.k1, t.k0 + 1)); ++k) {
+    if (k > kbot) break;
     if (ISTEP == 1) {
       D.hpp[IX(k, c)] = D.hp[IX(k, c)];
       D.hupp[IX(k, c)] = D.hup[IX(k, c)];
