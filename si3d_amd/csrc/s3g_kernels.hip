@@ -175,19 +175,11 @@ __device__ __forceinline__ double hcoef(const Dev<R> &D, int k, int w) {
 // ever be non-zero on k1 <= k <= kmq (kmq = bottom of the face, fixed), so
 // layers below kmq are never touched (they stay zero from allocation) and the
 // zeroing of the reference (645-656, 821-827) reduces to k1 <= k < k1q.
+// Explicit term of the momentum face of column c at level k1q <= k <= kmq
+// (exmom, 795-950 / 1031-1160).
 template <typename R, bool X>
-__device__ __forceinline__ void exmom_cell(const Dev<R> &D, int c, int k, double twodt1) {
-  const int S = D.lm;
+__device__ __forceinline__ double exmom_value(const Dev<R> &D, int c, int k, int k1q, int kmq, double twodt1) {
   const int lE = D.lE[c], ln = X ? lE : D.lN[c];
-  if (ln == S) return;  // no face: ex stays zero
-  const int kmq = min(D.kmz[ln], D.kmz[c]);
-  if (k > kmq) return;
-  const int k1q = X ? D.k1u[c] : D.k1v[c];
-  R *ex = X ? D.ex : D.ey;
-  if (k < k1q) {
-    ex[IX(k, c)] = R(0);
-    return;
-  }
   const int lW = D.lW[c], lN = D.lN[c], lS = D.lS[c];
   const R *q = X ? D.up : D.vp;
   const R *qpp = X ? D.upp : D.vpp;
@@ -195,10 +187,8 @@ __device__ __forceinline__ void exmom_cell(const Dev<R> &D, int c, int k, double
   const int l2 = X ? D.lS[lE] : D.lW[lN];  // SE corner (x) / NW corner (y)
   const double f = D.P.f;
   {
-    if ((double)D.hp[IX(k, c)] <= 0.0 || (double)D.hp[IX(k, ln)] <= 0.0) {
-      ex[IX(k, c)] = X ? D.uhpp[IX(k, c)] : D.vhpp[IX(k, c)];
-      return;
-    }
+    if ((double)D.hp[IX(k, c)] <= 0.0 || (double)D.hp[IX(k, ln)] <= 0.0)
+      return X ? (double)D.uhpp[IX(k, c)] : (double)D.vhpp[IX(k, c)];
     double cor, uE, uW, vN, vS, wU, wD;
     if (X) {
       cor = 0.25 * f * ((double)D.vhp[IX(k, lE)] + (double)D.vhp[IX(k, c)] + (double)D.vhp[IX(k, l2)] + (double)D.vhp[IX(k, lS)]);
@@ -271,11 +261,21 @@ __device__ __forceinline__ void exmom_cell(const Dev<R> &D, int c, int k, double
               axw * ((double)D.upp[IX(k, D.lN[lW])] - (double)D.upp[IX(k, lW)])) / D.P.dxdy;
       }
     }
-    if (X)
-      ex[IX(k, c)] = (R)((double)D.uhpp[IX(k, c)] - twodt1 * (adv * D.P.iadv - cor - hd));
-    else
-      ex[IX(k, c)] = (R)((double)D.vhpp[IX(k, c)] - twodt1 * (adv * D.P.iadv + cor - hd));
+    if (X) return (double)D.uhpp[IX(k, c)] - twodt1 * (adv * D.P.iadv - cor - hd);
+    return (double)D.vhpp[IX(k, c)] - twodt1 * (adv * D.P.iadv + cor - hd);
   }
+}
+
+template <typename R, bool X>
+__device__ __forceinline__ void exmom_cell(const Dev<R> &D, int c, int k, double twodt1) {
+  const int S = D.lm;
+  const int ln = X ? D.lE[c] : D.lN[c];
+  if (ln == S) return;  // no face: ex stays zero
+  const int kmq = min(D.kmz[ln], D.kmz[c]);
+  if (k > kmq) return;
+  const int k1q = X ? D.k1u[c] : D.k1v[c];
+  R *ex = X ? D.ex : D.ey;
+  ex[IX(k, c)] = (k < k1q) ? R(0) : (R)exmom_value<R, X>(D, c, k, k1q, kmq, twodt1);
 }
 
 template <typename R, bool X, bool CH>
