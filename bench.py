@@ -90,6 +90,9 @@ def parse():
     ap.add_argument("--river", type=float, default=0.0, metavar="Q",
                     help="a channel of the same size with flow Q (m3/s) entering at the west end and leaving at "
                          "the east end (open boundaries of type 2) instead of the elliptic lake")
+    ap.add_argument("--mixed-error", type=int, default=0, metavar="STEPS",
+                    help="advance the lake in FP64 and in mixed precision side by side for STEPS steps and report "
+                         "the per-field max / RMS difference (one JSON line) instead of the throughput")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (calibrates HBM byte counters) and exit")
     return ap.parse_args()
@@ -156,10 +159,64 @@ def copy_bandwidth(mib=1024, reps=10):
     return 2 * nbytes * reps / (ms.value * 1e-3) / 1e9
 
 
+def mixed_error(a):
+    """The mixed-precision run against the FP64 run of the same lake: per-field
+    max and RMS difference over the wet cells after a.mixed_error steps (and at
+    every quarter of the run), absolute and relative to the FP64 field's max /
+    RMS.  Both runs solve the free surface to the same tolerance."""
+    from si3d_amd import lake
+    from si3d_amd.model import GpuLake
+    nsteps = a.mixed_error
+    deck = lake.ellipse(nx=a.nx, ny=a.ny, dx=100.0, nlayers=a.layers, dt=60.0, nts=nsteps + 2, seed=1)
+    f64 = GpuLake(deck, precision="fp64", preconditioner=a.precond, tol=a.tol)
+    mix = GpuLake(deck, precision="mixed", preconditioner=a.precond, tol=a.tol)
+    fields = ["s", "uh", "vh", "wp", "sal", "h", "q2", "Av"]
+    h0 = f64.get("h")
+    wet3 = h0 > 0.0
+    lm = f64.lm
+
+    def errors():
+        out = {}
+        wet_now = f64.get("h") > 0.0
+        for name in fields:
+            x, y = f64.get(name), mix.get(name)
+            if x.ndim == 1:
+                x, y = x[:lm], y[:lm]
+            else:
+                m = wet_now[:lm] | wet3[:lm]
+                x, y = x[:lm][m], y[:lm][m]
+            d = np.abs(y.astype(np.float64) - x.astype(np.float64))
+            ax = np.abs(x.astype(np.float64))
+            rms_x = float(np.sqrt(np.mean(x.astype(np.float64) ** 2)))
+            rms_d = float(np.sqrt(np.mean(d ** 2)))
+            out[name] = {"max": float(d.max()), "rms": rms_d, "max_rel": float(d.max() / max(ax.max(), 1e-300)),
+                         "rms_rel": rms_d / max(rms_x, 1e-300)}
+        return out
+
+    checkpoints = {}
+    t0 = time.perf_counter()
+    for n in range(1, nsteps + 1):
+        f64.step(n)
+        mix.step(n)
+        if n % max(1, nsteps // 4) == 0 or n == nsteps:
+            checkpoints[n] = errors()
+            print(f"mixed-error: step {n} of {nsteps}, {time.perf_counter() - t0:.0f} s, "
+                  f"s max_rel {checkpoints[n]['s']['max_rel']:.2e}", file=sys.stderr, flush=True)
+    print(json.dumps({"report": "mixed precision vs FP64, same lake, same free-surface tolerance",
+                      "workload": f"ellipse lake {a.nx}x{a.ny}x{a.layers}, dx=100 m, dt=60 s, {nsteps} steps, "
+                                  f"PCG ({a.precond}) tol {a.tol:g}",
+                      "errors_after": {str(k): v for k, v in checkpoints.items()},
+                      "pcg_iters_per_step": {"fp64": f64.stats()["pcg_iters_total"] / nsteps,
+                                             "mixed": mix.stats()["pcg_iters_total"] / nsteps}}))
+
+
 def main():
     a = parse()
     if a.copy_probe:
         copy_probe(a.copy_probe)
+        return
+    if a.mixed_error:
+        mixed_error(a)
         return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {a.gpus} is not supported "
