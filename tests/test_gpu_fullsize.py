@@ -1,8 +1,8 @@
 """Parity at the benchmark sizes: the 1024x512x60 lake's grid, masks and
 initial state bit-exact against the oracle, one full time step against the
 oracle, and size-independent invariants over many steps (basin volume,
-free-stream preservation); the 100x50x20 rectangle over 200 steps against the
-oracle."""
+free-stream preservation); the 100x50x20 rectangle over its 1000 steps against
+the oracle."""
 import numpy as np
 import pytest
 
@@ -86,24 +86,26 @@ def test_c1_uniform_temperature_stays_uniform(precision):
     assert np.max(np.abs(g.get("uh"))) > 1e-4
 
 
-def test_c0_rectangle_200_steps_matches_oracle(oracle_lib):
+def test_c0_rectangle_1000_steps_matches_oracle(oracle_lib):
     """The 100x50x20 wind-driven rectangle (dx = 50 m, 20 m deep, 0.1 Pa
-    along x, dt = 30 s) over 200 steps against the oracle."""
+    along x, dt = 30 s) over its full 1000 steps against the oracle (zeta,
+    u, v, w and the layer thicknesses after 200 and 1000 steps)."""
     from si3d_amd.model import GpuLake
-    deck = lake.rectangle(nx=100, ny=50, dx=50.0, depth=20.0, dz=1.0, tau=0.1, dt=30.0, nts=210)
+    deck = lake.rectangle(nx=100, ny=50, dx=50.0, depth=20.0, dz=1.0, tau=0.1, dt=30.0, nts=1010)
     deck["pcg_tol"] = 1e-12
-    print("case: 100x50x20 rectangle, 200 steps", flush=True)
+    print("case: 100x50x20 rectangle, 1000 steps", flush=True)
     o = oracle_lib.Oracle(deck)
     g = GpuLake(deck, precision="fp64", preconditioner="mg", tol=1e-12)
-    for n in range(1, 201):
+    for n in range(1, 1001):
         o.step(n)
         g.step(n)
-    errs = {n: relerr(g.get(n), o.get(n)) for n in ["s", "u", "wp", "h"]}
-    # v is rounding noise under a wind along x: measured against u
-    errs["v"] = float(np.max(np.abs(g.get("v") - o.get("v"))) / np.max(np.abs(o.get("u"))))
-    print(errs, flush=True)
-    for n, e in errs.items():
-        assert e < 1e-7, (n, e)
+        if n in (200, 1000):
+            errs = {f: relerr(g.get(f), o.get(f)) for f in ["s", "u", "wp", "h"]}
+            # v is rounding noise under a wind along x: measured against u
+            errs["v"] = float(np.max(np.abs(g.get("v") - o.get("v"))) / np.max(np.abs(o.get("u"))))
+            print("step", n, errs, flush=True)
+            for f, e in errs.items():
+                assert e < 1e-9, (n, f, e)
 
 
 def test_c3_2048_volume_and_free_stream():
