@@ -210,6 +210,28 @@ def mixed_error(a):
                                              "mixed": mix.stats()["pcg_iters_total"] / nsteps}}))
 
 
+def per_solve_iterations(g, deck, n_first, nsteps):
+    """PCG iterations of each free-surface solve of a step (the leapfrog
+    solve, then the trapezoidal iterations), averaged over nsteps steps run
+    call by call as the time loop does (si3d.f90:324-393), untimed."""
+    niter = max(1, deck.get("niter", 2))
+    rows = []
+    for n in range(n_first, n_first + nsteps):
+        thrs = n * deck["dt"] / 3600.0
+        row = []
+        g.fd(n, 1, lastiter=0, thrs=thrs)
+        row.append(g.stats()["pcg_iters_last"])
+        g.settrap()
+        for it in range(1, niter + 1):
+            g.fd(n, 2, lastiter=1 if it == niter else 0, thrs=thrs, iter=it)
+            row.append(g.stats()["pcg_iters_last"])
+            if it < niter:
+                g.settrap2()
+        g.save(2)
+        rows.append(row)
+    return [round(float(v), 2) for v in np.mean(np.asarray(rows, dtype=np.float64), axis=0)]
+
+
 def main():
     a = parse()
     if a.copy_probe:
@@ -225,7 +247,7 @@ def main():
     from si3d_amd import lake
     from si3d_amd.model import GpuEnsemble, GpuLake
 
-    nts = 2 * (a.warmup + a.steps) + 2
+    nts = 2 * (a.warmup + a.steps) + 10
     t_setup = time.perf_counter()
     if a.ensemble:
         decks = lake.ensemble(a.ensemble, seed=7, side=(128, a.side_max), nts=nts)
@@ -289,6 +311,9 @@ def main():
 
     value = n_wet * a.steps / elapsed
     iters = (st1["pcg_iters_total"] - st0["pcg_iters_total"]) / a.steps
+    n_probe = 3
+    trap_every_step = not a.ensemble and deck.get("itrap", 1) == 1
+    per_solve = per_solve_iterations(g, deck, n0 + a.steps, n_probe) if trap_every_step else None
 
     learned = None
     if not a.no_learned:
@@ -302,16 +327,18 @@ def main():
         g.synchronize()
         sl0 = g.stats()
         t1 = time.perf_counter()
-        n1 = n0 + a.steps
+        n1 = n0 + a.steps + (n_probe if per_solve else 0)
         for n in range(n1, n1 + a.steps):
             g.step(n)
         g.synchronize()
         el1 = time.perf_counter() - t1
         sl1 = g.stats()
+        per_solve_l = per_solve_iterations(g, deck, n1 + a.steps, n_probe) if per_solve else None
         info = g.learn_info()
         learned = {"value": round(n_wet * a.steps / el1, 1), "unit": "wet-cell-steps/s",
                    "ms_per_step": round(1e3 * el1 / a.steps, 4),
                    "pcg_iters_per_step": round((sl1["pcg_iters_total"] - sl0["pcg_iters_total"]) / a.steps, 2),
+                   "pcg_iters_per_solve": per_solve_l,
                    "pcg_relres_last": sl1["pcg_relres_last"],
                    "guess_error_ratio": round(info["guess_error_ratio"], 4),
                    "pairs": info["pairs"], "train_steps": info["train_steps"],
@@ -372,6 +399,7 @@ def main():
         "config": {"workload": workload,
                    "n_wet": n_wet, "lm": g.lm, "km1": g.km1, "parallelism": "1 GPU"},
         "pcg_iters_per_step": round(iters, 2),
+        "pcg_iters_per_solve": per_solve,
         "pcg_relres_last": st1["pcg_relres_last"],
         "setup_s": round(t_setup, 2),
         "roofline": roofline,
