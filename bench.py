@@ -57,9 +57,9 @@ ARRAYS_PER_CELL = {
 # profiles/hbm_traffic_ellipse_1024x512x60_fp64.md.  Reported as
 # roofline.traffic for that workload only.
 TRAFFIC_FP64_1024x512x60 = {
-    "turbulence": 212.6, "turb_cell": 136.2, "exsal_imsal": 202.6, "exsal_rhs": 144.8, "imsal": 66.9,
-    "exmom_x": 134.9, "exmom_y": 176.5, "matmom_x": 142.3, "matmom_y": 124.5, "settrap_cell": 445.7,
-    "settrap2_cell": 285.9, "save_cell": 294.5, "exsal_hflux": 130.0, "continuity1": 107.6, "vel": 71.4,
+    "turbulence": 212.5, "turb_cell": 134.3, "exsal_imsal": 202.4, "exsal_rhs": 146.8, "imsal": 66.9,
+    "exmom_x": 124.7, "exmom_y": 199.3, "matmom_x": 142.3, "matmom_y": 124.5, "settrap_cell": 409.9,
+    "settrap2_cell": 274.3, "save_cell": 211.5, "exsal_hflux": 135.5, "continuity1": 107.5, "vel": 77.4,
     "vel2": 47.9, "slayer_h": 60.8, "continuity2": 49.1, "matcon": 29.0,
 }
 
