@@ -273,6 +273,8 @@ class GpuEnsemble(GpuLake):
     def __init__(self, decks, precision="fp64", preconditioner="mg", tol=None, maxit=None):
         L = _lib.load()
         n = len(decks)
+        if any(dk.get("openbc") is not None for dk in decks):
+            raise ValueError("ensemble lakes take no open boundaries (run such a lake alone)")
         keep = []
         darr = (S3gDeck * n)()
         parr = (S3gParams * n)()

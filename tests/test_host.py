@@ -64,3 +64,13 @@ def test_bench_refuses_multi_gpu():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_ensemble_refuses_open_boundaries():
+    """Open boundaries are per lake; the ensemble refuses a deck that has them
+    (before any device work)."""
+    from si3d_amd import lake
+    from si3d_amd.model import GpuEnsemble
+    d = lake.open_boundary(lake.channel(nx=12, ny=6), "W", 2, 2, 7, lake.steady_series(1.0, 20.0))
+    with pytest.raises(ValueError, match="open boundaries"):
+        GpuEnsemble([lake.channel(nx=12, ny=6), d])
