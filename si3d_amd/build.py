@@ -56,11 +56,21 @@ def build_fortran_binding(verbose=False):
     os.makedirs(out, exist_ok=True)
     src = os.path.join(HERE, "fortran", "si3d_gpu_mod.f90")
     obj = os.path.join(out, "si3d_gpu_mod.o")
-    r = subprocess.run([flang, "-c", src, "-o", obj], cwd=out, capture_output=True, text=True)
+    r = subprocess.run([flang, "-c", "-fPIC", src, "-o", obj], cwd=out, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"flang failed on si3d_gpu_mod.f90\n{r.stdout}\n{r.stderr}")
     if verbose:
         print("built", obj)
+    # a Fortran caller of the library through the module, as a shared library
+    # (the Fortran binding's check: tests call it in-process)
+    chk = os.path.join(HERE, "fortran", "s3g_fortran_check.f90")
+    lib = os.path.join(out, "libs3g_fortran_check.so")
+    r = subprocess.run([flang, "-shared", "-fPIC", chk, obj, "-L", OUT, "-lsi3dgpu", "-Wl,-rpath,$ORIGIN/..",
+                        "-o", lib], cwd=out, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"flang failed on s3g_fortran_check.f90\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print("built", lib)
     return obj
 
 

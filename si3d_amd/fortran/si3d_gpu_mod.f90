@@ -38,6 +38,13 @@ MODULE si3d_gpu
       INTEGER(C_INT) :: ntr
    END TYPE s3g_params
 
+   ! input deck of the standalone driver (s3g_create_from_deck)
+   TYPE, BIND(C) :: s3g_deck
+      INTEGER(C_INT) :: imm, jmm, ibathyf, km1_file
+      REAL(C_DOUBLE) :: ddz, dzmin, datadj, zetainit, zl
+      TYPE(C_PTR)    :: zlevel_file, bathy_dm, init_profile, tracer_profiles
+   END TYPE s3g_deck
+
    ! nopen block of si3d_inp.txt and the openbcNN.txt records (openbc0)
    TYPE, BIND(C) :: s3g_openbc
       INTEGER(C_INT) :: nopen
@@ -55,6 +62,14 @@ MODULE si3d_gpu
          TYPE(C_PTR), INTENT(OUT)     :: ctx
          INTEGER(C_INT) :: s3g_init
       END FUNCTION s3g_init
+
+      FUNCTION s3g_create_from_deck(deck, params, ctx) BIND(C, NAME="s3g_create_from_deck")
+         IMPORT :: C_INT, C_PTR, s3g_deck, s3g_params
+         TYPE(s3g_deck), INTENT(IN)   :: deck
+         TYPE(s3g_params), INTENT(IN) :: params
+         TYPE(C_PTR), INTENT(OUT)     :: ctx
+         INTEGER(C_INT) :: s3g_create_from_deck
+      END FUNCTION s3g_create_from_deck
 
       FUNCTION s3g_set_field_f32(ctx, name, host) BIND(C, NAME="s3g_set_field_f32")
          IMPORT :: C_INT, C_PTR, C_CHAR, C_FLOAT
