@@ -242,7 +242,7 @@ def main():
         return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {a.gpus} is not supported "
-              "(the multi-GPU decomposition is described in DESIGN.md)", file=sys.stderr)
+              "(the multi-GPU decomposition exists as a design, not as code)", file=sys.stderr)
         sys.exit(2)
     from si3d_amd import lake
     from si3d_amd.model import GpuEnsemble, GpuLake

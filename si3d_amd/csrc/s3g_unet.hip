@@ -12,7 +12,7 @@
 // output noise, which A amplifies, is penalised), by the forward and
 // backward kernels below, and used as the initial guess of the free-surface
 // CG, which still converges to the same FP64 true-residual tolerance: the
-// plain CG is the fallback and the judge of its error.
+// plain CG is the fallback and decides when the error is small enough.
 //
 // Tensors are NHWC bf16 on the lake's (i, j) box (H x W padded to multiples
 // of 8), channels contiguous.  Convolutions are implicit GEMMs on the gfx950
