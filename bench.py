@@ -241,7 +241,7 @@ def main():
         mixed_error(a)
         return
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only; --gpus {a.gpus} is not supported "
+        print(f"bench.py: runs on one GPU only; --gpus {a.gpus} is not supported "
               "(the multi-GPU decomposition exists as a design, not as code)", file=sys.stderr)
         sys.exit(2)
     from si3d_amd import lake
