@@ -664,9 +664,9 @@ int s3g_get_field_f64(s3g_ctx *c, const char *n, double *h) { return get_field(c
 int s3g_get_field_i32(s3g_ctx *c, const char *n, int *h) { return get_field(c, n, h); }
 
 int s3g_fd(s3g_ctx *c, int n, int istep, int iter, int lastiter, double thrs) {
-  (void)iter;
   if (!c || (istep != 1 && istep != 2)) return S3G_ERR_ARG;
   c->thrs = thrs;
+  c->solve_slot = (istep == 1) ? 0 : std::min(std::max(iter, 1), 3);
   c->stats.fd_calls++;
   return c->precision == S3G_FP64 ? s3g::fd<double>(c, n, istep, lastiter) : s3g::fd<float>(c, n, istep, lastiter);
 }

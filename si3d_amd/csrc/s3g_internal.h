@@ -216,6 +216,10 @@ struct s3g_ctx {
   // iterations (per storage type: [0] FP64, [1] FP32), rebuilt when the
   // kernel arguments change (S3G_GRAPH=0 launches every kernel instead)
   bool cg_graph = true;
+  // iterations of the latest solve of each kind (0 leapfrog, 1.. trapezoidal
+  // iteration): the first chunk of CG iterations is sized from the same kind
+  int solve_slot = 0;
+  int cg_hint[4] = {0, 0, 0, 0};
   hipGraphExec_t cg_exec[2] = {nullptr, nullptr};
   std::vector<char> cg_key[2];
   bool turb_split = true;  // turbulence closure: per-cell interface terms + sweep

@@ -893,6 +893,9 @@ int solve_free_surface(s3g_ctx *c, double tz) {
   int rc = 0, total = 0;
   double relres = 0.0;
   Outer O{D.lE, D.lW, D.lN, D.lS, D.rr, D.sx, D.sy, D.qq, P.x, P.n, P.nblk, P.part};
+  // size the first chunk of iterations from the latest solve of the same kind
+  const int slot = std::min(std::max(c->solve_slot, 0), 3);
+  const int hint = c->cg_hint[slot] > 0 ? c->cg_hint[slot] : c->stats.pcg_iters_last;
   // Initial guess: the latest free surface s.  The reference starts from sp
   // (1970); s is the same field on leapfrog steps and the previous estimate of
   // zeta^{n+1} on trapezoidal iterations, so it only shortens the iteration.
@@ -913,7 +916,7 @@ int solve_free_surface(s3g_ctx *c, double tz) {
                      c->stream);
     };
     for (int restart = 0; restart < 3; ++restart) {
-      int it = run_cg(c, A, c->pcg_maxit - total, c->stats.pcg_iters_last, true_res);
+      int it = run_cg(c, A, c->pcg_maxit - total, hint, true_res);
       if (it < 0) return S3G_ERR_HIP;
       total += it;
       double tr = 0.0, bb = 0.0;
@@ -947,7 +950,7 @@ int solve_free_surface(s3g_ctx *c, double tz) {
       // and FP32 holds about 1e-5 of reduction per sweep
       const double t_in = std::min(0.5, std::max(kInnerTol, 0.3 * tol / relres));
       A.tol2 = t_in * t_in;
-      int it = run_cg(c, A, c->pcg_maxit - total, c->stats.pcg_iters_last / 2, [] {});
+      int it = run_cg(c, A, c->pcg_maxit - total, hint / 2, [] {});
       if (it < 0) return S3G_ERR_HIP;
       total += it;
       hipLaunchKernelGGL((k_add_correction<float>), g, b, 0, c->stream, P.x, (const float *)P.df, P.n);
@@ -964,6 +967,7 @@ int solve_free_surface(s3g_ctx *c, double tz) {
                  c->precision == S3G_FP64 ? "fp64" : "mixed",
                  c->prec_kind == S3G_PREC_MG ? "mg" : (line ? "line" : "jacobi"), total, relres, rc);
   c->stats.pcg_iters_last = total;
+  c->cg_hint[slot] = total;
   c->stats.pcg_iters_total += total;
   c->stats.pcg_relres_last = relres;
   c->stats.solves += 1;
