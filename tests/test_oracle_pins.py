@@ -1,0 +1,543 @@
+"""Closed-form pins of the oracle's explicit momentum terms (exmom), its
+two-equation turbulence closure and its time-level shifts (settrap, settrap2,
+save).
+
+Each test puts a synthetic state into the oracle through its field setter,
+runs one phase, and checks the result against what the mathematics fixes for
+that state: a centred flux-form difference and a four-point average are exact
+on linear fields; a Fourier mode is an eigenvector of the five-point
+Laplacian; the eddy coefficients follow the quasi-equilibrium stability
+functions of Kantha & Clayson (1994) with the length-scale limit of Galperin
+et al. (1988); in a homogeneous column the implicit TKE and length-scale rows
+reduce to a local balance of storage, production, buoyancy and dissipation,
+and the boundary rows to the Mellor-Yamada values q^2 = B1^(2/3) u*^2; the
+layer thicknesses of a column add up to its water depth.  Every state is
+chosen so that a dropped term, a wrong neighbour or an x/y swap changes the
+checked value by far more than the tolerance."""
+import math
+
+import numpy as np
+import pytest
+
+from si3d_amd import lake
+
+# Mellor-Yamada level 2.5 constants and the other physical constants of the
+# closure (si3d_types.f90:33, 210-219)
+A1, A2, B1, B2, C2, C3 = 0.92, 0.74, 16.6, 10.1, 0.70, 0.20
+E1, E2, E3, SQ, KAPPA, G, RHOAIR = 1.80, 1.33, 0.25, 0.20, 0.41, 9.806, 1.3
+GH_MIN, GH_MAX, Q2_MIN, Q2L_MIN = -0.28, 0.029, 2.0e-6, 6.80236723501e-9
+AV_MOL, DV_MOL = 1.3e-6, 1.4e-7
+
+
+def _grid(o):
+    """Per column l (0-based): cell indices (i, j), neighbours (0-based, lm =
+    the dry sentinel column) and the wet layer range k1..kmz (1-based)."""
+    g = {n: o.get(n).astype(int) for n in ["l2i", "l2j", "lEC", "lWC", "lNC", "lSC", "kmz", "k1z", "k1u", "k1v"]}
+    for n in ["lEC", "lWC", "lNC", "lSC"]:
+        g[n] = g[n] - 1
+    return g
+
+
+def _interior(o, g, margin=3):
+    i, j = g["l2i"][:o.lm], g["l2j"][:o.lm]
+    return [l for l in range(o.lm) if margin + 1 <= i[l] <= o.im1 - margin and margin + 1 <= j[l] <= o.jm1 - margin]
+
+
+def _full(o, value=0.0):
+    return np.full((o.lm1, o.km1), value, dtype=np.float64)
+
+
+# ---------------------------------------------------------------- exmom
+
+def _momentum_lake(**kw):
+    d = lake.rectangle(nx=16, ny=14, dx=40.0, depth=8.0, dz=1.0, dt=60.0, nts=10, **kw)
+    d["dy"] = 60.0  # dx != dy, so that a swapped spacing shows
+    return d
+
+
+def _positions(o, g, dx, dy):
+    """Positions (m) of each column's cell centre, u face (east) and v face
+    (north), and the depth of each layer centre (uniform 1 m layers from
+    level 2)."""
+    i = g["l2i"].astype(float)
+    j = g["l2j"].astype(float)
+    depth = (np.arange(1, o.km1 + 1) - 1.5) * 1.0
+    return i * dx, j * dy, depth
+
+
+def test_exmom_centred_advection_exact_on_linear_fields(oracle_lib):
+    """itrmom = 1 (centred flux form).  With u = U0 + ax X + ay Y + az D on the
+    u faces, a uniform v = V and a uniform w = W, the horizontal terms equal
+    h d(u^2)/dx + h V du/dy and the vertical term h W du/dz exactly, since a
+    centred difference of a quadratic and a two-point mean of a linear field
+    are exact: ex = uhpp - 2dt h (2 u ax + V ay - W az).  Same for v."""
+    d = _momentum_lake(iadv=1, ihd=0)
+    d["f"] = 0.0
+    d["itrmom"] = 1
+    o = oracle_lib.Oracle(d)
+    g = _grid(o)
+    dx, dy = d["dx"], d["dy"]
+    X, Y, D = _positions(o, g, dx, dy)
+    twodt1 = 2.0 * d["dt"]
+    U0, ax, ay, az, V, W = 0.05, 2.0e-5, -3.0e-5, 4.0e-3, 0.03, 1.5e-4
+    h = o.get("hup")
+    # x momentum: u on the east faces (X + dx/2, Y)
+    u = U0 + ax * (X[:, None] + dx / 2) + ay * Y[:, None] + az * D[None, :]
+    for name, val in [("up", u), ("upp", u), ("uhp", u * h), ("uhpp", 0.7 * u * h),
+                      ("vhp", V * h), ("vp", _full(o, V)), ("vpp", _full(o, V)), ("wp", _full(o, W))]:
+        o.set(name, val)
+    o.phase(2, 2, 1)
+    ex = o.get("ex")
+    checked = 0
+    for l in _interior(o, g):
+        for k in range(g["k1u"][l] + 1, g["kmz"][l]):  # interior layers: w on both faces
+            expect = 0.7 * u[l, k - 1] * h[l, k - 1] - twodt1 * h[l, k - 1] * (2 * u[l, k - 1] * ax + V * ay - W * az)
+            assert ex[l, k - 1] == pytest.approx(expect, rel=1e-12, abs=1e-15), (l, k)
+            checked += 1
+    assert checked > 200
+    # y momentum: v on the north faces (X, Y + dy/2), uniform u = V
+    o = oracle_lib.Oracle(d)
+    hv = o.get("hvp")
+    v = U0 + ax * X[:, None] + ay * (Y[:, None] + dy / 2) + az * D[None, :]
+    for name, val in [("vp", v), ("vpp", v), ("vhp", v * hv), ("vhpp", 0.7 * v * hv),
+                      ("uhp", V * hv), ("up", _full(o, V)), ("upp", _full(o, V)), ("wp", _full(o, W))]:
+        o.set(name, val)
+    o.phase(4, 2, 1)
+    ey = o.get("ex")
+    for l in _interior(o, g):
+        for k in range(g["k1v"][l] + 1, g["kmz"][l]):
+            expect = 0.7 * v[l, k - 1] * hv[l, k - 1] - twodt1 * hv[l, k - 1] * (V * ax + 2 * v[l, k - 1] * ay - W * az)
+            assert ey[l, k - 1] == pytest.approx(expect, rel=1e-12, abs=1e-15), (l, k)
+
+
+def test_exmom_diffusion_eigenvalue_of_fourier_mode(oracle_lib):
+    """ihd = 1 (constant Ax0, Ay0).  u = A sin(a i) cos(b j) is an eigenvector
+    of the five-point Laplacian with eigenvalue -(2-2cos a)/dx^2 - (2-2cos
+    b)/dy^2 (per direction weighted by its coefficient), so with no advection
+    and no rotation ex = uhpp - 2dt h u (Ax0 (2-2cos a)/dx^2 + Ay0 (2-2cos
+    b)/dy^2).  Ax0 != Ay0 and a != b make an x/y swap visible."""
+    d = _momentum_lake(iadv=0, ihd=1, Ax0=3.0)
+    d["Ay0"] = 7.0
+    d["f"] = 0.0
+    o = oracle_lib.Oracle(d)
+    g = _grid(o)
+    twodt1 = 2.0 * d["dt"]
+    a, b = 0.9, 0.4
+    i = g["l2i"].astype(float)[:, None]
+    j = g["l2j"].astype(float)[:, None]
+    mode = 0.02 * np.sin(a * i) * np.cos(b * j) * np.ones((1, o.km1))
+    h = o.get("hupp")
+    o.set("upp", mode)
+    o.set("uhpp", mode * h)
+    o.set("up", mode)
+    o.phase(2, 2, 1)
+    ex = o.get("ex")
+    lam = d["Ax0"] * (2 - 2 * math.cos(a)) / d["dx"] ** 2 + d["Ay0"] * (2 - 2 * math.cos(b)) / d["dy"] ** 2
+    checked = 0
+    for l in _interior(o, g):
+        for k in range(g["k1u"][l], g["kmz"][l] + 1):
+            m = mode[l, k - 1]
+            expect = m * h[l, k - 1] - twodt1 * h[l, k - 1] * m * lam
+            assert ex[l, k - 1] == pytest.approx(expect, rel=1e-11, abs=1e-16), (l, k)
+            checked += 1
+    assert checked > 200
+    # the diffusion term is not negligible against the storage term
+    assert twodt1 * lam > 0.1
+
+
+def test_exmom_coriolis_is_the_four_point_mean(oracle_lib):
+    """The Coriolis term at a u face is f times the mean of the four
+    surrounding v-face transports; for a linear field on the v faces that
+    mean is the field's value at the u face (i + 1/2, j) exactly:
+    ex = uhpp + 2dt f h F(u face).  Same for v with the u-face field."""
+    d = _momentum_lake(iadv=0, ihd=0)
+    d["f"] = 1.0e-4
+    o = oracle_lib.Oracle(d)
+    g = _grid(o)
+    twodt1 = 2.0 * d["dt"]
+    dx, dy = d["dx"], d["dy"]
+    X, Y, _ = _positions(o, g, dx, dy)
+    ax, ay, c0 = 3.0e-4, -5.0e-4, 0.02
+    h = o.get("hvp")
+    field_v = c0 + ax * X + ay * (Y + dy / 2)  # value of F at each v face
+    o.set("vhp", field_v[:, None] * h)
+    o.set("uhpp", _full(o, 0.0))
+    o.phase(2, 2, 1)
+    ex = o.get("ex")
+    for l in _interior(o, g):
+        at_u = c0 + ax * (X[l] + dx / 2) + ay * Y[l]
+        for k in range(g["k1u"][l], g["kmz"][l] + 1):
+            assert ex[l, k - 1] == pytest.approx(twodt1 * d["f"] * h[l, k - 1] * at_u, rel=1e-12), (l, k)
+    o = oracle_lib.Oracle(d)
+    h = o.get("hup")
+    field_u = c0 + ax * (X + dx / 2) + ay * Y
+    o.set("uhp", field_u[:, None] * h)
+    o.set("vhpp", _full(o, 0.0))
+    o.phase(4, 2, 1)
+    ey = o.get("ex")
+    for l in _interior(o, g):
+        at_v = c0 + ax * X[l] + ay * (Y[l] + dy / 2)
+        for k in range(g["k1v"][l], g["kmz"][l] + 1):
+            assert ey[l, k - 1] == pytest.approx(-twodt1 * d["f"] * h[l, k - 1] * at_v, rel=1e-12), (l, k)
+
+
+def test_smagorinsky_coefficient_exact_on_linear_velocity(oracle_lib):
+    """ihd = 2: kh = max(Ax0, C dx dy |S|) with |S|^2 = (du/dx)^2 + (dv/dy)^2
+    + (du/dy + dv/dx)^2 / 2 and C = 0.03 (si3d_mixing.f90:894-966).  The one-
+    sided and four-point differences are exact for u, v linear in x and y, so
+    kh takes the closed-form value in every cell whose four neighbours are
+    wet, and Ax0 next to a wall."""
+    d = _momentum_lake(iadv=0, ihd=2, Ax0=1e-3)
+    o = oracle_lib.Oracle(d)
+    g = _grid(o)
+    dx, dy = d["dx"], d["dy"]
+    X, Y, _ = _positions(o, g, dx, dy)
+    gam, alp, dlt, eps = 2.0e-4, -5.0e-4, 1.5e-4, -3.0e-4
+    u = gam * (X + dx / 2) + alp * Y
+    v = dlt * X + eps * (Y + dy / 2)
+    o.set("upp", u[:, None] * np.ones((1, o.km1)))
+    o.set("vpp", v[:, None] * np.ones((1, o.km1)))
+    o.phase(1, 2, 1)
+    kh = o.get("kh")
+    expect = 0.03 * dx * dy * math.sqrt(gam ** 2 + eps ** 2 + 0.5 * (alp + dlt) ** 2)
+    assert expect > 10 * d["Ax0"]
+    inner = set(_interior(o, g, margin=2))  # all four neighbours wet
+    for l in range(o.lm):
+        for k in range(g["k1z"][l], g["kmz"][l] + 1):
+            want = expect if l in inner else d["Ax0"]
+            assert kh[l, k - 1] == pytest.approx(want, rel=1e-12), (l, k)
+
+
+def test_asselin_filter(oracle_lib):
+    """ismooth = 1: the n level of zeta and of the transports is filtered as
+    x^n + (beta/2)(x^{n+1} - 2 x^n + x^{n-1}); a state linear in time is a
+    fixed point, the velocities follow the filtered transports, and the new
+    layer thicknesses add up to bed depth + filtered zeta."""
+    d = _momentum_lake(iadv=0, ihd=0)
+    d["ismooth"] = 1
+    d["beta"] = 0.1
+    o = oracle_lib.Oracle(d)
+    g = _grid(o)
+    rng = np.random.default_rng(5)
+    lm = o.lm
+    sp = rng.uniform(-0.02, 0.02, o.lm1)
+    s = sp + rng.uniform(-0.01, 0.01, o.lm1)
+    spp = 2 * sp - s                                       # linear in time ...
+    spp[: lm // 2] += rng.uniform(-0.005, 0.005, lm // 2)  # ... in half the columns
+    o.set("s", s)
+    o.set("sp", sp)
+    o.set("spp", spp)
+    h = o.get("hup")
+    uhp = rng.uniform(-0.05, 0.05, h.shape) * h
+    uh = uhp + rng.uniform(-0.01, 0.01, h.shape)
+    uhpp = rng.uniform(-0.05, 0.05, h.shape) * h
+    for n, val in [("uh", uh), ("uhp", uhp), ("uhpp", uhpp)]:
+        o.set(n, val)
+    o.phase(12, 2, 1)
+    b2 = d["beta"] / 2
+    sp_new = o.get("sp")
+    np.testing.assert_allclose(sp_new[:lm], sp[:lm] + b2 * (s[:lm] - 2 * sp[:lm] + spp[:lm]), rtol=0, atol=1e-16)
+    np.testing.assert_allclose(sp_new[lm // 2:lm], sp[lm // 2:lm], rtol=0, atol=1e-17)
+    hp_new, hup_new = o.get("hp"), o.get("hup")
+    _column_sums(o, g, hp_new, hup_new, sp_new)
+    uhp_new, up_new = o.get("uhp"), o.get("up")
+    for l in range(lm):
+        e = g["lEC"][l]
+        if e >= lm:
+            continue
+        for k in range(g["k1u"][l], min(g["kmz"][l], g["kmz"][e]) + 1):
+            x = (l, k - 1)
+            assert uhp_new[x] == pytest.approx(uhp[x] + b2 * (uh[x] - 2 * uhp[x] + uhpp[x]), rel=1e-14, abs=1e-18)
+            assert up_new[x] == pytest.approx(uhp_new[x] / hup_new[x], rel=1e-15)
+
+
+# ----------------------------------------------------------- turbulence
+
+def _kc_stability(gh):
+    """Kantha & Clayson (1994) quasi-equilibrium stability functions."""
+    sh = A2 * (1 - 6 * A1 / B1) / (1 - 3 * A2 * gh * (6 * A1 + B2 * (1 - C3)))
+    sm = (B1 ** (-1.0 / 3.0) + 9 * A1 * (2 * A1 + A2 * (1 - C2)) * sh * gh) / (1 - 9 * A1 * A2 * gh)
+    return sm, sh
+
+
+def _turb_lake(depth=30.0):
+    d = lake.rectangle(nx=8, ny=7, dx=100.0, depth=depth, dz=1.0, dt=60.0, nts=10, iadv=0, ihd=0)
+    d["iturb"] = 1
+    d["f"] = 0.0
+    return d
+
+
+def test_stability_functions_neutral_values():
+    """At Gh = 0 the stability functions reduce to Sm = B1^(-1/3) = 0.3920 and
+    Sh = A2 (1 - 6 A1/B1) = 0.4939, the neutral values of the closure."""
+    sm, sh = _kc_stability(0.0)
+    assert sm == pytest.approx(0.39201, abs=5e-5)
+    assert sh == pytest.approx(0.49393, abs=5e-5)
+
+
+def test_eddy_coefficients_follow_kantha_clayson(oracle_lib):
+    """Av, Dv at every wet interface from the p-level state: N^2 from the
+    density jump over the mean of the two layer thicknesses, l = q2l/q2
+    limited to 0.53 sqrt(q2/N^2) when stable, Gh = -N^2 l^2/q2 clipped to
+    [-0.28, 0.029], Av = q l Sm(Gh) + nu, Dv = q l Sh(Gh) + kappa; zero above
+    the first and below the last wet interface.  The random state covers
+    stable, neutral and unstable interfaces and both clips."""
+    d = _turb_lake(depth=12.0)
+    o = oracle_lib.Oracle(d)
+    g = _grid(o)
+    rng = np.random.default_rng(11)
+    lm1, km1 = o.lm1, o.km1
+    depth = np.arange(km1) * 1.0
+    beta = rng.uniform(-0.02, 0.08, (lm1, 1))
+    beta[0] = 0.0                                      # one neutral column
+    rho = -2.0 + beta * depth[None, :] + np.where(beta != 0.0, rng.normal(0, 0.01, (lm1, km1)), 0.0)
+    q2p = rng.uniform(1e-5, 1e-3, (lm1, km1))
+    ell = rng.uniform(0.05, 2.0, (lm1, km1))
+    q2p[0], ell[0] = 4e-4, 0.8
+    o.set("rhop", rho)
+    o.set("q2p", q2p)
+    o.set("q2lp", q2p * ell)
+    hp = o.get("hp")
+    o.phase(13, 2, 1)
+    Av, Dv = o.get("Av"), o.get("Dv")
+    n_lim = n_lo = n_hi = 0
+    for l in range(o.lm):
+        k1s, kms = g["k1z"][l], g["kmz"][l]
+        for k in range(1, km1 + 1):
+            if k <= k1s or k > kms:
+                assert Av[l, k - 1] == 0.0 and Dv[l, k - 1] == 0.0
+                continue
+            a, b = k - 2, k - 1                          # 0-based layers k-1 and k
+            dz = 0.5 * (hp[l, a] + hp[l, b])
+            n2 = -G / (0.5 * (rho[l, a] + rho[l, b]) + 1000.0) * (rho[l, a] - rho[l, b]) / dz
+            xl = ell[l, b]
+            if n2 > 0 and 0.53 * math.sqrt(q2p[l, b] / n2) < xl:
+                xl = 0.53 * math.sqrt(q2p[l, b] / n2)
+                n_lim += 1
+            gh = -n2 * xl * xl / q2p[l, b]
+            n_lo += gh < GH_MIN
+            n_hi += gh > GH_MAX
+            gh = min(GH_MAX, max(GH_MIN, gh))
+            sm, sh = _kc_stability(gh)
+            q = math.sqrt(q2p[l, b])
+            assert Av[l, k - 1] == pytest.approx(q * xl * sm + AV_MOL, rel=1e-12), (l, k)
+            assert Dv[l, k - 1] == pytest.approx(q * xl * sh + DV_MOL, rel=1e-12), (l, k)
+            if l == 0:
+                assert Av[l, k - 1] == pytest.approx(0.02 * 0.8 * 0.39201 + AV_MOL, rel=2e-4)
+    assert n_lim > 20 and n_lo > 0 and n_hi > 20, (n_lim, n_lo, n_hi)
+
+
+PP = 0.8  # q2 and q2 l at level n-1 relative to level n
+
+
+def _column_state(o, Q, L, Su, Sv, beta, A0, D0, Ub, Ua, cdw):
+    km1 = o.km1
+    depth = np.arange(1, km1 + 1) - 1.5               # layer centres, 1 m layers from level 2
+    ones = np.ones((o.lm1, 1))
+    o.set("up", ones * (Su * depth)[None, :])
+    o.set("vp", ones * (Sv * depth)[None, :])
+    o.set("u", _full(o, Ub))
+    o.set("v", _full(o, 0.0))
+    o.set("rhop", ones * (-1.5 + beta * depth)[None, :])
+    for n, v in [("q2p", Q), ("q2pp", PP * Q), ("q2lp", Q * L), ("q2lpp", PP * Q * L), ("Av", A0), ("Dv", D0)]:
+        o.set(n, _full(o, v))
+    o.set("uair", np.full(o.lm1, Ua))
+    o.set("vair", np.zeros(o.lm1))
+    o.set("cdw", np.full(o.lm1, cdw))
+    return depth
+
+
+@pytest.mark.parametrize("sheared", [False, True])
+def test_tke_and_length_scale_homogeneous_column(oracle_lib, sheared):
+    """A column with uniform q2 = Q, q2 l = Q L at level n (P times those at
+    n-1), uniform shear (Su, Sv) and a linear stable density profile.  Away
+    from the boundaries the new q2 is locally uniform, the vertical diffusion
+    drops out, and each implicit row reduces to
+      q2  = ((4-P) Q/dt2 + 2 Ps) / (3/dt2 + 2 q/(B1 l) - 2 Pb/Q)
+      q2l = ((4-P) QL/dt2 + l E1 Ps) / (3/dt2 + q/(B1 l) (1 + E2 (l/(k zb))^2
+            + E3 (l/(k zt))^2) - E1 Pb/Q)
+    (BDF2 storage (3 x^{n+1} - 4 x^n + x^{n-1})/dt2, shear production Ps = Av S^2, buoyancy Pb = -Dv N^2 < 0
+    taken implicitly, dissipation q^3/(B1 l), wall function in the distances
+    zb, zt of the interface from bed and surface, dt2 = 2 dt).  The boundary
+    rows give the Mellor-Yamada values q2 = B1^(2/3) u*^2 with u*^2 =
+    cd |u_b|^2 at the bed and rho_air cdw |W - u_s|^2 / rho_0 at the surface."""
+    d = _turb_lake()
+    o = oracle_lib.Oracle(d)
+    g = _grid(o)
+    Q, L = 1.0e-4, 0.5
+    Su, Sv = (2.0e-3, -1.5e-3) if sheared else (0.0, 0.0)
+    beta = 0.05 if sheared else 0.0
+    A0, D0, Ub, Ua, cdw = 2.0e-3, 1.0e-3, 0.08, 6.0, 1.3e-3
+    depth = _column_state(o, Q, L, Su, Sv, beta, A0, D0, Ub, Ua, cdw)
+    hp = o.get("hp")
+    o.phase(13, 2, 1)
+    q2, q2l = o.get("q2"), o.get("q2l")
+    dt2 = 2.0 * d["dt"]
+    for l in _interior(o, g, margin=2):
+        k1s, kms = g["k1z"][l], g["kmz"][l]
+        assert kms - k1s > 20
+        # boundary rows
+        us = Su * depth[k1s - 1]
+        ustar_s = cdw * RHOAIR * ((Ua - us) ** 2 + (Sv * depth[k1s - 1]) ** 2) / 1000.0
+        assert q2[l, k1s - 1] == pytest.approx(max(Q2_MIN, B1 ** (2 / 3) * ustar_s), rel=1e-12)
+        assert q2[l, kms] == pytest.approx(max(Q2_MIN, B1 ** (2 / 3) * d["cd"] * Ub ** 2), rel=1e-12)
+        assert q2l[l, k1s - 1] == Q2L_MIN and q2l[l, kms] == Q2L_MIN
+        for k in range(k1s + 9, kms - 8):
+            rho_a, rho_b = -1.5 + beta * depth[k - 2], -1.5 + beta * depth[k - 1]
+            n2 = G * beta / (0.5 * (rho_a + rho_b) + 1000.0)
+            xl = L if n2 <= 0 else min(L, 0.53 * math.sqrt(Q / n2))
+            ps = A0 * (Su * Su + Sv * Sv)
+            pb = -D0 * n2
+            q = math.sqrt(Q)
+            e_q2 = ((4 - PP) * Q / dt2 + 2 * ps) / (3 / dt2 + 2 * q / (B1 * xl) - 2 * pb / Q)
+            zt = hp[l, k1s - 1:k - 1].sum()
+            zb = hp[l, k - 1:kms].sum()
+
+            def e_q2l(e2, e3):
+                wall = 1 + e2 * (xl / (KAPPA * zb)) ** 2 + e3 * (xl / (KAPPA * zt)) ** 2
+                return ((4 - PP) * Q * L / dt2 + xl * E1 * ps) / (3 / dt2 + q / (B1 * xl) * wall - E1 * pb / Q)
+
+            # q2 is uniform here to round-off; q2l varies with the wall function,
+            # whose curvature leaves a diffusion residual below 3e-6
+            assert q2[l, k - 1] == pytest.approx(e_q2, rel=1e-10), (l, k)
+            assert q2l[l, k - 1] == pytest.approx(e_q2l(E2, E3), rel=5e-6), (l, k)
+            # each wall-function term moves the answer by more than twice that
+            for e2, e3 in [(0.0, E3), (E2, 0.0)]:
+                assert abs(e_q2l(e2, e3) / e_q2l(E2, E3) - 1) > 1e-5
+    # the dissipation, production and buoyancy terms are far above the tolerances
+    assert 2 * q / (B1 * xl) * dt2 / 3 > 0.05
+    if sheared:
+        assert 2 * ps * dt2 / (3 * Q) > 1e-3 and -2 * pb / Q * dt2 / 3 > 1e-3 and xl < L
+
+
+# ------------------------------------------------------ settrap / save
+
+def _wet_cells(g, lm):
+    return [(l, k) for l in range(lm) for k in range(g["k1z"][l], g["kmz"][l] + 1)]
+
+
+def _snap(o, names):
+    return {n: o.get(n).copy() for n in names}
+
+
+TIME_FIELDS = ["s", "sp", "spp", "hp", "hpp", "h", "hup", "hupp", "hu", "sal", "salp", "salpp", "uh", "uhp",
+               "uhpp", "up", "upp", "u", "q2", "q2p", "q2pp", "q2l", "q2lp", "q2lpp"]
+
+
+def _stirred_lake():
+    d = lake.rectangle(nx=14, ny=10, dx=50.0, depth=10.0, dz=1.0, dt=60.0, nts=20, tau=0.2)
+    d["iturb"] = 1
+    d["init_profile"] = np.linspace(22.0, 12.0, len(d["init_profile"]))
+    d["itrap"] = 1
+    d["niter"] = 2
+    return d
+
+
+def _column_sums(o, g, hp, hup, sp):
+    """Layer thicknesses summed over each wet column and each u face against
+    bed depth + free surface (for a face: the higher of its two cells)."""
+    hhs, hhu = o.get("hhs"), o.get("hhu")
+    for l in range(o.lm):
+        assert hp[l, g["k1z"][l] - 1:g["kmz"][l]].sum() == pytest.approx(hhs[l] + sp[l], rel=1e-12)
+        e = g["lEC"][l]
+        if e < o.lm:
+            kmx = min(g["kmz"][l], g["kmz"][e])
+            assert hup[l, g["k1u"][l] - 1:kmx].sum() == pytest.approx(hhu[l] + max(sp[l], sp[e]), rel=1e-12)
+
+
+def test_settrap_levels_and_column_depths(oracle_lib):
+    """settrap: zeta^{n+1/2} = (zeta^{n+1} + zeta^n)/2, the n-1 level takes
+    the old n level, scalars, transports and turbulence are averaged over
+    n+1 and n (below the top cell, which is corrected separately), and the
+    new layer thicknesses add up to bed depth + zeta^{n+1/2} in every column
+    and u face.  settrap2 then averages with the unchanged n-1 level."""
+    d = _stirred_lake()
+    o = oracle_lib.Oracle(d)
+    g = _grid(o)
+    for n in range(1, 6):
+        o.step(n)
+    thrs = 6 * d["dt"] / 3600.0
+    o.fd(6, 1, 0, thrs)
+    a = _snap(o, TIME_FIELDS)
+    o.settrap()
+    b = _snap(o, TIME_FIELDS)
+    g = _grid(o)
+    lm = o.lm
+    np.testing.assert_array_equal(b["spp"][:lm], a["sp"][:lm])
+    np.testing.assert_allclose(b["sp"][:lm], 0.5 * (a["s"][:lm] + a["sp"][:lm]), rtol=0, atol=1e-15)
+    np.testing.assert_array_equal(b["hpp"], a["hp"])
+    np.testing.assert_array_equal(b["hupp"], a["hup"])
+    _column_sums(o, g, b["hp"], b["hup"], b["sp"])
+    for l, k in _wet_cells(g, lm):
+        x = (l, k - 1)
+        assert b["salpp"][x] == a["salp"][x]
+        assert b["salp"][x] == 0.5 * (a["sal"][x] + a["salp"][x])
+        if k > g["k1z"][l]:
+            assert b["q2pp"][x] == a["q2p"][x]
+            assert b["q2p"][x] == 0.5 * (a["q2"][x] + a["q2p"][x])
+    for l in range(lm):
+        e = g["lEC"][l]
+        if e >= lm:
+            continue
+        for k in range(g["k1u"][l] + 1, min(g["kmz"][l], g["kmz"][e]) + 1):
+            x = (l, k - 1)
+            assert b["uhpp"][x] == a["uhp"][x] and b["upp"][x] == a["up"][x]
+            assert b["uhp"][x] == 0.5 * (a["uh"][x] + a["uhp"][x])
+            assert b["up"][x] == b["uhp"][x] / b["hup"][x]
+    assert np.max(np.abs(a["uh"] - a["uhp"])) > 1e-6   # the averages are not trivial
+    # settrap2 after a trapezoidal fd: average with the unchanged n-1 level
+    o.fd(6, 2, 0, thrs)
+    a = _snap(o, TIME_FIELDS)
+    o.settrap2()
+    b = _snap(o, TIME_FIELDS)
+    g = _grid(o)
+    np.testing.assert_array_equal(b["spp"][:lm], a["spp"][:lm])
+    np.testing.assert_allclose(b["sp"][:lm], 0.5 * (a["s"][:lm] + a["spp"][:lm]), rtol=0, atol=1e-15)
+    np.testing.assert_array_equal(b["hpp"], a["hpp"])
+    _column_sums(o, g, b["hp"], b["hup"], b["sp"])
+    for l, k in _wet_cells(g, lm):
+        x = (l, k - 1)
+        assert b["salpp"][x] == a["salpp"][x]
+        assert b["salp"][x] == 0.5 * (a["sal"][x] + a["salpp"][x])
+
+
+@pytest.mark.parametrize("istep", [1, 2])
+def test_save_shifts_time_levels(oracle_lib, istep):
+    """save(1) after a leapfrog fd: n-1 := n, n := n+1 for zeta, thicknesses,
+    scalar, transports and turbulence; save(2) after the trapezoidal
+    iterations moves n+1 into n and leaves n-1 alone.  Velocities are the new
+    transports over the new face thicknesses, and the n+1 velocity equals
+    the n velocity afterwards."""
+    d = _stirred_lake()
+    o = oracle_lib.Oracle(d)
+    for n in range(1, 5):
+        o.step(n)
+    thrs = 5 * d["dt"] / 3600.0
+    if istep == 1:
+        o.fd(5, 1, 1, thrs)
+    else:
+        o.fd(5, 1, 0, thrs)
+        o.settrap()
+        o.fd(5, 2, 1, thrs)
+    a = _snap(o, TIME_FIELDS)
+    o.save(istep)
+    b = _snap(o, TIME_FIELDS)
+    g = _grid(o)
+    lm = o.lm
+    np.testing.assert_array_equal(b["sp"][:lm], a["s"][:lm])
+    np.testing.assert_array_equal(b["spp"][:lm], a["sp"][:lm] if istep == 1 else a["spp"][:lm])
+    np.testing.assert_array_equal(b["hp"], a["h"])
+    np.testing.assert_array_equal(b["hup"], a["hu"])
+    np.testing.assert_array_equal(b["hpp"], a["hp"] if istep == 1 else a["hpp"])
+    for l, k in _wet_cells(g, lm):
+        x = (l, k - 1)
+        assert b["salp"][x] == a["sal"][x]
+        assert b["salpp"][x] == (a["salp"][x] if istep == 1 else a["salpp"][x])
+        assert b["q2p"][x] == a["q2"][x] and b["q2lp"][x] == a["q2l"][x]
+        assert b["q2pp"][x] == (a["q2p"][x] if istep == 1 else a["q2pp"][x])
+        assert b["uhp"][x] == a["uh"][x]
+        assert b["uhpp"][x] == (a["uhp"][x] if istep == 1 else a["uhpp"][x])
+        assert b["upp"][x] == (a["up"][x] if istep == 1 else a["upp"][x])
+        if b["hup"][x] > 0:
+            assert b["up"][x] == a["uh"][x] / b["hup"][x] and b["u"][x] == b["up"][x]
+    assert np.max(np.abs(a["s"][:lm] - a["sp"][:lm])) > 1e-7
