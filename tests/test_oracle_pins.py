@@ -410,6 +410,38 @@ def test_tke_and_length_scale_homogeneous_column(oracle_lib, sheared):
         assert 2 * ps * dt2 / (3 * Q) > 1e-3 and -2 * pb / Q * dt2 / 3 > 1e-3 and xl < L
 
 
+# ------------------------------------------------- vel / continuity
+
+def test_kinematic_surface_condition(oracle_lib):
+    """After the free-surface solve, vel's transports (uh = agx - g dt/dx
+    (zeta_E - zeta) arx) fed through continuity give a vertical velocity at
+    the top of every wet column equal to the rate of change of the solved
+    free surface: (zeta^{n+1} - zeta^{n-1}) / 2dt on a leapfrog step and
+    (zeta^{n+1} - zeta^n) / dt on a trapezoidal iteration.  It ties the
+    momentum solution, the continuity system and the solve together."""
+    from tests.helpers import small_ellipse
+    d = small_ellipse(nts=10)
+    d["pcg_tol"] = 1e-13
+    o = oracle_lib.Oracle(d)
+    for n in range(1, 5):
+        o.step(n)
+    lm = o.lm
+    thrs = 5 * d["dt"] / 3600.0
+    for istep in (1, 2):
+        if istep == 1:
+            o.fd(5, 1, 0, thrs)
+        else:
+            o.settrap()
+            o.fd(5, 2, 1, thrs)
+        k1z = o.get("k1z").astype(int)
+        wp, s, spp = o.get("wp"), o.get("s"), o.get("spp")
+        top = np.array([wp[l, k1z[l] - 1] for l in range(lm)])
+        rate = (s[:lm] - spp[:lm]) / (2 * d["dt"] / istep)
+        scale = np.abs(rate).max()
+        assert scale > 1e-7
+        assert np.abs(top - rate).max() < 1e-9 * scale, istep
+
+
 # ------------------------------------------------------ settrap / save
 
 def _wet_cells(g, lm):
