@@ -591,10 +591,11 @@ extern "C" int s3g_set_open_boundaries(s3g_ctx *c, const s3g_openbc *bc) {
   if (hipMalloc(&p, nparts * sizeof(double)) != hipSuccess) return hip_check(hipGetLastError(), "openbc alloc");
   c->allocs.push_back(p);
   B.part = (double *)p;
-  hipMemcpy(B.cells, cells.data(), cells.size() * sizeof(int), hipMemcpyHostToDevice);
-  hipMemcpy(B.along, along.data(), along.size() * sizeof(int), hipMemcpyHostToDevice);
-  hipMemset(B.bq, 0, nb);
-  hipMemset(B.bh, 0, nb);
+  // ordered on the library's stream before the t = 0 kernels below
+  if (hipMemcpyAsync(B.cells, cells.data(), cells.size() * sizeof(int), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipMemcpyAsync(B.along, along.data(), along.size() * sizeof(int), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipMemsetAsync(B.bq, 0, nb, c->stream) != hipSuccess || hipMemsetAsync(B.bh, 0, nb, c->stream) != hipSuccess)
+    return hip_check(hipGetLastError(), "openbc upload");
   c->obc = B;
   // t = 0 values from the first record (varsOpenBC(nn,1,1), 337-647)
   ObcArgs A = obc_args(c, -1);
