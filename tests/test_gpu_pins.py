@@ -1,0 +1,164 @@
+"""The synthetic states of the oracle's closed-form pins, run through the GPU
+library (s3g_set_field_f64 + s3g_phase) and compared with the oracle on the
+same state: exmom on linear fields and a Fourier mode, Coriolis, Smagorinsky,
+the turbulence closure on a random state with every stability branch and
+both Gh clips, a homogeneous sheared and stratified column, and the Asselin
+filter.  FP64: every field to 1e-11 of its scale."""
+import numpy as np
+import pytest
+
+from tests.helpers import compare
+from tests.test_oracle_pins import _column_state, _full, _grid, _momentum_lake, _positions, _turb_lake
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-11
+
+
+def _pair(oracle_lib, deck):
+    from si3d_amd.model import GpuLake
+    deck = dict(deck)
+    print("case", deck["name"], flush=True)
+    return oracle_lib.Oracle(deck), GpuLake(deck, precision="fp64", preconditioner="mg", tol=1e-12)
+
+
+def _set(models, name, value):
+    for m in models:
+        m.set(name, value)
+
+
+def _check_ints(o, g):
+    for n in ["kmz", "k1z", "k1u", "k1v"]:
+        np.testing.assert_array_equal(g.get(n), o.get(n).astype(np.int32), err_msg=n)
+
+
+@pytest.mark.parametrize("state", ["advection", "diffusion", "coriolis"])
+def test_exmom_states_match_oracle(oracle_lib, state):
+    if state == "advection":
+        d = _momentum_lake(iadv=1, ihd=0)
+        d["f"] = 0.0
+        d["itrmom"] = 1
+    elif state == "diffusion":
+        d = _momentum_lake(iadv=0, ihd=1, Ax0=3.0)
+        d["Ay0"] = 7.0
+        d["f"] = 0.0
+    else:
+        d = _momentum_lake(iadv=0, ihd=0)
+        d["f"] = 1.0e-4
+    o, g = _pair(oracle_lib, d)
+    _check_ints(o, g)
+    gr = _grid(o)
+    dx, dy = d["dx"], d["dy"]
+    X, Y, D = _positions(o, gr, dx, dy)
+    h = o.get("hup")
+    hv = o.get("hvp")
+    if state == "advection":
+        u = 0.05 + 2.0e-5 * (X[:, None] + dx / 2) - 3.0e-5 * Y[:, None] + 4.0e-3 * D[None, :]
+        v = 0.04 - 1.0e-5 * X[:, None] + 2.5e-5 * (Y[:, None] + dy / 2) - 2.0e-3 * D[None, :]
+        for name, val in [("up", u), ("upp", u), ("uhp", u * h), ("uhpp", 0.7 * u * h),
+                          ("vp", v), ("vpp", v), ("vhp", v * hv), ("vhpp", 0.7 * v * hv),
+                          ("wp", _full(o, 1.5e-4))]:
+            _set((o, g), name, val)
+    elif state == "diffusion":
+        i = gr["l2i"].astype(float)[:, None]
+        j = gr["l2j"].astype(float)[:, None]
+        mode_u = 0.02 * np.sin(0.9 * i) * np.cos(0.4 * j) * np.ones((1, o.km1))
+        mode_v = 0.01 * np.cos(0.7 * i) * np.sin(0.3 * j) * np.ones((1, o.km1))
+        for name, val in [("upp", mode_u), ("up", mode_u), ("uhpp", mode_u * o.get("hupp")),
+                          ("vpp", mode_v), ("vp", mode_v), ("vhpp", mode_v * o.get("hvpp"))]:
+            _set((o, g), name, val)
+    else:
+        _set((o, g), "vhp", (0.02 + 3.0e-4 * X + -5.0e-4 * (Y + dy / 2))[:, None] * hv)
+        _set((o, g), "uhp", (0.01 - 2.0e-4 * (X + dx / 2) + 4.0e-4 * Y)[:, None] * h)
+        _set((o, g), "uhpp", _full(o, 0.0))
+        _set((o, g), "vhpp", _full(o, 0.0))
+    o.phase(2, 2, 1)
+    g.phase(2, 2, 1)
+    compare(g, o, ["ex"], TOL, f"exmom x, {state}")
+    o.phase(4, 2, 1)
+    g.phase(4, 2, 1)
+    ey = g.get("ey")
+    ex_o = o.get("ex")  # the oracle's exmom(2) writes the y terms into ex
+    err = np.max(np.abs(ey - ex_o)) / max(np.max(np.abs(ex_o)), 1e-30)
+    assert err <= TOL, (state, err)
+    assert np.max(np.abs(ex_o)) > 0
+
+
+def test_smagorinsky_state_matches_oracle(oracle_lib):
+    d = _momentum_lake(iadv=0, ihd=2, Ax0=1e-3)
+    o, g = _pair(oracle_lib, d)
+    gr = _grid(o)
+    dx, dy = d["dx"], d["dy"]
+    X, Y, _ = _positions(o, gr, dx, dy)
+    rng = np.random.default_rng(3)
+    u = (2.0e-4 * (X + dx / 2) - 5.0e-4 * Y)[:, None] + rng.normal(0, 1e-3, (o.lm1, o.km1))
+    v = (1.5e-4 * X - 3.0e-4 * (Y + dy / 2))[:, None] + rng.normal(0, 1e-3, (o.lm1, o.km1))
+    _set((o, g), "upp", u)
+    _set((o, g), "vpp", v)
+    o.phase(1, 2, 1)
+    g.phase(1, 2, 1)
+    compare(g, o, ["kh"], TOL, "Smagorinsky")
+
+
+def test_turbulence_random_state_matches_oracle(oracle_lib):
+    """Stable, neutral and unstable interfaces, the Galperin limit and both
+    Gh clips, random q2 and length scales, random shear."""
+    d = _turb_lake(depth=12.0)
+    o, g = _pair(oracle_lib, d)
+    o.phase(0, 2, 1, 2 * d["dt"] / 3600.0)
+    g.phase(0, 2, 1, 2 * d["dt"] / 3600.0)
+    rng = np.random.default_rng(11)
+    lm1, km1 = o.lm1, o.km1
+    depth = np.arange(km1) * 1.0
+    beta = rng.uniform(-0.02, 0.08, (lm1, 1))
+    rho = -2.0 + beta * depth[None, :] + rng.normal(0, 0.01, (lm1, km1))
+    q2p = rng.uniform(1e-5, 1e-3, (lm1, km1))
+    ell = rng.uniform(0.05, 2.0, (lm1, km1))
+    for name, val in [("rhop", rho), ("q2p", q2p), ("q2lp", q2p * ell), ("q2pp", 0.9 * q2p),
+                      ("q2lpp", 0.8 * q2p * ell), ("up", rng.normal(0, 0.05, (lm1, km1))),
+                      ("vp", rng.normal(0, 0.05, (lm1, km1))), ("u", rng.normal(0, 0.05, (lm1, km1))),
+                      ("v", rng.normal(0, 0.05, (lm1, km1))), ("Av", rng.uniform(1e-4, 1e-2, (lm1, km1))),
+                      ("Dv", rng.uniform(1e-5, 1e-3, (lm1, km1)))]:
+        _set((o, g), name, val)
+    o.phase(13, 2, 1)
+    g.phase(13, 2, 1)
+    compare(g, o, ["Av", "Dv", "q2", "q2l"], TOL, "turbulence, random state")
+
+
+def test_turbulence_homogeneous_column_matches_oracle(oracle_lib):
+    d = _turb_lake()
+    o, g = _pair(oracle_lib, d)
+    o.phase(0, 2, 1, 2 * d["dt"] / 3600.0)
+    g.phase(0, 2, 1, 2 * d["dt"] / 3600.0)
+
+    class Both:
+        lm1, km1 = o.lm1, o.km1
+
+        def set(self, name, value):
+            if name in ("uair", "vair", "cdw"):
+                return  # both models keep the deck's wind from phase 0
+            _set((o, g), name, value)
+
+    _column_state(Both(), 1.0e-4, 0.5, 2.0e-3, -1.5e-3, 0.05, 2.0e-3, 1.0e-3, 0.08, 6.0, 1.3e-3)
+    o.phase(13, 2, 1)
+    g.phase(13, 2, 1)
+    compare(g, o, ["Av", "Dv", "q2", "q2l"], TOL, "turbulence, homogeneous column")
+
+
+def test_asselin_state_matches_oracle(oracle_lib):
+    d = _momentum_lake(iadv=0, ihd=0)
+    d["ismooth"] = 1
+    d["beta"] = 0.1
+    o, g = _pair(oracle_lib, d)
+    rng = np.random.default_rng(5)
+    sp = rng.uniform(-0.02, 0.02, o.lm1)
+    _set((o, g), "s", sp + rng.uniform(-0.01, 0.01, o.lm1))
+    _set((o, g), "sp", sp)
+    _set((o, g), "spp", sp + rng.uniform(-0.01, 0.01, o.lm1))
+    h, hv = o.get("hup"), o.get("hvp")
+    for name, base in [("uh", h), ("uhp", h), ("uhpp", h), ("vh", hv), ("vhp", hv), ("vhpp", hv)]:
+        _set((o, g), name, rng.uniform(-0.05, 0.05, h.shape) * base)
+    o.phase(12, 2, 1)
+    g.phase(12, 2, 1)
+    _check_ints(o, g)
+    compare(g, o, ["sp", "hp", "hup", "hvp", "uhp", "up", "vhp", "vp"], TOL, "Asselin filter")
