@@ -259,3 +259,56 @@ class PlaneFiles:
         for f in self.files:
             f.close()
         self.files = []
+
+
+# ---- 3-D file for particle tracking ---------------------------------------------
+class ThreeDFile:
+    """si3d_3D (outp, si3d_utils.f90:2484-2602): every wet column in l order
+    (the reference's c loop over (i, j) visits the wet cells in the same order,
+    si3d_utils.f90:780-793), layers k1 = 2 .. kmz + 1.  Fortran unformatted
+    sequential records: the frame count, the point count, then per frame
+    (n, year, month, day, hour) and, point by point, the first frame's
+    (i, j, k, hp, up, vp, wp, salp[, Dv, q2p, q2lp, kh, Av]) or a later
+    frame's (hp, up, vp, wp, salp[, Dv, q2p, q2lp, kh, Av]) as REAL*4."""
+
+    K1 = 2
+
+    def __init__(self, g, deck, nts, ipxml, itspf=0, turb_vars=False, directory=".", start=(2000, 1, 1, 0)):
+        self.g, self.deck = g, deck
+        self.n_frames = int((nts - itspf) / max(abs(ipxml), 1))
+        self.turb = bool(turb_vars)
+        self.path = os.path.join(directory, "si3d_3D")
+        self.start = datetime.datetime(start[0], start[1], start[2]) + datetime.timedelta(hours=start[3] / 100.0)
+        self.ij = wet_columns(deck)
+        self.f = None
+
+    def _points(self, kmz):
+        lm = len(self.ij[0])
+        nk = kmz[:lm].astype(int) + 1 - self.K1 + 1  # layers K1 .. kmz + 1
+        cols = np.repeat(np.arange(lm), nk)
+        ks = np.concatenate([np.arange(self.K1, k + 2) for k in kmz[:lm].astype(int)]) if lm else np.zeros(0, int)
+        return cols, ks
+
+    def write(self, n):
+        g = self.g
+        kmz = g.get("kmz")
+        cols, ks = self._points(kmz)
+        names = ["hp", "up", "vp", "wp", "salp"] + (["Dv", "q2p", "q2lp", "kh", "Av"] if self.turb else [])
+        vals = [g.get(x)[cols, ks - 1] for x in names]
+        first = self.f is None
+        if first:
+            self.f = open(self.path, "wb")
+            _record(self.f, np.array([self.n_frames], dtype=np.int32))
+            _record(self.f, np.array([len(cols)], dtype=np.int32))
+            ii, jj = self.ij
+            vals = [ii[cols].astype(np.float64), jj[cols].astype(np.float64), ks.astype(np.float64)] + vals
+        t = self.start + datetime.timedelta(seconds=n * self.deck["dt"])
+        hrs = (t - datetime.datetime(t.year, t.month, t.day)).total_seconds() / 3600.0
+        ints = np.array([n, t.year, t.month, t.day], dtype=np.int32)
+        data = np.stack(vals, axis=1).astype(np.float32)  # (ipoints, nvar), written point by point
+        _record(self.f, ints, np.array([round(hrs * 100)], dtype=np.float32), data)
+
+    def close(self):
+        if self.f is not None:
+            self.f.close()
+            self.f = None

@@ -104,3 +104,42 @@ def test_plane_file_records(oracle_lib, tmp_path):
     np.testing.assert_array_equal(data[:, 3], np.float32(np.where(o.get("hp")[cells, kp - 1] <= 0, -99.0, sal)))
     wp = o.get("wp")
     np.testing.assert_allclose(data[:, 2], (wp[cells, kp - 1] + wp[cells, kp]) / 2, rtol=1e-6, atol=1e-12)
+
+
+@pytest.mark.parametrize("turb", [False, True])
+def test_three_d_file_records(oracle_lib, tmp_path, turb):
+    """si3d_3D: frame and point counts, then one record per frame with the
+    stamp and every wet cell from layer 2 to kmz + 1 of every column in l
+    order (i, j, k first; the fields as REAL*4)."""
+    d, o = _run(oracle_lib)
+    f3 = O.ThreeDFile(o, d, nts=10, ipxml=2, turb_vars=turb, directory=str(tmp_path), start=(2021, 3, 4, 600))
+    f3.write(0)
+    f3.write(2)
+    f3.close()
+    recs = _records(tmp_path / "si3d_3D")
+    assert len(recs) == 4
+    assert np.frombuffer(recs[0], np.int32)[0] == 5
+    kmz = o.get("kmz")[:o.lm].astype(int)
+    npts = int(kmz.sum())  # layers 2 .. kmz + 1
+    assert np.frombuffer(recs[1], np.int32)[0] == npts
+    nv = 13 if turb else 8
+    first = recs[2]
+    assert list(np.frombuffer(first[:16], np.int32)) == [0, 2021, 3, 4]
+    assert np.frombuffer(first[16:20], np.float32)[0] == 600.0
+    data = np.frombuffer(first[20:], np.float32).reshape(npts, nv)
+    ii, jj = O.wet_columns(d)
+    l, k = 37, 2  # the first point of column 37 is its layer 2
+    row = int(kmz[:l].sum())
+    assert data[row, 0] == ii[l] and data[row, 1] == jj[l] and data[row, 2] == k
+    assert data[row, 3] == np.float32(o.get("hp")[l, k - 1])
+    last = row + kmz[l] - 1  # its layer kmz + 1, the cell below the bed
+    assert data[last, 2] == kmz[l] + 1
+    assert data[last, 6] == np.float32(o.get("wp")[l, kmz[l]])
+    later = np.frombuffer(recs[3][20:], np.float32).reshape(npts, nv - 3)
+    assert list(np.frombuffer(recs[3][:16], np.int32)) == [2, 2021, 3, 4]
+    np.testing.assert_array_equal(later[:, 0], data[:, 3])
+    sal = o.get("salp")
+    np.testing.assert_array_equal(later[row:last + 1, 4], np.float32(sal[l, 1:kmz[l] + 1]))
+    if turb:
+        np.testing.assert_array_equal(later[row:last + 1, 9], np.float32(o.get("Av")[l, 1:kmz[l] + 1]))
+
