@@ -223,6 +223,52 @@ def n_wet_cells(deck) -> int:
     return int(kz.sum())
 
 
+def shallow_basin(nx=40, ny=8, dx=50.0, depth=0.9, tau=0.05, top=0.01, bottom=0.2, nlayers=15, dt=20.0,
+                  nts=300, iturb=1, second=None, stratified=False):
+    """Closed, shallow rectangular basin whose top layer is thinner than the
+    wind set-down, so the upwind end dries and re-wets its top layers as the
+    basin seiches: layers grow geometrically from `top` to `bottom` metres,
+    the two side rows are 0.1 m shallower than the rest, and a uniform
+    stress `tau` (Pa) toward +x starts at t = 0.  `second` (m) inserts a
+    layer that thin right below the top one, so that the surface can fall
+    through both in one call.  `stratified` transports temperature (22 C at
+    the surface to 18 C at the bottom, density from it) instead of keeping
+    a uniform 20 C."""
+    z = stretched_levels(nlayers, top, bottom)
+    if second is not None:
+        z = np.concatenate([z[:3], [z[2] + second], z[3:]])
+    bathy = np.full((nx, ny), int(round(depth * 10)), dtype=np.int32)
+    bathy[:, 0] = bathy[:, -1] = int(round(depth * 10)) - 1
+    cw = 1.2e-3
+    d = _deck(
+        name=f"shallow_basin_{nx}x{ny}x{nlayers}", imm=nx, jmm=ny, dx=dx, dy=dx, ddz=1.0, zl=float(z[-1]),
+        dt=dt, nts=nts, ibathyf=-1, zlevel=z, bathy_dm=bathy, iadv=1, itrmom=1, ihd=1, Ax0=1.0, Ay0=1.0,
+        f=0.0, ibc=0, isal=0, iturb=iturb, Av0=1e-3, Dv0=1e-4, ifsurfbc=0, cw=cw,
+        wa=math.sqrt(tau / (cw * G_RHOAIR)), phi=270.0,
+    )
+    d["init_profile"] = np.linspace(22.0, 18.0, len(z)) if stratified else np.full(len(z), 20.0)
+    if stratified:
+        d["isal"], d["ibc"] = 1, 1
+    return d
+
+
+def tidal_flat(nx=40, ny=8, dx=50.0, dz=0.2, amp=0.45, period=7200.0, tau=0.1, dt=30.0, nts=300, iturb=1):
+    """Channel (`channel`, 3 m deep on its axis, 2 m at the banks, layers of
+    `dz` metres) with a tide of amplitude `amp` (m) on a water-level boundary
+    across its west end and a wind stress `tau` (Pa) toward -x: the falling
+    tide empties the top layers, the rising one refills them, and the wind
+    tilts the surface so that the drying front crosses the channel over
+    several steps.  The tide starts falling at t = 0."""
+    d = channel(nx=nx, ny=ny, dx=dx, depth=3.0, shallow=2.0, dz=dz, dt=dt, nts=nts, iturb=iturb)
+    d["cw"] = 1.2e-3
+    d["wa"] = math.sqrt(tau / (d["cw"] * G_RHOAIR)) if tau > 0 else 0.0
+    d["phi"] = 90.0
+    npts = int(nts * dt / 60.0) + 8
+    t = np.arange(npts) * 60.0
+    series = np.stack([-amp * np.sin(2 * np.pi * t / period), np.full(npts, 20.0)], axis=1)
+    return open_boundary(d, "W", 1, 2, ny + 1, series, dtsec=60.0)
+
+
 SIDES = {"W": 1, "N": 2, "E": 3, "S": 4}
 
 
