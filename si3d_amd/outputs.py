@@ -70,6 +70,15 @@ def fortran_e(x, w, d):
     return s.rjust(w) if len(s) <= w else "*" * w
 
 
+def hour_stamp(seconds_of_day):
+    """The hour stamp of the time and plane files, ihr = NINT(hrs*100.) with
+    hrs = isec/3600. in single precision (compute_date, si3d_utils.f90:
+    3616-3618, 3627-3628).  NINT rounds halves away from zero: at 90 s
+    (dt = 30 s, n = 3) hrs*100 = 2.5 and the stamp is 3."""
+    v = np.float32(np.float32(int(seconds_of_day)) / np.float32(3600.0)) * np.float32(100.0)
+    return int(math.floor(float(v) + 0.5))
+
+
 def _layer_fields(z, u, v, w, av, dv, sc):
     return (fortran_f(z, 9, 2) + fortran_f(u, 10, 2, 2) + fortran_f(v, 10, 2, 2) + fortran_f(w, 9, 4, 2) +
             fortran_f(av, 15, 7, 4) + fortran_f(dv, 15, 7, 4) + fortran_e(sc, 15, 5))
@@ -212,7 +221,7 @@ class PlaneFiles:
     def _stamp(self, n):
         t = self.start + datetime.timedelta(seconds=n * self.deck["dt"])
         hrs = (t - datetime.datetime(t.year, t.month, t.day)).total_seconds() / 3600.0
-        return (np.array([n, t.year, t.month, t.day], dtype=np.int32), np.array([round(hrs * 100)], dtype=np.float32))
+        return (np.array([n, t.year, t.month, t.day], dtype=np.int32), np.array([hour_stamp(hrs * 3600.0)], dtype=np.float32))
 
     def write(self, n):
         g = self.g
@@ -306,7 +315,7 @@ class ThreeDFile:
         hrs = (t - datetime.datetime(t.year, t.month, t.day)).total_seconds() / 3600.0
         ints = np.array([n, t.year, t.month, t.day], dtype=np.int32)
         data = np.stack(vals, axis=1).astype(np.float32)  # (ipoints, nvar), written point by point
-        _record(self.f, ints, np.array([round(hrs * 100)], dtype=np.float32), data)
+        _record(self.f, ints, np.array([hour_stamp(hrs * 3600.0)], dtype=np.float32), data)
 
     def close(self):
         if self.f is not None:

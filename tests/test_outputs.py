@@ -143,3 +143,15 @@ def test_three_d_file_records(oracle_lib, tmp_path, turb):
     if turb:
         np.testing.assert_array_equal(later[row:last + 1, 9], np.float32(o.get("Av")[l, 1:kmz[l] + 1]))
 
+
+
+def test_hour_stamp_rounds_halves_away_from_zero():
+    """NINT(hrs*100.) of compute_date: 90 s is 2.5 hundredths of an hour and
+    stamps 3 (Python's round() would give 2); whole hours stamp 100 per hour."""
+    from si3d_amd.outputs import hour_stamp
+    assert hour_stamp(90) == 3
+    assert hour_stamp(54) == 2          # 1.5 -> 2
+    assert hour_stamp(18) == 1          # 0.5 -> 1
+    assert hour_stamp(3600) == 100
+    assert hour_stamp(86340) == 2398    # 23.98333 h
+    assert hour_stamp(0) == 0
