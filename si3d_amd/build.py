@@ -18,6 +18,11 @@ OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT, "libsi3dgpu.so")
 SOURCES = ["s3g_kernels.hip", "s3g_pcg.hip", "s3g_host.hip", "s3g_unet.hip", "s3g_deck.hip", "s3g_openbc.hip"]
 ARCH = os.environ.get("SI3D_OFFLOAD_ARCH", "gfx950")
+# The hydrodynamic kernels are built without fused multiply-add contraction:
+# each has several instantiations (per-cell chunked or not, split or fused
+# phases) that the autotuner picks between, and they must round alike so
+# that results do not depend on the pick.
+NO_CONTRACT = {"s3g_kernels.hip", "s3g_openbc.hip"}
 
 
 def hipcc():
@@ -33,6 +38,8 @@ def _compile(src):
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
            "-Wno-unused-result", "-I", os.path.join(HERE, "..", "include"),
            os.path.join(CSRC, src), "-o", obj]
+    if src in NO_CONTRACT:
+        cmd.insert(5, "-ffp-contract=off")
     if src.endswith(".cpp"):
         cmd = [hipcc(), "-c", "-fPIC", "-O3", "-std=c++17", "-x", "c++",
                "-I", os.path.join(HERE, "..", "include"), os.path.join(CSRC, src), "-o", obj]

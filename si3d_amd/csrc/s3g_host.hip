@@ -507,6 +507,8 @@ int s3g::init_ctx(const s3g_grid *g, const s3g_params *p, int nlakes, const doub
     }
     if (const char *gs = std::getenv("S3G_GUESS")) c->pcg_guess = (gs[0] == '1') ? 1 : 0;
     if (const char *gr = std::getenv("S3G_GRAPH")) c->cg_graph = (gr[0] != '0');
+    if (const char *x = std::getenv("S3G_EXSAL")) c->exsal_split = std::strcmp(x, "fused") != 0;
+    if (const char *x = std::getenv("S3G_TURB")) c->turb_split = std::strcmp(x, "split") == 0;
     // S3G_TILES="w0,w1,...": fixed tile words (chunk | order << 8), no tuning
     if (const char *w = std::getenv("S3G_TILES")) {
       c->tune = false;
@@ -789,6 +791,9 @@ void s3g_finalize(s3g_ctx *c) {
   }
   if (c->h_iters) hipHostFree(c->h_iters);
   if (c->h_scal) hipHostFree(c->h_scal);
+  if (c->pcg.lake_off) hipFree(c->pcg.lake_off);
+  if (c->pcg.lake_part) hipFree(c->pcg.lake_part);
+  if (c->pcg.h_lake) hipHostFree(c->pcg.h_lake);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }

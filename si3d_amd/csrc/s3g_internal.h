@@ -101,6 +101,11 @@ struct Pcg {
   double *lc, *ld;  // modified super-diagonal and inverse pivots
   float *lcf, *ldf;
   std::vector<MgLevel> mg;  // mg[0] unused (the fine grid), mg[1..] coarse levels
+  // ensembles: each lake's (r,r) and (b,b) of the true residual, so that every
+  // lake meets the tolerance against its own right-hand side
+  int *lake_off = nullptr;      // [nlakes+1] on the device
+  double *lake_part = nullptr;  // [nlakes][kLakeSlices][2]
+  double *h_lake = nullptr;     // pinned copy of lake_part
 };
 
 // FP64 view of the free-surface system for the outer solver level (true
@@ -222,7 +227,7 @@ struct s3g_ctx {
   int cg_hint[4] = {0, 0, 0, 0};
   hipGraphExec_t cg_exec[2] = {nullptr, nullptr};
   std::vector<char> cg_key[2];
-  bool turb_split = true;  // turbulence closure: per-cell interface terms + sweep
+  bool turb_split = false;  // turbulence closure: per-cell interface terms + sweep (else one fused sweep)
   bool tune = true, tuned = false;
   // ensemble of lakes (s3g_create_ensemble): column offset of each lake
   // ([nlakes+1]), each lake's km1 and forcing parameters

@@ -1816,36 +1816,13 @@ int autotune(s3g_ctx *c) {
     snapshot(outs, true);
     if (c->debug) std::fprintf(stderr, "[s3g] tile %d: kc=%d order=%d (%.3f ms)\n", tid, best_kc & 255, best_kc >> 8, best);
   }
-  // split or fused exsal/imsal
-  if (c->P.isal != 0) {
-    const std::vector<Out> outs = {{D.sal, bR}, {D.t0, bR}, {D.bex, bR}};
-    snapshot(outs, false);
-    const float split = time_it([&]() {
-      LAUNCHC("", (k_exsal_rhs<R, false, false>), (k_exsal_rhs<R, true, false>), TILE_RHS, L, c->km - 1, D, sal_of(D), twodt1);
-      LAUNCH("", (k_exsal_imsal<R, false, true, false>), L, D, sal_of(D), twodt1);
-    }, 3);
-    const float fused = time_it([&]() { LAUNCH("", (k_exsal_imsal<R, true, true, false>), L, D, sal_of(D), twodt1); }, 3);
-    c->exsal_split = split <= fused;
-    snapshot(outs, true);
-    if (c->debug) std::fprintf(stderr, "[s3g] exsal: split %.3f ms, fused %.3f ms\n", split, fused);
-  }
-  // split or fused turbulence closure
-  if (c->P.iturb >= 1) {
-    const std::vector<Out> outs = {{D.q2, bD}, {D.q2l, bD}, {D.Av, bR}, {D.Dv, bR}};
-    snapshot(outs, false);
-    const float split = time_it([&]() {
-      LAUNCHC("", (k_turb_cell<R, false>), (k_turb_cell<R, true>), TILE_TURB, L, c->km1, D);
-      LAUNCH("", (k_turb<R, true>), L, D, 1);
-      snapshot(outs, true);  // the closure updates Av/Dv in place: every run starts from the saved state
-    }, 3);
-    const float fused = time_it([&]() {
-      LAUNCH("", (k_turb<R, false>), L, D, 1);
-      snapshot(outs, true);
-    }, 3);
-    c->turb_split = split <= fused;
-    snapshot(outs, true);
-    if (c->debug) std::fprintf(stderr, "[s3g] turbulence: split %.3f ms, fused %.3f ms (each with the state restore)\n", split, fused);
-  }
+  // The split or fused forms of exsal/imsal and of the turbulence closure are
+  // not chosen by timing: the split exsal stores Bex in the state precision
+  // (FP32 in mixed mode) where the fused one keeps it in registers, so the two
+  // round differently, and results must not depend on which was faster on a
+  // given run.  They are fixed per context (split exsal, fused closure, the
+  // faster pair on every lake measured) and can be set with S3G_EXSAL /
+  // S3G_TURB = split | fused.
   const int e = hip_check(hipStreamSynchronize(c->stream), "autotune");
   hipEventDestroy(e0);
   hipEventDestroy(e1);

@@ -510,6 +510,31 @@ __global__ void __launch_bounds__(kBlock) k_true_residual(Outer O, T *rin, T *d,
   }
 }
 
+// Per-lake true residual of an ensemble: workgroup (slice, lake) sums r^2 and
+// b^2 over its share of the lake's contiguous columns off[lake] ..
+// off[lake+1]-1 into out[(lake * slices + slice) * 2 + {0, 1}].
+constexpr int kLakeSlices = 8;
+__global__ void __launch_bounds__(kBlock) k_lake_residual(Outer O, const int *off, double *out) {
+  __shared__ double lds[16];
+  const int lake = blockIdx.y;
+  const int a = off[lake], e = off[lake + 1];
+  double rr = 0.0, bb = 0.0;
+  for (int c = a + blockIdx.x * blockDim.x + threadIdx.x; c < e; c += gridDim.x * blockDim.x) {
+    const int lW = O.lW[c], lS = O.lS[c], lN = O.lN[c], lE = O.lE[c];
+    const double b = O.b[c];
+    const double r = b - (O.rr[c] * O.x[c] - O.sx[lW] * O.x[lW] - O.sy[lS] * O.x[lS] - O.sy[c] * O.x[lN] -
+                          O.sx[c] * O.x[lE]);
+    rr += r * r;
+    bb += b * b;
+  }
+  rr = block_sum(rr, lds);
+  bb = block_sum(bb, lds);
+  if (threadIdx.x == 0) {
+    out[((size_t)lake * gridDim.x + blockIdx.x) * 2] = rr;
+    out[((size_t)lake * gridDim.x + blockIdx.x) * 2 + 1] = bb;
+  }
+}
+
 // x += d (FP64 accumulation of the inner correction).
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_add_correction(double *x, const T *d, int n) {
@@ -882,6 +907,39 @@ static CgArgs<T> cg_args(s3g_ctx *c, const int *lE, const int *lW, const int *lN
   return A;
 }
 
+// Ensembles: enqueue the per-lake residual sums and their copy to the host.
+static int lake_residual_enqueue(s3g_ctx *c, const Outer &O) {
+  Pcg &P = c->pcg;
+  const int nl = c->nlakes;
+  if (!P.lake_off) {
+    if (hipMalloc(&P.lake_off, sizeof(int) * (nl + 1)) != hipSuccess ||
+        hipMalloc(&P.lake_part, sizeof(double) * 2 * kLakeSlices * nl) != hipSuccess ||
+        hipHostMalloc((void **)&P.h_lake, sizeof(double) * 2 * kLakeSlices * nl, hipHostMallocDefault) != hipSuccess)
+      return hip_check(hipErrorOutOfMemory, "ensemble residual buffers");
+    hipMemcpy(P.lake_off, c->lake_off.data(), sizeof(int) * (nl + 1), hipMemcpyHostToDevice);
+  }
+  hipLaunchKernelGGL(k_lake_residual, dim3(kLakeSlices, nl), dim3(kBlock), 0, c->stream, O, P.lake_off, P.lake_part);
+  hipMemcpyAsync(P.h_lake, P.lake_part, sizeof(double) * 2 * kLakeSlices * nl, hipMemcpyDeviceToHost, c->stream);
+  return 0;
+}
+
+// After the stream synchronised: the largest relative true residual of the
+// lakes (the absolute one for a lake with a zero right-hand side).
+static double lake_relres(const s3g_ctx *c) {
+  double worst = 0.0;
+  for (int i = 0; i < c->nlakes; ++i) {
+    double rr = 0.0, bb = 0.0;
+    for (int k = 0; k < kLakeSlices; ++k) {
+      rr += c->pcg.h_lake[(i * kLakeSlices + k) * 2];
+      bb += c->pcg.h_lake[(i * kLakeSlices + k) * 2 + 1];
+    }
+    const double r = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+    worst = std::isfinite(r) ? std::max(worst, r) : r;
+    if (!std::isfinite(r)) break;
+  }
+  return worst;
+}
+
 template <typename R>
 int solve_free_surface(s3g_ctx *c, double tz) {
   Pcg &P = c->pcg;
@@ -909,11 +967,13 @@ int solve_free_surface(s3g_ctx *c, double tz) {
                                        D.qq, P.lc, P.ld, 0.5 * tol);
     if (line) hipLaunchKernelGGL((k_line_factor<double>), gl, b, 0, c->stream, A, P.lc, P.ld);
     if (c->prec_kind == S3G_PREC_MG) mg_build(c, A);
+    const bool lakes = c->nlakes > 1;
     auto true_res = [&]() {  // behind each chunk: FP64 true residual, partials to the host
       hipLaunchKernelGGL((k_true_residual<double, false>), g, b, 0, c->stream, O, nullptr, nullptr, nullptr,
                          nullptr, nullptr);
       hipMemcpyAsync(c->h_scal, P.part + P_BB * P.nblk, 2 * P.nblk * sizeof(double), hipMemcpyDeviceToHost,
                      c->stream);
+      if (lakes) lake_residual_enqueue(c, O);
     };
     for (int restart = 0; restart < 3; ++restart) {
       int it = run_cg(c, A, c->pcg_maxit - total, hint, true_res);
@@ -922,6 +982,7 @@ int solve_free_surface(s3g_ctx *c, double tz) {
       double tr = 0.0, bb = 0.0;
       if (!true_residual_sums(c, bb, tr, true)) return S3G_ERR_HIP;
       relres = bb > 0 ? std::sqrt(tr / bb) : std::sqrt(tr);
+      if (lakes && std::isfinite(relres)) relres = lake_relres(c);
       if (relres <= tol || !std::isfinite(relres)) break;
       if (total >= c->pcg_maxit) {
         rc = S3G_WARN_ITMAX;
@@ -938,9 +999,11 @@ int solve_free_surface(s3g_ctx *c, double tz) {
     if (c->prec_kind == S3G_PREC_MG) mg_build(c, A);
     for (int sweep = 0; sweep < 12; ++sweep) {
       hipLaunchKernelGGL((k_true_residual<float, true>), g, b, 0, c->stream, O, P.bf, P.df, P.pf0, P.pf1, P.zf);
+      if (c->nlakes > 1 && lake_residual_enqueue(c, O) < 0) return S3G_ERR_HIP;
       double tr = 0.0, bb = 0.0;
       if (!true_residual_sums(c, bb, tr, false)) return S3G_ERR_HIP;
       relres = bb > 0 ? std::sqrt(tr / bb) : std::sqrt(tr);
+      if (c->nlakes > 1 && std::isfinite(relres)) relres = lake_relres(c);
       if (relres <= tol || !std::isfinite(relres)) break;
       if (total >= c->pcg_maxit) {
         rc = S3G_WARN_ITMAX;

@@ -47,6 +47,37 @@ def test_ensemble_matches_lakes_run_alone(precision, tol):
             assert e < tol, (i, name, e)
 
 
+def test_ensemble_lakes_match_oracle(oracle_lib):
+    """Every lake of the ensemble against the FP64 oracle run on that lake's
+    own deck: integer state identical, fields within 1e-8 after 5 steps.
+    The ensemble's free-surface solve stops only when each lake's true
+    residual is below the tolerance relative to that lake's right-hand side,
+    so a small, weakly forced lake is solved as tightly as a large one."""
+    from si3d_amd.model import GpuEnsemble
+    from tests.helpers import FIELDS_2D, FIELDS_3D, FIELDS_TURB
+    decks = _decks()
+    for d in decks:
+        d["pcg_tol"] = 1e-13
+    print("case: ensemble of 3 lakes against the oracle", flush=True)
+    E = GpuEnsemble(decks, precision="fp64", preconditioner="mg", tol=1e-13)
+    orcs = [oracle_lib.Oracle(d) for d in decks]
+    for n in range(1, 6):
+        E.step(n)
+        for o in orcs:
+            assert o.step(n) == 0
+    assert E.stats()["pcg_relres_last"] <= 1e-13
+    for i, o in enumerate(orcs):
+        lm, km1 = o.lm, o.km1
+        for name in INTS:
+            np.testing.assert_array_equal(E.lake_get(name, i), o.get(name)[:lm].astype(np.int32),
+                                          err_msg=f"lake {i} {name}")
+        for name in FIELDS_2D + FIELDS_3D + FIELDS_TURB:
+            ref = o.get(name)
+            ref = ref[:lm] if ref.ndim == 1 else ref[:lm, :km1]
+            e = relerr(E.lake_get(name, i), ref)
+            assert e < 1e-8, (i, name, e)
+
+
 def test_ensemble_rejects_mismatched_time_steps():
     from si3d_amd.model import GpuEnsemble
     decks = _decks()
