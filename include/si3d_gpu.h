@@ -138,6 +138,16 @@ int s3g_run(s3g_ctx *c, int n0, int nsteps);
  * computes the explicit transport terms, phase 10 the implicit solve. */
 int s3g_phase(s3g_ctx *c, int phase, int n, int istep, double thrs);
 
+/* Basin energy budget of the turbulence closure, the TKE columns of
+ * ScalarBalance.txt (OutScalarEnergyBalance, si3d_utils.f90:3698-3814, from
+ * the sums SolveTKEandLengthScale keeps, si3d_mixing.f90:333-338, 466-475,
+ * 539-548): out[0] TKinE (sum of q2 (rho+1000) h over the interfaces of the
+ * latest trapezoidal call), out[1..3] ShearCum, BuocyCum, DisspCum (shear
+ * and buoyancy production and dissipation, times dx dy dt, added on each
+ * leapfrog call).  reset != 0 then clears the three integrals, as the
+ * reference does after writing them.  Needs iturb >= 1 and one lake. */
+int s3g_energy_balance(s3g_ctx *c, double *out, int reset);
+
 int s3g_synchronize(s3g_ctx *c);
 int s3g_get_stats(s3g_ctx *c, s3g_stats *out);
 /* Average device duration (ms) of each timed kernel class since the last

@@ -131,6 +131,12 @@ static double parab(double frstpt, double x, const double *fx /*1-based*/, doubl
 }
 
 struct OrcLake {
+  // basin energy budget of SolveTKEandLengthScale (si3d_mixing.f90:465-474,
+  // 536-548): tke[0..3] = the per-call sums ShearProduction,
+  // BuoyancyProduction, Dissipation, TKinE (one thread's private values, i.e.
+  // the whole basin for a single-threaded run); tke[4..6] = ShearCum,
+  // BuocyCum, DisspCum, cleared by the writer of ScalarBalance.txt
+  double tke[7] = {0, 0, 0, 0, 0, 0, 0};
   // ---- parameters (si3d_types.f90:67-90, derived in si3d_utils.f90:299-310)
   OrcDeck d{};
   std::vector<double> zfile, profile, sbc;
@@ -1451,6 +1457,8 @@ void OrcLake::turbulence(int istep) {
   std::vector<double> zfromb(NK), zfromt(NK), xl(NK), N2(NK), ShearP(NK), BuoyP(NK), dco(NK), a1(NK),
       a2(NK), a3(NK), ds(NK), x(NK), dz(NK);
   const double b1_23 = std::pow(B_1, 2. / 3.);
+  if (istep > 1)
+    for (int i = 0; i < 4; ++i) tke[i] = 0.0;  // 333-338
   for (int m = 1; m <= lm; ++m) {
     int kms = kmz[m], k1s = k1z[m], nwl = kms - k1s + 1;
     if (nwl <= 1) continue;
@@ -1517,6 +1525,14 @@ void OrcLake::turbulence(int istep) {
     trid_ef(nwl + 1, &a1[k1s], &a2[k1s], &a3[k1s], &ds[k1s], &x[k1s]);
     for (int k = k1s; k <= kms + 1; ++k) q2(k, m) = std::max(Q2_MIN, x[k]);
     for (int k = k1; k <= k1s - 1; ++k) q2(k, m) = q2(k1s, m);
+    if (istep > 1)  // 466-475
+      for (int k = k1s + 1; k <= kms; ++k) {
+        const double rhoxh = (rhop(k, m) + 1000.) * hp(k, m);
+        tke[0] += ShearP[k] * rhoxh;
+        tke[1] += BuoyP[k] * rhoxh;
+        tke[2] += q2(k, m) * std::sqrt(q2p(k, m)) / (B_1 * xl[k]) * rhoxh;
+        tke[3] += q2(k, m) * rhoxh;
+      }
     // length scale (488-537)
     for (int k = k1s + 1; k <= kms; ++k) {
       double r1 = xl[k] / (KAPPAS * zfromb[k]), r2 = xl[k] / (KAPPAS * zfromt[k]);
@@ -1547,6 +1563,11 @@ void OrcLake::turbulence(int istep) {
     trid_ef(nwl + 1, &a1[k1s], &a2[k1s], &a3[k1s], &ds[k1s], &x[k1s]);
     for (int k = k1s; k <= kms + 1; ++k) q2l(k, m) = std::max(Q2L_MIN, x[k]);
     for (int k = k1; k <= k1s - 1; ++k) q2l(k, m) = q2l(k1s, m);
+  }
+  if (istep == 1) {  // 539-548
+    const double a = dx * dy * dt;
+    for (int i = 0; i < 3; ++i) tke[i] *= a, tke[4 + i] += tke[i];
+    tke[3] = 0.5 * tke[3] * a;
   }
   // eddy coefficients from q2p, q2lp (UpdateMixingCoefficients 245-290)
   for (int ll = 1; ll <= lm; ++ll) {
@@ -2343,6 +2364,10 @@ long orc_get(const OrcLake *o, const char *name, double *out) {
     out[0] = o->thrs;
     return 1;
   }
+  if (n == "energy_budget") {
+    for (int i = 0; i < 7; ++i) out[i] = o->tke[i];
+    return 7;
+  }
   return -1;
 }
 
@@ -2361,6 +2386,10 @@ long orc_set(OrcLake *o, const char *name, const double *in) {
   if (auto *a = fieldi(o, n)) {
     for (int l = 1; l <= o->lm1; ++l) (*a)[l] = (int)in[l - 1];
     return o->lm1;
+  }
+  if (n == "energy_budget") {
+    for (int i = 0; i < 7; ++i) o->tke[i] = in[i];
+    return 7;
   }
   return -1;
 }

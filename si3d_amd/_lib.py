@@ -22,7 +22,7 @@ EXPORTS = [
     "s3g_finalize", "s3g_last_error", "s3g_create_from_deck",
     "s3g_learn_setup", "s3g_learn_record", "s3g_learn_train", "s3g_learn_mode", "s3g_learn_num_weights",
     "s3g_learn_get_weights", "s3g_learn_set_weights", "s3g_learn_info", "s3g_unet_check",
-    "s3g_create_ensemble", "s3g_ensemble_info",
+    "s3g_create_ensemble", "s3g_ensemble_info", "s3g_energy_balance", "s3g_set_open_boundaries",
 ]
 
 dp = C.POINTER(C.c_double)
@@ -102,6 +102,7 @@ def load():
     L.s3g_phase.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_double]
     L.s3g_synchronize.argtypes = [vp]
     L.s3g_get_stats.argtypes = [vp, C.POINTER(S3gStats)]
+    L.s3g_energy_balance.argtypes = [vp, dp, C.c_int]
     L.s3g_kernel_times.argtypes = [vp, C.c_int, C.c_void_p, dp, C.POINTER(C.c_long)]
     L.s3g_reset_timers.argtypes = [vp, C.c_int]
     L.s3g_get_tiles.argtypes = [vp, C.c_int, ip]

@@ -431,6 +431,13 @@ static int alloc_state(s3g_ctx *c, Dev<R> &D, const int *lE, const int *lW, cons
     const size_t n = (c->P.iturb > 0 ? n3 : 1) * ((p == &D.d0 || p == &D.d1) ? 2 : 1);
     if (!(*p = dalloc<double>(c, n))) return S3G_ERR_HIP;
   }
+  D.tkb = nullptr;
+  D.tke_part = D.tke = nullptr;
+  if (c->P.iturb > 0) {
+    if (!(D.tkb = dalloc<double2>(c, n3)) || !(D.tke_part = dalloc<double>(c, 4 * ((size_t)c->lm / kBlock + 1))) ||
+        !(D.tke = dalloc<double>(c, 8)))
+      return S3G_ERR_HIP;
+  }
   std::vector<double> one(n2, 1.0);
   hipMemcpy(D.rr, one.data(), n2 * sizeof(double), hipMemcpyHostToDevice);
   return hip_check(hipGetLastError(), "alloc");
@@ -509,6 +516,7 @@ int s3g::init_ctx(const s3g_grid *g, const s3g_params *p, int nlakes, const doub
     if (const char *gr = std::getenv("S3G_GRAPH")) c->cg_graph = (gr[0] != '0');
     if (const char *x = std::getenv("S3G_EXSAL")) c->exsal_split = std::strcmp(x, "fused") != 0;
     if (const char *x = std::getenv("S3G_TURB")) c->turb_split = std::strcmp(x, "split") == 0;
+    if (const char *x = std::getenv("S3G_BUDGET")) c->tke_budget = x[0] != '0';
     // S3G_TILES="w0,w1,...": fixed tile words (chunk | order << 8), no tuning
     if (const char *w = std::getenv("S3G_TILES")) {
       c->tune = false;
@@ -748,6 +756,31 @@ int s3g_get_stats(s3g_ctx *c, s3g_stats *out) {
   for (int l = 0; l < c->lm; ++l) v += s[l];
   c->stats.zeta_volume = v;
   *out = c->stats;
+  return 0;
+}
+
+int s3g_energy_balance(s3g_ctx *c, double *out, int reset) {
+  if (!c || !out) return S3G_ERR_ARG;
+  if (c->P.iturb < 1 || c->nlakes != 1) {
+    set_error("s3g_energy_balance: the basin energy budget needs the turbulence closure (iturb >= 1) and one lake");
+    return S3G_ERR_UNSUPPORTED;
+  }
+  const double *tke = c->precision == S3G_FP64 ? c->d64.tke : c->d32.tke;
+  double v[7];
+  if (int e = hip_check(hipMemcpyAsync(v, tke, sizeof v, hipMemcpyDeviceToHost, c->stream), "energy balance"))
+    return e;
+  if (int e = hip_check(hipStreamSynchronize(c->stream), "energy balance")) return e;
+  out[0] = v[3];
+  out[1] = v[4];
+  out[2] = v[5];
+  out[3] = v[6];
+  if (reset) {  // OutScalarEnergyBalance clears the integrals after writing them (si3d_utils.f90:3809-3811)
+    static const double zero[3] = {0.0, 0.0, 0.0};
+    if (int e = hip_check(hipMemcpyAsync(const_cast<double *>(tke) + 4, zero, sizeof zero, hipMemcpyHostToDevice,
+                                         c->stream), "energy balance reset"))
+      return e;
+    return hip_check(hipStreamSynchronize(c->stream), "energy balance reset");
+  }
   return 0;
 }
 

@@ -73,6 +73,12 @@ struct Dev {
   int ntr;
   R *heat;  // HeatSource (surface heat budget, ifsurfbc 1-3), null otherwise
   double *q2, *q2p, *q2pp, *q2l, *q2lp, *q2lpp, *d0, *d1, *d2;
+  // basin energy budget of the closure (iturb >= 1, one lake): per-level
+  // (dissipation rate x rho h, rho h) of the running call, per-workgroup
+  // partial sums, and the budget state: the call's ShearProduction,
+  // BuoyancyProduction, Dissipation, TKinE, then ShearCum, BuocyCum, DisspCum
+  double2 *tkb;
+  double *tke_part, *tke;
 };
 
 // One level of the aggregation multigrid preconditioner (levels >= 1; level 0
@@ -227,6 +233,7 @@ struct s3g_ctx {
   int cg_hint[4] = {0, 0, 0, 0};
   hipGraphExec_t cg_exec[2] = {nullptr, nullptr};
   std::vector<char> cg_key[2];
+  bool tke_budget = true;   // accumulate the closure's basin energy budget (S3G_BUDGET=0: off)
   bool turb_split = false;  // turbulence closure: per-cell interface terms + sweep (else one fused sweep)
   bool tune = true, tuned = false;
   // ensemble of lakes (s3g_create_ensemble): column offset of each lake

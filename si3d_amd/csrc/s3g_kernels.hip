@@ -1090,12 +1090,20 @@ __global__ void __launch_bounds__(kBlock) k_turb_cell(Dev<R> D, int nk, int tile
 // SPLIT: the interface quantities and Av/Dv come from k_turb_cell (the sweep
 // reads them from the scratch planes and writes its factors over them);
 // otherwise they are computed one level ahead inside the sweep.
+//
+// With `budget` (trapezoidal calls, si3d_mixing.f90:466-475) the column also
+// returns its share of the basin energy budget over levels k1s+1..kms, each
+// weighted by rho h = (rhop + 1000) hp: shear and buoyancy production,
+// dissipation q2 sqrt(q2p) / (B1 l) and q2, the new q2 after its floor.
+struct TkeSums {
+  double sh, by, di, tk;
+};
+
 template <typename R, bool SPLIT>
-__global__ void __launch_bounds__(kBlock) k_turb(Dev<R> D, int istep) {
-  const int m = col();
-  if (m >= D.lm) return;
+__device__ __forceinline__ TkeSums turb_column(const Dev<R> &D, int m, int istep, bool budget) {
+  TkeSums bs{0.0, 0.0, 0.0, 0.0};
   const int kms = D.kmz[m], k1s = D.k1z[m], nwl = kms - k1s + 1;
-  if (nwl <= 1) return;
+  if (nwl <= 1) return bs;
   const double twodt1 = D.P.twodt / istep, r3 = 3. / twodt1;
   const int lW = D.lW[m], lS = D.lS[m];
   // forward-sweep factors of the two systems: split, E1/E2 in d0/d1 and the
@@ -1214,6 +1222,12 @@ __global__ void __launch_bounds__(kBlock) k_turb(Dev<R> D, int istep) {
       EP[IX(k, m)] = make_double2(e1, e2);
       FP[IX(k, m)] = make_double2(f1, f2);
     }
+    if (budget) {
+      const double rhoxh = ((SPLIT ? (double)D.rhop[IX(k, m)] : rcur.rho) + 1000.) * hk;
+      bs.sh += shp * rhoxh;
+      bs.by += byp * rhoxh;
+      D.tkb[IX(k, m)] = make_double2(sq / (B_1 * xlk) * rhoxh, rhoxh);
+    }
     // new eddy coefficients of interface k (the surface minimum, 281-284)
     if (!SPLIT) {
       double av = cur.av, dv = cur.dv;
@@ -1239,8 +1253,14 @@ __global__ void __launch_bounds__(kBlock) k_turb(Dev<R> D, int istep) {
       x1 = e.x * x1 + f.x;
       x2 = e.y * x2 + f.y;
     }
-    D.q2[IX(k, m)] = dmax(Q2_MIN, x1);
+    const double q2n = dmax(Q2_MIN, x1);
+    D.q2[IX(k, m)] = q2n;
     D.q2l[IX(k, m)] = dmax(Q2L_MIN, x2);
+    if (budget && k > k1s) {
+      const double2 w = D.tkb[IX(k, m)];
+      bs.di += q2n * w.x;
+      bs.tk += q2n * w.y;
+    }
   }
   for (int k = D.k1; k <= k1s - 1; ++k) {
     D.q2[IX(k, m)] = D.q2[IX(k1s, m)];
@@ -1249,6 +1269,64 @@ __global__ void __launch_bounds__(kBlock) k_turb(Dev<R> D, int istep) {
   if (!SPLIT) {
     for (int k = 1; k <= k1s; ++k) D.Av[IX(k, m)] = D.Dv[IX(k, m)] = R(0);
     for (int k = kms + 1; k <= D.km1; ++k) D.Av[IX(k, m)] = D.Dv[IX(k, m)] = R(0);
+  }
+  return bs;
+}
+
+// Sum of v over the workgroup in a fixed order (wavefront shuffles, then the
+// wavefronts' sums in LDS added by lane 0 of the workgroup).
+__device__ __forceinline__ double wg_sum(double v, double *lds) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) lds[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < kBlock / 64; ++i) t += lds[i];
+  return t;
+}
+
+// One lane per column (every lane reaches the workgroup reduction, so the
+// columns without work contribute zeros); with `budget` each workgroup
+// writes its four sums to tke_part[4 blockIdx.x ..].
+template <typename R, bool SPLIT>
+__global__ void __launch_bounds__(kBlock) k_turb(Dev<R> D, int istep, int budget) {
+  __shared__ double lds[kBlock / 64];
+  const int m = col();
+  TkeSums bs{0.0, 0.0, 0.0, 0.0};
+  if (m < D.lm) bs = turb_column<R, SPLIT>(D, m, istep, budget != 0);
+  if (budget) {
+    const double v[4] = {bs.sh, bs.by, bs.di, bs.tk};
+    for (int i = 0; i < 4; ++i) {
+      const double t = wg_sum(v[i], lds);
+      if (threadIdx.x == 0) D.tke_part[4 * blockIdx.x + i] = t;
+    }
+  }
+}
+
+// The budget state after a closure call (si3d_mixing.f90:333-338, 466-475,
+// 539-548): a trapezoidal call replaces the four sums by this call's, added
+// over the workgroups in a fixed order; a leapfrog call scales them by
+// dx dy dt (TKinE by dx dy dt / 2) and adds the first three to the
+// cumulative ShearCum, BuocyCum, DisspCum.  One workgroup.
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_tke_budget(Dev<R> D, int nblk, int istep) {
+  __shared__ double lds[kBlock / 64];
+  if (istep > 1) {
+    for (int i = 0; i < 4; ++i) {
+      double v = 0.0;
+      for (int b = threadIdx.x; b < nblk; b += kBlock) v += D.tke_part[4 * b + i];
+      const double t = wg_sum(v, lds);
+      if (threadIdx.x == 0) D.tke[i] = t;
+    }
+  } else if (threadIdx.x == 0) {
+    const double a = D.P.dx * D.P.dy * D.P.dt;
+    for (int i = 0; i < 3; ++i) {
+      D.tke[i] *= a;
+      D.tke[4 + i] += D.tke[i];
+    }
+    D.tke[3] = 0.5 * D.tke[3] * a;
   }
 }
 
@@ -1632,12 +1710,16 @@ template <typename R>
 static void turbulence(s3g_ctx *c, int istep) {
   Dev<R> &D = dev<R>(c);
   const int L = c->lm;
+  // the basin energy budget is a single lake's (an ensemble keeps none)
+  const bool track = c->tke_budget && c->nlakes == 1;
+  const int budget = (track && istep > 1) ? 1 : 0;
   if (c->turb_split) {
     LAUNCHC("turb_cell", (k_turb_cell<R, false>), (k_turb_cell<R, true>), TILE_TURB, L, c->km1, D);
-    LAUNCH("turbulence", (k_turb<R, true>), L, D, istep);
+    LAUNCH("turbulence", (k_turb<R, true>), L, D, istep, budget);
   } else {
-    LAUNCH("turbulence", (k_turb<R, false>), L, D, istep);
+    LAUNCH("turbulence", (k_turb<R, false>), L, D, istep, budget);
   }
+  if (track) hipLaunchKernelGGL((k_tke_budget<R>), dim3(1), dim3(kBlock), 0, c->stream, D, (int)grid_for(L).x, istep);
 }
 
 // surfbc on the host, then HeatSource on the device when the heat budget is

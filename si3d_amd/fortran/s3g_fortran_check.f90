@@ -49,3 +49,93 @@ FUNCTION s3g_fortran_check(nsteps, s_out, uh_out) BIND(C, NAME="s3g_fortran_chec
    IF (rc >= 0) rc = s3g_get_field_f64(ctx, cstr('uh'), uh_out)
    CALL s3g_finalize(ctx)
 END FUNCTION s3g_fortran_check
+
+! The drop-in hand-over and time loop of the patched si3d.f90, as
+! INTEGRATION.md shows it: the grid left by bathy / init (1-based index arrays
+! of the reference layout, REAL zlevel / hhs / hhu / hhv) goes to s3g_init;
+! every REAL state array goes over with s3g_set_field_f32 and the REAL*8
+! turbulence arrays with s3g_set_field_f64; then each time step replaces the
+! driver's fd / settrap / settrap2 / save calls (si3d.f90:324-393, itrap = 1:
+! a leapfrog fd and settrap from step 2 on, then niter trapezoidal fd with
+! settrap2 between them, then save(2)).  After nsteps steps zeta, uh and the
+! temperature come back with s3g_get_field_f32 and q2 with
+! s3g_get_field_f64, as before the driver's output routines.
+!   f2d(lm1, 3):        s, sp, spp
+!   f3d(km1, lm1, 28):  h hp hpp hu hup hupp hv hvp hvpp uh uhp uhpp u up upp
+!                       vh vhp vhpp v vp vpp wp sal salp salpp rhop Av Dv
+!   f8(km1, lm1, 6):    q2 q2p q2pp q2l q2lp q2lpp
+FUNCTION s3g_fortran_dropin(im1, jm1, km1, lm, l2i, l2j, lEC, lWC, lNC, lSC, kmz, k1z, k1u, k1v, &
+                            zlevel, hhs, hhu, hhv, prm, nsteps, f2d, f3d, f8, s_out, uh_out, sal_out, q2_out) &
+                            BIND(C, NAME="s3g_fortran_dropin") RESULT(rc)
+   USE ISO_C_BINDING
+   USE si3d_gpu
+   IMPLICIT NONE
+   INTEGER(C_INT), VALUE :: im1, jm1, km1, lm, nsteps
+   INTEGER(C_INT), DIMENSION(lm + 1), TARGET, INTENT(IN) :: l2i, l2j, lEC, lWC, lNC, lSC, kmz, k1z, k1u, k1v
+   REAL(C_FLOAT), DIMENSION(km1), TARGET, INTENT(IN) :: zlevel
+   REAL(C_FLOAT), DIMENSION(lm + 1), TARGET, INTENT(IN) :: hhs, hhu, hhv
+   TYPE(s3g_params), INTENT(IN) :: prm
+   REAL(C_FLOAT), INTENT(IN) :: f2d(lm + 1, 3), f3d(km1, lm + 1, 28)
+   REAL(C_DOUBLE), INTENT(IN) :: f8(km1, lm + 1, 6)
+   REAL(C_FLOAT), INTENT(OUT) :: s_out(lm + 1), uh_out(km1, lm + 1), sal_out(km1, lm + 1)
+   REAL(C_DOUBLE), INTENT(OUT) :: q2_out(km1, lm + 1)
+   INTEGER(C_INT) :: rc
+   CHARACTER(LEN=5), PARAMETER :: n2d(3) = [CHARACTER(LEN=5) :: 's', 'sp', 'spp']
+   CHARACTER(LEN=5), PARAMETER :: n3d(28) = [CHARACTER(LEN=5) :: 'h', 'hp', 'hpp', 'hu', 'hup', 'hupp', &
+      'hv', 'hvp', 'hvpp', 'uh', 'uhp', 'uhpp', 'u', 'up', 'upp', 'vh', 'vhp', 'vhpp', 'v', 'vp', 'vpp', &
+      'wp', 'sal', 'salp', 'salpp', 'rhop', 'Av', 'Dv']
+   CHARACTER(LEN=5), PARAMETER :: n8(6) = [CHARACTER(LEN=5) :: 'q2', 'q2p', 'q2pp', 'q2l', 'q2lp', 'q2lpp']
+   TYPE(s3g_grid) :: grid
+   TYPE(C_PTR) :: ctx
+   INTEGER :: i, n, iter, niter1, lastiter
+   REAL(C_DOUBLE) :: thrs
+
+   grid%im1 = im1; grid%jm1 = jm1; grid%km1 = km1; grid%lm = lm
+   grid%l2i = C_LOC(l2i); grid%l2j = C_LOC(l2j)
+   grid%lEC = C_LOC(lEC); grid%lWC = C_LOC(lWC); grid%lNC = C_LOC(lNC); grid%lSC = C_LOC(lSC)
+   grid%kmz = C_LOC(kmz); grid%k1z = C_LOC(k1z); grid%k1u = C_LOC(k1u); grid%k1v = C_LOC(k1v)
+   grid%zlevel = C_LOC(zlevel); grid%hhs = C_LOC(hhs); grid%hhu = C_LOC(hhu); grid%hhv = C_LOC(hhv)
+   rc = s3g_init(grid, prm, ctx)
+   IF (rc < 0) RETURN
+   DO i = 1, 3
+      rc = s3g_set_field_f32(ctx, cstr(n2d(i)), f2d(:, i))
+      IF (rc < 0) GOTO 90
+   END DO
+   DO i = 1, 28
+      rc = s3g_set_field_f32(ctx, cstr(n3d(i)), f3d(:, :, i))
+      IF (rc < 0) GOTO 90
+   END DO
+   DO i = 1, 6
+      rc = s3g_set_field_f64(ctx, cstr(n8(i)), f8(:, :, i))
+      IF (rc < 0) GOTO 90
+   END DO
+
+   DO n = 1, nsteps
+      thrs = n * prm%dt / 3600.0D0
+      niter1 = prm%niter
+      IF (n == 1) niter1 = MAX(prm%niter, 2)
+      lastiter = 0
+      IF (n > 1) THEN
+         rc = s3g_fd(ctx, n, 1, 0, lastiter, thrs)
+         IF (rc < 0) GOTO 90
+         rc = s3g_settrap(ctx)
+         IF (rc < 0) GOTO 90
+      END IF
+      DO iter = 1, niter1
+         IF (iter == niter1) lastiter = 1
+         rc = s3g_fd(ctx, n, 2, iter, lastiter, thrs)
+         IF (rc < 0) GOTO 90
+         IF (iter < niter1) THEN
+            rc = s3g_settrap2(ctx)
+            IF (rc < 0) GOTO 90
+         END IF
+      END DO
+      rc = s3g_save(ctx, 2)
+      IF (rc < 0) GOTO 90
+   END DO
+   rc = s3g_get_field_f32(ctx, cstr('s'), s_out)
+   IF (rc >= 0) rc = s3g_get_field_f32(ctx, cstr('uh'), uh_out)
+   IF (rc >= 0) rc = s3g_get_field_f32(ctx, cstr('sal'), sal_out)
+   IF (rc >= 0) rc = s3g_get_field_f64(ctx, cstr('q2'), q2_out)
+90 CALL s3g_finalize(ctx)
+END FUNCTION s3g_fortran_dropin

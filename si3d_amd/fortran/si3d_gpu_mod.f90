@@ -160,6 +160,15 @@ MODULE si3d_gpu
          INTEGER(C_INT) :: s3g_step
       END FUNCTION s3g_step
 
+      ! TKinE, ShearCum, BuocyCum, DisspCum for OutScalarEnergyBalance
+      FUNCTION s3g_energy_balance(ctx, out, reset) BIND(C, NAME="s3g_energy_balance")
+         IMPORT :: C_INT, C_PTR, C_DOUBLE
+         TYPE(C_PTR), VALUE :: ctx
+         REAL(C_DOUBLE), INTENT(OUT) :: out(4)
+         INTEGER(C_INT), VALUE :: reset
+         INTEGER(C_INT) :: s3g_energy_balance
+      END FUNCTION s3g_energy_balance
+
       FUNCTION s3g_synchronize(ctx) BIND(C, NAME="s3g_synchronize")
          IMPORT :: C_INT, C_PTR
          TYPE(C_PTR), VALUE :: ctx

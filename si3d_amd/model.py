@@ -199,6 +199,13 @@ class GpuLake:
         check(self.L.s3g_get_stats(self.h, C.byref(st)), "s3g_get_stats")
         return {name: getattr(st, name) for name, _ in S3gStats._fields_}
 
+    def energy_balance(self, reset=False):
+        """(TKinE, ShearCum, BuocyCum, DisspCum) of the closure's basin energy
+        budget; `reset` clears the three integrals afterwards."""
+        out = (C.c_double * 4)()
+        check(self.L.s3g_energy_balance(self.h, out, 1 if reset else 0), "s3g_energy_balance")
+        return tuple(out)
+
     def timers(self, enable=True):
         check(self.L.s3g_reset_timers(self.h, 1 if enable else 0), "s3g_reset_timers")
 

@@ -321,3 +321,101 @@ class ThreeDFile:
         if self.f is not None:
             self.f.close()
             self.f = None
+
+
+# ---- ScalarBalance.txt ---------------------------------------------------------
+def density(temperature, elevation, salinity=0.00004):
+    """Water density (kg/m3) of densty_s (si3d_utils.f90:3865-3949), in FP64
+    and element-wise over arrays: the UNESCO / Gill (1982) fresh-water
+    polynomial with the pressure of `elevation` metres of water found by
+    fixed-point iteration until the update is at most 1e-6.  salinity**(3/2)
+    is the integer power salinity**1 there (3/2 = 1 in Fortran)."""
+    t, z = np.broadcast_arrays(np.asarray(temperature, dtype=np.float64), np.asarray(elevation, dtype=np.float64))
+    s = salinity
+    t2, t3, t4 = t * t, t * t * t, t * t * t * t
+    densw = (999.842594 + 6.793952e-2 * t - 9.095290e-3 * t2 + 1.001685e-4 * t3 - 1.120083e-6 * t4
+             + 6.536332e-9 * t4 * t)
+    densws = (densw + s * (0.824493 - 4.0899e-3 * t + 7.6438e-5 * t2 - 8.2467e-7 * t3 + 5.3875e-9 * t4)
+              + s * (-5.72466e-3 + 1.0227e-4 * t - 1.6546e-6 * t2) + 4.8314e-4 * s * s)
+    kw = 19652.21 + 148.4206 * t - 2.327105 * t2 + 1.360477e-2 * t3 - 5.155288e-5 * t4
+    ks = kw + s * (54.6746 - 0.603459 * t + 1.09987e-2 * t2 - 6.1670e-5 * t3) + s * (7.944e-2 + 1.6483e-2 * t
+                                                                                       - 5.3009e-4 * t2)
+    rho = np.array(densws, dtype=np.float64, copy=True)
+    active = np.ones(rho.shape, dtype=bool)
+    for _ in range(10000):
+        p = 1e-5 * rho * 9.806 * z
+        k = (ks + p * (3.239908 + 1.43713e-3 * t + 1.16092e-4 * t2 - 5.77905e-7 * t3)
+             + p * s * (2.2838e-3 - 1.0981e-5 * t - 1.6078e-6 * t2) + 1.91075e-4 * p * s
+             + p * p * (8.50935e-5 - 6.12293e-6 * t + 5.2787e-8 * t2)
+             + p * p * s * (-9.9348e-7 + 2.0816e-8 * t + 9.1697e-10 * t2))
+        new = densws / (1.0 - p / k)
+        done = np.abs(new - rho) <= 1e-6
+        rho = np.where(active, new, rho)
+        active &= ~done
+        if not active.any():
+            break
+    return rho
+
+
+class ScalarBalance:
+    """ScalarBalance.txt as OutScalarEnergyBalance writes it (si3d_utils.f90:
+    3698-3814), one record (9E20.11) per call: time (h), heat content
+    sum(sal h), the last tracer's content, the basin's potential and kinetic
+    energy (densty_s of the p-level scalar at each layer's mid depth), and the
+    closure's TKinE, ShearCum, BuocyCum, DisspCum from s3g_energy_balance,
+    whose integrals are cleared after each record as the reference does.
+    The driver calls it at n = 0 and every ipt steps (si3d.f90:107, 442).
+    Sums run in FP64 (the reference's REAL*8 and REAL mix)."""
+
+    def __init__(self, g, deck, directory="."):
+        self.g, self.deck = g, deck
+        self.path = os.path.join(directory, "ScalarBalance.txt")
+        self.f = open(self.path, "w")
+        self.turb = deck.get("iturb", 0) >= 1
+        self.ntr = 0 if deck.get("tracer_profiles") is None else len(deck["tracer_profiles"])
+
+    def values(self, n, thrs):
+        g, d = self.g, self.deck
+        lm = g.lm
+        G = 9.806
+        kmz = g.get("kmz")[:lm].astype(int)
+        k1z = g.get("k1z")[:lm].astype(int)
+        lW, lS = (g.get(x)[:lm].astype(int) - 1 for x in ("lWC", "lSC"))
+        sal, h, hp, salp = g.get("sal"), g.get("h"), g.get("hp"), g.get("salp")
+        up, vp, wp = g.get("up"), g.get("vp"), g.get("wp")
+        hhs = g.get("hhs")
+        z = np.asarray(zlevels(d), dtype=np.float64)
+        tr = g.get(f"tracer{self.ntr}") if self.ntr else None
+        heat = oxyg = pot = kin = 0.0
+        for l in range(lm):
+            ks, k1 = kmz[l], k1z[l]
+            rng = slice(k1 - 1, ks)
+            heat += float(np.dot(sal[l, rng], h[l, rng]))
+            if tr is not None:
+                oxyg += float(np.dot(tr[l, rng], h[l, rng]))
+            ks_ = np.arange(ks, k1 - 1, -1)                        # kms, kms-1, ..., k1s (1-based)
+            zmid = np.where(z[ks_ - 1] == -100.0, 0.0, z[ks_ - 1]) + 0.5 * hp[l, ks_ - 1]
+            rho = density(salp[l, ks_ - 1], zmid) + 1000.0
+            elev = np.empty(len(ks_))
+            elev[0] = hhs[l] - h[l, ks - 1] / 2.0
+            if len(ks_) > 1:
+                elev[1:] = elev[0] + np.cumsum((hp[l, ks_[1:]] + hp[l, ks_[1:] - 1]) / 2.0)
+            hw = np.concatenate([[h[l, ks - 1]], hp[l, ks_[1:] - 1]])
+            pot += float(np.sum(rho * G * elev * hw))
+            uu = (up[l, ks_ - 1] + up[lW[l], ks_ - 1]) / 2.0
+            vv = (vp[l, ks_ - 1] + vp[lS[l], ks_ - 1]) / 2.0
+            ww = (wp[l, ks_ - 1] + wp[l, ks_]) / 2.0
+            kin += float(np.sum(0.5 * rho * (uu * uu + vv * vv + ww * ww) * h[l, ks_ - 1]))
+        pot *= d["dx"] * d["dy"]
+        kin *= d["dx"] * d["dy"]
+        tke = g.energy_balance(reset=True) if self.turb else (0.0, 0.0, 0.0, 0.0)
+        return [thrs, heat, oxyg, pot, kin] + list(tke)
+
+    def write(self, n, thrs):
+        self.f.write("".join(fortran_e(v, 20, 11) for v in self.values(n, thrs)) + "\n")
+        self.f.flush()
+
+    def close(self):
+        if self.f is not None:
+            self.f.close()
+            self.f = None

@@ -151,3 +151,72 @@ def test_cg_graph_replays_match_eager_launches(precision, precond):
         for name in ["s", "uh", "vh", "sal", "q2"]:
             np.testing.assert_array_equal(g.get(name), e.get(name), err_msg=f"step {n} {name}")
         assert g.stats()["pcg_iters_last"] == e.stats()["pcg_iters_last"]
+
+
+def test_energy_budget_over_steps_matches_oracle(oracle_lib):
+    """The closure's basin energy budget through 6 whole steps of the small
+    ellipse (three fd calls per step: the integrals grow on each leapfrog
+    call from the sums of the step before), read and cleared after step 3
+    as the writer of ScalarBalance.txt does, against the oracle."""
+    from si3d_amd.model import GpuLake
+    deck = small_ellipse()
+    deck["pcg_tol"] = TOL_SOLVE
+    o = oracle_lib.Oracle(deck)
+    g = GpuLake(deck, precision="fp64", preconditioner="mg", tol=TOL_SOLVE)
+    for n in range(1, 7):
+        o.step(n)
+        g.step(n)
+        want = o.get("energy_budget")
+        got = g.energy_balance(reset=(n == 3))
+        if n == 3:
+            o.set("energy_budget", list(want[:4]) + [0.0, 0.0, 0.0])
+        print("step", n, got, flush=True)
+        for a, b in zip(got, want[3:]):
+            assert a == pytest.approx(b, rel=1e-8, abs=1e-30), (n, got, want[3:])
+    assert all(abs(v) > 0 for v in got)
+
+
+def test_scalar_balance_records_match_oracle(oracle_lib, tmp_path):
+    """ScalarBalance.txt written from the GPU fields (outputs.ScalarBalance)
+    against the same writer fed the oracle's fields and budget: at n = 0 the
+    lake is at rest (kinetic energy and the closure's terms zero) and its
+    heat content is sum(T h); after 3 and 6 steps every column agrees to
+    1e-8."""
+    from si3d_amd import outputs
+    from si3d_amd.model import GpuLake
+    deck = small_ellipse()
+    deck["pcg_tol"] = TOL_SOLVE
+    o = oracle_lib.Oracle(deck)
+    g = GpuLake(deck, precision="fp64", preconditioner="mg", tol=TOL_SOLVE)
+
+    class OracleView:
+        lm = o.lm
+
+        def get(self, name):
+            return o.get(name)
+
+        def energy_balance(self, reset=False):
+            v = o.get("energy_budget")
+            if reset:
+                o.set("energy_budget", list(v[:4]) + [0.0, 0.0, 0.0])
+            return tuple(v[3:])
+
+    wg = outputs.ScalarBalance(g, deck, directory=str(tmp_path))
+    wo = outputs.ScalarBalance(OracleView(), deck, directory=str(tmp_path))
+    r0 = wg.values(0, 0.0)
+    h, sal, k1z, kmz = g.get("h"), g.get("sal"), g.get("k1z").astype(int), g.get("kmz").astype(int)
+    heat = sum(float(np.dot(sal[l, k1z[l] - 1:kmz[l]], h[l, k1z[l] - 1:kmz[l]])) for l in range(g.lm))
+    assert r0[1] == pytest.approx(heat, rel=1e-14) and r0[4] == 0.0 and r0[5:] == [0.0] * 4
+    np.testing.assert_allclose(r0, wo.values(0, 0.0), rtol=1e-12)
+    for n in range(1, 7):
+        o.step(n)
+        g.step(n)
+        if n % 3 == 0:
+            a, b = wg.values(n, n * deck["dt"] / 3600.0), wo.values(n, n * deck["dt"] / 3600.0)
+            print("step", n, a, flush=True)
+            np.testing.assert_allclose(a, b, rtol=1e-8)
+            assert a[4] > 0 and a[5] > 0
+    wg.write(6, 0.1)
+    wg.close()
+    wo.close()
+    assert len(open(tmp_path / "ScalarBalance.txt").readline()) == 181

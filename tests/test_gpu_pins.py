@@ -174,3 +174,43 @@ def test_asselin_state_matches_oracle(oracle_lib, prec):
     g.phase(12, 2, 1)
     _check_ints(o, g)
     compare(g, o, ["sp", "hp", "hup", "hvp", "uhp", "up", "vhp", "vp"], TOLS[prec], "Asselin filter")
+
+
+@PRECISIONS
+def test_energy_budget_homogeneous_column_matches_oracle(oracle_lib, prec):
+    """The closure's basin energy budget (s3g_energy_balance) on the
+    homogeneous-column state: a trapezoidal call sets the four sums, the
+    following leapfrog call scales them and adds three to the integrals;
+    reset clears the integrals.  Against the oracle, whose budget is pinned by
+    the closed forms of the same state."""
+    d = _turb_lake()
+    o, g = _pair(oracle_lib, d, prec)
+    o.phase(0, 2, 1, 2 * d["dt"] / 3600.0)
+    g.phase(0, 2, 1, 2 * d["dt"] / 3600.0)
+
+    class Both:
+        lm1, km1 = o.lm1, o.km1
+
+        def set(self, name, value):
+            if name in ("uair", "vair", "cdw"):
+                return
+            _set((o, g), name, value)
+
+    _column_state(Both(), 1.0e-4, 0.5, 2.0e-3, -1.5e-3, 0.05, 2.0e-3, 1.0e-3, 0.08, 6.0, 1.3e-3)
+    tol = TOLS[prec] if prec == "fp64" else 1e-5
+    o.set("energy_budget", [0.0] * 7)
+    g.energy_balance(reset=True)
+    o.phase(13, 2, 2)
+    g.phase(13, 2, 2)
+    want = o.get("energy_budget")
+    got = g.energy_balance()
+    assert got[0] == pytest.approx(want[3], rel=tol)
+    assert got[1:] == (0.0, 0.0, 0.0)
+    o.phase(13, 3, 1)
+    g.phase(13, 3, 1)
+    want = o.get("energy_budget")
+    got = g.energy_balance(reset=True)
+    print("budget", got, "oracle", want[3:], flush=True)
+    for a, b in zip(got, want[3:]):
+        assert a == pytest.approx(b, rel=tol)
+    assert g.energy_balance()[1:] == (0.0, 0.0, 0.0)

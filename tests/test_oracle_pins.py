@@ -573,3 +573,49 @@ def test_save_shifts_time_levels(oracle_lib, istep):
         if b["hup"][x] > 0:
             assert b["up"][x] == a["uh"][x] / b["hup"][x] and b["u"][x] == b["up"][x]
     assert np.max(np.abs(a["s"][:lm] - a["sp"][:lm])) > 1e-7
+
+
+def test_tke_energy_budget_homogeneous_column(oracle_lib):
+    """The basin energy budget of the closure (si3d_mixing.f90:333-338,
+    466-475, 539-548) on the sheared, stratified homogeneous column: a
+    trapezoidal call sums over every interface k1s < k <= kms of every
+    column, weighted by rho h = (rhop + 1000) hp, the shear production
+    Av S^2, the buoyancy production -Dv N^2, the dissipation
+    q2 q / (B1 l) (new q2, limited length scale) and q2 itself; the next
+    leapfrog call scales the four by dx dy dt (TKinE by half of it) and adds
+    the first three to the running integrals."""
+    d = _turb_lake()
+    o = oracle_lib.Oracle(d)
+    g = _grid(o)
+    Q, L = 1.0e-4, 0.5
+    Su, Sv, beta = 2.0e-3, -1.5e-3, 0.05
+    A0, D0 = 2.0e-3, 1.0e-3
+    _column_state(o, Q, L, Su, Sv, beta, A0, D0, 0.08, 6.0, 1.3e-3)
+    hp, rhop = o.get("hp"), o.get("rhop")
+    o.set("hpp", 1.3 * hp)  # other time levels differ, so a wrong one shows
+    o.set("energy_budget", [0.0] * 4 + [1.0, 2.0, 3.0])  # integrals carried from earlier steps
+    o.phase(13, 2, 2)
+    q2 = o.get("q2")
+    want = np.zeros(4)
+    for l in range(o.lm):
+        k1s, kms = g["k1z"][l], g["kmz"][l]
+        if kms - k1s < 1:
+            continue
+        for k in range(k1s + 1, kms + 1):
+            ra, rb = rhop[l, k - 2], rhop[l, k - 1]
+            dz = 0.5 * (hp[l, k - 2] + hp[l, k - 1])
+            n2 = -G / (0.5 * (ra + rb) + 1000.0) * (ra - rb) / dz
+            xl = L if n2 <= 0 else min(L, 0.53 * math.sqrt(Q / n2))
+            s2 = ((Su * 1.0) ** 2 + (Sv * 1.0) ** 2) / dz ** 2
+            rh = (rb + 1000.0) * hp[l, k - 1]
+            want += np.array([A0 * s2, -D0 * n2, q2[l, k - 1] * math.sqrt(Q) / (B1 * xl), q2[l, k - 1]]) * rh
+    got = o.get("energy_budget")
+    np.testing.assert_allclose(got[:4], want, rtol=1e-12)
+    np.testing.assert_array_equal(got[4:], [1.0, 2.0, 3.0])
+    assert want[0] > 0 and want[1] < 0 and want[2] > 0
+    a = d["dx"] * d["dy"] * d["dt"]
+    o.phase(13, 3, 1)
+    got2 = o.get("energy_budget")
+    np.testing.assert_allclose(got2[:3], want[:3] * a, rtol=1e-12)
+    np.testing.assert_allclose(got2[3], 0.5 * want[3] * a, rtol=1e-12)
+    np.testing.assert_allclose(got2[4:], [1.0, 2.0, 3.0] + want[:3] * a, rtol=1e-12)

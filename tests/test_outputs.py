@@ -155,3 +155,23 @@ def test_hour_stamp_rounds_halves_away_from_zero():
     assert hour_stamp(3600) == 100
     assert hour_stamp(86340) == 2398    # 23.98333 h
     assert hour_stamp(0) == 0
+
+
+def test_density_matches_oracle_eos(oracle_lib):
+    """The writer's vectorised densty_s against the oracle's (itself pinned by
+    the UNESCO check values and a root finder) over temperatures and depths."""
+    from si3d_amd.outputs import density
+    T = np.linspace(0.0, 30.0, 31)
+    Z = np.array([0.0, 0.25, 5.0, 50.0, 120.0])
+    got = density(T[:, None], Z[None, :])
+    want = np.array([[oracle_lib.densty(t, 0.00004, z) for z in Z] for t in T])
+    np.testing.assert_allclose(got, want, rtol=1e-13, atol=0)
+    assert abs(density(5.0, 0.0, 0.0) - 999.96675) < 1e-5
+
+
+def test_scalar_balance_record_format(tmp_path):
+    """A ScalarBalance.txt record: nine E20.11 fields."""
+    from si3d_amd.outputs import fortran_e
+    line = "".join(fortran_e(v, 20, 11) for v in [0.0, 1.5, 0.0, -2.5e10, 3.25e-3, 0.0, 1e-5, -2e-6, 4e-6])
+    assert len(line) == 180
+    assert line[:20] == "   0.00000000000E+00" and line[20:40] == "   0.15000000000E+01"
