@@ -94,6 +94,18 @@ def wet_columns(deck):
     return ii[order] + 2, jj[order] + 2
 
 
+def west_south(deck):
+    """0-based west and south neighbour of every wet column (lm, the dry
+    sentinel, where the neighbour is dry), as bathy sets lWC / lSC
+    (si3d_utils.f90:796-806)."""
+    ii, jj = wet_columns(deck)
+    lm = len(ii)
+    index = {(int(i), int(j)): l for l, (i, j) in enumerate(zip(ii, jj))}
+    lW = np.array([index.get((int(i) - 1, int(j)), lm) for i, j in zip(ii, jj)], dtype=np.int64)
+    lS = np.array([index.get((int(i), int(j) - 1), lm) for i, j in zip(ii, jj)], dtype=np.int64)
+    return lW, lS
+
+
 def zlevels(deck):
     """zlevel(1..km1) as ZGridDimensions defines it (si3d_utils.f90:318-382)."""
     if deck["ibathyf"] < 0:
@@ -373,6 +385,7 @@ class ScalarBalance:
         self.f = open(self.path, "w")
         self.turb = deck.get("iturb", 0) >= 1
         self.ntr = 0 if deck.get("tracer_profiles") is None else len(deck["tracer_profiles"])
+        self.lW, self.lS = west_south(deck)
 
     def values(self, n, thrs):
         g, d = self.g, self.deck
@@ -380,7 +393,7 @@ class ScalarBalance:
         G = 9.806
         kmz = g.get("kmz")[:lm].astype(int)
         k1z = g.get("k1z")[:lm].astype(int)
-        lW, lS = (g.get(x)[:lm].astype(int) - 1 for x in ("lWC", "lSC"))
+        lW, lS = self.lW, self.lS
         sal, h, hp, salp = g.get("sal"), g.get("h"), g.get("hp"), g.get("salp")
         up, vp, wp = g.get("up"), g.get("vp"), g.get("wp")
         hhs = g.get("hhs")

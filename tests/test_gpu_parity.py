@@ -202,7 +202,8 @@ def test_scalar_balance_records_match_oracle(oracle_lib, tmp_path):
             return tuple(v[3:])
 
     wg = outputs.ScalarBalance(g, deck, directory=str(tmp_path))
-    wo = outputs.ScalarBalance(OracleView(), deck, directory=str(tmp_path))
+    (tmp_path / "oracle").mkdir()
+    wo = outputs.ScalarBalance(OracleView(), deck, directory=str(tmp_path / "oracle"))
     r0 = wg.values(0, 0.0)
     h, sal, k1z, kmz = g.get("h"), g.get("sal"), g.get("k1z").astype(int), g.get("kmz").astype(int)
     heat = sum(float(np.dot(sal[l, k1z[l] - 1:kmz[l]], h[l, k1z[l] - 1:kmz[l]])) for l in range(g.lm))

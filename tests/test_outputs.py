@@ -175,3 +175,13 @@ def test_scalar_balance_record_format(tmp_path):
     line = "".join(fortran_e(v, 20, 11) for v in [0.0, 1.5, 0.0, -2.5e10, 3.25e-3, 0.0, 1e-5, -2e-6, 4e-6])
     assert len(line) == 180
     assert line[:20] == "   0.00000000000E+00" and line[20:40] == "   0.15000000000E+01"
+
+
+def test_west_south_neighbours_match_oracle(oracle_lib):
+    from si3d_amd.outputs import west_south
+    from tests.helpers import small_ellipse
+    d = small_ellipse()
+    o = oracle_lib.Oracle(d)
+    lW, lS = west_south(d)
+    np.testing.assert_array_equal(lW, o.get("lWC")[:o.lm].astype(int) - 1)
+    np.testing.assert_array_equal(lS, o.get("lSC")[:o.lm].astype(int) - 1)
