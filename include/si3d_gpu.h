@@ -231,6 +231,17 @@ int s3g_learn_info(s3g_ctx *c, s3g_learn_stats *out);
 int s3g_unet_check(int H, int W, const float *weights, const float *x3, const float *target, const float *mask,
                    float *out, float *grads, double *loss);
 
+/* Like s3g_unet_check, but allocates no device memory: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The network, its
+ * activations and gradients and the inputs live in a static device array of
+ * 40 MiB that every GPU the library runs on holds until the process ends;
+ * H * W may be at most 4096 cells (S3G_ERR_ARG otherwise, with the reason in
+ * s3g_last_error).  Each call clears what it uses; calls that overlap on one
+ * GPU (from several host threads) race on that array. */
+int s3g_unet_check_static(int H, int W, const float *weights, const float *x3, const float *target,
+                          const float *mask, float *out, float *grads, double *loss);
+
 /* --- Standalone driver (no Fortran host) --------------------------------- */
 
 /* Restates input/bathy/init on the host and calls s3g_init + s3g_set_field_*. */

@@ -23,6 +23,7 @@ EXPORTS = [
     "s3g_learn_setup", "s3g_learn_record", "s3g_learn_train", "s3g_learn_mode", "s3g_learn_num_weights",
     "s3g_learn_get_weights", "s3g_learn_set_weights", "s3g_learn_info", "s3g_unet_check",
     "s3g_create_ensemble", "s3g_ensemble_info", "s3g_energy_balance", "s3g_set_open_boundaries",
+    "s3g_unet_check_static",
 ]
 
 dp = C.POINTER(C.c_double)
@@ -116,6 +117,7 @@ def load():
     L.s3g_learn_set_weights.argtypes = [vp, fp, C.c_long]
     L.s3g_learn_info.argtypes = [vp, C.POINTER(S3gLearnStats)]
     L.s3g_unet_check.argtypes = [C.c_int, C.c_int, fp, fp, fp, fp, fp, fp, dp]
+    L.s3g_unet_check_static.argtypes = [C.c_int, C.c_int, fp, fp, fp, fp, fp, fp, dp]
     L.s3g_create_ensemble.argtypes = [C.c_int, C.POINTER(S3gDeck), C.POINTER(S3gParams), C.POINTER(vp)]
     L.s3g_ensemble_info.argtypes = [vp, C.c_int, ip, ip]
     L.s3g_set_open_boundaries.argtypes = [vp, C.POINTER(S3gOpenBc)]

@@ -388,12 +388,26 @@ struct UNet {
   double *lpart = nullptr;  // loss partials
   std::vector<void *> allocs;
   long adam_t = 0;
+  // Arena mode: buffers are carved out of caller-provided device memory
+  // (256-byte aligned) instead of allocated, and the network is built for
+  // evaluation only (no Adam moments, no random initial weights).
+  unsigned char *arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
 
   template <typename T>
   T *alloc(size_t n) {
+    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    if (arena) {
+      const size_t b = (bytes + 255) & ~(size_t)255;
+      if (arena_used + b > arena_cap) return nullptr;
+      void *p = arena + arena_used;
+      arena_used += b;
+      hipMemset(p, 0, bytes);
+      return (T *)p;
+    }
     void *p = nullptr;
-    if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return nullptr;
-    hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T));
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    hipMemset(p, 0, bytes);
     allocs.push_back(p);
     return (T *)p;
   }
@@ -417,17 +431,23 @@ struct UNet {
       L.Cout = c[1];
       L.level = c[2];
       L.nw = 9L * L.Cout * L.Cin;
+      const bool adam = arena == nullptr;
       L.w = alloc<float>(L.nw);
       L.gw = alloc<float>(L.nw);
-      L.mw = alloc<float>(L.nw);
-      L.vw = alloc<float>(L.nw);
+      L.mw = adam ? alloc<float>(L.nw) : nullptr;
+      L.vw = adam ? alloc<float>(L.nw) : nullptr;
       L.b = alloc<float>(L.Cout);
       L.gb = alloc<float>(L.Cout);
-      L.mb = alloc<float>(L.Cout);
-      L.vb = alloc<float>(L.Cout);
+      L.mb = adam ? alloc<float>(L.Cout) : nullptr;
+      L.vb = adam ? alloc<float>(L.Cout) : nullptr;
       L.wb = alloc<bf16>(L.nw);
       L.wt = alloc<bf16>(L.nw);
-      if (!L.w || !L.wt) return false;
+      if (!L.w || !L.gw || !L.b || !L.gb || !L.wb || !L.wt) return false;
+      conv.push_back(L);
+      const int nsl = slices_for(9 * (L.Cout / 32) * ((L.Cin + 31) / 32), px(L.level));
+      maxpart = std::max(maxpart, (size_t)nsl * L.nw);
+      maxpart = std::max(maxpart, (size_t)bias_slices(px(L.level)) * L.Cout);
+      if (!adam) continue;
       // He initialisation (the input layer's unused channels 3..15 get zeros)
       const int fan_in = 9 * (L.Cin == 16 ? 3 : L.Cin);
       std::normal_distribution<float> nd(0.f, std::sqrt(2.f / fan_in));
@@ -437,20 +457,18 @@ struct UNet {
         hwv[i] = (L.Cin == 16 && ci >= 3) ? 0.f : nd(rng);
       }
       hipMemcpy(L.w, hwv.data(), L.nw * sizeof(float), hipMemcpyHostToDevice);
-      conv.push_back(L);
-      const int nsl = slices_for(9 * (L.Cout / 32) * ((L.Cin + 31) / 32), px(L.level));
-      maxpart = std::max(maxpart, (size_t)nsl * L.nw);
-      maxpart = std::max(maxpart, (size_t)bias_slices(px(L.level)) * L.Cout);
     }
     hw = alloc<float>(32);
     hb = alloc<float>(1);
     hgw = alloc<float>(32);
     hgb = alloc<float>(1);
-    hmw = alloc<float>(32);
-    hvw = alloc<float>(32);
-    hmb = alloc<float>(1);
-    hvb = alloc<float>(1);
-    {
+    if (!arena) {
+      hmw = alloc<float>(32);
+      hvw = alloc<float>(32);
+      hmb = alloc<float>(1);
+      hvb = alloc<float>(1);
+    }
+    if (!arena) {
       std::normal_distribution<float> nd(0.f, std::sqrt(1.f / 32.f));
       std::vector<float> h(32);
       for (auto &v : h) v = 0.1f * nd(rng);
@@ -886,6 +904,38 @@ static int learn_fail(const char *m) {
   return S3G_ERR_HIP;
 }
 
+// Device memory of s3g_unet_check_static, the variant that allocates nothing:
+// a file-scope array every GPU holds from the library's load to the end of
+// the process, big enough for the network's weights, activations, gradients
+// and partial sums and the inputs of one dense field of at most 4096 cells.
+// Its host-side name is not a device address: each device's copy is reached
+// through the address hipGetSymbolAddress returns with that device current.
+constexpr size_t kStaticScratchBytes = (size_t)40 << 20;
+constexpr long kStaticMaxCells = 4096;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+static unsigned char *scratch_addr[kMaxDevices];
+
+// The current device's copy of the scratch array (looked up once per device).
+static int static_scratch(unsigned char **out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    (void)hipGetLastError();
+    return learn_fail("s3g_unet_check_static: no usable current device");
+  }
+  if (!scratch_addr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return learn_fail("s3g_unet_check_static: hipGetSymbolAddress of the scratch array failed");
+    }
+    scratch_addr[dev] = static_cast<unsigned char *>(p);
+  }
+  *out = scratch_addr[dev];
+  return 0;
+}
+
 // Before the CG: record r0 / sigma and the coupling field of this solve and,
 // in mode 1, move the guess by the network's correction.
 int learn_before_solve(s3g_ctx *c, const void *outer_v, double *x) {
@@ -1149,6 +1199,44 @@ int s3g_unet_check(int H, int W, const float *weights, const float *x3, const fl
   hipFree(d_in3);
   hipFree(d_t);
   hipFree(d_m);
+  return rc;
+}
+
+
+int s3g_unet_check_static(int H, int W, const float *weights, const float *x3, const float *target,
+                          const float *mask, float *out, float *grads, double *loss) {
+  if (H < 8 || W < 8 || H % 8 || W % 8 || !weights || !x3 || !target || !mask) return S3G_ERR_ARG;
+  if ((long)H * W > kStaticMaxCells)
+    return learn_fail("s3g_unet_check_static: the input has more than 4096 cells (the static scratch holds "
+                      "a network of at most 64 x 64)");
+  unsigned char *base = nullptr;
+  if (int e = static_scratch(&base)) return e;
+  UNet net;
+  net.arena = base;
+  net.arena_cap = kStaticScratchBytes;
+  if (!net.create(H, W, 1)) return learn_fail("s3g_unet_check_static: the network does not fit the static scratch");
+  const int npx = H * W;
+  float *d_in3 = net.alloc<float>(3 * (size_t)npx), *d_t = net.alloc<float>(npx), *d_m = net.alloc<float>(npx);
+  if (!d_in3 || !d_t || !d_m) return learn_fail("s3g_unet_check_static: the input does not fit the static scratch");
+  net.set_weights(weights);
+  hipMemcpy(d_in3, x3, 3 * (size_t)npx * sizeof(float), hipMemcpyHostToDevice);
+  hipMemcpy(d_t, target, (size_t)npx * sizeof(float), hipMemcpyHostToDevice);
+  hipMemcpy(d_m, mask, (size_t)npx * sizeof(float), hipMemcpyHostToDevice);
+  double nm = 0.0;
+  for (int p = 0; p < npx; ++p) nm += mask[p];
+  hipLaunchKernelGGL(k_pack_dense, UNet::ew_grid(npx), dim3(256), 0, 0, (const float *)d_in3, npx, net.X0);
+  net.forward(0);
+  const int lb = net.loss(0, d_t, d_m, (float)(1.0 / std::max(1.0, nm)));
+  net.backward(0, false);
+  int rc = hip_check(hipDeviceSynchronize(), "s3g_unet_check_static");
+  if (rc == 0 && out) rc = hip_check(hipMemcpy(out, net.out, (size_t)npx * sizeof(float), hipMemcpyDeviceToHost), "out");
+  if (rc == 0 && grads) net.get_grads(grads);
+  if (rc == 0 && loss) {
+    double t = 0.0, h[1024];
+    rc = hip_check(hipMemcpy(h, net.lpart, lb * sizeof(double), hipMemcpyDeviceToHost), "loss");
+    for (int k = 0; k < lb; ++k) t += h[k];
+    *loss = t / std::max(1.0, nm);
+  }
   return rc;
 }
 

@@ -253,10 +253,13 @@ class GpuLake:
         return {bytes(names[i]).split(b"\0")[0].decode(): (avg[i], cnt[i]) for i in range(n)}
 
 
-def unet_check(weights, x3, target, mask):
+def unet_check(weights, x3, target, mask, static=False):
     """Forward + backward of the library's U-Net on one dense input (x3: 3 x H x
-    W); returns (out H x W, weight gradients, loss)."""
+    W); returns (out H x W, weight gradients, loss).  `static` calls the
+    variant that allocates no device memory (s3g_unet_check_static, at most
+    4096 cells)."""
     L = _lib.load()
+    fn = L.s3g_unet_check_static if static else L.s3g_unet_check
     _, H, W = x3.shape
     w = np.ascontiguousarray(weights, dtype=np.float32)
     x = np.ascontiguousarray(x3, dtype=np.float32)
@@ -265,9 +268,9 @@ def unet_check(weights, x3, target, mask):
     out = np.empty((H, W), dtype=np.float32)
     grads = np.empty(L.s3g_learn_num_weights(), dtype=np.float32)
     loss = C.c_double()
-    check(L.s3g_unet_check(H, W, w.ctypes.data_as(_lib.fp), x.ctypes.data_as(_lib.fp), t.ctypes.data_as(_lib.fp),
-                           m.ctypes.data_as(_lib.fp), out.ctypes.data_as(_lib.fp), grads.ctypes.data_as(_lib.fp),
-                           C.byref(loss)), "s3g_unet_check")
+    check(fn(H, W, w.ctypes.data_as(_lib.fp), x.ctypes.data_as(_lib.fp), t.ctypes.data_as(_lib.fp),
+             m.ctypes.data_as(_lib.fp), out.ctypes.data_as(_lib.fp), grads.ctypes.data_as(_lib.fp), C.byref(loss)),
+          "s3g_unet_check_static" if static else "s3g_unet_check")
     return out, grads, loss.value
 
 
