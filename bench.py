@@ -72,6 +72,8 @@ def parse():
     ap.add_argument("--nx", type=int, default=1024)
     ap.add_argument("--ny", type=int, default=512)
     ap.add_argument("--layers", type=int, default=60)
+    ap.add_argument("--dx", type=float, default=None,
+                    help="grid spacing (m); default 50 for the 2048 x 2048 lake, 100 otherwise")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--precond", default="mg", choices=["jacobi", "line", "mg"])
     ap.add_argument("--tol", type=float, default=1e-10)
@@ -167,7 +169,7 @@ def mixed_error(a):
     from si3d_amd import lake
     from si3d_amd.model import GpuLake
     nsteps = a.mixed_error
-    deck = lake.ellipse(nx=a.nx, ny=a.ny, dx=100.0, nlayers=a.layers, dt=60.0, nts=nsteps + 2, seed=1)
+    deck = lake.ellipse(nx=a.nx, ny=a.ny, dx=a.dx, nlayers=a.layers, dt=60.0, nts=nsteps + 2, seed=1)
     f64 = GpuLake(deck, precision="fp64", preconditioner=a.precond, tol=a.tol)
     mix = GpuLake(deck, precision="mixed", preconditioner=a.precond, tol=a.tol)
     fields = ["s", "uh", "vh", "wp", "sal", "h", "q2", "Av"]
@@ -203,7 +205,7 @@ def mixed_error(a):
             print(f"mixed-error: step {n} of {nsteps}, {time.perf_counter() - t0:.0f} s, "
                   f"s max_rel {checkpoints[n]['s']['max_rel']:.2e}", file=sys.stderr, flush=True)
     print(json.dumps({"report": "mixed precision vs FP64, same lake, same free-surface tolerance",
-                      "workload": f"ellipse lake {a.nx}x{a.ny}x{a.layers}, dx=100 m, dt=60 s, {nsteps} steps, "
+                      "workload": f"ellipse lake {a.nx}x{a.ny}x{a.layers}, dx={a.dx:g} m, dt=60 s, {nsteps} steps, "
                                   f"PCG ({a.precond}) tol {a.tol:g}",
                       "errors_after": {str(k): v for k, v in checkpoints.items()},
                       "pcg_iters_per_step": {"fp64": f64.stats()["pcg_iters_total"] / nsteps,
@@ -234,6 +236,8 @@ def per_solve_iterations(g, deck, n_first, nsteps):
 
 def main():
     a = parse()
+    if a.dx is None:
+        a.dx = 50.0 if (a.nx, a.ny) == (2048, 2048) else 100.0
     if a.copy_probe:
         copy_probe(a.copy_probe)
         return
@@ -273,7 +277,7 @@ def main():
         if a.tracers:
             workload += f", {a.tracers} tracers"
     else:
-        deck = lake.ellipse(nx=a.nx, ny=a.ny, dx=100.0, nlayers=a.layers, dt=60.0, nts=nts, seed=1)
+        deck = lake.ellipse(nx=a.nx, ny=a.ny, dx=a.dx, nlayers=a.layers, dt=60.0, nts=nts, seed=1)
         if a.surfbc != 20:
             deck = lake.heat_budget(deck, a.surfbc)
         if a.tracers:
@@ -282,7 +286,7 @@ def main():
                                                 for q in range(a.tracers)])
         n_wet = lake.n_wet_cells(deck)
         g = GpuLake(deck, precision=a.precision, preconditioner=a.precond, tol=a.tol)
-        workload = (f"ellipse lake {a.nx}x{a.ny}x{a.layers}, dx=100 m, dt=60 s, TVD + "
+        workload = (f"ellipse lake {a.nx}x{a.ny}x{a.layers}, dx={a.dx:g} m, dt=60 s, TVD + "
                     f"turbulence, plain PCG ({a.precond}) tol {a.tol:g}, {a.precision}")
         if a.surfbc != 20:
             workload += f", surface heat budget ifsurfbc={a.surfbc}"

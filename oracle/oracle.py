@@ -66,6 +66,8 @@ def lib():
         L.orc_solver_stats.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.orc_densty.restype = C.c_double
         L.orc_densty.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.orc_tvd_flux.restype = C.c_double
+        L.orc_tvd_flux.argtypes = [C.c_double] * 7
         dp = C.POINTER(C.c_double)
         L.orc_trid.argtypes = [C.c_int, dp, dp, dp, dp, dp, dp, dp]
         L.orc_trid1.argtypes = [C.c_int, dp, dp, dp, dp, dp]
@@ -190,6 +192,10 @@ class Oracle:
 
 def densty(t, s, z):
     return lib().orc_densty(t, s, z)
+
+
+def tvd_flux(vel, s1, s2, s3, s4, cfdef, courant):
+    return lib().orc_tvd_flux(vel, s1, s2, s3, s4, cfdef, courant)
 
 
 def trid(a, b, c, g, r):

@@ -1240,31 +1240,39 @@ void OrcLake::continuity(int ist) {
 // exsal, si3d_procedures.f90:2827-3076 (superbee TVD; no open boundaries).
 void OrcLake::exsal() { exscalar(salpp, false); }
 
+// Face flux of exsal's TVD scheme (si3d_procedures.f90:2900-2930, 2948-2975,
+// 2995-3030): s1..s4 are the scalar at the four cells along the flow axis
+// (face between s2 and s3, positive velocity from s2 to s3), the superbee
+// limiter C_f of the upwind gradient ratio (2928) blends first-order upwind
+// (C_f = 0) and Lax-Wendroff (C_f = 1); cfdef is C_f where s3 = s2 (0 for the
+// horizontal faces, 1 for the vertical ones) and courant = dt'/dx.
+static double tvd_flux(double vel, double s1, double s2, double s3, double s4, double cfdef, double courant) {
+  const double ss[5] = {0.0, s1, s2, s3, s4};
+  double cf = cfdef;
+  const int g1 = (vel >= 0.0) ? -1 : 1;  // -SIGN(1.,vel)
+  if (ss[3] - ss[2] != 0) {
+    const double ratio = (ss[3 + g1] - ss[2 + g1]) / (ss[3] - ss[2]);
+    cf = std::max(0.0, std::max(std::min(1.0, 2. * ratio), std::min(2.0, ratio)));
+  }
+  return vel / 2. * (ss[3] + ss[2]) - ((1. - cf) * std::fabs(vel) + vel * vel * courant * cf) * (ss[3] - ss[2]) / 2.;
+}
+
 // The explicit transport terms of a scalar: exsal (salinity/temperature, cpp =
 // salpp) and exTracer (si3d_procedures.f90:4090-4357, cpp = tracerpp of one
 // tracer, which also sets Bex of the sediment layer, 4342).  Both write the
 // same flux buffers (the reference's thread buffers Bth2/Bth4/Bth3).
 void OrcLake::exscalar(const F3 &salpp, bool is_tracer) {
   const double twodt1 = twodt * tz;
-  auto superbee = [](double ratio) {
-    return std::max(0.0, std::max(std::min(1.0, 2. * ratio), std::min(2.0, ratio)));
-  };
   // EW fluxes at u-faces (2892-2930)
   for (int l = 1; l <= lm; ++l) {
     int kms = kmz[l], k1s = k1z[l], lE = lEC[l], lW = lWC[l];
     for (int k = k1s; k <= kms; ++k) {
       if (hup(k, l) > ZERO) {
-        double vel = (uhpp(k, l) + uh(k, l)) / 2.;
-        double ss[5];
-        ss[2] = salpp(k, l);
-        ss[3] = salpp(k, lE);
-        ss[1] = (hpp(k, lW) <= ZERO) ? ss[2] : salpp(k, lW);
-        ss[4] = (hpp(k, lEC[lE]) <= ZERO) ? ss[3] : salpp(k, lEC[lE]);
-        double cf = 0.0;
-        int g1 = (vel >= 0.0) ? -1 : 1;  // -SIGN(1.,vel)
-        if (ss[3] - ss[2] != 0) cf = superbee((ss[3 + g1] - ss[2 + g1]) / (ss[3] - ss[2]));
-        fx_ew(k, l) = vel / 2. * (ss[3] + ss[2]) -
-                      ((1. - cf) * std::fabs(vel) + vel * vel * twodt1 / dx * cf) * (ss[3] - ss[2]) / 2.;
+        const double vel = (uhpp(k, l) + uh(k, l)) / 2.;
+        const double s2 = salpp(k, l), s3 = salpp(k, lE);
+        const double s1 = (hpp(k, lW) <= ZERO) ? s2 : salpp(k, lW);
+        const double s4 = (hpp(k, lEC[lE]) <= ZERO) ? s3 : salpp(k, lEC[lE]);
+        fx_ew(k, l) = tvd_flux(vel, s1, s2, s3, s4, 0.0, twodt1 / dx);
       } else {
         fx_ew(k, l) = 0.0;
       }
@@ -1276,33 +1284,21 @@ void OrcLake::exscalar(const F3 &salpp, bool is_tracer) {
     for (int k = 0; k <= km1 + 1; ++k) fx_ud(k, l) = 0.0;
     for (int k = k1s; k <= kms; ++k) {
       if (hvp(k, l) > ZERO) {
-        double vel = (vhpp(k, l) + vh(k, l)) / 2.;
-        double ss[5];
-        ss[2] = salpp(k, l);
-        ss[3] = salpp(k, lN);
-        ss[1] = (hpp(k, lS) <= ZERO) ? ss[2] : salpp(k, lS);
-        ss[4] = (hpp(k, lNC[lN]) <= ZERO) ? ss[3] : salpp(k, lNC[lN]);
-        double cf = 0.0;
-        int g1 = (vel >= 0.0) ? -1 : 1;
-        if (ss[3] - ss[2] != 0) cf = superbee((ss[3 + g1] - ss[2 + g1]) / (ss[3] - ss[2]));
-        fx_ns(k, l) = vel / 2. * (ss[3] + ss[2]) -
-                      ((1. - cf) * std::fabs(vel) + vel * vel * twodt1 / dy * cf) * (ss[3] - ss[2]) / 2.;
+        const double vel = (vhpp(k, l) + vh(k, l)) / 2.;
+        const double s2 = salpp(k, l), s3 = salpp(k, lN);
+        const double s1 = (hpp(k, lS) <= ZERO) ? s2 : salpp(k, lS);
+        const double s4 = (hpp(k, lNC[lN]) <= ZERO) ? s3 : salpp(k, lNC[lN]);
+        fx_ns(k, l) = tvd_flux(vel, s1, s2, s3, s4, 0.0, twodt1 / dy);
       } else {
         fx_ns(k, l) = 0.0;
       }
       if (hp(k - 1, l) > ZERO) {
-        double vel = (k == k1s) ? 0.0 : wp(k, l);
-        double ss[5];
-        ss[2] = salpp(k, l);
-        ss[3] = salpp(k - 1, l);
-        ss[4] = (hpp(k - 2, l) <= ZERO) ? ss[3] : salpp(k - 2, l);
-        ss[1] = (hpp(k + 1, l) <= ZERO) ? ss[2] : salpp(k + 1, l);
-        double cf = 1.0;
-        int g1 = (vel >= 0.0) ? -1 : 1;
-        if (ss[3] - ss[2] != 0) cf = superbee((ss[3 + g1] - ss[2 + g1]) / (ss[3] - ss[2]));
-        double delz = (hp(k, l) + hp(k - 1, l)) / 2.;
-        fx_ud(k, l) = vel / 2. * (ss[3] + ss[2]) -
-                      ((1. - cf) * std::fabs(vel) + vel * vel * twodt1 / delz * cf) * (ss[3] - ss[2]) / 2.;
+        const double vel = (k == k1s) ? 0.0 : wp(k, l);
+        const double s2 = salpp(k, l), s3 = salpp(k - 1, l);
+        const double s4 = (hpp(k - 2, l) <= ZERO) ? s3 : salpp(k - 2, l);
+        const double s1 = (hpp(k + 1, l) <= ZERO) ? s2 : salpp(k + 1, l);
+        const double delz = (hp(k, l) + hp(k - 1, l)) / 2.;
+        fx_ud(k, l) = tvd_flux(vel, s1, s2, s3, s4, 1.0, twodt1 / delz);
       } else {
         fx_ud(k, l) = 0.0;
       }
@@ -2401,6 +2397,10 @@ void orc_solver_stats(const OrcLake *o, double *out3) {
 }
 
 double orc_densty(double t, double s, double e) { return densty_s(t, s, e); }
+
+double orc_tvd_flux(double vel, double s1, double s2, double s3, double s4, double cfdef, double courant) {
+  return tvd_flux(vel, s1, s2, s3, s4, cfdef, courant);
+}
 
 void orc_trid(int n, const double *a, const double *b, const double *c, const double *g, const double *r,
               double *ag, double *ar) {

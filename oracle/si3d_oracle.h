@@ -81,6 +81,7 @@ void orc_solver_stats(const OrcLake *o, double *out3);
 
 /* Unit-level entry points pinned by tests against closed forms. */
 double orc_densty(double temperature, double salinity, double elevation);
+double orc_tvd_flux(double vel, double s1, double s2, double s3, double s4, double cfdef, double courant);
 void orc_trid(int n, const double *a, const double *b, const double *c,
               const double *g, const double *r, double *ag, double *ar);
 void orc_trid1(int n, const double *a, const double *b, const double *c,

@@ -434,7 +434,7 @@ static int alloc_state(s3g_ctx *c, Dev<R> &D, const int *lE, const int *lW, cons
   D.tkb = nullptr;
   D.tke_part = D.tke = nullptr;
   if (c->P.iturb > 0) {
-    if (!(D.tkb = dalloc<double2>(c, n3)) || !(D.tke_part = dalloc<double>(c, 4 * ((size_t)c->lm / kBlock + 1))) ||
+    if (!(D.tkb = dalloc<float2>(c, n3)) || !(D.tke_part = dalloc<double>(c, 4 * ((size_t)c->lm / kBlock + 1))) ||
         !(D.tke = dalloc<double>(c, 8)))
       return S3G_ERR_HIP;
   }

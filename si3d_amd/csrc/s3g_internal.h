@@ -74,10 +74,11 @@ struct Dev {
   R *heat;  // HeatSource (surface heat budget, ifsurfbc 1-3), null otherwise
   double *q2, *q2p, *q2pp, *q2l, *q2lp, *q2lpp, *d0, *d1, *d2;
   // basin energy budget of the closure (iturb >= 1, one lake): per-level
-  // (dissipation rate x rho h, rho h) of the running call, per-workgroup
-  // partial sums, and the budget state: the call's ShearProduction,
-  // BuoyancyProduction, Dissipation, TKinE, then ShearCum, BuocyCum, DisspCum
-  double2 *tkb;
+  // (dissipation rate x rho h, rho h) of the running call, in single
+  // precision like the reference's rhoxh, per-workgroup partial sums, and the
+  // budget state: the call's ShearProduction, BuoyancyProduction,
+  // Dissipation, TKinE, then ShearCum, BuocyCum, DisspCum
+  float2 *tkb;
   double *tke_part, *tke;
 };
 

@@ -1094,7 +1094,9 @@ __global__ void __launch_bounds__(kBlock) k_turb_cell(Dev<R> D, int nk, int tile
 // With `budget` (trapezoidal calls, si3d_mixing.f90:466-475) the column also
 // returns its share of the basin energy budget over levels k1s+1..kms, each
 // weighted by rho h = (rhop + 1000) hp: shear and buoyancy production,
-// dissipation q2 sqrt(q2p) / (B1 l) and q2, the new q2 after its floor.
+// dissipation q2 sqrt(q2p) / (B1 l) and q2, the new q2 after its floor.  The
+// weights of the last two wait for the back substitution in single precision
+// (the reference's rhoxh is single precision too); the sums are FP64.
 struct TkeSums {
   double sh, by, di, tk;
 };
@@ -1226,7 +1228,7 @@ __device__ __forceinline__ TkeSums turb_column(const Dev<R> &D, int m, int istep
       const double rhoxh = ((SPLIT ? (double)D.rhop[IX(k, m)] : rcur.rho) + 1000.) * hk;
       bs.sh += shp * rhoxh;
       bs.by += byp * rhoxh;
-      D.tkb[IX(k, m)] = make_double2(sq / (B_1 * xlk) * rhoxh, rhoxh);
+      D.tkb[IX(k, m)] = make_float2((float)(sq / (B_1 * xlk) * rhoxh), (float)rhoxh);
     }
     // new eddy coefficients of interface k (the surface minimum, 281-284)
     if (!SPLIT) {
@@ -1257,9 +1259,9 @@ __device__ __forceinline__ TkeSums turb_column(const Dev<R> &D, int m, int istep
     D.q2[IX(k, m)] = q2n;
     D.q2l[IX(k, m)] = dmax(Q2L_MIN, x2);
     if (budget && k > k1s) {
-      const double2 w = D.tkb[IX(k, m)];
-      bs.di += q2n * w.x;
-      bs.tk += q2n * w.y;
+      const float2 w = D.tkb[IX(k, m)];
+      bs.di += q2n * (double)w.x;
+      bs.tk += q2n * (double)w.y;
     }
   }
   for (int k = D.k1; k <= k1s - 1; ++k) {

@@ -157,7 +157,9 @@ def test_energy_budget_over_steps_matches_oracle(oracle_lib):
     """The closure's basin energy budget through 6 whole steps of the small
     ellipse (three fd calls per step: the integrals grow on each leapfrog
     call from the sums of the step before), read and cleared after step 3
-    as the writer of ScalarBalance.txt does, against the oracle."""
+    as the writer of ScalarBalance.txt does, against the oracle, to 1e-6:
+    the dissipation and TKinE weights (rho h) are held in single precision
+    between the two sweeps, as the reference computes rho h."""
     from si3d_amd.model import GpuLake
     deck = small_ellipse()
     deck["pcg_tol"] = TOL_SOLVE
@@ -172,7 +174,7 @@ def test_energy_budget_over_steps_matches_oracle(oracle_lib):
             o.set("energy_budget", list(want[:4]) + [0.0, 0.0, 0.0])
         print("step", n, got, flush=True)
         for a, b in zip(got, want[3:]):
-            assert a == pytest.approx(b, rel=1e-8, abs=1e-30), (n, got, want[3:])
+            assert a == pytest.approx(b, rel=1e-6, abs=1e-30), (n, got, want[3:])
     assert all(abs(v) > 0 for v in got)
 
 
@@ -215,7 +217,8 @@ def test_scalar_balance_records_match_oracle(oracle_lib, tmp_path):
         if n % 3 == 0:
             a, b = wg.values(n, n * deck["dt"] / 3600.0), wo.values(n, n * deck["dt"] / 3600.0)
             print("step", n, a, flush=True)
-            np.testing.assert_allclose(a, b, rtol=1e-8)
+            np.testing.assert_allclose(a[:5], b[:5], rtol=1e-8)
+            np.testing.assert_allclose(a[5:], b[5:], rtol=1e-6)
             assert a[4] > 0 and a[5] > 0
     wg.write(6, 0.1)
     wg.close()

@@ -197,7 +197,7 @@ def test_energy_budget_homogeneous_column_matches_oracle(oracle_lib, prec):
             _set((o, g), name, value)
 
     _column_state(Both(), 1.0e-4, 0.5, 2.0e-3, -1.5e-3, 0.05, 2.0e-3, 1.0e-3, 0.08, 6.0, 1.3e-3)
-    tol = TOLS[prec] if prec == "fp64" else 1e-5
+    tol = 1e-6 if prec == "fp64" else 1e-5  # rho h of two terms held in single precision between the sweeps
     o.set("energy_budget", [0.0] * 7)
     g.energy_balance(reset=True)
     o.phase(13, 2, 2)
