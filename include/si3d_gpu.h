@@ -206,6 +206,10 @@ typedef struct {
   long guess_solves;     /* solves with the learned guess since the mode was set */
   double guess_error_ratio; /* mean ||x - x_learned||_A / ||x - x_plain||_A over
                               those solves (x: the converged solution)        */
+  long guess_rejected;   /* solves whose learned guess was not closer to the
+                            solution in the energy norm than the plain guess
+                            (the energy functional did not drop), which then
+                            started from the plain guess                     */
 } s3g_learn_stats;
 
 /* Allocate the network (He-initialised from `seed`) and a device ring of

@@ -74,7 +74,7 @@ class S3gStats(C.Structure):
 class S3gLearnStats(C.Structure):
     _fields_ = [("pairs", C.c_int), ("capacity", C.c_int), ("train_steps", C.c_long),
                 ("last_loss", C.c_double), ("height", C.c_int), ("width", C.c_int), ("mode", C.c_int),
-                ("guess_solves", C.c_long), ("guess_error_ratio", C.c_double)]
+                ("guess_solves", C.c_long), ("guess_error_ratio", C.c_double), ("guess_rejected", C.c_long)]
 
 
 _lib = None

@@ -842,6 +842,66 @@ __global__ void k_learn_gain(const double *part, int nb, double *acc) {
   }
 }
 
+// Change of the energy functional CG minimises, phi(x) = x^T A x / 2 - b^T x,
+// from the plain guess xg to the learned guess x: with d = x - xg and
+// r = b - A xg, phi(x) - phi(xg) = d^T A d / 2 - d^T r, which is half the change
+// of the squared A-norm error; negative exactly when the learned guess is
+// closer to the solution.  Partials of d^T A d (part[blk]) and d^T r
+// (part[nb + blk]).
+__global__ void __launch_bounds__(256) k_learn_functional(Outer O, const double *xg, const double *x, double *part) {
+  __shared__ double lds[2][256];
+  double sa = 0.0, sr = 0.0;
+  auto dv = [&](int j) { return j == O.n ? 0.0 : x[j] - xg[j]; };
+  auto gv = [&](int j) { return j == O.n ? 0.0 : xg[j]; };
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < O.n; c += gridDim.x * blockDim.x) {
+    const int lW = O.lW[c], lS = O.lS[c], lN = O.lN[c], lE = O.lE[c];
+    const double ad = O.rr[c] * dv(c) - O.sx[lW] * dv(lW) - O.sy[lS] * dv(lS) - O.sy[c] * dv(lN) - O.sx[c] * dv(lE);
+    const double r = O.b[c] - (O.rr[c] * gv(c) - O.sx[lW] * gv(lW) - O.sy[lS] * gv(lS) - O.sy[c] * gv(lN) -
+                               O.sx[c] * gv(lE));
+    sa += dv(c) * ad;
+    sr += dv(c) * r;
+  }
+  lds[0][threadIdx.x] = sa;
+  lds[1][threadIdx.x] = sr;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      lds[0][threadIdx.x] += lds[0][threadIdx.x + o];
+      lds[1][threadIdx.x] += lds[1][threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = lds[0][0];
+    part[gridDim.x + blockIdx.x] = lds[1][0];
+  }
+}
+
+// Keeps the learned guess only if it lowers the energy functional: every
+// workgroup adds the partials of k_learn_functional in the same order, so all
+// take the same decision; otherwise (or if the change is not finite) the plain
+// guess goes back into x and acc[2] counts the rejection.
+__global__ void __launch_bounds__(256) k_learn_keep(const double *part, int nb, const double *xg, double *x, int n,
+                                                    double *acc) {
+  __shared__ double lds[2][256];
+  double sa = 0.0, sr = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) sa += part[i], sr += part[nb + i];
+  lds[0][threadIdx.x] = sa;
+  lds[1][threadIdx.x] = sr;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      lds[0][threadIdx.x] += lds[0][threadIdx.x + o];
+      lds[1][threadIdx.x] += lds[1][threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  const double dphi = 0.5 * lds[0][0] - lds[1][0];
+  if (dphi < 0.0) return;  // closer: keep it (NaN compares false and is rejected)
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) x[c] = xg[c];
+  if (blockIdx.x == 0 && threadIdx.x == 0) acc[2] += 1.0;
+}
+
 __global__ void k_copy_f64(double *dst, const double *src, int n) {
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) dst[c] = src[c];
 }
@@ -961,6 +1021,13 @@ int learn_before_solve(s3g_ctx *c, const void *outer_v, double *x) {
     L->net.forward(c->stream);
     hipLaunchKernelGGL(k_learn_apply, dim3(nb), dim3(256), 0, c->stream, x, L->n, (const int *)L->pix,
                        (const float *)L->net.out, (const double *)L->sig);
+    // the plain guess is the fallback: keep the learned one only if it is closer
+    // to the solution in the energy norm (the CG then still runs to the same
+    // tolerance either way)
+    hipLaunchKernelGGL(k_learn_functional, dim3(nb), dim3(256), 0, c->stream, O, (const double *)L->xg,
+                       (const double *)x, L->part);
+    hipLaunchKernelGGL(k_learn_keep, dim3(nb), dim3(256), 0, c->stream, (const double *)L->part, nb,
+                       (const double *)L->xg, x, L->n, L->gain);
     hipLaunchKernelGGL(k_copy_f64, dim3(nb), dim3(256), 0, c->stream, L->xl, (const double *)x, L->n);
   }
   L->used = use;
@@ -1052,7 +1119,7 @@ int s3g_learn_setup(s3g_ctx *c, int capacity, unsigned seed) {
   L->xg = (double *)dal((size_t)(c->lm + 1) * sizeof(double));
   L->xl = (double *)dal((size_t)(c->lm + 1) * sizeof(double));
   L->part = (double *)dal(2048 * sizeof(double));
-  L->gain = (double *)dal(2 * sizeof(double));
+  L->gain = (double *)dal(3 * sizeof(double));
   if (!L->pix || !L->pairs || !L->part) {
     learn_destroy(c);
     return learn_fail("s3g_learn_setup: device allocation failed");
@@ -1075,7 +1142,7 @@ int s3g_learn_record(s3g_ctx *c, int on) {
 int s3g_learn_mode(s3g_ctx *c, int mode) {
   if (!c || !c->learn || mode < 0 || mode > 1) return S3G_ERR_ARG;
   c->learn->mode = mode;
-  hipMemsetAsync(c->learn->gain, 0, 2 * sizeof(double), c->stream);
+  hipMemsetAsync(c->learn->gain, 0, 3 * sizeof(double), c->stream);
   return hip_check(hipStreamSynchronize(c->stream), "s3g_learn_mode");
 }
 
@@ -1149,10 +1216,11 @@ int s3g_learn_info(s3g_ctx *c, s3g_learn_stats *out) {
   out->height = L->H;
   out->width = L->W;
   out->mode = L->mode;
-  double g[2] = {0.0, 0.0};
-  hipMemcpy(g, L->gain, 2 * sizeof(double), hipMemcpyDeviceToHost);
+  double g[3] = {0.0, 0.0, 0.0};
+  hipMemcpy(g, L->gain, 3 * sizeof(double), hipMemcpyDeviceToHost);
   out->guess_solves = (long)g[1];
   out->guess_error_ratio = g[1] > 0 ? g[0] / g[1] : NAN;
+  out->guess_rejected = (long)g[2];
   return hip_check(hipGetLastError(), "s3g_learn_info");
 }
 

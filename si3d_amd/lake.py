@@ -141,6 +141,18 @@ def wind_series(tl, dtsbc=600.0, amp=0.2, period=86400.0, direction_deg=30.0,
     return np.stack([np.full(n, cdw), ua, va], axis=1)
 
 
+def random_wind_series(tl, dtsbc=600.0, amp=(0.05, 0.3), seed=0, cdw=1.2e-3):
+    """si3d_surfbc_W.txt records (cdw, uair, vair) whose stress changes every
+    record to a random amplitude uniform in `amp` (Pa) blowing from a random
+    direction uniform in [0, 360) deg: a sequence of different forcings."""
+    n = int(tl / dtsbc) + 4
+    rng = np.random.default_rng(seed)
+    tau = rng.uniform(amp[0], amp[1], n)
+    th = np.radians(rng.uniform(0.0, 360.0, n))
+    U = np.sqrt(tau / (cdw * G_RHOAIR))
+    return np.stack([np.full(n, cdw), -U * np.sin(th), -U * np.cos(th)], axis=1)
+
+
 def heat_budget(deck, mode, eta=0.5, qsw_max=800.0, q_night=-150.0, period=86400.0):
     """The deck with a spatially uniform surface heat budget (ifsurfbc = mode,
     1, 2 or 3, si3d_surfbc.txt records) over its wind series: a diurnal

@@ -79,3 +79,34 @@ def test_learned_guess_gain_same_error():
     assert ia < ib, (ia, ib)  # fewer iterations to the same FP64 true-residual tolerance
     for name in ("s", "uh", "vh", "sal"):
         assert relerr(A.get(name), B.get(name)) < 1e-6, name
+    assert 0 <= info["guess_rejected"] < info["guess_solves"]
+
+
+def test_learned_guess_worse_than_plain_is_rejected():
+    """A network whose correction is far off (its trained weights scaled by
+    10 per layer, so the output grows by orders of magnitude) raises the energy
+    functional on every solve: each learned guess is rejected, CG starts from
+    the plain guess, and the run is bit-identical to the plain one with the
+    same iteration count."""
+    from si3d_amd.model import GpuLake
+    deck = small_ellipse(nts=20)
+    A = GpuLake(deck, preconditioner="mg", tol=1e-10)
+    B = GpuLake(deck, preconditioner="mg", tol=1e-10)
+    A.learn_setup(capacity=16, seed=5)
+    for n in range(1, 6):
+        A.step(n)
+        B.step(n)
+    A.learn_train(20, batch=4, lr=1e-3)
+    A.learn_weights(10.0 * A.learn_weights())
+    A.learn_record(False)
+    A.learn_mode(1)
+    ia0, ib0 = A.stats()["pcg_iters_total"], B.stats()["pcg_iters_total"]
+    for n in range(6, 11):
+        A.step(n)
+        B.step(n)
+    info = A.learn_info()
+    assert info["guess_solves"] == 15
+    assert info["guess_rejected"] == 15
+    assert A.stats()["pcg_iters_total"] - ia0 == B.stats()["pcg_iters_total"] - ib0
+    for name in ("s", "uh", "vh", "sal"):
+        assert np.array_equal(A.get(name), B.get(name)), name
