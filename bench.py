@@ -1,15 +1,26 @@
 """Benchmark of the pSi3D time step on one MI355X.
 
-Workload: the synthetic stratified elliptic lake of 1024 x 512 cells, dx = 100 m,
-60 stretched layers (0.5 -> 5 m), GRF bathymetry, tanh thermocline, sinusoidal
-wind from 30 deg, TVD transport + two-equation turbulence, dt = 60 s.  A step is
-one full si3d.f90 time step (leapfrog fd + settrap + 2 trapezoidal fd +
-settrap2 + save) with all three free-surface solves to a 1e-10 FP64 relative
-residual.  Inputs are resident in HBM before the timed region.
+Workload (default): the synthetic stratified elliptic lake of 2048 x 2048
+cells, dx = 50 m, 60 stretched layers (0.5 -> 5 m), GRF bathymetry, tanh
+thermocline, sinusoidal wind from 30 deg, TVD transport + two-equation
+turbulence, dt = 30 s (`--nx 1024 --ny 512` gives the 1024 x 512, dx = 100 m,
+dt = 60 s lake; both have the first internal mode's Courant number c1 dt / dx
+near 0.35).  A step is one full si3d.f90 time step (leapfrog fd + settrap + 2
+trapezoidal fd + settrap2 + save) with all three free-surface solves to a
+1e-10 FP64 relative true residual.  Inputs are resident in HBM before the
+timed region.
+
+`value` is the plain FP64 path.  The learned legs: the network is trained in a
+context of its own on pairs the plain solver records under random winds, then
+its weights drive the initial guess of FP64 and mixed-precision runs of the
+measured lake under its own (unseen) wind, from the same initial state and
+over the same steps as the plain leg; each reports its throughput, the speed-up
+over plain FP64, iterations and its free surface's difference from plain FP64.
 
 Prints one JSON line: wet-cell-steps/s (value), PCG iterations per step, the
 roofline of the dominant kernel (HIP events around every launch in the timed
-region) and a CPU baseline (the FP64 oracle on the host, a bounded sample).
+region), the learned legs and a CPU baseline (the FP64 oracle on the host, a
+bounded sample).
 """
 from __future__ import annotations
 
@@ -67,21 +78,35 @@ TRAFFIC_FP64_1024x512x60 = {
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--nx", type=int, default=1024)
-    ap.add_argument("--ny", type=int, default=512)
+    ap.add_argument("--nx", type=int, default=2048)
+    ap.add_argument("--ny", type=int, default=2048)
     ap.add_argument("--layers", type=int, default=60)
     ap.add_argument("--dx", type=float, default=None,
                     help="grid spacing (m); default 50 for the 2048 x 2048 lake, 100 otherwise")
+    ap.add_argument("--dt", type=float, default=None,
+                    help="time step (s); default 60 at dx = 100 m and 30 at dx = 50 m, which keeps the "
+                         "first internal mode's Courant number c1 dt / dx at about 0.35 (dt = 60 s at 50 m doubles "
+                         "it and the scheme's checkerboard internal-wave mode grows: the oracle and the GPU both "
+                         "diverge within 20 steps)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--precond", default="mg", choices=["jacobi", "line", "mg"])
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-side", type=int, default=512,
+                    help="for lakes above 1024 x 512 cells the CPU baseline runs the same generator on a "
+                         "CPU-SIDE x CPU-SIDE lake of the same spacing (the full lake needs minutes per step)")
     ap.add_argument("--no-timers", action="store_true")
-    ap.add_argument("--no-learned", action="store_true", help="skip the learned-initial-guess leg")
-    ap.add_argument("--train-steps", type=int, default=300)
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-initial-guess legs")
+    ap.add_argument("--train-steps", type=int, default=300, help="optimiser steps of the network (batch 4)")
+    ap.add_argument("--train-pairs", type=int, default=192,
+                    help="(right-hand side, solution) pairs recorded from plain solves under random winds")
+    ap.add_argument("--train-wind", default="0.005,0.05", metavar="LO,HI",
+                    help="range (Pa) of the random wind stresses of the training run (a new wind every 10 min)")
+    ap.add_argument("--flex-period", type=int, default=0,
+                    help="also run the learned correction stage inside flexible CG every N iterations")
     ap.add_argument("--ensemble", type=int, default=0, metavar="N",
                     help="advance N lakes of uneven size (sides 128-768, 20-80 layers, random wind) together")
     ap.add_argument("--side-max", type=int, default=768)
@@ -169,7 +194,7 @@ def mixed_error(a):
     from si3d_amd import lake
     from si3d_amd.model import GpuLake
     nsteps = a.mixed_error
-    deck = lake.ellipse(nx=a.nx, ny=a.ny, dx=a.dx, nlayers=a.layers, dt=60.0, nts=nsteps + 2, seed=1)
+    deck = lake.ellipse(nx=a.nx, ny=a.ny, dx=a.dx, nlayers=a.layers, dt=a.dt, nts=nsteps + 2, seed=1)
     f64 = GpuLake(deck, precision="fp64", preconditioner=a.precond, tol=a.tol)
     mix = GpuLake(deck, precision="mixed", preconditioner=a.precond, tol=a.tol)
     fields = ["s", "uh", "vh", "wp", "sal", "h", "q2", "Av"]
@@ -205,7 +230,7 @@ def mixed_error(a):
             print(f"mixed-error: step {n} of {nsteps}, {time.perf_counter() - t0:.0f} s, "
                   f"s max_rel {checkpoints[n]['s']['max_rel']:.2e}", file=sys.stderr, flush=True)
     print(json.dumps({"report": "mixed precision vs FP64, same lake, same free-surface tolerance",
-                      "workload": f"ellipse lake {a.nx}x{a.ny}x{a.layers}, dx={a.dx:g} m, dt=60 s, {nsteps} steps, "
+                      "workload": f"ellipse lake {a.nx}x{a.ny}x{a.layers}, dx={a.dx:g} m, dt={a.dt:g} s, {nsteps} steps, "
                                   f"PCG ({a.precond}) tol {a.tol:g}",
                       "errors_after": {str(k): v for k, v in checkpoints.items()},
                       "pcg_iters_per_step": {"fp64": f64.stats()["pcg_iters_total"] / nsteps,
@@ -234,10 +259,90 @@ def per_solve_iterations(g, deck, n_first, nsteps):
     return [round(float(v), 2) for v in np.mean(np.asarray(rows, dtype=np.float64), axis=0)]
 
 
+def train_on_random_winds(a, deck):
+    """The network's training run: the same lake under a sequence of random
+    winds (seeded; a new stress and direction every 10 min, amplitudes in
+    --train-wind) in a context of its own, plain FP64 PCG recording
+    --train-pairs solves, then --train-steps optimiser steps.  Returns the
+    weights and a summary; the context is released before the measured legs."""
+    from si3d_amd import lake
+    from si3d_amd.model import GpuLake
+    lo, hi = (float(v) for v in a.train_wind.split(","))
+    nrec = (a.train_pairs + 2) // 3 + 1  # 2 solves on step 1, 3 on each later step
+    td = dict(deck)
+    td["nts"] = max(deck["nts"], nrec + 2)
+    td["surfbc"] = lake.random_wind_series(td["nts"] * deck["dt"], 600.0, amp=(lo, hi), seed=11)
+    t0 = time.perf_counter()
+    g = GpuLake(td, precision="fp64", preconditioner=a.precond, tol=a.tol)
+    g.learn_setup(capacity=a.train_pairs, seed=1)
+    for n in range(1, nrec + 1):
+        g.step(n)
+    g.synchronize()
+    t_rec = time.perf_counter() - t0
+    g.learn_record(False)
+    t1 = time.perf_counter()
+    loss0 = g.learn_train(1, batch=4, lr=0.0)
+    loss1 = g.learn_train(a.train_steps, batch=4, lr=1e-3)
+    t_tr = time.perf_counter() - t1
+    info = g.learn_info()
+    w = g.learn_weights()
+    g.close()
+    print(f"bench: trained on {info['pairs']} pairs ({nrec} steps under random winds) in {t_tr:.1f} s, "
+          f"loss {loss0:.3e} -> {loss1:.3e}", file=sys.stderr, flush=True)
+    return w, {"pairs": info["pairs"], "record_steps": nrec, "forcing": f"random wind stress {lo:g}-{hi:g} Pa, any "
+               f"direction, a new wind every 600 s (seed 11)", "train_steps": info["train_steps"],
+               "batch": 4, "record_s": round(t_rec, 2), "train_s": round(t_tr, 2), "loss_before": loss0,
+               "loss_after": loss1,
+               "net": f"U-Net 4 levels, 32 base channels, bf16 MFMA, {info['height']}x{info['width']}"}
+
+
+def run_learned(a, deck, n_wet, weights, precision, s_plain, plain_value, flex=0):
+    """A measured leg with the learned initial guess (weights trained under
+    other forcings) from the same initial state as the plain FP64 leg, over
+    the same warmup + timed steps: throughput, iterations, and the free
+    surface's difference from the plain FP64 leg after those steps."""
+    from si3d_amd.model import GpuLake
+    g = GpuLake(deck, precision=precision, preconditioner=a.precond, tol=a.tol)
+    g.learn_setup(capacity=0, seed=1)
+    g.learn_weights(weights)
+    g.learn_record(False)
+    g.learn_mode(1)
+    if flex:
+        g.learn_flexible(flex)
+    for n in range(1, a.warmup + 1):
+        g.step(n)
+    g.synchronize()
+    st0 = g.stats()
+    t0 = time.perf_counter()
+    for n in range(a.warmup + 1, a.warmup + a.steps + 1):
+        g.step(n)
+    g.synchronize()
+    el = time.perf_counter() - t0
+    st1 = g.stats()
+    s = g.get("s")[:g.lm]
+    info = g.learn_info()
+    per_solve = per_solve_iterations(g, deck, a.warmup + a.steps + 1, 3) if deck.get("itrap", 1) == 1 else None
+    g.close()
+    value = n_wet * a.steps / el
+    d = float(np.max(np.abs(s - s_plain)) / max(float(np.max(np.abs(s_plain))), 1e-300))
+    print(f"bench: learned {precision}{' flex ' + str(flex) if flex else ''}: {1e3 * el / a.steps:.2f} ms/step, "
+          f"{(st1['pcg_iters_total'] - st0['pcg_iters_total']) / a.steps:.2f} iters/step", file=sys.stderr,
+          flush=True)
+    return {"precision": precision, "value": round(value, 1), "unit": "wet-cell-steps/s",
+            "ms_per_step": round(1e3 * el / a.steps, 4), "speedup_vs_plain_fp64": round(value / plain_value, 4),
+            "pcg_iters_per_step": round((st1["pcg_iters_total"] - st0["pcg_iters_total"]) / a.steps, 2),
+            "pcg_iters_per_solve": per_solve, "pcg_relres_last": st1["pcg_relres_last"],
+            "guess_error_ratio": round(info["guess_error_ratio"], 4), "guess_solves": info["guess_solves"],
+            "guess_rejected": info["guess_rejected"], "flex_period": flex, "corrections": info["corrections"],
+            "zeta_max_rel_diff_vs_plain_fp64": d}
+
+
 def main():
     a = parse()
     if a.dx is None:
         a.dx = 50.0 if (a.nx, a.ny) == (2048, 2048) else 100.0
+    if a.dt is None:
+        a.dt = 60.0 * min(1.0, a.dx / 100.0)
     if a.copy_probe:
         copy_probe(a.copy_probe)
         return
@@ -252,6 +357,7 @@ def main():
     from si3d_amd.model import GpuEnsemble, GpuLake
 
     nts = 2 * (a.warmup + a.steps) + 10
+    weights = training = None
     t_setup = time.perf_counter()
     if a.ensemble:
         decks = lake.ensemble(a.ensemble, seed=7, side=(128, a.side_max), nts=nts)
@@ -277,7 +383,7 @@ def main():
         if a.tracers:
             workload += f", {a.tracers} tracers"
     else:
-        deck = lake.ellipse(nx=a.nx, ny=a.ny, dx=a.dx, nlayers=a.layers, dt=60.0, nts=nts, seed=1)
+        deck = lake.ellipse(nx=a.nx, ny=a.ny, dx=a.dx, nlayers=a.layers, dt=a.dt, nts=nts, seed=1)
         if a.surfbc != 20:
             deck = lake.heat_budget(deck, a.surfbc)
         if a.tracers:
@@ -285,21 +391,22 @@ def main():
             deck["tracer_profiles"] = np.stack([np.where(np.arange(km1) < km1 // (2 + q), 10.0, 2.0)
                                                 for q in range(a.tracers)])
         n_wet = lake.n_wet_cells(deck)
+        if a.precision != "fp64":
+            a.no_learned = True  # the learned legs are measured against the plain FP64 leg
+        if not a.no_learned:
+            weights, training = train_on_random_winds(a, deck)
+        t_setup = time.perf_counter()
         g = GpuLake(deck, precision=a.precision, preconditioner=a.precond, tol=a.tol)
-        workload = (f"ellipse lake {a.nx}x{a.ny}x{a.layers}, dx={a.dx:g} m, dt=60 s, TVD + "
+        workload = (f"ellipse lake {a.nx}x{a.ny}x{a.layers}, dx={a.dx:g} m, dt={a.dt:g} s, TVD + "
                     f"turbulence, plain PCG ({a.precond}) tol {a.tol:g}, {a.precision}")
         if a.surfbc != 20:
             workload += f", surface heat budget ifsurfbc={a.surfbc}"
         if a.tracers:
             workload += f", {a.tracers} tracers"
     t_setup = time.perf_counter() - t_setup
-    if not a.no_learned:
-        g.learn_setup(capacity=256, seed=1)  # the warmup solves record training pairs
     for n in range(1, a.warmup + 1):
         g.step(n)
     g.synchronize()
-    if not a.no_learned:
-        g.learn_record(False)
     st0 = g.stats()
     if not a.no_timers:
         g.timers(True)
@@ -315,39 +422,10 @@ def main():
 
     value = n_wet * a.steps / elapsed
     iters = (st1["pcg_iters_total"] - st0["pcg_iters_total"]) / a.steps
+    s_plain = g.get("s")[:g.lm]  # after warmup + steps, before the per-solve probe
     n_probe = 3
     trap_every_step = not a.ensemble and deck.get("itrap", 1) == 1
     per_solve = per_solve_iterations(g, deck, n0 + a.steps, n_probe) if trap_every_step else None
-
-    learned = None
-    if not a.no_learned:
-        # learned initial guess: train on the recorded pairs (untimed), then the
-        # same number of steps from where the plain leg stopped
-        t_tr = time.perf_counter()
-        loss0 = g.learn_train(1, batch=4, lr=0.0)
-        loss1 = g.learn_train(a.train_steps, batch=4, lr=1e-3)
-        t_tr = time.perf_counter() - t_tr
-        g.learn_mode(1)
-        g.synchronize()
-        sl0 = g.stats()
-        t1 = time.perf_counter()
-        n1 = n0 + a.steps + (n_probe if per_solve else 0)
-        for n in range(n1, n1 + a.steps):
-            g.step(n)
-        g.synchronize()
-        el1 = time.perf_counter() - t1
-        sl1 = g.stats()
-        per_solve_l = per_solve_iterations(g, deck, n1 + a.steps, n_probe) if per_solve else None
-        info = g.learn_info()
-        learned = {"value": round(n_wet * a.steps / el1, 1), "unit": "wet-cell-steps/s",
-                   "ms_per_step": round(1e3 * el1 / a.steps, 4),
-                   "pcg_iters_per_step": round((sl1["pcg_iters_total"] - sl0["pcg_iters_total"]) / a.steps, 2),
-                   "pcg_iters_per_solve": per_solve_l,
-                   "pcg_relres_last": sl1["pcg_relres_last"],
-                   "guess_error_ratio": round(info["guess_error_ratio"], 4),
-                   "pairs": info["pairs"], "train_steps": info["train_steps"],
-                   "train_s": round(t_tr, 2), "loss_before": loss0, "loss_after": loss1,
-                   "net": f"U-Net 4 levels, 32 base channels, bf16 MFMA, {info['height']}x{info['width']}"}
     elem = 8 if a.precision == "fp64" else 4
 
     roofline = None
@@ -379,13 +457,34 @@ def main():
             cb = copy_bandwidth()
             roofline["copy_GBs"] = round(cb, 1) if cb else None
 
+    tiles = g.tiles()
+    lm, km1 = g.lm, g.km1
+    learned = None
+    if not a.no_learned:
+        # the measured legs run one context at a time (each 2048 x 2048 FP64
+        # context holds about half of the HBM)
+        g.close()
+        learned = {"training": training,
+                   "eval_forcing": "the lake's own wind series (0.2 Pa sin(2 pi t / 24 h) from 30 deg), not in "
+                                   "the training set",
+                   "legs": [run_learned(a, deck, n_wet, weights, "fp64", s_plain, value),
+                            run_learned(a, deck, n_wet, weights, "mixed", s_plain, value)]}
+        if a.flex_period:
+            learned["legs"].append(run_learned(a, deck, n_wet, weights, "fp64", s_plain, value, a.flex_period))
+
     cpu = None
     if not a.no_cpu_baseline:
-        dt, o = cpu_baseline(deck, a.cpu_steps)
-        cpu = {"value": round(n_wet * a.cpu_steps / dt, 1), "unit": "wet-cell-steps/s", "cores": 1,
+        cdeck, cn_wet, what = deck, n_wet, "same lake"
+        if a.nx * a.ny > 1024 * 512 and not (a.ensemble or a.river):
+            cdeck = lake.ellipse(nx=a.cpu_side, ny=a.cpu_side, dx=a.dx, nlayers=a.layers, dt=a.dt,
+                                 nts=a.cpu_steps + 2, seed=1)
+            cn_wet = lake.n_wet_cells(cdeck)
+            what = (f"the same generator on a {a.cpu_side}x{a.cpu_side} lake at dx={a.dx:g} m ({cn_wet} wet cells; "
+                    f"the full lake takes minutes per step on one core)")
+        dt, o = cpu_baseline(cdeck, a.cpu_steps)
+        cpu = {"value": round(cn_wet * a.cpu_steps / dt, 1), "unit": "wet-cell-steps/s", "cores": 1,
                "kind": "port",
-               "sample": f"FP64 oracle (C++ restatement, 1 thread), same lake, steps 1-{a.cpu_steps}, "
-                         f"{dt:.1f} s"}
+               "sample": f"FP64 oracle (C++ restatement, 1 thread), {what}, steps 1-{a.cpu_steps}, {dt:.1f} s"}
 
     out = {
         "metric": "wet-cell-steps/sec and PCG iters/step at 1 MI355X, learned vs plain, same error",
@@ -401,7 +500,7 @@ def main():
         "dtype": "f64" if a.precision == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": workload,
-                   "n_wet": n_wet, "lm": g.lm, "km1": g.km1, "parallelism": "1 GPU"},
+                   "n_wet": n_wet, "lm": lm, "km1": km1, "parallelism": "1 GPU"},
         "pcg_iters_per_step": round(iters, 2),
         "pcg_iters_per_solve": per_solve,
         "pcg_relres_last": st1["pcg_relres_last"],
@@ -409,7 +508,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "learned": learned,
-        "tiles": g.tiles(),
+        "tiles": tiles,
         "kernels": kernel_summary,
     }
     print(json.dumps(out), flush=True)

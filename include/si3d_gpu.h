@@ -210,6 +210,8 @@ typedef struct {
                             solution in the energy norm than the plain guess
                             (the energy functional did not drop), which then
                             started from the plain guess                     */
+  long corrections;      /* correction stages run since s3g_learn_flexible  */
+  int flex_period;       /* the period set by s3g_learn_flexible (0: off)    */
 } s3g_learn_stats;
 
 /* Allocate the network (He-initialised from `seed`) and a device ring of
@@ -224,6 +226,19 @@ int s3g_learn_train(s3g_ctx *c, int steps, int batch, double lr, double *loss_ou
 /* 0: plain initial guess (zeta of the latest solve); 1: plus the network's
  * correction. */
 int s3g_learn_mode(s3g_ctx *c, int mode);
+/* Learned correction stage inside flexible CG (period > 0; 0 turns it off):
+ * on iterations 0, period, 2 period, ... the preconditioned residual is
+ * z = y + M^-1 (r - A y), where y is the network's estimate of A^-1 r (the
+ * map it was trained on: normalised residual of an iterate -> its normalised
+ * error) and M the chosen preconditioner; on the other iterations z = M^-1 r.
+ * The search direction uses the Polak-Ribiere beta
+ * ((z_i, r_i) - (z_i, r_{i-1})) / (z_{i-1}, r_{i-1}), which stays valid for a
+ * preconditioner that changes between iterations, and the solve stops at the
+ * same FP64 true-residual tolerance.  Needs a trained network
+ * (s3g_learn_train); independent of s3g_learn_mode.  Replaces the CG
+ * accelerator of NSPCG (nspcg.f:8328-8450 cgw, 8967-9092 itcg), which has no
+ * such stage. */
+int s3g_learn_flexible(s3g_ctx *c, int period);
 long s3g_learn_num_weights(void);
 int s3g_learn_get_weights(s3g_ctx *c, float *host, long n);
 int s3g_learn_set_weights(s3g_ctx *c, const float *host, long n);

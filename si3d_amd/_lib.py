@@ -21,7 +21,7 @@ EXPORTS = [
     "s3g_synchronize", "s3g_get_stats", "s3g_kernel_times", "s3g_reset_timers", "s3g_get_tiles",
     "s3g_finalize", "s3g_last_error", "s3g_create_from_deck",
     "s3g_learn_setup", "s3g_learn_record", "s3g_learn_train", "s3g_learn_mode", "s3g_learn_num_weights",
-    "s3g_learn_get_weights", "s3g_learn_set_weights", "s3g_learn_info", "s3g_unet_check",
+    "s3g_learn_get_weights", "s3g_learn_set_weights", "s3g_learn_info", "s3g_learn_flexible", "s3g_unet_check",
     "s3g_create_ensemble", "s3g_ensemble_info", "s3g_energy_balance", "s3g_set_open_boundaries",
     "s3g_unet_check_static",
 ]
@@ -74,7 +74,8 @@ class S3gStats(C.Structure):
 class S3gLearnStats(C.Structure):
     _fields_ = [("pairs", C.c_int), ("capacity", C.c_int), ("train_steps", C.c_long),
                 ("last_loss", C.c_double), ("height", C.c_int), ("width", C.c_int), ("mode", C.c_int),
-                ("guess_solves", C.c_long), ("guess_error_ratio", C.c_double), ("guess_rejected", C.c_long)]
+                ("guess_solves", C.c_long), ("guess_error_ratio", C.c_double), ("guess_rejected", C.c_long),
+                ("corrections", C.c_long), ("flex_period", C.c_int)]
 
 
 _lib = None
@@ -111,6 +112,7 @@ def load():
     L.s3g_learn_record.argtypes = [vp, C.c_int]
     L.s3g_learn_train.argtypes = [vp, C.c_int, C.c_int, C.c_double, dp]
     L.s3g_learn_mode.argtypes = [vp, C.c_int]
+    L.s3g_learn_flexible.argtypes = [vp, C.c_int]
     L.s3g_learn_num_weights.restype = C.c_long
     L.s3g_learn_num_weights.argtypes = []
     L.s3g_learn_get_weights.argtypes = [vp, fp, C.c_long]

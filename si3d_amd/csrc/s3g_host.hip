@@ -649,7 +649,7 @@ int s3g::init_ctx(const s3g_grid *g, const s3g_params *p, int nlakes, const doub
   c->l2i.assign(g->l2i, g->l2i + c->lm);
   c->l2j.assign(g->l2j, g->l2j + c->lm);
   if (rc == 0) rc = pcg_setup(c, lN.data(), g->l2i, g->l2j);
-  if (rc == 0 && !(G.part = dalloc<double>(c, 8 * (size_t)G.nblk + 64))) rc = S3G_ERR_HIP;
+  if (rc == 0 && !(G.part = dalloc<double>(c, 10 * (size_t)G.nblk + 64))) rc = S3G_ERR_HIP;  // 10 partial slots
   if (rc == 0 && !(G.iters = dalloc<int>(c, 4))) rc = S3G_ERR_HIP;
   if (rc == 0 && !(G.tol2 = dalloc<double>(c, 1))) rc = S3G_ERR_HIP;
   if (rc != 0) {

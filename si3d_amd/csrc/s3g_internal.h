@@ -97,7 +97,7 @@ struct MgLevel {
 struct Pcg {
   int n, nblk;
   double *r, *r1, *z, *p0, *p1, *q, *diag, *x;  // r / r1: residual of even / odd iterations
-  double *part;    // partial sums: [4][nblk]
+  double *part;    // partial sums: [10][nblk]
   double *tol2 = nullptr;  // squared relative tolerance of the running CG
   double *scal;    // device scalars
   int *iters;      // device iteration counter
@@ -113,6 +113,10 @@ struct Pcg {
   int *lake_off = nullptr;      // [nlakes+1] on the device
   double *lake_part = nullptr;  // [nlakes][kLakeSlices][2]
   double *h_lake = nullptr;     // pinned copy of lake_part
+  // learned correction stage (flexible CG): the network's correction y and
+  // r - A y, allocated on first use
+  double *yc = nullptr, *ry = nullptr;
+  float *ycf = nullptr, *ryf = nullptr;
 };
 
 // FP64 view of the free-surface system for the outer solver level (true
@@ -281,6 +285,16 @@ void obc_shift(s3g_ctx *c, int mode);
 // unet.hip
 int learn_before_solve(s3g_ctx *c, const void *outer, double *x);
 int learn_after_solve(s3g_ctx *c, const void *outer);
+// Learned correction stage of flexible CG (s3g_learn_flexible): the period of
+// the stage for the current solve (0: off), and the network run on a CG
+// residual r (double or float) whose (r,r) partials are rr_part[0..nblk-1].
+struct LearnOut {
+  const float *out;   // dense network output (normalised correction)
+  const int *pix;     // wet column -> dense pixel
+  const double *sig;  // device scalar: the residual's rms (the output's scale)
+};
+int learn_flex_period(const s3g_ctx *c);
+int learn_correction(s3g_ctx *c, const void *r, bool fp32, const double *rr_part, int nblk, LearnOut *out);
 void learn_destroy(s3g_ctx *c);
 int pcg_setup(s3g_ctx *c, const int *lN0, const int *l2i, const int *l2j);
 // host

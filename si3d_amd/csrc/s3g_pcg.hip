@@ -83,8 +83,11 @@ __device__ __forceinline__ void reduce_slots(const double *part, const int (&slo
   __syncthreads();
 }
 
-// Partial sums layout: [RZ0 | RZ1 | RR0 | RR1 | PQ | BB | TR] x nblk
-enum { P_RZ0 = 0, P_RZ1 = 1, P_RR0 = 2, P_RR1 = 3, P_PQ = 4, P_BB = 5, P_TR = 6, P_NUM = 7 };
+// Partial sums layout: [RZ0 | RZ1 | RR0 | RR1 | PQ | BB | TR | ZR | RY | YAY] x nblk
+// (ZR: (z_i, r_{i-1}) of flexible CG; RY, YAY: (r, y) and (y, A y) of the
+// learned correction y)
+enum { P_RZ0 = 0, P_RZ1 = 1, P_RR0 = 2, P_RR1 = 3, P_PQ = 4, P_BB = 5, P_TR = 6, P_ZR = 7, P_RY = 8, P_YAY = 9,
+       P_NUM = 10 };
 
 template <typename T>
 struct CgArgs {
@@ -100,6 +103,8 @@ struct CgArgs {
   int nlines;
   const int *line_start;
   const T *lc, *ld;
+  int flex;  // flexible CG: Polak-Ribiere beta from the extra ZR partial
+  T *yc, *ry;  // correction stage: the network's y and r - A y
 };
 
 template <typename T>
@@ -170,10 +175,20 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) k_cg_dir(CgArgs<T> A, int i) {
   __shared__ double lds[32];
   const int cur = i & 1;
-  double v[4];
-  reduce_slots<4>(A.part, {P_RR0 + cur, P_BB, P_RZ0 + cur, P_RZ0 + (cur ^ 1)}, A.nblk, lds, v);
+  double v[5];
+  if (A.flex) {
+    reduce_slots<5>(A.part, {P_RR0 + cur, P_BB, P_RZ0 + cur, P_RZ0 + (cur ^ 1), P_ZR}, A.nblk, lds, v);
+  } else {
+    double u[4];
+    reduce_slots<4>(A.part, {P_RR0 + cur, P_BB, P_RZ0 + cur, P_RZ0 + (cur ^ 1)}, A.nblk, lds, u);
+    for (int q = 0; q < 4; ++q) v[q] = u[q];
+    v[4] = 0.0;
+  }
   if (v[0] <= *A.tol2p * v[1]) return;  // converged: every block takes the same branch
-  const double beta = (i > 0) ? v[2] / v[3] : 0.0;
+  // Fletcher-Reeves, or Polak-Ribiere for flexible CG (the two agree while
+  // the preconditioner is fixed, where (z_i, r_{i-1}) = 0)
+  double beta = (i > 0) ? (v[2] - v[4]) / v[3] : 0.0;
+  if (A.flex && !isfinite(beta)) beta = 0.0;  // restart along z
   const T *pold = cur ? A.p0 : A.p1;
   T *pnew = cur ? A.p1 : A.p0;
   double pq = 0.0;
@@ -227,6 +242,88 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(CgArgs<T> A, int i) {
     if (JACOBI) A.part[(P_RZ0 + nxt) * A.nblk + blockIdx.x] = rzn;
     A.part[(P_RR0 + nxt) * A.nblk + blockIdx.x] = rrn;
     if (blockIdx.x == 0) A.iters[0] += 1;  // iterations since the run's first residual
+  }
+}
+
+// ---- flexible CG: the learned correction stage ----------------------------
+// (z_i, r_{i-1}) partial for the Polak-Ribiere beta; rprev is the residual
+// entering the previous iteration.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_flex_zr(CgArgs<T> A, const T *rprev) {
+  __shared__ double lds[16];
+  double s = 0.0;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < A.n; c += gridDim.x * blockDim.x)
+    s += (double)A.z[c] * (double)rprev[c];
+  s = block_sum(s, lds);
+  if (threadIdx.x == 0) A.part[P_ZR * A.nblk + blockIdx.x] = s;
+}
+
+// y = sigma * out(pix) (the network's estimate of A^-1 r) into yc, A y into
+// ry, partials (r, y) and (y, A y).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_flex_apply(CgArgs<T> A, const T *r, const int *pix, const float *out,
+                                                      const double *sig) {
+  __shared__ double lds[16];
+  const double s = sig[0];
+  auto y = [&](int j) { return j == A.n ? 0.0 : (double)(T)(s * (double)out[pix[j]]); };
+  double ry = 0.0, yay = 0.0;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < A.n; c += gridDim.x * blockDim.x) {
+    const int lW = A.lW[c], lS = A.lS[c], lN = A.lN[c], lE = A.lE[c];
+    const double yc = y(c);
+    A.yc[c] = (T)yc;
+    const double ay = (double)A.rr[c] * yc - (double)A.sx[lW] * y(lW) - (double)A.sy[lS] * y(lS) -
+                      (double)A.sy[c] * y(lN) - (double)A.sx[c] * y(lE);
+    A.ry[c] = (T)ay;
+    ry += (double)r[c] * yc;
+    yay += yc * ay;
+  }
+  ry = block_sum(ry, lds);
+  yay = block_sum(yay, lds);
+  if (threadIdx.x == 0) {
+    A.part[P_RY * A.nblk + blockIdx.x] = ry;
+    A.part[P_YAY * A.nblk + blockIdx.x] = yay;
+  }
+}
+
+// Energy line search on y: gamma = (r, y) / (y, A y) minimises the A-norm of
+// the error after the step gamma y (0 if y is zero or not finite); yc :=
+// gamma y, ry := r - gamma A y.  The step cannot raise the error's A-norm and
+// the preconditioner that follows (multigrid, line or Jacobi, each an A-norm
+// contraction here) lowers it further, so z = gamma y + M^-1 (r - gamma A y)
+// satisfies 2 (r, z) > (z, A z) > 0: a descent direction whatever the network
+// returns.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_flex_scale(CgArgs<T> A, const T *r) {
+  __shared__ double lds[16];
+  double v[2];
+  reduce_slots<2>(A.part, {P_RY, P_YAY}, A.nblk, lds, v);
+  double gamma = v[1] > 0.0 ? v[0] / v[1] : 0.0;
+  if (!isfinite(gamma)) gamma = 0.0;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < A.n; c += gridDim.x * blockDim.x) {
+    A.yc[c] = (T)(gamma * (double)A.yc[c]);
+    A.ry[c] = (T)((double)r[c] - gamma * (double)A.ry[c]);
+  }
+}
+
+// z = y + M^-1 (r - A y): z already holds M^-1 (r - A y), or with JACOBI it is
+// (r - A y) / rr here; partials (r, z) into SLOT and (r_prev, z) into ZR
+// (zero without a previous residual).
+template <typename T, bool JACOBI>
+__global__ void __launch_bounds__(kBlock) k_flex_finish(CgArgs<T> A, const T *r, const T *rprev, int slot) {
+  __shared__ double lds[16];
+  double rz = 0.0, zr = 0.0;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < A.n; c += gridDim.x * blockDim.x) {
+    const double m = JACOBI ? (double)(T)((double)A.ry[c] / (double)A.rr[c]) : (double)A.z[c];
+    const double z = (double)(T)((double)A.yc[c] + m);
+    A.z[c] = (T)z;
+    rz += (double)r[c] * z;
+    if (rprev) zr += (double)rprev[c] * z;
+  }
+  rz = block_sum(rz, lds);
+  zr = block_sum(zr, lds);
+  if (threadIdx.x == 0) {
+    A.part[slot * A.nblk + blockIdx.x] = rz;
+    A.part[P_ZR * A.nblk + blockIdx.x] = zr;
   }
 }
 
@@ -783,11 +880,37 @@ static int run_cg(s3g_ctx *c, CgArgs<T> &A, int maxit, int hint, Tail tail) {
   const bool jac = (c->prec_kind == S3G_PREC_JACOBI) || (c->prec_kind == S3G_PREC_LINE && A.nlines == 0);
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c->stream, c->pcg.tol2, A.tol2);
   const bool mg = c->prec_kind == S3G_PREC_MG;
+  // flexible CG with the learned correction stage on iterations 0, period, ...
+  const int period = A.flex ? learn_flex_period(c) : 0;
+  bool corr_ok = true;
+  // the stage on the residual rc entering an iteration whose (r,z) partial
+  // goes to `slot` (rprev: the residual entering the iteration before)
+  auto correction = [&](const T *rc, const T *rprev, int slot) {
+    LearnOut lo{};
+    if (learn_correction(c, rc, sizeof(T) == 4, A.part + (P_RR0 + slot - P_RZ0) * A.nblk, A.nblk, &lo) < 0) {
+      corr_ok = false;
+      return;
+    }
+    hipLaunchKernelGGL((k_flex_apply<T>), g, b, 0, c->stream, A, rc, lo.pix, lo.out, lo.sig);
+    hipLaunchKernelGGL((k_flex_scale<T>), g, b, 0, c->stream, A, rc);
+    if (jac) {
+      hipLaunchKernelGGL((k_flex_finish<T, true>), g, b, 0, c->stream, A, rc, rprev, slot);
+    } else {
+      apply_prec(c, A, (const T *)A.ry, P_ZR, false);  // its (r - A y, z) partial is replaced below
+      hipLaunchKernelGGL((k_flex_finish<T, false>), g, b, 0, c->stream, A, rc, rprev, slot);
+    }
+  };
+  bool capturing = false;  // the graph of iterations 2 and 3 never holds the stage
+  auto corrects = [&](int i) { return !capturing && period > 0 && i % period == 0; };  // iteration i starts with it
   if (!jac) {
     hipLaunchKernelGGL((k_cg_residual<T, false>), g, b, 0, c->stream, A);
-    apply_prec(c, A, (const T *)A.r, P_RZ0, false);
+    if (corrects(0))
+      correction((const T *)A.r, (const T *)nullptr, P_RZ0);
+    else
+      apply_prec(c, A, (const T *)A.r, P_RZ0, false);
   } else {
     hipLaunchKernelGGL((k_cg_residual<T, true>), g, b, 0, c->stream, A);
+    if (corrects(0)) correction((const T *)A.r, (const T *)nullptr, P_RZ0);
   }
   MgPair<T> M0{};
   if (mg) {  // level 0 -> 1 of the hierarchy; fb is recomputed inside the fused kernel
@@ -800,7 +923,13 @@ static int run_cg(s3g_ctx *c, CgArgs<T> &A, int maxit, int hint, Tail tail) {
   }
   auto iteration = [&](int i) {
     const T *rnext = ((i + 1) & 1) ? A.r1 : A.r;
+    const T *rcur = (i & 1) ? A.r1 : A.r;
     hipLaunchKernelGGL((k_cg_dir<T>), g, b, 0, c->stream, A, i);
+    if (corrects(i + 1)) {
+      hipLaunchKernelGGL((k_cg_update<T, false>), g, b, 0, c->stream, A, i);
+      correction(rnext, rcur, P_RZ0 + ((i + 1) & 1));
+      return;
+    }
     if (mg) {
       hipLaunchKernelGGL((k_cg_update_mg<T>), g, b, 0, c->stream, A, M0, i);
       apply_prec(c, A, rnext, P_RZ0 + ((i + 1) & 1), true);
@@ -810,6 +939,7 @@ static int run_cg(s3g_ctx *c, CgArgs<T> &A, int maxit, int hint, Tail tail) {
     } else {
       hipLaunchKernelGGL((k_cg_update<T, true>), g, b, 0, c->stream, A, i);
     }
+    if (A.flex) hipLaunchKernelGGL((k_flex_zr<T>), g, b, 0, c->stream, A, rcur);
   };
   // The kernels depend on the iteration index only through i & 1 and i > 0,
   // so iterations (2m, 2m+1), m >= 1, are one launch of a graph of
@@ -829,8 +959,10 @@ static int run_cg(s3g_ctx *c, CgArgs<T> &A, int maxit, int hint, Tail tail) {
       hipGraph_t gr = nullptr;
       bool ok = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
       if (ok) {
+        capturing = true;
         iteration(2);
         iteration(3);
+        capturing = false;
         ok = hipStreamEndCapture(c->stream, &gr) == hipSuccess && gr;
       }
       if (ok) ok = hipGraphInstantiate(&c->cg_exec[gi], gr, nullptr, nullptr, 0) == hipSuccess;
@@ -848,12 +980,15 @@ static int run_cg(s3g_ctx *c, CgArgs<T> &A, int maxit, int hint, Tail tail) {
   int it = 0, slot = -1;
   int chunk = std::max(4, std::min(hint + 2, 512));
   while (it < maxit) {
-    const int nl = std::min(chunk, maxit - it);
+    int nl = std::min(chunk, maxit - it);
+    // a chunk ends before an iteration that runs the correction stage, so
+    // that the host sees convergence before it launches the network
+    if (period > 0) nl = std::min(nl, ((it + 1) / period + 1) * period - 1 - it);
     if (c->timing) timer_begin(c, sizeof(T) == 8 ? "pcg_chunk_f64" : "pcg_chunk_f32", &slot);
     int j = 0;
     while (j < nl) {
       const int i = it + j;
-      if (pair && i >= 2 && (i & 1) == 0 && j + 1 < nl) {
+      if (pair && i >= 2 && (i & 1) == 0 && j + 1 < nl && !corrects(i + 1) && !corrects(i + 2)) {
         hipGraphLaunch(pair, c->stream);
         j += 2;
       } else {
@@ -862,6 +997,7 @@ static int run_cg(s3g_ctx *c, CgArgs<T> &A, int maxit, int hint, Tail tail) {
       }
     }
     if (c->timing) timer_end(c, slot);
+    if (!corr_ok) return -1;
     it += nl;
     tail();
     hipMemcpyAsync(c->h_iters, A.iters, sizeof(int), hipMemcpyDeviceToHost, c->stream);
@@ -940,6 +1076,29 @@ static double lake_relres(const s3g_ctx *c) {
   return worst;
 }
 
+// Buffers of the learned correction stage (y and r - A y), zeroed, kept
+// until the context is destroyed.
+static int flex_buffers(s3g_ctx *c) {
+  Pcg &P = c->pcg;
+  const bool fp32 = c->precision == S3G_MIXED;
+  if (fp32 ? (P.ycf != nullptr) : (P.yc != nullptr)) return 0;
+  const size_t bytes = (size_t)(P.n + 1) * (fp32 ? sizeof(float) : sizeof(double));
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+    if (a) hipFree(a);
+    return hip_check(hipErrorOutOfMemory, "correction stage buffers");
+  }
+  hipMemsetAsync(a, 0, bytes, c->stream);
+  hipMemsetAsync(b, 0, bytes, c->stream);
+  c->allocs.push_back(a);
+  c->allocs.push_back(b);
+  if (fp32)
+    P.ycf = (float *)a, P.ryf = (float *)b;
+  else
+    P.yc = (double *)a, P.ry = (double *)b;
+  return 0;
+}
+
 template <typename R>
 int solve_free_surface(s3g_ctx *c, double tz) {
   Pcg &P = c->pcg;
@@ -961,10 +1120,13 @@ int solve_free_surface(s3g_ctx *c, double tz) {
   hipLaunchKernelGGL(k_start_guess, g, b, 0, c->stream, P.x, (const double *)D.s, (const double *)D.sp,
                      (const double *)D.spp, P.n, extrap);
   if (c->learn && learn_before_solve(c, &O, P.x) < 0) return S3G_ERR_HIP;
+  const bool flex = c->learn && learn_flex_period(c) > 0;
+  if (flex && flex_buffers(c) < 0) return S3G_ERR_HIP;
 
   if (c->precision == S3G_FP64) {
     CgArgs<double> A = cg_args<double>(c, D.lE, D.lW, D.lN, D.lS, D.rr, D.sx, D.sy, P.x, P.r, P.r1, P.z, P.p0, P.p1, P.q,
                                        D.qq, P.lc, P.ld, 0.5 * tol);
+    if (flex) A.flex = 1, A.yc = P.yc, A.ry = P.ry;
     if (line) hipLaunchKernelGGL((k_line_factor<double>), gl, b, 0, c->stream, A, P.lc, P.ld);
     if (c->prec_kind == S3G_PREC_MG) mg_build(c, A);
     const bool lakes = c->nlakes > 1;
@@ -995,6 +1157,7 @@ int solve_free_surface(s3g_ctx *c, double tz) {
                        (const double *)D.sy, P.rrf, P.sxf, P.syf, P.n);
     CgArgs<float> A = cg_args<float>(c, D.lE, D.lW, D.lN, D.lS, P.rrf, P.sxf, P.syf, P.df, P.rf, P.rf1, P.zf, P.pf0, P.pf1,
                                      P.qf, P.bf, P.lcf, P.ldf, kInnerTol);
+    if (flex) A.flex = 1, A.yc = P.ycf, A.ry = P.ryf;
     if (line) hipLaunchKernelGGL((k_line_factor<float>), gl, b, 0, c->stream, A, P.lcf, P.ldf);
     if (c->prec_kind == S3G_PREC_MG) mg_build(c, A);
     for (int sweep = 0; sweep < 12; ++sweep) {

@@ -943,6 +943,29 @@ __global__ void k_learn_apply(double *x, int n, const int *pix, const float *out
     x[c] += sig[0] * (double)out[pix[c]];
 }
 
+// Input plane 0 of the correction stage: r / sigma on the wet pixels, with
+// sigma = rms(r) from the CG's (r,r) partials (every workgroup adds them in
+// the same order); block 0 stores sigma.  The coupling and mask planes are the
+// ones the solve scattered before its first iteration.
+template <typename T>
+__global__ void __launch_bounds__(256) k_flex_input(const T *r, const double *part, int nblk, int n, const int *pix,
+                                                   float *in0, double *sig) {
+  __shared__ double lds[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x) s += part[i];
+  lds[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) lds[threadIdx.x] += lds[threadIdx.x + o];
+    __syncthreads();
+  }
+  const double sg = fmax(sqrt(lds[0] / fmax(1.0, (double)n)), 1e-300);
+  if (blockIdx.x == 0 && threadIdx.x == 0) sig[0] = sg;
+  const double inv = 1.0 / sg;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
+    in0[pix[c]] = (float)((double)r[c] * inv);
+}
+
 struct Learner {
   UNet net;
   int H = 0, W = 0, npx = 0, n = 0;
@@ -956,6 +979,9 @@ struct Learner {
   double *sig = nullptr, *xg = nullptr, *xl = nullptr, *part = nullptr, *gain = nullptr;
   bool used = false;  // the current solve started from the learned guess
   float *cur_slot = nullptr;
+  int flex_period = 0;      // correction stage every flex_period CG iterations (0: off)
+  bool flex_ready = false;  // this solve scattered the coupling and mask planes
+  long corrections = 0;     // correction stages run since the mode was set
   std::mt19937 rng;
 };
 
@@ -1003,7 +1029,9 @@ int learn_before_solve(s3g_ctx *c, const void *outer_v, double *x) {
   if (!L) return 0;
   const Outer &O = *(const Outer *)outer_v;
   const bool use = L->mode >= 1 && L->steps > 0;
-  if (!L->recording && !use) {
+  const bool flex = L->flex_period > 0 && L->steps > 0;
+  L->flex_ready = false;
+  if (!L->recording && !use && !flex) {
     L->cur_slot = nullptr;
     return 0;
   }
@@ -1013,9 +1041,12 @@ int learn_before_solve(s3g_ctx *c, const void *outer_v, double *x) {
   hipLaunchKernelGGL(k_learn_sigma, dim3(1), dim3(256), 0, c->stream, (const double *)L->part, nb, L->n, L->sig);
   hipLaunchKernelGGL(k_learn_normalise, dim3(nb), dim3(256), 0, c->stream, L->cur_slot, L->n, (const double *)L->sig,
                      (const double *)x, L->xg);
-  if (use) {
+  if (use || flex) {  // input planes: r0/sigma, coupling, mask (the last two also serve the correction stage)
     hipLaunchKernelGGL(k_learn_scatter, dim3(nb), dim3(256), 0, c->stream, (const float *)L->cur_slot, L->n,
                        (const int *)L->pix, L->npx, L->in3, (float *)nullptr);
+    L->flex_ready = flex;
+  }
+  if (use) {
     hipLaunchKernelGGL(k_pack_dense, UNet::ew_grid(L->npx), dim3(256), 0, c->stream, (const float *)L->in3, L->npx,
                        L->net.X0);
     L->net.forward(c->stream);
@@ -1035,6 +1066,31 @@ int learn_before_solve(s3g_ctx *c, const void *outer_v, double *x) {
 }
 
 // After the CG: the target (zeta - guess) / sigma completes the pair.
+int learn_flex_period(const s3g_ctx *c) {
+  const Learner *L = c->learn;
+  return (L && L->flex_ready) ? L->flex_period : 0;
+}
+
+int learn_correction(s3g_ctx *c, const void *r, bool fp32, const double *rr_part, int nblk, LearnOut *out) {
+  Learner *L = c->learn;
+  if (!L || !L->flex_ready) return learn_fail("learned correction stage without a prepared solve");
+  const int nb = std::min(1024, (L->n + 255) / 256);
+  if (fp32)
+    hipLaunchKernelGGL((k_flex_input<float>), dim3(nb), dim3(256), 0, c->stream, (const float *)r, rr_part, nblk,
+                       L->n, (const int *)L->pix, L->in3, L->sig + 1);
+  else
+    hipLaunchKernelGGL((k_flex_input<double>), dim3(nb), dim3(256), 0, c->stream, (const double *)r, rr_part, nblk,
+                       L->n, (const int *)L->pix, L->in3, L->sig + 1);
+  hipLaunchKernelGGL(k_pack_dense, UNet::ew_grid(L->npx), dim3(256), 0, c->stream, (const float *)L->in3, L->npx,
+                     L->net.X0);
+  L->net.forward(c->stream);
+  L->corrections += 1;
+  out->out = L->net.out;
+  out->pix = L->pix;
+  out->sig = L->sig + 1;
+  return hip_check(hipGetLastError(), "learned correction stage");
+}
+
 int learn_after_solve(s3g_ctx *c, const void *outer_v) {
   Learner *L = c->learn;
   if (!L || !L->cur_slot) return 0;
@@ -1115,7 +1171,7 @@ int s3g_learn_setup(s3g_ctx *c, int capacity, unsigned seed) {
   L->in3 = (float *)dal((size_t)3 * L->npx * sizeof(float));
   L->tgt = (float *)dal((size_t)L->npx * sizeof(float));
   L->mask = (float *)dal((size_t)L->npx * sizeof(float));
-  L->sig = (double *)dal(sizeof(double));
+  L->sig = (double *)dal(2 * sizeof(double));  // [0] initial residual, [1] correction stage
   L->xg = (double *)dal((size_t)(c->lm + 1) * sizeof(double));
   L->xl = (double *)dal((size_t)(c->lm + 1) * sizeof(double));
   L->part = (double *)dal(2048 * sizeof(double));
@@ -1144,6 +1200,13 @@ int s3g_learn_mode(s3g_ctx *c, int mode) {
   c->learn->mode = mode;
   hipMemsetAsync(c->learn->gain, 0, 3 * sizeof(double), c->stream);
   return hip_check(hipStreamSynchronize(c->stream), "s3g_learn_mode");
+}
+
+int s3g_learn_flexible(s3g_ctx *c, int period) {
+  if (!c || !c->learn || period < 0) return S3G_ERR_ARG;
+  c->learn->flex_period = period;
+  c->learn->corrections = 0;
+  return 0;
 }
 
 int s3g_learn_train(s3g_ctx *c, int steps, int batch, double lr, double *loss_out) {
@@ -1221,6 +1284,8 @@ int s3g_learn_info(s3g_ctx *c, s3g_learn_stats *out) {
   out->guess_solves = (long)g[1];
   out->guess_error_ratio = g[1] > 0 ? g[0] / g[1] : NAN;
   out->guess_rejected = (long)g[2];
+  out->corrections = L->corrections;
+  out->flex_period = L->flex_period;
   return hip_check(hipGetLastError(), "s3g_learn_info");
 }
 

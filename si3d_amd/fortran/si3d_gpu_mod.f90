@@ -215,6 +215,13 @@ MODULE si3d_gpu
          INTEGER(C_INT), VALUE :: mode
          INTEGER(C_INT) :: s3g_learn_mode
       END FUNCTION s3g_learn_mode
+
+      FUNCTION s3g_learn_flexible(ctx, period) BIND(C, NAME="s3g_learn_flexible")
+         IMPORT :: C_INT, C_PTR
+         TYPE(C_PTR), VALUE :: ctx
+         INTEGER(C_INT), VALUE :: period
+         INTEGER(C_INT) :: s3g_learn_flexible
+      END FUNCTION s3g_learn_flexible
    END INTERFACE
 
 CONTAINS

@@ -224,6 +224,11 @@ class GpuLake:
     def learn_mode(self, mode):
         return check(self.L.s3g_learn_mode(self.h, mode), "s3g_learn_mode")
 
+    def learn_flexible(self, period):
+        """Learned correction stage inside flexible CG every `period`
+        iterations (0: off)."""
+        return check(self.L.s3g_learn_flexible(self.h, period), "s3g_learn_flexible")
+
     def learn_weights(self, values=None):
         n = self.L.s3g_learn_num_weights()
         if values is None:

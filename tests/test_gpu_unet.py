@@ -110,3 +110,46 @@ def test_learned_guess_worse_than_plain_is_rejected():
     assert A.stats()["pcg_iters_total"] - ia0 == B.stats()["pcg_iters_total"] - ib0
     for name in ("s", "uh", "vh", "sal"):
         assert np.array_equal(A.get(name), B.get(name)), name
+
+
+@pytest.mark.parametrize("precision,prec,period", [("fp64", "mg", 4), ("fp64", "jacobi", 1), ("mixed", "mg", 3)])
+def test_flexible_correction_stage_same_error(precision, prec, period):
+    """The learned correction stage inside flexible CG (Polak-Ribiere beta):
+    after training on the recorded solves, lake A runs 10 steps with the stage
+    every `period` iterations and the plain initial guess, lake B without it.
+    Every solve still reaches the FP64 true-residual tolerance, the fields
+    agree to it, and the stage ran on the iterations it was asked for.  (With
+    a network trained on the initial residuals of plain solves the stage does
+    not cut the iteration count: the residuals inside CG have other spectra;
+    the bound below only excludes a broken iteration.)"""
+    from si3d_amd.model import GpuLake
+    print("case", precision, prec, period, flush=True)
+    deck = small_ellipse(nts=40)
+    A = GpuLake(deck, precision=precision, preconditioner=prec, tol=1e-10)
+    B = GpuLake(deck, precision=precision, preconditioner=prec, tol=1e-10)
+    A.learn_setup(capacity=64, seed=3)
+    for n in range(1, 21):
+        A.step(n)
+        B.step(n)
+    A.learn_train(400, batch=4, lr=1e-3)
+    A.learn_record(False)
+    A.learn_flexible(period)
+    ia0, ib0 = A.stats()["pcg_iters_total"], B.stats()["pcg_iters_total"]
+    for n in range(21, 31):
+        A.step(n)
+        B.step(n)
+        assert A.stats()["pcg_relres_last"] <= 1e-10
+    ia = A.stats()["pcg_iters_total"] - ia0
+    ib = B.stats()["pcg_iters_total"] - ib0
+    info = A.learn_info()
+    print(f"PCG iterations over 10 steps: flexible with the stage every {period}: {ia}, plain {ib}; "
+          f"stages run {info['corrections']}", flush=True)
+    assert info["flex_period"] == period
+    assert info["corrections"] >= 30  # at least the one on iteration 0 of every solve
+    assert info["guess_solves"] == 0  # the initial guess stayed plain
+    assert ia <= 2 * ib
+    # FP64: the solves agree to their tolerance; mixed: the FP32 state rounds
+    # the two solutions differently and the difference grows over 10 steps
+    tol = 1e-6 if precision == "fp64" else 1e-4
+    for name in ("s", "uh", "vh", "sal"):
+        assert relerr(A.get(name), B.get(name)) < tol, name
