@@ -52,9 +52,11 @@ class _Checker:
         self.o, self.d = o, deck
         self.g = _grid(o)
         self.hhs, self.hhu, self.hhv = o.get("hhs"), o.get("hhu"), o.get("hhv")
-        self.worst = dict(h=0.0, hu=0.0, hv=0.0, fu=0.0, fv=0.0, split=0.0, kin=0.0, trap=0.0)
+        self.zl = o.get("zlevel")  # zl[k]: depth of the bottom of layer k (1-based)
+        self.worst = dict(h=0.0, hu=0.0, hv=0.0, fu=0.0, fv=0.0, split=0.0, kin=0.0, trap=0.0, slayer=0.0)
         self.kin_checked = self.kin_skipped = self.kin_dried = 0
         self.double_dried = 0
+        self.fell = 0  # slayer_h calls in which a surface fell through its top layer
         # the cells of an open boundary take their faces' transports and
         # their surface from the boundary code (openbcUVH, MODvel4openBC),
         # not from the momentum solution: left out of the face and
@@ -130,11 +132,45 @@ class _Checker:
             self.kin_dried += int(k1z_after[l] > k1z_before[l])
             self.worst["kin"] = max(self.worst["kin"], abs(wp[l, k1z_before[l] - 1] - rate[l]) / scale)
 
+    def _slayer_sums(self, k1z):
+        """After the solve and slayer_h: the layers from the top index down to
+        the first one whose bottom lies below the new surface hold, together,
+        exactly the water between the surface and that bottom (the upper of
+        them emptied when the surface fell through the top layer).  Where it
+        fell through two layers in one call slayer_h empties both and
+        layer_h repairs the column after vel, so those columns are left
+        out; layers further down keep their thickness of the previous call."""
+        o, g, lm = self.o, self.g, self.o.lm
+        s, h = o.get("s"), o.get("h")
+        for l in range(lm):
+            k = k1z[l]
+            if k >= g["kmz"][l] or self.boundary[l]:
+                continue
+            kb = next((j for j in (k, k + 1) if min(self.zl[j], self.hhs[l]) + s[l] > 0.0), None)
+            if kb is None:
+                continue
+            want = min(self.zl[kb], self.hhs[l]) + s[l]
+            self.worst["slayer"] = max(self.worst["slayer"], abs(h[l, k - 1:kb].sum() - want))
+            self.fell += int(kb > k)
+
     def fd(self, n, istep, lastiter):
-        o = self.o
+        """fd phase by phase (the oracle's fd runs the same calls)."""
+        o, d = self.o, self.d
         before = _ints(o)
         k1z_b, k1u_b, k1v_b = (o.get(x).astype(int) for x in ("k1z", "k1u", "k1v"))
-        o.fd(n, istep, lastiter, n * self.d["dt"] / 3600.0)
+        thrs = n * d["dt"] / 3600.0
+        assert d.get("ntr", 0) == 0
+        for p in range(0, 8):
+            o.phase(p, n, istep, thrs)
+        self._slayer_sums(k1z_b)
+        o.phase(8, n, istep, thrs)
+        if d.get("isal", 0) != 0:
+            o.phase(9, n, istep, thrs)
+            o.phase(10, n, istep, thrs)
+        o.phase(11, n, istep, thrs)
+        if d.get("ismooth", 0) >= 1 and istep == 1:
+            o.phase(12, n, istep, thrs)
+        o.phase(13, n, istep, thrs)
         h = o.get("h")
         kmz = self.g["kmz"]
         self.double_dried += sum(1 for l in range(o.lm) if k1z_b[l] < kmz[l] and h[l, k1z_b[l] - 1] == 0
@@ -150,11 +186,16 @@ class _Checker:
             self.events[n][0] += int(np.sum(after[n] > before[n]))
             self.events[n][1] += int(np.sum(after[n] < before[n]))
 
-    def settrap(self):
+    def settrap(self, second=False):
+        """settrap, or settrap2 with `second` (whose n level is the pp one)."""
         o, g, lm = self.o, self.g, self.o.lm
         before = _ints(o)
-        old = {n: o.get(n) for n in ("uh", "hu", "uhp", "hup", "vh", "hv", "vhp", "hvp")}
-        o.settrap()
+        old = {n: o.get(n) for n in ("uh", "hu", "uhp", "hup", "vh", "hv", "vhp", "hvp",
+                                     "uhpp", "hupp", "vhpp", "hvpp")}
+        if second:
+            o.settrap2()
+        else:
+            o.settrap()
         self._top_index()
         self._count(before)
         # the n+1/2 transport of a face's wet layers, summed over depth, is
@@ -165,7 +206,8 @@ class _Checker:
         after = _ints(o)
         for x, nb, kq in (("u", g["lEC"], "k1u"), ("v", g["lNC"], "k1v")):
             q1, h1 = old[x + "h"], old["h" + x]
-            q0, h0 = old[x + "hp"], old["h" + x + "p"]
+            lvl = "pp" if second else "p"
+            q0, h0 = old[x + "h" + lvl], old["h" + x + lvl]
             qn, hn = o.get(x + "hp"), o.get("h" + x + "p")
             for l in range(lm):
                 e = nb[l]
@@ -194,9 +236,7 @@ class _Checker:
         self.fd(n, 1, 0)
         self.settrap()
         self.fd(n, 2, 0)
-        before = _ints(o)
-        o.settrap2()
-        self._count(before)
+        self.settrap(second=True)
         self.fd(n, 2, 1)
         before = _ints(o)
         o.save(2)
@@ -218,6 +258,7 @@ def _run(oracle_lib, deck, nsteps):
 def _assert_invariants(ck, trap=True):
     w = ck.worst
     assert w["h"] < 1e-12 and w["hu"] < 1e-12 and w["hv"] < 1e-12, w
+    assert w["slayer"] < 1e-12, w
     assert w["fu"] < 1e-11 and w["fv"] < 1e-11, w
     assert w["split"] < 1e-11, w
     assert w["kin"] < 1e-8, w
@@ -234,6 +275,8 @@ def test_shallow_basin_dries_and_rewets(oracle_lib):
         assert up >= 20 and down >= 20, (name, ck.events)
     _assert_invariants(ck)
     assert ck.kin_dried >= 10, ck.kin_dried
+    print("surface fell through the top layer", ck.fell, "times")
+    assert ck.fell >= 10, ck.fell
     assert ck.kin_skipped <= 0.01 * ck.kin_checked, (ck.kin_skipped, ck.kin_checked)
     assert dv < 1e-13 * o.lm, dv
 
@@ -247,6 +290,8 @@ def test_tidal_flat_dries_and_rewets(oracle_lib):
         assert up >= 100 and down >= 100, (name, ck.events)
     _assert_invariants(ck)
     assert ck.kin_dried >= 50, ck.kin_dried
+    print("surface fell through the top layer", ck.fell, "times")
+    assert ck.fell >= 50, ck.fell
     assert ck.kin_skipped <= 0.05 * ck.kin_checked, (ck.kin_skipped, ck.kin_checked)
 
 
