@@ -75,6 +75,16 @@ TRAFFIC_FP64_1024x512x60 = {
 }
 
 
+# The same for the default workload (2048 x 2048 x 60, dx = 50 m, dt = 30 s,
+# FP64): profiles/hbm_traffic_ellipse_2048x2048x60_dx50_fp64.md.
+TRAFFIC_FP64_2048x2048x60_DX50 = {
+    "exmom_x": 176.8, "exmom_y": 165.8, "matmom_x": 130.7, "matmom_y": 118.3, "matcon": 27.6, "slayer_h": 59.1,
+    "vel": 66.3, "continuity1": 73.1, "exsal_hflux": 126.0, "exsal_rhs": 137.0, "imsal": 65.3, "vel2": 40.6,
+    "turbulence": 216.8, "turb_cell": 133.4, "settrap_cell": 403.7, "settrap2_cell": 259.1, "save_cell": 208.7,
+    "continuity2": 40.4,
+}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +130,9 @@ def parse():
     ap.add_argument("--mixed-error", type=int, default=0, metavar="STEPS",
                     help="advance the lake in FP64 and in mixed precision side by side for STEPS steps and report "
                          "the per-field max / RMS difference (one JSON line) instead of the throughput")
+    ap.add_argument("--tol-floor", type=int, default=0, metavar="STEPS",
+                    help="like --mixed-error, but the second run is FP64 with the free surface solved to 1e-12: "
+                         "how far FP64 itself drifts with the solver tolerance")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (calibrates HBM byte counters) and exit")
     return ap.parse_args()
@@ -186,17 +199,21 @@ def copy_bandwidth(mib=1024, reps=10):
     return 2 * nbytes * reps / (ms.value * 1e-3) / 1e9
 
 
-def mixed_error(a):
+def mixed_error(a, floor=False):
     """The mixed-precision run against the FP64 run of the same lake: per-field
     max and RMS difference over the wet cells after a.mixed_error steps (and at
     every quarter of the run), absolute and relative to the FP64 field's max /
-    RMS.  Both runs solve the free surface to the same tolerance."""
+    RMS.  Both runs solve the free surface to the same tolerance.  With
+    `floor` the second run is FP64 solved to 1e-12 instead (a.tol_floor
+    steps): the spread FP64 itself has at the bench's tolerance."""
     from si3d_amd import lake
     from si3d_amd.model import GpuLake
-    nsteps = a.mixed_error
+    nsteps = a.tol_floor if floor else a.mixed_error
     deck = lake.ellipse(nx=a.nx, ny=a.ny, dx=a.dx, nlayers=a.layers, dt=a.dt, nts=nsteps + 2, seed=1)
     f64 = GpuLake(deck, precision="fp64", preconditioner=a.precond, tol=a.tol)
-    mix = GpuLake(deck, precision="mixed", preconditioner=a.precond, tol=a.tol)
+    mix = (GpuLake(deck, precision="fp64", preconditioner=a.precond, tol=1e-12) if floor else
+           GpuLake(deck, precision="mixed", preconditioner=a.precond, tol=a.tol))
+    second = "fp64_tol1e-12" if floor else "mixed"
     fields = ["s", "uh", "vh", "wp", "sal", "h", "q2", "Av"]
     h0 = f64.get("h")
     wet3 = h0 > 0.0
@@ -227,14 +244,15 @@ def mixed_error(a):
         mix.step(n)
         if n % max(1, nsteps // 4) == 0 or n == nsteps:
             checkpoints[n] = errors()
-            print(f"mixed-error: step {n} of {nsteps}, {time.perf_counter() - t0:.0f} s, "
+            print(f"{second} vs fp64: step {n} of {nsteps}, {time.perf_counter() - t0:.0f} s, "
                   f"s max_rel {checkpoints[n]['s']['max_rel']:.2e}", file=sys.stderr, flush=True)
-    print(json.dumps({"report": "mixed precision vs FP64, same lake, same free-surface tolerance",
+    print(json.dumps({"report": (f"FP64 with the free surface solved to 1e-12 vs FP64 at {a.tol:g}, same lake" if floor
+                                 else "mixed precision vs FP64, same lake, same free-surface tolerance"),
                       "workload": f"ellipse lake {a.nx}x{a.ny}x{a.layers}, dx={a.dx:g} m, dt={a.dt:g} s, {nsteps} steps, "
                                   f"PCG ({a.precond}) tol {a.tol:g}",
                       "errors_after": {str(k): v for k, v in checkpoints.items()},
                       "pcg_iters_per_step": {"fp64": f64.stats()["pcg_iters_total"] / nsteps,
-                                             "mixed": mix.stats()["pcg_iters_total"] / nsteps}}))
+                                             second: mix.stats()["pcg_iters_total"] / nsteps}}))
 
 
 def per_solve_iterations(g, deck, n_first, nsteps):
@@ -346,8 +364,8 @@ def main():
     if a.copy_probe:
         copy_probe(a.copy_probe)
         return
-    if a.mixed_error:
-        mixed_error(a)
+    if a.mixed_error or a.tol_floor:
+        mixed_error(a, floor=bool(a.tol_floor))
         return
     if a.gpus != 1:
         print(f"bench.py: runs on one GPU only; --gpus {a.gpus} is not supported "
@@ -445,10 +463,12 @@ def main():
             bytes_launch = (n_r * elem + n_f64 * 8) * n_wet
             achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
             traffic = None
-            if (not a.ensemble and not a.river and not a.tracers and a.surfbc == 20
-                    and (a.nx, a.ny, a.layers, a.precision) == (1024, 512, 60, "fp64")
-                    and name in TRAFFIC_FP64_1024x512x60):
-                traffic = int(TRAFFIC_FP64_1024x512x60[name] * n_wet)
+            if not a.ensemble and not a.river and not a.tracers and a.surfbc == 20 and a.precision == "fp64":
+                table = {(1024, 512, 60, 100.0, 60.0): TRAFFIC_FP64_1024x512x60,
+                         (2048, 2048, 60, 50.0, 30.0): TRAFFIC_FP64_2048x2048x60_DX50}.get(
+                    (a.nx, a.ny, a.layers, a.dx, a.dt), {})
+                if name in table:
+                    traffic = int(table[name] * n_wet)
             roofline = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                         "traffic": traffic, "bytes_per_launch": bytes_launch,
