@@ -101,6 +101,8 @@ typedef struct {
   double zeta_volume;        /* sum of zeta over wet columns (m)             */
   int lm, km1;
   long n_wet;                /* sum of wet layers at creation                */
+  long graph_replays;        /* kernel sequences of fd / settrap / settrap2 /
+                                save replayed from captured HIP graphs       */
 } s3g_stats;
 
 /* --- Drop-in entry points (replace fd/settrap/settrap2/save) ------------- */

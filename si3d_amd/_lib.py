@@ -68,7 +68,7 @@ class S3gStats(C.Structure):
     _fields_ = [("steps", C.c_long), ("fd_calls", C.c_long), ("solves", C.c_long),
                 ("pcg_iters_total", C.c_long), ("pcg_iters_last", C.c_int),
                 ("pcg_relres_last", C.c_double), ("zeta_volume", C.c_double),
-                ("lm", C.c_int), ("km1", C.c_int), ("n_wet", C.c_long)]
+                ("lm", C.c_int), ("km1", C.c_int), ("n_wet", C.c_long), ("graph_replays", C.c_long)]
 
 
 class S3gLearnStats(C.Structure):

@@ -392,15 +392,18 @@ static int alloc_state(s3g_ctx *c, Dev<R> &D, const int *lE, const int *lW, cons
   D.KPz = c->KP;
   D.lake = nullptr;
   D.lake_km1 = nullptr;
-  D.force = nullptr;
+  // the uniform wind of one lake lives in device memory too (three doubles
+  // written before every fd call), so the kernel sequences of a step take the
+  // same arguments on every call and can be replayed from a captured graph
+  if (!c->d_force) c->d_force = dalloc<double>(c, 3 * (size_t)c->nlakes);
+  D.force = c->d_force;
+  if (!D.force) return S3G_ERR_HIP;
   if (c->nlakes > 1) {
     D.lake = up_i(lake_of);
     int *k = dalloc<int>(c, c->nlakes);
     if (k) hipMemcpy(k, lake_km1, c->nlakes * sizeof(int), hipMemcpyHostToDevice);
     D.lake_km1 = k;
-    c->d_force = dalloc<double>(c, 3 * (size_t)c->nlakes);
-    D.force = c->d_force;
-    if (!D.lake || !D.lake_km1 || !D.force) return S3G_ERR_HIP;
+    if (!D.lake || !D.lake_km1) return S3G_ERR_HIP;
   }
   D.hhs = up_d(hhs, n2);
   D.hhu = up_d(hhu, n2);
@@ -514,6 +517,7 @@ int s3g::init_ctx(const s3g_grid *g, const s3g_params *p, int nlakes, const doub
     }
     if (const char *gs = std::getenv("S3G_GUESS")) c->pcg_guess = (gs[0] == '1') ? 1 : 0;
     if (const char *gr = std::getenv("S3G_GRAPH")) c->cg_graph = (gr[0] != '0');
+    if (const char *gs = std::getenv("S3G_STEP_GRAPH")) c->step_graph = (gs[0] != '0');
     if (const char *x = std::getenv("S3G_EXSAL")) c->exsal_split = std::strcmp(x, "fused") != 0;
     if (const char *x = std::getenv("S3G_TURB")) c->turb_split = std::strcmp(x, "split") == 0;
     if (const char *x = std::getenv("S3G_BUDGET")) c->tke_budget = x[0] != '0';
@@ -755,6 +759,7 @@ int s3g_get_stats(s3g_ctx *c, s3g_stats *out) {
   double v = 0.0;
   for (int l = 0; l < c->lm; ++l) v += s[l];
   c->stats.zeta_volume = v;
+  c->stats.graph_replays = c->seg_replays;
   *out = c->stats;
   return 0;
 }
@@ -816,6 +821,10 @@ void s3g_finalize(s3g_ctx *c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   for (auto &e : c->cg_exec)
     if (e) hipGraphExecDestroy(e);
+  for (auto &a : c->seg_exec)
+    for (auto &b : a)
+      for (auto &e : b)
+        if (e) hipGraphExecDestroy(e);
   s3g::learn_destroy(c);
   for (void *p : c->allocs) hipFree(p);
   for (auto &e : c->ev_pool) {

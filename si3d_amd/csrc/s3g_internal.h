@@ -238,6 +238,15 @@ struct s3g_ctx {
   int cg_hint[4] = {0, 0, 0, 0};
   hipGraphExec_t cg_exec[2] = {nullptr, nullptr};
   std::vector<char> cg_key[2];
+  // The kernel sequences of a step around the free-surface solves -- fd before
+  // and after the solve, settrap, settrap2, save -- captured once per kind
+  // ([segment][istep - 1][lastiter]) and replayed as HIP graphs
+  // (S3G_STEP_GRAPH=0 launches every kernel instead).  Not used with open
+  // boundaries (their values are kernel arguments of each call) or while
+  // kernel timers run.
+  bool step_graph = true;
+  hipGraphExec_t seg_exec[5][2][2] = {};
+  long seg_replays = 0;
   bool tke_budget = true;   // accumulate the closure's basin energy budget (S3G_BUDGET=0: off)
   bool turb_split = false;  // turbulence closure: per-cell interface terms + sweep (else one fused sweep)
   bool tune = true, tuned = false;

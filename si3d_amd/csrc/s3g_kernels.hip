@@ -129,7 +129,7 @@ struct Wind {
 template <typename R>
 __device__ __forceinline__ Wind wind(const Dev<R> &D, int c) {
   if (D.force) {
-    const double *f = D.force + 3 * D.lake[c];
+    const double *f = D.force + (D.lake ? 3 * D.lake[c] : 0);
     return {f[0], f[1], f[2]};
   }
   return {D.uair, D.vair, D.cdw};
@@ -1678,6 +1678,10 @@ static void sync_forcing(s3g_ctx *c) {
   D.uair = c->uair;
   D.vair = c->vair;
   D.cdw = c->cdw;
+  if (c->nlakes == 1 && D.force) {  // the kernels read the wind from D.force
+    c->lake_force.assign({c->uair, c->vair, c->cdw});
+    set_forcing(c);
+  }
 }
 
 template <typename R>
@@ -1917,6 +1921,39 @@ int autotune(s3g_ctx *c) {
 }
 
 
+// Segments of a step whose kernel sequence and arguments are fixed by
+// (istep, lastiter): captured into a HIP graph on first use, replayed after.
+enum { SEG_FD_PRE = 0, SEG_FD_POST = 1, SEG_SETTRAP = 2, SEG_SETTRAP2 = 3, SEG_SAVE = 4 };
+
+template <typename F>
+static int run_segment(s3g_ctx *c, int seg, int istep, int lastiter, F &&launch) {
+  if (!c->step_graph || c->timing || c->obc.n > 0) {
+    launch();
+    return 0;
+  }
+  hipGraphExec_t &ex = c->seg_exec[seg][istep == 1 ? 0 : 1][lastiter ? 1 : 0];
+  if (!ex) {
+    hipGraph_t g = nullptr;
+    bool ok = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (ok) {
+      launch();
+      ok = hipStreamEndCapture(c->stream, &g) == hipSuccess && g;
+    }
+    if (ok) ok = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) == hipSuccess;
+    if (g) hipGraphDestroy(g);
+    if (!ok) {  // launch kernel by kernel from now on
+      (void)hipGetLastError();
+      ex = nullptr;
+      c->step_graph = false;
+      if (c->debug) std::fprintf(stderr, "[s3g] step graph capture failed; launching kernels one by one\n");
+      launch();
+      return 0;
+    }
+  }
+  ++c->seg_replays;
+  return hip_check(hipGraphLaunch(ex, c->stream), "step graph launch");
+}
+
 template <typename R>
 int fd(s3g_ctx *c, int n, int istep, int lastiter) {
   if (!c->tuned) {
@@ -1932,40 +1969,46 @@ int fd(s3g_ctx *c, int n, int istep, int lastiter) {
   sync_forcing<R>(c);
   Dev<R> &D = dev<R>(c);
   const int L = c->lm;
-  if (c->P.ihd > 1) LAUNCH("hmix", (k_hmix<R>), L, D);
-  LAUNCHC("exmom_x", (k_exmom1<R, true>), (k_exmom<R, true, true>), TILE_EXMOM_X, L, c->km - 1, D, twodt1);
-  obc_exmom<R>(c, true, twodt1);
-  LAUNCH("matmom_x", (k_matmom<R, true>), L, D, tz);
-  LAUNCHC("exmom_y", (k_exmom1<R, false>), (k_exmom<R, false, true>), TILE_EXMOM_Y, L, c->km - 1, D, twodt1);
-  obc_exmom<R>(c, false, twodt1);
-  LAUNCH("matmom_y", (k_matmom<R, false>), L, D, tz);
-  LAUNCH("matcon", (k_matcon<R>), L, D, tz);
-  obc_qq<R>(c, tz);
+  int e = run_segment(c, SEG_FD_PRE, istep, 0, [&] {
+    if (c->P.ihd > 1) LAUNCH("hmix", (k_hmix<R>), L, D);
+    LAUNCHC("exmom_x", (k_exmom1<R, true>), (k_exmom<R, true, true>), TILE_EXMOM_X, L, c->km - 1, D, twodt1);
+    obc_exmom<R>(c, true, twodt1);
+    LAUNCH("matmom_x", (k_matmom<R, true>), L, D, tz);
+    LAUNCHC("exmom_y", (k_exmom1<R, false>), (k_exmom<R, false, true>), TILE_EXMOM_Y, L, c->km - 1, D, twodt1);
+    obc_exmom<R>(c, false, twodt1);
+    LAUNCH("matmom_y", (k_matmom<R, false>), L, D, tz);
+    LAUNCH("matcon", (k_matcon<R>), L, D, tz);
+    obc_qq<R>(c, tz);
+  });
+  if (e < 0) return e;
   int rc = solve_free_surface<R>(c, tz);
   if (rc < 0) return rc;
-  LAUNCHC("slayer_h", (k_slayer_h<R, false>), (k_slayer_h<R, true>), TILE_SLAYER_H, L, c->km1, D);
-  LAUNCH("slayer_h_sentinel", (k_slayer_h_sentinel<R>), c->km1 + 2, D);
-  obc_uvh<R>(c);  // openbcUVH after the solve (482)
-  LAUNCHC("vel", (k_vel<R, false>), (k_vel<R, true>), TILE_VEL, L, c->km - 1, D, tz);
-  LAUNCH("continuity1", (k_continuity<R, 1>), L, D);
-  obc_vel<R>(c);
-  if (c->P.isal != 0) {
-    transport<R, false>(c, sal_of(D), twodt1, 1);
-  }
-  // non-active scalars on the last iteration only (521-547); sources and
-  // sinks are zero (srcsnk00)
-  if (c->ntr > 0 && c->P.niter > 0 && lastiter == 1) {
-    const size_t n3 = (size_t)c->KP * c->LP;
-    for (int nt = 0; nt < c->ntr; ++nt)
-      transport<R, true>(c, Scal<R>{D.trpp + nt * n3, D.tr + nt * n3, nullptr}, twodt1, 2 + nt);
-  }
-  LAUNCHC("vel2", (k_vel2<R, false>), (k_vel2<R, true>), TILE_VEL2, L, c->km - 1, D);
-  if (c->P.ismooth >= 1 && istep == 1) {
-    LAUNCH("smooth_zeta", (k_zeta_levels<R, 2>), L + 1, D);
-    LAUNCH("smooth_col", (k_smooth_col<R>), L, D);
-  }
-  if (c->P.iturb >= 1) turbulence<R>(c, istep);
-  int e = hip_check(hipGetLastError(), "fd launch");
+  e = run_segment(c, SEG_FD_POST, istep, lastiter, [&] {
+    LAUNCHC("slayer_h", (k_slayer_h<R, false>), (k_slayer_h<R, true>), TILE_SLAYER_H, L, c->km1, D);
+    LAUNCH("slayer_h_sentinel", (k_slayer_h_sentinel<R>), c->km1 + 2, D);
+    obc_uvh<R>(c);  // openbcUVH after the solve (482)
+    LAUNCHC("vel", (k_vel<R, false>), (k_vel<R, true>), TILE_VEL, L, c->km - 1, D, tz);
+    LAUNCH("continuity1", (k_continuity<R, 1>), L, D);
+    obc_vel<R>(c);
+    if (c->P.isal != 0) {
+      transport<R, false>(c, sal_of(D), twodt1, 1);
+    }
+    // non-active scalars on the last iteration only (521-547); sources and
+    // sinks are zero (srcsnk00)
+    if (c->ntr > 0 && c->P.niter > 0 && lastiter == 1) {
+      const size_t n3 = (size_t)c->KP * c->LP;
+      for (int nt = 0; nt < c->ntr; ++nt)
+        transport<R, true>(c, Scal<R>{D.trpp + nt * n3, D.tr + nt * n3, nullptr}, twodt1, 2 + nt);
+    }
+    LAUNCHC("vel2", (k_vel2<R, false>), (k_vel2<R, true>), TILE_VEL2, L, c->km - 1, D);
+    if (c->P.ismooth >= 1 && istep == 1) {
+      LAUNCH("smooth_zeta", (k_zeta_levels<R, 2>), L + 1, D);
+      LAUNCH("smooth_col", (k_smooth_col<R>), L, D);
+    }
+    if (c->P.iturb >= 1) turbulence<R>(c, istep);
+  });
+  if (e < 0) return e;
+  e = hip_check(hipGetLastError(), "fd launch");
   return e < 0 ? e : rc;
 }
 
@@ -1973,11 +2016,15 @@ template <typename R>
 int settrap(s3g_ctx *c) {
   Dev<R> &D = dev<R>(c);
   const int L = c->lm;
-  LAUNCH("settrap_zeta", (k_zeta_levels<R, 0>), L + 1, D);
-  LAUNCHC("settrap_cell", (k_settrap_cell<R, 0, false>), (k_settrap_cell<R, 0, true>), TILE_SETTRAP, L + 1, c->km1 + 2, D);
-  LAUNCH("settrap_top", (k_settrap_top<R>), L, D);
-  LAUNCH("continuity2", (k_continuity<R, 2>), L, D);
-  obc_vel<R>(c);
+  const int e = run_segment(c, SEG_SETTRAP, 2, 0, [&] {
+    LAUNCH("settrap_zeta", (k_zeta_levels<R, 0>), L + 1, D);
+    LAUNCHC("settrap_cell", (k_settrap_cell<R, 0, false>), (k_settrap_cell<R, 0, true>), TILE_SETTRAP, L + 1,
+            c->km1 + 2, D);
+    LAUNCH("settrap_top", (k_settrap_top<R>), L, D);
+    LAUNCH("continuity2", (k_continuity<R, 2>), L, D);
+    obc_vel<R>(c);
+  });
+  if (e < 0) return e;
   obc_shift(c, 0);
   return hip_check(hipGetLastError(), "settrap launch");
 }
@@ -1986,11 +2033,15 @@ template <typename R>
 int settrap2(s3g_ctx *c) {
   Dev<R> &D = dev<R>(c);
   const int L = c->lm;
-  LAUNCH("settrap2_zeta", (k_zeta_levels<R, 1>), L + 1, D);
-  LAUNCHC("settrap2_cell", (k_settrap_cell<R, 1, false>), (k_settrap_cell<R, 1, true>), TILE_SETTRAP, L, c->km1 + 2, D);
-  LAUNCH("settrap_top", (k_settrap_top<R>), L, D);
-  LAUNCH("continuity2", (k_continuity<R, 2>), L, D);
-  obc_vel<R>(c);
+  const int e = run_segment(c, SEG_SETTRAP2, 2, 0, [&] {
+    LAUNCH("settrap2_zeta", (k_zeta_levels<R, 1>), L + 1, D);
+    LAUNCHC("settrap2_cell", (k_settrap_cell<R, 1, false>), (k_settrap_cell<R, 1, true>), TILE_SETTRAP, L,
+            c->km1 + 2, D);
+    LAUNCH("settrap_top", (k_settrap_top<R>), L, D);
+    LAUNCH("continuity2", (k_continuity<R, 2>), L, D);
+    obc_vel<R>(c);
+  });
+  if (e < 0) return e;
   return hip_check(hipGetLastError(), "settrap2 launch");
 }
 
@@ -1998,18 +2049,22 @@ template <typename R>
 int save(s3g_ctx *c, int istep) {
   Dev<R> &D = dev<R>(c);
   const int L = c->lm;
-  if (istep == 1) {
-    LAUNCHC("save_cell", (k_save_cell<R, 1, false>), (k_save_cell<R, 1, true>), TILE_SAVE, L + 1, c->km1 + 2, D);
-    LAUNCH("save_top", (k_save_top<R, 1>), L + 1, D);
-  } else {
-    LAUNCHC("save_cell", (k_save_cell<R, 2, false>), (k_save_cell<R, 2, true>), TILE_SAVE, L + 1, c->km1 + 2, D);
-    LAUNCH("save_top", (k_save_top<R, 2>), L + 1, D);
-  }
-  LAUNCH("continuity2", (k_continuity<R, 2>), L, D);
-  obc_vel<R>(c);
+  const int e = run_segment(c, SEG_SAVE, istep, 0, [&] {
+    if (istep == 1) {
+      LAUNCHC("save_cell", (k_save_cell<R, 1, false>), (k_save_cell<R, 1, true>), TILE_SAVE, L + 1, c->km1 + 2, D);
+      LAUNCH("save_top", (k_save_top<R, 1>), L + 1, D);
+    } else {
+      LAUNCHC("save_cell", (k_save_cell<R, 2, false>), (k_save_cell<R, 2, true>), TILE_SAVE, L + 1, c->km1 + 2, D);
+      LAUNCH("save_top", (k_save_top<R, 2>), L + 1, D);
+    }
+    LAUNCH("continuity2", (k_continuity<R, 2>), L, D);
+    obc_vel<R>(c);
+    if (c->ntr > 0)  // tracerpp = tracer (3772-3774, 3927-3929)
+      hipMemcpyAsync(D.trpp, D.tr, (size_t)c->ntr * c->KP * c->LP * sizeof(R), hipMemcpyDeviceToDevice,
+                     c->stream);
+  });
+  if (e < 0) return e;
   obc_shift(c, istep == 1 ? 1 : 2);
-  if (c->ntr > 0)  // tracerpp = tracer (3772-3774, 3927-3929)
-    hipMemcpyAsync(D.trpp, D.tr, (size_t)c->ntr * c->KP * c->LP * sizeof(R), hipMemcpyDeviceToDevice, c->stream);
   return hip_check(hipGetLastError(), "save launch");
 }
 
