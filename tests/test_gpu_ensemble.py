@@ -93,3 +93,40 @@ def test_ensemble_rejects_heat_budget():
     decks[2] = lake.heat_budget(decks[2], 1)
     with pytest.raises(RuntimeError, match="ifsurfbc"):
         GpuEnsemble(decks)
+
+
+def test_c4_64_lakes_volume_residual_and_one_lake_alone():
+    """The 64-lake ensemble of the scope (square sides 128-768, 20-80 layers,
+    winds 0.05-0.3 Pa from any direction; 12.1M columns, N_wet 4.2e8) in mixed
+    precision, 4 steps: every lake's volume is conserved on its own (closed
+    basins; zeta is FP64 in every mode), every solve meets the tolerance for
+    the worst lake's own right-hand side, and the smallest lake's free surface
+    equals the same lake run alone to what FP32 state arithmetic and the
+    solver tolerance leave (the ensemble's CG takes other iterations)."""
+    from si3d_amd.model import GpuEnsemble, GpuLake
+    decks = lake.ensemble(64, seed=7, side=(128, 768), nts=12)
+    print("case: 64-lake ensemble, mixed precision", sum(lake.n_wet_cells(d) for d in decks), "wet cells",
+          flush=True)
+    E = GpuEnsemble(decks, precision="mixed", preconditioner="mg", tol=1e-10)
+    print("ensemble created", E.lm, "columns", flush=True)
+    off = E.offsets
+    s0 = E.get("s")
+    v0 = np.array([s0[off[i]:off[i + 1]].sum() for i in range(64)])
+    for n in range(1, 5):
+        E.step(n)
+        st = E.stats()
+        print("step", n, "iterations", st["pcg_iters_last"], "worst lake relres", st["pcg_relres_last"], flush=True)
+        assert st["pcg_relres_last"] <= 1e-10
+    s = E.get("s")
+    i0 = int(np.argmin([lake.n_wet_cells(d) for d in decks]))
+    for i in range(64):
+        si = s[off[i]:off[i + 1]]
+        assert np.all(np.isfinite(si)), i
+        assert np.max(np.abs(si)) > 0.0, i
+        assert abs(si.sum() - v0[i]) <= 1e-9 * si.size * np.max(np.abs(si)), (i, si.sum() - v0[i])
+    alone = GpuLake(decks[i0], precision="mixed", preconditioner="mg", tol=1e-10)
+    for n in range(1, 5):
+        alone.step(n)
+    e = relerr(s[off[i0]:off[i0 + 1]], alone.get("s")[:alone.lm])
+    print("smallest lake", i0, "zeta vs alone", e, flush=True)
+    assert e < 1e-5
