@@ -1,5 +1,6 @@
-"""Writers of pSi3D's node time files (`outt`, si3d_utils.f90:1020-1202) and
-horizontal plane files (`outh`, 2142-2334) for the Python driver, from fields
+"""Writers of pSi3D's output files for the Python driver -- node time files
+(`outt`, si3d_utils.f90:1020-1202), horizontal plane files (`outh`,
+2142-2334) and the others listed below -- from fields
 copied back from the device (`GpuLake.get`, the library's sync_to_host).
 
 With the Fortran driver these files come from the reference's own routines
@@ -19,6 +20,10 @@ from Python:
   per wet cell of the plane, (i, j, u, v, w, scalar, Av, s) on the first frame
   and (u, v, w, scalar, Av, s) with u/v/w averaged to the cell centre on the
   later ones (float32).
+
+* si3d_out.txt (outr and the solver's lines), si3d_log.txt (outlog),
+  section_<j> (outv), spacefile.xml with its empty binary companions (outs)
+  and ScalarBalance.txt (OutScalarEnergyBalance): see their classes.
 
 The run line of a time file carries the wall-clock date of the run, as the
 reference's does; the model date of the plane records is the start date plus
@@ -42,7 +47,7 @@ def fortran_f(x, w, d, scale=0):
     """Fw.d output of x with scale factor kP (x * 10**k), right-justified;
     asterisks when it does not fit."""
     v = float(x) * (10.0 ** scale)
-    s = f"{v:.{d}f}"
+    s = f"{v:.{d}f}" + ("." if d == 0 else "")
     if len(s) > w and s.startswith("0."):
         s = s[1:]
     elif len(s) > w and s.startswith("-0."):
@@ -432,3 +437,301 @@ class ScalarBalance:
         if self.f is not None:
             self.f.close()
             self.f = None
+
+
+# ---- si3d_out.txt, si3d_log.txt ----------------------------------------------------
+def fortran_g(x, w, d):
+    """Gw.d output of a REAL: F editing with (w-4).(d-k) and four trailing
+    blanks when 0.1 <= |x| (rounded to d significant digits) < 10**d, k its
+    number of integer digits; Ew.d otherwise."""
+    v = float(x)
+    a = abs(v)
+    if v == 0.0:
+        return (fortran_f(0.0, w - 4, d - 1) + " " * 4) if d >= 1 else fortran_e(v, w, d)
+    if 0.1 - 0.5 * 10.0 ** (-d - 1) <= a < 10.0 ** d - 0.5:
+        k = 0
+        while k < d and a >= 10.0 ** k - 0.5 * 10.0 ** (k - d):
+            k += 1
+        return fortran_f(v, w - 4, d - k) + " " * 4
+    return fortran_e(v, w, d)
+
+
+def run_parameters(deck, title=None, start=None):
+    """The values of si3d_inp.txt that the run log prints (input,
+    si3d_utils.f90:36-51): from the deck's own si3d_inp.txt when it was read
+    from one (`inputs.read_case`), else derived from the deck as
+    `inputs.write_inp` writes it."""
+    p = dict(deck.get("inp") or {})
+    if not p:
+        imm, jmm = int(deck["imm"]), int(deck["jmm"])
+        p = dict(xl=imm * deck["dx"], yl=jmm * deck["dy"], zl=deck["zl"], tl=deck["nts"] * deck["dt"],
+                 idx=deck["dx"], idy=deck["dy"], idz=deck["ddz"], dzmin=deck["dzmin"], zetainit=deck["zetainit"],
+                 idt=deck["dt"], itrsca=4, idbg=0,
+                 title=title or "si3d case written by si3d_amd")
+        for k in ("itrap", "niter", "ismooth", "beta", "iturb", "Av0", "Dv0", "iadv", "itrmom", "ihd", "Ax0", "Ay0",
+                  "f", "theta", "ibc", "isal", "cd", "ifsurfbc", "dtsurfbc", "cw", "wa", "phi"):
+            p[k] = deck.get(k, 0)
+        st = start or (2000, 1, 1, 0)
+        p["iyr0"], p["imon0"], p["iday0"], p["ihr0"] = st
+    ob = deck.get("openbc")
+    p["nopen"] = len(ob["iside"]) if ob is not None else 0
+    if title:
+        p["title"] = title
+    return p
+
+
+class RunLog:
+    """si3d_out.txt: the run header of outr (si3d_utils.f90:944-1017) -- title,
+    run number (wall-clock date and time) and start date, the domain and
+    algorithm parameters -- and, per free-surface solve, the lines
+    SolverSparse writes around its NSPCG call (si3d_procedures.f90:1997-2002)
+    with the solver's own line between them at message level 1
+    (nspcg.f:8420-8421): "cg  has converged in N iterations", N being this
+    solver's iteration count (multigrid-preconditioned CG, not NSPCG's
+    MIC(0) CG), and the error line when the solve stops at its iteration
+    limit (ier = 1)."""
+
+    def __init__(self, deck, directory=".", title=None, start=None, now=None):
+        self.p = run_parameters(deck, title, start)
+        self.path = os.path.join(directory, "si3d_out.txt")
+        self.f = open(self.path, "w")
+        self.f.write(self.header(now or datetime.datetime.now()))
+        self.f.flush()
+
+    def header(self, now):
+        p = self.p
+        date, time = now.strftime("%Y%m%d"), now.strftime("%H%M%S")
+        L = [str(p["title"]).rstrip() + "\n",
+             f"Run number = {date:8.8}{time[:4]},  Start date of run:  {int(p['imon0']):2d}/{int(p['iday0']):2d}/"
+             f"{int(p['iyr0']):4d} at {int(p['ihr0']):04d} hours\n", "\n"]
+        F, G = fortran_f, fortran_g
+        L += [f" xl=   {F(p['xl'], 9, 1)}\n", f" yl=   {F(p['yl'], 9, 1)}\n", f" zl=   {F(p['zl'], 9, 3)}\n",
+              f" idx=  {F(p['idx'], 9, 3)}\n", f" idy=  {F(p['idy'], 9, 3)}\n", f" idz=  {F(p['idz'], 9, 3)}\n",
+              f" tl=   {G(p['tl'], 9, 2)}\n", f" idt=  {G(p['idt'], 9, 2)}\n", f" dzmin={F(p['dzmin'], 9, 2)}\n",
+              f" zeta0={F(p['zetainit'], 9, 2)}\n", "\n", "\n"]
+        rows = [("itrap= ", "itrap", "I"), ("niter= ", "niter", "I"), ("smooth=", "ismooth", "I"),
+                ("beta=  ", "beta", "F4"), ("iturb= ", "iturb", "I"), ("Av0=   ", "Av0", "F4"),
+                ("Dv0=   ", "Dv0", "F4"), ("iadv=  ", "iadv", "I"), ("trmom= ", "itrmom", "I"),
+                ("ihd=   ", "ihd", "I"), ("Ax0=   ", "Ax0", "F4"), ("Ay0=   ", "Ay0", "F4"), ("f=     ", "f", "F4"),
+                ("theta= ", "theta", "F4"), ("ibc=   ", "ibc", "I"), ("isal=  ", "isal", "I"),
+                ("trsal= ", "itrsca", "I"), ("nopen= ", "nopen", "I"), ("cd=    ", "cd", "F4"),
+                ("isbc=  ", "ifsurfbc", "I"), ("tsbc=  ", "dtsurfbc", "F2"), ("cw=    ", "cw", "F2"),
+                ("wa=    ", "wa", "F2"), ("phi=   ", "phi", "F2"), ("idbg=  ", "idbg", "I")]
+        for lab, k, e in rows:
+            v = p.get(k, 0)
+            s = f"{int(v):9d}" if e == "I" else F(v, 9, int(e[1]))
+            L.append(f" {lab}{s}\n")
+        L.append("\n")
+        return "".join(L)
+
+    def solve(self, n, iterations, ier=0):
+        self.f.write(f"**Enter nspcg   n = {n:6d}\n")
+        if ier == 0:
+            self.f.write(f"\n cg  has converged in {iterations:5d} iterations\n")
+        self.f.write(f"**Exit nspcg    n = {n:6d}\n")
+        if ier != 0:
+            self.f.write(f"**ERROR{ier:5d} from nspcg\n")
+        self.f.flush()
+
+    def close(self):
+        if self.f is not None:
+            self.f.close()
+            self.f = None
+
+
+class LogFile:
+    """si3d_log.txt (outlog, si3d_utils.f90:3450-3483): opened, written and
+    closed on every step, so it holds the latest " time=  hhh.hhhh hours
+    n= nnnnnn" line while the run goes on, and deleted after the last step
+    (n = nts), as the reference does."""
+
+    def __init__(self, nts, directory="."):
+        self.nts = int(nts)
+        self.path = os.path.join(directory, "si3d_log.txt")
+
+    def write(self, n, thrs):
+        with open(self.path, "w") as f:
+            f.write(f" time={fortran_f(np.float32(thrs), 10, 4)} hours    n={n:6d}\n")
+        if n >= self.nts:
+            os.remove(self.path)
+
+
+def step_logged(g, deck, n, log):
+    """One time step called phase by phase as si3d.f90:324-393 does (and
+    s3g_step: step 1 is trapezoidal only, with at least two iterations; a
+    leapfrog-only step when itrap = 0 or n is not a multiple of itrap), each
+    free-surface solve reported to `log` (a RunLog)."""
+    thrs = n * deck["dt"] / 3600.0
+    niter = int(deck.get("niter", 2))
+    itrap = int(deck.get("itrap", 1))
+
+    def fd(istep, it, last):
+        ier = g.fd(n, istep, lastiter=last, thrs=thrs, iter=it)
+        log.solve(n, g.stats()["pcg_iters_last"], int(ier or 0))
+
+    if n == 1:
+        niter = max(niter, 2)
+    else:
+        lf_only = itrap == 0 or n % max(itrap, 1) != 0
+        fd(1, 0, 1 if lf_only else 0)
+        if lf_only:
+            g.save(1)
+            return
+        g.settrap()
+    for it in range(1, niter + 1):
+        fd(2, it, 1 if it == niter else 0)
+        if it < niter:
+            g.settrap2()
+    g.save(2)
+
+
+# ---- cross-section files ------------------------------------------------------------
+class SectionFiles:
+    """section_<j> files (outv, si3d_utils.f90:1313-1463) for the vertical
+    sections of si3d_inp.txt (a list of (i nodes, j nodes) per section).
+    Fortran unformatted sequential records: the frame count, then per
+    section the point count (wet cells of its columns, layers k1..kmz: kmz-1
+    per column, CountSectionCells 1464-1498) and km1; per frame the
+    time stamp (n, year, month, day, hour) and per point (i, j, node, depth
+    of the layer's top, uhp, vhp, wp, salp, a tracer or the interface mean
+    of Av, the interface mean of Dv) on the first frame, the last six on the
+    later ones; salp (and the tracer) is -99 where hp <= 0.  The first frame
+    takes the n-1 level tracer (tracerpp) and later frames the current one,
+    as the reference does."""
+
+    def __init__(self, g, deck, sections, nts, iox, directory=".", start=(2000, 1, 1, 0)):
+        self.g, self.deck = g, deck
+        self.sections = [(list(map(int, xs)), list(map(int, ys))) for xs, ys in sections]
+        self.n_frames = int(nts / max(iox, 1))
+        self.dir = directory
+        self.start = datetime.datetime(start[0], start[1], start[2]) + datetime.timedelta(hours=start[3] / 100.0)
+        ii, jj = wet_columns(deck)
+        self.pos = {(int(i), int(j)): l for l, (i, j) in enumerate(zip(ii, jj))}
+        self.ntr = 0 if deck.get("tracer_profiles") is None else len(deck["tracer_profiles"])
+        self.files = []
+
+    def _points(self, s, kmz):
+        xs, ys = self.sections[s]
+        rows = []
+        for m, (i, j) in enumerate(zip(xs, ys), start=1):
+            l = self.pos.get((i, j))
+            if l is None:  # a dry cell of the section is skipped (and warned about)
+                continue
+            for k in range(2, int(kmz[l]) + 1):
+                rows.append((i, j, m, l, k))
+        return rows
+
+    def write(self, n):
+        g = self.g
+        kmz = g.get("kmz")
+        f = {x: g.get(x) for x in ("uhp", "vhp", "wp", "salp", "hp", "Av", "Dv")}
+        z = np.asarray(zlevels(self.deck), dtype=np.float64)
+        first = not self.files
+        if self.ntr:
+            trac = g.get(f"tracerpp{self.ntr}" if first else f"tracer{self.ntr}")
+        if first:
+            for s in range(len(self.sections)):
+                fh = open(os.path.join(self.dir, f"section_{s + 1}"), "wb")
+                _record(fh, np.array([self.n_frames], dtype=np.int32))
+                _record(fh, np.array([len(self._points(s, kmz)), lake.km1_of(self.deck)], dtype=np.int32))
+                self.files.append(fh)
+        t = self.start + datetime.timedelta(seconds=n * self.deck["dt"])
+        hrs = (t - datetime.datetime(t.year, t.month, t.day)).total_seconds() / 3600.0
+        ints = np.array([n, t.year, t.month, t.day], dtype=np.int32)
+        hour = np.array([hour_stamp(hrs * 3600.0)], dtype=np.float32)
+        for s, fh in enumerate(self.files):
+            out = []
+            for i, j, m, l, k in self._points(s, kmz):
+                kk = k - 1
+                dry = f["hp"][l, kk] <= 0.0
+                sal = -99.0 if dry else f["salp"][l, kk]
+                if self.ntr:
+                    c9 = -99.0 if dry else trac[l, kk]
+                else:
+                    c9 = 0.5 * (f["Av"][l, kk] + f["Av"][l, kk + 1])
+                c10 = 0.5 * (f["Dv"][l, kk] + f["Dv"][l, kk + 1])
+                vals = [f["uhp"][l, kk], f["vhp"][l, kk], f["wp"][l, kk], sal, c9, c10]
+                if first:
+                    vals = [float(i), float(j), float(m), z[k] - f["hp"][l, kk]] + vals
+                out.append(vals)
+            data = np.asarray(out, dtype=np.float32).reshape(-1, 10 if first else 6)
+            _record(fh, ints, hour, data)
+
+    def close(self):
+        for fh in self.files:
+            fh.close()
+        self.files = []
+
+
+# ---- spacefile.xml ------------------------------------------------------------------
+class SpaceFileXml:
+    """spacefile.xml with its binary companions spacefile2d.bin and
+    spacefile3d.bin (outs, si3d_utils.f90:2605-2760; called every |ipxml|
+    steps when ipxml < 0): on the first call, the xml header for the Gr
+    viewer -- run parameters, title, run number (wall-clock date), the date of
+    the first snapshot (the start date plus itspf rounded up to a multiple of
+    ipxml dt) and its time step ipxml dt (negative, as the reference's
+    INTEGER product of a negative ipxml), and the 2-D (zeta) and 3-D (u, v,
+    w, dz) series descriptions.  The reference never writes records into the
+    two binary files (outs_bin, 2763-2829, is not called), so they are
+    created empty.  Layers of uniform thickness only (ibathyf >= 0), as the
+    reference requires."""
+
+    def __init__(self, deck, ipxml, itspf=0, directory=".", title=None, start=(2000, 1, 1, 0), now=None):
+        if deck.get("ibathyf", 0) < 0:
+            raise ValueError("STOP - Output to xml files only allowed for ibathyf > 0")
+        self.deck, self.ipxml, self.itspf = deck, int(ipxml), int(itspf)
+        self.dir, self.start, self.now = directory, start, now
+        self.p = run_parameters(deck, title, start)
+        self.written = False
+
+    def text(self, now):
+        d, p = self.deck, self.p
+        idt = float(d["dt"])
+        im, jm = int(d["imm"]) + 1, int(d["jmm"]) + 1
+        F = fortran_f
+        idt2 = int(self.ipxml * idt)
+        t0 = datetime.datetime(*self.start[:3]) + datetime.timedelta(hours=self.start[3] / 100.0)
+        if self.itspf == 0:
+            first = 0
+        else:
+            first = int(self.itspf / idt2) * idt2
+            if self.itspf % idt2 != 0:
+                first += idt2
+        t = t0 + datetime.timedelta(seconds=first)
+        isec = int((t - datetime.datetime(t.year, t.month, t.day)).total_seconds())
+        hh, mm, ss = isec // 3600, (isec % 3600) // 60, isec % 60
+        ch = '"' + f"{idt2:8d}" .strip() + '"'
+        L = ['<?xml version="1.0"?>', " ", "<!-- Si3D Output for Gr -->", "<!-- xml header file for binary spacefiles -->",
+             f"<!-- Run Parameters:  dt ={F(idt, 5, 2)} s  dz ={F(d['ddz'], 5, 2)} m  im ={im:5d}  jm ={jm:5d}",
+             f"     itrsca ={int(p.get('itrsca', 4)):2d}  itrap ={int(d.get('itrap', 1)):2d}  cd = {F(d['cd'], 7, 4)}"
+             f"  ismooth ={int(d.get('ismooth', 0)):2d}  beta ={F(d.get('beta', 0.0), 6, 3)}"
+             f"  niter ={int(d.get('niter', 2)):2d}",
+             f"     itrmom ={int(d.get('itrmom', 1)):2d}  f ={F(d.get('f', 0.0), 7, 4)}  wa ={F(d.get('wa', 0.0), 8, 1)}"
+             " -->", " ", "<gov.usgs.gr>", '   <obj class="gov.usgs.sfhydro.si3d.Si3dOutput"',
+             f'        runTitle="{str(p["title"]).strip()}"', f'        runNumber="{now.strftime("%Y%m%d%H%M")}"',
+             f'        startTime="{t.year:4d}/{t.month:02d}/{t.day:02d} {hh:02d}:{mm:02d}:{ss:02d}"'
+             f" timeStepInSeconds={ch}>", " ",
+             '      <obj class="gov.usgs.gr.GrObjectContainer" title="2-D x-y Variables">',
+             '         <dataseries class="gov.usgs.gr.data.RafDataSeries"', '            file="spacefile2d.bin"',
+             '            title="2-D Variables" numDimensions="1">',
+             '            <dim num="0" title="Zeta" units="meters" unitScale="0.0001"/>', " ",
+             "         </dataseries>", "      </obj>", " ",
+             '      <obj class="gov.usgs.gr.GrObjectContainer" title="3-D Variables">',
+             '         <dataseries class="gov.usgs.gr.data.RafDataSeries"', '            file="spacefile3d.bin"',
+             '            title="3-D Variables" numDimensions="4">',
+             '            <dim num="0" title="u" units="m/s" unitScale="0.0001"/>',
+             '            <dim num="1" title="v" units="m/s" unitScale="0.0001"/>',
+             '            <dim num="2" title="w" units="m/s" unitScale="0.00001"/>',
+             '            <dim num="3" title="dz" units="m**2/s" unitScale="0.0001"/>', " ",
+             "         </dataseries>", "      </obj>", "   </obj>", "</gov.usgs.gr>", " "]
+        return "\n".join(L) + "\n"
+
+    def write(self, n):
+        if self.written:
+            return
+        self.written = True
+        with open(os.path.join(self.dir, "spacefile.xml"), "w") as f:
+            f.write(self.text(self.now or datetime.datetime.now()))
+        for name in ("spacefile2d.bin", "spacefile3d.bin"):
+            open(os.path.join(self.dir, name), "wb").close()

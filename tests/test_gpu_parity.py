@@ -224,3 +224,43 @@ def test_scalar_balance_records_match_oracle(oracle_lib, tmp_path):
     wg.close()
     wo.close()
     assert len(open(tmp_path / "ScalarBalance.txt").readline()) == 181
+
+
+def test_section_files_and_run_log_from_the_gpu(oracle_lib, tmp_path):
+    """section_<j> files written from the GPU fields against the same writer
+    fed the oracle's fields after 3 steps (float32 records, the fields agree
+    to the solver tolerance), and si3d_out.txt from a run driven call by call:
+    one Enter/converged/Exit block per free-surface solve (2 on step 1, 3 on
+    each later step) with the GPU's iteration counts."""
+    from si3d_amd import outputs
+    from si3d_amd.model import GpuLake
+    deck = small_ellipse()
+    deck["pcg_tol"] = TOL_SOLVE
+    o = oracle_lib.Oracle(deck)
+    g = GpuLake(deck, precision="fp64", preconditioner="mg", tol=TOL_SOLVE)
+    ii, jj = outputs.wet_columns(deck)
+    secs = [([int(ii[x]) for x in (100, 101, 102, 103)], [int(jj[x]) for x in (100, 101, 102, 103)]),
+            ([int(ii[200])], [int(jj[200])])]
+    (tmp_path / "gpu").mkdir()
+    (tmp_path / "oracle").mkdir()
+    log = outputs.RunLog(deck, directory=str(tmp_path / "gpu"))
+    for n in range(1, 4):
+        o.step(n)
+        outputs.step_logged(g, deck, n, log)
+    log.close()
+    for src, sub in ((g, "gpu"), (o, "oracle")):
+        sf = outputs.SectionFiles(src, deck, secs, nts=10, iox=3, directory=str(tmp_path / sub))
+        sf.write(0)
+        sf.write(3)
+        sf.close()
+    for s in (1, 2):
+        a = (tmp_path / "gpu" / f"section_{s}").read_bytes()
+        b = (tmp_path / "oracle" / f"section_{s}").read_bytes()
+        assert len(a) == len(b)
+        x, y = np.frombuffer(a, np.float32), np.frombuffer(b, np.float32)
+        scale = np.maximum(np.abs(y), 1e-6)
+        assert np.max(np.abs(x - y) / scale) < 1e-5, s
+    text = (tmp_path / "gpu" / "si3d_out.txt").read_text().splitlines()
+    enters = [l for l in text if l.startswith("**Enter nspcg")]
+    done = [int(l.split()[4]) for l in text if "has converged in" in l]
+    assert len(enters) == 2 + 3 + 3 and len(done) == 8 and all(0 < k < 200 for k in done)
