@@ -33,6 +33,8 @@ class OrcDeck(C.Structure):
         ("ntr", C.c_int), ("tracer_profiles", C.POINTER(C.c_double)),
         ("nopen", C.c_int), ("dtsec_openbc", C.c_double), ("openbc", C.POINTER(C.c_int)),
         ("openbc_npts", C.c_int), ("openbc_values", C.POINTER(C.c_double)),
+        ("icw", C.c_int), ("nmetstat", C.c_int), ("metxy", C.POINTER(C.c_double)), ("iinterp", C.c_int),
+        ("gammaB", C.c_double), ("delNfactor", C.c_double),
     ]
 
 
@@ -90,7 +92,8 @@ class Oracle:
         d = OrcDeck()
         for name, _ in OrcDeck._fields_:
             if name in ("zlevel_file", "bathy_dm", "init_profile", "surfbc", "km1_file", "n_surfbc", "ntr",
-                        "tracer_profiles", "nopen", "dtsec_openbc", "openbc", "openbc_npts", "openbc_values"):
+                        "tracer_profiles", "nopen", "dtsec_openbc", "openbc", "openbc_npts", "openbc_values",
+                        "metxy"):
                 continue
             if name in deck:
                 setattr(d, name, deck[name])
@@ -115,6 +118,11 @@ class Oracle:
             self._keep.append(tp)
             d.ntr = tp.shape[0]
             d.tracer_profiles = _dp(tp)
+        if deck.get("metxy") is not None:
+            mx = np.ascontiguousarray(deck["metxy"], dtype=np.float64).ravel()
+            self._keep.append(mx)
+            d.metxy = _dp(mx)
+            d.nmetstat = mx.size // 2
         ob = deck.get("openbc")
         if ob is not None:
             spec = np.ascontiguousarray(np.stack([ob["iside"], ob["itype"], ob["isbc"], ob["jsbc"], ob["iebc"],

@@ -197,7 +197,16 @@ struct OrcLake {
 
   // phases of fd
   void surfbc(int n, int istep);
-  int sbc_nvar() const { return d.ifsurfbc == 20 ? 3 : d.ifsurfbc == 1 ? 6 : (d.ifsurfbc == 2 || d.ifsurfbc == 3) ? 9 : 0; }
+  int sbc_nvar() const {
+    return d.ifsurfbc == 20 ? 3 : d.ifsurfbc == 1 ? 6 : (d.ifsurfbc == 2 || d.ifsurfbc == 3) ? 9
+           : (d.ifsurfbc == 10 || d.ifsurfbc == 11) ? 6 * d.nmetstat + 2 : 0;
+  }
+  // met stations (ifsurfbc 10, 11): positions, per-column weights w[m][l]
+  // and the station-to-station weights of the two-pass Barnes scheme
+  std::vector<double> metxy, wst, wn;
+  void met_weights();
+  void met_distribute(const double *v, bool heat_too);
+  void surface_heat(int l, double Qsw, double Qbra, double ea, double Ta, double Pa, double ws);
   double solar_fr(double eta, double depth) const;
   void distribute_qsw(double eta);
   void distribute_heat(const double *v);
@@ -246,6 +255,8 @@ bool OrcLake::build(const OrcDeck *deck) {
   d.zlevel_file = nullptr;
   if (deck->n_surfbc > 0 && deck->surfbc) sbc.assign(deck->surfbc, deck->surfbc + (size_t)sbc_nvar() * deck->n_surfbc);
   d.surfbc = nullptr;
+  if (d.ifsurfbc >= 10 && deck->metxy) metxy.assign(deck->metxy, deck->metxy + 2 * (size_t)d.nmetstat);
+  d.metxy = nullptr;
 
   dt = d.dt;
   dx = d.dx;
@@ -508,8 +519,142 @@ void OrcLake::init_state() {
     q2lpp = q2l;
   }
   const int nv = sbc_nvar();
-  if (nv > 0 && (int)sbc.size() >= nv) {  // momentum flux terms at t = 0 (surfbc0)
+  if (d.ifsurfbc >= 10) {  // surfbc0 cases 10, 11: weights, then the first record's wind (4301-4450)
+    met_weights();
+    if ((int)sbc.size() >= nv) met_distribute(sbc.data(), false);
+  } else if (nv > 0 && (int)sbc.size() >= nv) {  // momentum flux terms at t = 0 (surfbc0)
     for (int l = 0; l <= lm1; ++l) cdw[l] = sbc[nv - 3], uair[l] = sbc[nv - 2], vair[l] = sbc[nv - 1];
+  }
+}
+
+static double sat_vapor_pressure(double T);
+static double latent_heat_vaporization(double T);
+
+// InitializeInterpolationMethods, si3d_boundaryconditions.f90:5510-5646: the
+// weight of each met station at each wet column (i, j) -- 1: Barnes,
+// exp(-ln(100) d^2 / da^2) normalised, da the column's mean distance to the
+// stations; 2: two-pass Barnes (Koch et al. 1983), exp(-d^2 / kappa) with
+// kappa from the mean nearest-station distance, gammaB and delNfactor, not
+// normalised, plus the same weights between stations (at the cell nearest to
+// each station); 3: inverse squared distance (1 at the station's own cell),
+// normalised.  Distances in grid units.
+void OrcLake::met_weights() {
+  const int nm = d.nmetstat;
+  wst.assign((size_t)nm * (lm1 + 1), 0.0);
+  wn.assign((size_t)nm * nm, 0.0);
+  auto dist2 = [&](int m, double x, double y) {
+    const double xi = metxy[2 * m] - x, yi = metxy[2 * m + 1] - y;
+    return xi * xi + yi * yi;
+  };
+  double kappa1 = 0.0;
+  if (d.iinterp == 2) {
+    double da = 0.0;
+    for (int m = 0; m < nm; ++m) {
+      double mn = HUGE_VAL;  // MINVAL of no other station: the largest value
+      for (int q = 0; q < nm; ++q)
+        if (q != m) mn = std::min(mn, std::sqrt(dist2(q, metxy[2 * m], metxy[2 * m + 1])));
+      da += mn;
+    }
+    da /= nm;
+    const double g = d.gammaB;
+    const double f = g <= 0.2 ? 5.0515 : g <= 0.3 ? 3.5132 : g <= 0.5 ? 2.3838 : g <= 0.6 ? 2.1153 : 1.7826;
+    kappa1 = f * 4.0 / (PI * PI) * (d.delNfactor * da) * (d.delNfactor * da);
+    for (int m = 0; m < nm; ++m) {
+      const double x = std::nearbyint(metxy[2 * m]), y = std::nearbyint(metxy[2 * m + 1]);
+      for (int q = 0; q < nm; ++q) wn[(size_t)m * nm + q] = std::exp(-dist2(q, x, y) / kappa1);
+    }
+  }
+  for (int l = 1; l <= lm; ++l) {
+    const double x = l2i[l], y = l2j[l];
+    auto w = [&](int m) -> double & { return wst[(size_t)m * (lm1 + 1) + l]; };
+    if (d.iinterp == 1) {
+      double da = 0.0;
+      for (int m = 0; m < nm; ++m) da += std::sqrt(dist2(m, x, y));
+      da /= nm;
+      double sum = 0.0;
+      for (int m = 0; m < nm; ++m) sum += (w(m) = std::exp(-4.60517018598809 * dist2(m, x, y) / (da * da)));
+      for (int m = 0; m < nm; ++m) w(m) /= sum;
+    } else if (d.iinterp == 2) {
+      for (int m = 0; m < nm; ++m) w(m) = std::exp(-dist2(m, x, y) / kappa1);
+    } else {
+      double sum = 0.0;
+      for (int m = 0; m < nm; ++m) {
+        const double di = dist2(m, x, y);
+        sum += (w(m) = di > 0.0 ? 1.0 / di : 1.0);
+      }
+      for (int m = 0; m < nm; ++m) w(m) /= sum;
+    }
+  }
+}
+
+// The met-station modes on one record of interpolated series values v (eta,
+// Pa, then per station Qsw Ta Rh Cc|Qlw uair vair): CalculateMetResiduals
+// (5650-5700), InterpMetData (5703-5748) and the cases 10 / 11 of
+// DistributeMomentumHeatSourcesH (5331-5483) -- per wet column the station
+// values interpolated (the two-pass scheme adds the first pass's residuals at
+// the stations, spread with the weights to the power 1/gammaB), the column's
+// wind, its drag (icw) and, with heat_too, its heat sources.  Mode 10 reads
+// the cloud cover of the module variable Cc, which nothing sets in that mode
+// (0), and interpolates no longwave (the reference sums the station longwave
+// array there, which mode 10 never allocates, into a value it does not use).
+void OrcLake::met_distribute(const double *v, bool heat_too) {
+  const int nm = d.nmetstat, mode = d.ifsurfbc;
+  const double eta = v[0], Pa = v[1];
+  auto st = [&](int m, int q) { return v[2 + 6 * m + q]; };  // q: 0 Qsw 1 Ta 2 Rh 3 Cc|Qlw 4 uair 5 vair
+  std::vector<double> resid((size_t)6 * nm, 0.0);
+  if (d.iinterp == 2) {
+    for (int k = 0; k < nm; ++k) {
+      double sumw = 0.0;
+      for (int m = 0; m < nm; ++m) sumw += wn[(size_t)k * nm + m];
+      for (int q = 0; q < 6; ++q) {
+        if (q == 3 && mode == 10) continue;
+        double dum = 0.0;
+        for (int m = 0; m < nm; ++m) dum += wn[(size_t)k * nm + m] / sumw * st(m, q);
+        resid[(size_t)6 * k + q] = st(k, q) - dum;
+      }
+    }
+  }
+  if (heat_too) {
+    distribute_qsw(eta);
+    for (int l = 1; l <= lm; ++l)
+      for (int k = 1; k <= km1; ++k) heat(k, l) = 0.0;
+  }
+  const double EmAirC = 0.642, StephanBoltzman = 5.6697e-8, Al = 0.03;
+  for (int l = 1; l <= lm; ++l) {
+    double x[6] = {0, 0, 0, 0, 0, 0};
+    auto w = [&](int m) { return wst[(size_t)m * (lm1 + 1) + l]; };
+    for (int m = 0; m < nm; ++m)
+      for (int q = 0; q < 6; ++q) x[q] += w(m) * st(m, q);
+    if (d.iinterp == 2) {
+      double sumw = 0.0, sumg = 0.0;
+      for (int m = 0; m < nm; ++m) sumw += w(m), sumg += std::pow(w(m), 1.0 / d.gammaB);
+      for (int q = 0; q < 6; ++q) x[q] /= sumw;
+      for (int m = 0; m < nm; ++m)
+        for (int q = 0; q < 6; ++q) x[q] += std::pow(w(m), 1.0 / d.gammaB) / sumg * resid[(size_t)6 * m + q];
+    }
+    uair[l] = x[4];
+    vair[l] = x[5];
+    const double ws = std::sqrt(std::pow(uair[l], 2.) + std::pow(vair[l], 2.));
+    if (d.icw == 0) {
+      cdw[l] = d.cw;
+    } else if (d.icw == 1) {
+      cdw[l] = 0.0015 * 1. / (1.0 + std::exp((12.5 - ws) / 1.56)) + 0.00104;
+    } else {  // (1/2) is the integer 0 there: the square root term is 1
+      cdw[l] = ws == 0 ? d.cw : std::pow((0.239 + 0.0433 * ((ws - 8.271) + 1.0)) / ws, 2);
+    }
+    if (!heat_too) continue;
+    const double Qsw = x[0], Ta = x[1], Rh = x[2];
+    const double ea = sat_vapor_pressure(Ta + 273.) * Rh;
+    double Qbra;
+    if (mode == 10) {
+      const double Cc = 0.0;
+      double EmAir = EmAirC * std::pow(ea / (Ta + 273.), 0.14285714);
+      EmAir = EmAir * (1. + 0.17 * std::pow(Cc, 2.));
+      Qbra = EmAir * StephanBoltzman * std::pow(Ta + 273., 4.) * (1. - Al);
+    } else {
+      Qbra = x[3] * (1. - Al);
+    }
+    surface_heat(l, Qsw, Qbra, ea, Ta, Pa, ws);
   }
 }
 
@@ -532,6 +677,10 @@ void OrcLake::surfbc(int n, int istep) {
   for (int q = 0; q < nv; ++q) {
     for (int j = 0; j < np; ++j) f[j + 1] = sbc[(size_t)nv * j + q];
     v[q] = parab(0., thrs, f.data(), dth);
+  }
+  if (d.ifsurfbc >= 10) {
+    met_distribute(v.data(), true);
+    return;
   }
   for (int l = 0; l <= lm1; ++l) cdw[l] = v[nv - 3], uair[l] = v[nv - 2], vair[l] = v[nv - 1];
   if (d.ifsurfbc >= 1 && d.ifsurfbc <= 3) {
@@ -578,12 +727,7 @@ static double latent_heat_vaporization(double T) { return 1.91846e6 * std::pow(T
 // DistributeMomentumHeatSourcesH, si3d_boundaryconditions.f90:5144-5485,
 // cases 1-3.  v holds the interpolated series values of one record.
 void OrcLake::distribute_heat(const double *v) {
-  const double range[6] = {0.000, 2.2000, 5.0000, 8.0000, 25.00, 50.0};
-  const double a_h[5] = {0.000, 0.9270, 1.1500, 1.1700, 1.652}, b_h[5] = {1.185, 0.0521, 0.0100, 0.0075, -0.017};
-  const double c_h[5] = {0.000, 0.0000, 0.0000, -4.5E-4, 0.000}, p_h[5] = {-0.157, 1.0000, 1.0000, 1.0000, 1.000};
-  const double a_e[5] = {0.000, 0.9690, 1.1800, 1.1960, 1.680}, b_e[5] = {1.230, 0.0521, 0.0100, 0.0080, -0.016};
-  const double c_e[5] = {0.000, 0.0000, 0.0000, -4.0E-4, 0.000}, p_e[5] = {-0.160, 1.0000, 1.0000, 1.0000, 1.000};
-  const double EmWater = 0.97, StephanBoltzman = 5.6697e-8, Al = 0.03, Cp = 4181.6, SpecificHeatAir = 1012.0;
+  const double StephanBoltzman = 5.6697e-8, Al = 0.03, Cp = 4181.6;
   const int mode = d.ifsurfbc;
   const double Qsw = v[1];
   for (int l = 1; l <= lm; ++l)
@@ -605,29 +749,47 @@ void OrcLake::distribute_heat(const double *v) {
     }
   }
   for (int l = 1; l <= lm; ++l) {
-    int k1s = k1z[l], kms = kmz[l];
-    if (mode == 1) {
-      heat(k1s, l) = heat(k1s, l) + (Qn - Qsw);
-    } else {
-      double ws = std::sqrt(std::pow(uair[l], 2.) + std::pow(vair[l], 2.));
-      int irg;
-      if (ws > range[0] && ws <= range[1]) irg = 0;
-      else if (ws > range[1] && ws <= range[2]) irg = 1;
-      else if (ws > range[2] && ws <= range[3]) irg = 2;
-      else if (ws > range[3] && ws <= range[4]) irg = 3;
-      else irg = 4;
-      double coeffH = (a_h[irg] + b_h[irg] * std::pow(ws, p_h[irg]) + c_h[irg] * std::pow(ws - 8., 2.)) * 1.e-3;
-      double coeffE = (a_e[irg] + b_e[irg] * std::pow(ws, p_e[irg]) + c_e[irg] * std::pow(ws - 8., 2.)) * 1.e-3;
-      double esw = sat_vapor_pressure(salp(k1s, l) + 273.);
-      double Lv = latent_heat_vaporization(salp(k1s, l) + 273.);
-      heat(k1s, l) = Qbra - EmWater * StephanBoltzman * std::pow(salp(k1s, l) + 273., 4.);
-      heat(k1s, l) = heat(k1s, l) - RHOAIR * Lv * coeffE * (esw - ea) * 0.622 / Pa * ws -
-                     RHOAIR * SpecificHeatAir * coeffH * (salp(k1s, l) - Ta) * ws;
+    if (mode != 1) {
+      surface_heat(l, Qsw, Qbra, ea, Ta, Pa, std::sqrt(std::pow(uair[l], 2.) + std::pow(vair[l], 2.)));
+      continue;
     }
+    const int k1s = k1z[l], kms = kmz[l];
+    heat(k1s, l) = heat(k1s, l) + (Qn - Qsw);
     for (int k = k1s; k <= kms; ++k) {
       heat(k, l) = (heat(k, l) + Qsw * qswfr(k, l)) / ((rhop(k, l) + 1000.) * Cp);
       if (Qsw < -100.) heat(k, l) = 0.0;
     }
+  }
+}
+
+// The column part of DistributeMomentumHeatSourcesH for the budget modes
+// (cases 2, 3, 10, 11): Kondo (1975) neutral transfer coefficients for the
+// column's wind speed, longwave, latent and sensible fluxes into the surface
+// layer, shortwave over the column, all in temperature units.
+void OrcLake::surface_heat(int l, double Qsw, double Qbra, double ea, double Ta, double Pa, double ws) {
+  const double range[6] = {0.000, 2.2000, 5.0000, 8.0000, 25.00, 50.0};
+  const double a_h[5] = {0.000, 0.9270, 1.1500, 1.1700, 1.652}, b_h[5] = {1.185, 0.0521, 0.0100, 0.0075, -0.017};
+  const double c_h[5] = {0.000, 0.0000, 0.0000, -4.5E-4, 0.000}, p_h[5] = {-0.157, 1.0000, 1.0000, 1.0000, 1.000};
+  const double a_e[5] = {0.000, 0.9690, 1.1800, 1.1960, 1.680}, b_e[5] = {1.230, 0.0521, 0.0100, 0.0080, -0.016};
+  const double c_e[5] = {0.000, 0.0000, 0.0000, -4.0E-4, 0.000}, p_e[5] = {-0.160, 1.0000, 1.0000, 1.0000, 1.000};
+  const double EmWater = 0.97, StephanBoltzman = 5.6697e-8, Cp = 4181.6, SpecificHeatAir = 1012.0;
+  int irg;
+  if (ws > range[0] && ws <= range[1]) irg = 0;
+  else if (ws > range[1] && ws <= range[2]) irg = 1;
+  else if (ws > range[2] && ws <= range[3]) irg = 2;
+  else if (ws > range[3] && ws <= range[4]) irg = 3;
+  else irg = 4;
+  const int k1s = k1z[l], kms = kmz[l];
+  double coeffH = (a_h[irg] + b_h[irg] * std::pow(ws, p_h[irg]) + c_h[irg] * std::pow(ws - 8., 2.)) * 1.e-3;
+  double coeffE = (a_e[irg] + b_e[irg] * std::pow(ws, p_e[irg]) + c_e[irg] * std::pow(ws - 8., 2.)) * 1.e-3;
+  double esw = sat_vapor_pressure(salp(k1s, l) + 273.);
+  double Lv = latent_heat_vaporization(salp(k1s, l) + 273.);
+  heat(k1s, l) = Qbra - EmWater * StephanBoltzman * std::pow(salp(k1s, l) + 273., 4.);
+  heat(k1s, l) = heat(k1s, l) - RHOAIR * Lv * coeffE * (esw - ea) * 0.622 / Pa * ws -
+                 RHOAIR * SpecificHeatAir * coeffH * (salp(k1s, l) - Ta) * ws;
+  for (int k = k1s; k <= kms; ++k) {
+    heat(k, l) = (heat(k, l) + Qsw * qswfr(k, l)) / ((rhop(k, l) + 1000.) * Cp);
+    if (Qsw < -100.) heat(k, l) = 0.0;
   }
 }
 

@@ -49,6 +49,16 @@ typedef struct {
                                  value q at (nn*npts + r)*(ntr+2) + q; q = 0
                                  zeta (m) or flow (m3/s), 1 active scalar,
                                  2.. tracers (openbcNN.txt)                   */
+  int icw;                    /* drag of the met-station modes: 0 cw, 1
+                                 Amorocho & DeVries, 2 Andreas et al.         */
+  int nmetstat;               /* met stations (ifsurfbc 10, 11); the series
+                                 then holds 6*nmetstat+2 values per record:
+                                 eta Pa, then per station Qsw Ta Rh Cc|Qlw
+                                 uair vair                                    */
+  const double *metxy;        /* 2*nmetstat: station x, y in grid units       */
+  int iinterp;                /* 1 Barnes, 2 two-pass Barnes, 3 inverse
+                                 distance                                     */
+  double gammaB, delNfactor;  /* iinterp 2                                    */
 } OrcDeck;
 
 typedef struct OrcLake OrcLake;

@@ -181,6 +181,42 @@ def heat_budget(deck, mode, eta=0.5, qsw_max=800.0, q_night=-150.0, period=86400
     return d
 
 
+def met_stations(deck, mode, stations, iinterp=1, gammaB=0.3, delNfactor=1.0, icw=0, eta=0.5, qsw_max=800.0,
+                 period=86400.0, same=False):
+    """The deck with the met-station surface heat budget (ifsurfbc = mode, 10
+    or 11; si3d_surfbc.txt with 6 * nmetstat + 2 values per record): stations
+    at the (x, y) grid positions `stations`, each with its own diurnal
+    shortwave (max(0, qsw_max cos(2 pi t/period)) scaled by 1 - 0.1 m), air
+    temperature (20 + 5 cos +- 1.5 m C), humidity (0.6 + 0.05 m), cloud cover
+    0.3 (mode 10) or incoming longwave 320 + 10 m W/m2 (mode 11) and wind
+    (the deck's wind series turned by 25 m degrees and scaled by 1 + 0.3 m),
+    where m is the station index; `same` gives every station station 0's
+    series.  Interpolation to the columns: iinterp 1 (Barnes), 2 (two-pass
+    Barnes with gammaB, delNfactor) or 3 (inverse distance); drag icw."""
+    if mode not in (10, 11):
+        raise ValueError("met-station mode must be 10 or 11")
+    w = np.asarray(deck["surfbc"], dtype=np.float64)
+    if w.ndim != 2 or w.shape[1] != 3:
+        raise ValueError("met_stations needs a deck with a wind series (ifsurfbc = 20)")
+    n = w.shape[0]
+    t = np.arange(n) * deck["dtsurfbc"]
+    ph = 2 * np.pi * t / period
+    cols = [np.full(n, eta), np.full(n, 101325.0)]
+    for m in range(len(stations)):
+        q = 0 if same else m
+        qsw = np.maximum(0.0, qsw_max * np.cos(ph)) * (1.0 - 0.1 * q)
+        ta = 20.0 + 5.0 * np.cos(ph - 0.5) + 1.5 * (q % 2) - 0.75 * q
+        th = np.radians(25.0 * q)
+        ua = (w[:, 1] * np.cos(th) - w[:, 2] * np.sin(th)) * (1.0 + 0.3 * q)
+        va = (w[:, 1] * np.sin(th) + w[:, 2] * np.cos(th)) * (1.0 + 0.3 * q)
+        last = np.full(n, 0.3) if mode == 10 else np.full(n, 320.0 + 10.0 * q)
+        cols += [qsw, ta, np.full(n, 0.6 + 0.05 * q), last, ua, va]
+    d = dict(deck)
+    d.update(ifsurfbc=mode, surfbc=np.stack(cols, axis=1), metxy=np.asarray(stations, dtype=np.float64).ravel(),
+             nmetstat=len(stations), iinterp=iinterp, gammaB=gammaB, delNfactor=delNfactor, icw=icw)
+    return d
+
+
 def ellipse(nx=1024, ny=512, dx=100.0, nlayers=60, dt=60.0, nts=2000, seed=1,
             wind_amp=0.2, wind_dir=30.0, f=1e-4, Ax0=1.0, depth=100.0,
             top=0.5, bottom=5.0, iturb=1):
