@@ -70,7 +70,10 @@ typedef struct {
                             ifsurfbc 20: cdw uair vair (nv = 3);
                             1: eta Qsw Qn cdw uair vair (nv = 6);
                             2: eta Qsw Ta Pa Rh Cc cdw uair vair (nv = 9);
-                            3: eta Qsw Ta Pa Rh Qlw cdw uair vair (nv = 9) */
+                            3: eta Qsw Ta Pa Rh Qlw cdw uair vair (nv = 9);
+                            10, 11: eta Pa, then per met station Qsw Ta Rh
+                            Cc (10) or Qlw (11) uair vair
+                            (nv = 6*nmetstat + 2; 4301-4450)            */
   int precision;         /* S3G_FP64 or S3G_MIXED                          */
   int preconditioner;    /* S3G_PREC_*                                     */
   double pcg_tol;        /* relative FP64 true-residual tolerance          */
@@ -78,6 +81,13 @@ typedef struct {
   int ntr;               /* conservative tracers (ecomod = 0; exTracer/ImTracer,
                             si3d_procedures.f90:4090-4484), fields "tracer1".. and
                             "tracerpp1".. */
+  /* met stations of ifsurfbc 10 and 11 (si3d_boundaryconditions.f90:
+     4301-4450, 5510-5748): ignored for the other modes                    */
+  int icw;               /* drag: 0 cw, 1 Amorocho & DeVries, 2 Andreas    */
+  int nmetstat;          /* stations, at most 16                           */
+  const double *metxy;   /* 2*nmetstat: x, y of each station (grid units)  */
+  int iinterp;           /* 1 Barnes, 2 two-pass Barnes, 3 inverse distance*/
+  double gammaB, delNfactor; /* iinterp 2 (si3d_utils.f90:266-275)        */
 } s3g_params;
 
 /* Input deck for the standalone driver: the content of si3d_inp.txt,

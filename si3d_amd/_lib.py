@@ -48,7 +48,8 @@ class S3gParams(C.Structure):
                 ("cw", C.c_double), ("wa", C.c_double), ("phi", C.c_double), ("dtsurfbc", C.c_double),
                 ("n_surfbc", C.c_int), ("surfbc", dp), ("precision", C.c_int),
                 ("preconditioner", C.c_int), ("pcg_tol", C.c_double), ("pcg_maxit", C.c_int),
-                ("ntr", C.c_int)]
+                ("ntr", C.c_int), ("icw", C.c_int), ("nmetstat", C.c_int), ("metxy", dp), ("iinterp", C.c_int),
+                ("gammaB", C.c_double), ("delNfactor", C.c_double)]
 
 
 class S3gDeck(C.Structure):

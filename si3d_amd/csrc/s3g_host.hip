@@ -99,11 +99,12 @@ static double parab(double frstpt, double x, const std::vector<double> &fx1, dou
 // vair; 1: eta Qsw Qn cdw uair vair; 2: eta Qsw Ta Pa Rh Cc cdw uair vair;
 // 3: eta Qsw Ta Pa Rh Qlw cdw uair vair (si3d_boundaryconditions.f90:4160-4300,
 // 4450-4484; nvSurfbcW/P/R, si3d_types.f90:173-175).
-int surfbc_nvar(int ifsurfbc) {
+int surfbc_nvar(int ifsurfbc, int nmet = 0) {
   switch (ifsurfbc) {
     case 20: return 3;
     case 1: return 6;
     case 2: case 3: return 9;
+    case 10: case 11: return 6 * nmet + 2;  // eta Pa, 6 per station (4301-4450)
     default: return 0;
   }
 }
@@ -173,6 +174,99 @@ static s3g::HeatForce heat_force_of(int mode, const double *v) {
   return H;
 }
 
+// Station weights of the met-station modes (InitializeInterpolationMethods,
+// si3d_boundaryconditions.f90:5510-5646) at every wet column (i, j), in
+// grid units: 1 Barnes, exp(-ln(100) d^2/da^2) normalised with da the mean
+// distance to the stations; 2 two-pass Barnes (Koch et al. 1983),
+// exp(-d^2/kappa) unnormalised, kappa from the mean nearest-station distance,
+// gammaB and delNfactor, and the same weights between stations (at the cell
+// nearest each station); 3 inverse squared distance (1 at the station's own
+// cell), normalised.  Device layout [m][LP], zero for the sentinel.
+static int met_setup(s3g_ctx *c) {
+  const int nm = (int)c->metxy.size() / 2, LP = c->LP;
+  auto d2 = [&](int m, double x, double y) {
+    const double dx = c->metxy[2 * m] - x, dy = c->metxy[2 * m + 1] - y;
+    return dx * dx + dy * dy;
+  };
+  double kappa = 0.0;
+  if (c->iinterp == 2) {
+    double da = 0.0;
+    for (int m = 0; m < nm; ++m) {
+      double nearest = HUGE_VAL;  // no other station: the largest value
+      for (int q = 0; q < nm; ++q)
+        if (q != m) nearest = std::min(nearest, std::sqrt(d2(q, c->metxy[2 * m], c->metxy[2 * m + 1])));
+      da += nearest;
+    }
+    da /= nm;
+    const double g = c->gammaB;
+    const double f = g <= 0.2 ? 5.0515 : g <= 0.3 ? 3.5132 : g <= 0.5 ? 2.3838 : g <= 0.6 ? 2.1153 : 1.7826;
+    const double pi = 3.14159265358979;
+    kappa = f * 4.0 / (pi * pi) * (c->delNfactor * da) * (c->delNfactor * da);
+    c->met_wn.assign((size_t)nm * nm, 0.0);
+    for (int m = 0; m < nm; ++m) {
+      const double x = std::nearbyint(c->metxy[2 * m]), y = std::nearbyint(c->metxy[2 * m + 1]);
+      for (int q = 0; q < nm; ++q) c->met_wn[(size_t)m * nm + q] = std::exp(-d2(q, x, y) / kappa);
+    }
+  }
+  std::vector<double> w((size_t)nm * LP, 0.0);
+  for (int col = 0; col < c->lm; ++col) {
+    const double x = c->l2i[col], y = c->l2j[col];
+    double sum = 0.0, da = 0.0;
+    if (c->iinterp == 1) {
+      for (int m = 0; m < nm; ++m) da += std::sqrt(d2(m, x, y));
+      da /= nm;
+    }
+    for (int m = 0; m < nm; ++m) {
+      const double dd = d2(m, x, y);
+      double v;
+      if (c->iinterp == 1) v = std::exp(-4.60517018598809 * dd / (da * da));
+      else if (c->iinterp == 2) v = std::exp(-dd / kappa);
+      else v = dd > 0.0 ? 1.0 / dd : 1.0;
+      w[(size_t)m * LP + col] = v;
+      sum += v;
+    }
+    if (c->iinterp != 2)
+      for (int m = 0; m < nm; ++m) w[(size_t)m * LP + col] /= sum;
+  }
+  if (hipMalloc(&c->d_wmet, w.size() * sizeof(double)) != hipSuccess) return hip_check(hipErrorOutOfMemory, "met");
+  c->allocs.push_back(c->d_wmet);
+  return hip_check(hipMemcpy(c->d_wmet, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice), "met weights");
+}
+
+// surfbc cases 10 and 11 (si3d_boundaryconditions.f90:4596-4650, on leapfrog
+// steps and step 1): every series value at thrs, then the first pass's
+// residuals at the stations for the two-pass scheme (CalculateMetResiduals,
+// 5650-5700); the columns are done on the device.  Mode 10 has no longwave
+// series to interpolate (its fourth station value is cloud cover, which the
+// column budget does not read).
+static void met_surfbc(s3g_ctx *c, int n, int istep) {
+  if (n > 1 && istep == 2) return;
+  const int nm = (int)c->metxy.size() / 2, nv = 6 * nm + 2;
+  std::vector<double> v(nv);
+  if (!interp_series(c->surfbc, nv, c->P.dtsurfbc, c->thrs, v.data())) return;
+  s3g::MetArgs &M = c->met;
+  M.mode = c->P.ifsurfbc;
+  M.nmet = nm;
+  M.iinterp = c->iinterp;
+  M.icw = c->icw;
+  M.eta = v[0];
+  M.Pa = v[1];
+  M.gamma = c->gammaB;
+  M.cw = c->P.cw;
+  for (int i = 0; i < 6 * nm; ++i) M.st[i] = v[2 + i], M.res[i] = 0.0;
+  if (c->iinterp == 2)
+    for (int k = 0; k < nm; ++k) {
+      double sumw = 0.0;
+      for (int m = 0; m < nm; ++m) sumw += c->met_wn[(size_t)k * nm + m];
+      for (int q = 0; q < 6; ++q) {
+        double first = 0.0;
+        for (int m = 0; m < nm; ++m) first += c->met_wn[(size_t)k * nm + m] / sumw * M.st[6 * m + q];
+        M.res[6 * k + q] = M.st[6 * k + q] - first;
+      }
+    }
+  c->met_due = true;
+}
+
 // surfbc cases 0, 1-3 and 20 (si3d_boundaryconditions.f90:4490-4669): the
 // forcing is uniform in space, so wind is kept as three scalars; the heat
 // budget cases also flag HeatSource for recomputation on the device.
@@ -198,6 +292,10 @@ void surfbc(s3g_ctx *c, int n, int istep) {
     return;
   }
   const int mode = c->P.ifsurfbc;
+  if (mode >= 10) {
+    met_surfbc(c, n, istep);
+    return;
+  }
   if (mode == 0) {
     c->uair = -c->P.wa * std::sin(3.1415926535897932 * c->P.phi / 180.);
     c->vair = -c->P.wa * std::cos(3.1415926535897932 * c->P.phi / 180.);
@@ -422,7 +520,14 @@ static int alloc_state(s3g_ctx *c, Dev<R> &D, const int *lE, const int *lW, cons
   D.t1 = nullptr;
   if (c->P.ihd > 1 && !(D.kh = dalloc<R>(c, n3))) return S3G_ERR_HIP;
   D.heat = nullptr;
-  if (c->P.ifsurfbc >= 1 && c->P.ifsurfbc <= 3 && !(D.heat = dalloc<R>(c, n3))) return S3G_ERR_HIP;
+  const bool budget = (c->P.ifsurfbc >= 1 && c->P.ifsurfbc <= 3) || c->P.ifsurfbc == 10 || c->P.ifsurfbc == 11;
+  if (budget && !(D.heat = dalloc<R>(c, n3))) return S3G_ERR_HIP;
+  D.wind = nullptr;
+  if (c->P.ifsurfbc >= 10) {  // per-column wind of the met-station modes
+    if (!c->d_wind) c->d_wind = dalloc<double>(c, 3 * (size_t)c->LP);
+    D.wind = c->d_wind;
+    if (!D.wind) return S3G_ERR_HIP;
+  }
   D.ntr = c->ntr;
   D.tr = D.trpp = nullptr;
   if (c->ntr > 0) {
@@ -478,13 +583,20 @@ int s3g::init_ctx(const s3g_grid *g, const s3g_params *p, int nlakes, const doub
     set_error("s3g_init: iturb < 0 (GOTM) is not supported");
     return S3G_ERR_UNSUPPORTED;
   }
-  if (p->ifsurfbc != 0 && p->ifsurfbc != 20 && (p->ifsurfbc < 1 || p->ifsurfbc > 3)) {
-    set_error("s3g_init: ifsurfbc must be 0, 1, 2, 3 or 20 (the Barnes-interpolated met stations of 10/11 are "
-              "not supported)");
+  if (p->ifsurfbc != 0 && p->ifsurfbc != 20 && (p->ifsurfbc < 1 || p->ifsurfbc > 3) && p->ifsurfbc != 10 &&
+      p->ifsurfbc != 11) {
+    set_error("s3g_init: ifsurfbc must be 0, 1, 2, 3, 10, 11 or 20");
     return S3G_ERR_UNSUPPORTED;
   }
-  if (nlakes > 1 && p->ifsurfbc >= 1 && p->ifsurfbc <= 3) {
-    set_error("s3g_create_ensemble: the surface heat budget (ifsurfbc 1-3) is not supported in an ensemble");
+  if (p->ifsurfbc >= 10 && p->ifsurfbc <= 11 &&
+      (p->nmetstat < 1 || p->nmetstat > s3g::kMaxMet || !p->metxy || p->iinterp < 1 || p->iinterp > 3 ||
+       (p->iinterp == 2 && !(p->gammaB > 0.0)) || p->icw < 0 || p->icw > 2)) {
+    set_error("s3g_init: ifsurfbc 10/11 need 1-16 met stations with their positions, iinterp 1-3 (gammaB > 0 "
+              "for 2) and icw 0-2");
+    return S3G_ERR_ARG;
+  }
+  if (nlakes > 1 && p->ifsurfbc != 0 && p->ifsurfbc != 20) {
+    set_error("s3g_create_ensemble: the surface heat budget (ifsurfbc 1-3, 10, 11) is not supported in an ensemble");
     return S3G_ERR_UNSUPPORTED;
   }
   int ndev = 0;
@@ -592,9 +704,16 @@ int s3g::init_ctx(const s3g_grid *g, const s3g_params *p, int nlakes, const doub
   P.t_3 = std::pow(B1, -1. / 3.);
   P.t_4 = 9. * A1 * (2. * A1 + A2 * (1. - C2));
   P.t_5 = 9. * A1 * A2;
-  const int nv = surfbc_nvar(p->ifsurfbc);
+  const int nmet = p->ifsurfbc >= 10 ? p->nmetstat : 0;
+  const int nv = surfbc_nvar(p->ifsurfbc, nmet);
   if (p->n_surfbc > 0 && p->surfbc && nv > 0) c->surfbc.assign(p->surfbc, p->surfbc + (size_t)nv * p->n_surfbc);
-  if (nv > 0 && (int)c->surfbc.size() >= nv) {  // momentum flux terms at t = 0 (surfbc0)
+  if (nmet > 0) {
+    c->metxy.assign(p->metxy, p->metxy + 2 * (size_t)nmet);
+    c->iinterp = p->iinterp;
+    c->icw = p->icw;
+    c->gammaB = p->gammaB;
+    c->delNfactor = p->delNfactor;
+  } else if (nv > 0 && (int)c->surfbc.size() >= nv) {  // momentum flux terms at t = 0 (surfbc0)
     c->cdw = c->surfbc[nv - 3];
     c->uair = c->surfbc[nv - 2];
     c->vair = c->surfbc[nv - 1];
@@ -652,6 +771,7 @@ int s3g::init_ctx(const s3g_grid *g, const s3g_params *p, int nlakes, const doub
     if (rc == 0 && !(*q = dalloc<float>(c, c->precision == S3G_MIXED ? n2 : 1))) rc = S3G_ERR_HIP;
   c->l2i.assign(g->l2i, g->l2i + c->lm);
   c->l2j.assign(g->l2j, g->l2j + c->lm);
+  if (rc == 0 && !c->metxy.empty()) rc = met_setup(c);
   if (rc == 0) rc = pcg_setup(c, lN.data(), g->l2i, g->l2j);
   if (rc == 0 && !(G.part = dalloc<double>(c, 10 * (size_t)G.nblk + 64))) rc = S3G_ERR_HIP;  // 10 partial slots
   if (rc == 0 && !(G.iters = dalloc<int>(c, 4))) rc = S3G_ERR_HIP;

@@ -46,6 +46,17 @@ struct HeatForce {
   double coeffE, coeffH;  // bulk transfer coefficients (Kondo 1975)
 };
 
+// The met-station modes (ifsurfbc 10, 11) of one surfbc call: the stations'
+// series values interpolated in time and, for the two-pass Barnes scheme,
+// the first pass's residuals at the stations; per station q = 0 Qsw, 1 Ta,
+// 2 Rh, 3 Cc|Qlw, 4 uair, 5 vair at [6*m + q].
+constexpr int kMaxMet = 16;
+struct MetArgs {
+  int mode, nmet, iinterp, icw;
+  double eta, Pa, gamma, cw;
+  double st[6 * kMaxMet], res[6 * kMaxMet];
+};
+
 // Everything a kernel needs, passed by value.  R is the storage type of the
 // 3-D hydrodynamic fields (double, or float in the mixed-precision mode).  The
 // free surface, the 2-D solver arrays and the turbulence quantities are FP64
@@ -63,6 +74,8 @@ struct Dev {
   const int *lake, *lake_km1;
   int KPz;
   const double *force;
+  // per-column wind of the met-station modes: uair, vair, cdw at 3 c .. 3 c + 2
+  const double *wind;
   const double *hhs, *hhu, *hhv;
   double *s, *sp, *spp, *sx, *sy, *dd, *qq, *rr, *eagx, *earx, *eagy, *eary;
   R *h, *hp, *hpp, *hu, *hup, *hupp, *hv, *hvp, *hvpp;
@@ -195,6 +208,15 @@ struct s3g_ctx {
   // surface heat budget: set by surfbc when HeatSource must be recomputed
   bool heat_due = false;
   s3g::HeatForce heat_force{};
+  // met stations (ifsurfbc 10, 11): positions, settings, the station-to-
+  // station weights of the two-pass scheme (host) and the per-column
+  // station weights [m][LP] and winds [3 LP] (device)
+  std::vector<double> metxy, met_wn;
+  int iinterp = 1, icw = 0;
+  double gammaB = 0.0, delNfactor = 0.0;
+  double *d_wmet = nullptr, *d_wind = nullptr;
+  bool met_due = false;
+  s3g::MetArgs met{};
   hipStream_t stream = nullptr;
   // device memory
   std::vector<void *> allocs;

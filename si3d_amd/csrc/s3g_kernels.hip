@@ -128,6 +128,7 @@ struct Wind {
 };
 template <typename R>
 __device__ __forceinline__ Wind wind(const Dev<R> &D, int c) {
+  if (D.wind) return {D.wind[3 * c], D.wind[3 * c + 1], D.wind[3 * c + 2]};  // met-station modes
   if (D.force) {
     const double *f = D.force + (D.lake ? 3 * D.lake[c] : 0);
     return {f[0], f[1], f[2]};
@@ -929,25 +930,13 @@ __device__ __forceinline__ double solar_fraction(double eta, double z) {
 // HeatSource in temperature units (DistributeMomentumHeatSourcesH, 5144-5485):
 // the non-penetrative fluxes enter the surface layer, shortwave is spread over
 // the column.  HeatSource is zero above and below the wet layers.
+// HeatSource of one column from its non-penetrative surface flux `surf` (W/m2)
+// and its shortwave Qsw spread over the wet layers with attenuation eta.
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_heat_source(Dev<R> D, HeatForce H) {
-  const int c = col();
-  if (c >= D.lm) return;
-  constexpr double kStefanBoltzmann = 5.6697e-8, kEmWater = 0.97, kCp = 4181.6, kCpAir = 1012.0;
+__device__ void heat_column(const Dev<R> &D, int c, double eta, double Qsw, double surf) {
+  constexpr double kCp = 4181.6;
   const int kms = D.kmz[c], k1s = D.k1z[c], nwl = kms - k1s + 1;
   for (int k = 1; k <= D.km1; ++k) D.heat[IX(k, c)] = R(0);
-  double surf;
-  if (H.mode == 1) {
-    surf = H.Qn - H.Qsw;
-  } else {
-    const double ts = D.salp[IX(k1s, c)];
-    const double T = ts + 273.;
-    const double esw = 2.1718e10 * exp(-4157. / (T - 33.91));
-    const double lv = 1.91846e6 * (T / (T - 33.91)) * (T / (T - 33.91));
-    surf = H.Qbra - kEmWater * kStefanBoltzmann * T * T * T * T -
-           RHOAIR * lv * H.coeffE * (esw - H.ea) * 0.622 / H.Pa * H.ws -
-           RHOAIR * kCpAir * H.coeffH * (ts - H.Ta) * H.ws;
-  }
   if (nwl < 1) {  // the surface term alone, as the reference leaves it
     D.heat[IX(k1s, c)] = (R)surf;
     return;
@@ -958,7 +947,7 @@ __global__ void __launch_bounds__(kBlock) k_heat_source(Dev<R> D, HeatForce H) {
     double z = 0.0, sum = 0.0;
     for (int k = k1s; k <= kms; ++k) {
       const double zn = z + (double)D.hp[IX(k, c)];
-      sum += solar_fraction(H.eta, z) - solar_fraction(H.eta, zn);
+      sum += solar_fraction(eta, z) - solar_fraction(eta, zn);
       z = zn;
     }
     rem = 1. - sum;
@@ -969,13 +958,117 @@ __global__ void __launch_bounds__(kBlock) k_heat_source(Dev<R> D, HeatForce H) {
     double fr = 1.0;
     if (nwl >= 2) {
       const double zn = z + (double)D.hp[IX(k, c)];
-      fr = solar_fraction(H.eta, z) - solar_fraction(H.eta, zn) + rem;
+      fr = solar_fraction(eta, z) - solar_fraction(eta, zn) + rem;
       z = zn;
     }
-    double hs = ((k == k1s ? surf : 0.0) + H.Qsw * fr) / (((double)D.rhop[IX(k, c)] + 1000.) * kCp);
-    if (H.Qsw < -100.) hs = 0.0;
+    double hs = ((k == k1s ? surf : 0.0) + Qsw * fr) / (((double)D.rhop[IX(k, c)] + 1000.) * kCp);
+    if (Qsw < -100.) hs = 0.0;
     D.heat[IX(k, c)] = (R)hs;
   }
+}
+
+// Longwave, latent and sensible flux into the surface layer (W/m2) for the
+// budget modes (si3d_boundaryconditions.f90:5268-5316): Qbra the absorbed
+// atmospheric longwave, ea the air's vapour pressure, cE / cH the Kondo
+// coefficients at wind speed ws.
+template <typename R>
+__device__ double surface_flux(const Dev<R> &D, int c, double Qbra, double ea, double Ta, double Pa, double ws,
+                               double cE, double cH) {
+  constexpr double kStefanBoltzmann = 5.6697e-8, kEmWater = 0.97, kCpAir = 1012.0;
+  const double ts = D.salp[IX(D.k1z[c], c)];
+  const double T = ts + 273.;
+  const double esw = 2.1718e10 * exp(-4157. / (T - 33.91));
+  const double lv = 1.91846e6 * (T / (T - 33.91)) * (T / (T - 33.91));
+  return Qbra - kEmWater * kStefanBoltzmann * T * T * T * T - RHOAIR * lv * cE * (esw - ea) * 0.622 / Pa * ws -
+         RHOAIR * kCpAir * cH * (ts - Ta) * ws;
+}
+
+// Shortwave fractions of each wet layer (DistributeQswH,
+// si3d_boundaryconditions.f90:4672-4727) and HeatSource in temperature units
+// (DistributeMomentumHeatSourcesH, 5144-5485) for the spatially uniform
+// budget modes 1-3: the non-penetrative fluxes enter the surface layer,
+// shortwave is spread over the column; HeatSource is zero above and below
+// the wet layers.
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_heat_source(Dev<R> D, HeatForce H) {
+  const int c = col();
+  if (c >= D.lm) return;
+  const double surf = H.mode == 1 ? H.Qn - H.Qsw
+                                  : surface_flux(D, c, H.Qbra, H.ea, H.Ta, H.Pa, H.ws, H.coeffE, H.coeffH);
+  heat_column(D, c, H.eta, H.Qsw, surf);
+}
+
+// Kondo (1975) neutral transfer coefficients of vapour (cE) and heat (cH) at
+// wind speed ws: the five wind ranges of si3d_boundaryconditions.f90:5166-5178.
+__device__ void kondo(double ws, double &cE, double &cH) {
+  const double lim[6] = {0.0, 2.2, 5.0, 8.0, 25.0, 50.0};
+  const double ah[5] = {0.000, 0.9270, 1.1500, 1.1700, 1.652}, bh[5] = {1.185, 0.0521, 0.0100, 0.0075, -0.017},
+               ch[5] = {0.000, 0.0000, 0.0000, -4.5E-4, 0.000}, ph[5] = {-0.157, 1., 1., 1., 1.};
+  const double ae[5] = {0.000, 0.9690, 1.1800, 1.1960, 1.680}, be[5] = {1.230, 0.0521, 0.0100, 0.0080, -0.016},
+               ce[5] = {0.000, 0.0000, 0.0000, -4.0E-4, 0.000}, pe[5] = {-0.160, 1., 1., 1., 1.};
+  int r = 4;
+  for (int q = 0; q < 4; ++q)
+    if (ws > lim[q] && ws <= lim[q + 1]) {
+      r = q;
+      break;
+    }
+  cH = (ah[r] + bh[r] * pow(ws, ph[r]) + ch[r] * (ws - 8.) * (ws - 8.)) * 1.e-3;
+  cE = (ae[r] + be[r] * pow(ws, pe[r]) + ce[r] * (ws - 8.) * (ws - 8.)) * 1.e-3;
+}
+
+// The met-station modes per wet column (ifsurfbc 10, 11; InterpMetData and
+// DistributeMomentumHeatSourcesH cases 10 / 11, si3d_boundaryconditions.f90:
+// 5331-5483, 5703-5748): the station values weighted by the column's weights
+// W[m][c] (the two-pass scheme divides by their sum and adds the residuals
+// weighted by W^(1/gammaB), normalised), the column's wind and its drag (icw
+// 0: cw; 1: Amorocho & DeVries; 2: Andreas et al., whose square-root term
+// the reference raises to (1/2) = 0) into D.wind, and HeatSource.  Mode 10
+// computes the air's longwave from a cloud cover of 0 (the module value the
+// reference reads there, which that mode never sets).
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_met_heat(Dev<R> D, MetArgs M, const double *W, double *wind) {
+  const int c = col();
+  if (c >= D.lm) return;
+  double x[6] = {0, 0, 0, 0, 0, 0};
+  double sw = 0.0, sg = 0.0;
+  for (int m = 0; m < M.nmet; ++m) {
+    const double w = W[(size_t)m * D.LP + c];
+    sw += w;
+    for (int q = 0; q < 6; ++q) x[q] += w * M.st[6 * m + q];
+  }
+  if (M.iinterp == 2) {
+    for (int q = 0; q < 6; ++q) x[q] /= sw;
+    for (int m = 0; m < M.nmet; ++m) sg += pow(W[(size_t)m * D.LP + c], 1.0 / M.gamma);
+    for (int m = 0; m < M.nmet; ++m) {
+      const double g = pow(W[(size_t)m * D.LP + c], 1.0 / M.gamma) / sg;
+      for (int q = 0; q < 6; ++q) x[q] += g * M.res[6 * m + q];
+    }
+  }
+  const double ua = x[4], va = x[5], ws = sqrt(ua * ua + va * va);
+  double cdw;
+  if (M.icw == 0) cdw = M.cw;
+  else if (M.icw == 1) cdw = 0.0015 / (1.0 + exp((12.5 - ws) / 1.56)) + 0.00104;
+  else if (ws == 0.0) cdw = M.cw;
+  else {
+    const double us = (0.239 + 0.0433 * ((ws - 8.271) + 1.0)) / ws;
+    cdw = us * us;
+  }
+  wind[3 * c] = ua;
+  wind[3 * c + 1] = va;
+  wind[3 * c + 2] = cdw;
+  constexpr double kStefanBoltzmann = 5.6697e-8, kAlbedoLw = 0.03;
+  const double Qsw = x[0], Ta = x[1], TaK = Ta + 273.;
+  const double ea = 2.1718e10 * exp(-4157. / (TaK - 33.91)) * x[2];
+  double Qbra;
+  if (M.mode == 10) {
+    const double em_air = 0.642 * pow(ea / TaK, 0.14285714);  // cloud cover 0
+    Qbra = em_air * kStefanBoltzmann * TaK * TaK * TaK * TaK * (1. - kAlbedoLw);
+  } else {
+    Qbra = x[3] * (1. - kAlbedoLw);
+  }
+  double cE, cH;
+  kondo(ws, cE, cH);
+  heat_column(D, c, M.eta, Qsw, surface_flux(D, c, Qbra, ea, Ta, M.Pa, ws, cE, cH));
 }
 
 // UpdateHorizontalMixingCoefficients (Smagorinsky), si3d_mixing.f90:894-966.
@@ -1736,6 +1829,10 @@ static void surface_forcing(s3g_ctx *c, int n, int istep) {
   if (c->heat_due) {
     c->heat_due = false;
     LAUNCH("heat_source", (k_heat_source<R>), c->lm, dev<R>(c), c->heat_force);
+  }
+  if (c->met_due) {
+    c->met_due = false;
+    LAUNCH("met_heat", (k_met_heat<R>), c->lm, dev<R>(c), c->met, (const double *)c->d_wmet, c->d_wind);
   }
 }
 

@@ -36,6 +36,11 @@ MODULE si3d_gpu
       REAL(C_DOUBLE) :: pcg_tol
       INTEGER(C_INT) :: pcg_maxit
       INTEGER(C_INT) :: ntr
+      ! met stations of ifsurfbc 10 / 11 (unused by the other modes)
+      INTEGER(C_INT) :: icw = 0, nmetstat = 0
+      TYPE(C_PTR)    :: metxy = C_NULL_PTR
+      INTEGER(C_INT) :: iinterp = 1
+      REAL(C_DOUBLE) :: gammaB = 0.0_C_DOUBLE, delNfactor = 0.0_C_DOUBLE
    END TYPE s3g_params
 
    ! input deck of the standalone driver (s3g_create_from_deck)

@@ -330,6 +330,34 @@ def read_surfbc(path, nvar):
 _SURFBC_NVAR = {1: 6, 2: 9, 3: 9, 20: 3}
 
 
+def read_met_surfbc(path):
+    """si3d_surfbc.txt of the met-station modes (ifsurfbc 10, 11;
+    si3d_boundaryconditions.f90:4301-4450): six header records, the station
+    count in (10X,I7), the stations' x, y (grid units) in (10X, 2n G11.2),
+    the record count in (10X,I7), then records of 6 n + 2 values in
+    (10X, G11.2 ...): eta, Pa, and per station Qsw, Ta, Rh, Cc (10) or Qlw
+    (11), uair, vair.  Returns (metxy as n x 2, series as npts x (6 n + 2))."""
+    F = _File(path)
+    F.skip(6)
+    r = F.record()
+    r.skip(10)
+    n = r.int(7)
+    r = F.record()
+    r.skip(10)
+    xy = np.array([r.real(11) for _ in range(2 * n)]).reshape(n, 2)
+    r = F.record()
+    r.skip(10)
+    npts = r.int(7)
+    nv = 6 * n + 2
+    out = np.zeros((npts, nv))
+    for j in range(npts):
+        r = F.record()
+        r.skip(10)
+        for q in range(nv):
+            out[j, q] = r.real(11)
+    return xy, out
+
+
 def read_openbc(path, nvar):
     """openbcNN.txt (si3d_boundaryconditions.f90:55-86): five header records,
     the record count in (10X,I7), then records of nvar = ntr + 2 values in
@@ -359,7 +387,7 @@ def read_case(directory, bathy_file="si3d_bathy", missing_icw=False):
     Open boundaries of types 1 and 2 come with their openbcNN.txt series.
     Raises InputError for options this path does not run (submerged-flow and
     nested-grid boundaries, point sources, water-quality or tracer-cloud
-    models, met-station interpolation, GOTM)."""
+    models, GOTM)."""
     p = read_inp(os.path.join(directory, "si3d_inp.txt"), missing_icw=missing_icw)
     why = []
     if p["nopen"] > 0 and any(t not in (1, 2) for t in p["itype"]):
@@ -368,7 +396,7 @@ def read_case(directory, bathy_file="si3d_bathy", missing_icw=False):
         why.append("point sources and sinks (iopss > 0)")
     if p["ntr"] > 0 and p["ecomod"] != 0:
         why.append(f"tracer model ecomod = {p['ecomod']}")
-    if p["ifsurfbc"] not in (0, 1, 2, 3, 20):
+    if p["ifsurfbc"] not in (0, 1, 2, 3, 10, 11, 20):
         why.append(f"ifsurfbc = {p['ifsurfbc']}")
     if p["iturb"] < 0:
         why.append("GOTM turbulence (iturb < 0)")
@@ -394,6 +422,10 @@ def read_case(directory, bathy_file="si3d_bathy", missing_icw=False):
     if p["ifsurfbc"] in _SURFBC_NVAR:
         name = "si3d_surfbc_W.txt" if p["ifsurfbc"] == 20 else "si3d_surfbc.txt"
         d["surfbc"] = read_surfbc(os.path.join(directory, name), _SURFBC_NVAR[p["ifsurfbc"]])
+    elif p["ifsurfbc"] in (10, 11):
+        xy, series = read_met_surfbc(os.path.join(directory, "si3d_surfbc.txt"))
+        d.update(surfbc=series, metxy=xy.ravel(), nmetstat=len(xy), iinterp=p["iinterp"], icw=p["icw"],
+                 gammaB=p.get("gammaB", 0.0), delNfactor=p.get("delNfactor", 0.0))
     if p["nopen"] > 0:
         series = [read_openbc(os.path.join(directory, openbc_name(nn + 1)), p["ntr"] + 2)
                   for nn in range(p["nopen"])]
@@ -442,7 +474,7 @@ def write_inp(deck, path, title="si3d case written by si3d_amd", ntr=0, start=(2
     labels = dict(ismooth="smooth", Av0="az0", Dv0="dz0", Ax0="ax0", Ay0="ay0", itrsca="itrsch", ifsurfbc="ifsbc",
                   dtsurfbc="dtsbc", wa="ws", num_threads="nth")
     for name in _SIM_NAMES:
-        v = {"itrsca": 4, "icw": 0, "idbg": 0, "num_threads": 1}.get(name, deck.get(name, 0))
+        v = {"itrsca": 4, "idbg": 0, "num_threads": 1}.get(name, deck.get(name, 0))
         L.append(_line(labels.get(name, name), str(int(v)) if name in _INT_NAMES else _fmt_real(v)))
     L += _section("Output specifications for time files (individual nodes)")
     L += [_line("ipt", "0"), _line("nnodes", "0"), " inodes      !\n", " jnodes      !\n"]
@@ -471,7 +503,10 @@ def write_inp(deck, path, title="si3d case written by si3d_amd", ntr=0, start=(2
     L += [_line("iopts", "0")] + [f" {k:<12}!\n" for k in ("Nodev", "iodt", "ipss", "jpss", "iodev")]
     if deck.get("ifsurfbc", 0) >= 10:  # read for every ifsurfbc >= 10, the wind series (20) included
         L += _section("Specification for interpolation method")
-        L += [_line("iinterp", "1")]
+        ii = int(deck.get("iinterp", 1))
+        L += [_line("iinterp", str(ii))]
+        if ii == 2:
+            L += [_line("gammaB", _fmt_real(deck["gammaB"])), _line("delNfactor", _fmt_real(deck["delNfactor"]))]
     with open(path, "w") as f:
         f.writelines(L)
 
@@ -513,10 +548,15 @@ def write_init(profile, path, tracers=None):
         f.writelines(L)
 
 
-def write_surfbc(series, dtsurfbc, path):
+def write_surfbc(series, dtsurfbc, path, metxy=None):
+    """si3d_surfbc*.txt; with `metxy` (n x 2) the met-station layout of
+    ifsurfbc 10 / 11 (station count and positions before the record count)."""
     s = np.asarray(series, dtype=np.float64)
-    L = ["Surface boundary condition file for si3d model\n", "written by si3d_amd\n", "-\n", "-\n", "-\n", "-\n",
-         f"   npts = {s.shape[0]:7d}\n"]
+    L = ["Surface boundary condition file for si3d model\n", "written by si3d_amd\n", "-\n", "-\n", "-\n", "-\n"]
+    if metxy is not None:
+        xy = np.asarray(metxy, dtype=np.float64).ravel()
+        L += [f"   nmet = {len(xy) // 2:7d}\n", " stations " + "".join(f"{v:11.4f}" for v in xy) + "\n"]
+    L += [f"   npts = {s.shape[0]:7d}\n"]
     for j in range(s.shape[0]):
         L.append(f"{j * dtsurfbc / 3600.0:10.4f}" + "".join(f"{v:11.4e}" for v in s[j]) + "\n")
     with open(path, "w") as f:
@@ -546,7 +586,8 @@ def write_case(deck, directory):
     write_init(deck["init_profile"], os.path.join(directory, "si3d_init.txt"), deck.get("tracer_profiles"))
     if deck.get("surfbc") is not None:
         name = "si3d_surfbc_W.txt" if deck["ifsurfbc"] == 20 else "si3d_surfbc.txt"
-        write_surfbc(deck["surfbc"], deck["dtsurfbc"], os.path.join(directory, name))
+        write_surfbc(deck["surfbc"], deck["dtsurfbc"], os.path.join(directory, name),
+                     metxy=deck.get("metxy") if deck["ifsurfbc"] in (10, 11) else None)
     if deck.get("openbc") is not None:
         for nn, series in enumerate(np.asarray(deck["openbc"]["values"])):
             write_openbc(series, deck["openbc"]["dtsec"], os.path.join(directory, openbc_name(nn + 1)))

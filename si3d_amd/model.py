@@ -29,7 +29,8 @@ TWO_D = ("s", "sp", "spp", "sx", "sy", "dd", "qq", "rr", "eagx", "earx", "eagy",
 def make_params(deck, precision="fp64", preconditioner="jacobi", tol=None, maxit=None):
     p = S3gParams()
     for name, _ in S3gParams._fields_:
-        if name in ("surfbc", "n_surfbc", "precision", "preconditioner", "pcg_tol", "pcg_maxit", "ntr"):
+        if name in ("surfbc", "n_surfbc", "precision", "preconditioner", "pcg_tol", "pcg_maxit", "ntr", "metxy",
+                    "nmetstat"):
             continue
         if name in deck:
             setattr(p, name, deck[name])
@@ -41,6 +42,11 @@ def make_params(deck, precision="fp64", preconditioner="jacobi", tol=None, maxit
         p.n_surfbc = sb.shape[0]
     if deck.get("tracer_profiles") is not None:
         p.ntr = int(np.asarray(deck["tracer_profiles"]).shape[0])
+    if deck.get("metxy") is not None:
+        mx = np.ascontiguousarray(deck["metxy"], dtype=np.float64).ravel()
+        keep.append(mx)
+        p.metxy = mx.ctypes.data_as(_lib.dp)
+        p.nmetstat = mx.size // 2
     p.precision = _PREC[precision]
     p.preconditioner = _PRECOND[preconditioner]
     p.pcg_tol = tol if tol is not None else deck.get("pcg_tol", 1e-10)

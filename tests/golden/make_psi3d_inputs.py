@@ -5,9 +5,10 @@ is present:
 
     python tests/golden/make_psi3d_inputs.py /root/reference
 
-The met series sample_files/si3d_surfbc.txt (8917 records) is cut to its
-first 48 records, with the record count of its header line rewritten to 48;
-every other file is copied unchanged."""
+The met series sample_files/si3d_surfbc.txt (8917 records) and the Clear
+Lake met-station series CLWaterQuality/si3d_surfbc.txt (7 stations, 2252
+records) are cut to their first 48 records, with the record count of their
+header line rewritten to 48; every other file is copied unchanged."""
 import os
 import shutil
 import sys
@@ -15,7 +16,7 @@ import sys
 CASES = {
     "sample_files": ["si3d_inp.txt", "si3d_bathy", "si3d_layer.txt", "si3d_init.txt", "si3d_surfbc.txt"],
     "windDrivenFlow": ["si3d_inp.txt", "h", "si3d_init.txt", "readme.txt"],
-    "CLWaterQuality": ["si3d_inp.txt", "si3d_layer.txt", "si3d_init.txt"],
+    "CLWaterQuality": ["si3d_inp.txt", "si3d_layer.txt", "si3d_init.txt", "si3d_surfbc.txt"],
 }
 SRC = {"sample_files": "sample_files", "windDrivenFlow": "sample_runs/windDrivenFlow",
        "CLWaterQuality": "sample_runs/CLWaterQuality"}
@@ -29,11 +30,12 @@ def main(ref):
         os.makedirs(dst, exist_ok=True)
         for f in files:
             src = os.path.join(ref, SRC[case], f)
-            if case == "sample_files" and f == "si3d_surfbc.txt":
+            if f == "si3d_surfbc.txt":
                 with open(src) as fi:
                     lines = fi.readlines()
-                head, rec = lines[:7], lines[7:7 + KEEP]
-                head[6] = head[6].replace(head[6].split("=")[1].strip(), str(KEEP).rjust(4))
+                nh = 7 if case == "sample_files" else 9  # the met-station file has 2 more header records
+                head, rec = lines[:nh], lines[nh:nh + KEEP]
+                head[-1] = head[-1].replace(head[-1].split("=")[1].strip(), str(KEEP).rjust(4))
                 with open(os.path.join(dst, f), "w") as fo:
                     fo.writelines(head + rec)
             else:

@@ -66,17 +66,30 @@ def test_wind_driven_flow_sample_run():
     assert np.all(prof == 4.0)
 
 
+def test_clear_lake_met_station_series_is_read():
+    """CLWaterQuality's si3d_surfbc.txt (ifsurfbc 11; the fixture keeps the
+    first 48 of its 2252 hourly records): 7 stations at their grid positions,
+    then records of attenuation, pressure and, per station, shortwave, air
+    temperature, humidity, incoming longwave and wind, as its header says."""
+    xy, s = I.read_met_surfbc(os.path.join(G, "CLWaterQuality", "si3d_surfbc.txt"))
+    assert xy.shape == (7, 2) and tuple(xy[0]) == (143.0, 82.0) and tuple(xy[2]) == (227.0, 26.0)
+    assert s.shape == (48, 6 * 7 + 2)
+    assert (s[0, 0], s[0, 1]) == (0.5, 96630.603)
+    assert tuple(s[0, 2:8]) == (0.0, 11.35, 0.705, 129.5495, 0.7534, -0.4771)
+    assert s[2, 2] == 0.7271
+
+
 def test_clear_lake_case_is_read_and_refused():
-    """CLWaterQuality: met-station interpolation (ifsurfbc 11) and the water
-    quality model with 11 tracers; its input records are read, and read_case
-    names the options this path does not run."""
+    """CLWaterQuality: met-station interpolation (ifsurfbc 11, read) and the
+    water quality model with 11 tracers; its input records are read, and
+    read_case names the option this path does not run."""
     p = I.read_inp(os.path.join(G, "CLWaterQuality", "si3d_inp.txt"))
     assert (p["im"], p["jm"], p["ifsurfbc"], p["ntr"], p["ecomod"], p["iinterp"]) == (240, 194, 11, 11, 1, 1)
     z = I.read_layers(os.path.join(G, "CLWaterQuality", "si3d_layer.txt"))
     assert len(z) == 37 and z[2] == 0.5
     prof, tr = I.read_init(os.path.join(G, "CLWaterQuality", "si3d_init.txt"), 37, 11)
     assert prof[0] == 20.6994 and tr[0, 0] == 6393.75 and tr.shape == (11, 37)
-    with pytest.raises(I.InputError, match="ecomod = 1.*ifsurfbc = 11"):
+    with pytest.raises(I.InputError, match="ecomod = 1"):
         I.read_case(os.path.join(G, "CLWaterQuality"))
 
 
@@ -91,9 +104,12 @@ def _river():
     return lake.open_boundary(d, "S", 2, 3, 8, lake.steady_series(4.0, 16.5, [0.5], npts=50))
 
 
-@pytest.mark.parametrize("which", ["ellipse", "rectangle", "heat3", "river"])
+@pytest.mark.parametrize("which", ["ellipse", "rectangle", "heat3", "river", "met11"])
 def test_write_read_round_trip(tmp_path, which):
-    if which == "ellipse":
+    if which == "met11":
+        base = lake.ellipse(nx=40, ny=24, dx=100.0, nlayers=12, nts=20, top=1.0, bottom=12.0, depth=40.0, seed=3)
+        d = lake.met_stations(base, 11, [(6.0, 5.0), (33.5, 9.25)], iinterp=2, gammaB=0.3, delNfactor=0.8, icw=1)
+    elif which == "ellipse":
         d = lake.ellipse(nx=40, ny=24, dx=100.0, nlayers=12, nts=20, top=1.0, bottom=12.0, depth=40.0, seed=3)
     elif which == "rectangle":
         d = lake.rectangle(nx=100, ny=50, dx=50.0, depth=20.0, dz=1.0, tau=0.1, dt=30.0, nts=210)
@@ -114,6 +130,10 @@ def test_write_read_round_trip(tmp_path, which):
     np.testing.assert_allclose(e["init_profile"], d["init_profile"], atol=5e-6)  # G11.5
     if d.get("surfbc") is not None:
         np.testing.assert_allclose(e["surfbc"], d["surfbc"], rtol=5e-5, atol=1e-12)  # G11.4
+    if d.get("metxy") is not None:
+        np.testing.assert_allclose(e["metxy"], d["metxy"], atol=5e-5)
+        for k in ("nmetstat", "iinterp", "gammaB", "delNfactor", "icw"):
+            assert e[k] == d[k], k
     if d.get("openbc") is not None:
         for k in ("iside", "itype", "isbc", "jsbc", "iebc", "jebc", "dtsec"):
             assert e["openbc"][k] == d["openbc"][k], k
