@@ -255,7 +255,7 @@ bool OrcLake::build(const OrcDeck *deck) {
   d.zlevel_file = nullptr;
   if (deck->n_surfbc > 0 && deck->surfbc) sbc.assign(deck->surfbc, deck->surfbc + (size_t)sbc_nvar() * deck->n_surfbc);
   d.surfbc = nullptr;
-  if (d.ifsurfbc >= 10 && deck->metxy) metxy.assign(deck->metxy, deck->metxy + 2 * (size_t)d.nmetstat);
+  if ((d.ifsurfbc == 10 || d.ifsurfbc == 11) && deck->metxy) metxy.assign(deck->metxy, deck->metxy + 2 * (size_t)d.nmetstat);
   d.metxy = nullptr;
 
   dt = d.dt;
@@ -519,7 +519,7 @@ void OrcLake::init_state() {
     q2lpp = q2l;
   }
   const int nv = sbc_nvar();
-  if (d.ifsurfbc >= 10) {  // surfbc0 cases 10, 11: weights, then the first record's wind (4301-4450)
+  if (d.ifsurfbc == 10 || d.ifsurfbc == 11) {  // surfbc0 cases 10, 11: weights, then the first record's wind (4301-4450)
     met_weights();
     if ((int)sbc.size() >= nv) met_distribute(sbc.data(), false);
   } else if (nv > 0 && (int)sbc.size() >= nv) {  // momentum flux terms at t = 0 (surfbc0)
@@ -678,7 +678,7 @@ void OrcLake::surfbc(int n, int istep) {
     for (int j = 0; j < np; ++j) f[j + 1] = sbc[(size_t)nv * j + q];
     v[q] = parab(0., thrs, f.data(), dth);
   }
-  if (d.ifsurfbc >= 10) {
+  if (d.ifsurfbc == 10 || d.ifsurfbc == 11) {
     met_distribute(v.data(), true);
     return;
   }

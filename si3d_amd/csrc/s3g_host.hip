@@ -292,7 +292,7 @@ void surfbc(s3g_ctx *c, int n, int istep) {
     return;
   }
   const int mode = c->P.ifsurfbc;
-  if (mode >= 10) {
+  if (mode == 10 || mode == 11) {
     met_surfbc(c, n, istep);
     return;
   }
@@ -523,7 +523,7 @@ static int alloc_state(s3g_ctx *c, Dev<R> &D, const int *lE, const int *lW, cons
   const bool budget = (c->P.ifsurfbc >= 1 && c->P.ifsurfbc <= 3) || c->P.ifsurfbc == 10 || c->P.ifsurfbc == 11;
   if (budget && !(D.heat = dalloc<R>(c, n3))) return S3G_ERR_HIP;
   D.wind = nullptr;
-  if (c->P.ifsurfbc >= 10) {  // per-column wind of the met-station modes
+  if (c->P.ifsurfbc == 10 || c->P.ifsurfbc == 11) {  // per-column wind of the met-station modes
     if (!c->d_wind) c->d_wind = dalloc<double>(c, 3 * (size_t)c->LP);
     D.wind = c->d_wind;
     if (!D.wind) return S3G_ERR_HIP;
@@ -588,7 +588,7 @@ int s3g::init_ctx(const s3g_grid *g, const s3g_params *p, int nlakes, const doub
     set_error("s3g_init: ifsurfbc must be 0, 1, 2, 3, 10, 11 or 20");
     return S3G_ERR_UNSUPPORTED;
   }
-  if (p->ifsurfbc >= 10 && p->ifsurfbc <= 11 &&
+  if ((p->ifsurfbc == 10 || p->ifsurfbc == 11) &&
       (p->nmetstat < 1 || p->nmetstat > s3g::kMaxMet || !p->metxy || p->iinterp < 1 || p->iinterp > 3 ||
        (p->iinterp == 2 && !(p->gammaB > 0.0)) || p->icw < 0 || p->icw > 2)) {
     set_error("s3g_init: ifsurfbc 10/11 need 1-16 met stations with their positions, iinterp 1-3 (gammaB > 0 "
@@ -704,7 +704,7 @@ int s3g::init_ctx(const s3g_grid *g, const s3g_params *p, int nlakes, const doub
   P.t_3 = std::pow(B1, -1. / 3.);
   P.t_4 = 9. * A1 * (2. * A1 + A2 * (1. - C2));
   P.t_5 = 9. * A1 * A2;
-  const int nmet = p->ifsurfbc >= 10 ? p->nmetstat : 0;
+  const int nmet = (p->ifsurfbc == 10 || p->ifsurfbc == 11) ? p->nmetstat : 0;
   const int nv = surfbc_nvar(p->ifsurfbc, nmet);
   if (p->n_surfbc > 0 && p->surfbc && nv > 0) c->surfbc.assign(p->surfbc, p->surfbc + (size_t)nv * p->n_surfbc);
   if (nmet > 0) {
