@@ -464,6 +464,7 @@ static int alloc_state(s3g_ctx *c, Dev<R> &D, const int *lE, const int *lW, cons
   const size_t n3 = (size_t)c->KP * c->LP, n2 = c->LP;
   D.lm = c->lm;
   D.LP = c->LP;
+  D.KP = c->KP;
   D.km1 = c->km1;
   D.km = c->km;
   D.k1 = c->k1;
@@ -483,6 +484,16 @@ static int alloc_state(s3g_ctx *c, Dev<R> &D, const int *lE, const int *lW, cons
   D.lN = up_i(lN);
   D.lS = up_i(lS);
   D.kmz = up_i(kmz);
+  {
+    const int S = c->lm;
+    std::vector<int> kmu(n2, -1), kmv(n2, -1);
+    for (int l = 0; l < S; ++l) {
+      if (lE[l] != S) kmu[l] = std::min(kmz[lE[l]], kmz[l]);
+      if (lN[l] != S) kmv[l] = std::min(kmz[lN[l]], kmz[l]);
+    }
+    D.kmu = up_i(kmu.data());
+    D.kmv = up_i(kmv.data());
+  }
   D.k1z = up_i(k1z);
   D.k1u = up_i(k1u);
   D.k1v = up_i(k1v);
@@ -633,6 +644,7 @@ int s3g::init_ctx(const s3g_grid *g, const s3g_params *p, int nlakes, const doub
     if (const char *x = std::getenv("S3G_EXSAL")) c->exsal_split = std::strcmp(x, "fused") != 0;
     if (const char *x = std::getenv("S3G_TURB")) c->turb_split = std::strcmp(x, "split") == 0;
     if (const char *x = std::getenv("S3G_BUDGET")) c->tke_budget = x[0] != '0';
+    if (const char *x = std::getenv("S3G_WINDOW")) c->window = x[0] != '0';
     // S3G_TILES="w0,w1,...": fixed tile words (chunk | order << 8), no tuning
     if (const char *w = std::getenv("S3G_TILES")) {
       c->tune = false;
@@ -754,6 +766,10 @@ int s3g::init_ctx(const s3g_grid *g, const s3g_params *p, int nlakes, const doub
     for (int k = 1; k <= c->km1; ++k) zl[k] = g->zlevel[k - 1];
   }
   c->n_wet = nwet;
+  c->nb_symmetric = true;
+  for (int l = 0; l < lm && c->nb_symmetric; ++l)
+    c->nb_symmetric = (lE[l] == S || lW[lE[l]] == l) && (lW[l] == S || lE[lW[l]] == l) &&
+                      (lN[l] == S || lS[lN[l]] == l) && (lS[l] == S || lN[lS[l]] == l);
   int rc = (c->precision == S3G_FP64)
                ? alloc_state(c, c->d64, lE.data(), lW.data(), lN.data(), lS.data(), kmz.data(), k1z.data(),
                              k1u.data(), k1v.data(), zl.data(), hhs.data(), hhu.data(), hhv.data(), lake.data(),

@@ -63,10 +63,14 @@ struct MetArgs {
 // in every mode.
 template <typename R>
 struct Dev {
-  int lm, LP, km1, km, k1;
+  int lm, LP, km1, km, k1, KP;  // KP: planes per 3-D field
   Params P;
   double uair, vair, cdw;  // spatially uniform forcing (surfbc cases 0, 20)
   const int *lE, *lW, *lN, *lS, *kmz;
+  // bottom layer of the u face east of each column (min(kmz, kmz east)) and of
+  // the v face north of it; -1 where the face does not exist (dry neighbour)
+  // and at the sentinel column
+  const int *kmu, *kmv;
   int *k1z, *k1u, *k1v;
   const double *zlevel;  // [km1+3], zlevel[km1+1] = +huge; [nlakes][KPz] for an ensemble
   // ensemble of lakes (null for one lake): lake of each column (sentinel: 0),
@@ -272,6 +276,10 @@ struct s3g_ctx {
   bool tke_budget = true;   // accumulate the closure's basin energy budget (S3G_BUDGET=0: off)
   bool turb_split = false;  // turbulence closure: per-cell interface terms + sweep (else one fused sweep)
   bool tune = true, tuned = false;
+  // windowed per-cell kernels (S3G_WINDOW=0 keeps the generic ones); every
+  // wet neighbour points back (lW[lE[c]] = c, lS[lN[c]] = c, ...), which the
+  // windowed exmom kernel relies on
+  bool window = true, nb_symmetric = false;
   // ensemble of lakes (s3g_create_ensemble): column offset of each lake
   // ([nlakes+1]), each lake's km1 and forcing parameters
   int ntr = 0;                           // conservative tracers

@@ -300,6 +300,140 @@ k_exmom1(Dev<R> D, double twodt1, int nk, int tile) {
   exmom_tile<R, X, false>(D, twodt1, nk, tile);
 }
 
+// Buffer-resource loads of the per-cell kernels.  In a per-cell kernel the
+// layer k is the same for the whole workgroup, so each 3-D field is read
+// through a buffer resource of the whole field (scalar registers), a scalar
+// plane offset k*LP*sizeof(R) shared by every field, and a 32-bit byte offset
+// per neighbour column shared by every field: one VGPR per neighbour instead
+// of a 64-bit address (and its carry arithmetic) per field and neighbour.
+// Fields must be below 4 GiB (checked on the host before these kernels run).
+template <typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const T *a, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void *)a, (short)0, (int)bytes, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ double bld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  if constexpr (sizeof(T) == 8)
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  else
+    return (double)__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+template <typename T>
+__device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, T v, unsigned voff, unsigned soff) {
+  if constexpr (sizeof(T) == 8)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((__vector_size__(8))) unsigned, v), r,
+                                          voff, soff, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+}
+
+// exmom for the centred scheme (itrmom 1) with constant-coefficient or no
+// horizontal diffusion (ihd 0, 1): the same value as exmom_value, evaluated
+// in the same order (so the two kernels agree bit for bit), for a chunk of
+// layers per lane.  The column's neighbour indices, face bottoms (kmu / kmv)
+// and top indices are loaded once per chunk; the vertical neighbours k-1,
+// k+1 of q, its face thickness and wp at k+1 slide down the chunk in
+// registers.  Relies on symmetric neighbours: the east neighbour of the west
+// neighbour is the column itself (lE[lW[c]] = c), and likewise north of south.
+template <typename R, bool X>
+__global__ void __launch_bounds__(kBlock) k_exmom_w(Dev<R> D, double twodt1, int nk, int tile) {
+  CellTile t;
+  if (!cell_tile(D.lm, 2, nk, tile, t)) return;
+  const int c = t.c, S = D.lm;
+  const int lE = D.lE[c], lW = D.lW[c], lN = D.lN[c], lS = D.lS[c];
+  const int ln = X ? lE : lN;
+  if (ln == S) return;  // no face: ex stays zero
+  const int *kmf = X ? D.kmu : D.kmv;
+  const int *k1f = X ? D.k1u : D.k1v;
+  const int kmq = kmf[c];
+  if (t.k0 > kmq) return;
+  const int k1q = k1f[c];
+  const int l2 = X ? D.lS[lE] : D.lW[lN];  // SE corner (x) / NW corner (y)
+  // diffusion coefficient of the west face (x) and south face (y) of the cell:
+  // non-zero on the neighbour face's own wet range
+  const int k1W = k1f[lW], kmW = kmf[lW], k1S = k1f[lS], kmS = kmf[lS];
+  const unsigned sz = sizeof(R), nb = (unsigned)D.KP * (unsigned)D.LP * sz, pb = (unsigned)D.LP * sz;
+  const unsigned oc = (unsigned)c * sz, oE = (unsigned)lE * sz, oW = (unsigned)lW * sz, oN = (unsigned)lN * sz,
+                 oS = (unsigned)lS * sz, o2 = (unsigned)l2 * sz, on = (unsigned)ln * sz;
+  const auto q = rsrc(X ? D.up : D.vp, nb), qpp = rsrc(X ? D.upp : D.vpp, nb);
+  const auto hq = rsrc(X ? D.hup : D.hvp, nb), hqq = rsrc(X ? D.hupp : D.hvpp, nb);
+  const auto qhpp = rsrc(X ? D.uhpp : D.vhpp, nb), ex = rsrc(X ? D.ex : D.ey, nb);
+  const auto uhp = rsrc(D.uhp, nb), vhp = rsrc(D.vhp, nb), wp = rsrc(D.wp, nb), hp = rsrc(D.hp, nb);
+  const double f = D.P.f, iadv = D.P.iadv, Ax0 = D.P.Ax0, Ay0 = D.P.Ay0;
+  const int ihd = D.P.ihd;
+  unsigned pk = (unsigned)t.k0 * pb;  // byte offset of plane k
+  double qm = bld<R>(q, oc, pk - pb), q0 = bld<R>(q, oc, pk);
+  double hm = bld<R>(hq, oc, pk - pb), h0 = bld<R>(hq, oc, pk);
+  double w0c = bld<R>(wp, oc, pk), w0n = bld<R>(wp, on, pk);
+  for (int k = t.k0; k < t.k1; ++k, pk += pb) {
+    if (k > kmq) break;
+    const unsigned pk1 = pk + pb;
+    const double qp = bld<R>(q, oc, pk1), hpk = bld<R>(hq, oc, pk1);
+    const double w1c = bld<R>(wp, oc, pk1), w1n = bld<R>(wp, on, pk1);
+    double val = 0.0;
+    if (k >= k1q) {
+      if (bld<R>(hp, oc, pk) <= 0.0 || bld<R>(hp, on, pk) <= 0.0) {
+        val = bld<R>(qhpp, oc, pk);
+      } else {
+        double cor, uE, uW, vN, vS;
+        if (X) {
+          const double vhE = bld<R>(vhp, oE, pk), vhc = bld<R>(vhp, oc, pk), vh2 = bld<R>(vhp, o2, pk),
+                       vhS = bld<R>(vhp, oS, pk);
+          const double uhc = bld<R>(uhp, oc, pk);
+          cor = 0.25 * f * (vhE + vhc + vh2 + vhS);
+          uE = bld<R>(uhp, oE, pk) + uhc;
+          uW = uhc + bld<R>(uhp, oW, pk);
+          vN = vhE + vhc;
+          vS = vh2 + vhS;
+        } else {
+          const double uhN = bld<R>(uhp, oN, pk), uhc = bld<R>(uhp, oc, pk), uh2 = bld<R>(uhp, o2, pk),
+                       uhW = bld<R>(uhp, oW, pk);
+          const double vhc = bld<R>(vhp, oc, pk);
+          cor = 0.25 * f * (uhN + uhc + uh2 + uhW);
+          uE = uhN + uhc;
+          uW = uh2 + uhW;
+          vN = bld<R>(vhp, oN, pk) + vhc;
+          vS = vhc + bld<R>(vhp, oS, pk);
+        }
+        double wU = w0n + w0c, wD = w1n + w1c;
+        if (k == k1q) wU = 0.0;
+        if (k == kmq) wD = 0.0;
+        const double qc = q0;
+        const double scE = bld<R>(q, oE, pk) + qc, scW = bld<R>(q, oW, pk) + qc;
+        const double scN = bld<R>(q, oN, pk) + qc, scS = bld<R>(q, oS, pk) + qc;
+        double adv = (uE * scE - uW * scW) / D.P.fourdx + (vN * scN - vS * scS) / D.P.fourdy;
+        const double hkm = hm, hk = h0, hkp = hpk;
+        const double th = (k == k1q) ? 0.0 : hkm / (hkm + hk);
+        const double th1 = (k == k1q) ? 0.0 : 1.0 - th;
+        const double thp = (k == kmq) ? 0.5 : hk / (hk + hkp);
+        const double th1p = (k == kmq) ? 0.5 : 1.0 - thp;
+        adv += (wU * (th * qc + th1 * qm) - wD * (thp * qp + th1p * qc)) / 2.;
+        double hd = 0.0;
+        if (ihd >= 1) {
+          const double pc = bld<R>(qpp, oc, pk), hqc = bld<R>(hqq, oc, pk);
+          const double axc = Ax0 * dmin(bld<R>(hqq, oE, pk), hqc);
+          const double axw = (k < k1W || k > kmW) ? 0.0 : Ax0 * dmin(hqc, bld<R>(hqq, oW, pk));
+          const double ayc = Ay0 * dmin(bld<R>(hqq, oN, pk), hqc);
+          const double ays = (k < k1S || k > kmS) ? 0.0 : Ay0 * dmin(hqc, bld<R>(hqq, oS, pk));
+          hd = (ayc * (bld<R>(qpp, oN, pk) - pc) - ays * (pc - bld<R>(qpp, oS, pk))) / D.P.dydy +
+               (axc * (bld<R>(qpp, oE, pk) - pc) - axw * (pc - bld<R>(qpp, oW, pk))) / D.P.dxdx;
+        }
+        if (X)
+          val = bld<R>(qhpp, oc, pk) - twodt1 * (adv * iadv - cor - hd);
+        else
+          val = bld<R>(qhpp, oc, pk) - twodt1 * (adv * iadv + cor - hd);
+      }
+    }
+    bst<R>(ex, (R)val, oc, pk);
+    qm = q0;
+    q0 = qp;
+    hm = h0;
+    h0 = hpk;
+    w0c = w1c;
+    w0n = w1n;
+  }
+}
+
 // bclnc_km: baroclinic term of the bottom layer on a horizontal plane,
 // si3d_procedures.f90:1633-1746.
 template <typename R, bool X>
@@ -864,6 +998,122 @@ __global__ void __launch_bounds__(kBlock) k_exsal_rhs(Dev<R> D, Scal<R> sv, doub
   }
   if (TR && kms >= t.k0 && kms < t.k1)  // Bex of the sediment layer (exTracer 4342)
     D.bex[IX(kms + 1, c)] = (R)((double)D.hpp[IX(kms + 1, c)] * (double)sv.cpp[IX(kms + 1, c)] / twodt1);
+}
+
+// Windowed forms of exsal's two per-cell kernels: the same values in the same
+// order as k_exsal_hflux / k_exsal_rhs, with the chunk's layers walked
+// together by the whole workgroup (k is uniform, so the fields are read
+// through buffer resources and one 32-bit offset per neighbour column), the
+// column's indices loaded once per chunk, and, in the right-hand side, the
+// vertical neighbours k-2 .. k+2 of the scalar and of hpp, k-1 .. k+1 of hp
+// and wp at k, k+1 carried down the chunk in registers.
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_exsal_hflux_w(Dev<R> D, Scal<R> sv, double twodt1, int nk, int tile) {
+  CellTile t;
+  if (!cell_tile(D.lm, 2, nk, tile, t)) return;
+  const int c = t.c;
+  const int kms = D.kmz[c], k1s = D.k1z[c];
+  const int kb = max(t.k0, k1s), ke = min(t.k1, kms + 1);
+  if (kb >= ke) return;
+  const int lE = D.lE[c], lW = D.lW[c], lN = D.lN[c], lS = D.lS[c];
+  const int lEE = D.lE[lE], lNN = D.lN[lN];
+  const unsigned sz = sizeof(R), nb = (unsigned)D.KP * (unsigned)D.LP * sz, pb = (unsigned)D.LP * sz;
+  const unsigned oc = (unsigned)c * sz, oE = (unsigned)lE * sz, oW = (unsigned)lW * sz, oN = (unsigned)lN * sz,
+                 oS = (unsigned)lS * sz, oEE = (unsigned)lEE * sz, oNN = (unsigned)lNN * sz;
+  const auto cpp = rsrc(sv.cpp, nb), hpp = rsrc(D.hpp, nb), hup = rsrc(D.hup, nb), hvp = rsrc(D.hvp, nb);
+  const auto uhpp = rsrc(D.uhpp, nb), uh = rsrc(D.uh, nb), vhpp = rsrc(D.vhpp, nb), vh = rsrc(D.vh, nb);
+  const auto few = rsrc(D.fx_ew, nb), fns = rsrc(D.fx_ns, nb);
+  const double cx = twodt1 / D.P.dx, cy = twodt1 / D.P.dy;
+  unsigned pk = (unsigned)t.k0 * pb;
+  for (int k = t.k0; k < t.k1; ++k, pk += pb) {
+    if (k >= ke) break;
+    if (k < kb) continue;
+    double fe = 0.0, fn = 0.0;
+    const double s2 = bld<R>(cpp, oc, pk);
+    if (bld<R>(hup, oc, pk) > 0.0) {
+      const double vel = (bld<R>(uhpp, oc, pk) + bld<R>(uh, oc, pk)) / 2.;
+      const double s3 = bld<R>(cpp, oE, pk);
+      const double s1 = (bld<R>(hpp, oW, pk) <= 0.0) ? s2 : bld<R>(cpp, oW, pk);
+      const double s4 = (bld<R>(hpp, oEE, pk) <= 0.0) ? s3 : bld<R>(cpp, oEE, pk);
+      fe = tvd_flux(vel, s1, s2, s3, s4, 0.0, cx);
+    }
+    if (bld<R>(hvp, oc, pk) > 0.0) {
+      const double vel = (bld<R>(vhpp, oc, pk) + bld<R>(vh, oc, pk)) / 2.;
+      const double s3 = bld<R>(cpp, oN, pk);
+      const double s1 = (bld<R>(hpp, oS, pk) <= 0.0) ? s2 : bld<R>(cpp, oS, pk);
+      const double s4 = (bld<R>(hpp, oNN, pk) <= 0.0) ? s3 : bld<R>(cpp, oNN, pk);
+      fn = tvd_flux(vel, s1, s2, s3, s4, 0.0, cy);
+    }
+    bst<R>(few, (R)fe, oc, pk);
+    bst<R>(fns, (R)fn, oc, pk);
+  }
+}
+
+template <typename R, bool TR>
+__global__ void __launch_bounds__(kBlock) k_exsal_rhs_w(Dev<R> D, Scal<R> sv, double twodt1, int nk, int tile) {
+  CellTile t;
+  if (!cell_tile(D.lm, 2, nk, tile, t)) return;
+  const int c = t.c, S = D.lm;
+  const int kms = D.kmz[c], k1s = D.k1z[c];
+  const int kb = max(t.k0, k1s), ke = min(t.k1, kms + 1);
+  const unsigned sz = sizeof(R), nb = (unsigned)D.KP * (unsigned)D.LP * sz, pb = (unsigned)D.LP * sz;
+  const unsigned oc = (unsigned)c * sz;
+  const auto cpp = rsrc(sv.cpp, nb), hpp = rsrc(D.hpp, nb), bex = rsrc(D.bex, nb);
+  if (kb < ke) {
+    const int lE = D.lE[c], lW = D.lW[c], lN = D.lN[c], lS = D.lS[c];
+    const int kmW = D.kmz[lW], kmS = D.kmz[lS];
+    const unsigned oE = (unsigned)lE * sz, oW = (unsigned)lW * sz, oN = (unsigned)lN * sz, oS = (unsigned)lS * sz;
+    const auto hp = rsrc(D.hp, nb), wp = rsrc(D.wp, nb), few = rsrc(D.fx_ew, nb), fns = rsrc(D.fx_ns, nb);
+    const auto hupp = rsrc(D.hupp, nb), hvpp = rsrc(D.hvpp, nb);
+    const bool ihd = D.P.ihd > 0, wW = lW != S, wS = lS != S;
+    // vertical TVD flux through the top of layer j (vflux), from the window
+    auto vf = [&](double hp_above, double vel, double s1, double s2, double s3, double s4, double hp_j) -> double {
+      if (!(hp_above > 0.0)) return 0.0;
+      const double delz = (hp_j + hp_above) / 2.;
+      return tvd_flux(vel, s1, s2, s3, s4, 1.0, twodt1 / delz);
+    };
+    unsigned pk = (unsigned)t.k0 * pb;
+    double cm2 = bld<R>(cpp, oc, pk - 2 * pb), cm1 = bld<R>(cpp, oc, pk - pb), c0 = bld<R>(cpp, oc, pk),
+           c1 = bld<R>(cpp, oc, pk + pb);
+    double hm2 = bld<R>(hpp, oc, pk - 2 * pb), hm1 = bld<R>(hpp, oc, pk - pb), h0 = bld<R>(hpp, oc, pk),
+           h1 = bld<R>(hpp, oc, pk + pb);
+    double pm1 = bld<R>(hp, oc, pk - pb), p0 = bld<R>(hp, oc, pk);
+    double w0 = bld<R>(wp, oc, pk);
+    double fud_kp = 0.0;
+    for (int k = t.k0; k < t.k1; ++k, pk += pb) {
+      if (k >= ke) break;
+      const double c2 = bld<R>(cpp, oc, pk + 2 * pb), h2 = bld<R>(hpp, oc, pk + 2 * pb);
+      const double p1 = bld<R>(hp, oc, pk + pb), w1 = bld<R>(wp, oc, pk + pb);
+      if (k >= kb) {
+        // flux through the top of k: carried, or evaluated at the first layer
+        const double fud_k =
+            (k == kb) ? vf(pm1, (k == k1s) ? 0.0 : w0, (h1 <= 0.0) ? c0 : c1, c0, cm1, (hm2 <= 0.0) ? cm1 : cm2, p0)
+                      : fud_kp;
+        fud_kp = (k + 1 <= kms) ? vf(p0, w1, (h2 <= 0.0) ? c1 : c2, c1, c0, (hm1 <= 0.0) ? c0 : cm1, p1) : 0.0;
+        const double sc = c0;
+        double b = h0 * sc / twodt1 -
+                   (bld<R>(few, oc, pk) - (wW ? bld<R>(few, oW, pk) : 0.0)) / D.P.dx -
+                   (bld<R>(fns, oc, pk) - (wS ? bld<R>(fns, oS, pk) : 0.0)) / D.P.dy - (fud_k - fud_kp);
+        if (ihd) {
+          const double bc = (k > kms) ? 0.0 : D.P.Ax0 * bld<R>(hupp, oc, pk);
+          const double bw = (!wW || k > kmW) ? 0.0 : D.P.Ax0 * bld<R>(hupp, oW, pk);
+          const double bn = (k > kms) ? 0.0 : D.P.Ay0 * bld<R>(hvpp, oc, pk);
+          const double bs = (!wS || k > kmS) ? 0.0 : D.P.Ay0 * bld<R>(hvpp, oS, pk);
+          b += (bc * (bld<R>(cpp, oE, pk) - sc) - bw * (sc - bld<R>(cpp, oW, pk))) / D.P.dxdx +
+               (bn * (bld<R>(cpp, oN, pk) - sc) - bs * (sc - bld<R>(cpp, oS, pk))) / D.P.dydy;
+        }
+        bst<R>(bex, (R)b, oc, pk);
+      }
+      cm2 = cm1, cm1 = c0, c0 = c1, c1 = c2;
+      hm2 = hm1, hm1 = h0, h0 = h1, h1 = h2;
+      pm1 = p0, p0 = p1;
+      w0 = w1;
+    }
+  }
+  if (TR && kms >= t.k0 && kms < t.k1) {  // Bex of the sediment layer (exTracer 4342)
+    const unsigned ps = (unsigned)(kms + 1) * pb;
+    bst<R>(bex, (R)(bld<R>(hpp, oc, ps) * bld<R>(cpp, oc, ps) / twodt1), oc, ps);
+  }
 }
 
 // vel2: layer_h (si3d_procedures.f90:2181-2231) then the top-cell flux split for
@@ -1765,6 +2015,26 @@ static inline dim3 grid_for(int n) { return dim3((n + kBlock - 1) / kBlock); }
     if (c->timing) timer_end(c, _slot);                                     \
   } while (0)
 
+// Windowed per-cell kernels (buffer-resource loads, one chunk of layers per
+// lane): used unless S3G_WINDOW=0, for fields below 4 GiB; exmom's also needs
+// centred momentum advection, ihd 0 or 1 and symmetric neighbours.
+static inline bool use_window(const s3g_ctx *c) {
+  const size_t field = (size_t)c->KP * c->LP * (c->precision == S3G_FP64 ? 8 : 4);
+  return c->window && field < ((size_t)1 << 32);
+}
+static inline bool exmom_window(const s3g_ctx *c) {
+  return use_window(c) && c->nb_symmetric && c->P.itrmom == 1 && c->P.ihd <= 1;
+}
+#define LAUNCH_EXMOM(name, X)                                                                              \
+  do {                                                                                                     \
+    if (exmom_window(c))                                                                                   \
+      LAUNCHC(name, (k_exmom_w<R, X>), (k_exmom_w<R, X>), (X) ? TILE_EXMOM_X : TILE_EXMOM_Y, L, c->km - 1, \
+              D, twodt1);                                                                                  \
+    else                                                                                                   \
+      LAUNCHC(name, (k_exmom1<R, X>), (k_exmom<R, X, true>), (X) ? TILE_EXMOM_X : TILE_EXMOM_Y, L,         \
+              c->km - 1, D, twodt1);                                                                       \
+  } while (0)
+
 template <typename R>
 static void sync_forcing(s3g_ctx *c) {
   Dev<R> &D = dev<R>(c);
@@ -1792,11 +2062,19 @@ static void transport(s3g_ctx *c, Scal<R> S, double twodt1, int q) {
   Dev<R> &D = dev<R>(c);
   const int L = c->lm;
   const bool tr = TR;
-  LAUNCHC(tr ? "tracer_hflux" : "exsal_hflux", (k_exsal_hflux<R, false>), (k_exsal_hflux<R, true>), TILE_HFLUX, L,
-          c->km - 1, D, S, twodt1);
-  if (c->exsal_split || c->obc.any_flow) {
-    LAUNCHC(tr ? "tracer_rhs" : "exsal_rhs", (k_exsal_rhs<R, false, TR>), (k_exsal_rhs<R, true, TR>), TILE_RHS, L,
+  if (use_window(c))
+    LAUNCHC(tr ? "tracer_hflux" : "exsal_hflux", (k_exsal_hflux_w<R>), (k_exsal_hflux_w<R>), TILE_HFLUX, L,
             c->km - 1, D, S, twodt1);
+  else
+    LAUNCHC(tr ? "tracer_hflux" : "exsal_hflux", (k_exsal_hflux<R, false>), (k_exsal_hflux<R, true>), TILE_HFLUX,
+            L, c->km - 1, D, S, twodt1);
+  if (c->exsal_split || c->obc.any_flow) {
+    if (use_window(c))
+      LAUNCHC(tr ? "tracer_rhs" : "exsal_rhs", (k_exsal_rhs_w<R, TR>), (k_exsal_rhs_w<R, TR>), TILE_RHS, L,
+              c->km - 1, D, S, twodt1);
+    else
+      LAUNCHC(tr ? "tracer_rhs" : "exsal_rhs", (k_exsal_rhs<R, false, TR>), (k_exsal_rhs<R, true, TR>), TILE_RHS,
+              L, c->km - 1, D, S, twodt1);
     obc_exscalar<R>(c, S.cpp, q);
     LAUNCH(tr ? "imtracer" : "imsal", (k_exsal_imsal<R, false, true, TR>), L, D, S, twodt1);
   } else {
@@ -1852,12 +2130,12 @@ int phase(s3g_ctx *c, int ph, int n, int istep) {
       if (c->P.ihd > 1) LAUNCH("hmix", (k_hmix<R>), L, D);
       break;
     case 2:
-      LAUNCHC("exmom_x", (k_exmom1<R, true>), (k_exmom<R, true, true>), TILE_EXMOM_X, L, c->km - 1, D, twodt1);
+      LAUNCH_EXMOM("exmom_x", true);
       obc_exmom<R>(c, true, twodt1);
       break;
     case 3: LAUNCH("matmom_x", (k_matmom<R, true>), L, D, tz); break;
     case 4:
-      LAUNCHC("exmom_y", (k_exmom1<R, false>), (k_exmom<R, false, true>), TILE_EXMOM_Y, L, c->km - 1, D, twodt1);
+      LAUNCH_EXMOM("exmom_y", false);
       obc_exmom<R>(c, false, twodt1);
       break;
     case 5: LAUNCH("matmom_y", (k_matmom<R, false>), L, D, tz); break;
@@ -1877,10 +2155,16 @@ int phase(s3g_ctx *c, int ph, int n, int istep) {
       LAUNCH("continuity1", (k_continuity<R, 1>), L, D);
       obc_vel<R>(c);
       break;
-    case 9: LAUNCHC("exsal_hflux", (k_exsal_hflux<R, false>), (k_exsal_hflux<R, true>), TILE_HFLUX, L, c->km - 1, D, sal_of(D), twodt1);
-            LAUNCH("exsal_div", (k_exsal_imsal<R, true, false, false>), L, D, sal_of(D), twodt1);
-            obc_exscalar<R>(c, D.salpp, 1);
-            break;
+    case 9:
+      if (use_window(c))
+        LAUNCHC("exsal_hflux", (k_exsal_hflux_w<R>), (k_exsal_hflux_w<R>), TILE_HFLUX, L, c->km - 1, D, sal_of(D),
+                twodt1);
+      else
+        LAUNCHC("exsal_hflux", (k_exsal_hflux<R, false>), (k_exsal_hflux<R, true>), TILE_HFLUX, L, c->km - 1, D,
+                sal_of(D), twodt1);
+      LAUNCH("exsal_div", (k_exsal_imsal<R, true, false, false>), L, D, sal_of(D), twodt1);
+      obc_exscalar<R>(c, D.salpp, 1);
+      break;
     case 10:
       LAUNCH("imsal", (k_exsal_imsal<R, false, true, false>), L, D, sal_of(D), twodt1);
       obc_scalar<R>(c, D.sal, 1);
@@ -1911,7 +2195,7 @@ int autotune(s3g_ctx *c) {
   const int L = c->lm;
   const double tz = 1.0, twodt1 = c->P.twodt;
   const size_t n3 = (size_t)c->KP * c->LP;
-  const int cand[] = {1, 2, 4, 8, 16, 256 + 1, 256 + 4, 512 + 1, 512 + 2, 512 + 4, 768 + 1, 768 + 2, 768 + 4};
+  const int cand[] = {1, 2, 4, 8, 16, 32, 256 + 1, 256 + 4, 512 + 1, 512 + 2, 512 + 4, 768 + 1, 768 + 2, 768 + 4};
   const bool timing = c->timing;
   c->timing = false;
   char *save = nullptr;
@@ -1980,12 +2264,25 @@ int autotune(s3g_ctx *c) {
     snapshot(outs, false);
     auto launch = [&]() {
       switch (tid) {
-        case TILE_EXMOM_X: LAUNCHC("", (k_exmom1<R, true>), (k_exmom<R, true, true>), TILE_EXMOM_X, L, c->km - 1, D, twodt1); break;
-        case TILE_EXMOM_Y: LAUNCHC("", (k_exmom1<R, false>), (k_exmom<R, false, true>), TILE_EXMOM_Y, L, c->km - 1, D, twodt1); break;
+        case TILE_EXMOM_X: LAUNCH_EXMOM("", true); break;
+        case TILE_EXMOM_Y: LAUNCH_EXMOM("", false); break;
         case TILE_SLAYER_H: LAUNCHC("", (k_slayer_h<R, false>), (k_slayer_h<R, true>), TILE_SLAYER_H, L, c->km1, D); break;
         case TILE_VEL: LAUNCHC("", (k_vel<R, false>), (k_vel<R, true>), TILE_VEL, L, c->km - 1, D, tz); break;
-        case TILE_HFLUX: LAUNCHC("", (k_exsal_hflux<R, false>), (k_exsal_hflux<R, true>), TILE_HFLUX, L, c->km - 1, D, sal_of(D), twodt1); break;
-        case TILE_RHS: LAUNCHC("", (k_exsal_rhs<R, false, false>), (k_exsal_rhs<R, true, false>), TILE_RHS, L, c->km - 1, D, sal_of(D), twodt1); break;
+        case TILE_HFLUX:
+          if (use_window(c))
+            LAUNCHC("", (k_exsal_hflux_w<R>), (k_exsal_hflux_w<R>), TILE_HFLUX, L, c->km - 1, D, sal_of(D), twodt1);
+          else
+            LAUNCHC("", (k_exsal_hflux<R, false>), (k_exsal_hflux<R, true>), TILE_HFLUX, L, c->km - 1, D, sal_of(D),
+                    twodt1);
+          break;
+        case TILE_RHS:
+          if (use_window(c))
+            LAUNCHC("", (k_exsal_rhs_w<R, false>), (k_exsal_rhs_w<R, false>), TILE_RHS, L, c->km - 1, D, sal_of(D),
+                    twodt1);
+          else
+            LAUNCHC("", (k_exsal_rhs<R, false, false>), (k_exsal_rhs<R, true, false>), TILE_RHS, L, c->km - 1, D,
+                    sal_of(D), twodt1);
+          break;
         case TILE_TURB: LAUNCHC("", (k_turb_cell<R, false>), (k_turb_cell<R, true>), TILE_TURB, L, c->km1, D); break;
         default: LAUNCHC("", (k_vel2<R, false>), (k_vel2<R, true>), TILE_VEL2, L, c->km - 1, D); break;
       }
@@ -2068,10 +2365,10 @@ int fd(s3g_ctx *c, int n, int istep, int lastiter) {
   const int L = c->lm;
   int e = run_segment(c, SEG_FD_PRE, istep, 0, [&] {
     if (c->P.ihd > 1) LAUNCH("hmix", (k_hmix<R>), L, D);
-    LAUNCHC("exmom_x", (k_exmom1<R, true>), (k_exmom<R, true, true>), TILE_EXMOM_X, L, c->km - 1, D, twodt1);
+    LAUNCH_EXMOM("exmom_x", true);
     obc_exmom<R>(c, true, twodt1);
     LAUNCH("matmom_x", (k_matmom<R, true>), L, D, tz);
-    LAUNCHC("exmom_y", (k_exmom1<R, false>), (k_exmom<R, false, true>), TILE_EXMOM_Y, L, c->km - 1, D, twodt1);
+    LAUNCH_EXMOM("exmom_y", false);
     obc_exmom<R>(c, false, twodt1);
     LAUNCH("matmom_y", (k_matmom<R, false>), L, D, tz);
     LAUNCH("matcon", (k_matcon<R>), L, D, tz);
