@@ -317,7 +317,7 @@ void surfbc(s3g_ctx *c, int n, int istep) {
 
 // ---------------------------------------------------------------------------
 // Field table: name -> device storage.
-enum Kind { K3R, K3D, K2D, KI };
+enum Kind { K3R, K3T, K2D, KI };  // K3T: turbulence state (storage type, held only with iturb > 0)
 struct Field {
   const char *name;
   Kind kind;
@@ -335,9 +335,9 @@ static std::vector<Field> field_table(s3g_ctx *c, Dev<R> &D) {
       {"vpp", K3R, D.vpp},     {"wp", K3R, D.wp},       {"sal", K3R, D.sal},     {"salp", K3R, D.salp},
       {"salpp", K3R, D.salpp}, {"rhop", K3R, D.rhop},   {"Av", K3R, D.Av},       {"Dv", K3R, D.Dv},
       {"kh", K3R, D.kh},       {"ex", K3R, D.ex},       {"ey", K3R, D.ey},       {"agx", K3R, D.agx},     {"arx", K3R, D.arx},
-      {"agy", K3R, D.agy},     {"ary", K3R, D.ary},     {"bex", K3R, D.bex},     {"q2", K3D, D.q2},
-      {"q2p", K3D, D.q2p},     {"q2pp", K3D, D.q2pp},   {"q2l", K3D, D.q2l},     {"q2lp", K3D, D.q2lp},
-      {"q2lpp", K3D, D.q2lpp}, {"s", K2D, D.s},         {"sp", K2D, D.sp},       {"spp", K2D, D.spp},
+      {"agy", K3R, D.agy},     {"ary", K3R, D.ary},     {"bex", K3R, D.bex},     {"q2", K3T, D.q2},
+      {"q2p", K3T, D.q2p},     {"q2pp", K3T, D.q2pp},   {"q2l", K3T, D.q2l},     {"q2lp", K3T, D.q2lp},
+      {"q2lpp", K3T, D.q2lpp}, {"s", K2D, D.s},         {"sp", K2D, D.sp},       {"spp", K2D, D.spp},
       {"sx", K2D, D.sx},       {"sy", K2D, D.sy},       {"dd", K2D, D.dd},       {"qq", K2D, D.qq},
       {"rr", K2D, D.rr},       {"eagx", K2D, D.eagx},   {"earx", K2D, D.earx},   {"eagy", K2D, D.eagy},
       {"eary", K2D, D.eary},   {"hhs", K2D, (void *)D.hhs}, {"hhu", K2D, (void *)D.hhu},
@@ -386,7 +386,7 @@ static int download3(s3g_ctx *c, HT *dst, const DT *src) {
 // iturb = 0, Smagorinsky coefficients with ihd <= 1) cannot be accessed.
 static bool allocated(const s3g_ctx *c, const Field &f) {
   if (!f.ptr) return false;
-  if (f.kind == K3D) return c->P.iturb > 0;
+  if (f.kind == K3T) return c->P.iturb > 0;
   return true;
 }
 
@@ -404,9 +404,9 @@ static int set_field(s3g_ctx *c, const char *name, const HT *src) {
   const int lm1 = c->lm + 1;
   switch (f.kind) {
     case K3R:
+    case K3T:
       if (c->precision == S3G_FP64) return upload3(c, (double *)f.ptr, src);
       return upload3(c, (float *)f.ptr, src);
-    case K3D: return upload3(c, (double *)f.ptr, src);
     case K2D: {
       std::vector<double> b(c->LP, 0.0);
       for (int l = 0; l < lm1; ++l) b[l] = (double)src[l];
@@ -436,9 +436,9 @@ static int get_field(s3g_ctx *c, const char *name, HT *dst) {
   const int lm1 = c->lm + 1;
   switch (f.kind) {
     case K3R:
+    case K3T:
       if (c->precision == S3G_FP64) return download3(c, dst, (const double *)f.ptr);
       return download3(c, dst, (const float *)f.ptr);
-    case K3D: return download3(c, dst, (const double *)f.ptr);
     case K2D: {
       std::vector<double> b(c->LP);
       if (int e = hip_check(hipMemcpy(b.data(), f.ptr, c->LP * sizeof(double), hipMemcpyDeviceToHost), "get2")) return e;
@@ -545,10 +545,10 @@ static int alloc_state(s3g_ctx *c, Dev<R> &D, const int *lE, const int *lW, cons
     if (!(D.tr = dalloc<R>(c, (size_t)c->ntr * n3)) || !(D.trpp = dalloc<R>(c, (size_t)c->ntr * n3)))
       return S3G_ERR_HIP;
   }
-  double **q3[] = {&D.q2, &D.q2p, &D.q2pp, &D.q2l, &D.q2lp, &D.q2lpp, &D.d0, &D.d1, &D.d2};
-  for (double **p : q3) {  // d0, d1 also hold the fused turbulence sweep's factor pairs: 2 n3
+  R **q3[] = {&D.q2, &D.q2p, &D.q2pp, &D.q2l, &D.q2lp, &D.q2lpp, &D.d0, &D.d1, &D.d2};
+  for (R **p : q3) {  // d0, d1 also hold the fused turbulence sweep's factor pairs: 2 n3
     const size_t n = (c->P.iturb > 0 ? n3 : 1) * ((p == &D.d0 || p == &D.d1) ? 2 : 1);
-    if (!(*p = dalloc<double>(c, n))) return S3G_ERR_HIP;
+    if (!(*p = dalloc<R>(c, n))) return S3G_ERR_HIP;
   }
   D.tkb = nullptr;
   D.tke_part = D.tke = nullptr;

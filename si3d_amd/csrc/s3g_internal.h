@@ -58,9 +58,9 @@ struct MetArgs {
 };
 
 // Everything a kernel needs, passed by value.  R is the storage type of the
-// 3-D hydrodynamic fields (double, or float in the mixed-precision mode).  The
-// free surface, the 2-D solver arrays and the turbulence quantities are FP64
-// in every mode.
+// 3-D fields, the turbulence state included (double, or float in the
+// mixed-precision mode).  The free surface and the 2-D solver arrays are FP64
+// in every mode; kernels compute in FP64 throughout.
 template <typename R>
 struct Dev {
   int lm, LP, km1, km, k1, KP;  // KP: planes per 3-D field
@@ -89,7 +89,10 @@ struct Dev {
   R *tr, *trpp;  // ntr tracers, each a full plane stack, one after the other
   int ntr;
   R *heat;  // HeatSource (surface heat budget, ifsurfbc 1-3), null otherwise
-  double *q2, *q2p, *q2pp, *q2l, *q2lp, *q2lpp, *d0, *d1, *d2;
+  // turbulence state (q2, q2l and their time levels; REAL*8 in the reference)
+  // and the closure's scratch planes, in the storage type like the other
+  // column systems; d0, d1 hold 2 planes per level (the sweep's factor pairs)
+  R *q2, *q2p, *q2pp, *q2l, *q2lp, *q2lpp, *d0, *d1, *d2;
   // basin energy budget of the closure (iturb >= 1, one lake): per-level
   // (dissipation rate x rho h, rho h) of the running call, in single
   // precision like the reference's rhoxh, per-workgroup partial sums, and the

@@ -7,6 +7,7 @@
 // registers; the forward-sweep factors go to a per-column scratch plane stack
 // in the same coalesced layout and are read back by the same lane.
 #include <algorithm>
+#include <type_traits>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -325,6 +326,59 @@ __device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, T v, unsigned voff
                                           voff, soff, 0);
   else
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+}
+
+// Memory access of the column kernels at (layer k, column c), where each lane
+// has its own k: through plain pointers (PtrIO, any field size), or through
+// buffer resources with one 32-bit byte offset per access (BufIO, fields below
+// 4 GiB), so that an address takes one VGPR instead of a 64-bit pair, and the
+// compiler cannot turn each field's address into a 64-bit pointer stepped down
+// the column.
+struct PtrIO {
+  int LP;
+  template <typename T>
+  __device__ __forceinline__ T ld(const T *a, int k, int c) const {
+    return a[(size_t)k * LP + c];
+  }
+  template <typename T>
+  __device__ __forceinline__ void st(T *a, int k, int c, T v) const {
+    a[(size_t)k * LP + c] = v;
+  }
+};
+struct BufIO {
+  unsigned LP, KP;
+  template <typename T>
+  __device__ __forceinline__ T ld(const T *a, int k, int c) const {
+    const auto r = rsrc(a, KP * LP * (unsigned)sizeof(T));
+    const unsigned off = ((unsigned)k * LP + (unsigned)c) * (unsigned)sizeof(T);
+    if constexpr (sizeof(T) == 4)
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+    else if constexpr (sizeof(T) == 8)
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    else
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+  }
+  template <typename T>
+  __device__ __forceinline__ void st(T *a, int k, int c, T v) const {
+    const auto r = rsrc(a, KP * LP * (unsigned)sizeof(T));
+    const unsigned off = ((unsigned)k * LP + (unsigned)c) * (unsigned)sizeof(T);
+    if constexpr (sizeof(T) == 4)
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+    else if constexpr (sizeof(T) == 8)
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((__vector_size__(8))) unsigned, v), r,
+                                            off, 0, 0);
+    else
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((__vector_size__(16))) unsigned, v), r,
+                                             off, 0, 0);
+  }
+};
+
+template <typename IO, typename R>
+__device__ __forceinline__ IO make_io(const Dev<R> &D) {
+  if constexpr (std::is_same<IO, BufIO>::value)
+    return BufIO{(unsigned)D.LP, (unsigned)D.KP};
+  else
+    return PtrIO{D.LP};
 }
 
 // exmom for the centred scheme (itrmom 1) with constant-coefficient or no
@@ -1348,14 +1402,28 @@ __global__ void __launch_bounds__(kBlock) k_hmix(Dev<R> D) {
 // The p-level values of level k of column m that the interface terms of
 // levels k and k+1 share: hp, rhop, up at m and its west face, vp at m and
 // its south face.  The column sweep loads each level once and carries it.
+// A pair of values of the storage type (one 16- or 8-byte access).
+template <typename R>
+struct Pair;
+template <>
+struct Pair<double> {
+  using T = double2;
+  static __device__ __forceinline__ T make(double a, double b) { return make_double2(a, b); }
+};
+template <>
+struct Pair<float> {
+  using T = float2;
+  static __device__ __forceinline__ T make(double a, double b) { return make_float2((float)a, (float)b); }
+};
+
 struct TurbRaw {
   double h, rho, um, uw, vm, vs;
 };
 
-template <typename R>
-__device__ __forceinline__ TurbRaw turb_raw(const Dev<R> &D, int m, int k, int lW, int lS) {
-  return TurbRaw{(double)D.hp[IX(k, m)], (double)D.rhop[IX(k, m)], (double)D.up[IX(k, m)],
-                 (double)D.up[IX(k, lW)], (double)D.vp[IX(k, m)],  (double)D.vp[IX(k, lS)]};
+template <typename R, typename IO>
+__device__ __forceinline__ TurbRaw turb_raw(const Dev<R> &D, const IO &io, int m, int k, int lW, int lS) {
+  return TurbRaw{(double)io.ld(D.hp, k, m), (double)io.ld(D.rhop, k, m), (double)io.ld(D.up, k, m),
+                 (double)io.ld(D.up, k, lW), (double)io.ld(D.vp, k, m),  (double)io.ld(D.vp, k, lS)};
 }
 
 // Interface quantities of level k (k1s < k <= kms) of column m from the
@@ -1367,8 +1435,9 @@ struct TurbLev {
   double xl, sp, bp, av, dv, q2p, q2lp;
 };
 
-template <typename R>
-__device__ __forceinline__ TurbLev turb_level(const Dev<R> &D, int m, int k, const TurbRaw &a, const TurbRaw &b) {
+template <typename R, typename IO>
+__device__ __forceinline__ TurbLev turb_level(const Dev<R> &D, const IO &io, int m, int k, const TurbRaw &a,
+                                             const TurbRaw &b) {
   TurbLev L;
   const double hkm = a.h, hk = b.h;
   const double dz = 0.5 * (hkm + hk);
@@ -1376,14 +1445,14 @@ __device__ __forceinline__ TurbLev turb_level(const Dev<R> &D, int m, int k, con
   const double N2 = -G / (0.5 * (rkm + rk) + 1000.) * ((rkm - rk) / dz);
   const double dudz = 0.5 * (a.um + a.uw - b.um - b.uw) / dz;
   const double dvdz = 0.5 * (a.vm + a.vs - b.vm - b.vs) / dz;
-  const double q2pk = D.q2p[IX(k, m)], q2lpk = D.q2lp[IX(k, m)];
+  const double q2pk = io.ld(D.q2p, k, m), q2lpk = io.ld(D.q2lp, k, m);
   L.q2p = q2pk;
   L.q2lp = q2lpk;
   double xl = q2lpk / q2pk;
   if (N2 > 0.0) xl = dmin(xl, 0.53 * sqrt(q2pk / N2));
   L.xl = xl;
-  L.sp = (double)D.Av[IX(k, m)] * (dudz * dudz + dvdz * dvdz);
-  L.bp = -(double)D.Dv[IX(k, m)] * N2;
+  L.sp = (double)io.ld(D.Av, k, m) * (dudz * dudz + dvdz * dvdz);
+  L.bp = -(double)io.ld(D.Dv, k, m) * N2;
   // eddy coefficients (UpdateMixingCoefficients evaluates N^2 in this order)
   const double N2b = -G * ((rkm - rk) / dz) / (0.5 * (rkm + rk) + 1000.);
   double xlb = q2lpk / q2pk;
@@ -1410,19 +1479,20 @@ __global__ void __launch_bounds__(kBlock) k_turb_cell(Dev<R> D, int nk, int tile
   const int kms = D.kmz[m], k1s = D.k1z[m];
   if (kms - k1s + 1 <= 1) return;
   const int lW = D.lW[m], lS = D.lS[m];
+  const PtrIO io = make_io<PtrIO>(D);
   for (int k = t.k0; k < (CH ? t.k1 : min(t.k1, t.k0 + 1)); ++k) {
     if (k <= k1s || k > kms) {
-      D.Av[IX(k, m)] = D.Dv[IX(k, m)] = R(0);
+      io.st(D.Av, k, m, R(0)), io.st(D.Dv, k, m, R(0));
       continue;
     }
-    const TurbLev L = turb_level(D, m, k, turb_raw(D, m, k - 1, lW, lS), turb_raw(D, m, k, lW, lS));
-    D.d0[IX(k, m)] = L.xl;
-    D.d1[IX(k, m)] = L.sp;
-    D.d2[IX(k, m)] = L.bp;
+    const TurbLev L = turb_level(D, io, m, k, turb_raw(D, io, m, k - 1, lW, lS), turb_raw(D, io, m, k, lW, lS));
+    io.st(D.d0, k, m, (R)L.xl);
+    io.st(D.d1, k, m, (R)L.sp);
+    io.st(D.d2, k, m, (R)L.bp);
     double av = L.av, dv = L.dv;
-    if (k == k1s + 1 && (double)D.hp[IX(k1s, m)] < 1.E-2) av = dmax((double)(R)av, 1.E-4), dv = dmax((double)(R)dv, 1.E-4);
-    D.Av[IX(k, m)] = (R)av;
-    D.Dv[IX(k, m)] = (R)dv;
+    if (k == k1s + 1 && (double)io.ld(D.hp, k1s, m) < 1.E-2) av = dmax((double)(R)av, 1.E-4), dv = dmax((double)(R)dv, 1.E-4);
+    io.st(D.Av, k, m, (R)av);
+    io.st(D.Dv, k, m, (R)dv);
   }
 }
 
@@ -1444,8 +1514,8 @@ struct TkeSums {
   double sh, by, di, tk;
 };
 
-template <typename R, bool SPLIT>
-__device__ __forceinline__ TkeSums turb_column(const Dev<R> &D, int m, int istep, bool budget) {
+template <typename R, bool SPLIT, typename IO>
+__device__ __forceinline__ TkeSums turb_column(const Dev<R> &D, const IO &io, int m, int istep, bool budget) {
   TkeSums bs{0.0, 0.0, 0.0, 0.0};
   const int kms = D.kmz[m], k1s = D.k1z[m], nwl = kms - k1s + 1;
   if (nwl <= 1) return bs;
@@ -1454,74 +1524,75 @@ __device__ __forceinline__ TkeSums turb_column(const Dev<R> &D, int m, int istep
   // forward-sweep factors of the two systems: split, E1/E2 in d0/d1 and the
   // right-hand sides in q2/q2l; fused, (e1, e2) and (f1, f2) as pairs in d0 and
   // d1 (2 n3 doubles each), one 16-byte access per level and direction
-  double *E1 = D.d0, *E2 = D.d1;
-  double2 *EP = reinterpret_cast<double2 *>(D.d0), *FP = reinterpret_cast<double2 *>(D.d1);
+  R *E1 = D.d0, *E2 = D.d1;
+  using R2 = typename Pair<R>::T;
+  R2 *EP = reinterpret_cast<R2 *>(D.d0), *FP = reinterpret_cast<R2 *>(D.d1);
   double hbelow = 0.0;
-  for (int k = k1s + 1; k <= kms; ++k) hbelow += D.hp[IX(k, m)];
+  for (int k = k1s + 1; k <= kms; ++k) hbelow += io.ld(D.hp, k, m);
   // interface quantities of level k: read back (split) or computed from the
   // carried raw values of level k-1 (a) and the new ones of level k (b)
   auto level = [&](int k, const TurbRaw &a, const TurbRaw &b) -> TurbLev {
     if (SPLIT) {
       TurbLev L{};
-      L.xl = D.d0[IX(k, m)];
-      L.sp = D.d1[IX(k, m)];
-      L.bp = D.d2[IX(k, m)];
-      L.q2p = D.q2p[IX(k, m)];
-      L.q2lp = D.q2lp[IX(k, m)];
+      L.xl = io.ld(D.d0, k, m);
+      L.sp = io.ld(D.d1, k, m);
+      L.bp = io.ld(D.d2, k, m);
+      L.q2p = io.ld(D.q2p, k, m);
+      L.q2lp = io.ld(D.q2lp, k, m);
       return L;
     }
-    return turb_level(D, m, k, a, b);
+    return turb_level(D, io, m, k, a, b);
   };
   // boundary values (445-457)
   const double b1_23 = pow(B_1, 2. / 3.);
-  double ub = 0.5 * ((double)D.u[IX(kms, m)] + (double)D.u[IX(kms, lW)]);
-  double vb = 0.5 * ((double)D.v[IX(kms, m)] + (double)D.v[IX(kms, lS)]);
+  double ub = 0.5 * ((double)io.ld(D.u, kms, m) + (double)io.ld(D.u, kms, lW));
+  double vb = 0.5 * ((double)io.ld(D.v, kms, m) + (double)io.ld(D.v, kms, lS));
   const double ds_bot = 3. * b1_23 * (D.P.cd * (ub * ub + vb * vb)) / twodt1;
   const Wind wd = wind(D, m);
-  double us = 0.5 * ((double)D.up[IX(D.k1u[m], m)] + (double)D.up[IX(D.k1u[lW], lW)]);
-  double vs = 0.5 * ((double)D.vp[IX(D.k1v[m], m)] + (double)D.vp[IX(D.k1v[lS], lS)]);
+  double us = 0.5 * ((double)io.ld(D.up, D.k1u[m], m) + (double)io.ld(D.up, D.k1u[lW], lW));
+  double vs = 0.5 * ((double)io.ld(D.vp, D.k1v[m], m) + (double)io.ld(D.vp, D.k1v[lS], lS));
   const double ds_top = 3. * b1_23 *
                         (wd.cdw * RHOAIR * ((wd.uair - us) * (wd.uair - us) + (wd.vair - vs) * (wd.vair - vs)) / 1000.) / twodt1;
-  const bool thin_top = (double)D.hp[IX(k1s, m)] < 1.E-2;
+  const bool thin_top = (double)io.ld(D.hp, k1s, m) < 1.E-2;
   // One forward sweep over rows k1s..kms of both systems (row k1s and row
   // kms+1 are the Dirichlet rows: a1 = a3 = 0, a2 = 3/twodt1), with the
   // interface quantities of level k+1 computed one level ahead.
   double e1 = 0.0, f1 = ds_top / r3, e2 = 0.0, f2 = 0.0;  // row k1s
   if (SPLIT) {
-    D.q2[IX(k1s, m)] = f1;
-    D.q2l[IX(k1s, m)] = f2;
-    E1[IX(k1s, m)] = 0.0;
-    E2[IX(k1s, m)] = 0.0;
+    io.st(D.q2, k1s, m, (R)f1);
+    io.st(D.q2l, k1s, m, (R)f2);
+    io.st(E1, k1s, m, R(0));
+    io.st(E2, k1s, m, R(0));
   } else {
-    EP[IX(k1s, m)] = make_double2(0.0, 0.0);
-    FP[IX(k1s, m)] = make_double2(f1, f2);
+    io.st(EP, k1s, m, Pair<R>::make(0.0, 0.0));
+    io.st(FP, k1s, m, Pair<R>::make(f1, f2));
   }
   TurbRaw rcur{};  // raw values of level k (unused when split)
-  if (!SPLIT) rcur = turb_raw(D, m, k1s + 1, lW, lS);
-  TurbLev cur = level(k1s + 1, SPLIT ? rcur : turb_raw(D, m, k1s, lW, lS), rcur);
-  double dco_prev = SQ * 0.5 * (0.0 + cur.xl) * sqrt(0.5 * (D.q2p[IX(k1s, m)] + cur.q2p));
+  if (!SPLIT) rcur = turb_raw(D, io, m, k1s + 1, lW, lS);
+  TurbLev cur = level(k1s + 1, SPLIT ? rcur : turb_raw(D, io, m, k1s, lW, lS), rcur);
+  double dco_prev = SQ * 0.5 * (0.0 + cur.xl) * sqrt(0.5 * (io.ld(D.q2p, k1s, m) + cur.q2p));
   double zfromt = 0.0;
-  double hkm = D.hp[IX(k1s, m)];
+  double hkm = io.ld(D.hp, k1s, m);
   for (int k = k1s + 1; k <= kms; ++k) {
     const bool more = k + 1 <= kms;
     TurbRaw rnxt{};
     TurbLev nxt{};
     if (more) {
-      if (!SPLIT) rnxt = turb_raw(D, m, k + 1, lW, lS);
+      if (!SPLIT) rnxt = turb_raw(D, io, m, k + 1, lW, lS);
       nxt = level(k + 1, rcur, rnxt);
     }
     const double xlk = cur.xl, xlp = more ? nxt.xl : 0.0;
     const double q2pk = cur.q2p;
-    const double q2p_next = more ? nxt.q2p : D.q2p[IX(k + 1, m)];
+    const double q2p_next = more ? nxt.q2p : io.ld(D.q2p, k + 1, m);
     const double dco = SQ * 0.5 * (xlk + xlp) * sqrt(0.5 * (q2pk + q2p_next));
-    const double hk = SPLIT ? (double)D.hp[IX(k, m)] : rcur.h;
+    const double hk = SPLIT ? (double)io.ld(D.hp, k, m) : rcur.h;
     const double a3 = -dco / (hk * 0.5 * (hk + hkm));
     const double a1 = -dco_prev / (hkm * 0.5 * (hk + hkm));
     const double shp = cur.sp, byp = cur.bp;
     const double sq = sqrt(q2pk);
     // TKE row
     double a2 = r3 - a1 - a3 + 2. * (sq / (B_1 * xlk));
-    double ds = 4. * q2pk / twodt1 - D.q2pp[IX(k, m)] / twodt1;
+    double ds = 4. * q2pk / twodt1 - io.ld(D.q2pp, k, m) / twodt1;
     if (byp >= 0.0) {
       ds += 2. * (shp + byp);
     } else {
@@ -1550,7 +1621,7 @@ __device__ __forceinline__ TkeSums turb_column(const Dev<R> &D, int m, int istep
         a2l -= 5.06 * byp / q2pk;
       }
     }
-    dsl += 4 * cur.q2lp / twodt1 - D.q2lpp[IX(k, m)] / twodt1;
+    dsl += 4 * cur.q2lp / twodt1 - io.ld(D.q2lpp, k, m) / twodt1;
     // forward sweeps
     double den = a2 + a1 * e1;
     e1 = -a3 / den;
@@ -1559,26 +1630,26 @@ __device__ __forceinline__ TkeSums turb_column(const Dev<R> &D, int m, int istep
     e2 = -a3 / den;
     f2 = (dsl - a1 * f2) / den;
     if (SPLIT) {
-      E1[IX(k, m)] = e1;
-      E2[IX(k, m)] = e2;
-      D.q2[IX(k, m)] = f1;
-      D.q2l[IX(k, m)] = f2;
+      io.st(E1, k, m, (R)e1);
+      io.st(E2, k, m, (R)e2);
+      io.st(D.q2, k, m, (R)f1);
+      io.st(D.q2l, k, m, (R)f2);
     } else {
-      EP[IX(k, m)] = make_double2(e1, e2);
-      FP[IX(k, m)] = make_double2(f1, f2);
+      io.st(EP, k, m, Pair<R>::make(e1, e2));
+      io.st(FP, k, m, Pair<R>::make(f1, f2));
     }
     if (budget) {
-      const double rhoxh = ((SPLIT ? (double)D.rhop[IX(k, m)] : rcur.rho) + 1000.) * hk;
+      const double rhoxh = ((SPLIT ? (double)io.ld(D.rhop, k, m) : rcur.rho) + 1000.) * hk;
       bs.sh += shp * rhoxh;
       bs.by += byp * rhoxh;
-      D.tkb[IX(k, m)] = make_float2((float)(sq / (B_1 * xlk) * rhoxh), (float)rhoxh);
+      io.st(D.tkb, k, m, make_float2((float)(sq / (B_1 * xlk) * rhoxh), (float)rhoxh));
     }
     // new eddy coefficients of interface k (the surface minimum, 281-284)
     if (!SPLIT) {
       double av = cur.av, dv = cur.dv;
       if (k == k1s + 1 && thin_top) av = dmax((double)(R)av, 1.E-4), dv = dmax((double)(R)dv, 1.E-4);
-      D.Av[IX(k, m)] = (R)av;
-      D.Dv[IX(k, m)] = (R)dv;
+      io.st(D.Av, k, m, (R)av);
+      io.st(D.Dv, k, m, (R)dv);
     }
     dco_prev = dco;
     hkm = hk;
@@ -1587,33 +1658,33 @@ __device__ __forceinline__ TkeSums turb_column(const Dev<R> &D, int m, int istep
   }
   // bottom Dirichlet row kms+1 (a1 = 0) and back substitution
   double x1 = ds_bot / r3, x2 = 0.0;
-  D.q2[IX(kms + 1, m)] = dmax(Q2_MIN, x1);
-  D.q2l[IX(kms + 1, m)] = dmax(Q2L_MIN, x2);
+  io.st(D.q2, kms + 1, m, (R)dmax(Q2_MIN, x1));
+  io.st(D.q2l, kms + 1, m, (R)dmax(Q2L_MIN, x2));
   for (int k = kms; k >= k1s; --k) {
     if (SPLIT) {
-      x1 = E1[IX(k, m)] * x1 + D.q2[IX(k, m)];
-      x2 = E2[IX(k, m)] * x2 + D.q2l[IX(k, m)];
+      x1 = (double)io.ld(E1, k, m) * x1 + (double)io.ld(D.q2, k, m);
+      x2 = (double)io.ld(E2, k, m) * x2 + (double)io.ld(D.q2l, k, m);
     } else {
-      const double2 e = EP[IX(k, m)], f = FP[IX(k, m)];
-      x1 = e.x * x1 + f.x;
-      x2 = e.y * x2 + f.y;
+      const R2 e = io.ld(EP, k, m), f = io.ld(FP, k, m);
+      x1 = (double)e.x * x1 + (double)f.x;
+      x2 = (double)e.y * x2 + (double)f.y;
     }
-    const double q2n = dmax(Q2_MIN, x1);
-    D.q2[IX(k, m)] = q2n;
-    D.q2l[IX(k, m)] = dmax(Q2L_MIN, x2);
+    const double q2n = (double)(R)dmax(Q2_MIN, x1);
+    io.st(D.q2, k, m, (R)q2n);
+    io.st(D.q2l, k, m, (R)dmax(Q2L_MIN, x2));
     if (budget && k > k1s) {
-      const float2 w = D.tkb[IX(k, m)];
+      const float2 w = io.ld(D.tkb, k, m);
       bs.di += q2n * (double)w.x;
       bs.tk += q2n * (double)w.y;
     }
   }
   for (int k = D.k1; k <= k1s - 1; ++k) {
-    D.q2[IX(k, m)] = D.q2[IX(k1s, m)];
-    D.q2l[IX(k, m)] = D.q2l[IX(k1s, m)];
+    io.st(D.q2, k, m, io.ld(D.q2, k1s, m));
+    io.st(D.q2l, k, m, io.ld(D.q2l, k1s, m));
   }
   if (!SPLIT) {
-    for (int k = 1; k <= k1s; ++k) D.Av[IX(k, m)] = D.Dv[IX(k, m)] = R(0);
-    for (int k = kms + 1; k <= D.km1; ++k) D.Av[IX(k, m)] = D.Dv[IX(k, m)] = R(0);
+    for (int k = 1; k <= k1s; ++k) io.st(D.Av, k, m, R(0)), io.st(D.Dv, k, m, R(0));
+    for (int k = kms + 1; k <= D.km1; ++k) io.st(D.Av, k, m, R(0)), io.st(D.Dv, k, m, R(0));
   }
   return bs;
 }
@@ -1635,12 +1706,17 @@ __device__ __forceinline__ double wg_sum(double v, double *lds) {
 // One lane per column (every lane reaches the workgroup reduction, so the
 // columns without work contribute zeros); with `budget` each workgroup
 // writes its four sums to tke_part[4 blockIdx.x ..].
-template <typename R, bool SPLIT>
+template <typename R, bool SPLIT, bool BUF>
 __global__ void __launch_bounds__(kBlock) k_turb(Dev<R> D, int istep, int budget) {
   __shared__ double lds[kBlock / 64];
   const int m = col();
   TkeSums bs{0.0, 0.0, 0.0, 0.0};
-  if (m < D.lm) bs = turb_column<R, SPLIT>(D, m, istep, budget != 0);
+  if (m < D.lm) {
+    if constexpr (BUF)
+      bs = turb_column<R, SPLIT>(D, make_io<BufIO>(D), m, istep, budget != 0);
+    else
+      bs = turb_column<R, SPLIT>(D, make_io<PtrIO>(D), m, istep, budget != 0);
+  }
   if (budget) {
     const double v[4] = {bs.sh, bs.by, bs.di, bs.tk};
     for (int i = 0; i < 4; ++i) {
@@ -1833,8 +1909,8 @@ __global__ void __launch_bounds__(kBlock) k_settrap_cell(Dev<R> D, int nk, int t
         D.q2pp[IX(k, c)] = D.q2p[IX(k, c)];
         D.q2lpp[IX(k, c)] = D.q2lp[IX(k, c)];
       }
-      D.q2p[IX(k, c)] = 0.5 * (D.q2[IX(k, c)] + D.q2pp[IX(k, c)]);
-      D.q2lp[IX(k, c)] = 0.5 * (D.q2l[IX(k, c)] + D.q2lpp[IX(k, c)]);
+      D.q2p[IX(k, c)] = (R)(0.5 * ((double)D.q2[IX(k, c)] + (double)D.q2pp[IX(k, c)]));
+      D.q2lp[IX(k, c)] = (R)(0.5 * ((double)D.q2l[IX(k, c)] + (double)D.q2lpp[IX(k, c)]));
     }
   }
 }
@@ -2092,9 +2168,11 @@ static void turbulence(s3g_ctx *c, int istep) {
   const int budget = (track && istep > 1) ? 1 : 0;
   if (c->turb_split) {
     LAUNCHC("turb_cell", (k_turb_cell<R, false>), (k_turb_cell<R, true>), TILE_TURB, L, c->km1, D);
-    LAUNCH("turbulence", (k_turb<R, true>), L, D, istep, budget);
+    LAUNCH("turbulence", (k_turb<R, true, false>), L, D, istep, budget);
+  } else if (use_window(c) && 2 * sizeof(R) * c->KP * (size_t)c->LP < ((size_t)1 << 32)) {
+    LAUNCH("turbulence", (k_turb<R, false, true>), L, D, istep, budget);  // factor pairs: 2 planes per level
   } else {
-    LAUNCH("turbulence", (k_turb<R, false>), L, D, istep, budget);
+    LAUNCH("turbulence", (k_turb<R, false, false>), L, D, istep, budget);
   }
   if (track) hipLaunchKernelGGL((k_tke_budget<R>), dim3(1), dim3(kBlock), 0, c->stream, D, (int)grid_for(L).x, istep);
 }
