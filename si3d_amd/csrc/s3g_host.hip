@@ -645,6 +645,9 @@ int s3g::init_ctx(const s3g_grid *g, const s3g_params *p, int nlakes, const doub
     if (const char *x = std::getenv("S3G_TURB")) c->turb_split = std::strcmp(x, "split") == 0;
     if (const char *x = std::getenv("S3G_BUDGET")) c->tke_budget = x[0] != '0';
     if (const char *x = std::getenv("S3G_WINDOW")) c->window = x[0] != '0';
+    // the windowed settrap / save kernels (not tuned: they rewrite their
+    // inputs) take 8 layers per lane
+    if (c->window) c->kc[s3g::TILE_SETTRAP] = c->kc[s3g::TILE_SAVE] = 8;
     // S3G_TILES="w0,w1,...": fixed tile words (chunk | order << 8), no tuning
     if (const char *w = std::getenv("S3G_TILES")) {
       c->tune = false;

@@ -1915,6 +1915,129 @@ __global__ void __launch_bounds__(kBlock) k_settrap_cell(Dev<R> D, int nk, int t
   }
 }
 
+// Windowed form of k_settrap_cell (same values, same order): the chunk's
+// layers are walked by the whole workgroup together, the fields are read and
+// written through buffer resources with one 32-bit offset per column, and the
+// column's indices, depths, neighbour elevations and zlevel pointer are loaded
+// once per chunk.  The thicknesses written in a layer are reused from
+// registers by the fluxes and the density of the same layer.
+template <typename R, int MODE>
+__global__ void __launch_bounds__(kBlock) k_settrap_cell_w(Dev<R> D, int nk, int tile) {
+  CellTile t;
+  if (!cell_tile(D.lm + (MODE == 0 ? 1 : 0), 0, nk, tile, t)) return;
+  const int c = t.c, S = D.lm;
+  const bool sent = c == S;
+  const unsigned sz = sizeof(R), nb = (unsigned)D.KP * (unsigned)D.LP * sz, pb = (unsigned)D.LP * sz;
+  const unsigned oc = (unsigned)c * sz;
+  const auto hp = rsrc(D.hp, nb), hup = rsrc(D.hup, nb), hvp = rsrc(D.hvp, nb);
+  const auto hpp = rsrc(D.hpp, nb), hupp = rsrc(D.hupp, nb), hvpp = rsrc(D.hvpp, nb);
+  const int kbot = sent ? D.km1 + 2 : D.kmz[c] + 1;  // below the sediment cell nothing ever changes
+  const int kend = min(t.k1, kbot + 1);
+  if (sent) {
+    if (MODE == 0)
+      for (unsigned k = t.k0, pk = k * pb; (int)k < kend; ++k, pk += pb) {
+        bst<R>(hpp, (R)bld<R>(hp, oc, pk), oc, pk);
+        bst<R>(hupp, (R)bld<R>(hup, oc, pk), oc, pk);
+        bst<R>(hvpp, (R)bld<R>(hvp, oc, pk), oc, pk);
+      }
+    return;
+  }
+  const int k1 = D.k1, kmz = D.kmz[c], lE = D.lE[c], lN = D.lN[c];
+  const int kmu = (lE != S) ? min(kmz, D.kmz[lE]) : -1, kmv = (lN != S) ? min(kmz, D.kmz[lN]) : -1;
+  const double spc = D.sp[c], hhs = D.hhs[c];
+  const double etau = (lE != S) ? dmax(spc, D.sp[lE]) : 0.0, etav = (lN != S) ? dmax(spc, D.sp[lN]) : 0.0;
+  const double hhu = D.hhu[c], hhv = D.hhv[c];
+  const double *zl = zlv(D, c);
+  const bool turb = D.P.iturb > 0;
+  const auto sal = rsrc(D.sal, nb), salp = rsrc(D.salp, nb), salpp = rsrc(D.salpp, nb), rhop = rsrc(D.rhop, nb);
+  const auto uh = rsrc(D.uh, nb), uhp = rsrc(D.uhp, nb), uhpp = rsrc(D.uhpp, nb), up = rsrc(D.up, nb),
+             upp = rsrc(D.upp, nb);
+  const auto vh = rsrc(D.vh, nb), vhp = rsrc(D.vhp, nb), vhpp = rsrc(D.vhpp, nb), vp = rsrc(D.vp, nb),
+             vpp = rsrc(D.vpp, nb);
+  // pass 1 of the n+1/2 fluxes of one face (half_level_flux_cell), with the
+  // face thickness hk of this layer
+  auto flux = [&](const auto &qh, const auto &qhp, const auto &qhpp, const auto &qp, const auto &qpp, double hk,
+                  unsigned pk) {
+    double fpp;  // the flux at pp, after settrap moved p into it
+    if (MODE == 0) {
+      fpp = bld<R>(qhp, oc, pk);
+      bst<R>(qhpp, (R)fpp, oc, pk);
+      bst<R>(qpp, (R)bld<R>(qp, oc, pk), oc, pk);
+    } else {
+      fpp = bld<R>(qhpp, oc, pk);
+    }
+    if (hk > 0.0) {
+      const double f = 0.5 * (bld<R>(qh, oc, pk) + fpp);
+      bst<R>(qhp, (R)f, oc, pk);
+      bst<R>(qp, (R)(f / hk), oc, pk);
+    } else {
+      bst<R>(qhp, R(0), oc, pk);
+      bst<R>(qp, R(0), oc, pk);
+    }
+  };
+  for (unsigned k = t.k0, pk = k * pb; (int)k < kend; ++k, pk += pb) {
+    const int kk = (int)k;
+    if (MODE == 0) {
+      bst<R>(hpp, (R)bld<R>(hp, oc, pk), oc, pk);
+      bst<R>(hupp, (R)bld<R>(hup, oc, pk), oc, pk);
+      bst<R>(hvpp, (R)bld<R>(hvp, oc, pk), oc, pk);
+    }
+    const bool wet = kk >= k1 && kk <= kmz;
+    R hpk = R(0);
+    if (wet) {
+      const double v = cell_h(zl, kk, hhs, spc);
+      hpk = (v <= 0.0) ? R(0) : (R)v;
+      bst<R>(hp, hpk, oc, pk);
+    }
+    const bool fu = kk >= k1 && kk <= kmu, fv = kk >= k1 && kk <= kmv;
+    double hu = 0.0, hv = 0.0;
+    if (fu) {
+      const double v = cell_h(zl, kk, hhu, etau);
+      const R h = (v <= 0.0) ? R(0) : (R)v;
+      bst<R>(hup, h, oc, pk);
+      hu = (double)h;
+    }
+    if (fv) {
+      const double v = cell_h(zl, kk, hhv, etav);
+      const R h = (v <= 0.0) ? R(0) : (R)v;
+      bst<R>(hvp, h, oc, pk);
+      hv = (double)h;
+    }
+    if (wet) {
+      double spp;
+      if (MODE == 0) {
+        spp = bld<R>(salp, oc, pk);
+        bst<R>(salpp, (R)spp, oc, pk);
+      } else {
+        spp = bld<R>(salpp, oc, pk);
+      }
+      const double tm = 0.5 * (bld<R>(sal, oc, pk) + spp);
+      bst<R>(salp, (R)tm, oc, pk);
+      const double zk = zl[kk];
+      const double zd = (zk == -100.0) ? 0.5 * (double)hpk : zk + 0.5 * (double)hpk;
+      bst<R>(rhop, (R)(densty((double)(R)tm, 0.00004, zd) - 1000.), oc, pk);
+    }
+    if (fu) flux(uh, uhp, uhpp, up, upp, hu, pk);
+    if (fv) flux(vh, vhp, vhpp, vp, vpp, hv, pk);
+    if (turb && wet) {
+      const auto q2 = rsrc(D.q2, nb), q2p = rsrc(D.q2p, nb), q2pp = rsrc(D.q2pp, nb);
+      const auto q2l = rsrc(D.q2l, nb), q2lp = rsrc(D.q2lp, nb), q2lpp = rsrc(D.q2lpp, nb);
+      double qpp, qlpp;
+      if (MODE == 0) {
+        qpp = bld<R>(q2p, oc, pk);
+        qlpp = bld<R>(q2lp, oc, pk);
+        bst<R>(q2pp, (R)qpp, oc, pk);
+        bst<R>(q2lpp, (R)qlpp, oc, pk);
+      } else {
+        qpp = bld<R>(q2pp, oc, pk);
+        qlpp = bld<R>(q2lpp, oc, pk);
+      }
+      bst<R>(q2p, (R)(0.5 * (bld<R>(q2, oc, pk) + qpp)), oc, pk);
+      bst<R>(q2lp, (R)(0.5 * (bld<R>(q2l, oc, pk) + qlpp)), oc, pk);
+    }
+  }
+}
+
 // settrap / settrap2 per column, after k_settrap_cell: top-layer indices, the
 // top-cell flux corrections and the turbulence fix of a dry top cell.
 template <typename R>
@@ -1988,6 +2111,81 @@ __global__ void __launch_bounds__(kBlock) k_save_cell(Dev<R> D, int nk, int tile
     } else {
       D.vhp[IX(k, c)] = D.vp[IX(k, c)] = D.v[IX(k, c)] = R(0);
     }
+  }
+}
+
+// Windowed form of k_save_cell (same values, same order; see k_settrap_cell_w).
+template <typename R, int ISTEP>
+__global__ void __launch_bounds__(kBlock) k_save_cell_w(Dev<R> D, int nk, int tile) {
+  CellTile t;
+  if (!cell_tile(D.lm + 1, 0, nk, tile, t)) return;
+  const int c = t.c, S = D.lm;
+  const bool sent = c == S;
+  const unsigned sz = sizeof(R), nb = (unsigned)D.KP * (unsigned)D.LP * sz, pb = (unsigned)D.LP * sz;
+  const unsigned oc = (unsigned)c * sz;
+  // below a column's sediment cell every array holds its initial value
+  const int kbot = sent ? D.km1 + 2 : D.kmz[c] + 1;
+  const int kend = min(t.k1, kbot + 1), kwet = sent ? -1 : km1c(D, c) - 1, k1 = D.k1;
+  const double *zl = sent ? nullptr : zlv(D, c);
+  const bool turb = D.P.iturb > 0;
+  const auto h = rsrc(D.h, nb), hu = rsrc(D.hu, nb), hv = rsrc(D.hv, nb);
+  const auto hp = rsrc(D.hp, nb), hup = rsrc(D.hup, nb), hvp = rsrc(D.hvp, nb);
+  const auto hpp = rsrc(D.hpp, nb), hupp = rsrc(D.hupp, nb), hvpp = rsrc(D.hvpp, nb);
+  const auto sal = rsrc(D.sal, nb), salp = rsrc(D.salp, nb), salpp = rsrc(D.salpp, nb), rhop = rsrc(D.rhop, nb);
+  const auto uh = rsrc(D.uh, nb), uhp = rsrc(D.uhp, nb), uhpp = rsrc(D.uhpp, nb), u = rsrc(D.u, nb),
+             up = rsrc(D.up, nb), upp = rsrc(D.upp, nb);
+  const auto vh = rsrc(D.vh, nb), vhp = rsrc(D.vhp, nb), vhpp = rsrc(D.vhpp, nb), v = rsrc(D.v, nb),
+             vp = rsrc(D.vp, nb), vpp = rsrc(D.vpp, nb);
+  // the new face flux and velocity from the transport and the new thickness
+  auto face = [&](const auto &qh, const auto &qhp, const auto &q, const auto &qp, double hq, unsigned pk) {
+    if (hq > 0.0) {
+      const R f = (R)bld<R>(qh, oc, pk);
+      bst<R>(qhp, f, oc, pk);
+      const R w = (R)((double)f / hq);
+      bst<R>(qp, w, oc, pk);
+      bst<R>(q, w, oc, pk);
+    } else {
+      bst<R>(qhp, R(0), oc, pk);
+      bst<R>(qp, R(0), oc, pk);
+      bst<R>(q, R(0), oc, pk);
+    }
+  };
+  for (unsigned k = t.k0, pk = k * pb; (int)k < kend; ++k, pk += pb) {
+    const int kk = (int)k;
+    if (ISTEP == 1) {
+      bst<R>(hpp, (R)bld<R>(hp, oc, pk), oc, pk);
+      bst<R>(hupp, (R)bld<R>(hup, oc, pk), oc, pk);
+      bst<R>(hvpp, (R)bld<R>(hvp, oc, pk), oc, pk);
+    }
+    const R hk = (R)bld<R>(h, oc, pk), huk = (R)bld<R>(hu, oc, pk), hvk = (R)bld<R>(hv, oc, pk);
+    bst<R>(hp, hk, oc, pk);
+    bst<R>(hup, huk, oc, pk);
+    bst<R>(hvp, hvk, oc, pk);
+    if (turb) {
+      const auto q2 = rsrc(D.q2, nb), q2p = rsrc(D.q2p, nb), q2pp = rsrc(D.q2pp, nb);
+      const auto q2l = rsrc(D.q2l, nb), q2lp = rsrc(D.q2lp, nb), q2lpp = rsrc(D.q2lpp, nb);
+      if (ISTEP == 1) {
+        bst<R>(q2pp, (R)bld<R>(q2p, oc, pk), oc, pk);
+        bst<R>(q2lpp, (R)bld<R>(q2lp, oc, pk), oc, pk);
+      }
+      bst<R>(q2p, (R)bld<R>(q2, oc, pk), oc, pk);
+      bst<R>(q2lp, (R)bld<R>(q2l, oc, pk), oc, pk);
+    }
+    if (sent || kk < k1 || kk > kwet) continue;
+    if (ISTEP == 1) {
+      bst<R>(salpp, (R)bld<R>(salp, oc, pk), oc, pk);
+      bst<R>(uhpp, (R)bld<R>(uhp, oc, pk), oc, pk);
+      bst<R>(vhpp, (R)bld<R>(vhp, oc, pk), oc, pk);
+      bst<R>(upp, (R)bld<R>(up, oc, pk), oc, pk);
+      bst<R>(vpp, (R)bld<R>(vp, oc, pk), oc, pk);
+    }
+    const R tm = (R)bld<R>(sal, oc, pk);
+    bst<R>(salp, tm, oc, pk);
+    const double zk = zl[kk];
+    const double zd = (zk == -100.0) ? 0.5 * (double)hk : zk + 0.5 * (double)hk;
+    bst<R>(rhop, (R)(densty((double)tm, 0.00004, zd) - 1000.), oc, pk);
+    face(uh, uhp, u, up, (double)huk, pk);
+    face(vh, vhp, v, vp, (double)hvk, pk);
   }
 }
 
@@ -2490,8 +2688,11 @@ int settrap(s3g_ctx *c) {
   const int L = c->lm;
   const int e = run_segment(c, SEG_SETTRAP, 2, 0, [&] {
     LAUNCH("settrap_zeta", (k_zeta_levels<R, 0>), L + 1, D);
-    LAUNCHC("settrap_cell", (k_settrap_cell<R, 0, false>), (k_settrap_cell<R, 0, true>), TILE_SETTRAP, L + 1,
-            c->km1 + 2, D);
+    if (use_window(c))
+      LAUNCHC("settrap_cell", (k_settrap_cell_w<R, 0>), (k_settrap_cell_w<R, 0>), TILE_SETTRAP, L + 1, c->km1 + 2, D);
+    else
+      LAUNCHC("settrap_cell", (k_settrap_cell<R, 0, false>), (k_settrap_cell<R, 0, true>), TILE_SETTRAP, L + 1,
+              c->km1 + 2, D);
     LAUNCH("settrap_top", (k_settrap_top<R>), L, D);
     LAUNCH("continuity2", (k_continuity<R, 2>), L, D);
     obc_vel<R>(c);
@@ -2507,8 +2708,11 @@ int settrap2(s3g_ctx *c) {
   const int L = c->lm;
   const int e = run_segment(c, SEG_SETTRAP2, 2, 0, [&] {
     LAUNCH("settrap2_zeta", (k_zeta_levels<R, 1>), L + 1, D);
-    LAUNCHC("settrap2_cell", (k_settrap_cell<R, 1, false>), (k_settrap_cell<R, 1, true>), TILE_SETTRAP, L,
-            c->km1 + 2, D);
+    if (use_window(c))
+      LAUNCHC("settrap2_cell", (k_settrap_cell_w<R, 1>), (k_settrap_cell_w<R, 1>), TILE_SETTRAP, L, c->km1 + 2, D);
+    else
+      LAUNCHC("settrap2_cell", (k_settrap_cell<R, 1, false>), (k_settrap_cell<R, 1, true>), TILE_SETTRAP, L,
+              c->km1 + 2, D);
     LAUNCH("settrap_top", (k_settrap_top<R>), L, D);
     LAUNCH("continuity2", (k_continuity<R, 2>), L, D);
     obc_vel<R>(c);
@@ -2523,10 +2727,16 @@ int save(s3g_ctx *c, int istep) {
   const int L = c->lm;
   const int e = run_segment(c, SEG_SAVE, istep, 0, [&] {
     if (istep == 1) {
-      LAUNCHC("save_cell", (k_save_cell<R, 1, false>), (k_save_cell<R, 1, true>), TILE_SAVE, L + 1, c->km1 + 2, D);
+      if (use_window(c))
+        LAUNCHC("save_cell", (k_save_cell_w<R, 1>), (k_save_cell_w<R, 1>), TILE_SAVE, L + 1, c->km1 + 2, D);
+      else
+        LAUNCHC("save_cell", (k_save_cell<R, 1, false>), (k_save_cell<R, 1, true>), TILE_SAVE, L + 1, c->km1 + 2, D);
       LAUNCH("save_top", (k_save_top<R, 1>), L + 1, D);
     } else {
-      LAUNCHC("save_cell", (k_save_cell<R, 2, false>), (k_save_cell<R, 2, true>), TILE_SAVE, L + 1, c->km1 + 2, D);
+      if (use_window(c))
+        LAUNCHC("save_cell", (k_save_cell_w<R, 2>), (k_save_cell_w<R, 2>), TILE_SAVE, L + 1, c->km1 + 2, D);
+      else
+        LAUNCHC("save_cell", (k_save_cell<R, 2, false>), (k_save_cell<R, 2, true>), TILE_SAVE, L + 1, c->km1 + 2, D);
       LAUNCH("save_top", (k_save_top<R, 2>), L + 1, D);
     }
     LAUNCH("continuity2", (k_continuity<R, 2>), L, D);
