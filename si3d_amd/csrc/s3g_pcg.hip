@@ -404,21 +404,44 @@ __global__ void __launch_bounds__(kBlock) k_mg_galerkin(MgPair<T> M, double cs) 
   }
 }
 
+// The restriction kernels give each child slot of an aggregate its own lane:
+// lane 4C + s handles child s of aggregate C (ch[4C + s], the finer level's
+// sentinel when absent), so the four children of an aggregate sit in four
+// consecutive lanes of one wavefront and a wavefront reads two runs of 32
+// consecutive fine cells instead of 64 cells two apart.  The aggregate's sum
+// is formed by the first lane of the four in child order (the order of a
+// sequential loop over the children), with the other three values fetched by
+// in-wavefront shuffles.
+__device__ __forceinline__ double child_sum(double v, bool valid) {
+  const double v1 = __shfl(v, 1, 4), v2 = __shfl(v, 2, 4), v3 = __shfl(v, 3, 4);
+  const int m = __shfl((int)valid, 0, 4) | (__shfl((int)valid, 1, 4) << 1) | (__shfl((int)valid, 2, 4) << 2) |
+                (__shfl((int)valid, 3, 4) << 3);
+  double acc = 0.0;
+  if (m & 1) acc += v;
+  if (m & 2) acc += v1;
+  if (m & 4) acc += v2;
+  if (m & 8) acc += v3;
+  return acc;
+}
+
 // Pre-smoothing from a zero guess (x = omega D^-1 b), residual, restriction.
+// Launched with a multiple of 4 lanes per workgroup; every group of four
+// lanes runs the loop the same number of times (4 nc is a multiple of 4).
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_mg_down(MgPair<T> M) {
-  for (int C = blockIdx.x * blockDim.x + threadIdx.x; C < M.nc; C += gridDim.x * blockDim.x) {
-    double acc = 0.0;
-    for (int s = 0; s < 4; ++s) {
-      const int f = M.ch[4 * C + s];
-      if (f == M.nf) continue;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < 4 * M.nc; q += gridDim.x * blockDim.x) {
+    const int f = M.ch[q];
+    const bool valid = f != M.nf;
+    double t = 0.0;
+    if (valid) {
       auto xs = [&](int j) { return kMgOmega * (double)M.fb[j] / (double)M.frr[j]; };  // fb(S)=0, frr(S)=1
       const int w = M.fW[f], so = M.fS[f], n = M.fN[f], e = M.fE[f];
       const double ax = (double)M.frr[f] * xs(f) - (double)M.fsx[w] * xs(w) - (double)M.fsy[so] * xs(so) -
                         (double)M.fsy[f] * xs(n) - (double)M.fsx[f] * xs(e);
-      acc += (double)M.fb[f] - ax;
+      t = (double)M.fb[f] - ax;
     }
-    M.cb[C] = (T)acc;
+    const double acc = child_sum(t, valid);
+    if ((q & 3) == 0) M.cb[q >> 2] = (T)acc;
   }
 }
 
@@ -467,11 +490,12 @@ __global__ void __launch_bounds__(kBlock) k_cg_update_mg(CgArgs<T> A, MgPair<T> 
   T *rn = cur ? A.r : A.r1;
   auto rnew = [&](int j) { return (double)(T)((double)ro[j] - alpha * (double)A.q[j]); };  // r(S) = q(S) = 0
   double rrn = 0.0;
-  for (int C = blockIdx.x * blockDim.x + threadIdx.x; C < M.nc; C += gridDim.x * blockDim.x) {
-    double acc = 0.0;
-    for (int s2 = 0; s2 < 4; ++s2) {
-      const int f = M.ch[4 * C + s2];
-      if (f == M.nf) continue;
+  // one lane per child slot of a level-1 aggregate (see k_mg_down)
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < 4 * M.nc; q += gridDim.x * blockDim.x) {
+    const int f = M.ch[q];
+    const bool valid = f != M.nf;
+    double t = 0.0;
+    if (valid) {
       A.x[f] = (T)((double)A.x[f] + alpha * (double)p[f]);
       const double rf = rnew(f);
       rn[f] = (T)rf;
@@ -480,9 +504,10 @@ __global__ void __launch_bounds__(kBlock) k_cg_update_mg(CgArgs<T> A, MgPair<T> 
       const int w = M.fW[f], so = M.fS[f], n = M.fN[f], e = M.fE[f];
       const double ax = (double)M.frr[f] * (kMgOmega * rf / (double)M.frr[f]) - (double)M.fsx[w] * xs(w) -
                         (double)M.fsy[so] * xs(so) - (double)M.fsy[f] * xs(n) - (double)M.fsx[f] * xs(e);
-      acc += rf - ax;
+      t = rf - ax;
     }
-    M.cb[C] = (T)acc;
+    const double acc = child_sum(t, valid);
+    if ((q & 3) == 0) M.cb[q >> 2] = (T)acc;
   }
   rrn = block_sum(rrn, lds);
   if (threadIdx.x == 0) {
@@ -822,7 +847,7 @@ static void apply_prec(s3g_ctx *c, CgArgs<T> &A, const T *rc, int slot, bool dow
   if (nl - lc > kMgMaxLevels) lc = nl - kMgMaxLevels;
   for (int l = down0_done ? 1 : 0; l < lc; ++l) {
     MgPair<T> M = pair(l);
-    hipLaunchKernelGGL((k_mg_down<T>), grid(M.nc), b, 0, c->stream, M);
+    hipLaunchKernelGGL((k_mg_down<T>), grid(4 * M.nc), b, 0, c->stream, M);
   }
   {
     MgChain<T> Ch{};
