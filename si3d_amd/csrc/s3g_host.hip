@@ -401,6 +401,11 @@ static int set_field(s3g_ctx *c, const char *name, const HT *src) {
     set_error(std::string(name) + " is not held in this configuration");
     return S3G_ERR_ARG;
   }
+  if (!std::strcmp(name, "agx") || !std::strcmp(name, "arx") || !std::strcmp(name, "agy") ||
+      !std::strcmp(name, "ary")) {
+    set_error(std::string(name) + " is formed by matmom on every fd call and cannot be set");
+    return S3G_ERR_ARG;
+  }
   const int lm1 = c->lm + 1;
   switch (f.kind) {
     case K3R:
@@ -434,6 +439,26 @@ static int get_field(s3g_ctx *c, const char *name, HT *dst) {
   }
   if (int e = hip_check(hipStreamSynchronize(c->stream), "sync")) return e;
   const int lm1 = c->lm + 1;
+  // agx/arx/agy/ary hold matmom's downward-pass factors (vel back-substitutes
+  // them); the solution the reference holds there is formed on request
+  const bool mx = !std::strcmp(name, "agx") || !std::strcmp(name, "arx");
+  if (mx || !std::strcmp(name, "agy") || !std::strcmp(name, "ary")) {
+    const size_t bytes = (size_t)c->KP * c->LP * (c->precision == S3G_FP64 ? sizeof(double) : sizeof(float));
+    void *ag = nullptr, *ar = nullptr;
+    int rc = hip_check(hipMalloc(&ag, bytes), "momentum solution");
+    if (!rc) rc = hip_check(hipMalloc(&ar, bytes), "momentum solution");
+    if (!rc) rc = hip_check(hipMemsetAsync(ag, 0, bytes, c->stream), "momentum solution");
+    if (!rc) rc = hip_check(hipMemsetAsync(ar, 0, bytes, c->stream), "momentum solution");
+    if (!rc)
+      rc = (c->precision == S3G_FP64) ? momentum_solution<double>(c, mx, ag, ar) : momentum_solution<float>(c, mx, ag, ar);
+    if (!rc) rc = hip_check(hipStreamSynchronize(c->stream), "sync");
+    const void *src = (name[1] == 'g') ? ag : ar;
+    if (!rc)
+      rc = (c->precision == S3G_FP64) ? download3(c, dst, (const double *)src) : download3(c, dst, (const float *)src);
+    if (ag) hipFree(ag);
+    if (ar) hipFree(ar);
+    return rc;
+  }
   switch (f.kind) {
     case K3R:
     case K3T:
@@ -523,7 +548,7 @@ static int alloc_state(s3g_ctx *c, Dev<R> &D, const int *lE, const int *lW, cons
   R **d3[] = {&D.h,    &D.hp,   &D.hpp,  &D.hu,  &D.hup,  &D.hupp, &D.hv,  &D.hvp,  &D.hvpp, &D.uh,
               &D.uhp,  &D.uhpp, &D.u,    &D.up,  &D.upp,  &D.vh,   &D.vhp, &D.vhpp, &D.v,    &D.vp,
               &D.vpp,  &D.wp,   &D.sal,  &D.salp, &D.salpp, &D.rhop, &D.Av, &D.Dv,   &D.ex, &D.ey,
-              &D.agx,  &D.arx,  &D.agy,  &D.ary, &D.fx_ew, &D.fx_ns, &D.bex, &D.t0};
+              &D.agx,  &D.arx,  &D.agy,  &D.ary, &D.fx_ew, &D.fx_ns, &D.bex, &D.t0, &D.cfx, &D.cfy};
   for (R **p : d3)
     if (!(*p = dalloc<R>(c, n3))) return S3G_ERR_HIP;
   // Smagorinsky coefficients only for ihd > 1, turbulence state only for iturb > 0

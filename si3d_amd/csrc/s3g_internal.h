@@ -86,6 +86,9 @@ struct Dev {
   R *uh, *uhp, *uhpp, *u, *up, *upp, *vh, *vhp, *vhpp, *v, *vp, *vpp, *wp;
   R *sal, *salp, *salpp, *rhop, *Av, *Dv, *kh, *ex, *ey, *agx, *arx, *agy, *ary;
   R *fx_ew, *fx_ns, *bex, *t0, *t1;
+  // matmom's downward-pass factors c1 of the x and y systems (its d1, e1 go
+  // to agx/arx, agy/ary); vel back-substitutes from them
+  R *cfx, *cfy;
   R *tr, *trpp;  // ntr tracers, each a full plane stack, one after the other
   int ntr;
   R *heat;  // HeatSource (surface heat budget, ifsurfbc 1-3), null otherwise
@@ -313,6 +316,7 @@ template <typename R> int settrap2(s3g_ctx *c);
 template <typename R> int save(s3g_ctx *c, int istep);
 int init_density(s3g_ctx *c);
 template <typename R> int autotune(s3g_ctx *c);
+template <typename R> int momentum_solution(s3g_ctx *c, bool x, void *ag, void *ar);
 // pcg.hip
 template <typename R> int solve_free_surface(s3g_ctx *c, double tz);
 // openbc.hip: the open-boundary steps of fd / settrap / save (no-ops without

@@ -516,10 +516,17 @@ __device__ double bclnc_km(const Dev<R> &D, int c, int ln, int kb, double plain)
   return plain;
 }
 
-// matmom: per-face tridiagonal momentum system and its two-RHS double sweep,
+// matmom: per-face tridiagonal momentum system and its two-RHS elimination,
 // si3d_procedures.f90:1176-1630 with trid 3210-3289.  Pass A: column-mean
-// density; pass B: assemble each row and eliminate downward (c1 -> t0, d1/e1 ->
-// ag/ar in place); pass C: back-substitute upward and sum the column.
+// density; pass B: assemble each row and eliminate downward, storing the
+// factors (c1 -> cfx/cfy, d1 -> agx/agy, e1 -> arx/ary).  The back
+// substitution (trid's upward pass) is left to vel, the only consumer of the
+// per-layer solution; the column sums eagx/earx that matcon needs come out of
+// the downward pass itself: with x = U^-1 d (U unit upper bidiagonal with
+// superdiagonal c1), sum_k x_k = w . d where U^T w = 1, i.e. w_k1 = 1,
+// w_k = 1 - c1_(k-1) w_(k-1), a downward recurrence.  The weights and terms
+// use the factors as stored, so the sums are those of the solution vel
+// back-substitutes.
 template <typename R, bool X>
 __global__ void __launch_bounds__(kBlock) k_matmom(Dev<R> D, double tz) {
   const int c = col();
@@ -579,6 +586,8 @@ __global__ void __launch_bounds__(kBlock) k_matmom(Dev<R> D, double tz) {
   const double bot_a = taub * twodt1;
 
   double bcl = 0.0, hdr_prev = 0.0, c1p = 0.0, d1p = 0.0, e1p = 0.0;
+  R *CF = X ? D.cfx : D.cfy;
+  double w = 1.0, sg = 0.0, sr = 0.0, c1s = 0.0;  // column-sum weight, sums, c1 as stored
   // thicknesses, face viscosity and qpp of levels k-1, k (k+1) carried down the sweep
   double hkm = hq[IX(k1q - 1, c)], hk = hq[IX(k1q, c)], Avk = 0.0;
   double qkm = 0.0, qk = (nwl > 1) ? (double)qpp[IX(k1q, c)] : 0.0;
@@ -622,9 +631,14 @@ __global__ void __launch_bounds__(kBlock) k_matmom(Dev<R> D, double tz) {
       d1 = (gg - a1 * d1p) / den;
       e1 = (hh - a1 * e1p) / den;
     }
-    D.t0[IX(k, c)] = (R)c1;
-    AG[IX(k, c)] = (R)d1;
-    AR[IX(k, c)] = (R)e1;
+    const R c1r = (R)c1, d1r = (R)d1, e1r = (R)e1;
+    CF[IX(k, c)] = c1r;
+    AG[IX(k, c)] = d1r;
+    AR[IX(k, c)] = e1r;
+    if (k > k1q) w = 1.0 - c1s * w;
+    sg += w * (double)d1r;
+    sr += w * (double)e1r;
+    c1s = (double)c1r;
     c1p = c1;
     d1p = d1;
     e1p = e1;
@@ -633,18 +647,6 @@ __global__ void __launch_bounds__(kBlock) k_matmom(Dev<R> D, double tz) {
     Avk = Avkp;
     qkm = qk;
     qk = qkp;
-  }
-  double xg = d1p, xr = e1p, sg = xg, sr = xr;
-  AG[IX(kmq, c)] = (R)xg;
-  AR[IX(kmq, c)] = (R)xr;
-  for (int k = kmq - 1; k >= k1q; --k) {
-    const double c1 = D.t0[IX(k, c)];
-    xg = (double)AG[IX(k, c)] - c1 * xg;
-    xr = (double)AR[IX(k, c)] - c1 * xr;
-    AG[IX(k, c)] = (R)xg;
-    AR[IX(k, c)] = (R)xr;
-    sg += xg;
-    sr += xr;
   }
   if (X) {
     D.eagx[c] = sg;
@@ -749,41 +751,83 @@ __global__ void __launch_bounds__(kBlock) k_slayer_h_sentinel(Dev<R> D) {
 }
 
 // vel: fluxes from the new zeta plus the top-cell drying merge,
-// si3d_procedures.f90:2430-2545.
-// One lane per (column, layer k = 2..km).  The drying merge moves the
-// top flux one layer down: the lane of layer k0+1 adds the top flux, the lane
-// of k0 zeroes it.
+// si3d_procedures.f90:2430-2545, with matmom's back substitution (trid
+// 3282-3288) in the same upward sweep: ag, ar of layer k from the stored
+// factors, uh = ag - g dt/dx 1000 (s_E - s) ar.  One lane per column and
+// face.  The drying merge moves the top flux one layer down: when the top
+// face cell k0 is dry its flux joins layer k0+1's (or, with a single wet
+// layer, the layer below the face) and k0's becomes zero.
 template <typename R, bool X>
-__device__ __forceinline__ void vel_face(const Dev<R> &D, int c, int k, double tz) {
+__device__ __forceinline__ void vel_column(const Dev<R> &D, int c, double tz) {
   const int ln = X ? D.lE[c] : D.lN[c];
   if (ln == D.lm) return;
-  const int kmq = min(D.kmz[ln], D.kmz[c]);
+  const int kmq = X ? D.kmu[c] : D.kmv[c];
   const int k0 = X ? D.k1u[c] : D.k1v[c];
-  if (k < k0 || k > kmq) return;
+  if (k0 > kmq) return;
+  const R *cf = X ? D.cfx : D.cfy;
   const R *ag = X ? D.agx : D.agy;
   const R *ar = X ? D.arx : D.ary;
   R *qh = X ? D.uh : D.vh;
   const R *hq = X ? D.hu : D.hv;
   const double cq = (X ? D.P.gdtdx : D.P.gdtdy) * tz * 1000. * (D.s[ln] - D.s[c]);
   const bool dry_top = (double)hq[IX(k0, c)] <= 0.0;
-  R v = (R)((double)ag[IX(k, c)] - cq * (double)ar[IX(k, c)]);
-  if (dry_top && k == k0) {
-    // a single wet layer: the merged flux lands one layer below the face
-    if (k0 + 1 > kmq) qh[IX(k0 + 1, c)] = v + qh[IX(k0 + 1, c)];
-    qh[IX(k, c)] = R(0);
-    return;  // the top index moves in k_continuity<R,1>, after every lane has read it
+  double xg = ag[IX(kmq, c)], xr = ar[IX(kmq, c)];
+  R v1 = R(0);  // flux of layer k0+1, held until k0's is known (dry top)
+  for (int k = kmq; k >= k0; --k) {
+    if (k < kmq) {
+      const double c1 = cf[IX(k, c)];
+      xg = (double)ag[IX(k, c)] - c1 * xg;
+      xr = (double)ar[IX(k, c)] - c1 * xr;
+    }
+    const R v = (R)((double)(R)xg - cq * (double)(R)xr);
+    if (!dry_top || k > k0 + 1) {
+      qh[IX(k, c)] = v;
+    } else if (k == k0 + 1) {
+      v1 = v;
+    } else {  // the dry top cell: its flux lands one layer below (2490-2497, 2535-2541)
+      qh[IX(k0 + 1, c)] = (k0 + 1 > kmq) ? v + qh[IX(k0 + 1, c)] : v + v1;
+      qh[IX(k0, c)] = R(0);
+    }
   }
-  if (dry_top && k == k0 + 1) v = (R)((double)ag[IX(k0, c)] - cq * (double)ar[IX(k0, c)]) + v;
-  qh[IX(k, c)] = v;
 }
 
-template <typename R, bool CH>
-__global__ void __launch_bounds__(kBlock) k_vel(Dev<R> D, double tz, int nk, int tile) {
-  CellTile t;
-  if (!cell_tile(D.lm, 2, nk, tile, t)) return;
-  for (int k = t.k0; k < (CH ? t.k1 : min(t.k1, t.k0 + 1)); ++k) {
-    vel_face<R, true>(D, t.c, k, tz);
-    vel_face<R, false>(D, t.c, k, tz);
+// Both faces of a column; the top indices move in k_continuity<R,1>.
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_vel(Dev<R> D, double tz) {
+  const int c = col();
+  if (c >= D.lm) return;
+  vel_column<R, true>(D, c, tz);
+  vel_column<R, false>(D, c, tz);
+}
+
+// The per-layer solution of one face's momentum system (what vel
+// back-substitutes), written to ag_out / ar_out on the face's wet layers and
+// zero elsewhere: the reference's agx/arx (agy/ary) for callers that read
+// them after matmom.
+template <typename R, bool X>
+__global__ void __launch_bounds__(kBlock) k_momentum_solution(Dev<R> D, R *ag_out, R *ar_out) {
+  const int c = col();
+  if (c >= D.lm) return;
+  for (int k = 0; k < D.KP; ++k) ag_out[IX(k, c)] = ar_out[IX(k, c)] = R(0);
+  const int ln = X ? D.lE[c] : D.lN[c];
+  if (ln == D.lm) return;
+  const int kmq = X ? D.kmu[c] : D.kmv[c];
+  const int k0 = X ? D.k1u[c] : D.k1v[c];
+  const R *cf = X ? D.cfx : D.cfy;
+  const R *ag = X ? D.agx : D.agy;
+  const R *ar = X ? D.arx : D.ary;
+  double xg = 0.0, xr = 0.0;
+  for (int k = kmq; k >= k0; --k) {
+    if (k == kmq) {
+      xg = ag[IX(k, c)];
+      xr = ar[IX(k, c)];
+    } else {
+      const double c1 = cf[IX(k, c)];
+      xg = (double)ag[IX(k, c)] - c1 * xg;
+      xr = (double)ar[IX(k, c)] - c1 * xr;
+    }
+    ag_out[IX(k, c)] = (R)xg;
+    ar_out[IX(k, c)] = (R)xr;
   }
 }
 
@@ -2427,7 +2471,7 @@ int phase(s3g_ctx *c, int ph, int n, int istep) {
       return rc;
     }
     case 8:
-      LAUNCHC("vel", (k_vel<R, false>), (k_vel<R, true>), TILE_VEL, L, c->km - 1, D, tz);
+      LAUNCH("vel", (k_vel<R>), L, D, tz);
       LAUNCH("continuity1", (k_continuity<R, 1>), L, D);
       obc_vel<R>(c);
       break;
@@ -2526,12 +2570,12 @@ int autotune(s3g_ctx *c) {
   for (int tid = 0; tid < TILE_TUNED; ++tid) {
     if ((tid == TILE_HFLUX || tid == TILE_RHS) && c->P.isal == 0) continue;
     if (tid == TILE_TURB && c->P.iturb < 1) continue;
+    if (tid == TILE_VEL) continue;  // vel is a column kernel (it back-substitutes matmom's systems)
     std::vector<Out> outs;
     switch (tid) {
       case TILE_EXMOM_X: outs = {{D.ex, bR}}; break;
       case TILE_EXMOM_Y: outs = {{D.ey, bR}}; break;
       case TILE_SLAYER_H: outs = {{D.h, bR}, {D.hu, bR}, {D.hv, bR}}; break;
-      case TILE_VEL: outs = {{D.uh, bR}, {D.vh, bR}}; break;
       case TILE_HFLUX: outs = {{D.fx_ew, bR}, {D.fx_ns, bR}}; break;
       case TILE_RHS: outs = {{D.bex, bR}}; break;
       case TILE_TURB: outs = {{D.Av, bR}, {D.Dv, bR}}; break;  // d0..d2 are scratch
@@ -2543,7 +2587,6 @@ int autotune(s3g_ctx *c) {
         case TILE_EXMOM_X: LAUNCH_EXMOM("", true); break;
         case TILE_EXMOM_Y: LAUNCH_EXMOM("", false); break;
         case TILE_SLAYER_H: LAUNCHC("", (k_slayer_h<R, false>), (k_slayer_h<R, true>), TILE_SLAYER_H, L, c->km1, D); break;
-        case TILE_VEL: LAUNCHC("", (k_vel<R, false>), (k_vel<R, true>), TILE_VEL, L, c->km - 1, D, tz); break;
         case TILE_HFLUX:
           if (use_window(c))
             LAUNCHC("", (k_exsal_hflux_w<R>), (k_exsal_hflux_w<R>), TILE_HFLUX, L, c->km - 1, D, sal_of(D), twodt1);
@@ -2657,7 +2700,7 @@ int fd(s3g_ctx *c, int n, int istep, int lastiter) {
     LAUNCHC("slayer_h", (k_slayer_h<R, false>), (k_slayer_h<R, true>), TILE_SLAYER_H, L, c->km1, D);
     LAUNCH("slayer_h_sentinel", (k_slayer_h_sentinel<R>), c->km1 + 2, D);
     obc_uvh<R>(c);  // openbcUVH after the solve (482)
-    LAUNCHC("vel", (k_vel<R, false>), (k_vel<R, true>), TILE_VEL, L, c->km - 1, D, tz);
+    LAUNCH("vel", (k_vel<R>), L, D, tz);
     LAUNCH("continuity1", (k_continuity<R, 1>), L, D);
     obc_vel<R>(c);
     if (c->P.isal != 0) {
@@ -2777,6 +2820,23 @@ int init_density(s3g_ctx *c) {
     hipLaunchKernelGGL(k_init_density<float>, grid_for(c->lm + 1), dim3(kBlock), 0, c->stream, c->d32);
   return hip_check(hipStreamSynchronize(c->stream), "init_density");
 }
+
+// agx/arx (x) or agy/ary (y) as the reference holds them after matmom: the
+// per-layer solution, back-substituted from the stored factors into ag / ar
+// (device buffers of the field size, zeroed by the caller).
+template <typename R>
+int momentum_solution(s3g_ctx *c, bool x, void *ag, void *ar) {
+  Dev<R> &D = dev<R>(c);
+  if (x)
+    hipLaunchKernelGGL((k_momentum_solution<R, true>), grid_for(c->lm), dim3(kBlock), 0, c->stream, D, (R *)ag,
+                       (R *)ar);
+  else
+    hipLaunchKernelGGL((k_momentum_solution<R, false>), grid_for(c->lm), dim3(kBlock), 0, c->stream, D, (R *)ag,
+                       (R *)ar);
+  return hip_check(hipGetLastError(), "momentum solution");
+}
+template int momentum_solution<double>(s3g_ctx *, bool, void *, void *);
+template int momentum_solution<float>(s3g_ctx *, bool, void *, void *);
 
 template int fd<double>(s3g_ctx *, int, int, int);
 template int fd<float>(s3g_ctx *, int, int, int);
