@@ -44,10 +44,10 @@ HBM_PEAK_GBS = 8000.0
 # time levels are FP64 in every precision mode.
 ARRAYS_PER_CELL = {
     "exmom_x": (10, 0), "exmom_y": (10, 0),      # uhp vhp wp up upp uhpp hup hupp hp -> ex
-    "matmom_x": (7, 0), "matmom_y": (7, 0),      # ex Av rhop hup upp uhpp -> agx arx
+    "matmom_x": (8, 0), "matmom_y": (8, 0),      # ex Av rhop hup upp (uhpp at the bed) -> cfx agx arx
     "matcon": (2, 0),                            # uhpp vhpp
     "slayer_h": (6, 0),                          # hp hup hvp -> h hu hv
-    "vel": (8, 0),                               # agx arx agy ary hu hv -> uh vh
+    "vel": (8, 0),                               # cfx agx arx cfy agy ary (hu hv at the top) -> uh vh
     "continuity1": (5, 0),                       # uh uhpp vh vhpp -> wp
     "exsal_hflux": (10, 0),                      # salpp uhpp uh vhpp vh hup hvp hpp -> fx_ew fx_ns
     "exsal_imsal": (11, 0),                      # fx_ew fx_ns salpp hpp hp hupp hvpp wp h Dv -> sal
@@ -64,23 +64,14 @@ ARRAYS_PER_CELL = {
 
 
 # HBM bytes per wet cell per launch measured with rocprofv3 FETCH_SIZE /
-# WRITE_SIZE (scaled by a 512 MiB device copy) on the default workload in FP64:
-# profiles/hbm_traffic_ellipse_1024x512x60_fp64.md.  Reported as
+# WRITE_SIZE (scaled by a 512 MiB device copy) on the default workload
+# (2048 x 2048 x 60, dx = 50 m, dt = 30 s, FP64):
+# profiles/hbm_traffic_ellipse_2048x2048x60_dx50_fp64.md.  Reported as
 # roofline.traffic for that workload only.
-TRAFFIC_FP64_1024x512x60 = {
-    "turbulence": 212.5, "turb_cell": 134.3, "exsal_imsal": 202.4, "exsal_rhs": 146.8, "imsal": 66.9,
-    "exmom_x": 124.7, "exmom_y": 199.3, "matmom_x": 142.3, "matmom_y": 124.5, "settrap_cell": 409.9,
-    "settrap2_cell": 274.3, "save_cell": 211.5, "exsal_hflux": 135.5, "continuity1": 107.5, "vel": 77.4,
-    "vel2": 47.9, "slayer_h": 60.8, "continuity2": 49.1, "matcon": 29.0,
-}
-
-
-# The same for the default workload (2048 x 2048 x 60, dx = 50 m, dt = 30 s,
-# FP64): profiles/hbm_traffic_ellipse_2048x2048x60_dx50_fp64.md.
 TRAFFIC_FP64_2048x2048x60_DX50 = {
-    "exmom_x": 176.8, "exmom_y": 165.8, "matmom_x": 130.7, "matmom_y": 118.3, "matcon": 27.6, "slayer_h": 59.1,
-    "vel": 66.3, "continuity1": 73.1, "exsal_hflux": 126.0, "exsal_rhs": 137.0, "imsal": 65.3, "vel2": 40.6,
-    "turbulence": 216.8, "turb_cell": 133.4, "settrap_cell": 403.7, "settrap2_cell": 259.1, "save_cell": 208.7,
+    "exmom_x": 160.9, "exmom_y": 136.7, "matmom_x": 89.8, "matmom_y": 78.5, "matcon": 27.6, "slayer_h": 61.6,
+    "vel": 77.6, "continuity1": 73.1, "exsal_hflux": 117.7, "exsal_rhs": 116.9, "imsal": 65.2, "vel2": 50.8,
+    "turbulence": 223.3, "turb_cell": 140.2, "settrap_cell": 304.7, "settrap2_cell": 180.2, "save_cell": 178.0,
     "continuity2": 40.4,
 }
 
@@ -464,8 +455,7 @@ def main():
             achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
             traffic = None
             if not a.ensemble and not a.river and not a.tracers and a.surfbc == 20 and a.precision == "fp64":
-                table = {(1024, 512, 60, 100.0, 60.0): TRAFFIC_FP64_1024x512x60,
-                         (2048, 2048, 60, 50.0, 30.0): TRAFFIC_FP64_2048x2048x60_DX50}.get(
+                table = {(2048, 2048, 60, 50.0, 30.0): TRAFFIC_FP64_2048x2048x60_DX50}.get(
                     (a.nx, a.ny, a.layers, a.dx, a.dt), {})
                 if name in table:
                     traffic = int(table[name] * n_wet)

@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <random>
@@ -111,6 +112,105 @@ __global__ void __launch_bounds__(256) k_conv3x3(TView in, int Cin, const bf16 *
     if (MASK && !(bf2f(mask.ptr[(size_t)pr * mask.stride + mask.off + co]) > 0.f)) v = 0.f;
     out.ptr[(size_t)pr * out.stride + out.off + co] = f2bf(v);
   }
+}
+
+// The same convolution with both operands staged in LDS, for images whose
+// width is a multiple of 256.  Workgroup: 4 wavefronts and an output tile of
+// 256 consecutive pixels of one image row x TC output channels; each
+// wavefront owns 64 pixels x TC channels, i.e. 2 x TC/32 accumulators, so an
+// A fragment (8 channels of one shifted pixel) read from LDS feeds TC/32
+// MFMAs and a B fragment (8 input channels of one output channel) feeds 2.
+// Per chunk of 16 input channels the workgroup stages image rows h-1 .. h+1,
+// pixels w0-1 .. w0+256 (zeros outside the image) and the chunk of all nine
+// taps of its TC output channels; every fragment read is 64 lanes x 16 bytes
+// of one contiguous kilobyte.  The FP32 sums run over chunks, then taps, then
+// channels (k_conv3x3: taps, then channels), so outputs agree with
+// k_conv3x3 up to the rounding of single bf16 values.
+// Requires W % 256 == 0, Cin % 16 == 0, Cout % TC == 0.
+template <bool RELU, bool MASK, int TC>
+__global__ void __launch_bounds__(256) k_conv3x3_lds(TView in, int Cin, const bf16 *__restrict__ Wt,
+                                                     const float *__restrict__ bias, int Cout, TView out,
+                                                     TView mask, int H, int W) {
+  constexpr int TP = 256, PW = TP + 2, NT = TC / 32;
+  __shared__ u16x8 sA[3 * PW * 2];  // [row 0..2][pixel 0..PW-1][channel half]
+  __shared__ u16x8 sB[9 * TC * 2];  // [tap][output channel][input channel half]
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, m = lane & 31, kg = lane >> 5;
+  const int tiles_per_row = W / TP;
+  const int h = blockIdx.x / tiles_per_row, w0 = (blockIdx.x - h * tiles_per_row) * TP;
+  const int co0 = blockIdx.y * TC;
+  f32x16 acc[2][NT];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[i][j] = f32x16{};
+  for (int ci0 = 0; ci0 < Cin; ci0 += 16) {
+    __syncthreads();  // the previous chunk's fragments have been read
+    for (int v = tid; v < 3 * PW * 2; v += 256) {
+      const int half = v & 1, q = v >> 1, r = q / PW, px = q - r * PW;
+      const int qh = h + r - 1, qw = w0 + px - 1;
+      u16x8 a = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (qh >= 0 && qh < H && qw >= 0 && qw < W)
+        a = *(const u16x8 *)(in.ptr + (size_t)(qh * W + qw) * in.stride + in.off + ci0 + 8 * half);
+      sA[v] = a;
+    }
+    for (int v = tid; v < 9 * TC * 2; v += 256) {
+      const int half = v & 1, q = v >> 1, t = q / TC, co = q - t * TC;
+      sB[v] = *(const u16x8 *)(Wt + ((size_t)t * Cout + co0 + co) * Cin + ci0 + 8 * half);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      const int r = t / 3, dx = t - 3 * r;  // image row h + r - 1, pixel offset dx - 1
+      bf16x8 b[NT];
+#pragma unroll
+      for (int j = 0; j < NT; ++j) b[j] = __builtin_bit_cast(bf16x8, sB[(t * TC + 32 * j + m) * 2 + kg]);
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const bf16x8 a = __builtin_bit_cast(bf16x8, sA[(r * PW + wv * 64 + 32 * i + m + dx) * 2 + kg]);
+#pragma unroll
+        for (int j = 0; j < NT; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[j], acc[i][j], 0, 0, 0);
+      }
+    }
+  }
+  const int prow = h * W + w0 + wv * 64;
+#pragma unroll
+  for (int j = 0; j < NT; ++j) {
+    const int co = co0 + 32 * j + m;
+    const float bc = bias ? bias[co] : 0.f;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int pr = prow + 32 * i + 8 * (e >> 2) + 4 * kg + (e & 3);
+        float v = acc[i][j][e] + bc;
+        if (RELU) v = fmaxf(v, 0.f);
+        if (MASK && !(bf2f(mask.ptr[(size_t)pr * mask.stride + mask.off + co]) > 0.f)) v = 0.f;
+        out.ptr[(size_t)pr * out.stride + out.off + co] = f2bf(v);
+      }
+  }
+}
+
+// 3x3 convolution: the LDS-staged kernel where the image width allows it,
+// the direct one otherwise.
+template <bool RELU, bool MASK>
+static void conv3x3(hipStream_t s, TView in, int Cin, const bf16 *Wt, const float *bias, int Cout, TView out,
+                    TView mask, int H, int W) {
+  const int npx = H * W;
+  static const bool direct = [] {
+    const char *e = std::getenv("S3G_CONV");
+    return e && std::strcmp(e, "direct") == 0;
+  }();
+  if (!direct && W % 256 == 0 && Cin % 16 == 0 && Cout % 32 == 0) {
+    if (Cout % 64 == 0)
+      hipLaunchKernelGGL((k_conv3x3_lds<RELU, MASK, 64>), dim3(npx / 256, Cout / 64), dim3(256), 0, s, in, Cin, Wt,
+                         bias, Cout, out, mask, H, W);
+    else
+      hipLaunchKernelGGL((k_conv3x3_lds<RELU, MASK, 32>), dim3(npx / 256, Cout / 32), dim3(256), 0, s, in, Cin, Wt,
+                         bias, Cout, out, mask, H, W);
+    return;
+  }
+  hipLaunchKernelGGL((k_conv3x3<RELU, MASK>), dim3((npx + 127) / 128, Cout / 32), dim3(256), 0, s, in, Cin, Wt, bias,
+                     Cout, out, mask, H, W);
 }
 
 // Weight gradient of a 3x3 convolution:
@@ -577,10 +677,8 @@ struct UNet {
 
   void conv_fwd(hipStream_t s, const ConvLayer &L, TView in, TView out) {
     const int l = L.level;
-    const int npx = Hl[l] * Wl[l];
-    dim3 g((npx + 127) / 128, L.Cout / 32);
-    hipLaunchKernelGGL((k_conv3x3<true, false>), g, dim3(256), 0, s, in, L.Cin, (const bf16 *)L.wb,
-                       (const float *)L.b, L.Cout, out, TView{nullptr, 0, 0}, Hl[l], Wl[l]);
+    conv3x3<true, false>(s, in, L.Cin, (const bf16 *)L.wb, (const float *)L.b, L.Cout, out, TView{nullptr, 0, 0},
+                         Hl[l], Wl[l]);
   }
   // dx = conv_T(dy) masked by the ReLU of `mask`; then dW, db from (dy, x).
   void conv_bwd(hipStream_t s, const ConvLayer &L, TView dy, TView x, TView dx, TView mask, bool need_dx,
@@ -588,13 +686,11 @@ struct UNet {
     const int l = L.level;
     const int npx = Hl[l] * Wl[l];
     if (need_dx) {
-      dim3 g((npx + 127) / 128, L.Cin / 32);
       if (mask.ptr)
-        hipLaunchKernelGGL((k_conv3x3<false, true>), g, dim3(256), 0, s, dy, L.Cout, (const bf16 *)L.wt,
-                           (const float *)nullptr, L.Cin, dx, mask, Hl[l], Wl[l]);
+        conv3x3<false, true>(s, dy, L.Cout, (const bf16 *)L.wt, nullptr, L.Cin, dx, mask, Hl[l], Wl[l]);
       else
-        hipLaunchKernelGGL((k_conv3x3<false, false>), g, dim3(256), 0, s, dy, L.Cout, (const bf16 *)L.wt,
-                           (const float *)nullptr, L.Cin, dx, TView{nullptr, 0, 0}, Hl[l], Wl[l]);
+        conv3x3<false, false>(s, dy, L.Cout, (const bf16 *)L.wt, nullptr, L.Cin, dx, TView{nullptr, 0, 0}, Hl[l],
+                              Wl[l]);
     }
     const int tiles = 9 * (L.Cout / 32) * ((L.Cin + 31) / 32);
     const int ns = slices_for(tiles, npx);

@@ -20,8 +20,15 @@ def _rel(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
 
 
-@pytest.mark.parametrize("H,W,seed", [(16, 24, 7), (32, 8, 8)])
-def test_unet_forward_backward_match_oracle(H, W, seed):
+# Widths that are multiples of 256 at a level run the LDS-staged convolution
+# there (8 x 2048: every level; 16 x 256: level 0 only), the others the direct
+# one.  On the larger images more single bf16 values flip between summation
+# orders, and the weight gradients, sums over every pixel of products with the
+# flipped ReLU masks, move further: the direct kernel sits 6% from the oracle
+# at 8 x 2048 (measured), the LDS kernel 4.6%.
+@pytest.mark.parametrize("H,W,seed,grad_tol", [(16, 24, 7, GRAD_TOL), (32, 8, 8, GRAD_TOL), (8, 2048, 9, 0.07),
+                                               (16, 256, 10, 0.07)])
+def test_unet_forward_backward_match_oracle(H, W, seed, grad_tol):
     from si3d_amd.model import unet_check
     rng = np.random.default_rng(seed)
     wv = U.init_weights(seed + 4)
@@ -37,9 +44,9 @@ def test_unet_forward_backward_match_oracle(H, W, seed):
     for i, (ci, co) in enumerate(U.CFG):
         for what, n in (("w", 9 * ci * co), ("b", co)):
             e = _rel(g[o:o + n], g_ref[o:o + n])
-            assert e < GRAD_TOL, (i, what, e)
+            assert e < grad_tol, (i, what, e)
             o += n
-    assert _rel(g[o:], g_ref[o:]) < GRAD_TOL
+    assert _rel(g[o:], g_ref[o:]) < grad_tol
 
 
 def test_learned_guess_gain_same_error():
