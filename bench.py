@@ -126,6 +126,9 @@ def parse():
                          "how far FP64 itself drifts with the solver tolerance")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (calibrates HBM byte counters) and exit")
+    ap.add_argument("--unet-check", type=int, default=0, metavar="N",
+                    help="only run the network's forward and backward pass three times on a random N x N input "
+                         "(s3g_unet_check; a workload for profiling the network's kernels) and exit")
     return ap.parse_args()
 
 
@@ -154,6 +157,22 @@ def copy_probe(mib):
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_mib": mib, "bytes_read_per_copy": n * 4, "bytes_written_per_copy": n * 4}))
+
+
+def unet_probe(n):
+    """Forward + backward of the network on a random n x n input, three times
+    (the first call includes the allocation), for kernel profiles."""
+    from si3d_amd.model import unet_check, unet_num_weights
+    rng = np.random.default_rng(3)
+    wv = (0.05 * rng.standard_normal(unet_num_weights())).astype(np.float32)
+    x3 = rng.normal(size=(3, n, n)).astype(np.float32)
+    target = rng.normal(size=(n, n)).astype(np.float32)
+    mask = (rng.random((n, n)) > 0.2).astype(np.float32)
+    for rep in range(3):
+        t0 = time.perf_counter()
+        _, _, loss = unet_check(wv, x3, target, mask)
+        print(json.dumps({"unet_check": f"{n}x{n}", "rep": rep, "seconds": round(time.perf_counter() - t0, 3),
+                          "loss": loss}), flush=True)
 
 
 def copy_bandwidth(mib=1024, reps=10):
@@ -354,6 +373,9 @@ def main():
         a.dt = 60.0 * min(1.0, a.dx / 100.0)
     if a.copy_probe:
         copy_probe(a.copy_probe)
+        return
+    if a.unet_check:
+        unet_probe(a.unet_check)
         return
     if a.mixed_error or a.tol_floor:
         mixed_error(a, floor=bool(a.tol_floor))

@@ -264,6 +264,11 @@ class GpuLake:
         return {bytes(names[i]).split(b"\0")[0].decode(): (avg[i], cnt[i]) for i in range(n)}
 
 
+def unet_num_weights():
+    """Number of weights of the library's U-Net (the length of a weight vector)."""
+    return int(_lib.load().s3g_learn_num_weights())
+
+
 def unet_check(weights, x3, target, mask, static=False):
     """Forward + backward of the library's U-Net on one dense input (x3: 3 x H x
     W); returns (out H x W, weight gradients, loss).  `static` calls the
