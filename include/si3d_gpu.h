@@ -122,7 +122,9 @@ int s3g_init(const s3g_grid *grid, const s3g_params *params, s3g_ctx **out);
 
 /* Copy one state array in / out (names as in si3d_types.f90: "uh", "sal",
  * "q2p", "s", "k1z", ...).  *_f32 for REAL arrays, *_f64 for REAL*8 arrays
- * (q2, q2l and their time levels) or for FP64 callers, *_i32 for indices. */
+ * (q2, q2l and their time levels) or for FP64 callers, *_i32 for indices.
+ * "agx", "arx", "agy", "ary" can be read (matmom's per-layer solution, formed
+ * on request from the factors the library keeps) but not set. */
 int s3g_set_field_f32(s3g_ctx *c, const char *name, const float *host);
 int s3g_set_field_f64(s3g_ctx *c, const char *name, const double *host);
 int s3g_set_field_i32(s3g_ctx *c, const char *name, const int *host);
@@ -173,6 +175,13 @@ int s3g_reset_timers(s3g_ctx *c, int enable);
  * the defaults; S3G_TILES="w0,w1,..." fixes them).  Returns the number
  * written. */
 int s3g_get_tiles(s3g_ctx *c, int max, int *kc);
+/* Streaming copy of `bytes` bytes (rounded down to 16) on the current device
+ * with this library's own kernel (16-byte vector loads and stores,
+ * grid-stride), `reps` timed passes after one untimed pass; *gbs receives the
+ * read + write rate in GB/s.  No counterpart in the reference: it measures
+ * the HBM rate a plain streaming kernel reaches on the device, the ceiling
+ * against which the bench sets its kernels' measured traffic. */
+int s3g_probe_stream(size_t bytes, int reps, double *gbs);
 void s3g_finalize(s3g_ctx *c);
 const char *s3g_last_error(void);
 

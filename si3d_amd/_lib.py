@@ -23,7 +23,7 @@ EXPORTS = [
     "s3g_learn_setup", "s3g_learn_record", "s3g_learn_train", "s3g_learn_mode", "s3g_learn_num_weights",
     "s3g_learn_get_weights", "s3g_learn_set_weights", "s3g_learn_info", "s3g_learn_flexible", "s3g_unet_check",
     "s3g_create_ensemble", "s3g_ensemble_info", "s3g_energy_balance", "s3g_set_open_boundaries",
-    "s3g_unet_check_static",
+    "s3g_unet_check_static", "s3g_probe_stream",
 ]
 
 dp = C.POINTER(C.c_double)
@@ -109,6 +109,7 @@ def load():
     L.s3g_kernel_times.argtypes = [vp, C.c_int, C.c_void_p, dp, C.POINTER(C.c_long)]
     L.s3g_reset_timers.argtypes = [vp, C.c_int]
     L.s3g_get_tiles.argtypes = [vp, C.c_int, ip]
+    L.s3g_probe_stream.argtypes = [C.c_size_t, C.c_int, dp]
     L.s3g_learn_setup.argtypes = [vp, C.c_int, C.c_uint]
     L.s3g_learn_record.argtypes = [vp, C.c_int]
     L.s3g_learn_train.argtypes = [vp, C.c_int, C.c_int, C.c_double, dp]

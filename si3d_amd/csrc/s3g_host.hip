@@ -832,6 +832,14 @@ int s3g::init_ctx(const s3g_grid *g, const s3g_params *p, int nlakes, const doub
   return hip_check(hipDeviceSynchronize(), "init");
 }
 
+namespace s3g {
+// Streaming copy for s3g_probe_stream: 16-byte loads and stores, grid-stride.
+__global__ void __launch_bounds__(256) k_stream_copy(const float4 *__restrict__ a, float4 *__restrict__ b, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+}  // namespace s3g
+
 extern "C" {
 
 int s3g_set_field_f32(s3g_ctx *c, const char *n, const float *h) { return set_field(c, n, h); }
@@ -966,6 +974,34 @@ int s3g_get_tiles(s3g_ctx *c, int max, int *kc) {
   const int n = std::min<int>(max, s3g::TILE_NUM);
   for (int i = 0; i < n; ++i) kc[i] = c->kc[i];
   return n;
+}
+
+int s3g_probe_stream(size_t bytes, int reps, double *gbs) {
+  if (!gbs || reps < 1 || bytes < 16) return S3G_ERR_ARG;
+  const size_t n = bytes / 16;
+  float4 *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = hip_check(hipMalloc(&a, n * 16), "s3g_probe_stream");
+  if (!rc) rc = hip_check(hipMalloc(&b, n * 16), "s3g_probe_stream");
+  if (!rc) rc = hip_check(hipMemset(a, 0, n * 16), "s3g_probe_stream");
+  if (!rc) rc = hip_check(hipEventCreate(&e0), "s3g_probe_stream");
+  if (!rc) rc = hip_check(hipEventCreate(&e1), "s3g_probe_stream");
+  if (!rc) {
+    const dim3 grid(8192), block(256);  // 32 workgroups per CU
+    hipLaunchKernelGGL(s3g::k_stream_copy, grid, block, 0, nullptr, (const float4 *)a, b, n);
+    hipEventRecord(e0, nullptr);
+    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(s3g::k_stream_copy, grid, block, 0, nullptr, (const float4 *)a, b, n);
+    hipEventRecord(e1, nullptr);
+    rc = hip_check(hipEventSynchronize(e1), "s3g_probe_stream");
+    float ms = 0.f;
+    if (!rc) rc = hip_check(hipEventElapsedTime(&ms, e0, e1), "s3g_probe_stream");
+    if (!rc) *gbs = ms > 0.f ? 2.0 * (double)n * 16.0 * reps / (ms * 1e-3) / 1e9 : 0.0;
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (a) hipFree(a);
+  if (b) hipFree(b);
+  return rc;
 }
 
 int s3g_kernel_times(s3g_ctx *c, int max, char (*names)[32], double *avg_ms, long *launches) {
