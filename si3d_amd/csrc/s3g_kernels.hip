@@ -739,6 +739,57 @@ __global__ void __launch_bounds__(kBlock) k_slayer_h(Dev<R> D, int nk, int tile)
   for (int k = t.k0; k < (CH ? t.k1 : min(t.k1, t.k0 + 1)); ++k) slayer_h_cell<R>(D, t.c, k);
 }
 
+// Windowed form of k_slayer_h (same values, same order): the column's top
+// indices, levels and surface elevations are read once per chunk of layers
+// and the 3-D fields through buffer resources with a scalar plane offset.
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_slayer_h_w(Dev<R> D, int nk, int tile) {
+  CellTile t;
+  if (!cell_tile(D.lm, 1, nk, tile, t)) return;
+  const int c = t.c, S = D.lm;
+  const int kend = min(t.k1, D.kmz[c] + 2);
+  if (t.k0 >= kend) return;
+  const unsigned sz = sizeof(R), nb = (unsigned)D.KP * (unsigned)D.LP * sz, pb = (unsigned)D.LP * sz;
+  const unsigned oc = (unsigned)c * sz;
+  const int km1 = km1c(D, c), k1z = D.k1z[c], k1u = D.k1u[c], k1v = D.k1v[c];
+  const int lE = D.lE[c], lN = D.lN[c];
+  const double sc = D.s[c], hhs = D.hhs[c], hhu = D.hhu[c], hhv = D.hhv[c];
+  const double *zl = zlv(D, c);
+  const double smE = (lE != S) ? dmax(sc, D.s[lE]) : 0.0;
+  const double smN = (lN != S) ? dmax(sc, D.s[lN]) : 0.0;
+  const double haux0z = dmin(zl[k1z + 1], hhs) + sc;
+  const double haux0v = dmin(zl[k1v + 1], hhv) + smN;
+  const auto h = rsrc(D.h, nb), hu = rsrc(D.hu, nb), hv = rsrc(D.hv, nb);
+  const auto hp = rsrc(D.hp, nb), hup = rsrc(D.hup, nb), hvp = rsrc(D.hvp, nb);
+  for (int k = t.k0; k < kend; ++k) {
+    const unsigned pk = (unsigned)k * pb;
+    if (k < k1z || k >= k1z + 2) {
+      bst<R>(h, (R)bld<R>(hp, oc, pk), oc, pk);
+    } else if (k == k1z) {
+      bst<R>(h, (haux0z <= 0.0) ? R(0) : (R)haux0z, oc, pk);
+    } else if (haux0z <= 0.0) {
+      const double haux = dmin(zl[k + 1], hhs) + sc;
+      bst<R>(h, (haux <= 0.0) ? R(0) : (R)haux, oc, pk);
+    }
+    if (k != k1u || lE == S) {
+      bst<R>(hu, (R)bld<R>(hup, oc, pk), oc, pk);
+    } else {
+      const double haux = dmin(zl[k + 1], hhu) + smE;
+      bst<R>(hu, (haux <= 0.0) ? R(0) : (R)haux, oc, pk);
+    }
+    if (lN == S) {
+      if (k < k1v || k >= k1v + 2 || k <= min(k1v + 1, km1)) bst<R>(hv, (R)bld<R>(hvp, oc, pk), oc, pk);
+    } else if (k < k1v || k >= k1v + 2) {
+      bst<R>(hv, (R)bld<R>(hvp, oc, pk), oc, pk);
+    } else if (k == k1v) {
+      bst<R>(hv, (haux0v <= 0.0) ? R(0) : (R)haux0v, oc, pk);
+    } else if (haux0v <= 0.0) {
+      const double haux = dmin(zl[k + 1], hhv) + smN;
+      bst<R>(hv, (haux <= 0.0) ? R(0) : (R)haux, oc, pk);
+    }
+  }
+}
+
 // The sentinel column: h := hp, hu := hup, hv := hvp (2087-2089).
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_slayer_h_sentinel(Dev<R> D) {
@@ -1259,6 +1310,65 @@ __global__ void __launch_bounds__(kBlock) k_vel2(Dev<R> D, int nk, int tile) {
     }
     vel2_face<R, true>(D, c, k, sc);
     vel2_face<R, false>(D, c, k, sc);
+  }
+}
+
+// Windowed form of k_vel2 (same values, same order): the column's levels,
+// face bottoms and neighbour surface elevations are read once per chunk of
+// layers, the level pair zl(k), zl(k+1) is carried down the chunk, and the
+// thicknesses are stored through buffer resources with a scalar plane offset.
+// The top-face flux split (one lane per column and face) keeps plain loads.
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_vel2_w(Dev<R> D, int nk, int tile) {
+  CellTile t;
+  if (!cell_tile(D.lm, 2, nk, tile, t)) return;
+  const int c = t.c, S = D.lm;
+  const unsigned sz = sizeof(R), nb = (unsigned)D.KP * (unsigned)D.LP * sz, pb = (unsigned)D.LP * sz;
+  const unsigned oc = (unsigned)c * sz;
+  const int kmz = D.kmz[c], k1 = D.k1;
+  const int lE = D.lE[c], lN = D.lN[c];
+  const bool fx = lE != S, fy = lN != S;
+  const int kmu = fx ? min(kmz, D.kmz[lE]) : -1, kmv = fy ? min(kmz, D.kmz[lN]) : -1;
+  const int k1u = D.k1u[c], k1v = D.k1v[c];
+  const double sc = D.s[c], hhs = D.hhs[c], hhu = D.hhu[c], hhv = D.hhv[c];
+  const double smE = fx ? dmax(sc, D.s[lE]) : 0.0, smN = fy ? dmax(sc, D.s[lN]) : 0.0;
+  const double *zl = zlv(D, c);
+  const auto h = rsrc(D.h, nb), hu = rsrc(D.hu, nb), hv = rsrc(D.hv, nb);
+  auto lay = [](double zt, double zb, double hh, double sm) {
+    const double v = dmin(zt, hh) - dmax(zb, -sm);
+    return (v <= 0.0) ? 0.0 : v;
+  };
+  // the top-cell flux split of one face (vel2_face after its thickness)
+  auto split = [&](R *hq, R *qh, int kt, int kmq, double hh, double sm) {
+    const double hm = (kt - 1 >= k1 && kt - 1 <= kmq) ? (double)(R)lay(zl[kt], zl[kt - 1], hh, sm)
+                                                      : (double)hq[IX(kt - 1, c)];
+    if (hm > 0.0) {
+      const double hk = (kt <= kmq) ? (double)(R)lay(zl[kt + 1], zl[kt], hh, sm) : (double)hq[IX(kt, c)];
+      const double tq = (double)qh[IX(kt, c)] / (hk + hm);
+      qh[IX(kt, c)] = (R)(tq * hk);
+      qh[IX(kt - 1, c)] = (R)(tq * hm);
+    }
+  };
+  // nothing below the column's bed other than a top face's split
+  const int kend = min(t.k1, max(kmz, max(fx ? k1u : -1, fy ? k1v : -1)) + 1);
+  if (t.k0 >= kend) return;
+  double zk = zl[t.k0];
+  for (int k = t.k0; k < kend; ++k) {
+    const unsigned pk = (unsigned)k * pb;
+    const double zk1 = zl[k + 1];
+    if (k <= kmz) {
+      const double v = dmin(zk1, hhs) - dmax(zk, -sc);
+      bst<R>(h, (v <= 0.0) ? R(0) : (R)v, oc, pk);
+    }
+    if (fx) {
+      if (k <= kmu) bst<R>(hu, (R)lay(zk1, zk, hhu, smE), oc, pk);
+      if (k == k1u) split(D.hu, D.uh, k1u, kmu, hhu, smE);
+    }
+    if (fy) {
+      if (k <= kmv) bst<R>(hv, (R)lay(zk1, zk, hhv, smN), oc, pk);
+      if (k == k1v) split(D.hv, D.vh, k1v, kmv, hhv, smN);
+    }
+    zk = zk1;
   }
 }
 
@@ -2353,6 +2463,22 @@ static inline bool exmom_window(const s3g_ctx *c) {
               c->km - 1, D, twodt1);                                                                       \
   } while (0)
 
+#define LAUNCH_SLAYER_H(name)                                                                         \
+  do {                                                                                                \
+    if (use_window(c))                                                                                \
+      LAUNCHC(name, (k_slayer_h_w<R>), (k_slayer_h_w<R>), TILE_SLAYER_H, L, c->km1, D);               \
+    else                                                                                              \
+      LAUNCHC(name, (k_slayer_h<R, false>), (k_slayer_h<R, true>), TILE_SLAYER_H, L, c->km1, D);      \
+  } while (0)
+
+#define LAUNCH_VEL2(name)                                                                             \
+  do {                                                                                                \
+    if (use_window(c))                                                                                \
+      LAUNCHC(name, (k_vel2_w<R>), (k_vel2_w<R>), TILE_VEL2, L, c->km - 1, D);                        \
+    else                                                                                              \
+      LAUNCHC(name, (k_vel2<R, false>), (k_vel2<R, true>), TILE_VEL2, L, c->km - 1, D);               \
+  } while (0)
+
 template <typename R>
 static void sync_forcing(s3g_ctx *c) {
   Dev<R> &D = dev<R>(c);
@@ -2465,7 +2591,7 @@ int phase(s3g_ctx *c, int ph, int n, int istep) {
       break;
     case 7: {
       int rc = solve_free_surface<R>(c, tz);
-      LAUNCHC("slayer_h", (k_slayer_h<R, false>), (k_slayer_h<R, true>), TILE_SLAYER_H, L, c->km1, D);
+      LAUNCH_SLAYER_H("slayer_h");
       LAUNCH("slayer_h_sentinel", (k_slayer_h_sentinel<R>), c->km1 + 2, D);
       obc_uvh<R>(c);
       return rc;
@@ -2489,7 +2615,7 @@ int phase(s3g_ctx *c, int ph, int n, int istep) {
       LAUNCH("imsal", (k_exsal_imsal<R, false, true, false>), L, D, sal_of(D), twodt1);
       obc_scalar<R>(c, D.sal, 1);
       break;
-    case 11: LAUNCHC("vel2", (k_vel2<R, false>), (k_vel2<R, true>), TILE_VEL2, L, c->km - 1, D); break;
+    case 11: LAUNCH_VEL2("vel2"); break;
     case 12:
       LAUNCH("smooth_zeta", (k_zeta_levels<R, 2>), L + 1, D);
       LAUNCH("smooth_col", (k_smooth_col<R>), L, D);
@@ -2586,7 +2712,7 @@ int autotune(s3g_ctx *c) {
       switch (tid) {
         case TILE_EXMOM_X: LAUNCH_EXMOM("", true); break;
         case TILE_EXMOM_Y: LAUNCH_EXMOM("", false); break;
-        case TILE_SLAYER_H: LAUNCHC("", (k_slayer_h<R, false>), (k_slayer_h<R, true>), TILE_SLAYER_H, L, c->km1, D); break;
+        case TILE_SLAYER_H: LAUNCH_SLAYER_H(""); break;
         case TILE_HFLUX:
           if (use_window(c))
             LAUNCHC("", (k_exsal_hflux_w<R>), (k_exsal_hflux_w<R>), TILE_HFLUX, L, c->km - 1, D, sal_of(D), twodt1);
@@ -2603,7 +2729,7 @@ int autotune(s3g_ctx *c) {
                     sal_of(D), twodt1);
           break;
         case TILE_TURB: LAUNCHC("", (k_turb_cell<R, false>), (k_turb_cell<R, true>), TILE_TURB, L, c->km1, D); break;
-        default: LAUNCHC("", (k_vel2<R, false>), (k_vel2<R, true>), TILE_VEL2, L, c->km - 1, D); break;
+        default: LAUNCH_VEL2(""); break;
       }
     };
     int best_kc = c->kc[tid];
@@ -2697,7 +2823,7 @@ int fd(s3g_ctx *c, int n, int istep, int lastiter) {
   int rc = solve_free_surface<R>(c, tz);
   if (rc < 0) return rc;
   e = run_segment(c, SEG_FD_POST, istep, lastiter, [&] {
-    LAUNCHC("slayer_h", (k_slayer_h<R, false>), (k_slayer_h<R, true>), TILE_SLAYER_H, L, c->km1, D);
+    LAUNCH_SLAYER_H("slayer_h");
     LAUNCH("slayer_h_sentinel", (k_slayer_h_sentinel<R>), c->km1 + 2, D);
     obc_uvh<R>(c);  // openbcUVH after the solve (482)
     LAUNCH("vel", (k_vel<R>), L, D, tz);
@@ -2713,7 +2839,7 @@ int fd(s3g_ctx *c, int n, int istep, int lastiter) {
       for (int nt = 0; nt < c->ntr; ++nt)
         transport<R, true>(c, Scal<R>{D.trpp + nt * n3, D.tr + nt * n3, nullptr}, twodt1, 2 + nt);
     }
-    LAUNCHC("vel2", (k_vel2<R, false>), (k_vel2<R, true>), TILE_VEL2, L, c->km - 1, D);
+    LAUNCH_VEL2("vel2");
     if (c->P.ismooth >= 1 && istep == 1) {
       LAUNCH("smooth_zeta", (k_zeta_levels<R, 2>), L + 1, D);
       LAUNCH("smooth_col", (k_smooth_col<R>), L, D);

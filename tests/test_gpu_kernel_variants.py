@@ -1,7 +1,8 @@
 """Kernel variants must not change results.  The windowed per-cell kernels
 (one lane per column and chunk of layers, buffer-resource loads, vertical
 neighbours carried in registers) replace the generic ones (exmom for centred
-momentum advection with ihd 0 or 1, exsal's face fluxes and right-hand side);
+momentum advection with ihd 0 or 1, exsal's face fluxes and right-hand side,
+slayer_h, vel2's thicknesses and top-cell flux split, settrap, save);
 S3G_WINDOW=0 keeps the generic ones.  Both evaluate the same expressions in
 the same order, so every field must agree bit for bit after many steps, for
 every layer-chunk size (S3G_TILES fixes it), in both precisions, with and
