@@ -548,7 +548,7 @@ __global__ void __launch_bounds__(kBlock) k_matmom(Dev<R> D, double tz) {
 
   double rsum = 0.0;
   for (int k = k1q; k <= kmq; ++k) rsum += 1000. + (double)D.rhop[IX(k, c)];
-  const double rhoq = rsum / nwl;
+  const double rhoq = rsum / nwl, irhoq = 1.0 / rhoq;
   const double ws0 = rhoq * gq * (D.spp[ln] - D.spp[c]);
 
   // wind stress at the top face (1348-1360 / 1541-1553)
@@ -588,8 +588,13 @@ __global__ void __launch_bounds__(kBlock) k_matmom(Dev<R> D, double tz) {
   double bcl = 0.0, hdr_prev = 0.0, c1p = 0.0, d1p = 0.0, e1p = 0.0;
   R *CF = X ? D.cfx : D.cfy;
   double w = 1.0, sg = 0.0, sr = 0.0, c1s = 0.0;  // column-sum weight, sums, c1 as stored
-  // thicknesses, face viscosity and qpp of levels k-1, k (k+1) carried down the sweep
-  double hkm = hq[IX(k1q - 1, c)], hk = hq[IX(k1q, c)], Avk = 0.0;
+  // thicknesses (and 1/h), face viscosity, qpp and the viscous flux of levels
+  // k-1, k (k+1) carried down the sweep.  The sweep is bound by FP64
+  // arithmetic: divisions by a quantity used more than once are
+  // multiplications by its reciprocal (1/h, 1/rho, the pivot), which moves
+  // results by rounding only; the viscous flux through the top of layer k is
+  // the one through the bottom of layer k-1.
+  double hkm = hq[IX(k1q - 1, c)], hk = hq[IX(k1q, c)], Avk = 0.0, rhk = 1.0 / hk, dkp_prev = 0.0;
   double qkm = 0.0, qk = (nwl > 1) ? (double)qpp[IX(k1q, c)] : 0.0;
   for (int k = k1q; k <= kmq; ++k) {
     const double hkp = hq[IX(k + 1, c)];
@@ -603,17 +608,18 @@ __global__ void __launch_bounds__(kBlock) k_matmom(Dev<R> D, double tz) {
     }
     double vd = 0.0;
     if (nwl > 1) {
-      double dk = (k > k1q) ? Avk * (qkm - qk) / (hkm + hk) : 0.0;
-      double dkp = (k < kmq) ? Avkp * (qk - qkp) / (hk + hkp) : 0.0;
+      const double dk = dkp_prev;
+      const double dkp = (k < kmq) ? Avkp * (qk - qkp) / (hk + hkp) : 0.0;
       vd = D.P.twodt * (dk - dkp) * 2. * (1. - theta);
+      dkp_prev = dkp;
     }
-    const double hh = hk / rhoq;
+    const double hh = hk * irhoq;
     double gg = (double)(X ? D.ex : D.ey)[IX(k, c)] - hh * (bcl + ws0) * tz + vd * tz;
     double a1 = 0.0, a2 = 1.0, a3 = 0.0;
     if (nwl > 1) {
       a3 = (k < kmq) ? -twodt1 * Avkp / (hkp * (hk + hkp)) * 2. * theta : 0.0;
       a1 = (k > k1q) ? -twodt1 * Avk / (hkm * (hkm + hk)) * 2. * theta : 0.0;
-      a2 = 1.0 - (hkm / hk) * a1 - (hkp / hk) * a3;
+      a2 = 1.0 - (hkm * rhk) * a1 - (hkp * rhk) * a3;
     }
     if (k == k1q) {
       gg += wind_g;
@@ -622,14 +628,15 @@ __global__ void __launch_bounds__(kBlock) k_matmom(Dev<R> D, double tz) {
     if (k == kmq) a2 += bot_a;
     double c1, d1, e1;
     if (k == k1q) {
-      c1 = a3 / a2;
-      d1 = gg / a2;
-      e1 = hh / a2;
+      const double rd = 1.0 / a2;
+      c1 = a3 * rd;
+      d1 = gg * rd;
+      e1 = hh * rd;
     } else {
-      const double den = a2 - a1 * c1p;
-      c1 = a3 / den;
-      d1 = (gg - a1 * d1p) / den;
-      e1 = (hh - a1 * e1p) / den;
+      const double rd = 1.0 / (a2 - a1 * c1p);
+      c1 = a3 * rd;
+      d1 = (gg - a1 * d1p) * rd;
+      e1 = (hh - a1 * e1p) * rd;
     }
     const R c1r = (R)c1, d1r = (R)d1, e1r = (R)e1;
     CF[IX(k, c)] = c1r;
@@ -644,6 +651,7 @@ __global__ void __launch_bounds__(kBlock) k_matmom(Dev<R> D, double tz) {
     e1p = e1;
     hkm = hk;
     hk = hkp;
+    rhk = 1.0 / hkp;
     Avk = Avkp;
     qkm = qk;
     qk = qkp;
@@ -822,22 +830,41 @@ __device__ __forceinline__ void vel_column(const Dev<R> &D, int c, double tz) {
   const R *hq = X ? D.hu : D.hv;
   const double cq = (X ? D.P.gdtdx : D.P.gdtdy) * tz * 1000. * (D.s[ln] - D.s[c]);
   const bool dry_top = (double)hq[IX(k0, c)] <= 0.0;
-  double xg = ag[IX(kmq, c)], xr = ar[IX(kmq, c)];
+  double xg = 0.0, xr = 0.0;
   R v1 = R(0);  // flux of layer k0+1, held until k0's is known (dry top)
-  for (int k = kmq; k >= k0; --k) {
-    if (k < kmq) {
-      const double c1 = cf[IX(k, c)];
-      xg = (double)ag[IX(k, c)] - c1 * xg;
-      xr = (double)ar[IX(k, c)] - c1 * xr;
+  // Layers in groups of four, the group's factors loaded together before its
+  // substitutions and stores (as in k_continuity); layers of a group above k0
+  // read a valid layer (clamped at 0) and are discarded.
+  for (int kg = kmq; kg >= k0; kg -= 4) {
+    R fc[4], fg[4], fr[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int kk = max(kg - j, 0);
+      fc[j] = cf[IX(kk, c)];
+      fg[j] = ag[IX(kk, c)];
+      fr[j] = ar[IX(kk, c)];
     }
-    const R v = (R)((double)(R)xg - cq * (double)(R)xr);
-    if (!dry_top || k > k0 + 1) {
-      qh[IX(k, c)] = v;
-    } else if (k == k0 + 1) {
-      v1 = v;
-    } else {  // the dry top cell: its flux lands one layer below (2490-2497, 2535-2541)
-      qh[IX(k0 + 1, c)] = (k0 + 1 > kmq) ? v + qh[IX(k0 + 1, c)] : v + v1;
-      qh[IX(k0, c)] = R(0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = kg - j;
+      if (k < k0) break;
+      if (k < kmq) {
+        const double c1 = fc[j];
+        xg = (double)fg[j] - c1 * xg;
+        xr = (double)fr[j] - c1 * xr;
+      } else {
+        xg = fg[j];
+        xr = fr[j];
+      }
+      const R v = (R)((double)(R)xg - cq * (double)(R)xr);
+      if (!dry_top || k > k0 + 1) {
+        qh[IX(k, c)] = v;
+      } else if (k == k0 + 1) {
+        v1 = v;
+      } else {  // the dry top cell: its flux lands one layer below (2490-2497, 2535-2541)
+        qh[IX(k0 + 1, c)] = (k0 + 1 > kmq) ? v + qh[IX(k0 + 1, c)] : v + v1;
+        qh[IX(k0, c)] = R(0);
+      }
     }
   }
 }
@@ -901,10 +928,21 @@ __global__ void __launch_bounds__(kBlock) k_continuity(Dev<R> D) {
     };
     for (int k = 0; k <= D.km1 + 1; ++k)
       if (k < k1s || k > kms) D.wp[IX(k, c)] = R(0);
+    // Levels in groups of four: the group's loads are issued together before
+    // its sums and stores (a level at a time, each level's loads wait for the
+    // previous level's store in program order).  Levels of a group above k1s
+    // read a valid level (clamped at 0) and are discarded.
     double w = 0.0;
-    for (int k = kms; k >= k1s; --k) {
-      w -= div(k);
-      if (k > k1s) D.wp[IX(k, c)] = (R)w;
+    for (int k = kms; k >= k1s; k -= 4) {
+      double d[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) d[j] = div(max(k - j, 0));
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (k - j < k1s) break;
+        w -= d[j];
+        if (k - j > k1s) D.wp[IX(k - j, c)] = (R)w;
+      }
     }
     for (int k = D.k1; k <= k1s - 1; ++k) w -= div(k);
     D.wp[IX(k1s, c)] = (R)w;
@@ -1586,36 +1624,45 @@ __device__ __forceinline__ TurbRaw turb_raw(const Dev<R> &D, const IO &io, int m
 // the new eddy coefficients (245-290), plus q2p(k) and q2lp(k).  Av(k)/Dv(k)
 // are read for the productions before the new values replace them.
 struct TurbLev {
-  double xl, sp, bp, av, dv, q2p, q2lp;
+  double xl, sp, bp, av, dv, q2p, q2lp, rq, sq;  // rq = 1 / q2p, sq = sqrt(q2p)
 };
 
 template <typename R, typename IO>
 __device__ __forceinline__ TurbLev turb_level(const Dev<R> &D, const IO &io, int m, int k, const TurbRaw &a,
                                              const TurbRaw &b) {
   TurbLev L;
+  // The closure is bound by FP64 arithmetic (about 30 divisions per level as
+  // written): divisions by a quantity used more than once are multiplications
+  // by its reciprocal (1/dz, 1/q2p here; 1/twodt1 and the elimination pivots
+  // in the sweep), which moves results by rounding only.
   const double hkm = a.h, hk = b.h;
-  const double dz = 0.5 * (hkm + hk);
+  const double dz = 0.5 * (hkm + hk), idz = 1.0 / dz;
   const double rkm = a.rho, rk = b.rho;
-  const double N2 = -G / (0.5 * (rkm + rk) + 1000.) * ((rkm - rk) / dz);
-  const double dudz = 0.5 * (a.um + a.uw - b.um - b.uw) / dz;
-  const double dvdz = 0.5 * (a.vm + a.vs - b.vm - b.vs) / dz;
+  const double drdz = (rkm - rk) * idz, rden = 0.5 * (rkm + rk) + 1000.;
+  const double N2 = -G / rden * drdz;
+  const double dudz = 0.5 * (a.um + a.uw - b.um - b.uw) * idz;
+  const double dvdz = 0.5 * (a.vm + a.vs - b.vm - b.vs) * idz;
   const double q2pk = io.ld(D.q2p, k, m), q2lpk = io.ld(D.q2lp, k, m);
+  const double rq = 1.0 / q2pk;
   L.q2p = q2pk;
   L.q2lp = q2lpk;
-  double xl = q2lpk / q2pk;
+  L.rq = rq;
+  L.sq = sqrt(q2pk);
+  const double xl0 = q2lpk * rq;
+  double xl = xl0;
   if (N2 > 0.0) xl = dmin(xl, 0.53 * sqrt(q2pk / N2));
   L.xl = xl;
   L.sp = (double)io.ld(D.Av, k, m) * (dudz * dudz + dvdz * dvdz);
   L.bp = -(double)io.ld(D.Dv, k, m) * N2;
   // eddy coefficients (UpdateMixingCoefficients evaluates N^2 in this order)
-  const double N2b = -G * ((rkm - rk) / dz) / (0.5 * (rkm + rk) + 1000.);
-  double xlb = q2lpk / q2pk;
+  const double N2b = -G * drdz / rden;
+  double xlb = xl0;
   if (N2b > 0.0) xlb = dmin(xlb, 0.53 * sqrt(q2pk / N2b));
-  double Gh = -N2b * xlb * xlb / q2pk;
+  double Gh = -N2b * xlb * xlb * rq;
   Gh = dmax(GH_MIN, dmin(GH_MAX, Gh));
   const double Sh = D.P.t_1 / (1. - Gh * D.P.t_2);
   const double Sm = (D.P.t_3 + Sh * Gh * D.P.t_4) / (1. - D.P.t_5 * Gh);
-  const double Rt = sqrt(q2pk) * xlb;
+  const double Rt = L.sq * xlb;
   L.av = Rt * Sm + AV_MOL;
   L.dv = Rt * Sh + DV_MOL;
   return L;
@@ -1673,7 +1720,7 @@ __device__ __forceinline__ TkeSums turb_column(const Dev<R> &D, const IO &io, in
   TkeSums bs{0.0, 0.0, 0.0, 0.0};
   const int kms = D.kmz[m], k1s = D.k1z[m], nwl = kms - k1s + 1;
   if (nwl <= 1) return bs;
-  const double twodt1 = D.P.twodt / istep, r3 = 3. / twodt1;
+  const double twodt1 = D.P.twodt / istep, r3 = 3. / twodt1, itw = 1.0 / twodt1;
   const int lW = D.lW[m], lS = D.lS[m];
   // forward-sweep factors of the two systems: split, E1/E2 in d0/d1 and the
   // right-hand sides in q2/q2l; fused, (e1, e2) and (f1, f2) as pairs in d0 and
@@ -1693,6 +1740,8 @@ __device__ __forceinline__ TkeSums turb_column(const Dev<R> &D, const IO &io, in
       L.bp = io.ld(D.d2, k, m);
       L.q2p = io.ld(D.q2p, k, m);
       L.q2lp = io.ld(D.q2lp, k, m);
+      L.rq = 1.0 / L.q2p;
+      L.sq = sqrt(L.q2p);
       return L;
     }
     return turb_level(D, io, m, k, a, b);
@@ -1742,47 +1791,47 @@ __device__ __forceinline__ TkeSums turb_column(const Dev<R> &D, const IO &io, in
     const double hk = SPLIT ? (double)io.ld(D.hp, k, m) : rcur.h;
     const double a3 = -dco / (hk * 0.5 * (hk + hkm));
     const double a1 = -dco_prev / (hkm * 0.5 * (hk + hkm));
-    const double shp = cur.sp, byp = cur.bp;
-    const double sq = sqrt(q2pk);
+    const double shp = cur.sp, byp = cur.bp, rq = cur.rq;
+    const double sqb = cur.sq / (B_1 * xlk);  // sqrt(q2p) / (B1 l)
     // TKE row
-    double a2 = r3 - a1 - a3 + 2. * (sq / (B_1 * xlk));
-    double ds = 4. * q2pk / twodt1 - io.ld(D.q2pp, k, m) / twodt1;
+    double a2 = r3 - a1 - a3 + 2. * sqb;
+    double ds = (4. * q2pk - io.ld(D.q2pp, k, m)) * itw;
     if (byp >= 0.0) {
       ds += 2. * (shp + byp);
     } else {
       ds += 2. * shp;
-      a2 -= byp * 2. / q2pk;
+      a2 -= byp * 2. * rq;
     }
     // length-scale row (wall function of M&Y)
     zfromt += hkm;
     const double zfromb = hbelow;  // sum of hp(k..kms)
     hbelow -= hk;
     const double r1 = xlk / (KAPPAS * zfromb), r2 = xlk / (KAPPAS * zfromt);
-    double a2l = r3 - a1 - a3 + sq / (B_1 * xlk) * (1. + E_2 * r1 * r1 + E_3 * r2 * r2);
+    double a2l = r3 - a1 - a3 + sqb * (1. + E_2 * r1 * r1 + E_3 * r2 * r2);
     double dsl;
     if (D.P.iturb == 1) {
       if (byp >= 0) {
         dsl = xlk * (E_1 * shp + E_1 * byp);
       } else {
         dsl = xlk * (E_1 * shp);
-        a2l -= E_1 * byp / q2pk;
+        a2l -= E_1 * byp * rq;
       }
     } else {
       if (byp >= 0) {
         dsl = xlk * (E_1 * shp + 2.00 * byp);
       } else {
         dsl = xlk * (E_1 * shp);
-        a2l -= 5.06 * byp / q2pk;
+        a2l -= 5.06 * byp * rq;
       }
     }
-    dsl += 4 * cur.q2lp / twodt1 - io.ld(D.q2lpp, k, m) / twodt1;
+    dsl += (4. * cur.q2lp - io.ld(D.q2lpp, k, m)) * itw;
     // forward sweeps
-    double den = a2 + a1 * e1;
-    e1 = -a3 / den;
-    f1 = (ds - a1 * f1) / den;
-    den = a2l + a1 * e2;
-    e2 = -a3 / den;
-    f2 = (dsl - a1 * f2) / den;
+    double rd = 1.0 / (a2 + a1 * e1);
+    e1 = -a3 * rd;
+    f1 = (ds - a1 * f1) * rd;
+    rd = 1.0 / (a2l + a1 * e2);
+    e2 = -a3 * rd;
+    f2 = (dsl - a1 * f2) * rd;
     if (SPLIT) {
       io.st(E1, k, m, (R)e1);
       io.st(E2, k, m, (R)e2);
@@ -1796,7 +1845,7 @@ __device__ __forceinline__ TkeSums turb_column(const Dev<R> &D, const IO &io, in
       const double rhoxh = ((SPLIT ? (double)io.ld(D.rhop, k, m) : rcur.rho) + 1000.) * hk;
       bs.sh += shp * rhoxh;
       bs.by += byp * rhoxh;
-      io.st(D.tkb, k, m, make_float2((float)(sq / (B_1 * xlk) * rhoxh), (float)rhoxh));
+      io.st(D.tkb, k, m, make_float2((float)(sqb * rhoxh), (float)rhoxh));
     }
     // new eddy coefficients of interface k (the surface minimum, 281-284)
     if (!SPLIT) {
