@@ -1112,35 +1112,53 @@ __global__ void __launch_bounds__(kBlock) k_exsal_imsal(Dev<R> D, Scal<R> sv, do
     if (TR) tracer_tail();
     return;
   }
-  // forward e/f sweep: e -> t0, f -> sal (in place), then back substitution
-  double hk = hn(k1s), hkp = hn(k1s + 1), hkm = 0.0;
-  double ep = 0.0, fp = 0.0;
+  // forward e/f sweep: e -> t0, f -> sal (in place), then back substitution.
+  // The coupling through the top of layer k (a1) is the one through the
+  // bottom of layer k-1 (a3), carried; 1/twodt1 and the pivots are multiplied
+  // by as reciprocals (a change of rounding only).
+  const double itw = 1.0 / twodt1;
+  double hk = hn(k1s), hkp = hn(k1s + 1);
+  double ep = 0.0, fp = 0.0, a3p = 0.0;
   for (int k = k1s; k <= kms; ++k) {
-    const double a1 = (k > k1s) ? -(double)D.Dv[IX(k, c)] / (hkm + hk) * 2. : 0.0;
+    const double a1 = a3p;
     const double a3 = (k < kms) ? -(double)D.Dv[IX(k + 1, c)] / (hk + hkp) * 2. : 0.0;
-    const double a2 = hk / twodt1 - a1 - a3;
+    const double a2 = hk * itw - a1 - a3;
     const double d = bex(k);
     if (k == k1s) {
-      ep = -a3 / a2;
-      fp = d / a2;
+      const double rd = 1.0 / a2;
+      ep = -a3 * rd;
+      fp = d * rd;
     } else if (k < kms) {
-      const double den = a2 + a1 * ep;
-      ep = -a3 / den;
-      fp = (d - a1 * fp) / den;
+      const double rd = 1.0 / (a2 + a1 * ep);
+      ep = -a3 * rd;
+      fp = (d - a1 * fp) * rd;
     } else {
       fp = (d - a1 * fp) / (a2 + a1 * ep);  // bottom value
       ep = 0.0;
     }
     D.t0[IX(k, c)] = (R)ep;
     sv.cn[IX(k, c)] = (R)fp;
-    hkm = hk;
+    a3p = a3;
     hk = hkp;
     hkp = (k + 2 <= kms) ? hn(k + 2) : 0.0;
   }
+  // back substitution, the factors of four layers loaded together (layers
+  // above k1s in a group read a valid layer, clamped at 0, and are discarded)
   double x = fp;
-  for (int k = kms - 1; k >= k1s; --k) {
-    x = (double)D.t0[IX(k, c)] * x + (double)sv.cn[IX(k, c)];
-    sv.cn[IX(k, c)] = (R)x;
+  for (int kg = kms - 1; kg >= k1s; kg -= 4) {
+    R te[4], tf[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int kk = max(kg - j, 0);
+      te[j] = D.t0[IX(kk, c)];
+      tf[j] = sv.cn[IX(kk, c)];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (kg - j < k1s) break;
+      x = (double)te[j] * x + (double)tf[j];
+      sv.cn[IX(kg - j, c)] = (R)x;
+    }
   }
   for (int k = D.k1; k <= k1s - 1; ++k) sv.cn[IX(k, c)] = (R)x;
   if (TR) {
