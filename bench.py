@@ -69,10 +69,10 @@ ARRAYS_PER_CELL = {
 # profiles/hbm_traffic_ellipse_2048x2048x60_dx50_fp64.md.  Reported as
 # roofline.traffic for that workload only.
 TRAFFIC_FP64_2048x2048x60_DX50 = {
-    "exmom_x": 160.9, "exmom_y": 136.7, "matmom_x": 89.8, "matmom_y": 78.5, "matcon": 27.6, "slayer_h": 61.6,
-    "vel": 77.6, "continuity1": 73.1, "exsal_hflux": 117.7, "exsal_rhs": 116.9, "imsal": 65.2, "vel2": 50.8,
-    "turbulence": 223.3, "turb_cell": 140.2, "settrap_cell": 304.7, "settrap2_cell": 180.2, "save_cell": 178.0,
-    "continuity2": 40.4,
+    "exmom_x": 160.9, "exmom_y": 137.0, "matmom_x": 89.6, "matmom_y": 78.5, "matcon": 27.6, "slayer_h": 52.2,
+    "vel": 73.8, "continuity1": 73.6, "exsal_hflux": 117.8, "exsal_rhs": 116.9, "imsal": 65.7, "vel2": 27.6,
+    "turbulence": 220.7, "turb_cell": 140.4, "settrap_cell": 304.7, "settrap2_cell": 180.2, "save_cell": 178.0,
+    "continuity2": 40.7,
 }
 
 
