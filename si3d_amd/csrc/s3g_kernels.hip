@@ -585,55 +585,61 @@ __global__ void __launch_bounds__(kBlock) k_matmom(Dev<R> D, double tz) {
   }
   const double bot_a = taub * twodt1;
 
-  double bcl = 0.0, hdr_prev = 0.0, c1p = 0.0, d1p = 0.0, e1p = 0.0;
+  // The level arithmetic runs in the storage precision C = R (FP64, or FP32
+  // in mixed mode); the per-column terms above and the column sums below are
+  // FP64.
+  using C = R;
+  const C twodt1c = (C)twodt1, thetac = (C)theta, tzc = (C)tz, gqc = (C)gq, ws0c = (C)ws0, irhoqc = (C)irhoq;
+  const C twodtc = (C)D.P.twodt;
+  C bcl = 0, hdr_prev = 0, c1p = 0, d1p = 0, e1p = 0;
   R *CF = X ? D.cfx : D.cfy;
   double w = 1.0, sg = 0.0, sr = 0.0, c1s = 0.0;  // column-sum weight, sums, c1 as stored
   // thicknesses (and 1/h), face viscosity, qpp and the viscous flux of levels
-  // k-1, k (k+1) carried down the sweep.  The sweep is bound by FP64
+  // k-1, k (k+1) carried down the sweep.  The sweep is bound by its
   // arithmetic: divisions by a quantity used more than once are
   // multiplications by its reciprocal (1/h, 1/rho, the pivot), which moves
   // results by rounding only; the viscous flux through the top of layer k is
   // the one through the bottom of layer k-1.
-  double hkm = hq[IX(k1q - 1, c)], hk = hq[IX(k1q, c)], Avk = 0.0, rhk = 1.0 / hk, dkp_prev = 0.0;
-  double qkm = 0.0, qk = (nwl > 1) ? (double)qpp[IX(k1q, c)] : 0.0;
+  C hkm = hq[IX(k1q - 1, c)], hk = hq[IX(k1q, c)], Avk = 0, rhk = C(1) / hk, dkp_prev = 0;
+  C qkm = 0, qk = (nwl > 1) ? (C)qpp[IX(k1q, c)] : C(0);
   for (int k = k1q; k <= kmq; ++k) {
-    const double hkp = hq[IX(k + 1, c)];
-    const double Avkp = (k < kmq) ? 0.5 * ((double)D.Av[IX(k + 1, ln)] + (double)D.Av[IX(k + 1, c)]) : 0.0;
-    const double qkp = (nwl > 1 && k < kmq) ? (double)qpp[IX(k + 1, c)] : 0.0;
+    const C hkp = hq[IX(k + 1, c)];
+    const C Avkp = (k < kmq) ? C(0.5) * ((C)D.Av[IX(k + 1, ln)] + (C)D.Av[IX(k + 1, c)]) : C(0);
+    const C qkp = (nwl > 1 && k < kmq) ? (C)qpp[IX(k + 1, c)] : C(0);
     if (D.P.ibc != 0) {
-      double hdr = gq * hk * ((double)D.rhop[IX(k, ln)] - (double)D.rhop[IX(k, c)]);
-      if (k == kmq && kmq > k1q) hdr = bclnc_km<R, X>(D, c, ln, kmq, hdr);
+      C hdr = gqc * hk * ((C)D.rhop[IX(k, ln)] - (C)D.rhop[IX(k, c)]);
+      if (k == kmq && kmq > k1q) hdr = (C)bclnc_km<R, X>(D, c, ln, kmq, (double)hdr);
       bcl = (k == k1q) ? hdr : bcl + (hdr_prev + hdr);
       hdr_prev = hdr;
     }
-    double vd = 0.0;
+    C vd = 0;
     if (nwl > 1) {
-      const double dk = dkp_prev;
-      const double dkp = (k < kmq) ? Avkp * (qk - qkp) / (hk + hkp) : 0.0;
-      vd = D.P.twodt * (dk - dkp) * 2. * (1. - theta);
+      const C dk = dkp_prev;
+      const C dkp = (k < kmq) ? Avkp * (qk - qkp) / (hk + hkp) : C(0);
+      vd = twodtc * (dk - dkp) * C(2) * (C(1) - thetac);
       dkp_prev = dkp;
     }
-    const double hh = hk * irhoq;
-    double gg = (double)(X ? D.ex : D.ey)[IX(k, c)] - hh * (bcl + ws0) * tz + vd * tz;
-    double a1 = 0.0, a2 = 1.0, a3 = 0.0;
+    const C hh = hk * irhoqc;
+    C gg = (C)(X ? D.ex : D.ey)[IX(k, c)] - hh * (bcl + ws0c) * tzc + vd * tzc;
+    C a1 = 0, a2 = 1, a3 = 0;
     if (nwl > 1) {
-      a3 = (k < kmq) ? -twodt1 * Avkp / (hkp * (hk + hkp)) * 2. * theta : 0.0;
-      a1 = (k > k1q) ? -twodt1 * Avk / (hkm * (hkm + hk)) * 2. * theta : 0.0;
-      a2 = 1.0 - (hkm * rhk) * a1 - (hkp * rhk) * a3;
+      a3 = (k < kmq) ? -twodt1c * Avkp / (hkp * (hk + hkp)) * C(2) * thetac : C(0);
+      a1 = (k > k1q) ? -twodt1c * Avk / (hkm * (hkm + hk)) * C(2) * thetac : C(0);
+      a2 = C(1) - (hkm * rhk) * a1 - (hkp * rhk) * a3;
     }
     if (k == k1q) {
-      gg += wind_g;
-      a2 += wind_a;
+      gg += (C)wind_g;
+      a2 += (C)wind_a;
     }
-    if (k == kmq) a2 += bot_a;
-    double c1, d1, e1;
+    if (k == kmq) a2 += (C)bot_a;
+    C c1, d1, e1;
     if (k == k1q) {
-      const double rd = 1.0 / a2;
+      const C rd = C(1) / a2;
       c1 = a3 * rd;
       d1 = gg * rd;
       e1 = hh * rd;
     } else {
-      const double rd = 1.0 / (a2 - a1 * c1p);
+      const C rd = C(1) / (a2 - a1 * c1p);
       c1 = a3 * rd;
       d1 = (gg - a1 * d1p) * rd;
       e1 = (hh - a1 * e1p) * rd;
@@ -651,7 +657,7 @@ __global__ void __launch_bounds__(kBlock) k_matmom(Dev<R> D, double tz) {
     e1p = e1;
     hkm = hk;
     hk = hkp;
-    rhk = 1.0 / hkp;
+    rhk = C(1) / hkp;
     Avk = Avkp;
     qkm = qk;
     qk = qkp;
