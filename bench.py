@@ -40,8 +40,9 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per wet cell for one launch of each kernel: distinct state
 # arrays read plus distinct arrays written that a later phase consumes, counted
 # once per wet cell (phase-private scratch, index arrays and 2-D arrays not
-# counted), as (arrays of the storage type, FP64 arrays).  q2/q2l and their
-# time levels are FP64 in every precision mode.
+# counted), as (arrays of the storage type, FP64 arrays).  Every 3-D field,
+# the turbulence state included, has the storage type (FP64, or FP32 in mixed
+# mode), so the second count is zero.
 ARRAYS_PER_CELL = {
     "exmom_x": (10, 0), "exmom_y": (10, 0),      # uhp vhp wp up upp uhpp hup hupp hp -> ex
     "matmom_x": (8, 0), "matmom_y": (8, 0),      # ex Av rhop hup upp (uhpp at the bed) -> cfx agx arx
@@ -54,11 +55,11 @@ ARRAYS_PER_CELL = {
     "exsal_rhs": (10, 0),                        # fx_ew fx_ns salpp hpp hp hupp hvpp wp -> bex
     "imsal": (5, 0),                             # bex Dv h (hpp, wp at the top) -> sal
     "vel2": (3, 0),                              # -> h hu hv
-    "turbulence": (8, 6),                        # hp rhop up vp Av Dv -> Av Dv; q2p q2pp q2lp q2lpp -> q2 q2l
-    "turb_cell": (8, 2),                         # hp rhop up vp Av Dv -> Av Dv; q2p q2lp
-    "settrap_cell": (28, 8),                     # 11 read, 17 written; 4 + 4 FP64
-    "settrap2_cell": (15, 6),                    # 6 read, 9 written; 4 + 2 FP64
-    "save_cell": (17, 4),                        # trapezoidal save: 6 read, 11 written; 2 + 2 FP64
+    "turbulence": (14, 0),                       # hp rhop up vp Av Dv q2p q2pp q2lp q2lpp -> Av Dv q2 q2l
+    "turb_cell": (10, 0),                        # hp rhop up vp Av Dv q2p q2lp -> Av Dv
+    "settrap_cell": (36, 0),                     # 15 read, 21 written (turbulence state included)
+    "settrap2_cell": (21, 0),                    # 10 read, 11 written
+    "save_cell": (21, 0),                        # trapezoidal save: 8 read, 13 written
     "continuity2": (3, 0),                       # uhp vhp -> wp
 }
 
