@@ -97,6 +97,9 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-procs", type=int, default=min(16, os.cpu_count() or 1),
+                    help="host cores for the CPU baseline: one single-threaded copy of the oracle per core, timed "
+                         "together (1: one copy)")
     ap.add_argument("--cpu-side", type=int, default=512,
                     help="for lakes above 1024 x 512 cells the CPU baseline runs the same generator on a "
                          "CPU-SIDE x CPU-SIDE lake of the same spacing (the full lake needs minutes per step)")
@@ -146,6 +149,47 @@ def cpu_baseline(deck, nsteps):
         o.step(n)
     dt = time.perf_counter() - t0
     return dt, o
+
+
+def _cpu_worker(deck, nsteps, barrier, out):
+    from oracle.oracle import Oracle
+    o = Oracle(deck)
+    o.step(1)
+    barrier.wait()
+    t0 = time.perf_counter()
+    for n in range(2, nsteps + 2):
+        o.step(n)
+    out.put(time.perf_counter() - t0)
+
+
+def cpu_baseline_procs(deck, nsteps, procs):
+    """The oracle on `procs` host cores: one copy of the lake per core, each a
+    single-threaded process (started fresh, not forked from this one), set up
+    and stepped once, then steps 2 .. nsteps+1 timed together after a common
+    barrier.  Returns the longest of the timed spans."""
+    import multiprocessing as mp
+    import queue
+    from oracle.oracle import build
+    build()
+    ctx = mp.get_context("spawn")
+    barrier, out = ctx.Barrier(procs, timeout=900), ctx.Queue()
+    ps = [ctx.Process(target=_cpu_worker, args=(deck, nsteps, barrier, out)) for _ in range(procs)]
+    for p in ps:
+        p.start()
+    spans = []
+    try:
+        while len(spans) < procs:
+            try:
+                spans.append(out.get(timeout=5))
+            except queue.Empty:
+                if any(p.exitcode not in (None, 0) for p in ps):
+                    raise RuntimeError("a CPU-baseline process failed")
+    finally:
+        for p in ps:
+            p.join(timeout=10)
+            if p.is_alive():
+                p.terminate()
+    return max(spans)
 
 
 def copy_probe(mib):
@@ -528,11 +572,19 @@ def main():
             cn_wet = lake.n_wet_cells(cdeck)
             what = (f"the same generator on a {a.cpu_side}x{a.cpu_side} lake at dx={a.dx:g} m ({cn_wet} wet cells; "
                     f"the full lake takes minutes per step on one core)")
-        dt, o = cpu_baseline(cdeck, a.cpu_steps)
-        cpu = {"value": round(cn_wet * a.cpu_steps / dt, 1), "unit": "wet-cell-steps/s", "cores": 1,
-               "kind": "port",
-               "sample": f"FP64 oracle (C++ restatement, 1 thread), {what}, steps 2-{a.cpu_steps + 1} "
-                         f"(three fd calls each; step 1 untimed), {dt:.1f} s"}
+        if a.cpu_procs > 1:
+            dt = cpu_baseline_procs(cdeck, a.cpu_steps, a.cpu_procs)
+            cpu = {"value": round(a.cpu_procs * cn_wet * a.cpu_steps / dt, 1), "unit": "wet-cell-steps/s",
+                   "cores": a.cpu_procs, "kind": "port",
+                   "sample": f"FP64 oracle (C++ restatement), {a.cpu_procs} single-threaded processes, one copy of "
+                             f"{what} each, steps 2-{a.cpu_steps + 1} (three fd calls each; step 1 untimed) timed "
+                             f"together: {a.cpu_procs} x {a.cpu_steps} lake-steps in {dt:.1f} s (the slowest copy)"}
+        else:
+            dt, o = cpu_baseline(cdeck, a.cpu_steps)
+            cpu = {"value": round(cn_wet * a.cpu_steps / dt, 1), "unit": "wet-cell-steps/s", "cores": 1,
+                   "kind": "port",
+                   "sample": f"FP64 oracle (C++ restatement, 1 thread), {what}, steps 2-{a.cpu_steps + 1} "
+                             f"(three fd calls each; step 1 untimed), {dt:.1f} s"}
 
     out = {
         "metric": "wet-cell-steps/sec and PCG iters/step at 1 MI355X, learned vs plain, same error",
