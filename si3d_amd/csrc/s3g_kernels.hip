@@ -772,7 +772,7 @@ __global__ void __launch_bounds__(kBlock) k_slayer_h_w(Dev<R> D, int nk, int til
   const double smE = (lE != S) ? dmax(sc, D.s[lE]) : 0.0;
   const double smN = (lN != S) ? dmax(sc, D.s[lN]) : 0.0;
   const double haux0z = dmin(zl[k1z + 1], hhs) + sc;
-  const double haux0v = dmin(zl[k1v + 1], hhv) + smN;
+  const double haux0v = (lN != S) ? dmin(zl[k1v + 1], hhv) + smN : 0.0;
   const auto h = rsrc(D.h, nb), hu = rsrc(D.hu, nb), hv = rsrc(D.hv, nb);
   const auto hp = rsrc(D.hp, nb), hup = rsrc(D.hup, nb), hvp = rsrc(D.hvp, nb);
   for (int k = t.k0; k < kend; ++k) {
